@@ -1,0 +1,329 @@
+"""Procedural scenes for the affine-body contact path.
+
+The reference loads meshes from .bgeo/.obj/.usd files through a JSON scene
+(simulator/scene.py:23-60, simulator/mesh.py:8-54, affine_body.py:59-71); those
+readers and assets are absent, so scenes here are generated procedurally and
+seeded.  A scene is plain numpy data (`SceneSpec`) that both the GPU path and the
+CPU oracle consume.
+
+Edge ordering: the reference takes edges from ``igl.edges(F)`` (affine_body.py:114),
+whose order is not pinned offline.  Here edges are the unique undirected
+triangle edges ``(min, max)`` sorted lexicographically.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+
+# --------------------------------------------------------------------- meshes
+def edges_of(F: np.ndarray) -> np.ndarray:
+    """Unique undirected edges of a triangle list, (min, max), lexicographic."""
+    e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]], axis=0)
+    e = np.sort(e, axis=1)
+    e = np.unique(e, axis=0)
+    return e.astype(np.int32)
+
+
+def cube_mesh() -> Tuple[np.ndarray, np.ndarray]:
+    """Unit cube [-0.5, 0.5]^3: 8 vertices, 12 outward triangles, 18 edges.
+
+    The bottom (y = -0.5) face is split along the (-x,-z)-(+x,+z) diagonal and
+    the top (y = +0.5) face along the other one, so that two stacked copies
+    have crossing face diagonals (a valid edge-edge contact at the face centre).
+    """
+    V = np.array([[x, y, z] for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.5, 0.5)], dtype=np.float64)
+    # index = 4*ix + 2*iy + iz
+    def idx(ix, iy, iz):
+        return 4 * ix + 2 * iy + iz
+    F = []
+    # -x face (ix = 0), outward -x
+    F += [[idx(0, 0, 0), idx(0, 0, 1), idx(0, 1, 1)], [idx(0, 0, 0), idx(0, 1, 1), idx(0, 1, 0)]]
+    # +x face
+    F += [[idx(1, 0, 0), idx(1, 1, 1), idx(1, 0, 1)], [idx(1, 0, 0), idx(1, 1, 0), idx(1, 1, 1)]]
+    # -y face (bottom), diagonal (0,0,0)-(1,0,1)
+    F += [[idx(0, 0, 0), idx(1, 0, 1), idx(0, 0, 1)], [idx(0, 0, 0), idx(1, 0, 0), idx(1, 0, 1)]]
+    # +y face (top), diagonal (0,1,1)-(1,1,0)
+    F += [[idx(0, 1, 1), idx(1, 1, 0), idx(0, 1, 0)], [idx(0, 1, 1), idx(1, 1, 1), idx(1, 1, 0)]]
+    # -z face
+    F += [[idx(0, 0, 0), idx(0, 1, 0), idx(1, 1, 0)], [idx(0, 0, 0), idx(1, 1, 0), idx(1, 0, 0)]]
+    # +z face
+    F += [[idx(0, 0, 1), idx(1, 1, 1), idx(0, 1, 1)], [idx(0, 0, 1), idx(1, 0, 1), idx(1, 1, 1)]]
+    F = np.array(F, dtype=np.int32)
+    return V, F
+
+
+def icosphere_mesh(subdiv: int, radius: float = 1.0) -> Tuple[np.ndarray, np.ndarray]:
+    """Icosahedron subdivided `subdiv` times, vertices projected to `radius`.
+
+    Counts: V = 10*4^n + 2, F = 20*4^n, E = 30*4^n (n=2: 162/320/480, n=3: 642/1280/1920).
+    """
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    V = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    F = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+         [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+         [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    V = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in V]
+    for _ in range(subdiv):
+        mid = {}
+        nF = []
+
+        def m(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                p = V[a] + V[b]
+                V.append(p / np.linalg.norm(p))
+                mid[key] = len(V) - 1
+            return mid[key]
+        for a, b, c in F:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            nF += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        F = nF
+    return np.array(V) * radius, np.array(F, dtype=np.int32)
+
+
+def subdivided_box_mesh(n: int = 6, size: float = 1.0) -> Tuple[np.ndarray, np.ndarray]:
+    """Surface of an n x n x n subdivided box of edge `size` centred at 0.
+
+    n = 6: 218 vertices, 432 triangles, 648 edges.
+    """
+    idx = {}
+    V = []
+
+    def vid(i, j, k):
+        key = (i, j, k)
+        if key not in idx:
+            idx[key] = len(V)
+            V.append([i / n - 0.5, j / n - 0.5, k / n - 0.5])
+        return idx[key]
+    F = []
+    for axis in range(3):
+        for side in (0, n):
+            for a in range(n):
+                for b in range(n):
+                    def P(u, v):
+                        c = [0, 0, 0]
+                        c[axis] = side
+                        c[(axis + 1) % 3] = u
+                        c[(axis + 2) % 3] = v
+                        return vid(*c)
+                    q = [P(a, b), P(a + 1, b), P(a + 1, b + 1), P(a, b + 1)]
+                    if side == n:
+                        F += [[q[0], q[1], q[2]], [q[0], q[2], q[3]]]
+                    else:
+                        F += [[q[0], q[2], q[1]], [q[0], q[3], q[2]]]
+    return np.array(V, dtype=np.float64) * size, np.array(F, dtype=np.int32)
+
+
+def ellipsoid_mesh(rng: np.random.Generator, subdiv: int = 4) -> Tuple[np.ndarray, np.ndarray]:
+    """Icosphere-4 ellipsoid with radii U[0.3, 0.8]^3 and a seeded radial perturbation (+-5%)."""
+    V, F = icosphere_mesh(subdiv, 1.0)
+    radii = rng.uniform(0.3, 0.8, size=3)
+    pert = 1.0 + 0.05 * rng.uniform(-1.0, 1.0, size=V.shape[0])
+    return V * pert[:, None] * radii[None, :], F
+
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    """Uniform random rotation: QR of a Gaussian 3x3 with the sign fix, det +1."""
+    M = rng.normal(size=(3, 3))
+    Q, R = np.linalg.qr(M)
+    Q = Q * np.sign(np.diag(R))[None, :]
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] = -Q[:, 0]
+    return Q
+
+
+# ---------------------------------------------------------------------- scene
+@dataclasses.dataclass
+class SceneSpec:
+    """Flattened multi-body scene (host numpy).
+
+    Per body b: vertices x0[voff[b]:voff[b+1]] (rest shape), triangles
+    tris[foff[b]:foff[b+1]] and edges edges[eoff[b]:eoff[b+1]] in body-local
+    vertex indices.  Initial state per body: A (rows a1, a2, a3; x = A^T x0 + p),
+    p, Adot, pdot.  body_scene[b] is the independent scene a body belongs to
+    (bodies of different scenes never interact).
+    """
+    x0: np.ndarray
+    tris: np.ndarray
+    edges: np.ndarray
+    voff: np.ndarray
+    foff: np.ndarray
+    eoff: np.ndarray
+    A: np.ndarray
+    p: np.ndarray
+    Adot: np.ndarray
+    pdot: np.ndarray
+    mass: np.ndarray
+    I0: np.ndarray
+    body_scene: np.ndarray
+    ground: float
+    name: str = "scene"
+
+    @property
+    def n_bodies(self) -> int:
+        return int(self.voff.shape[0] - 1)
+
+    @property
+    def n_scenes(self) -> int:
+        return int(self.body_scene.max()) + 1 if self.n_bodies else 0
+
+    @property
+    def n_verts(self) -> int:
+        return int(self.voff[-1])
+
+    @property
+    def n_tris(self) -> int:
+        return int(self.foff[-1])
+
+    @property
+    def n_edges(self) -> int:
+        return int(self.eoff[-1])
+
+
+def build_scene(meshes: Sequence[Tuple[np.ndarray, np.ndarray]], A: np.ndarray, p: np.ndarray, Adot: np.ndarray,
+                pdot: np.ndarray, ground: float, mass: Optional[np.ndarray] = None, I0: Optional[np.ndarray] = None,
+                body_scene: Optional[np.ndarray] = None, name: str = "scene",
+                mass_const: float = 1e3, I0_const: float = 1e2) -> SceneSpec:
+    nb = len(meshes)
+    voff = np.zeros(nb + 1, np.int32)
+    foff = np.zeros(nb + 1, np.int32)
+    eoff = np.zeros(nb + 1, np.int32)
+    xs, fs, es = [], [], []
+    for b, (V, F) in enumerate(meshes):
+        E = edges_of(F)
+        voff[b + 1] = voff[b] + V.shape[0]
+        foff[b + 1] = foff[b] + F.shape[0]
+        eoff[b + 1] = eoff[b] + E.shape[0]
+        xs.append(np.asarray(V, np.float64))
+        fs.append(np.asarray(F, np.int32))
+        es.append(E)
+    return SceneSpec(
+        x0=np.ascontiguousarray(np.concatenate(xs, 0)),
+        tris=np.ascontiguousarray(np.concatenate(fs, 0)),
+        edges=np.ascontiguousarray(np.concatenate(es, 0)),
+        voff=voff, foff=foff, eoff=eoff,
+        A=np.ascontiguousarray(A, np.float64), p=np.ascontiguousarray(p, np.float64),
+        Adot=np.ascontiguousarray(Adot, np.float64), pdot=np.ascontiguousarray(pdot, np.float64),
+        mass=np.full(nb, mass_const) if mass is None else np.asarray(mass, np.float64),
+        I0=np.full(nb, I0_const) if I0 is None else np.asarray(I0, np.float64),
+        body_scene=np.zeros(nb, np.int32) if body_scene is None else np.asarray(body_scene, np.int32),
+        ground=float(ground), name=name)
+
+
+# -------------------------------------------------------------------- configs
+DHAT = 1e-3
+
+
+def config_small(dhat: float = DHAT) -> SceneSpec:
+    """Plain small: two unit cubes, B at the origin, A at height 1.05, zero velocity.
+
+    The ground sits at -0.5 - dhat/2 so that B's bottom face starts at distance
+    dhat/2 (a ground exactly at -0.5 would put B at d = 0, where the barrier is
+    infinite).
+    """
+    V, F = cube_mesh()
+    A = np.stack([np.eye(3)] * 2)
+    p = np.array([[0.0, 0.0, 0.0], [0.0, 1.05, 0.0]])
+    z = np.zeros((2, 3))
+    return build_scene([(V, F), (V, F)], A, p, np.zeros((2, 3, 3)), z, ground=-0.5 - dhat / 2, name="small")
+
+
+def config_medium(seed: int = 0, n_side: int = 4, subdiv: int = 3) -> SceneSpec:
+    """Plain medium: 64 icosphere-3 bodies (r 0.5) on a 4x4x4 grid, spacing 1.1.
+
+    Velocities U[-1,1]^3, uniformly random rotations, zero angular rate.  The
+    floor is y = 0 and the lowest layer's centres sit at 0.5 + 0.1.
+    """
+    rng = np.random.default_rng(seed)
+    V, F = icosphere_mesh(subdiv, 0.5)
+    n = n_side ** 3
+    ps, As = [], []
+    for i in range(n_side):
+        for j in range(n_side):
+            for k in range(n_side):
+                ps.append([1.1 * i, 0.6 + 1.1 * j, 1.1 * k])
+    for _ in range(n):
+        As.append(random_rotation(rng))
+    pdot = rng.uniform(-1.0, 1.0, size=(n, 3))
+    return build_scene([(V, F)] * n, np.array(As), np.array(ps), np.zeros((n, 3, 3)), pdot, ground=0.0,
+                       name="medium")
+
+
+def _stack_scene(rng: np.random.Generator, n_bodies: int, meshes_lib, gap: float = 0.02):
+    """Random non-overlapping stacks: bodies drop into random (x, z) cells of a
+    4 x 4 grid and are stacked in y with `gap` between bounding boxes."""
+    cells = {}
+    meshes, ps = [], []
+    for _ in range(n_bodies):
+        kind = int(rng.integers(0, len(meshes_lib)))
+        V, F = meshes_lib[kind]
+        cx, cz = int(rng.integers(0, 4)), int(rng.integers(0, 4))
+        ext = V.max(0) - V.min(0)
+        y0 = cells.get((cx, cz), 0.0) + gap
+        p = np.array([1.2 * cx, y0 - V[:, 1].min(), 1.2 * cz])
+        cells[(cx, cz)] = y0 + ext[1]
+        meshes.append((V, F))
+        ps.append(p)
+    return meshes, np.array(ps)
+
+
+def config_batched(n_scenes: int = 256, seed0: int = 1000, nb_range=(2, 64)) -> SceneSpec:
+    """Batched: n_scenes independent scenes, scene s seeded 1000 + s; each draws
+    N_b ~ U{2..64} bodies, each a cube, icosphere-2 or 6x6x6 subdivided box,
+    stacked without overlap, speed U[0, 2] in a uniform direction."""
+    lib = [cube_mesh(), icosphere_mesh(2, 0.5), subdivided_box_mesh(6)]
+    meshes, ps, pd, bs = [], [], [], []
+    for s in range(n_scenes):
+        rng = np.random.default_rng(seed0 + s)
+        nb = int(rng.integers(nb_range[0], nb_range[1] + 1))
+        m, p = _stack_scene(rng, nb, lib)
+        # scenes are independent; offset them in x so boxes of different scenes never touch
+        p = p + np.array([10.0 * s, 0.0, 0.0])
+        d = rng.normal(size=(nb, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        v = d * rng.uniform(0.0, 2.0, size=(nb, 1))
+        meshes += m
+        ps.append(p)
+        pd.append(v)
+        bs += [s] * nb
+    nb = len(meshes)
+    return build_scene(meshes, np.stack([np.eye(3)] * nb), np.concatenate(ps), np.zeros((nb, 3, 3)),
+                       np.concatenate(pd), ground=0.0 - 0.5 * DHAT, body_scene=np.array(bs), name="batched")
+
+
+def config_stress(seed: int = 0, n_bodies: int = 1024) -> SceneSpec:
+    """Stress: 1024 bodies (equal thirds cube / icosphere-3 / icosphere-4
+    ellipsoid with radii U[0.3,0.8]^3 and a radial perturbation) on a jittered
+    10 x 10 x 10.24 lattice with gap 0.02."""
+    rng = np.random.default_rng(seed)
+    ico3 = icosphere_mesh(3, 0.5)
+    cube = cube_mesh()
+    meshes = []
+    for b in range(n_bodies):
+        k = b % 3
+        meshes.append(cube if k == 0 else ico3 if k == 1 else ellipsoid_mesh(rng))
+    spacing = 1.0 + 0.6 + 0.02
+    ps = []
+    for b in range(n_bodies):
+        i, j, k = b % 10, (b // 10) % 10, b // 100
+        jit = rng.uniform(-0.005, 0.005, size=3)
+        ps.append([spacing * i + jit[0], 0.9 + spacing * k + jit[1], spacing * j + jit[2]])
+    A = np.stack([np.eye(3)] * n_bodies)
+    return build_scene(meshes, A, np.array(ps), np.zeros((n_bodies, 3, 3)), np.zeros((n_bodies, 3)), ground=0.0,
+                       name="stress")
+
+
+def subset_scene(spec: SceneSpec, bodies: Sequence[int]) -> SceneSpec:
+    """A scene made of some bodies of another (keeps their meshes and states)."""
+    meshes = []
+    for b in bodies:
+        V = spec.x0[spec.voff[b]:spec.voff[b + 1]]
+        F = spec.tris[spec.foff[b]:spec.foff[b + 1]]
+        meshes.append((V, F))
+    idx = np.asarray(bodies)
+    return build_scene(meshes, spec.A[idx], spec.p[idx], spec.Adot[idx], spec.pdot[idx], spec.ground,
+                       mass=spec.mass[idx], I0=spec.I0[idx], body_scene=spec.body_scene[idx], name=spec.name + "-sub")

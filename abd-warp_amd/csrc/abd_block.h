@@ -1,0 +1,107 @@
+// abd_block.h — workgroup-level building blocks (wave64 ballots, fixed-order
+// reductions and scans through LDS).  Every helper ends with a barrier so the
+// LDS it used can be reused immediately by the caller.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace abd {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+__device__ __forceinline__ int wave_id() { return (int)(threadIdx.x / kWave); }
+__device__ __forceinline__ int n_waves() { return (int)((blockDim.x + kWave - 1) / kWave); }
+
+// Sum over the workgroup, fixed pairing (deterministic); result returned to every thread.
+// sh must hold blockDim.x doubles.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = (int)blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = sh[t] + sh[t + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+// Minimum over the workgroup (blockDim.x a power of two).
+__device__ __forceinline__ double block_min(double v, double* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = (int)blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = sh[t + s] < sh[t] ? sh[t + s] : sh[t];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ float block_minf(float v, float* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = (int)blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = fminf(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    float r = sh[0];
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ float block_maxf(float v, float* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = (int)blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = fmaxf(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    float r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Exclusive prefix sum of one int64 per thread (sh: blockDim.x entries).
+__device__ __forceinline__ int64_t block_exclusive_scan_i64(int64_t v, int64_t* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+        int64_t add = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    int64_t incl = sh[t];
+    __syncthreads();
+    return incl - v;
+}
+
+// Ordered compaction rank inside the workgroup: returns the number of threads
+// with a lower thread index whose predicate holds; *total = count over the
+// workgroup.  sh_waves: n_waves() ints.
+__device__ __forceinline__ int block_rank(bool pred, int* sh_waves, int* total) {
+    unsigned long long mask = __ballot(pred ? 1 : 0);
+    const int lane = lane_id();
+    const int w = wave_id();
+    unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (kWave - lane)));
+    int r = __popcll(below);
+    if (lane == 0) sh_waves[w] = __popcll(mask);
+    __syncthreads();
+    int off = 0, tot = 0;
+    const int nw = n_waves();
+    for (int k = 0; k < nw; ++k) {
+        int c = sh_waves[k];
+        if (k < w) off += c;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + r;
+}
+
+}  // namespace abd

@@ -1,0 +1,227 @@
+// abd_common.hip — library housekeeping (version, errors, scratch arena) and the
+// generic deterministic scan / reduction kernels the stages share.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+#include "abd_block.h"
+#include "abd_internal.h"
+
+namespace abd {
+
+namespace {
+struct DeviceArena {
+    void* ptr[kArenaSlots] = {};
+    size_t size[kArenaSlots] = {};
+    void* host = nullptr;
+    size_t host_size = 0;
+};
+constexpr int kMaxDevices = 64;
+DeviceArena g_arena[kMaxDevices];
+std::mutex g_arena_mutex;
+
+int current_device(int* dev) {
+    hipError_t e = hipGetDevice(dev);
+    if (e != hipSuccess) return -static_cast<int>(e);
+    if (*dev < 0 || *dev >= kMaxDevices) return ABD_ERR_NODEVICE;
+    return ABD_OK;
+}
+}  // namespace
+
+int arena_get(int slot, size_t bytes, void** ptr) {
+    int dev;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_arena_mutex);
+    DeviceArena& A = g_arena[dev];
+    if (bytes == 0) bytes = 256;
+    if (A.size[slot] < bytes) {
+        size_t want = bytes + bytes / 2;
+        want = (want + 255) & ~size_t(255);
+        if (A.ptr[slot]) {
+            ABD_CHECK(hipDeviceSynchronize());
+            ABD_CHECK(hipFree(A.ptr[slot]));
+            A.ptr[slot] = nullptr;
+            A.size[slot] = 0;
+        }
+        ABD_CHECK(hipMalloc(&A.ptr[slot], want));
+        A.size[slot] = want;
+    }
+    *ptr = A.ptr[slot];
+    return ABD_OK;
+}
+
+int arena_host(size_t bytes, void** ptr) {
+    int dev;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_arena_mutex);
+    DeviceArena& A = g_arena[dev];
+    if (A.host_size < bytes) {
+        if (A.host) {
+            ABD_CHECK(hipDeviceSynchronize());
+            ABD_CHECK(hipHostFree(A.host));
+            A.host = nullptr;
+        }
+        size_t want = bytes < 4096 ? 4096 : bytes;
+        ABD_CHECK(hipHostMalloc(&A.host, want, hipHostMallocDefault));
+        A.host_size = want;
+    }
+    *ptr = A.host;
+    return ABD_OK;
+}
+
+int arena_release_current() {
+    int dev;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_arena_mutex);
+    DeviceArena& A = g_arena[dev];
+    ABD_CHECK(hipDeviceSynchronize());
+    for (int i = 0; i < kArenaSlots; ++i) {
+        if (A.ptr[i]) ABD_CHECK(hipFree(A.ptr[i]));
+        A.ptr[i] = nullptr;
+        A.size[i] = 0;
+    }
+    if (A.host) ABD_CHECK(hipHostFree(A.host));
+    A.host = nullptr;
+    A.host_size = 0;
+    return ABD_OK;
+}
+
+// ------------------------------------------------------------------ scan
+// Single workgroup exclusive scan; each pass covers 1024 threads x 8 items.
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 8;
+
+__global__ __launch_bounds__(kScanThreads) void scan_i64_kernel(const int64_t* __restrict__ in,
+                                                                 int64_t* __restrict__ out, int64_t n,
+                                                                 int64_t* __restrict__ total) {
+    __shared__ int64_t sh[kScanThreads];
+    int64_t carry = 0;
+    const int t = threadIdx.x;
+    for (int64_t base = 0; base < n; base += (int64_t)kScanThreads * kScanItems) {
+        int64_t v[kScanItems];
+        int64_t s = 0;
+        int64_t i0 = base + (int64_t)t * kScanItems;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            int64_t i = i0 + k;
+            v[k] = i < n ? in[i] : 0;
+            s += v[k];
+        }
+        int64_t excl = block_exclusive_scan_i64(s, sh);
+        int64_t run = carry + excl;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            int64_t i = i0 + k;
+            if (i < n) out[i] = run;
+            run += v[k];
+        }
+        // last thread holds the tile total
+        if (t == kScanThreads - 1) sh[0] = excl + s;
+        __syncthreads();
+        carry += sh[0];
+        __syncthreads();
+    }
+    if (t == 0 && total) *total = carry;
+}
+
+int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total) {
+    hipLaunchKernelGGL(scan_i64_kernel, dim3(1), dim3(kScanThreads), 0, s, in, out, n, total);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+// -------------------------------------------------------------- reduction
+constexpr int kRedBlocks = 512;
+
+__global__ __launch_bounds__(kBlock) void sum_partial_kernel(const double* __restrict__ in, int64_t n,
+                                                              double* __restrict__ part) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) acc += in[i];
+    double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+__global__ __launch_bounds__(kBlock) void sum_final_kernel(const double* __restrict__ part, int n,
+                                                            double* __restrict__ out) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock) acc += part[i];
+    double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) *out = r;
+}
+
+int reduce_sum(hipStream_t s, const double* in, int64_t n, double* out) {
+    int nblk = (int)grid_for(n, kBlock, kRedBlocks);
+    double* part;
+    int rc = arena_get(kSlotReduce, sizeof(double) * kRedBlocks, (void**)&part);
+    if (rc) return rc;
+    hipLaunchKernelGGL(sum_partial_kernel, dim3(nblk), dim3(kBlock), 0, s, in, n, part);
+    ABD_CHECK_LAUNCH();
+    hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(kBlock), 0, s, part, nblk, out);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+// ------------------------------------------------------------------- dot
+__global__ __launch_bounds__(kBlock) void dot_partial_kernel(const double* __restrict__ a,
+                                                              const double* __restrict__ b, int64_t n,
+                                                              double* __restrict__ part) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        acc += a[i] * b[i];
+    double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+int read_i64(hipStream_t s, const int64_t* dev, int64_t* host) {
+    int64_t* h;
+    int rc = arena_host(sizeof(int64_t), (void**)&h);
+    if (rc) return rc;
+    ABD_CHECK(hipMemcpyAsync(h, dev, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    ABD_CHECK(hipStreamSynchronize(s));
+    *host = *h;
+    return ABD_OK;
+}
+
+}  // namespace abd
+
+using namespace abd;
+
+extern "C" {
+
+int abd_version(void) { return 10000; }
+
+const char* abd_error_string(int code) {
+    switch (code) {
+        case ABD_OK: return "ok";
+        case ABD_ERR_OVERFLOW: return "output list too small (required count returned)";
+        case ABD_ERR_ARG: return "invalid argument";
+        case ABD_ERR_UNSORTED: return "candidate list not grouped by body pair / body";
+        case ABD_ERR_NODEVICE: return "no HIP device";
+        default: break;
+    }
+    if (code < 0) return hipGetErrorString(static_cast<hipError_t>(-code));
+    return "unknown status";
+}
+
+int abd_release(void) { return arena_release_current(); }
+
+int abd_dot(abd_stream_t stream, const double* a, const double* b, int64_t n, double* out) {
+    if (n < 0 || (!a && n) || (!b && n) || !out) return ABD_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    int nblk = (int)grid_for(n, kBlock, kRedBlocks);
+    double* part;
+    int rc = arena_get(kSlotReduce, sizeof(double) * kRedBlocks, (void**)&part);
+    if (rc) return rc;
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(nblk), dim3(kBlock), 0, s, a, b, n, part);
+    ABD_CHECK_LAUNCH();
+    hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(kBlock), 0, s, part, nblk, out);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,746 @@
+// abd_contact.hip — barrier energy, gradient and PSD-projected Hessian of the
+// point-triangle (PT), edge-edge (EE) and vertex-ground (VG) contacts, and the
+// inertia / orthogonality terms (ipc.py, ipcsk/, psd/, orthogonal_energy.py).
+//
+// Hessian structure.  For one contact the reference forms (ipcsk/pt_ipc.py:111-185)
+//   Hl  = sum_k L_k q_k q_k^T / |q_k|^2          (9x9, psd/hl.py:156-195; L_4 = 2)
+//   H12 = D~^T Hl D~,  g12 = D~^T q_4           (D~ = dcdx (x) I3, 9x12)
+//   Hipc = b' H12 + b'' g12 g12^T,   Hi/Hj/Hij = J^T Hipc J   (J = [1, x0^T] (x) I3)
+// which is a sum of five rank-one terms in body space: with V_k = J^T D~^T q_k
+// (24 entries: 12 per body),
+//   H24 = sum_{k<4} (b' L_k/|q_k|^2) V_k V_k^T + (b' L_4/|q_4|^2 + b'') V_4 V_4^T,
+//   g24 = b' V_4.
+// A contact is therefore stored as 5 coefficients + 5 x 24 vectors, and the
+// body-pair blocks are reduced from those in a fixed order (no atomics):
+//   stage A  classify every candidate (active = s < d2hat and valid)      [HBM stream]
+//   stage B  ordered compaction of the active candidates
+//   stage C  per active contact: coefficients and V_k (one thread each)
+//   stage D  per body pair: sum of its contacts' 24x24 terms; the off-diagonal
+//            12x12 goes straight into the pair's two slots
+//   stage E  per body: diagonal slot and gradient += its pairs' diagonal parts
+//            + its vertex-ground terms.
+#include <hip/hip_runtime.h>
+
+#include "abd_block.h"
+#include "abd_device.h"
+#include "abd_internal.h"
+#include "abd_profile.h"
+
+namespace abd {
+
+constexpr int kCItems = 8;          // candidates per thread in the compaction passes
+constexpr int kContactStride = 128;  // doubles per stored contact: c[0..4], b', pad, then V[5][24]
+constexpr int kCoefOff = 0;
+constexpr int kB1Off = 5;
+constexpr int kVOff = 8;
+
+__device__ __forceinline__ SceneView view_of(const AbdScene& sc) {
+    SceneView v;
+    v.voff = sc.voff;
+    v.foff = sc.foff;
+    v.eoff = sc.eoff;
+    v.tris = sc.tris;
+    v.edges = sc.edges;
+    v.x0 = sc.x0;
+    return v;
+}
+
+__device__ __forceinline__ void load4(const double* __restrict__ x, const int64_t v[4], d3 out[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = ld3(x + 3 * v[k]);
+}
+
+// -------------------------------------------------------------- energy
+// ipcsk/energy.py:10-48: EE uses l^2, PT |e2p|^2, VG (y - ground)^2 (no validity).
+__global__ __launch_bounds__(kBlock) void energy_partial_kernel(AbdScene sc, AbdParams P,
+                                                                 const double* __restrict__ xk,
+                                                                 const int32_t* __restrict__ pt, int64_t npt,
+                                                                 const int32_t* __restrict__ ee, int64_t nee,
+                                                                 const int32_t* __restrict__ vg, int64_t nvg,
+                                                                 double* __restrict__ part) {
+    __shared__ double sh[kBlock];
+    const SceneView S = view_of(sc);
+    const int64_t n = nee + npt + nvg;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        double e = 0.0;
+        if (i < nee) {
+            int64_t vi[4];
+            d3 v[4];
+            fetch_ee_idx(S, ee + 5 * i, vi);
+            load4(xk, vi, v);
+            CMap c = cmap_ee(v[0], v[1], v[2], v[3]);
+            double d = signed_distance(c.e0, c.e1p, c.e2p);
+            if (valid_ee(c.beta, c.gamma)) e = barrier(d * d, P.d2hat, P.kappa);
+        } else if (i < nee + npt) {
+            int64_t k = i - nee, vi[4];
+            d3 v[4];
+            fetch_pt_idx(S, pt + 5 * k, vi);
+            load4(xk, vi, v);
+            CMap c = cmap_pt(v[0], v[1], v[2], v[3]);
+            if (valid_pt(c.beta, c.gamma)) e = barrier(dot(c.e2p, c.e2p), P.d2hat, P.kappa);
+        } else {
+            int64_t k = i - nee - npt;
+            int64_t vid = sc.voff[vg[2 * k]] + vg[2 * k + 1];
+            double d = xk[3 * vid + 1] - P.ground;
+            e = barrier(d * d, P.d2hat, P.kappa);
+        }
+        acc += e;
+    }
+    double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void sum_final_kernel2(const double* __restrict__ part, int n,
+                                                             double* __restrict__ out) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock) acc += part[i];
+    double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) *out = r;
+}
+
+// ------------------------------------------------------------ contact terms
+// One contact: writes coefficients and V_k into out[kContactStride]; returns
+// active.  kind 0 PT (x0 point; x1..x3 triangle), 1 EE (x0x1, x2x3).
+__device__ bool contact_terms(int kind, const d3 v[4], const d3 r[4], double d2hat, double kappa, int psd,
+                              double* __restrict__ out) {
+    CMap c = kind == 0 ? cmap_pt(v[0], v[1], v[2], v[3]) : cmap_ee(v[0], v[1], v[2], v[3]);
+    bool valid = kind == 0 ? valid_pt(c.beta, c.gamma) : valid_ee(c.beta, c.gamma);
+    double l = signed_distance(c.e0, c.e1p, c.e2p);
+    // _mask_valid: PT gh uses l^2 (pt_ipc.py:20-22), EE gh |e2p|^2 (ee_ipc.py:23)
+    double s = kind == 0 ? l * l : dot(c.e2p, c.e2p);
+    if (!(s < d2hat && valid)) return false;
+    double b1 = barrier_d1(s, d2hat, kappa);
+    double b2 = barrier_d2(s, d2hat, kappa);
+    // psd/hl.py:156-195 eigensystem of d2(l^2)/dc2
+    double e0pn = dot(c.e0, c.e0), e1pn = dot(c.e1p, c.e1p), e2pn = dot(c.e2p, c.e2p);
+    double f12 = sqrt(1.0 + 4.0 * (e1pn / e2pn));
+    double f02 = sqrt(1.0 + 4.0 * (e0pn / e2pn));
+    double lam[4];
+    lam[0] = -l / (2.0 * e1pn) * (1.0 + f12);
+    lam[1] = -l / (2.0 * e1pn) * (1.0 - f12);
+    lam[2] = -l / (2.0 * e0pn) * (1.0 + f02);
+    lam[3] = -l / (2.0 * e0pn) * (1.0 - f02);
+    const double lt = l / e2pn;
+    // dcdx (psd/vf.py:44-72, psd/ee.py:311-339), row-major 3x4
+    double D[12];
+    if (kind == 0) {
+        D[0] = 0.0; D[1] = 1.0; D[2] = -1.0; D[3] = 0.0;
+        D[4] = 0.0; D[5] = -c.alpha; D[6] = c.alpha - 1.0; D[7] = 1.0;
+        D[8] = 1.0; D[9] = -c.beta; D[10] = c.beta + c.gamma - 1.0; D[11] = -c.gamma;
+    } else {
+        D[0] = -1.0; D[1] = 1.0; D[2] = 0.0; D[3] = 0.0;
+        D[4] = c.alpha; D[5] = -c.alpha; D[6] = -1.0; D[7] = 1.0;
+        D[8] = c.beta - 1.0; D[9] = -c.beta; D[10] = c.gamma + 1.0; D[11] = -c.gamma;
+    }
+    const int nI = kind == 0 ? 1 : 2;  // contact vertices on body I
+    const d3 gl2 = (c.e2p * l) / e2pn;  // psd/hl.py:37-39
+    const d3 q4 = (gl2 * 2.0) * l;
+    for (int k = 0; k < 5; ++k) {
+        d3 qb[3];
+        double L;
+        if (k < 4) {
+            double om = lam[k] / (lam[k] - lt);
+            L = lam[k] * 2.0 * l;
+            if (kind == 0) L = L > 0.0 ? L : 0.0;  // pt_ipc.py:266-269 (EE unclamped, ee_ipc.py:133-136)
+            if (k < 2) {
+                qb[0] = mk(0.0, 0.0, 0.0);
+                qb[1] = c.e2p;
+                qb[2] = c.e1p * om;
+            } else {
+                qb[0] = c.e2p;
+                qb[1] = mk(0.0, 0.0, 0.0);
+                qb[2] = c.e0 * om;
+            }
+        } else {
+            L = 2.0;
+            qb[0] = mk(0.0, 0.0, 0.0);
+            qb[1] = mk(0.0, 0.0, 0.0);
+            qb[2] = q4;
+        }
+        double qn = dot(qb[0], qb[0]) + dot(qb[1], qb[1]) + dot(qb[2], qb[2]);
+        if (psd) {
+            // contact-space eigenvalue of b' Hl + b'' q4 q4^T, clamped at zero
+            double mu = k < 4 ? b1 * L : b1 * L + b2 * qn;
+            out[kCoefOff + k] = (mu > 0.0 ? mu : 0.0) * (1.0 / qn);
+        } else {
+            double w = L * (1.0 / qn);
+            out[kCoefOff + k] = k < 4 ? b1 * w : b1 * w + b2;
+        }
+        // u = D~^T q (12, per contact vertex), then V = J^T u per body
+        double V[24];
+#pragma unroll
+        for (int i = 0; i < 24; ++i) V[i] = 0.0;
+#pragma unroll
+        for (int vtx = 0; vtx < 4; ++vtx) {
+            d3 u = (qb[0] * D[0 * 4 + vtx] + qb[1] * D[1 * 4 + vtx]) + qb[2] * D[2 * 4 + vtx];
+            const double Jr[4] = {1.0, r[vtx].x, r[vtx].y, r[vtx].z};
+            const int base = vtx < nI ? 0 : 12;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                V[base + 3 * a + 0] += Jr[a] * u.x;
+                V[base + 3 * a + 1] += Jr[a] * u.y;
+                V[base + 3 * a + 2] += Jr[a] * u.z;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 24; ++i) out[kVOff + 24 * k + i] = V[i];
+    }
+    out[kB1Off] = b1;
+    return true;
+}
+
+// stage A: active flags over [PT | EE], plus ordering checks.
+__global__ __launch_bounds__(kBlock) void classify_kernel(AbdScene sc, AbdParams P, const double* __restrict__ x,
+                                                           const int32_t* __restrict__ pt, int64_t npt,
+                                                           const int32_t* __restrict__ ee, int64_t nee,
+                                                           const int32_t* __restrict__ vg, int64_t nvg,
+                                                           uint8_t* __restrict__ flag, int* __restrict__ err) {
+    const SceneView S = view_of(sc);
+    const int64_t n = npt + nee;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n + nvg; i += (int64_t)gridDim.x * kBlock) {
+        if (i >= n) {
+            int64_t k = i - n;
+            if (k > 0 && vg[2 * k] < vg[2 * (k - 1)]) *err = 1;
+            continue;
+        }
+        const bool is_ee = i >= npt;
+        const int64_t k = is_ee ? i - npt : i;
+        const int32_t* t5 = (is_ee ? ee : pt) + 5 * k;
+        if (k > 0 && t5[2] < t5[2 - 5]) *err = 1;
+        int64_t vi[4];
+        d3 v[4];
+        if (is_ee)
+            fetch_ee_idx(S, t5, vi);
+        else
+            fetch_pt_idx(S, t5, vi);
+        load4(x, vi, v);
+        bool act;
+        if (is_ee) {
+            CMap c = cmap_ee(v[0], v[1], v[2], v[3]);
+            act = P.ee && dot(c.e2p, c.e2p) < P.d2hat && valid_ee(c.beta, c.gamma);
+        } else {
+            CMap c = cmap_pt(v[0], v[1], v[2], v[3]);
+            double l = signed_distance(c.e0, c.e1p, c.e2p);
+            act = P.pt && l * l < P.d2hat && valid_pt(c.beta, c.gamma);
+        }
+        flag[i] = act ? 1 : 0;
+    }
+}
+
+// stage B: ordered compaction of the flags (count / write passes).
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void compact_flags_kernel(const uint8_t* __restrict__ flag, int64_t n,
+                                                                int64_t* __restrict__ bcnt,
+                                                                const int64_t* __restrict__ boff,
+                                                                int32_t* __restrict__ out) {
+    __shared__ int shw[kBlock / kWave];
+    int64_t base = (int64_t)blockIdx.x * kBlock * kCItems;
+    int64_t run = WRITE ? boff[blockIdx.x] : 0;
+    for (int it = 0; it < kCItems; ++it) {
+        int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
+        bool pred = i < n && flag[i];
+        int tot;
+        int r = block_rank(pred, shw, &tot);
+        if (WRITE && pred) out[run + r] = (int32_t)i;
+        run += tot;
+    }
+    if (!WRITE && threadIdx.x == 0) bcnt[blockIdx.x] = run;
+}
+
+// stage C: one thread per active contact.
+__global__ __launch_bounds__(kBlock) void contact_kernel(AbdScene sc, AbdParams P, const double* __restrict__ x,
+                                                          const int32_t* __restrict__ pt, int64_t npt,
+                                                          const int32_t* __restrict__ ee,
+                                                          const int32_t* __restrict__ act, int64_t nact,
+                                                          double* __restrict__ cbuf, int64_t* __restrict__ ckey,
+                                                          uint8_t* __restrict__ swap, uint8_t* __restrict__ hl_pt,
+                                                          uint8_t* __restrict__ hl_ee) {
+    const SceneView S = view_of(sc);
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nact; e += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = act[e];
+        const bool is_ee = i >= npt;
+        const int32_t* t5 = is_ee ? ee + 5 * (i - npt) : pt + 5 * i;
+        int64_t vi[4];
+        if (is_ee)
+            fetch_ee_idx(S, t5, vi);
+        else
+            fetch_pt_idx(S, t5, vi);
+        d3 v[4], r[4];
+        load4(x, vi, v);
+        load4(sc.x0, vi, r);
+        contact_terms(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, P.psd, cbuf + e * kContactStride);
+        ckey[e] = ((int64_t)(is_ee ? 1 : 0) << 32) | (int64_t)t5[2];
+        swap[e] = t5[0] > t5[1] ? 1 : 0;  // body I is ij1 (PT second order)
+        if (is_ee)
+            hl_ee[t5[1]] = 1;
+        else
+            hl_pt[t5[1]] = 1;
+    }
+}
+
+__device__ __forceinline__ int64_t lower_bound_key(const int64_t* __restrict__ a, int64_t n, int64_t key) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+constexpr int kPairThreads = 576;  // one thread per entry of the 24x24 body-pair block
+constexpr int kPairBatch = 8;
+constexpr int kPairBufStride = 2 * 144 + 24;  // diag(ij0), diag(ij1), g(ij0), g(ij1)
+
+// stage D: one workgroup per body pair ii.
+__global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64_t nij,
+                                                                    const double* __restrict__ cbuf,
+                                                                    const int64_t* __restrict__ ckey,
+                                                                    const uint8_t* __restrict__ swap, int64_t nact,
+                                                                    double* __restrict__ pbuf,
+                                                                    double* __restrict__ blocks) {
+    __shared__ double sc[kPairBatch][kContactStride];
+    __shared__ uint8_t ssw[kPairBatch];
+    const int64_t ii = blockIdx.x;
+    const int t = threadIdx.x;
+    int64_t seg[2][2];
+    for (int typ = 0; typ < 2; ++typ) {
+        int64_t key = ((int64_t)typ << 32) | ii;
+        seg[typ][0] = lower_bound_key(ckey, nact, key);
+        seg[typ][1] = lower_bound_key(ckey, nact, key + 1);
+    }
+    const int R = t / 24, C = t % 24;
+    double h = 0.0, gacc = 0.0;
+    bool any = false;
+    for (int typ = 0; typ < 2; ++typ) {
+        for (int64_t e0 = seg[typ][0]; e0 < seg[typ][1]; e0 += kPairBatch) {
+            int nbat = (int)((seg[typ][1] - e0) < kPairBatch ? (seg[typ][1] - e0) : kPairBatch);
+            for (int q = t; q < nbat * kContactStride; q += kPairThreads)
+                sc[q / kContactStride][q % kContactStride] = cbuf[(e0 + q / kContactStride) * kContactStride +
+                                                                  q % kContactStride];
+            if (t < nbat) ssw[t] = swap[e0 + t];
+            __syncthreads();
+            for (int b = 0; b < nbat; ++b) {
+                // body order [ij0 | ij1]: swap halves when body I is ij1
+                const int sw = ssw[b] ? 12 : 0;
+                const int r = (R + sw) % 24, cc = (C + sw) % 24;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    const double* V = &sc[b][kVOff + 24 * k];
+                    h += sc[b][kCoefOff + k] * (V[r] * V[cc]);
+                }
+                if (t < 24) gacc += sc[b][kB1Off] * sc[b][kVOff + 24 * 4 + (t + sw) % 24];
+            }
+            any = true;
+            __syncthreads();
+        }
+    }
+    double* pb = pbuf + ii * kPairBufStride;
+    // diagonal parts to the pair buffer (stage E adds them per body)
+    if (R < 12 && C < 12) pb[R * 12 + C] = h;
+    if (R >= 12 && C >= 12) pb[144 + (R - 12) * 12 + (C - 12)] = h;
+    if (t < 24) pb[288 + t] = gacc;
+    // off-diagonal: upper slot nb + ii holds rows ij0 x cols ij1, lower slot its transpose
+    if (any && R < 12 && C >= 12) {
+        const int c2 = C - 12;
+        const int bi = R / 3, rr = R % 3, bj = c2 / 3, cc = c2 % 3;
+        const int64_t up = nb + ii, lo = nb + nij + ii;
+        blocks[9 * (16 * up + bi + 4 * bj) + 3 * rr + cc] += h;
+        blocks[9 * (16 * lo + bj + 4 * bi) + 3 * cc + rr] += h;
+    }
+}
+
+__device__ __forceinline__ int64_t lower_bound_body(const int32_t* __restrict__ vg, int64_t n, int b) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (vg[2 * mid] < b)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// stage E: one workgroup per body (144 Hessian entries + 12 gradient entries).
+__global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdParams P,
+                                                                const double* __restrict__ x, int64_t nij,
+                                                                const int32_t* __restrict__ row_ptr,
+                                                                const int32_t* __restrict__ slots,
+                                                                const double* __restrict__ pbuf,
+                                                                const int32_t* __restrict__ vg, int64_t nvg,
+                                                                double* __restrict__ g, double* __restrict__ blocks,
+                                                                const uint8_t* __restrict__ hl_pt,
+                                                                const uint8_t* __restrict__ hl_ee,
+                                                                uint8_t* __restrict__ highlight) {
+    const int b = blockIdx.x;
+    const int t = threadIdx.x;
+    const int nb = sc.n_bodies;
+    if (t >= 156) return;
+    const bool isg = t >= 144;
+    const int R = isg ? t - 144 : t / 12, C = isg ? 0 : t % 12;
+    double acc = 0.0;
+    if (nij > 0) {
+        for (int e = row_ptr[b]; e < row_ptr[b + 1]; ++e) {
+            const int s = slots[e];
+            if (s < nb) continue;
+            const int64_t ii = s < nb + nij ? s - nb : s - nb - nij;
+            const int side = s < nb + nij ? 0 : 1;  // upper slot in row b: b is ij0
+            const double* pb = pbuf + ii * kPairBufStride;
+            acc += isg ? pb[288 + 12 * side + R] : pb[144 * side + R * 12 + C];
+        }
+    }
+    if (P.vg && nvg > 0) {
+        // ipcsk/vg_ipc.py:5-34 (no 2 b' grad d grad d^T term, D11)
+        const int64_t k0 = lower_bound_body(vg, nvg, b), k1 = lower_bound_body(vg, nvg, b + 1);
+        const int ia = R / 3, ra = R % 3, ja = C / 3, ca = C % 3;
+        for (int64_t k = k0; k < k1; ++k) {
+            const int64_t vid = sc.voff[b] + vg[2 * k + 1];
+            const double y = x[3 * vid + 1];
+            const d3 vt = ld3(sc.x0 + 3 * vid);
+            const double d = y - P.ground;
+            const double d2b = barrier_d2(d * d, P.d2hat, P.kappa);
+            const double db = barrier_d1(d * d, P.d2hat, P.kappa);
+            const double th_i = ia == 0 ? 1.0 : comp(vt, ia - 1);
+            if (isg) {
+                const double nr = ra == 1 ? 1.0 : 0.0;
+                acc += (((d * 2.0) * nr) * th_i) * db;
+            } else {
+                const double th_j = ja == 0 ? 1.0 : comp(vt, ja - 1);
+                const double scl = th_i * th_j * d2b * 4.0 * d * d;
+                const double nn = (ra == 1 ? 1.0 : 0.0) * (ca == 1 ? 1.0 : 0.0);
+                acc += nn * scl;
+            }
+        }
+    }
+    if (isg) {
+        g[3 * (4 * (int64_t)b + R / 3) + R % 3] += acc;
+    } else {
+        const int ia = R / 3, ra = R % 3, ja = C / 3, ca = C % 3;
+        blocks[9 * (16 * (int64_t)b + ia + 4 * ja) + 3 * ra + ca] += acc;
+    }
+    if (t == 0 && highlight) highlight[b] = (uint8_t)((hl_pt[b] ? 1 : 0) | (hl_ee[b] ? 2 : 0));
+}
+
+__global__ void zero_u8_kernel(uint8_t* a, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = 0;
+}
+__global__ void zero_int_kernel(int* a) { *a = 0; }
+
+// Per-contact entry (tests, microbenchmarks): dense 24x24 and gradient per contact.
+__global__ __launch_bounds__(kBlock) void pair_gh_kernel(int kind, AbdParams P, const double* __restrict__ x4,
+                                                          const double* __restrict__ r4, int64_t n,
+                                                          uint8_t* __restrict__ active, double* __restrict__ g24,
+                                                          double* __restrict__ H24, double* __restrict__ scratch) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        d3 v[4], r[4];
+        for (int k = 0; k < 4; ++k) {
+            v[k] = ld3(x4 + 12 * i + 3 * k);
+            r[k] = ld3(r4 + 12 * i + 3 * k);
+        }
+        double* buf = scratch + i * kContactStride;
+        bool act = contact_terms(kind, v, r, P.d2hat, P.kappa, P.psd, buf);
+        active[i] = act ? 1 : 0;
+        for (int a = 0; a < 24; ++a) {
+            g24[24 * i + a] = act ? buf[kB1Off] * buf[kVOff + 24 * 4 + a] : 0.0;
+            for (int c = 0; c < 24; ++c) {
+                double h = 0.0;
+                if (act)
+                    for (int k = 0; k < 5; ++k) {
+                        const double* V = buf + kVOff + 24 * k;
+                        h += buf[kCoefOff + k] * (V[a] * V[c]);
+                    }
+                H24[576 * i + 24 * a + c] = h;
+            }
+        }
+    }
+}
+
+// -------------------------------------------------------------- inertia
+__device__ __forceinline__ double row_dot(const double* A, int i, int j) {
+    return A[3 * i] * A[3 * j] + A[3 * i + 1] * A[3 * j + 1] + A[3 * i + 2] * A[3 * j + 2];
+}
+// orthogonal_energy.py:129-135
+__device__ __forceinline__ void tilde(const AbdParams& P, const double* A0, const double* Adot, const double* p0,
+                                      const double* pdot, double At[9], double pt[3]) {
+    for (int i = 0; i < 9; ++i) At[i] = A0[i] + P.dt * Adot[i];
+    const double gv[3] = {0.0, P.gravity, 0.0};
+    for (int k = 0; k < 3; ++k) pt[k] = p0[k] + P.dt * pdot[k] + P.dt * P.dt * gv[k];
+}
+
+// orthogonal_energy.py:25-51 (per body), partial sums per workgroup.
+__global__ __launch_bounds__(kBlock) void inertia_energy_kernel(int nb, AbdParams P, const double* __restrict__ mass,
+                                                                 const double* __restrict__ I0v,
+                                                                 const double* __restrict__ Ak,
+                                                                 const double* __restrict__ pk,
+                                                                 const double* __restrict__ A0,
+                                                                 const double* __restrict__ p0,
+                                                                 const double* __restrict__ Adot,
+                                                                 const double* __restrict__ pdot,
+                                                                 double* __restrict__ part) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int b = blockIdx.x * kBlock + threadIdx.x; b < nb; b += gridDim.x * kBlock) {
+        double At[9], pt[3];
+        tilde(P, A0 + 9 * b, Adot + 9 * b, p0 + 3 * b, pdot + 3 * b, At, pt);
+        const double* A = Ak + 9 * b;
+        const double m = mass ? mass[b] : P.mass, i0 = I0v ? I0v[b] : P.I0;
+        double dq0[3], dq[9];
+        for (int k = 0; k < 3; ++k) dq0[k] = pk[3 * b + k] - pt[k];
+        for (int k = 0; k < 9; ++k) dq[k] = A[k] - At[k];
+        const double n0 = dq0[0] * dq0[0] + dq0[1] * dq0[1] + dq0[2] * dq0[2];
+        const double n1 = dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2];
+        const double n2 = dq[3] * dq[3] + dq[4] * dq[4] + dq[5] * dq[5];
+        const double n3 = dq[6] * dq[6] + dq[7] * dq[7] + dq[8] * dq[8];
+        const double dqTMdq = n0 * m + (n1 + n2 + n3) * i0;
+        double eo = 0.0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double term = row_dot(A, i, j) - (i == j ? 1.0 : 0.0);
+                eo += term * term;
+            }
+        eo = eo * P.stiffness;
+        acc += eo * P.dt * P.dt + 0.5 * dqTMdq;
+    }
+    double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// orthogonal_energy.py:99-112 (overwrites g)
+__global__ void inertia_gradient_kernel(int nb, AbdParams P, const double* __restrict__ mass,
+                                        const double* __restrict__ I0v, const double* __restrict__ A,
+                                        const double* __restrict__ p, const double* __restrict__ A0,
+                                        const double* __restrict__ p0, const double* __restrict__ Adot,
+                                        const double* __restrict__ pdot, double* __restrict__ g) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const double* Ab = A + 9 * b;
+    double At[9], pt[3];
+    tilde(P, A0 + 9 * b, Adot + 9 * b, p0 + 3 * b, pdot + 3 * b, At, pt);
+    const double m = mass ? mass[b] : P.mass, i0 = I0v ? I0v[b] : P.I0;
+    for (int k = 0; k < 3; ++k) g[3 * (4 * b) + k] = 0.0 + (p[3 * b + k] - pt[k]) * m;
+    for (int ii = 1; ii < 4; ++ii) {
+        const int i = ii - 1;
+        double gr[3] = {-Ab[3 * i], -Ab[3 * i + 1], -Ab[3 * i + 2]};
+        for (int j = 0; j < 3; ++j) {
+            double d = row_dot(Ab, i, j);
+            for (int k = 0; k < 3; ++k) gr[k] += d * Ab[3 * j + k];
+        }
+        for (int k = 0; k < 3; ++k)
+            g[3 * (4 * b + ii) + k] = P.dt * P.dt * (gr[k] * 4.0 * P.stiffness) + (Ab[3 * i + k] - At[3 * i + k]) * i0;
+    }
+}
+
+// orthogonal_energy.py:63-96 (+= into the diagonal slots); one thread per (body, sub-block).
+__global__ void inertia_hessian_kernel(int nb, AbdParams P, const double* __restrict__ mass,
+                                       const double* __restrict__ I0v, const double* __restrict__ A,
+                                       double* __restrict__ blocks) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 16 * nb) return;
+    const int b = t / 16, e = t % 16, ii = e % 4, jj = e / 4;
+    const double* Ab = A + 9 * b;
+    const double m_ = mass ? mass[b] : P.mass, i0 = I0v ? I0v[b] : P.I0;
+    const double m = ii == jj ? (ii == 0 ? m_ : i0) : 0.0;
+    double dh[9] = {m, 0, 0, 0, m, 0, 0, 0, m};
+    if (ii > 0 && jj > 0) {
+        const int i = ii - 1, j = jj - 1;
+        double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (i == j) {
+            const double qq = row_dot(Ab, i, i) - 1.0;
+            for (int k = 0; k < 3; ++k)
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) h[3 * r + c] += Ab[3 * k + r] * Ab[3 * k + c];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) h[3 * r + c] += Ab[3 * i + r] * Ab[3 * i + c] + (r == c ? qq : 0.0);
+        } else {
+            const double d = row_dot(Ab, j, i);
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) h[3 * r + c] = Ab[3 * j + r] * Ab[3 * i + c] + (r == c ? d : 0.0);
+        }
+        for (int k = 0; k < 9; ++k) dh[k] += h[k] * 4.0 * P.stiffness * P.dt * P.dt;
+    }
+    double* B = blocks + 9 * (16 * (int64_t)b + ii + jj * 4);
+    for (int k = 0; k < 9; ++k) B[k] += dh[k];
+}
+
+}  // namespace abd
+
+using namespace abd;
+
+extern "C" {
+
+int abd_contact_energy(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* xk,
+                       const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
+                       double* E) {
+    if (!sc || !P || !xk || !E || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
+    if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    // the switches of const_params.py:42-44 drop a whole list
+    if (!P->pt) npt = 0;
+    if (!P->ee) nee = 0;
+    if (!P->vg) nvg = 0;
+    const int64_t n = npt + nee + nvg;
+    const int nblk = (int)grid_for(n, kBlock, 512);
+    double* part;
+    int rc = arena_get(kSlotReduce, sizeof(double) * 512, (void**)&part);
+    if (rc) return rc;
+    int ph = prof_start(kProfEnergy, s, 128.0 * npt + 132.0 * nee + 32.0 * nvg);
+    hipLaunchKernelGGL(energy_partial_kernel, dim3(nblk), dim3(kBlock), 0, s, *sc, *P, xk, pt, npt, ee, nee, vg, nvg,
+                       part);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    hipLaunchKernelGGL(sum_final_kernel2, dim3(1), dim3(kBlock), 0, s, part, nblk, E);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x, const int32_t* pt,
+                   int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t nij,
+                   const int32_t* row_ptr, const int32_t* slots, double* g, double* blocks, uint8_t* highlight,
+                   int64_t* n_active) {
+    if (!sc || !P || !x || !g || !blocks || npt < 0 || nee < 0 || nvg < 0 || nij < 0) return ABD_ERR_ARG;
+    if ((npt && !pt) || (nee && !ee) || (nvg && !vg) || (nij && (!row_ptr || !slots))) return ABD_ERR_ARG;
+    if (nij == 0 && (npt || nee)) return ABD_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    const int nb = sc->n_bodies;
+    if (n_active) *n_active = 0;
+    if (nb == 0) return ABD_OK;
+    if (!P->vg) nvg = 0;
+    const int64_t n = npt + nee;
+    int rc;
+    // misc: err flag + highlight bytes
+    uint8_t* misc;
+    if ((rc = arena_get(kSlotMisc, 64 + 2 * (size_t)nb, (void**)&misc))) return rc;
+    int* err = reinterpret_cast<int*>(misc);
+    uint8_t* hl_pt = misc + 64;
+    uint8_t* hl_ee = hl_pt + nb;
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, err);
+    ABD_CHECK_LAUNCH();
+    hipLaunchKernelGGL(zero_u8_kernel, dim3(grid_for(2 * nb)), dim3(kBlock), 0, s, hl_pt, (int64_t)(2 * nb));
+    ABD_CHECK_LAUNCH();
+    int64_t nact = 0;
+    int32_t* act = nullptr;
+    if (n + nvg > 0) {
+        uint8_t* flag;
+        if ((rc = arena_get(kSlotFlags, (size_t)n + 1, (void**)&flag))) return rc;
+        int ph = prof_start(kProfClassify, s, 129.0 * npt + 133.0 * nee + 8.0 * nvg);
+        hipLaunchKernelGGL(classify_kernel, dim3(grid_for(n + nvg, kBlock, 8192)), dim3(kBlock), 0, s, *sc, *P, x,
+                           pt, npt, ee, nee, vg, nvg, flag, err);
+        ABD_CHECK_LAUNCH();
+        prof_stop(ph, s);
+        if (n > 0) {
+            const int64_t nblk = ceil_div(n, (int64_t)kBlock * kCItems);
+            int64_t *bcnt, *boff;
+            if ((rc = arena_get(kSlotScanA, sizeof(int64_t) * (nblk + 1), (void**)&bcnt))) return rc;
+            if ((rc = arena_get(kSlotScanB, sizeof(int64_t) * (nblk + 1), (void**)&boff))) return rc;
+            if ((rc = arena_get(kSlotList, sizeof(int32_t) * (size_t)n, (void**)&act))) return rc;
+            hipLaunchKernelGGL(compact_flags_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s,
+                               (const uint8_t*)flag, n, bcnt, (const int64_t*)nullptr, (int32_t*)nullptr);
+            ABD_CHECK_LAUNCH();
+            if ((rc = scan_exclusive_i64(s, bcnt, boff, nblk, boff + nblk))) return rc;
+            hipLaunchKernelGGL(compact_flags_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s,
+                               (const uint8_t*)flag, n, bcnt, (const int64_t*)boff, act);
+            ABD_CHECK_LAUNCH();
+            if ((rc = read_i64(s, boff + nblk, &nact))) return rc;
+        }
+    }
+    int herr = 0;
+    ABD_CHECK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    ABD_CHECK(hipStreamSynchronize(s));
+    if (herr) return ABD_ERR_UNSORTED;
+    double* pbuf = nullptr;
+    if (nij > 0) {
+        if ((rc = arena_get(kSlotPair, sizeof(double) * kPairBufStride * (size_t)nij, (void**)&pbuf))) return rc;
+    }
+    if (nact > 0) {
+        double* cbuf;
+        int64_t* ckey;
+        uint8_t* swp;
+        if ((rc = arena_get(kSlotContact, (sizeof(double) * kContactStride + sizeof(int64_t) + 1) * (size_t)nact,
+                            (void**)&cbuf)))
+            return rc;
+        ckey = reinterpret_cast<int64_t*>(cbuf + kContactStride * nact);
+        swp = reinterpret_cast<uint8_t*>(ckey + nact);
+        int ph = prof_start(kProfContact, s, (4.0 + 20 + 16 + 192 + 8.0 * kContactStride + 9) * nact);
+        hipLaunchKernelGGL(contact_kernel, dim3(grid_for(nact, kBlock, 65535)), dim3(kBlock), 0, s, *sc, *P, x, pt,
+                           npt, ee, (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee);
+        ABD_CHECK_LAUNCH();
+        prof_stop(ph, s);
+        ph = prof_start(kProfPairReduce, s, (8.0 * kContactStride + 9) * nact + 8.0 * kPairBufStride * nij);
+        hipLaunchKernelGGL(pair_reduce_kernel, dim3((unsigned)nij), dim3(kPairThreads), 0, s, nb, nij,
+                           (const double*)cbuf, (const int64_t*)ckey, (const uint8_t*)swp, nact, pbuf, blocks);
+        ABD_CHECK_LAUNCH();
+        prof_stop(ph, s);
+    } else if (nij > 0) {
+        ABD_CHECK(hipMemsetAsync(pbuf, 0, sizeof(double) * kPairBufStride * (size_t)nij, s));
+    }
+    int phf = prof_start(kProfBodyFinalize, s, 2.0 * 8 * 156 * nb + 8.0 * kPairBufStride * nij + 40.0 * nvg);
+    hipLaunchKernelGGL(body_finalize_kernel, dim3(nb), dim3(kBlock), 0, s, *sc, *P, x, nij, row_ptr, slots,
+                       (const double*)pbuf, vg, nvg, g, blocks, (const uint8_t*)hl_pt, (const uint8_t*)hl_ee,
+                       highlight);
+    ABD_CHECK_LAUNCH();
+    prof_stop(phf, s);
+    ABD_CHECK(hipStreamSynchronize(s));
+    if (n_active) *n_active = nact;
+    return ABD_OK;
+}
+
+int abd_pair_gh(abd_stream_t stream, int32_t kind, const AbdParams* P, const double* x4, const double* r4, int64_t n,
+                uint8_t* active, double* g24, double* H24) {
+    if (!P || n < 0 || (n && (!x4 || !r4 || !active || !g24 || !H24)) || (kind != 0 && kind != 1)) return ABD_ERR_ARG;
+    if (n == 0) return ABD_OK;
+    double* scratch;
+    int rc = arena_get(kSlotContact, sizeof(double) * kContactStride * (size_t)n, (void**)&scratch);
+    if (rc) return rc;
+    hipLaunchKernelGGL(pair_gh_kernel, dim3(grid_for(n)), dim3(kBlock), 0, to_stream(stream), kind, *P, x4, r4, n,
+                       active, g24, H24, scratch);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_inertia_energy(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* Ak,
+                       const double* pk, const double* A0, const double* p0, const double* Adot, const double* pdot,
+                       double* E) {
+    if (!sc || !P || !Ak || !pk || !A0 || !p0 || !Adot || !pdot || !E) return ABD_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    const int nb = sc->n_bodies;
+    const int nblk = (int)grid_for(nb, kBlock, 512);
+    double* part;
+    int rc = arena_get(kSlotReduce, sizeof(double) * 512, (void**)&part);
+    if (rc) return rc;
+    hipLaunchKernelGGL(inertia_energy_kernel, dim3(nblk), dim3(kBlock), 0, s, nb, *P, sc->mass, sc->I0, Ak, pk, A0,
+                       p0, Adot, pdot, part);
+    ABD_CHECK_LAUNCH();
+    hipLaunchKernelGGL(sum_final_kernel2, dim3(1), dim3(kBlock), 0, s, part, nblk, E);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_inertia_gradient(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* A, const double* p,
+                         const double* A0, const double* p0, const double* Adot, const double* pdot, double* g) {
+    if (!sc || !P || !A || !p || !A0 || !p0 || !Adot || !pdot || !g) return ABD_ERR_ARG;
+    const int nb = sc->n_bodies;
+    if (!nb) return ABD_OK;
+    hipLaunchKernelGGL(inertia_gradient_kernel, dim3(grid_for(nb)), dim3(kBlock), 0, to_stream(stream), nb, *P,
+                       sc->mass, sc->I0, A, p, A0, p0, Adot, pdot, g);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_inertia_hessian(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* A,
+                        double* blocks) {
+    if (!sc || !P || !A || !blocks) return ABD_ERR_ARG;
+    const int nb = sc->n_bodies;
+    if (!nb) return ABD_OK;
+    hipLaunchKernelGGL(inertia_hessian_kernel, dim3(grid_for(16 * (int64_t)nb)), dim3(kBlock), 0, to_stream(stream),
+                       nb, *P, sc->mass, sc->I0, A, blocks);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+}  // extern "C"

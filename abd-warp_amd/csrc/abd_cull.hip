@@ -1,0 +1,541 @@
+// abd_cull.hip — candidate culling (culling.py / culling2.py) and the sparse
+// structure of the Newton matrix (orthogonal_energy.py:138-165).
+//
+// Culling replaces the reference's per-body Warp BVHs with an exact two-level
+// scheme that needs no tree build or refit:
+//   1. body pairs: every (i < j) pair tested against the FP32 body boxes;
+//   2. per body pair and side, the primitives whose box overlaps the hull of the
+//      OTHER side's primitive boxes are kept (a primitive box that misses that
+//      hull cannot overlap any box of the other side, so nothing is lost);
+//   3. the filtered lists are tested exhaustively in chunks of kChunk tests.
+// Every list is produced by a count -> exclusive scan -> write sequence with
+// wave-ballot ranks, so outputs are emitted in sorted order (ii, order, a, b)
+// and are identical from run to run.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "abd_block.h"
+#include "abd_device.h"
+#include "abd_internal.h"
+#include "abd_profile.h"
+
+namespace abd {
+
+__device__ __forceinline__ bool overlap(const float* __restrict__ alo, const float* __restrict__ ahi,
+                                        const float* __restrict__ blo, const float* __restrict__ bhi) {
+    return !(alo[0] > bhi[0] || ahi[0] < blo[0] || alo[1] > bhi[1] || ahi[1] < blo[1] || alo[2] > bhi[2] ||
+             ahi[2] < blo[2]);
+}
+
+// ---------------------------------------------------------------- bodies
+constexpr int kItems = 8;  // pairs per thread in the body-pair compaction
+
+// k -> (i, j), i < j.  Order 0: row-major (i, j); order 1: sorted by (j, i).
+__device__ __forceinline__ void pair_of(int64_t k, int nb, int order, int& i, int& j) {
+    if (order == 0) {
+        double n2 = 2.0 * nb - 1.0;
+        int64_t r = (int64_t)floor((n2 - sqrt(n2 * n2 - 8.0 * (double)k)) * 0.5);
+        if (r < 0) r = 0;
+        auto off = [nb](int64_t q) { return q * nb - q * (q + 1) / 2; };
+        while (r > 0 && off(r) > k) --r;
+        while (off(r + 1) <= k) ++r;
+        i = (int)r;
+        j = (int)(k - off(r) + r + 1);
+    } else {
+        int64_t c = (int64_t)floor((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
+        if (c < 1) c = 1;
+        while (c > 1 && c * (c - 1) / 2 > k) --c;
+        while ((c + 1) * c / 2 <= k) ++c;
+        j = (int)c;
+        i = (int)(k - c * (c - 1) / 2);
+    }
+}
+
+__device__ __forceinline__ bool body_pred(int i, int j, const int32_t* __restrict__ scene,
+                                          const float* __restrict__ lo, const float* __restrict__ hi) {
+    if (scene && scene[i] != scene[j]) return false;
+    return overlap(lo + 3 * i, hi + 3 * i, lo + 3 * j, hi + 3 * j);
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void body_pairs_kernel(int nb, int64_t npairs, int order,
+                                                             const int32_t* __restrict__ scene,
+                                                             const float* __restrict__ lo,
+                                                             const float* __restrict__ hi, int64_t* __restrict__ bcnt,
+                                                             const int64_t* __restrict__ boff, int32_t* out,
+                                                             int64_t cap) {
+    __shared__ int shw[kBlock / kWave];
+    int64_t base = (int64_t)blockIdx.x * kBlock * kItems;
+    int64_t run = WRITE ? boff[blockIdx.x] : 0;
+    for (int it = 0; it < kItems; ++it) {
+        int64_t k = base + (int64_t)it * kBlock + threadIdx.x;
+        bool pred = false;
+        int i = 0, j = 0;
+        if (k < npairs) {
+            pair_of(k, nb, order, i, j);
+            pred = body_pred(i, j, scene, lo, hi);
+        }
+        int tot;
+        int r = block_rank(pred, shw, &tot);
+        if (WRITE && pred) {
+            int64_t o = run + r;
+            if (o < cap) {
+                out[2 * o] = i;
+                out[2 * o + 1] = j;
+            }
+        }
+        run += tot;
+    }
+    if (!WRITE && threadIdx.x == 0) bcnt[blockIdx.x] = run;
+}
+
+// ------------------------------------------------------------- primitives
+constexpr int kChunk = kBlock * 16;
+
+// Hull of the FP32 boxes of each body's primitives (one workgroup per body).
+__global__ __launch_bounds__(kBlock) void hull_kernel(const int32_t* __restrict__ offs, const float* __restrict__ lo,
+                                                       const float* __restrict__ hi, float* __restrict__ hlo,
+                                                       float* __restrict__ hhi) {
+    __shared__ float sh[kBlock];
+    const int b = blockIdx.x;
+    const int p0 = offs[b], p1 = offs[b + 1];
+    for (int k = 0; k < 3; ++k) {
+        float mn = INFINITY, mx = -INFINITY;
+        for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
+            mn = fminf(mn, lo[3 * (int64_t)p + k]);
+            mx = fmaxf(mx, hi[3 * (int64_t)p + k]);
+        }
+        float a = block_minf(mn, sh);
+        float c = block_maxf(mx, sh);
+        if (threadIdx.x == 0) {
+            hlo[3 * b + k] = a;
+            hhi[3 * b + k] = c;
+        }
+    }
+}
+
+struct PrimSets {
+    const int32_t* off1;  // per-body offsets of set 1 (PT points / EE edges)
+    const int32_t* off2;  // set 2 (PT triangles / EE edges)
+    const float *lo1, *hi1, *lo2, *hi2;
+    const float *hlo1, *hhi1, *hlo2, *hhi2;  // per-body hulls
+};
+
+// unit -> (ii, I (set-1 body), J (set-2 body))
+__device__ __forceinline__ void unit_bodies(int pt, int64_t u, const int32_t* __restrict__ ij, int64_t& ii, int& I,
+                                            int& J) {
+    if (pt) {
+        ii = u >> 1;
+        int a = ij[2 * ii], b = ij[2 * ii + 1];
+        I = (u & 1) ? b : a;  // intersection_prims: odd threads take (ij1, ij0), culling.py:88-91
+        J = a + b - I;
+    } else {
+        ii = u;
+        I = ij[2 * ii];
+        J = ij[2 * ii + 1];
+    }
+}
+
+// One workgroup per (unit, side): side 0 keeps set-1 prims of I overlapping hull2[J],
+// side 1 keeps set-2 prims of J overlapping hull1[I]; local ids in ascending order.
+__global__ __launch_bounds__(kBlock) void filter_kernel(int pt, const int32_t* __restrict__ ij, PrimSets S,
+                                                         int64_t maxs, int32_t* __restrict__ flist,
+                                                         int64_t* __restrict__ fcnt) {
+    __shared__ int shw[kBlock / kWave];
+    const int64_t u = blockIdx.x >> 1;
+    const int side = blockIdx.x & 1;
+    int64_t ii;
+    int I, J;
+    unit_bodies(pt, u, ij, ii, I, J);
+    const int32_t* offs = side ? S.off2 : S.off1;
+    const float* lo = side ? S.lo2 : S.lo1;
+    const float* hi = side ? S.hi2 : S.hi1;
+    const int body = side ? J : I;
+    const int other = side ? I : J;
+    const float* hlo = side ? S.hlo1 + 3 * other : S.hlo2 + 3 * other;
+    const float* hhi = side ? S.hhi1 + 3 * other : S.hhi2 + 3 * other;
+    const int p0 = offs[body], p1 = offs[body + 1];
+    int32_t* dst = flist + blockIdx.x * maxs;
+    int64_t run = 0;
+    for (int base = p0; base < p1; base += kBlock) {
+        int p = base + threadIdx.x;
+        bool pred = p < p1 && overlap(lo + 3 * (int64_t)p, hi + 3 * (int64_t)p, hlo, hhi);
+        int tot;
+        int r = block_rank(pred, shw, &tot);
+        if (pred) dst[run + r] = p - p0;
+        run += tot;
+    }
+    if (threadIdx.x == 0) fcnt[blockIdx.x] = run;
+}
+
+__global__ void unit_chunks_kernel(int64_t nunits, const int64_t* __restrict__ fcnt, int64_t* __restrict__ nchunk) {
+    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    int64_t nt = fcnt[2 * u] * fcnt[2 * u + 1];
+    nchunk[u] = (nt + kChunk - 1) / kChunk;
+}
+
+// last u with choff[u] <= c (choff non-decreasing, choff[0] = 0)
+__device__ __forceinline__ int64_t find_unit(const int64_t* __restrict__ choff, int64_t nunits, int64_t c) {
+    int64_t lo = 0, hi = nunits;  // invariant: choff[lo] <= c, answer in [lo, hi)
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if (choff[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void prim_pairs_kernel(int pt, const int32_t* __restrict__ ij, PrimSets S,
+                                                             int64_t maxs, const int32_t* __restrict__ flist,
+                                                             const int64_t* __restrict__ fcnt,
+                                                             const int64_t* __restrict__ choff, int64_t nunits,
+                                                             const int64_t* __restrict__ total_chunks,
+                                                             int64_t* __restrict__ ccnt,
+                                                             const int64_t* __restrict__ ccoff,
+                                                             int32_t* __restrict__ out, int64_t cap) {
+    __shared__ int shw[kBlock / kWave];
+    const int64_t T = *total_chunks;
+    for (int64_t c = blockIdx.x; c < T; c += gridDim.x) {
+        int64_t u = find_unit(choff, nunits, c);
+        int64_t ii;
+        int I, J;
+        unit_bodies(pt, u, ij, ii, I, J);
+        const uint32_t nA = (uint32_t)fcnt[2 * u], nB = (uint32_t)fcnt[2 * u + 1];
+        const uint32_t nt = nA * nB;
+        const int32_t* LA = flist + (2 * u) * maxs;
+        const int32_t* LB = flist + (2 * u + 1) * maxs;
+        const int oA = S.off1[I], oB = S.off2[J];
+        const uint32_t f0 = (uint32_t)(c - choff[u]) * kChunk;
+        const uint32_t f1 = f0 + kChunk < nt ? f0 + kChunk : nt;
+        int64_t run = WRITE ? ccoff[c] : 0;
+        for (uint32_t base = f0; base < f1; base += kBlock) {
+            uint32_t f = base + threadIdx.x;
+            bool pred = false;
+            int32_t a = 0, b = 0;
+            if (f < f1) {
+                uint32_t i = f / nB, j = f - i * nB;
+                a = LA[i];
+                b = LB[j];
+                int64_t ga = oA + a, gb = oB + b;
+                pred = overlap(S.lo1 + 3 * ga, S.hi1 + 3 * ga, S.lo2 + 3 * gb, S.hi2 + 3 * gb);
+            }
+            int tot;
+            int r = block_rank(pred, shw, &tot);
+            if (WRITE && pred) {
+                int64_t o = run + r;
+                if (o < cap) {
+                    int32_t* t5 = out + 5 * o;
+                    t5[0] = I;
+                    t5[1] = J;
+                    t5[2] = (int32_t)ii;
+                    t5[3] = a;
+                    t5[4] = b;
+                }
+            }
+            run += tot;
+        }
+        if (!WRITE && threadIdx.x == 0) ccnt[c] = run;
+    }
+}
+
+// ------------------------------------------------------------------- VG
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void vg_kernel(int64_t nv, const int32_t* __restrict__ vert_body,
+                                                     const int32_t* __restrict__ voff, const float* __restrict__ blo,
+                                                     const float* __restrict__ plo, const float* __restrict__ phi,
+                                                     double ythresh, float fground, int64_t* __restrict__ bcnt,
+                                                     const int64_t* __restrict__ boff, int32_t* __restrict__ out,
+                                                     int64_t cap) {
+    __shared__ int shw[kBlock / kWave];
+    const float qlo[3] = {-1e3f, -1e3f, -1e3f};
+    const float qhi[3] = {1e3f, fground, 1e3f};
+    int64_t base = (int64_t)blockIdx.x * kBlock * kItems;
+    int64_t run = WRITE ? boff[blockIdx.x] : 0;
+    for (int it = 0; it < kItems; ++it) {
+        int64_t v = base + (int64_t)it * kBlock + threadIdx.x;
+        bool pred = false;
+        int b = 0;
+        if (v < nv) {
+            b = vert_body[v];
+            // culling.py:67: lowers[i][1] < 0.5 * dhat + ground (FP64)
+            pred = (double)blo[3 * b + 1] < ythresh && overlap(plo + 3 * v, phi + 3 * v, qlo, qhi);
+        }
+        int tot;
+        int r = block_rank(pred, shw, &tot);
+        if (WRITE && pred) {
+            int64_t o = run + r;
+            if (o < cap) {
+                out[2 * o] = b;
+                out[2 * o + 1] = (int32_t)(v - voff[b]);
+            }
+        }
+        run += tot;
+    }
+    if (!WRITE && threadIdx.x == 0) bcnt[blockIdx.x] = run;
+}
+
+// ------------------------------------------------------------ structure
+__global__ void triplets_kernel(int nb, int64_t nij, const int32_t* __restrict__ ij, int32_t* __restrict__ rows,
+                                int32_t* __restrict__ cols) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t nslot = nb + 2 * nij;
+    if (t >= 16 * nslot) return;
+    int64_t i = t / 16;
+    int e = (int)(t % 16), ii = e % 4, jj = e / 4;
+    int64_t I, J;
+    if (i < nb) {
+        I = J = i;
+    } else if (i < nb + nij) {
+        I = ij[2 * (i - nb)];
+        J = ij[2 * (i - nb) + 1];
+    } else {
+        I = ij[2 * (i - nb - nij) + 1];
+        J = ij[2 * (i - nb - nij)];
+    }
+    rows[16 * i + ii + jj * 4] = (int32_t)(ii + 4 * I);
+    cols[16 * i + ii + jj * 4] = (int32_t)(jj + 4 * J);
+}
+
+// first k in [lo, hi) with key(k) >= v; keys non-decreasing, key(k) = a[2k + comp]
+__device__ __forceinline__ int64_t lower_bound2(const int32_t* __restrict__ a, int comp, int64_t lo, int64_t hi,
+                                                int v) {
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[2 * mid + comp] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void rowlen_kernel(int nb, int64_t nij, const int32_t* __restrict__ ij, const int32_t* __restrict__ ji,
+                              int64_t* __restrict__ len) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int64_t nu = lower_bound2(ij, 0, 0, nij, b + 1) - lower_bound2(ij, 0, 0, nij, b);
+    int64_t nl = lower_bound2(ji, 1, 0, nij, b + 1) - lower_bound2(ji, 1, 0, nij, b);
+    len[b] = nl + 1 + nu;
+}
+
+__global__ void rowfill_kernel(int nb, int64_t nij, const int32_t* __restrict__ ij, const int32_t* __restrict__ ji,
+                               const int64_t* __restrict__ off, int64_t total, int32_t* __restrict__ row_ptr,
+                               int32_t* __restrict__ cols, int32_t* __restrict__ slots) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int64_t o = off[b];
+    row_ptr[b] = (int32_t)o;
+    if (b == nb - 1) row_ptr[nb] = (int32_t)total;
+    int64_t l0 = lower_bound2(ji, 1, 0, nij, b), l1 = lower_bound2(ji, 1, 0, nij, b + 1);
+    for (int64_t k = l0; k < l1; ++k) {
+        int i = ji[2 * k];
+        int64_t u0 = lower_bound2(ij, 0, 0, nij, i), u1 = lower_bound2(ij, 0, 0, nij, i + 1);
+        int64_t ii = lower_bound2(ij, 1, u0, u1, b);
+        cols[o] = i;
+        slots[o] = (int32_t)(nb + nij + ii);
+        ++o;
+    }
+    cols[o] = b;
+    slots[o] = b;
+    ++o;
+    int64_t u0 = lower_bound2(ij, 0, 0, nij, b), u1 = lower_bound2(ij, 0, 0, nij, b + 1);
+    for (int64_t k = u0; k < u1; ++k) {
+        cols[o] = ij[2 * k + 1];
+        slots[o] = (int32_t)(nb + k);
+        ++o;
+    }
+}
+
+}  // namespace abd
+
+using namespace abd;
+
+extern "C" {
+
+int abd_cull_bodies(abd_stream_t stream, const AbdScene* sc, const float* lo, const float* hi, int32_t* ij,
+                    int32_t* ji, int64_t cap, int64_t* count) {
+    if (!sc || !lo || !hi || !count || cap < 0 || (cap > 0 && (!ij || !ji))) return ABD_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    const int nb = sc->n_bodies;
+    const int64_t npairs = (int64_t)nb * (nb - 1) / 2;
+    *count = 0;
+    if (npairs <= 0) return ABD_OK;
+    const int64_t nblk = ceil_div(npairs, (int64_t)kBlock * kItems);
+    if (nblk > INT32_MAX) return ABD_ERR_ARG;
+    int64_t *bcnt, *boff;
+    int rc = arena_get(kSlotScanA, sizeof(int64_t) * (nblk + 1), (void**)&bcnt);
+    if (rc) return rc;
+    rc = arena_get(kSlotScanB, sizeof(int64_t) * (nblk + 1), (void**)&boff);
+    if (rc) return rc;
+    int ph = prof_start(kProfBodyPairs, s, 24.0 * nb);
+    hipLaunchKernelGGL(body_pairs_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 0,
+                       sc->body_scene, lo, hi, bcnt, (const int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    if ((rc = scan_exclusive_i64(s, bcnt, boff, nblk, boff + nblk))) return rc;
+    int64_t n;
+    if ((rc = read_i64(s, boff + nblk, &n))) return rc;
+    *count = n;
+    if (n > cap) return ABD_ERR_OVERFLOW;
+    if (n == 0) return ABD_OK;
+    hipLaunchKernelGGL(body_pairs_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 0,
+                       sc->body_scene, lo, hi, bcnt, (const int64_t*)boff, ij, cap);
+    ABD_CHECK_LAUNCH();
+    // (j, i) order: same predicate, second enumeration
+    hipLaunchKernelGGL(body_pairs_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 1,
+                       sc->body_scene, lo, hi, bcnt, (const int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
+    ABD_CHECK_LAUNCH();
+    if ((rc = scan_exclusive_i64(s, bcnt, boff, nblk, boff + nblk))) return rc;
+    hipLaunchKernelGGL(body_pairs_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 1,
+                       sc->body_scene, lo, hi, bcnt, (const int64_t*)boff, ji, cap);
+    ABD_CHECK_LAUNCH();
+    ABD_CHECK(hipStreamSynchronize(s));
+    return ABD_OK;
+}
+
+int abd_cull_prims(abd_stream_t stream, const AbdScene* sc, int32_t mode, const int32_t* ij, int64_t nij,
+                   const float* lo1, const float* hi1, const float* lo2, const float* hi2, int32_t* out, int64_t cap,
+                   int64_t* count) {
+    if (!sc || !count || nij < 0 || cap < 0 || (nij > 0 && !ij) || !lo1 || !hi1 || !lo2 || !hi2 ||
+        (cap > 0 && !out))
+        return ABD_ERR_ARG;
+    if (mode != ABD_CULL_PT && mode != ABD_CULL_EE) return ABD_ERR_ARG;
+    *count = 0;
+    if (nij == 0) return ABD_OK;
+    hipStream_t s = to_stream(stream);
+    const int pt = mode == ABD_CULL_PT;
+    const int nb = sc->n_bodies;
+    PrimSets S;
+    S.off1 = pt ? sc->voff : sc->eoff;
+    S.off2 = pt ? sc->foff : sc->eoff;
+    S.lo1 = lo1;
+    S.hi1 = hi1;
+    S.lo2 = lo2;
+    S.hi2 = hi2;
+    const int64_t max1 = pt ? sc->max_verts : sc->max_edges;
+    const int64_t max2 = pt ? sc->max_tris : sc->max_edges;
+    const int64_t maxs = max1 > max2 ? max1 : max2;
+    if (maxs > 65535) return ABD_ERR_ARG;  // n1 * n2 tests per body pair must fit in 32 bits
+    const int64_t nunits = pt ? 2 * nij : nij;
+    int rc;
+    float *h1, *h2;
+    if ((rc = arena_get(kSlotHull1, sizeof(float) * 6 * (size_t)nb, (void**)&h1))) return rc;
+    if ((rc = arena_get(kSlotHull2, sizeof(float) * 6 * (size_t)nb, (void**)&h2))) return rc;
+    S.hlo1 = h1;
+    S.hhi1 = h1 + 3 * nb;
+    S.hlo2 = h2;
+    S.hhi2 = h2 + 3 * nb;
+    hipLaunchKernelGGL(hull_kernel, dim3(nb), dim3(kBlock), 0, s, S.off1, lo1, hi1, h1, h1 + 3 * nb);
+    ABD_CHECK_LAUNCH();
+    hipLaunchKernelGGL(hull_kernel, dim3(nb), dim3(kBlock), 0, s, S.off2, lo2, hi2, h2, h2 + 3 * nb);
+    ABD_CHECK_LAUNCH();
+    int32_t* flist;
+    int64_t *fcnt, *nchunk, *choff;
+    if ((rc = arena_get(kSlotFilter, sizeof(int32_t) * (size_t)(2 * nunits * maxs), (void**)&flist))) return rc;
+    if ((rc = arena_get(kSlotFilterCount, sizeof(int64_t) * (size_t)(2 * nunits), (void**)&fcnt))) return rc;
+    if ((rc = arena_get(kSlotScanA, sizeof(int64_t) * (size_t)(nunits + 1), (void**)&nchunk))) return rc;
+    if ((rc = arena_get(kSlotScanB, sizeof(int64_t) * (size_t)(nunits + 1), (void**)&choff))) return rc;
+    if (2 * nunits > INT32_MAX) return ABD_ERR_ARG;
+    int ph = prof_start(kProfFilter, s, 0.0);
+    hipLaunchKernelGGL(filter_kernel, dim3((unsigned)(2 * nunits)), dim3(kBlock), 0, s, pt, ij, S, maxs, flist, fcnt);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    hipLaunchKernelGGL(unit_chunks_kernel, dim3(grid_for(nunits)), dim3(kBlock), 0, s, nunits, fcnt, nchunk);
+    ABD_CHECK_LAUNCH();
+    if ((rc = scan_exclusive_i64(s, nchunk, choff, nunits, choff + nunits))) return rc;
+    int64_t T;
+    if ((rc = read_i64(s, choff + nunits, &T))) return rc;
+    if (T == 0) return ABD_OK;
+    int64_t *ccnt, *ccoff;
+    if ((rc = arena_get(kSlotChunkCount, sizeof(int64_t) * (size_t)(T + 1), (void**)&ccnt))) return rc;
+    if ((rc = arena_get(kSlotChunk, sizeof(int64_t) * (size_t)(T + 1), (void**)&ccoff))) return rc;
+    const unsigned grid = (unsigned)(T < 16384 ? T : 16384);
+    ph = prof_start(kProfPrimCount, s, 0.0);
+    hipLaunchKernelGGL(prim_pairs_kernel<false>, dim3(grid), dim3(kBlock), 0, s, pt, ij, S, maxs,
+                       (const int32_t*)flist, (const int64_t*)fcnt, (const int64_t*)choff, nunits,
+                       (const int64_t*)(choff + nunits), ccnt, (const int64_t*)nullptr, (int32_t*)nullptr,
+                       (int64_t)0);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    if ((rc = scan_exclusive_i64(s, ccnt, ccoff, T, ccoff + T))) return rc;
+    int64_t n;
+    if ((rc = read_i64(s, ccoff + T, &n))) return rc;
+    *count = n;
+    if (n > cap) return ABD_ERR_OVERFLOW;
+    if (n == 0) return ABD_OK;
+    ph = prof_start(kProfPrimFill, s, 20.0 * n);
+    hipLaunchKernelGGL(prim_pairs_kernel<true>, dim3(grid), dim3(kBlock), 0, s, pt, ij, S, maxs,
+                       (const int32_t*)flist, (const int64_t*)fcnt, (const int64_t*)choff, nunits,
+                       (const int64_t*)(choff + nunits), ccnt, (const int64_t*)ccoff, out, cap);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    ABD_CHECK(hipStreamSynchronize(s));
+    return ABD_OK;
+}
+
+int abd_cull_vg(abd_stream_t stream, const AbdScene* sc, const float* blo, const float* plo, const float* phi,
+                double dhat, double ground, int32_t* out, int64_t cap, int64_t* count) {
+    if (!sc || !blo || !plo || !phi || !count || cap < 0 || (cap > 0 && !out)) return ABD_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    *count = 0;
+    const int64_t nv = sc->n_verts;
+    if (nv == 0) return ABD_OK;
+    const int64_t nblk = ceil_div(nv, (int64_t)kBlock * kItems);
+    int64_t *bcnt, *boff;
+    int rc;
+    if ((rc = arena_get(kSlotScanA, sizeof(int64_t) * (nblk + 1), (void**)&bcnt))) return rc;
+    if ((rc = arena_get(kSlotScanB, sizeof(int64_t) * (nblk + 1), (void**)&boff))) return rc;
+    const double yth = 0.5 * dhat + ground;
+    const float fg = (float)ground;
+    int ph = prof_start(kProfVg, s, 28.0 * nv);
+    hipLaunchKernelGGL(vg_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nv, sc->vert_body, sc->voff, blo,
+                       plo, phi, yth, fg, bcnt, (const int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    if ((rc = scan_exclusive_i64(s, bcnt, boff, nblk, boff + nblk))) return rc;
+    int64_t n;
+    if ((rc = read_i64(s, boff + nblk, &n))) return rc;
+    *count = n;
+    if (n > cap) return ABD_ERR_OVERFLOW;
+    if (n == 0) return ABD_OK;
+    hipLaunchKernelGGL(vg_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nv, sc->vert_body, sc->voff, blo,
+                       plo, phi, yth, fg, bcnt, (const int64_t*)boff, out, cap);
+    ABD_CHECK_LAUNCH();
+    ABD_CHECK(hipStreamSynchronize(s));
+    return ABD_OK;
+}
+
+int abd_set_triplets(abd_stream_t stream, int32_t nb, const int32_t* ij, int64_t nij, int32_t* rows,
+                     int32_t* cols) {
+    if (nb < 0 || nij < 0 || (nij > 0 && !ij) || !rows || !cols) return ABD_ERR_ARG;
+    int64_t n = 16 * ((int64_t)nb + 2 * nij);
+    if (n == 0) return ABD_OK;
+    hipLaunchKernelGGL(triplets_kernel, dim3(grid_for(n, kBlock, INT32_MAX)), dim3(kBlock), 0, to_stream(stream), nb,
+                       nij, ij, rows, cols);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_bsr_structure(abd_stream_t stream, int32_t nb, const int32_t* ij, const int32_t* ji, int64_t nij,
+                      int32_t* row_ptr, int32_t* cols, int32_t* slots) {
+    if (nb <= 0 || nij < 0 || (nij > 0 && (!ij || !ji)) || !row_ptr || !cols || !slots) return ABD_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    int64_t *len, *off;
+    int rc;
+    if ((rc = arena_get(kSlotScanA, sizeof(int64_t) * (nb + 1), (void**)&len))) return rc;
+    if ((rc = arena_get(kSlotScanB, sizeof(int64_t) * (nb + 1), (void**)&off))) return rc;
+    hipLaunchKernelGGL(rowlen_kernel, dim3(grid_for(nb)), dim3(kBlock), 0, s, nb, nij, ij, ji, len);
+    ABD_CHECK_LAUNCH();
+    if ((rc = scan_exclusive_i64(s, len, off, nb, off + nb))) return rc;
+    hipLaunchKernelGGL(rowfill_kernel, dim3(grid_for(nb)), dim3(kBlock), 0, s, nb, nij, ij, ji, (const int64_t*)off,
+                       (int64_t)nb + 2 * nij, row_ptr, cols, slots);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,269 @@
+// abd_geom.hip — per-vertex / per-primitive / per-body geometry: vertex
+// projection, state updates and the FP32 bounding boxes of culling2.py.
+// All of it is streaming, HBM-bound work: one thread per element, coalesced
+// along the flattened arrays.
+#include <hip/hip_runtime.h>
+
+#include "abd_block.h"
+#include "abd_device.h"
+#include "abd_internal.h"
+#include "abd_profile.h"
+
+namespace abd {
+
+// abd.py:305-324: x = A^T x0 + p, i.e. x_k = ((p_k + x0_0 A[0][k]) + x0_1 A[1][k]) + x0_2 A[2][k]
+__global__ __launch_bounds__(kBlock) void project_kernel(const int32_t* __restrict__ vert_body,
+                                                          const double* __restrict__ x0,
+                                                          const double* __restrict__ A,
+                                                          const double* __restrict__ p, double* __restrict__ x,
+                                                          int64_t nv) {
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kBlock) {
+        int b = vert_body[v];
+        const double* Ab = A + 9 * (int64_t)b;
+        double r0 = x0[3 * v], r1 = x0[3 * v + 1], r2 = x0[3 * v + 2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double r = p[3 * (int64_t)b + k] + r0 * Ab[k];
+            r = r + r1 * Ab[3 + k];
+            r = r + r2 * Ab[6 + k];
+            x[3 * v + k] = r;
+        }
+    }
+}
+
+// abd.py:337-353
+__global__ void update_q_kernel(int nb, const double* __restrict__ A, const double* __restrict__ p,
+                                const double* __restrict__ dq, double alpha, double* A_out, double* p_out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 12 * nb) return;
+    int b = i / 12, r = i % 12;
+    if (r < 3) {
+        p_out[3 * b + r] = p[3 * b + r] - dq[3 * (4 * b) + r] * alpha;
+    } else {
+        int row = (r - 3) / 3, k = (r - 3) % 3;
+        A_out[9 * b + 3 * row + k] = A[9 * b + 3 * row + k] - dq[3 * (4 * b + 1 + row) + k] * alpha;
+    }
+}
+
+// abd.py:355-362
+__global__ void update_q0qdot_kernel(int nb, double dt, const double* __restrict__ A, const double* __restrict__ p,
+                                     double* A0, double* p0, double* Adot, double* pdot) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 12 * nb) return;
+    int b = i / 12, r = i % 12;
+    if (r < 3) {
+        int k = 3 * b + r;
+        pdot[k] = (p[k] - p0[k]) / dt;
+        p0[k] = p[k];
+    } else {
+        int k = 9 * b + (r - 3);
+        Adot[k] = (A[k] - A0[k]) / dt;
+        A0[k] = A[k];
+    }
+}
+
+// culling2.py:72-79 bvh_points: cast, then FP32 +- f32(dil)
+__global__ __launch_bounds__(kBlock) void point_boxes_kernel(const double* __restrict__ x, int64_t n, float dil,
+                                                              float* __restrict__ lo, float* __restrict__ hi) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < 3 * n; i += (int64_t)gridDim.x * kBlock) {
+        float c = (float)x[i];
+        lo[i] = c - dil;
+        hi[i] = c + dil;
+    }
+}
+// culling2.py:33-69 bvh_triangles / bvh_edges: FP64 min/max +- dil, then cast.
+template <int NV>
+__global__ __launch_bounds__(kBlock) void prim_boxes_kernel(const int32_t* __restrict__ prim_body,
+                                                             const int32_t* __restrict__ voff,
+                                                             const int32_t* __restrict__ topo,
+                                                             const double* __restrict__ x, int64_t n, double dil,
+                                                             float* __restrict__ lo, float* __restrict__ hi) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        int vb = voff[prim_body[i]];
+        int64_t vi[NV];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) vi[j] = vb + topo[NV * i + j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double mn = x[3 * vi[0] + k], mx = mn;
+#pragma unroll
+            for (int j = 1; j < NV; ++j) {
+                double c = x[3 * vi[j] + k];
+                mn = c < mn ? c : mn;
+                mx = c > mx ? c : mx;
+            }
+            lo[3 * i + k] = (float)(mn - dil);
+            hi[3 * i + k] = (float)(mx + dil);
+        }
+    }
+}
+// culling2.py:273-280 bvh_point_trajectory
+__global__ __launch_bounds__(kBlock) void point_traj_kernel(const double* __restrict__ x, int64_t n,
+                                                             float* __restrict__ lo, float* __restrict__ hi) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < 3 * n; i += (int64_t)gridDim.x * kBlock) {
+        float c = (float)x[i];
+        hi[i] = fmaxf(hi[i], c);
+        lo[i] = fminf(lo[i], c);
+    }
+}
+// culling2.py:230-270 bvh_triangle_trajectory / bvh_edge_trajectory
+template <int NV>
+__global__ __launch_bounds__(kBlock) void prim_traj_kernel(const int32_t* __restrict__ prim_body,
+                                                            const int32_t* __restrict__ voff,
+                                                            const int32_t* __restrict__ topo,
+                                                            const double* __restrict__ x, int64_t n,
+                                                            float* __restrict__ lo, float* __restrict__ hi) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        int vb = voff[prim_body[i]];
+        int64_t vi[NV];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) vi[j] = vb + topo[NV * i + j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double mn = x[3 * vi[0] + k], mx = mn;
+#pragma unroll
+            for (int j = 1; j < NV; ++j) {
+                double c = x[3 * vi[j] + k];
+                mn = c < mn ? c : mn;
+                mx = c > mx ? c : mx;
+            }
+            hi[3 * i + k] = fmaxf(hi[3 * i + k], (float)mx);
+            lo[3 * i + k] = fminf(lo[3 * i + k], (float)mn);
+        }
+    }
+}
+
+// culling2.py:171-210: one workgroup per body reduces its vertices in FP64.
+// traj = 0: lo/hi = f32(min/max -+ dil); traj = 1: lo/hi = min/max(lo/hi, f32(min/max)).
+__global__ __launch_bounds__(kBlock) void body_boxes_kernel(const int32_t* __restrict__ voff,
+                                                             const double* __restrict__ x, double dil, int traj,
+                                                             float* __restrict__ lo, float* __restrict__ hi) {
+    __shared__ double sh[kBlock];
+    const int b = blockIdx.x;
+    const int v0 = voff[b], v1 = voff[b + 1];
+    for (int k = 0; k < 3; ++k) {
+        double mn = 1.0 / 0.0, mx = -1.0 / 0.0;
+        for (int v = v0 + threadIdx.x; v < v1; v += kBlock) {
+            double c = x[3 * (int64_t)v + k];
+            mn = c < mn ? c : mn;
+            mx = c > mx ? c : mx;
+        }
+        double gmn = block_min(mn, sh);
+        double gmx = -block_min(-mx, sh);
+        if (threadIdx.x == 0) {
+            if (traj) {
+                hi[3 * b + k] = fmaxf(hi[3 * b + k], (float)gmx);
+                lo[3 * b + k] = fminf(lo[3 * b + k], (float)gmn);
+            } else {
+                lo[3 * b + k] = (float)(gmn - dil);
+                hi[3 * b + k] = (float)(gmx + dil);
+            }
+        }
+    }
+}
+
+}  // namespace abd
+
+using namespace abd;
+
+extern "C" {
+
+int abd_project_vertices(abd_stream_t stream, const AbdScene* sc, const double* A, const double* p, double* x) {
+    if (!sc || !A || !p || !x) return ABD_ERR_ARG;
+    if (sc->n_verts == 0) return ABD_OK;
+    int ph = prof_start(kProfProject, to_stream(stream), 52.0 * sc->n_verts + 96.0 * sc->n_bodies);
+    hipLaunchKernelGGL(project_kernel, dim3(grid_for(sc->n_verts)), dim3(kBlock), 0, to_stream(stream),
+                       sc->vert_body, sc->x0, A, p, x, (int64_t)sc->n_verts);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, to_stream(stream));
+    return ABD_OK;
+}
+
+int abd_update_q(abd_stream_t stream, int32_t nb, const double* A, const double* p, const double* dq, double alpha,
+                 double* A_out, double* p_out) {
+    if (nb < 0 || !A || !p || !dq || !A_out || !p_out) return ABD_ERR_ARG;
+    if (nb == 0) return ABD_OK;
+    hipLaunchKernelGGL(update_q_kernel, dim3(grid_for(12 * (int64_t)nb)), dim3(kBlock), 0, to_stream(stream), nb, A,
+                       p, dq, alpha, A_out, p_out);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_update_q0qdot(abd_stream_t stream, int32_t nb, double dt, const double* A, const double* p, double* A0,
+                      double* p0, double* Adot, double* pdot) {
+    if (nb < 0 || !A || !p || !A0 || !p0 || !Adot || !pdot) return ABD_ERR_ARG;
+    if (nb == 0) return ABD_OK;
+    hipLaunchKernelGGL(update_q0qdot_kernel, dim3(grid_for(12 * (int64_t)nb)), dim3(kBlock), 0, to_stream(stream),
+                       nb, dt, A, p, A0, p0, Adot, pdot);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_prim_boxes(abd_stream_t stream, const AbdScene* sc, int32_t kind, const double* x, double dil, float* lo,
+                   float* hi) {
+    if (!sc || !x || !lo || !hi) return ABD_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    const double bytes = kind == ABD_KIND_POINT ? 48.0 * sc->n_verts
+                         : kind == ABD_KIND_EDGE ? (12.0 + 48 + 24) * sc->n_edges
+                                                 : (16.0 + 72 + 24) * sc->n_tris;
+    const int64_t nprim = kind == ABD_KIND_POINT ? sc->n_verts : kind == ABD_KIND_EDGE ? sc->n_edges : sc->n_tris;
+    if (kind < ABD_KIND_POINT || kind > ABD_KIND_TRI) return ABD_ERR_ARG;
+    if (nprim == 0) return ABD_OK;
+    int ph = prof_start(kProfBoxes, s, bytes);
+    if (kind == ABD_KIND_POINT) {
+        hipLaunchKernelGGL(point_boxes_kernel, dim3(grid_for(3 * (int64_t)sc->n_verts)), dim3(kBlock), 0, s, x,
+                           (int64_t)sc->n_verts, (float)dil, lo, hi);
+    } else if (kind == ABD_KIND_EDGE) {
+        hipLaunchKernelGGL(prim_boxes_kernel<2>, dim3(grid_for(sc->n_edges)), dim3(kBlock), 0, s, sc->edge_body,
+                           sc->voff, sc->edges, x, (int64_t)sc->n_edges, dil, lo, hi);
+    } else {
+        hipLaunchKernelGGL(prim_boxes_kernel<3>, dim3(grid_for(sc->n_tris)), dim3(kBlock), 0, s, sc->tri_body,
+                           sc->voff, sc->tris, x, (int64_t)sc->n_tris, dil, lo, hi);
+    }
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    return ABD_OK;
+}
+
+int abd_prim_boxes_traj(abd_stream_t stream, const AbdScene* sc, int32_t kind, const double* x_end, float* lo,
+                        float* hi) {
+    if (!sc || !x_end || !lo || !hi) return ABD_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    if (kind == ABD_KIND_POINT) {
+        if (!sc->n_verts) return ABD_OK;
+        hipLaunchKernelGGL(point_traj_kernel, dim3(grid_for(3 * (int64_t)sc->n_verts)), dim3(kBlock), 0, s, x_end,
+                           (int64_t)sc->n_verts, lo, hi);
+    } else if (kind == ABD_KIND_EDGE) {
+        if (!sc->n_edges) return ABD_OK;
+        hipLaunchKernelGGL(prim_traj_kernel<2>, dim3(grid_for(sc->n_edges)), dim3(kBlock), 0, s, sc->edge_body,
+                           sc->voff, sc->edges, x_end, (int64_t)sc->n_edges, lo, hi);
+    } else if (kind == ABD_KIND_TRI) {
+        if (!sc->n_tris) return ABD_OK;
+        hipLaunchKernelGGL(prim_traj_kernel<3>, dim3(grid_for(sc->n_tris)), dim3(kBlock), 0, s, sc->tri_body,
+                           sc->voff, sc->tris, x_end, (int64_t)sc->n_tris, lo, hi);
+    } else {
+        return ABD_ERR_ARG;
+    }
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_body_boxes(abd_stream_t stream, const AbdScene* sc, const double* x, double dil, float* lo, float* hi) {
+    if (!sc || !x || !lo || !hi) return ABD_ERR_ARG;
+    if (!sc->n_bodies) return ABD_OK;
+    hipLaunchKernelGGL(body_boxes_kernel, dim3(sc->n_bodies), dim3(kBlock), 0, to_stream(stream), sc->voff, x, dil, 0,
+                       lo, hi);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_body_boxes_traj(abd_stream_t stream, const AbdScene* sc, const double* x_end, float* lo, float* hi) {
+    if (!sc || !x_end || !lo || !hi) return ABD_ERR_ARG;
+    if (!sc->n_bodies) return ABD_OK;
+    hipLaunchKernelGGL(body_boxes_kernel, dim3(sc->n_bodies), dim3(kBlock), 0, to_stream(stream), sc->voff, x_end,
+                       0.0, 1, lo, hi);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// abd_profile.hip — see abd_profile.h.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "abd_internal.h"
+#include "abd_profile.h"
+
+namespace abd {
+
+namespace {
+struct Rec {
+    int kernel;
+    hipEvent_t a, b;
+    double bytes;
+    const int32_t* scale;
+};
+std::mutex g_mu;
+bool g_on = false;
+std::vector<Rec> g_recs;
+double g_ms[kProfCount] = {};
+double g_bytes[kProfCount] = {};
+int64_t g_n[kProfCount] = {};
+
+int resolve_locked() {
+    for (Rec& r : g_recs) {
+        ABD_CHECK(hipEventSynchronize(r.b));
+        float ms = 0.f;
+        ABD_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+        double mult = 1.0;
+        if (r.scale) {
+            int32_t v = 0;
+            ABD_CHECK(hipMemcpy(&v, r.scale, sizeof(v), hipMemcpyDeviceToHost));
+            mult = v;
+        }
+        g_ms[r.kernel] += ms;
+        g_bytes[r.kernel] += r.bytes * mult;
+        g_n[r.kernel] += 1;
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    g_recs.clear();
+    return ABD_OK;
+}
+}  // namespace
+
+bool prof_enabled() { return g_on; }
+
+int prof_start(int kernel, hipStream_t s, double bytes) {
+    if (!g_on) return -1;
+    std::lock_guard<std::mutex> lock(g_mu);
+    Rec r{kernel, nullptr, nullptr, bytes, nullptr};
+    if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
+    if (hipEventRecord(r.a, s) != hipSuccess) return -1;
+    g_recs.push_back(r);
+    return (int)g_recs.size() - 1;
+}
+
+void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale) {
+    if (handle < 0) return;
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (handle >= (int)g_recs.size()) return;
+    g_recs[handle].scale = dev_scale;
+    hipEventRecord(g_recs[handle].b, s);
+}
+
+}  // namespace abd
+
+using namespace abd;
+
+extern "C" {
+
+int abd_profile_begin(void) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    int rc = resolve_locked();
+    for (int k = 0; k < kProfCount; ++k) {
+        g_ms[k] = 0.0;
+        g_bytes[k] = 0.0;
+        g_n[k] = 0;
+    }
+    g_on = true;
+    return rc;
+}
+
+int abd_profile_end(void) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_on = false;
+    return resolve_locked();
+}
+
+int abd_profile_query(int32_t kernel, double* ms, int64_t* launches, double* bytes) {
+    if (kernel < 0 || kernel >= kProfCount) return ABD_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_mu);
+    int rc = resolve_locked();
+    if (rc) return rc;
+    if (ms) *ms = g_ms[kernel];
+    if (launches) *launches = g_n[kernel];
+    if (bytes) *bytes = g_bytes[kernel];
+    return ABD_OK;
+}
+
+}  // extern "C"

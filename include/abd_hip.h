@@ -1,0 +1,237 @@
+/*
+ * abd_hip.h — C ABI of libabd_hip.so, the MI355X (gfx950) implementation of
+ * the abd-warp per-Newton-iteration contact path.
+ *
+ * Conventions
+ *  - Every pointer argument is a DEVICE pointer (contiguous arrays in the
+ *    layouts below) unless its comment says "host".
+ *  - Every call is stream-ordered on `stream` (a hipStream_t; NULL = the
+ *    default stream).  Calls that must return a size to the host (candidate
+ *    counts, active counts) synchronize `stream` before returning — as the
+ *    reference's `compress()` does (simulator/fenwick.py:289).
+ *  - Return value: ABD_OK (0); ABD_ERR_OVERFLOW (1) when an output list is too
+ *    small (the required count is written to *count and nothing is written past
+ *    `cap`); negative on error (ABD_ERR_* below, or -hipError_t).
+ *  - Scratch memory (partial sums, compaction counters, filtered primitive
+ *    lists) comes from a per-device arena that the library grows on demand and
+ *    keeps until abd_release().  Calls on one device must not overlap in time
+ *    (they share the arena); the library is otherwise stateless.
+ *  - No C++ exception crosses this boundary.
+ *
+ * Data layouts (shared with the Python mirror and the CPU oracle)
+ *  - vertices: double [V][3]; per body b the range voff[b]..voff[b+1].
+ *  - triangles int32 [F][3], edges int32 [E][2], in BODY-LOCAL vertex ids
+ *    (the reference's WarpMesh.triangles / .edges, affine_body.py:85-95).
+ *  - A: double [Nb][3][3], row r = a_{r+1}, x = A^T x0 + p (abd.py:313-316);
+ *    p: double [Nb][3]; gradient / dq: double [4 Nb][3] (dp, da1, da2, da3).
+ *  - boxes: float [n][3] lower and upper.
+ *  - candidate tuples: int32 [n][5] = (I, J, ii, prim_I, prim_J) with body-local
+ *    primitive ids and ii the index of the body pair in the ij list; VG tuples
+ *    int32 [n][2] = (body, local vertex) (the reference's vec5i / vec2i).
+ *  - body pairs ij: int32 [Nij][2], i < j, sorted lexicographically; ji: the
+ *    same pairs sorted by (j, i).
+ *  - Hessian slots: double [(Nb + 2 Nij) * 16][3][3]; slot b < Nb is body b's
+ *    diagonal 12x12 block, slot Nb + ii the (ij[ii][0], ij[ii][1]) block, slot
+ *    Nb + Nij + ii its transpose; 3x3 sub-block (ii, jj) of a slot at
+ *    16*slot + ii + 4*jj (ipcsk/put.py:26-49, orthogonal_energy.py:138-165).
+ */
+#ifndef ABD_HIP_H
+#define ABD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* abd_stream_t; /* hipStream_t */
+
+#define ABD_OK 0
+#define ABD_ERR_OVERFLOW 1
+#define ABD_ERR_ARG (-1000)      /* bad argument (null pointer, negative size, ...) */
+#define ABD_ERR_UNSORTED (-1001) /* candidate list not grouped by body pair / body */
+#define ABD_ERR_NODEVICE (-1002) /* no HIP device */
+
+#define ABD_KIND_POINT 0
+#define ABD_KIND_EDGE 1
+#define ABD_KIND_TRI 2
+#define ABD_CULL_EE 0
+#define ABD_CULL_PT 1
+#define ABD_SOLVER_BICGSTAB 0
+#define ABD_SOLVER_CG_JACOBI 1
+
+/* Runtime parameters (the reference's module constants, const_params.py:24-39). */
+/* psd: 0 = the reference's contact Hessian (PT eigenvalues of d2(l^2)/dc2 clamped, EE
+ * not, then b' H + b'' g g^T: indefinite since b' < 0); 1 = the whole barrier Hessian
+ * projected in contact space (eigenvalues b' L_k and b' L_4 + b'' |q_4|^2 clamped at 0),
+ * which keeps the Newton matrix positive semi-definite.  No reference counterpart. */
+typedef struct AbdParams {
+    double dt, dhat, d2hat, kappa, stiffness, gravity, ground, mass, I0, c1, alpha_min, tol;
+    double solver_tol;
+    int32_t max_iter, solver_maxiter, pt, ee, vg, solver, psd;
+} AbdParams;
+
+/* Flattened scene (replaces the per-body WarpMesh structs, affine_body.py:85-95).
+ * All pointers device; body_scene / mass / I0 may be NULL. */
+typedef struct AbdScene {
+    int32_t n_bodies, n_verts, n_tris, n_edges;
+    int32_t max_verts, max_tris, max_edges; /* per-body maxima */
+    const int32_t* voff;                     /* [Nb+1] */
+    const int32_t* foff;                     /* [Nb+1] */
+    const int32_t* eoff;                     /* [Nb+1] */
+    const int32_t* vert_body;                /* [V] */
+    const int32_t* tri_body;                 /* [F] */
+    const int32_t* edge_body;                /* [E] */
+    const int32_t* tris;                     /* [F][3] local */
+    const int32_t* edges;                    /* [E][2] local */
+    const double* x0;                        /* [V][3] rest shape */
+    const int32_t* body_scene;               /* [Nb] scene id; bodies of different scenes never pair */
+    const double* mass;                      /* [Nb] (NULL: params.mass) */
+    const double* I0;                        /* [Nb] (NULL: params.I0) */
+} AbdScene;
+
+/* ---- library ---- */
+int abd_version(void);
+const char* abd_error_string(int code);
+/* Frees the scratch arena of the current device. */
+int abd_release(void);
+
+/* ---- state updates ---- */
+/* abd.py:305-324 update_mesh_vertex: x = A^T x0 + p for every vertex. */
+int abd_project_vertices(abd_stream_t stream, const AbdScene* scene, const double* A, const double* p, double* x);
+/* abd.py:337-353 _update_q / _update_qk: (A_out, p_out) = (A, p) - alpha * dq (in place allowed). */
+int abd_update_q(abd_stream_t stream, int32_t nb, const double* A, const double* p, const double* dq, double alpha,
+                 double* A_out, double* p_out);
+/* abd.py:355-362 _update_q0qdot: qdot = (q - q0) / dt; q0 = q. */
+int abd_update_q0qdot(abd_stream_t stream, int32_t nb, double dt, const double* A, const double* p, double* A0,
+                      double* p0, double* Adot, double* pdot);
+/* abd.py:143-144 dot (warp.utils.array_inner): *out (device) = a . b, fixed-order reduction. */
+int abd_dot(abd_stream_t stream, const double* a, const double* b, int64_t n, double* out);
+
+/* ---- bounding boxes (culling2.py) ---- */
+/* culling2.py:33-79 bvh_triangles / bvh_edges / bvh_points: FP32 boxes of every primitive
+ * of `kind` (ABD_KIND_*) dilated by `dilation` (points: cast then +-f32(dil); edges and
+ * triangles: FP64 min/max +- dil, then cast). */
+int abd_prim_boxes(abd_stream_t stream, const AbdScene* scene, int32_t kind, const double* x, double dilation,
+                   float* lo, float* hi);
+/* culling2.py:230-280 bvh_*_trajectory: lo/hi <- union(lo/hi, f32 box of x_end, undilated). */
+int abd_prim_boxes_traj(abd_stream_t stream, const AbdScene* scene, int32_t kind, const double* x_end, float* lo,
+                        float* hi);
+/* culling2.py:171-186 update_body_bvh: per-body FP64 min/max of x +- dilation, cast to f32. */
+int abd_body_boxes(abd_stream_t stream, const AbdScene* scene, const double* x, double dilation, float* lo, float* hi);
+/* culling2.py:188-210 body_bvh_to_traj: lo/hi <- union(lo/hi, per-body box of x_end, undilated). */
+int abd_body_boxes_traj(abd_stream_t stream, const AbdScene* scene, const double* x_end, float* lo, float* hi);
+
+/* ---- culling ---- */
+/* culling2.py:214-227 intersection_bodies + abd.py:163-168 ij_list: every body pair
+ * (i < j, same scene) whose boxes overlap (closed intervals).  Writes ij sorted by (i, j)
+ * and ji (the same pairs sorted by (j, i)); *count (host) = number of pairs.
+ * Synchronizes.  ABD_ERR_OVERFLOW when *count > cap. */
+int abd_cull_bodies(abd_stream_t stream, const AbdScene* scene, const float* lo, const float* hi, int32_t* ij,
+                    int32_t* ji, int64_t cap, int64_t* count);
+/* culling.py:32-52 cull_ + culling.py:86-101 intersection_prims (the BVH semantics the
+ * live stub culling.py:8-30 stands in for): for each body pair ii,
+ *   ABD_CULL_PT: (I, J, ii, p, t) for every point p of I (set 1 = point boxes) whose box
+ *                overlaps triangle t of J (set 2 = triangle boxes), both orders
+ *                (I, J) = (ij0, ij1) then (ij1, ij0);
+ *   ABD_CULL_EE: (ij0, ij1, ii, a, b) for edge a of ij0 overlapping edge b of ij1
+ *                (set 1 = set 2 = edge boxes).
+ * Output is sorted by (ii, order, prim_I, prim_J).  *count (host).  Synchronizes.
+ * ABD_ERR_OVERFLOW when *count > cap. */
+int abd_cull_prims(abd_stream_t stream, const AbdScene* scene, int32_t mode, const int32_t* ij, int64_t nij,
+                   const float* lo1, const float* hi1, const float* lo2, const float* hi2, int32_t* out, int64_t cap,
+                   int64_t* count);
+/* culling.py:54-76 cull_vg + intersection_ground: (b, v) for bodies with f64(blo.y) <
+ * ground + dhat/2 and vertices whose point box overlaps [(-1e3,-1e3,-1e3),(1e3,f32(ground),1e3)].
+ * Sorted by (b, v).  *count (host).  Synchronizes. */
+int abd_cull_vg(abd_stream_t stream, const AbdScene* scene, const float* blo, const float* plo, const float* phi,
+                double dhat, double ground, int32_t* out, int64_t cap, int64_t* count);
+
+/* ---- sparse structure ---- */
+/* orthogonal_energy.py:138-165 _set_triplets (over all Nb + 2 Nij slots): block row/col of
+ * every 3x3 sub-block, rows[16 s + ii + 4 jj] = ii + 4 I, cols[...] = jj + 4 J. */
+int abd_set_triplets(abd_stream_t stream, int32_t nb, const int32_t* ij, int64_t nij, int32_t* rows, int32_t* cols);
+/* abd.py:240-243 bsr_set_from_triplets, as a CSR over bodies with 12x12 blocks:
+ * row b holds (col, slot) for its lower pairs (from ji), its diagonal, its upper pairs
+ * (from ij), columns ascending.  row_ptr [Nb+1], cols/slots [Nb + 2 Nij]. */
+int abd_bsr_structure(abd_stream_t stream, int32_t nb, const int32_t* ij, const int32_t* ji, int64_t nij,
+                      int32_t* row_ptr, int32_t* cols, int32_t* slots);
+
+/* ---- energies, gradient, Hessian ---- */
+/* ipc.py:19-46 IPCContactEnergy.energy at xk: *E (device) = sum of barrier energies of
+ * valid PT (|e2p|^2), valid EE (l^2) and all VG candidates (ipcsk/energy.py). */
+int abd_contact_energy(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* xk,
+                       const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
+                       double* E);
+/* ipc.py:54-62 IPCContactEnergy.gh at x (ipcsk/pt_ipc.py:24-190, ipcsk/ee_ipc.py:50-143,
+ * ipcsk/vg_ipc.py:5-34, ipcsk/put.py): ADDS the barrier gradient into g [4Nb][3] and the
+ * PSD-projected Hessian into the slots `blocks`.  pt/ee must be grouped by ii ascending
+ * and vg by body ascending (as abd_cull_* emit them; else ABD_ERR_UNSORTED).  row_ptr/
+ * slots: the body CSR of abd_bsr_structure for the nij body pairs the tuples index.
+ * highlight (nullable) [Nb] gets bit 0 for bodies J of active PT contacts and bit 1
+ * for EE.  *n_active (host, nullable) = active PT + EE contacts.  Synchronizes. */
+int abd_contact_gh(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* x,
+                   const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
+                   int64_t nij, const int32_t* row_ptr, const int32_t* slots, double* g, double* blocks,
+                   uint8_t* highlight, int64_t* n_active);
+/* Per-contact form of the gh stage for n independent contacts (kind 0 PT, 1 EE): x4/r4
+ * [n][4][3] current / rest positions; writes active [n] (uint8), g24 [n][24] and the 24x24
+ * body-pair Hessian H24 [n][24][24] (rows/cols 0-11 body I, 12-23 body J). */
+int abd_pair_gh(abd_stream_t stream, int32_t kind, const AbdParams* params, const double* x4, const double* r4,
+                int64_t n, uint8_t* active, double* g24, double* H24);
+/* orthogonal_energy.py:25-32 energy_inertia at (Ak, pk): *E (device). */
+int abd_inertia_energy(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* Ak,
+                       const double* pk, const double* A0, const double* p0, const double* Adot, const double* pdot,
+                       double* E);
+/* orthogonal_energy.py:17-19,99-112: g <- inertia gradient at (A, p) (overwrites g). */
+int abd_inertia_gradient(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* A,
+                         const double* p, const double* A0, const double* p0, const double* Adot, const double* pdot,
+                         double* g);
+/* orthogonal_energy.py:21-22,84-96: diagonal slots += inertia Hessian at A. */
+int abd_inertia_hessian(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* A,
+                        double* blocks);
+
+/* ---- linear solve ---- */
+/* abd.py:244 bicgstab(hess, g, dq, tol) (method ABD_SOLVER_BICGSTAB, warp.optim.linear) or
+ * bsr_utils.py:126-235 bsr_cg with the diagonal preconditioner (ABD_SOLVER_CG_JACOBI):
+ * solves H x = b with x as the warm start, H the BSR (row_ptr/cols/slots over 12x12
+ * slots).  tol: BiCGSTAB relative tol on |r| (|r| <= tol |b|); CG absolute tol^2 n on r.z.
+ * maxiter 0 -> 12 Nb.  iters/resid (device, nullable) receive the iteration count and
+ * the final |r|^2 (BiCGSTAB) or r.z (CG).  One workgroup per system. */
+int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t* row_ptr, const int32_t* cols,
+                  const int32_t* slots, const double* blocks, const double* b, double* x, double tol, int32_t maxiter,
+                  int32_t check_every, int32_t* iters, double* resid);
+
+/* ---- CCD ---- */
+/* ccd.py:9-40 toi_vg / toi_pt / toi_ee + abd.py:284-299 (in-repo cubic TOI; the reference
+ * calls the absent abdtk instead): *toi (device) = min(1, TOIs < 1 of every candidate),
+ * trajectories x_t0 -> x_t1. */
+int abd_toi(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* x_t0,
+            const double* x_t1, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg,
+            int64_t nvg, double* toi);
+/* cubic_roots.py:95-193 for n polynomials coef [n][4] (constant first) on [x0, x1]:
+ * nroots [n], roots [n][3]. */
+int abd_cubic_roots(abd_stream_t stream, const double* coef, int64_t n, double x0, double x1, int32_t* nroots,
+                    double* roots);
+/* ccd.py:133-186 pt_collision_time (kind 0) / ee_collision_time (kind 1) for n vertex
+ * quadruples p_t0, p_t1 [n][4][3]: t [n]. */
+int abd_collision_time(abd_stream_t stream, int32_t kind, const double* p_t0, const double* p_t1, int64_t n,
+                       double* t);
+
+/* ---- measurement ---- */
+/* Kernel timing with HIP events recorded on each launch's stream, and the
+ * algorithmic bytes each launch moves (no reference counterpart: it exists so
+ * the benchmark can report achieved bandwidth per kernel).  Kernel ids:
+ * 0 gh classify, 1 gh contact terms, 2 gh pair reduce, 3 gh body finalize,
+ * 4 energy, 5 CCD, 6 cull filter, 7 cull count, 8 cull fill, 9 body pairs,
+ * 10 vertex-ground cull, 11 Krylov solve, 12 vertex projection, 13 boxes. */
+int abd_profile_begin(void);
+int abd_profile_end(void);
+/* Synchronizes outstanding events; totals since abd_profile_begin(). */
+int abd_profile_query(int32_t kernel, double* total_ms, int64_t* launches, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ABD_HIP_H */

@@ -1,0 +1,111 @@
+"""Shared test utilities: seeded contact geometries and oracle/GPU glue."""
+from __future__ import annotations
+
+import numpy as np
+
+from abd_hip.const_params import ContactParams
+from oracle import oracle as O
+
+
+def rel_err(a, b) -> float:
+    """max |a - b| / max(max |b|, tiny): the norm-wise relative error used for
+    the 1e-9 Hessian / gradient / energy bound."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = max(float(np.max(np.abs(b))) if b.size else 0.0, 1e-300)
+    return float(np.max(np.abs(a - b))) / scale if a.size else 0.0
+
+
+def near_contact_pt(rng, dist):
+    """A point at signed distance `dist` above a random triangle, projecting inside it."""
+    while True:
+        t = rng.uniform(-1, 1, size=(3, 3))
+        e0, e1 = t[1] - t[0], t[2] - t[0]
+        n = np.cross(e0, e1)
+        if np.linalg.norm(n) < 0.3:
+            continue
+        n /= np.linalg.norm(n)
+        b = rng.uniform(0.05, 0.9)
+        g = rng.uniform(0.05, 0.95 - b)
+        p = t[0] + b * e0 + g * e1 + dist * n
+        return np.stack([p, t[0], t[1], t[2]])
+
+
+def near_contact_ee(rng, dist):
+    """Two skew edges whose closest points are interior, at distance `dist`."""
+    while True:
+        a0 = rng.uniform(-1, 1, 3)
+        d0 = rng.uniform(-1, 1, 3)
+        d1 = rng.uniform(-1, 1, 3)
+        n = np.cross(d0, d1)
+        if np.linalg.norm(n) < 0.3:
+            continue
+        n /= np.linalg.norm(n)
+        s, t = rng.uniform(0.15, 0.85, 2)
+        c0 = a0 + s * d0
+        c1 = c0 + dist * n
+        b0 = c1 - t * d1
+        return np.stack([a0, a0 + d0, b0, b0 + d1])
+
+
+def rest_of(x4, rng, scale=0.3):
+    """Rest positions: a random affine pre-image-like perturbation of x4."""
+    return x4 + scale * rng.uniform(-1, 1, size=x4.shape)
+
+
+def params(**kw) -> ContactParams:
+    return ContactParams(**kw)
+
+
+def orc_params(cp: ContactParams):
+    return O.params(cp)
+
+
+def rot_y(a):
+    c, s = np.cos(a), np.sin(a)
+    return np.array([[c, 0, -s], [0, 1, 0], [s, 0, c]])
+
+
+def rot_axis(axis, a):
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * K @ K
+
+
+def contact_scene(gap=4e-4, dhat=1e-3, seed=0):
+    """Four bodies with active PT, EE and VG contacts at distance ~gap < dhat:
+    cube 0 on the ground, cube 1 (rotated about y) on cube 0, an icosphere-2 on the
+    ground, a tilted icosphere-2 against cube 0's +x face.  A holds rotations as
+    rows, so vertices map to x = A^T x0 + p."""
+    from abd_hip import scenes
+    rng = np.random.default_rng(seed)
+    cube = scenes.cube_mesh()
+    ico = scenes.icosphere_mesh(2, 0.5)
+    ground = -0.5 - gap
+    R1 = rot_y(0.3 + 0.01 * rng.uniform())
+    R3 = rot_axis([0.3, 1.0, 0.2], 0.7)
+    A = np.stack([np.eye(3), R1.T, np.eye(3), R3.T])  # x = A^T x0 + p = R x0 + p
+    ico3 = ico[0] @ R3.T
+    p = np.array([
+        [0.0, 0.0, 0.0],
+        [0.05, 1.0 + gap, -0.03],
+        [2.0, ground + gap - ico[0][:, 1].min(), 0.0],
+        [0.5 + gap - ico3[:, 0].min(), 0.0, 0.02],
+    ])
+    z = np.zeros((4, 3))
+    return scenes.build_scene([cube, cube, ico, ico], A, p, np.zeros((4, 3, 3)), z, ground=ground, name="contact")
+
+
+def oracle_lists(sc, x, dhat, ground, traj_from=None):
+    """The oracle's collision_set (abd.py:187-207) at positions x: boxes, ij, ee, pt, vg."""
+    dh = 0.5 * dhat
+    blo, bhi = O.body_boxes(sc, x, dh)
+    tlo, thi = O.prim_boxes(sc, O.KIND_TRI, x, 0.0)
+    elo, ehi = O.prim_boxes(sc, O.KIND_EDGE, x, dh)
+    plo, phi = O.prim_boxes(sc, O.KIND_POINT, x, dhat)
+    ij = O.cull_bodies(sc, blo, bhi)
+    ee = O.cull_prims(False, ij, sc.eoff, elo, ehi, sc.eoff, elo, ehi)
+    pt = O.cull_prims(True, ij, sc.voff, plo, phi, sc.foff, tlo, thi)
+    vg = O.cull_vg(sc, blo, plo, phi, dhat, ground)
+    return dict(blo=blo, bhi=bhi, tlo=tlo, thi=thi, elo=elo, ehi=ehi, plo=plo, phi=phi, ij=ij, ee=ee, pt=pt, vg=vg)

@@ -1,0 +1,409 @@
+"""GPU parity of every stage of the contact path against the CPU oracle, through
+libabd_hip.so.  Tolerances: candidate lists, boxes, validity and active flags
+bit-exact; energies, gradients and Hessians 1e-9 relative (norm-wise: max |diff|
+<= 1e-9 max |ref|); TOI 1e-12 absolute (same arithmetic, same root solver)."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import contact_scene, near_contact_ee, near_contact_pt, oracle_lists, params, rel_err, rest_of
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def T(a, dtype=None):
+    return torch.as_tensor(np.ascontiguousarray(a), device=DEV, dtype=dtype)
+
+
+def N(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import abd_hip
+    return abd_hip.load()
+
+
+def _gpu_scene(spec):
+    from abd_hip.affine_body import Meshes
+    return Meshes(spec, DEV)
+
+
+# ------------------------------------------------------------------ per contact
+@pytest.mark.parametrize("kind", [0, 1])
+def test_pair_gh_matches_oracle(lib, kind):
+    from abd_hip.ipc import pair_gh
+    rng = np.random.default_rng(10 + kind)
+    cp = params(dhat=1e-3)
+    xs, rs = [], []
+    for i in range(256):
+        d = rng.uniform(1e-5, 9.9e-4) * (1 if i % 2 else -1)
+        x4 = near_contact_pt(rng, d) if kind == 0 else near_contact_ee(rng, d)
+        xs.append(x4)
+        rs.append(rest_of(x4, rng))
+    # inactive (far) and degenerate (parallel edges / collinear) cases too
+    for i in range(16):
+        x4 = near_contact_pt(rng, 0.5) if kind == 0 else near_contact_ee(rng, 0.5)
+        xs.append(x4)
+        rs.append(rest_of(x4, rng))
+    par = np.array([[0, 0, 0], [1, 0, 0], [0, 1e-4, 0], [1, 1e-4, 0]], np.float64)
+    xs.append(par)
+    rs.append(par)
+    X, R = np.array(xs), np.array(rs)
+    act, g24, H24 = pair_gh(kind, cp, T(X), T(R))
+    act, g24, H24 = N(act), N(g24), N(H24)
+    prm = O.params(cp)
+    n_act = 0
+    for i in range(len(X)):
+        a_ref, s, g_ref, Hi, Hj, Hij = O.pair_gh(kind, prm, X[i], R[i])
+        assert bool(act[i]) == a_ref, i
+        if not a_ref:
+            assert np.all(H24[i] == 0) and np.all(g24[i] == 0)
+            continue
+        n_act += 1
+        Href = np.block([[Hi, Hij], [Hij.T, Hj]])
+        assert rel_err(g24[i], g_ref) <= 1e-9, (i, rel_err(g24[i], g_ref))
+        assert rel_err(H24[i], Href) <= 1e-9, (i, rel_err(H24[i], Href))
+    assert n_act >= 200
+
+
+# ------------------------------------------------------------- boxes and culling
+def _stage_lists(m, cp):
+    from abd_hip.culling import BvhBuilder, cull, cull_vg, ij_list
+    bb = BvhBuilder(m)
+    dh = 0.5 * cp.dhat
+    bodies = bb.build_body_bvh(m, dh, field="xk")
+    tri = bb.bulid_triangle_bvh(m.xk, m.triangles, 0.0)
+    edg = bb.build_edge_bvh(m.xk, m.edges, dh)
+    pts = bb.build_point_bvh(m.xk, cp.dhat)
+    pairs = ij_list(bodies, m)
+    ee = cull(pairs, edg)
+    pt = cull(pairs, pts, tri)
+    vg = cull_vg(bodies.lowers, pts, m, cp.dhat, cp.ground)
+    return dict(bodies=bodies, tri=tri, edg=edg, pts=pts, pairs=pairs, ee=ee, pt=pt, vg=vg)
+
+
+def _scenes():
+    from abd_hip import scenes
+    med = scenes.config_medium(seed=3, n_side=2)  # 8 icosphere-3 bodies
+    # squeeze the grid so neighbours' boxes overlap
+    med.p[:] = med.p * 0.93
+    return [contact_scene(), med, scenes.config_small()]
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_boxes_and_culling_bit_exact(lib, which):
+    spec = _scenes()[which]
+    cp = params(ground=spec.ground)
+    m = _gpu_scene(spec)
+    st = m.states()
+    from abd_hip import _lib
+    _lib.call("abd_project_vertices", _lib.stream_handle(), m.c, _lib.ptr(st.A), _lib.ptr(st.p), _lib.ptr(m.xk))
+    sc = O.OracleScene(spec)
+    x_ref = O.project(sc, spec.A, spec.p)
+    np.testing.assert_array_equal(N(m.xk), x_ref)
+    G = _stage_lists(m, cp)
+    R = oracle_lists(sc, x_ref, cp.dhat, cp.ground)
+    np.testing.assert_array_equal(N(G["bodies"].lowers), R["blo"])
+    np.testing.assert_array_equal(N(G["bodies"].uppers), R["bhi"])
+    for k, rk in (("tri", "t"), ("edg", "e"), ("pts", "p")):
+        np.testing.assert_array_equal(N(G[k].lowers), R[rk + "lo"])
+        np.testing.assert_array_equal(N(G[k].uppers), R[rk + "hi"])
+    np.testing.assert_array_equal(N(G["pairs"].ij), R["ij"])
+    ji_ref = R["ij"][np.lexsort((R["ij"][:, 0], R["ij"][:, 1]))] if len(R["ij"]) else R["ij"]
+    np.testing.assert_array_equal(N(G["pairs"].ji), ji_ref)
+    for k in ("ee", "pt", "vg"):
+        np.testing.assert_array_equal(N(G[k]), R[k])
+    assert len(R["ij"]) > 0
+
+
+def test_trajectory_boxes_bit_exact(lib):
+    from abd_hip import _lib
+    from abd_hip.culling import BvhBuilder
+    spec = contact_scene()
+    m = _gpu_scene(spec)
+    sc = O.OracleScene(spec)
+    rng = np.random.default_rng(5)
+    x0 = O.project(sc, spec.A, spec.p)
+    x1 = x0 + 0.01 * rng.normal(size=x0.shape)
+    m.x.copy_(T(x0))
+    m.xk.copy_(T(x1))
+    bb = BvhBuilder(m)
+    for kind, fn, dil, off in ((O.KIND_POINT, "build_point_bvh", 1e-3, None), (O.KIND_EDGE, "build_edge_bvh", 5e-4, m.edges),
+                               (O.KIND_TRI, "bulid_triangle_bvh", 0.0, m.triangles)):
+        args = (m.x, dil) if off is None else (m.x, off, dil)
+        b0 = getattr(bb, fn)(*args)
+        b1 = getattr(bb, fn)(*args)
+        if kind == O.KIND_POINT:
+            bb.point_bvh_to_traj(m.xk, b0, b1)
+        elif kind == O.KIND_EDGE:
+            bb.edge_bvh_to_traj(m.xk, m.edges, b0, b1)
+        else:
+            bb.triangle_bvh_to_traj(m.xk, m.triangles, b0, b1)
+        lo, hi = O.prim_boxes(sc, kind, x0, dil)
+        lo, hi = O.prim_boxes_traj(sc, kind, x1, lo, hi)
+        np.testing.assert_array_equal(N(b1.lowers), lo)
+        np.testing.assert_array_equal(N(b1.uppers), hi)
+    bod = bb.build_body_bvh(m, 5e-4)
+    trj = bb.build_body_bvh(m, 5e-4)
+    bb.body_bvh_to_traj(m, bod, trj)
+    lo, hi = O.body_boxes(sc, x0, 5e-4)
+    lo, hi = O.body_boxes_traj(sc, x1, lo, hi)
+    np.testing.assert_array_equal(N(trj.lowers), lo)
+    np.testing.assert_array_equal(N(trj.uppers), hi)
+
+
+# --------------------------------------------------------------- energy and gh
+def test_energy_and_gh_match_oracle(lib):
+    from abd_hip.ipc import IPCContactEnergy
+    spec = contact_scene()
+    cp = params(ground=spec.ground)
+    m = _gpu_scene(spec)
+    sc = O.OracleScene(spec)
+    x = O.project(sc, spec.A, spec.p)
+    m.x.copy_(T(x))
+    m.xk.copy_(T(x))
+    G = _stage_lists(m, cp)
+    R = oracle_lists(sc, x, cp.dhat, cp.ground)
+    prm = O.params(cp)
+    ipc = IPCContactEnergy(cp)
+    E = ipc.energy([G["ee"], G["pt"], G["vg"], m])
+    E_ref = O.contact_energy(sc, prm, x, R["pt"], R["ee"], R["vg"])
+    assert E_ref > 0
+    assert abs(E - E_ref) <= 1e-9 * abs(E_ref)
+    nb, nij = spec.n_bodies, len(R["ij"])
+    rng = np.random.default_rng(1)
+    g0 = rng.normal(size=(4 * nb, 3))
+    b0 = rng.normal(size=((nb + 2 * nij) * 16, 3, 3))
+    g, blocks = T(g0), T(b0)
+    hl, nact = ipc.gh([G["pt"], G["ee"], G["vg"], nij, m, g, blocks], return_active=True)
+    g_ref, b_ref, hl_ref = O.contact_gh(sc, prm, x, R["pt"], R["ee"], R["vg"], nij, g0, b0)
+    assert nact > 0
+    np.testing.assert_array_equal(N(ipc.highlight_bits), hl_ref)
+    np.testing.assert_array_equal(N(hl), (hl_ref & 2) != 0)
+    assert rel_err(N(g) - g0, g_ref - g0) <= 1e-9
+    assert rel_err(N(blocks) - b0, b_ref - b0) <= 1e-9
+
+
+def test_gh_rejects_unsorted_lists(lib):
+    from abd_hip._lib import AbdError
+    from abd_hip.ipc import IPCContactEnergy
+    spec = contact_scene()
+    cp = params(ground=spec.ground)
+    m = _gpu_scene(spec)
+    st = m.states()
+    from abd_hip import _lib
+    _lib.call("abd_project_vertices", _lib.stream_handle(), m.c, _lib.ptr(st.A), _lib.ptr(st.p), _lib.ptr(m.xk))
+    m.x.copy_(m.xk)
+    G = _stage_lists(m, cp)
+    ee = G["ee"]
+    assert G["pairs"].nij >= 2 and ee.shape[0] >= 2
+    bad = torch.flip(ee, dims=[0]).contiguous()
+    if bool((bad[1:, 2] >= bad[:-1, 2]).all()):
+        pytest.skip("all EE candidates belong to one body pair")
+    nb, nij = spec.n_bodies, G["pairs"].nij
+    g = torch.zeros((4 * nb, 3), dtype=torch.float64, device=DEV)
+    blocks = torch.zeros(((nb + 2 * nij) * 16, 3, 3), dtype=torch.float64, device=DEV)
+    with pytest.raises(AbdError) as e:
+        IPCContactEnergy(cp).gh([G["pt"], bad, G["vg"], nij, m, g, blocks])
+    assert e.value.code == _lib.ABD_ERR_UNSORTED
+
+
+def test_inertia_matches_oracle(lib):
+    from abd_hip.orthogonal_energy import InertialEnergy
+    from abd_hip import scenes
+    spec = scenes.config_medium(seed=1, n_side=2)
+    cp = params(ground=spec.ground)
+    m = _gpu_scene(spec)
+    st = m.states()
+    rng = np.random.default_rng(2)
+    A = spec.A + 0.01 * rng.normal(size=spec.A.shape)
+    p = spec.p + 0.01 * rng.normal(size=spec.p.shape)
+    Adot = 0.1 * rng.normal(size=spec.A.shape)
+    for name, v in (("A", A), ("Ak", A), ("p", p), ("pk", p), ("Adot", Adot)):
+        getattr(st, name).copy_(T(v))
+    prm = O.params(cp)
+    ie = InertialEnergy(cp, m)
+    E = ie.energy(st)
+    E_ref = O.inertia_energy(prm, A, p, spec.A, spec.p, Adot, spec.pdot, spec.mass, spec.I0)
+    assert abs(E - E_ref) <= 1e-9 * abs(E_ref)
+    g = torch.full((4 * spec.n_bodies, 3), 7.0, dtype=torch.float64, device=DEV)
+    ie.gradient(g, st)
+    g_ref = O.inertia_gradient(prm, A, p, spec.A, spec.p, Adot, spec.pdot, spec.mass, spec.I0)
+    assert rel_err(N(g), g_ref) <= 1e-12
+    b0 = rng.normal(size=(16 * spec.n_bodies, 3, 3))
+    blocks = T(b0)
+    ie.hessian(blocks, st)
+    b_ref = O.inertia_hessian(prm, A, b0, spec.mass, spec.I0)
+    assert rel_err(N(blocks), b_ref) <= 1e-12
+
+
+# ---------------------------------------------------------------------- solver
+def _random_system(nb, ij, rng, spd=True):
+    nij = len(ij)
+    nslot = nb + 2 * nij
+    blocks = np.zeros((nslot * 16, 3, 3))
+    dense = np.zeros((12 * nb, 12 * nb))
+    def put(slot, M):
+        for ii in range(4):
+            for jj in range(4):
+                blocks[16 * slot + ii + 4 * jj] = M[3 * ii:3 * ii + 3, 3 * jj:3 * jj + 3]
+    offs = []
+    for k, (i, j) in enumerate(ij):
+        C = 0.3 * rng.normal(size=(12, 12))
+        put(nb + k, C)
+        put(nb + nij + k, C.T)
+        dense[12 * i:12 * i + 12, 12 * j:12 * j + 12] += C
+        dense[12 * j:12 * j + 12, 12 * i:12 * i + 12] += C.T
+        offs.append(C)
+    for b in range(nb):
+        Q = rng.normal(size=(12, 12))
+        D = Q @ Q.T + 12.0 * np.eye(12) * (1 + len(ij))
+        if not spd:
+            D = D + 0.5 * rng.normal(size=(12, 12))
+        put(b, D)
+        dense[12 * b:12 * b + 12, 12 * b:12 * b + 12] += D
+    return blocks, dense
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_bsr_solve_matches_oracle(lib, method):
+    from abd_hip import _lib
+    from abd_hip.sparse import BsrMatrix, SolveInfo, bicgstab, bsr_cg
+    rng = np.random.default_rng(3 + method)
+    nb = 40
+    pairs = sorted({tuple(sorted(rng.choice(nb, 2, replace=False))) for _ in range(70)})
+    ij = np.array(pairs, np.int32)
+    blocks, dense = _random_system(nb, ij, rng, spd=True)
+    b = rng.normal(size=(4 * nb, 3))
+    x0 = 0.01 * rng.normal(size=(4 * nb, 3))
+    ji = ij[np.lexsort((ij[:, 0], ij[:, 1]))]
+    nij = len(ij)
+    row_ptr = torch.empty(nb + 1, dtype=torch.int32, device=DEV)
+    cols = torch.empty(nb + 2 * nij, dtype=torch.int32, device=DEV)
+    slots = torch.empty(nb + 2 * nij, dtype=torch.int32, device=DEV)
+    _lib.call("abd_bsr_structure", _lib.stream_handle(), nb, _lib.ptr(T(ij)), _lib.ptr(T(ji)), nij,
+              _lib.ptr(row_ptr), _lib.ptr(cols), _lib.ptr(slots))
+    # the CSR: columns ascending, every slot once
+    rp, cl, sl = N(row_ptr), N(cols), N(slots)
+    assert rp[-1] == nb + 2 * nij and sorted(sl.tolist()) == list(range(nb + 2 * nij))
+    for r in range(nb):
+        assert np.all(np.diff(cl[rp[r]:rp[r + 1]]) > 0)
+    A = BsrMatrix(nb, row_ptr, cols, slots, T(blocks))
+    x = T(x0)
+    info = SolveInfo(DEV)
+    tol = 1e-10
+    if method == 0:
+        bicgstab(A, T(b), x, tol, 0, 10, info)
+        x_ref, it_ref, _ = O.bicgstab(nb, ij, blocks, b, x0, tol)
+    else:
+        bsr_cg(A, x, T(b), 0, tol, 10, info)
+        x_ref, it_ref, _ = O.bsr_cg(nb, ij, blocks, b, x0, tol)
+    xs = np.linalg.solve(dense, b.reshape(-1)).reshape(-1, 3)
+    assert rel_err(N(x), xs) <= 1e-7
+    assert rel_err(N(x), x_ref) <= 1e-7
+    assert abs(int(N(info.iters)[0]) - it_ref) <= 10
+
+
+# -------------------------------------------------------------------------- CCD
+def test_cubic_roots_match_oracle(lib):
+    from abd_hip.ccd import cubic_roots
+    rng = np.random.default_rng(7)
+    coefs = []
+    for _ in range(400):
+        r = np.sort(rng.uniform(-0.2, 1.2, 3))
+        a0 = rng.uniform(0.1, 5.0) * (1 if rng.uniform() < 0.5 else -1)
+        coefs.append(np.array([-r[0] * r[1] * r[2], r[0] * r[1] + r[0] * r[2] + r[1] * r[2], -r.sum(), 1.0]) * a0)
+    coefs += [np.array([0.3, -1.0, 0.0, 0.0]), np.array([0.1, 0.0, -1.0, 0.0]), np.zeros(4),
+              np.array([1.0, 2.0, 3.0, 4.0])]
+    C = np.array(coefs)
+    nr, roots = cubic_roots(T(C), 0.0, 1.0)
+    nr, roots = N(nr), N(roots)
+    for i, c in enumerate(C):
+        n_ref, r_ref = O.cubic_roots(c, 0.0, 1.0)
+        assert nr[i] == n_ref, i
+        np.testing.assert_allclose(roots[i][:n_ref], r_ref[:n_ref], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_collision_time_matches_oracle(lib, kind):
+    from abd_hip.ccd import collision_time
+    rng = np.random.default_rng(8 + kind)
+    P0, P1 = [], []
+    for i in range(300):
+        x4 = near_contact_pt(rng, 0.05) if kind == 0 else near_contact_ee(rng, 0.05)
+        v = 0.1 * rng.normal(size=(4, 3))
+        if i % 3 == 0:
+            v[0] -= 0.08 * np.cross(x4[2] - x4[1], x4[3] - x4[1])  # drive through
+        P0.append(x4)
+        P1.append(x4 + v)
+    P0, P1 = np.array(P0), np.array(P1)
+    t = N(collision_time(kind, T(P0), T(P1)))
+    hits = 0
+    for i in range(len(P0)):
+        t_ref = O.collision_time(kind, P0[i], P1[i])
+        assert abs(t[i] - t_ref) <= 1e-12, (i, t[i], t_ref)
+        hits += t_ref < 1.0
+    assert hits > 10
+
+
+def test_scene_toi_matches_oracle(lib):
+    from abd_hip.ccd import toi
+    from abd_hip.culling import BvhBuilder, cull, cull_vg, ij_list
+    spec = contact_scene()
+    cp = params(ground=spec.ground)
+    m = _gpu_scene(spec)
+    sc = O.OracleScene(spec)
+    x0 = O.project(sc, spec.A, spec.p)
+    rng = np.random.default_rng(11)
+    # move the bodies rigidly toward each other / the ground
+    p1 = spec.p + np.array([[0, -0.002, 0], [0, -0.004, 0], [0, -0.003, 0], [-0.003, -0.001, 0]])
+    x1 = O.project(sc, spec.A, p1)
+    m.x.copy_(T(x0))
+    m.xk.copy_(T(x1))
+    bb = BvhBuilder(m)
+    dh = 0.5 * cp.dhat
+    b = bb.build_body_bvh(m, dh)
+    bt = bb.build_body_bvh(m, dh)
+    bb.body_bvh_to_traj(m, b, bt)
+    tri, edg, pts = (bb.bulid_triangle_bvh(m.x, m.triangles, 0.0), bb.build_edge_bvh(m.x, m.edges, dh),
+                     bb.build_point_bvh(m.x, cp.dhat))
+    trit, edgt, ptst = (bb.bulid_triangle_bvh(m.x, m.triangles, 0.0), bb.build_edge_bvh(m.x, m.edges, dh),
+                        bb.build_point_bvh(m.x, cp.dhat))
+    bb.triangle_bvh_to_traj(m.xk, m.triangles, tri, trit)
+    bb.edge_bvh_to_traj(m.xk, m.edges, edg, edgt)
+    bb.point_bvh_to_traj(m.xk, pts, ptst)
+    pairs = ij_list(bt, m)
+    ee = cull(pairs, edgt)
+    pt = cull(pairs, ptst, trit)
+    vg = cull_vg(bt.lowers, ptst, m, cp.dhat, cp.ground)
+    t = toi(m, cp, pt, ee, vg)
+    t_ref = O.toi(sc, O.params(cp), x0, x1, N(pt), N(ee), N(vg))
+    assert t_ref < 1.0
+    assert abs(t - t_ref) <= 1e-12
+
+
+def test_dot_and_updates(lib):
+    from abd_hip import _lib
+    rng = np.random.default_rng(12)
+    a, b = rng.normal(size=10001), rng.normal(size=10001)
+    out = torch.zeros(1, dtype=torch.float64, device=DEV)
+    _lib.call("abd_dot", _lib.stream_handle(), _lib.ptr(T(a)), _lib.ptr(T(b)), a.size, _lib.ptr(out))
+    assert abs(N(out)[0] - np.dot(a, b)) <= 1e-12 * np.abs(a * b).sum()
+    nb = 5
+    A, p, dq = rng.normal(size=(nb, 3, 3)), rng.normal(size=(nb, 3)), rng.normal(size=(4 * nb, 3))
+    Ao, po = torch.zeros((nb, 3, 3), dtype=torch.float64, device=DEV), torch.zeros((nb, 3), dtype=torch.float64,
+                                                                                  device=DEV)
+    _lib.call("abd_update_q", _lib.stream_handle(), nb, _lib.ptr(T(A)), _lib.ptr(T(p)), _lib.ptr(T(dq)), 0.37,
+              _lib.ptr(Ao), _lib.ptr(po))
+    A_ref, p_ref = np.zeros_like(A), np.zeros_like(p)
+    O.lib().orc_update_q(nb, A.ctypes.data, p.ctypes.data, dq.ctypes.data, 0.37, A_ref.ctypes.data, p_ref.ctypes.data)
+    np.testing.assert_array_equal(N(Ao), A_ref)
+    np.testing.assert_array_equal(N(po), p_ref)
