@@ -35,9 +35,10 @@ def toi(meshes, params: ContactParams, pt_list, ee_list, vg_list) -> float:
 def cubic_roots(coef: torch.Tensor, x0: float, x1: float):
     """coef [n,4] constant-first -> (n_roots [n] int32, roots [n,3])."""
     n = int(coef.shape[0])
+    coef = coef.contiguous()  # held for the duration of the launch
     nr = torch.zeros(n, dtype=torch.int32, device=coef.device)
     roots = torch.zeros((n, 3), dtype=torch.float64, device=coef.device)
-    _lib.call("abd_cubic_roots", _lib.stream_handle(), _lib.ptr(coef.contiguous()), n, float(x0), float(x1),
+    _lib.call("abd_cubic_roots", _lib.stream_handle(), _lib.ptr(coef), n, float(x0), float(x1),
               _lib.ptr(nr), _lib.ptr(roots))
     return nr, roots
 
@@ -45,7 +46,8 @@ def cubic_roots(coef: torch.Tensor, x0: float, x1: float):
 def collision_time(kind: int, p_t0: torch.Tensor, p_t1: torch.Tensor) -> torch.Tensor:
     """kind 0 pt_collision_time, 1 ee_collision_time; p_t0/p_t1 [n,4,3] -> t [n]."""
     n = int(p_t0.shape[0])
+    p_t0, p_t1 = p_t0.contiguous(), p_t1.contiguous()  # held for the duration of the launch
     t = torch.zeros(n, dtype=torch.float64, device=p_t0.device)
-    _lib.call("abd_collision_time", _lib.stream_handle(), int(kind), _lib.ptr(p_t0.contiguous()),
-              _lib.ptr(p_t1.contiguous()), n, _lib.ptr(t))
+    _lib.call("abd_collision_time", _lib.stream_handle(), int(kind), _lib.ptr(p_t0), _lib.ptr(p_t1), n,
+              _lib.ptr(t))
     return t

@@ -67,10 +67,11 @@ def pair_gh(kind: int, params: ContactParams, x4: torch.Tensor, r4: torch.Tensor
     x4/r4 [n,4,3] current/rest positions -> (active [n] bool, g24 [n,24], H24 [n,24,24])."""
     n = int(x4.shape[0])
     dev = x4.device
+    x4, r4 = x4.contiguous(), r4.contiguous()  # held for the duration of the launch
     act = torch.zeros(n, dtype=torch.uint8, device=dev)
     g24 = torch.zeros((n, 24), dtype=torch.float64, device=dev)
     H24 = torch.zeros((n, 24, 24), dtype=torch.float64, device=dev)
     pc = params.to_c()
-    _lib.call("abd_pair_gh", _lib.stream_handle(), int(kind), ctypes.byref(pc), _lib.ptr(x4.contiguous()),
-              _lib.ptr(r4.contiguous()), n, _lib.ptr(act), _lib.ptr(g24), _lib.ptr(H24))
+    _lib.call("abd_pair_gh", _lib.stream_handle(), int(kind), ctypes.byref(pc), _lib.ptr(x4), _lib.ptr(r4), n,
+              _lib.ptr(act), _lib.ptr(g24), _lib.ptr(H24))
     return act.bool(), g24, H24

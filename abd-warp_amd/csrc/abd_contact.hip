@@ -143,7 +143,9 @@ __device__ bool contact_terms(int kind, const d3 v[4], const d3 r[4], double d2h
         if (k < 4) {
             double om = lam[k] / (lam[k] - lt);
             L = lam[k] * 2.0 * l;
-            if (kind == 0) L = L > 0.0 ? L : 0.0;  // pt_ipc.py:266-269 (EE unclamped, ee_ipc.py:133-136)
+            // pt_ipc.py:266-269 (EE unclamped, ee_ipc.py:133-136); the psd mode projects the
+            // whole contact-space Hessian below and starts from the unclamped eigenvalue
+            if (kind == 0 && !psd) L = L > 0.0 ? L : 0.0;
             if (k < 2) {
                 qb[0] = mk(0.0, 0.0, 0.0);
                 qb[1] = c.e2p;

@@ -607,7 +607,9 @@ int orc_pair_gh(int kind, const orc_params* P, const v3 v[4], const v3 r[4], dou
     }
     if (s_out) *s_out = s;
     if (!(s < P->d2hat && valid)) return 0;
-    Q_lambda(e0p, e1p, e2p, kind == 0, Q, lam);
+    /* psd mode projects the whole contact-space Hessian, so it starts from the
+     * unclamped eigenvalues */
+    Q_lambda(e0p, e1p, e2p, kind == 0 && !P->psd, Q, lam);
     if (kind == 0) dcdx_pt(v[0], v[1], v[2], v[3], D); else dcdx_ee(v[0], v[1], v[2], v[3], D);
     double B1 = orc_barrier_d1(s, P->d2hat, P->kappa);
     double B2 = orc_barrier_d2(s, P->d2hat, P->kappa);

@@ -1,0 +1,69 @@
+"""The C ABI (CPU-only checks, no compute calls): libabd_hip.so loads, exports
+every entry point include/abd_hip.h declares, and the ctypes mirrors of AbdParams
+/ AbdScene match the C layout compiled from the header."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "abd_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(abd_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from abd_hip import _lib
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 28
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # every binding the Python mirror sets up exists in the header too
+    L = _lib.load()
+    assert L.abd_version() > 0
+    assert _lib.error_string(_lib.ABD_ERR_UNSORTED).startswith("candidate list")
+
+
+def test_struct_layouts_match_header(tmp_path):
+    from abd_hip import _lib
+    from abd_hip.const_params import AbdParamsC
+    src = tmp_path / "layout.c"
+    fields_p = [f for f, _ in AbdParamsC._fields_]
+    fields_s = [f for f, _ in _lib.AbdScene._fields_]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {',
+             'printf("%zu %zu\\n", sizeof(AbdParams), sizeof(AbdScene));']
+    lines += [f'printf("%zu\\n", offsetof(AbdParams, {f}));' for f in fields_p]
+    lines += [f'printf("%zu\\n", offsetof(AbdScene, {f}));' for f in fields_s]
+    lines += ['return 0; }']
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    vals = [int(v) for v in out]
+    assert vals[0] == ctypes.sizeof(AbdParamsC)
+    assert vals[1] == ctypes.sizeof(_lib.AbdScene)
+    for k, f in enumerate(fields_p):
+        assert vals[2 + k] == getattr(AbdParamsC, f).offset, f
+    for k, f in enumerate(fields_s):
+        assert vals[2 + len(fields_p) + k] == getattr(_lib.AbdScene, f).offset, f
+
+
+def test_oracle_params_mirror_matches_header():
+    from oracle import oracle as O
+    from abd_hip.const_params import AbdParamsC
+    assert [f for f, _ in O.OrcParams._fields_] == [f for f, _ in AbdParamsC._fields_]
+    assert ctypes.sizeof(O.OrcParams) == ctypes.sizeof(AbdParamsC)
+
+
+def test_product_does_not_import_the_oracle():
+    pkg = os.path.join(ROOT, "abd-warp_amd", "abd_hip")
+    for fn in os.listdir(pkg):
+        if fn.endswith(".py"):
+            text = open(os.path.join(pkg, fn)).read()
+            assert not re.search(r"^\s*(from|import)\s+oracle", text, re.M), fn

@@ -46,3 +46,30 @@ def test_medium_subset_frames():
     cp = ContactParams(ground=spec.ground, solver_tol=1e-12)
     sim, ref, worst = _run(spec, cp, 8)
     assert sim.stats["gh_pt"] + sim.stats["gh_ee"] > 0
+
+
+def test_small_frames_match_golden_fixture():
+    import os
+    from abd_hip import scenes
+    from abd_hip.abd import AffineBodySimulator
+    from abd_hip.const_params import ContactParams
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "small_frames.npz"))
+    spec = scenes.config_small()
+    sim = AffineBodySimulator(spec, ContactParams(ground=spec.ground, solver_tol=1e-12), "cuda")
+    for k in range(f["A"].shape[0]):
+        sim.step(k)
+        torch.cuda.synchronize()
+        assert np.abs(sim.states.A.cpu().numpy() - f["A"][k]).max() <= 1e-7
+        assert np.abs(sim.states.p.cpu().numpy() - f["p"][k]).max() <= 1e-7
+
+
+def test_medium_subset_psd_cg_frames():
+    """The stable benchmark mode (contact-space PSD projection + Jacobi CG) against
+    the oracle in the same mode."""
+    from abd_hip import scenes
+    from abd_hip.const_params import ContactParams
+    spec = scenes.config_medium(seed=0, n_side=2, subdiv=2)
+    spec.p[:] = spec.p * 0.93
+    cp = ContactParams(ground=spec.ground, psd=1, solver=1, solver_tol=1e-10)
+    sim, ref, worst = _run(spec, cp, 10)
+    assert sim.stats["gh_active"] > 0

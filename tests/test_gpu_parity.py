@@ -93,9 +93,11 @@ def _stage_lists(m, cp):
 def _scenes():
     from abd_hip import scenes
     med = scenes.config_medium(seed=3, n_side=2)  # 8 icosphere-3 bodies
-    # squeeze the grid so neighbours' boxes overlap
-    med.p[:] = med.p * 0.93
-    return [contact_scene(), med, scenes.config_small()]
+    # squeeze the grid so that neighbours overlap
+    med.p[:] = med.p * 0.9
+    small = scenes.config_small()
+    small.p[1, 1] = 1.0 + 4e-4  # A resting on B
+    return [contact_scene(), med, small]
 
 
 @pytest.mark.parametrize("which", [0, 1, 2])
@@ -289,7 +291,8 @@ def test_bsr_solve_matches_oracle(lib, method):
     row_ptr = torch.empty(nb + 1, dtype=torch.int32, device=DEV)
     cols = torch.empty(nb + 2 * nij, dtype=torch.int32, device=DEV)
     slots = torch.empty(nb + 2 * nij, dtype=torch.int32, device=DEV)
-    _lib.call("abd_bsr_structure", _lib.stream_handle(), nb, _lib.ptr(T(ij)), _lib.ptr(T(ji)), nij,
+    tij, tji = T(ij), T(ji)
+    _lib.call("abd_bsr_structure", _lib.stream_handle(), nb, _lib.ptr(tij), _lib.ptr(tji), nij,
               _lib.ptr(row_ptr), _lib.ptr(cols), _lib.ptr(slots))
     # the CSR: columns ascending, every slot once
     rp, cl, sl = N(row_ptr), N(cols), N(slots)
@@ -395,15 +398,58 @@ def test_dot_and_updates(lib):
     rng = np.random.default_rng(12)
     a, b = rng.normal(size=10001), rng.normal(size=10001)
     out = torch.zeros(1, dtype=torch.float64, device=DEV)
-    _lib.call("abd_dot", _lib.stream_handle(), _lib.ptr(T(a)), _lib.ptr(T(b)), a.size, _lib.ptr(out))
+    ta, tb = T(a), T(b)  # keep both alive: a freed temporary's block is handed to the next one
+    _lib.call("abd_dot", _lib.stream_handle(), _lib.ptr(ta), _lib.ptr(tb), a.size, _lib.ptr(out))
     assert abs(N(out)[0] - np.dot(a, b)) <= 1e-12 * np.abs(a * b).sum()
     nb = 5
     A, p, dq = rng.normal(size=(nb, 3, 3)), rng.normal(size=(nb, 3)), rng.normal(size=(4 * nb, 3))
     Ao, po = torch.zeros((nb, 3, 3), dtype=torch.float64, device=DEV), torch.zeros((nb, 3), dtype=torch.float64,
                                                                                   device=DEV)
-    _lib.call("abd_update_q", _lib.stream_handle(), nb, _lib.ptr(T(A)), _lib.ptr(T(p)), _lib.ptr(T(dq)), 0.37,
+    tA, tp, tdq = T(A), T(p), T(dq)
+    _lib.call("abd_update_q", _lib.stream_handle(), nb, _lib.ptr(tA), _lib.ptr(tp), _lib.ptr(tdq), 0.37,
               _lib.ptr(Ao), _lib.ptr(po))
     A_ref, p_ref = np.zeros_like(A), np.zeros_like(p)
     O.lib().orc_update_q(nb, A.ctypes.data, p.ctypes.data, dq.ctypes.data, 0.37, A_ref.ctypes.data, p_ref.ctypes.data)
     np.testing.assert_array_equal(N(Ao), A_ref)
     np.testing.assert_array_equal(N(po), p_ref)
+
+
+# ---------------------------------------------------------------- fixtures
+def test_gpu_matches_golden_fixtures(lib):
+    import os
+    from abd_hip.ipc import pair_gh
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fixed_pair.npz"))
+    cp = params(dhat=float(f["dhat"]))
+    for kind, tag in ((0, "pt"), (1, "ee")):
+        X = np.concatenate([f["x"][None], f[f"{tag}_act_x"]])
+        R = np.concatenate([f["x"][None], f[f"{tag}_act_r"]])
+        act, g24, H24 = pair_gh(kind, cp, T(X), T(R))
+        act, g24, H24 = N(act), N(g24), N(H24)
+        assert not act[0] and act[1:].all()
+        for i in range(1, len(X)):
+            assert rel_err(g24[i], f[f"{tag}_act_g24"][i - 1]) <= 1e-9
+            assert rel_err(H24[i], f[f"{tag}_act_H24"][i - 1]) <= 1e-9
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_pair_gh_psd_mode_matches_oracle(lib, kind):
+    from abd_hip.ipc import pair_gh
+    rng = np.random.default_rng(20 + kind)
+    cp = params(psd=1)
+    X, R = [], []
+    for i in range(128):
+        d = rng.uniform(1e-5, 9.9e-4) * (1 if i % 2 else -1)
+        x4 = near_contact_pt(rng, d) if kind == 0 else near_contact_ee(rng, d)
+        X.append(x4)
+        R.append(rest_of(x4, rng))
+    X, R = np.array(X), np.array(R)
+    act, g24, H24 = pair_gh(kind, cp, T(X), T(R))
+    act, g24, H24 = N(act), N(g24), N(H24)
+    prm = O.params(cp)
+    for i in range(len(X)):
+        a, s, g, Hi, Hj, Hij = O.pair_gh(kind, prm, X[i], R[i])
+        assert bool(act[i]) == a
+        if a:
+            Href = np.block([[Hi, Hij], [Hij.T, Hj]])
+            assert rel_err(H24[i], Href) <= 1e-9
+            assert rel_err(g24[i], g) <= 1e-9
