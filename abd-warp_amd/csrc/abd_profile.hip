@@ -14,8 +14,11 @@ struct Rec {
     int kernel;
     hipEvent_t a, b;
     double bytes;
-    const int32_t* scale;
+    int scale_slot;  // index into the pinned scale buffer, -1 when unscaled
 };
+constexpr int kScaleSlots = 1 << 16;
+int32_t* g_scale = nullptr;  // pinned host copies of per-launch scale values
+int g_scale_used = 0;
 std::mutex g_mu;
 bool g_on = false;
 std::vector<Rec> g_recs;
@@ -28,12 +31,7 @@ int resolve_locked() {
         ABD_CHECK(hipEventSynchronize(r.b));
         float ms = 0.f;
         ABD_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
-        double mult = 1.0;
-        if (r.scale) {
-            int32_t v = 0;
-            ABD_CHECK(hipMemcpy(&v, r.scale, sizeof(v), hipMemcpyDeviceToHost));
-            mult = v;
-        }
+        const double mult = r.scale_slot >= 0 ? (double)g_scale[r.scale_slot] : 1.0;
         g_ms[r.kernel] += ms;
         g_bytes[r.kernel] += r.bytes * mult;
         g_n[r.kernel] += 1;
@@ -41,6 +39,7 @@ int resolve_locked() {
         hipEventDestroy(r.b);
     }
     g_recs.clear();
+    g_scale_used = 0;
     return ABD_OK;
 }
 }  // namespace
@@ -50,7 +49,7 @@ bool prof_enabled() { return g_on; }
 int prof_start(int kernel, hipStream_t s, double bytes) {
     if (!g_on) return -1;
     std::lock_guard<std::mutex> lock(g_mu);
-    Rec r{kernel, nullptr, nullptr, bytes, nullptr};
+    Rec r{kernel, nullptr, nullptr, bytes, -1};
     if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
     if (hipEventRecord(r.a, s) != hipSuccess) return -1;
     g_recs.push_back(r);
@@ -61,7 +60,12 @@ void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale) {
     if (handle < 0) return;
     std::lock_guard<std::mutex> lock(g_mu);
     if (handle >= (int)g_recs.size()) return;
-    g_recs[handle].scale = dev_scale;
+    if (dev_scale && g_scale && g_scale_used < kScaleSlots) {
+        // the value as of this launch, copied on the launch stream
+        const int slot = g_scale_used++;
+        if (hipMemcpyAsync(&g_scale[slot], dev_scale, sizeof(int32_t), hipMemcpyDeviceToHost, s) == hipSuccess)
+            g_recs[handle].scale_slot = slot;
+    }
     hipEventRecord(g_recs[handle].b, s);
 }
 
@@ -79,6 +83,8 @@ int abd_profile_begin(void) {
         g_bytes[k] = 0.0;
         g_n[k] = 0;
     }
+    if (!g_scale && hipHostMalloc((void**)&g_scale, sizeof(int32_t) * kScaleSlots, hipHostMallocDefault) != hipSuccess)
+        g_scale = nullptr;
     g_on = true;
     return rc;
 }
