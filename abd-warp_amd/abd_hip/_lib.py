@@ -29,6 +29,11 @@ class AbdScene(C.Structure):
                                   "body_scene", "mass", "I0")]
 
 
+class AbdSimState(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("A", "p", "Ak", "pk", "A0", "p0", "Adot", "pdot", "x", "xk", "x_view", "g",
+                                          "dq")]
+
+
 class AbdError(RuntimeError):
     def __init__(self, fn: str, code: int):
         self.code = code
@@ -76,6 +81,9 @@ def load():
         "abd_toi": [p, S, P, p, p, p, i64, p, i64, p, i64, p],
         "abd_cubic_roots": [p, p, i64, d, d, p, p],
         "abd_collision_time": [p, i32, p, p, i64, p],
+        "abd_sim_create": [S, P, C.c_void_p, C.POINTER(C.c_void_p)],
+        "abd_sim_step": [p, p, C.POINTER(C.c_int64)],
+        "abd_sim_destroy": [p],
         "abd_profile_begin": [],
         "abd_profile_end": [],
         "abd_profile_query": [i32, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)],

@@ -31,7 +31,10 @@ from .sparse import SolveInfo, bicgstab, bsr_cg, bsr_from_pairs
 
 
 class AffineBodySimulator:
-    def __init__(self, spec: SceneSpec, params: Optional[ContactParams] = None, device="cuda"):
+    """native=True runs step() in the library's C++ frame driver (abd_sim_step): the
+    same kernels in the same order, without a Python round trip per stage."""
+
+    def __init__(self, spec: SceneSpec, params: Optional[ContactParams] = None, device="cuda", native=False):
         self.params = params if params is not None else ContactParams(ground=spec.ground)
         self.meshes = Meshes(spec, device)
         self.warp_meshes = self.meshes
@@ -66,11 +69,46 @@ class AffineBodySimulator:
         self.profile_gh = False
         self.gh_time_ms = 0.0
         self._gh_events = []
+        self._sim = None
+        if native:
+            self._make_native()
+
+    def _make_native(self):
+        import ctypes
+        st, m = self.states, self.meshes
+        self._state_c = _lib.AbdSimState(*[_lib.ptr(t) for t in (st.A, st.p, st.Ak, st.pk, st.A0, st.p0, st.Adot,
+                                                                  st.pdot, m.x, m.xk, m.x_view, self.g, self.dq)])
+        self._params_c = self.params.to_c()
+        h = ctypes.c_void_p(0)
+        _lib.call("abd_sim_create", m.c, self._params_c, ctypes.byref(self._state_c), ctypes.byref(h))
+        self._sim = h
+        self._native_stats = (ctypes.c_int64 * 10)()
+
+    def __del__(self):
+        if getattr(self, "_sim", None) is not None:
+            try:
+                _lib.call("abd_sim_destroy", self._sim)
+            except Exception:
+                pass
+            self._sim = None
+
+    def _step_native(self) -> int:
+        before = list(self._native_stats)
+        _lib.call("abd_sim_step", self._sim, _lib.stream_handle(), self._native_stats)
+        s = list(self._native_stats)
+        d = [a - b for a, b in zip(s, before)]
+        for k, name in enumerate(("newton_iters", "gh_pt", "gh_ee", "gh_vg", "gh_active", "energy_evals",
+                                  "solver_iters", "frames", "ccd_pt", "ccd_ee")):
+            # library order: iters, pt, ee, vg, active, energy evals, solver iters, frames, ccd pt, ccd ee
+            self.stats[name] += d[k]
+        return d[0]
 
     # ------------------------------------------------------------ states
     def reset(self):
-        """abd.py:97-104"""
-        self.states = self.meshes.states()
+        """abd.py:97-104 (copies into the existing buffers, which the native driver holds)"""
+        fresh = self.meshes.states()
+        for name in ("A", "p", "Ak", "pk", "A0", "p0", "Adot", "pdot", "mass", "I0"):
+            getattr(self.states, name).copy_(getattr(fresh, name))
         self.dq.zero_()
         self.update_mesh_vertex("x")
 
@@ -180,6 +218,8 @@ class AffineBodySimulator:
 
     def step(self, frame: int = 1):
         """abd.py:212-276: one frame."""
+        if self._sim is not None:
+            return self._step_native()
         states = self.states
         p = self.params
         self.q_gets_q0()

@@ -65,6 +65,31 @@ __device__ __forceinline__ float block_maxf(float v, float* sh) {
     return r;
 }
 
+// Sum over the workgroup with two barriers: a fixed shuffle tree inside each
+// wave (lane 0 ends with the wave's sum), wave sums through LDS, then the first
+// wave reduces those in the same fixed order.  Every thread receives the value
+// computed by one lane, so all waves branch alike on it.  blockDim.x must be a
+// multiple of 64 and at most 1024; sh must hold 17 doubles.
+__device__ __forceinline__ double wave_sum_down(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    return v;
+}
+__device__ __forceinline__ double block_sum_fast(double v, double* sh) {
+    const int lane = lane_id(), w = wave_id(), nw = n_waves();
+    v = wave_sum_down(v);
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        double t = lane < nw ? sh[lane] : 0.0;
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) t += __shfl_down(t, off, kWave);
+        if (lane == 0) sh[16] = t;
+    }
+    __syncthreads();
+    return sh[16];
+}
+
 // Exclusive prefix sum of one int64 per thread (sh: blockDim.x entries).
 __device__ __forceinline__ int64_t block_exclusive_scan_i64(int64_t v, int64_t* sh) {
     const int t = threadIdx.x;

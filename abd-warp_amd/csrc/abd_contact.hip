@@ -687,7 +687,6 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
                        highlight);
     ABD_CHECK_LAUNCH();
     prof_stop(phf, s);
-    ABD_CHECK(hipStreamSynchronize(s));
     if (n_active) *n_active = nact;
     return ABD_OK;
 }

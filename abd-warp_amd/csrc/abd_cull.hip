@@ -394,7 +394,6 @@ int abd_cull_bodies(abd_stream_t stream, const AbdScene* sc, const float* lo, co
     hipLaunchKernelGGL(body_pairs_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 1,
                        sc->body_scene, lo, hi, bcnt, (const int64_t*)boff, ji, cap);
     ABD_CHECK_LAUNCH();
-    ABD_CHECK(hipStreamSynchronize(s));
     return ABD_OK;
 }
 
@@ -474,7 +473,6 @@ int abd_cull_prims(abd_stream_t stream, const AbdScene* sc, int32_t mode, const 
                        (const int64_t*)(choff + nunits), ccnt, (const int64_t*)ccoff, out, cap);
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s);
-    ABD_CHECK(hipStreamSynchronize(s));
     return ABD_OK;
 }
 
@@ -506,7 +504,6 @@ int abd_cull_vg(abd_stream_t stream, const AbdScene* sc, const float* blo, const
     hipLaunchKernelGGL(vg_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nv, sc->vert_body, sc->voff, blo,
                        plo, phi, yth, fg, bcnt, (const int64_t*)boff, out, cap);
     ABD_CHECK_LAUNCH();
-    ABD_CHECK(hipStreamSynchronize(s));
     return ABD_OK;
 }
 

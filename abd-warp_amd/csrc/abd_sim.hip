@@ -1,0 +1,355 @@
+// abd_sim.hip — native frame driver: AffineBodySimulator.step (abd.py:212-276)
+// run from C++ over the stage entry points of this library, so that a Newton
+// iteration costs a handful of host synchronizations (the sizes of candidate
+// lists and the scalars the line search and convergence test branch on)
+// instead of a Python round trip per stage.  The arithmetic is the same
+// sequence of kernels as the Python mirror, so both give identical states.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "abd_internal.h"
+
+namespace abd {
+
+namespace {
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+    int reserve(size_t n) {
+        if (n <= cap && p) return ABD_OK;
+        size_t want = std::max<size_t>(n + n / 2, 64);
+        if (p) {
+            ABD_CHECK(hipDeviceSynchronize());
+            ABD_CHECK(hipFree(p));
+            p = nullptr;
+        }
+        ABD_CHECK(hipMalloc((void**)&p, want * sizeof(T)));
+        cap = want;
+        return ABD_OK;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct BoxBuf {
+    DevBuf<float> lo, hi;
+    int reserve(size_t n) {
+        int rc = lo.reserve(3 * n);
+        return rc ? rc : hi.reserve(3 * n);
+    }
+};
+
+struct Sim {
+    AbdScene sc;
+    AbdParams P;
+    AbdSimState st;
+    int nb, nv, ne, nf;
+    BoxBuf body, tri, edge, pt;         // boxes of the last collision_set
+    BoxBuf tbody, ttri, tedge, tpt;     // trajectory boxes
+    DevBuf<int32_t> ij, ji, ptl, eel, vgl;
+    int64_t nij = 0, npt = 0, nee = 0, nvg = 0;
+    DevBuf<int32_t> row_ptr, cols, slots;
+    DevBuf<double> blocks;
+    DevBuf<double> scal;  // [0] E contact, [1] E inertia, [2] toi, [3] dot, [4] resid
+    DevBuf<int32_t> iters;
+    double* host = nullptr;  // pinned scalars
+    int64_t stats[10] = {};
+};
+
+int get_scalars(hipStream_t s, Sim& S, int first, int count) {
+    ABD_CHECK(hipMemcpyAsync(S.host + first, S.scal.p + first, sizeof(double) * count, hipMemcpyDeviceToHost, s));
+    ABD_CHECK(hipStreamSynchronize(s));
+    return ABD_OK;
+}
+
+int cull_list(hipStream_t s, Sim& S, int mode, const BoxBuf& b1, const BoxBuf& b2, DevBuf<int32_t>& out,
+              int64_t* n) {
+    for (;;) {
+        int64_t cnt = 0;
+        int rc = abd_cull_prims(s, &S.sc, mode, S.ij.p, S.nij, b1.lo.p, b1.hi.p, b2.lo.p, b2.hi.p, out.p,
+                                (int64_t)(out.cap / 5), &cnt);
+        if (rc == ABD_ERR_OVERFLOW) {
+            if ((rc = out.reserve(5 * (size_t)cnt))) return rc;
+            continue;
+        }
+        if (rc) return rc;
+        *n = cnt;
+        return ABD_OK;
+    }
+}
+
+int cull_vg_list(hipStream_t s, Sim& S, const BoxBuf& body, const BoxBuf& pts) {
+    for (;;) {
+        int64_t cnt = 0;
+        int rc = abd_cull_vg(s, &S.sc, body.lo.p, pts.lo.p, pts.hi.p, S.P.dhat, S.P.ground, S.vgl.p,
+                             (int64_t)(S.vgl.cap / 2), &cnt);
+        if (rc == ABD_ERR_OVERFLOW) {
+            if ((rc = S.vgl.reserve(2 * (size_t)cnt))) return rc;
+            continue;
+        }
+        if (rc) return rc;
+        S.nvg = cnt;
+        return ABD_OK;
+    }
+}
+
+int body_pairs(hipStream_t s, Sim& S, const BoxBuf& body, bool structure) {
+    for (;;) {
+        int64_t cnt = 0;
+        int rc = abd_cull_bodies(s, &S.sc, body.lo.p, body.hi.p, S.ij.p, S.ji.p, (int64_t)(S.ij.cap / 2), &cnt);
+        if (rc == ABD_ERR_OVERFLOW) {
+            if ((rc = S.ij.reserve(2 * (size_t)cnt))) return rc;
+            if ((rc = S.ji.reserve(2 * (size_t)cnt))) return rc;
+            continue;
+        }
+        if (rc) return rc;
+        S.nij = cnt;
+        break;
+    }
+    if (!structure) return ABD_OK;
+    int rc;
+    if ((rc = S.cols.reserve(S.nb + 2 * (size_t)S.nij))) return rc;
+    if ((rc = S.slots.reserve(S.nb + 2 * (size_t)S.nij))) return rc;
+    return abd_bsr_structure(s, S.nb, S.ij.p, S.ji.p, S.nij, S.row_ptr.p, S.cols.p, S.slots.p);
+}
+
+// abd.py:187-207 (on xk)
+int collision_set(hipStream_t s, Sim& S, bool structure) {
+    const double dh = S.P.dhat * 0.5;
+    int rc;
+    if ((rc = abd_body_boxes(s, &S.sc, S.st.xk, dh, S.body.lo.p, S.body.hi.p))) return rc;
+    if ((rc = body_pairs(s, S, S.body, structure))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_TRI, S.st.xk, 0.0, S.tri.lo.p, S.tri.hi.p))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_EDGE, S.st.xk, dh, S.edge.lo.p, S.edge.hi.p))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_POINT, S.st.xk, S.P.dhat, S.pt.lo.p, S.pt.hi.p))) return rc;
+    if ((rc = cull_list(s, S, ABD_CULL_EE, S.edge, S.edge, S.eel, &S.nee))) return rc;
+    if ((rc = cull_list(s, S, ABD_CULL_PT, S.pt, S.tri, S.ptl, &S.npt))) return rc;
+    return cull_vg_list(s, S, S.body, S.pt);
+}
+
+// abd.py:170-185
+int trajectory_set(hipStream_t s, Sim& S) {
+    int rc;
+    auto copy = [&](BoxBuf& dst, const BoxBuf& src, size_t n) -> int {
+        ABD_CHECK(hipMemcpyAsync(dst.lo.p, src.lo.p, 12 * n, hipMemcpyDeviceToDevice, s));
+        ABD_CHECK(hipMemcpyAsync(dst.hi.p, src.hi.p, 12 * n, hipMemcpyDeviceToDevice, s));
+        return ABD_OK;
+    };
+    if ((rc = copy(S.tbody, S.body, S.nb))) return rc;
+    if ((rc = abd_body_boxes_traj(s, &S.sc, S.st.xk, S.tbody.lo.p, S.tbody.hi.p))) return rc;
+    if ((rc = body_pairs(s, S, S.tbody, false))) return rc;
+    if ((rc = copy(S.ttri, S.tri, S.nf))) return rc;
+    if ((rc = copy(S.tedge, S.edge, S.ne))) return rc;
+    if ((rc = copy(S.tpt, S.pt, S.nv))) return rc;
+    if ((rc = abd_prim_boxes_traj(s, &S.sc, ABD_KIND_TRI, S.st.xk, S.ttri.lo.p, S.ttri.hi.p))) return rc;
+    if ((rc = abd_prim_boxes_traj(s, &S.sc, ABD_KIND_EDGE, S.st.xk, S.tedge.lo.p, S.tedge.hi.p))) return rc;
+    if ((rc = abd_prim_boxes_traj(s, &S.sc, ABD_KIND_POINT, S.st.xk, S.tpt.lo.p, S.tpt.hi.p))) return rc;
+    if ((rc = cull_list(s, S, ABD_CULL_EE, S.tedge, S.tedge, S.eel, &S.nee))) return rc;
+    if ((rc = cull_list(s, S, ABD_CULL_PT, S.tpt, S.ttri, S.ptl, &S.npt))) return rc;
+    return cull_vg_list(s, S, S.tbody, S.tpt);
+}
+
+// abd.py:133-141
+int compute_energy(hipStream_t s, Sim& S, double alpha, double* E) {
+    int rc;
+    const AbdSimState& st = S.st;
+    if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, alpha, st.Ak, st.pk))) return rc;
+    if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
+    if ((rc = collision_set(s, S, false))) return rc;
+    if ((rc = abd_contact_energy(s, &S.sc, &S.P, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.scal.p)))
+        return rc;
+    if ((rc = abd_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot, S.scal.p + 1)))
+        return rc;
+    if ((rc = get_scalars(s, S, 0, 2))) return rc;
+    S.stats[5] += 1;
+    *E = S.host[0] + S.host[1];
+    return ABD_OK;
+}
+
+double dot_host(hipStream_t s, Sim& S, const double* a, const double* b, int* rc) {
+    *rc = abd_dot(s, a, b, 12 * (int64_t)S.nb, S.scal.p + 3);
+    if (*rc) return 0.0;
+    *rc = get_scalars(s, S, 3, 1);
+    return S.host[3];
+}
+
+// abd.py:278-303
+int ccd(hipStream_t s, Sim& S, double* alpha_cap) {
+    int rc;
+    const AbdSimState& st = S.st;
+    if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, 1.0, st.Ak, st.pk))) return rc;
+    if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
+    if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
+    if ((rc = trajectory_set(s, S))) return rc;
+    S.stats[8] += S.npt;
+    S.stats[9] += S.nee;
+    if ((rc = abd_toi(s, &S.sc, &S.P, st.x, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.scal.p + 2)))
+        return rc;
+    if ((rc = get_scalars(s, S, 2, 1))) return rc;
+    double t = S.host[2];
+    t = std::max(0.0, std::min(1.0, t));
+    if (t < 1.0) t *= 0.9;
+    *alpha_cap = t;
+    return ABD_OK;
+}
+
+int step(hipStream_t s, Sim& S) {
+    const AbdSimState& st = S.st;
+    const AbdParams& P = S.P;
+    const int nb = S.nb;
+    int rc;
+    // q_gets_q0 (abd.py:124-128)
+    ABD_CHECK(hipMemcpyAsync(st.A, st.A0, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+    ABD_CHECK(hipMemcpyAsync(st.p, st.p0, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+    ABD_CHECK(hipMemcpyAsync(st.Ak, st.A0, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+    ABD_CHECK(hipMemcpyAsync(st.pk, st.p0, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+    double E0;
+    if ((rc = compute_energy(s, S, 0.0, &E0))) return rc;
+    int it = 0;
+    for (;;) {
+        if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
+        if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
+        if ((rc = collision_set(s, S, true))) return rc;
+        const size_t nslot = nb + 2 * (size_t)S.nij;
+        if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
+        ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
+        ABD_CHECK(hipMemsetAsync(st.g, 0, sizeof(double) * 12 * nb, s));
+        if ((rc = abd_inertia_gradient(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g))) return rc;
+        if ((rc = abd_inertia_hessian(s, &S.sc, &P, st.A, S.blocks.p))) return rc;
+        int64_t nact = 0;
+        if ((rc = abd_contact_gh(s, &S.sc, &P, st.x, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.nij,
+                                 S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, &nact)))
+            return rc;
+        S.stats[1] += S.npt;
+        S.stats[2] += S.nee;
+        S.stats[3] += S.nvg;
+        S.stats[4] += nact;
+        if ((rc = abd_bsr_solve(s, P.solver, nb, S.row_ptr.p, S.cols.p, S.slots.p, S.blocks.p, st.g, st.dq,
+                                P.solver_tol, P.solver_maxiter, 10, S.iters.p, S.scal.p + 4)))
+            return rc;
+        double alpha_cap;
+        if ((rc = ccd(s, S, &alpha_cap))) return rc;
+        // abd.py:146-160 line search (Armijo test as written)
+        const double qTg = dot_host(s, S, st.dq, st.g, &rc);
+        if (rc) return rc;
+        double alpha = alpha_cap, E1;
+        for (;;) {
+            if ((rc = compute_energy(s, S, alpha, &E1))) return rc;
+            const bool wolfe = E1 < E0 + P.c1 * alpha * qTg;
+            if (wolfe || alpha < P.alpha_min) break;
+            alpha *= 0.5;
+        }
+        E0 = E1;
+        if ((rc = abd_update_q(s, nb, st.A, st.p, st.dq, alpha, st.A, st.p))) return rc;
+        ++it;
+        // g.g (abd.py:265) and the solver's iteration count in one read-back
+        if ((rc = abd_dot(s, st.g, st.g, 12 * (int64_t)nb, S.scal.p + 3))) return rc;
+        ABD_CHECK(hipMemcpyAsync(S.host + 6, S.iters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        if ((rc = get_scalars(s, S, 3, 1))) return rc;
+        const double gTg = S.host[3];
+        int32_t sit;
+        memcpy(&sit, S.host + 6, sizeof(int32_t));
+        S.stats[6] += sit;
+        if (gTg < P.tol || it >= P.max_iter) break;
+    }
+    if ((rc = abd_update_q0qdot(s, nb, P.dt, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot))) return rc;
+    if ((rc = abd_project_vertices(s, &S.sc, st.A0, st.p0, st.x_view))) return rc;
+    S.stats[0] += it;
+    S.stats[7] += 1;
+    return ABD_OK;
+}
+
+}  // namespace
+
+}  // namespace abd
+
+using namespace abd;
+
+extern "C" {
+
+int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimState* state, void** out) {
+    if (!scene || !params || !state || !out || scene->n_bodies <= 0) return ABD_ERR_ARG;
+    Sim* S = new (std::nothrow) Sim();
+    if (!S) return ABD_ERR_ARG;
+    S->sc = *scene;
+    S->P = *params;
+    S->st = *state;
+    S->nb = scene->n_bodies;
+    S->nv = scene->n_verts;
+    S->ne = scene->n_edges;
+    S->nf = scene->n_tris;
+    int rc = 0;
+    const size_t nb = S->nb;
+    BoxBuf* b4[] = {&S->body, &S->tbody};
+    for (BoxBuf* b : b4)
+        if (!rc) rc = b->reserve(nb);
+    if (!rc) rc = S->tri.reserve(S->nf + 1);
+    if (!rc) rc = S->ttri.reserve(S->nf + 1);
+    if (!rc) rc = S->edge.reserve(S->ne + 1);
+    if (!rc) rc = S->tedge.reserve(S->ne + 1);
+    if (!rc) rc = S->pt.reserve(S->nv);
+    if (!rc) rc = S->tpt.reserve(S->nv);
+    if (!rc) rc = S->ij.reserve(2 * 256);
+    if (!rc) rc = S->ji.reserve(2 * 256);
+    if (!rc) rc = S->ptl.reserve(5 * 4096);
+    if (!rc) rc = S->eel.reserve(5 * 4096);
+    if (!rc) rc = S->vgl.reserve(2 * 1024);
+    if (!rc) rc = S->row_ptr.reserve(nb + 1);
+    if (!rc) rc = S->cols.reserve(nb + 512);
+    if (!rc) rc = S->slots.reserve(nb + 512);
+    if (!rc) rc = S->blocks.reserve((nb + 512) * 144);
+    if (!rc) rc = S->scal.reserve(8);
+    if (!rc) rc = S->iters.reserve(4);
+    if (!rc && hipHostMalloc((void**)&S->host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = ABD_ERR_ARG;
+    if (rc) {
+        delete S;
+        return rc;
+    }
+    *out = S;
+    return ABD_OK;
+}
+
+int abd_sim_step(void* sim, abd_stream_t stream, int64_t* stats) {
+    if (!sim) return ABD_ERR_ARG;
+    Sim& S = *static_cast<Sim*>(sim);
+    int rc = step(to_stream(stream), S);
+    if (stats)
+        for (int k = 0; k < 10; ++k) stats[k] = S.stats[k];
+    return rc;
+}
+
+int abd_sim_destroy(void* sim) {
+    if (!sim) return ABD_OK;
+    Sim* S = static_cast<Sim*>(sim);
+    hipDeviceSynchronize();
+    for (BoxBuf* b : {&S->body, &S->tri, &S->edge, &S->pt, &S->tbody, &S->ttri, &S->tedge, &S->tpt}) {
+        b->lo.release();
+        b->hi.release();
+    }
+    S->ij.release();
+    S->ji.release();
+    S->ptl.release();
+    S->eel.release();
+    S->vgl.release();
+    S->row_ptr.release();
+    S->cols.release();
+    S->slots.release();
+    S->blocks.release();
+    S->scal.release();
+    S->iters.release();
+    if (S->host) hipHostFree(S->host);
+    delete S;
+    return ABD_OK;
+}
+
+}  // extern "C"

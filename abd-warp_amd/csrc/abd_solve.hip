@@ -16,6 +16,8 @@
 namespace abd {
 
 constexpr int kSolveThreads = 1024;
+// Systems with 6 n doubles up to this size keep every Krylov vector in LDS.
+constexpr size_t kLdsVecBytes = 96 * 1024;
 
 struct Bsr {
     int nb;
@@ -49,21 +51,33 @@ __device__ __forceinline__ double vdot(const double* __restrict__ a, const doubl
                                        double* sh) {
     double acc = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) acc += a[i] * b[i];
-    return block_sum(acc, sh);
+    return block_sum_fast(acc, sh);
 }
 
+// Vector storage: LDS (dynamic shared memory) or the scratch arena.
+template <bool LDS>
+__device__ __forceinline__ double* vec_base(double* ws) {
+    extern __shared__ double lds_vec[];
+    return LDS ? lds_vec : ws;
+}
+
+template <bool LDS>
 __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const double* __restrict__ b,
-                                                                  double* __restrict__ x, double tol, int maxiter,
+                                                                  double* __restrict__ xg, double tol, int maxiter,
                                                                   int check_every, double* __restrict__ ws,
                                                                   int32_t* __restrict__ iters,
                                                                   double* __restrict__ resid) {
-    __shared__ double sh[kSolveThreads];
+    __shared__ double sh[17];
     const int n = 12 * M.nb;
-    double* r = ws;
-    double* r0 = ws + n;
-    double* p = ws + 2 * n;
-    double* v = ws + 3 * n;
-    double* t = ws + 4 * n;
+    double* V = vec_base<LDS>(ws);
+    double* x = V;
+    double* r = V + n;
+    double* r0 = V + 2 * n;
+    double* p = V + 3 * n;
+    double* v = V + 4 * n;
+    double* t = V + 5 * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = xg[i];
+    __syncthreads();
     spmv(M, x, t, n);
     for (int i = threadIdx.x; i < n; i += blockDim.x) r[i] = b[i] - t[i];
     __syncthreads();
@@ -109,6 +123,7 @@ __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const do
             if (rr <= atol_sq || it >= maxiter) break;
         }
     }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) xg[i] = x[i];
     if (threadIdx.x == 0) {
         if (iters) *iters = it;
         if (resid) *resid = rr;
@@ -125,22 +140,26 @@ __device__ __forceinline__ void jacobi(const double* __restrict__ dg, const doub
     __syncthreads();
 }
 
+template <bool LDS>
 __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const double* __restrict__ b,
-                                                            double* __restrict__ x, double tol, int maxiter,
+                                                            double* __restrict__ xg, double tol, int maxiter,
                                                             int check_every, double* __restrict__ ws,
                                                             int32_t* __restrict__ iters,
                                                             double* __restrict__ resid) {
-    __shared__ double sh[kSolveThreads];
+    __shared__ double sh[17];
     const int n = 12 * M.nb;
-    double* r = ws;
-    double* p = ws + n;
-    double* Ap = ws + 2 * n;
-    double* z = ws + 3 * n;
-    double* dg = ws + 4 * n;
+    double* V = vec_base<LDS>(ws);
+    double* x = V;
+    double* r = V + n;
+    double* p = V + 2 * n;
+    double* Ap = V + 3 * n;
+    double* z = V + 4 * n;
+    double* dg = V + 5 * n;
     // bsr_get_diag: diagonal of the diagonal 3x3 blocks (sub-block (ii, ii) of slot b)
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int bb = i / 12, ii = (i % 12) / 3, a = i % 3;
         dg[i] = M.blocks[9 * (16 * (int64_t)bb + ii + 4 * ii) + 4 * a];
+        x[i] = xg[i];
     }
     __syncthreads();
     spmv(M, x, Ap, n);
@@ -180,6 +199,7 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const double* 
             err = rz_old;
         }
     }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) xg[i] = x[i];
     if (threadIdx.x == 0) {
         if (iters) *iters = end_iter;
         if (resid) *resid = err;
@@ -200,7 +220,7 @@ int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t
     const int n = 12 * nb;
     if (maxiter <= 0) maxiter = n;
     double* ws;
-    int rc = arena_get(kSlotSolver, sizeof(double) * 5 * (size_t)n, (void**)&ws);
+    int rc = arena_get(kSlotSolver, sizeof(double) * 6 * (size_t)n, (void**)&ws);
     if (rc) return rc;
     Bsr M{nb, row_ptr, cols, slots, blocks};
     hipStream_t s = to_stream(stream);
@@ -210,12 +230,31 @@ int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t
     if (prof_enabled()) ABD_CHECK(hipMemcpy(&nnzb, row_ptr + nb, sizeof(int32_t), hipMemcpyDeviceToHost));
     const double per_it = 2.0 * (1156.0 * nnzb + 4.0 * (nb + 1) + 2.0 * 96 * nb) + 10.0 * 96 * nb;
     int ph = prof_start(kProfSolve, s, per_it);
-    if (method == ABD_SOLVER_BICGSTAB)
-        hipLaunchKernelGGL(bicgstab_kernel, dim3(1), dim3(kSolveThreads), 0, s, M, b, x, tol, maxiter, check_every,
-                           ws, iters, resid);
-    else
-        hipLaunchKernelGGL(cg_kernel, dim3(1), dim3(kSolveThreads), 0, s, M, b, x, tol, maxiter, check_every, ws,
-                           iters, resid);
+    const size_t vec_bytes = sizeof(double) * 6 * (size_t)n;
+    const bool lds = vec_bytes <= kLdsVecBytes;
+    if (lds) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipFuncSetAttribute((const void*)bicgstab_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kLdsVecBytes);
+            hipFuncSetAttribute((const void*)cg_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kLdsVecBytes);
+            attr_set = true;
+        }
+        if (method == ABD_SOLVER_BICGSTAB)
+            hipLaunchKernelGGL(bicgstab_kernel<true>, dim3(1), dim3(kSolveThreads), vec_bytes, s, M, b, x, tol,
+                               maxiter, check_every, ws, iters, resid);
+        else
+            hipLaunchKernelGGL(cg_kernel<true>, dim3(1), dim3(kSolveThreads), vec_bytes, s, M, b, x, tol, maxiter,
+                               check_every, ws, iters, resid);
+    } else {
+        if (method == ABD_SOLVER_BICGSTAB)
+            hipLaunchKernelGGL(bicgstab_kernel<false>, dim3(1), dim3(kSolveThreads), 0, s, M, b, x, tol, maxiter,
+                               check_every, ws, iters, resid);
+        else
+            hipLaunchKernelGGL(cg_kernel<false>, dim3(1), dim3(kSolveThreads), 0, s, M, b, x, tol, maxiter,
+                               check_every, ws, iters, resid);
+    }
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s, iters);
     return ABD_OK;

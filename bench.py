@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--solver", type=int, default=1, help="0 BiCGSTAB, 1 Jacobi CG")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--python-driver", action="store_true", help="run the Python mirror of the driver")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the contact path of one scene is "
@@ -96,7 +97,7 @@ def main():
     abd_hip.load()
     spec, desc = make_workload(a.workload)
     cp = ContactParams(ground=spec.ground, psd=a.psd, solver=a.solver)
-    sim = AffineBodySimulator(spec, cp, "cuda")
+    sim = AffineBodySimulator(spec, cp, "cuda", native=not a.python_driver)
     for f in range(a.warmup):
         it = sim.step(f)
         log(f"[gpu] warmup frame {f} newton {it}")
@@ -145,6 +146,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": desc, "mode": "plain", "psd_projection": bool(a.psd),
+                   "driver": "python" if a.python_driver else "native",
                    "solver": "jacobi-cg" if a.solver == 1 else "bicgstab", "frames_timed": a.steps},
         "gh_pairs_per_s": cands / gh_s if gh_s > 0 else None,
         "gh_candidates": cands,

@@ -7,8 +7,9 @@
  *    layouts below) unless its comment says "host".
  *  - Every call is stream-ordered on `stream` (a hipStream_t; NULL = the
  *    default stream).  Calls that must return a size to the host (candidate
- *    counts, active counts) synchronize `stream` before returning — as the
- *    reference's `compress()` does (simulator/fenwick.py:289).
+ *    counts, active counts) wait for `stream` up to the point where that size
+ *    is known — as the reference's `compress()` does (simulator/fenwick.py:289);
+ *    the list itself is written by work still queued on `stream`.
  *  - Return value: ABD_OK (0); ABD_ERR_OVERFLOW (1) when an output list is too
  *    small (the required count is written to *count and nothing is written past
  *    `cap`); negative on error (ABD_ERR_* below, or -hipError_t).
@@ -218,6 +219,23 @@ int abd_cubic_roots(abd_stream_t stream, const double* coef, int64_t n, double x
  * quadruples p_t0, p_t1 [n][4][3]: t [n]. */
 int abd_collision_time(abd_stream_t stream, int32_t kind, const double* p_t0, const double* p_t1, int64_t n,
                        double* t);
+
+/* ---- native frame driver ---- */
+/* Device buffers of one simulation, owned by the caller (all device pointers). */
+typedef struct AbdSimState {
+    double *A, *p, *Ak, *pk, *A0, *p0, *Adot, *pdot; /* [Nb][3][3] / [Nb][3] */
+    double *x, *xk, *x_view;                       /* [V][3] */
+    double *g, *dq;                                /* [4 Nb][3] */
+} AbdSimState;
+/* abd.py:21-95 AffineBodySimulator.__init__: *sim receives a driver bound to `scene`
+ * (its pointers must stay valid), `params` and `state`. */
+int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimState* state, void** sim);
+/* abd.py:212-276 step(): one frame (Newton loop with culling, gh, solve, CCD and line
+ * search) on `stream`.  stats (host, nullable) [10] receives running totals: [0] Newton
+ * iterations, [1..3] gh PT / EE / VG candidates, [4] active contacts, [5] energy
+ * evaluations, [6] solver iterations, [7] frames, [8..9] CCD PT / EE candidates. */
+int abd_sim_step(void* sim, abd_stream_t stream, int64_t* stats);
+int abd_sim_destroy(void* sim);
 
 /* ---- measurement ---- */
 /* Kernel timing with HIP events recorded on each launch's stream, and the

@@ -11,9 +11,9 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def _run(spec, cp, frames):
+def _run(spec, cp, frames, native=False):
     from abd_hip.abd import AffineBodySimulator
-    sim = AffineBodySimulator(spec, cp, "cuda")
+    sim = AffineBodySimulator(spec, cp, "cuda", native=native)
     ref = O.OracleSim(spec, cp)
     worst = 0.0
     for f in range(frames):
@@ -27,24 +27,26 @@ def _run(spec, cp, frames):
     return sim, ref, worst
 
 
-def test_small_two_cubes_100_frames():
+@pytest.mark.parametrize("native", [False, True])
+def test_small_two_cubes_100_frames(native):
     from abd_hip import scenes
     from abd_hip.const_params import ContactParams
     spec = scenes.config_small()
     cp = ContactParams(ground=spec.ground, solver_tol=1e-12)
-    sim, ref, worst = _run(spec, cp, 100)
+    sim, ref, worst = _run(spec, cp, 100, native)
     # A came to rest on B: gap of about dhat, no interpenetration
     assert 0.0 < sim.states.p[1, 1].item() - sim.states.p[0, 1].item() - 1.0 < 2e-3
     assert sim.stats["newton_iters"] == ref.stats[0]
 
 
-def test_medium_subset_frames():
+@pytest.mark.parametrize("native", [False, True])
+def test_medium_subset_frames(native):
     from abd_hip import scenes
     from abd_hip.const_params import ContactParams
     spec = scenes.config_medium(seed=0, n_side=2, subdiv=2)
     spec.p[:] = spec.p * 0.93
     cp = ContactParams(ground=spec.ground, solver_tol=1e-12)
-    sim, ref, worst = _run(spec, cp, 8)
+    sim, ref, worst = _run(spec, cp, 8, native)
     assert sim.stats["gh_pt"] + sim.stats["gh_ee"] > 0
 
 
@@ -72,5 +74,5 @@ def test_medium_subset_psd_cg_frames():
     spec.p[:] = spec.p * 0.93
     # CG runs to convergence (maxiter well above n) so that both sides reach the same solution
     cp = ContactParams(ground=spec.ground, psd=1, solver=1, solver_tol=1e-10, solver_maxiter=20000)
-    sim, ref, worst = _run(spec, cp, 10)
+    sim, ref, worst = _run(spec, cp, 10, native=True)
     assert sim.stats["gh_active"] > 0
