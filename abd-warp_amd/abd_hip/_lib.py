@@ -26,7 +26,9 @@ class AbdScene(C.Structure):
     _fields_ = [("n_bodies", C.c_int32), ("n_verts", C.c_int32), ("n_tris", C.c_int32), ("n_edges", C.c_int32),
                 ("max_verts", C.c_int32), ("max_tris", C.c_int32), ("max_edges", C.c_int32)] + [
         (n, C.c_void_p) for n in ("voff", "foff", "eoff", "vert_body", "tri_body", "edge_body", "tris", "edges", "x0",
-                                  "body_scene", "mass", "I0")]
+                                  "body_scene", "mass", "I0")] + [
+        ("n_scenes", C.c_int32), ("scene_boff", C.c_void_p), ("scene_poff", C.c_void_p),
+        ("n_scene_pairs", C.c_int64)]
 
 
 class AbdSimState(C.Structure):
@@ -81,8 +83,15 @@ def load():
         "abd_toi": [p, S, P, p, p, p, i64, p, i64, p, i64, p],
         "abd_cubic_roots": [p, p, i64, d, d, p, p],
         "abd_collision_time": [p, i32, p, p, i64, p],
+        "abd_scene_contact_energy": [p, S, P, p, p, i64, p, i64, p, i64, p],
+        "abd_scene_inertia_energy": [p, S, P, p, p, p, p, p, p, p],
+        "abd_scene_toi": [p, S, P, p, p, p, i64, p, i64, p, i64, p],
+        "abd_scene_dot": [p, S, p, p, p],
+        "abd_update_q_scenes": [p, S, p, p, p, p, p, p],
+        "abd_bsr_solve_scenes": [p, i32, S, i32, p, p, p, p, p, p, d, i32, i32, p, p, p],
         "abd_sim_create": [S, P, C.c_void_p, C.POINTER(C.c_void_p)],
         "abd_sim_step": [p, p, C.POINTER(C.c_int64)],
+        "abd_sim_scene_iters": [p, C.POINTER(C.c_int32)],
         "abd_sim_destroy": [p],
         "abd_profile_begin": [],
         "abd_profile_end": [],

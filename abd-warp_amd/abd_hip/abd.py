@@ -32,7 +32,11 @@ from .sparse import SolveInfo, bicgstab, bsr_cg, bsr_from_pairs
 
 class AffineBodySimulator:
     """native=True runs step() in the library's C++ frame driver (abd_sim_step): the
-    same kernels in the same order, without a Python round trip per stage."""
+    same kernels in the same order, without a Python round trip per stage.
+
+    A SceneSpec with several independent scenes (body_scene) needs native=True: the
+    driver then runs each scene's own Newton loop, with one launch per stage for all
+    scenes; `scene_iters()` gives each scene's Newton iterations of the last frame."""
 
     def __init__(self, spec: SceneSpec, params: Optional[ContactParams] = None, device="cuda", native=False):
         self.params = params if params is not None else ContactParams(ground=spec.ground)
@@ -70,6 +74,8 @@ class AffineBodySimulator:
         self.gh_time_ms = 0.0
         self._gh_events = []
         self._sim = None
+        if self.meshes.n_scenes > 1 and not native:
+            raise ValueError("independent scenes run in the native driver: pass native=True")
         if native:
             self._make_native()
 
@@ -83,6 +89,15 @@ class AffineBodySimulator:
         _lib.call("abd_sim_create", m.c, self._params_c, ctypes.byref(self._state_c), ctypes.byref(h))
         self._sim = h
         self._native_stats = (ctypes.c_int64 * 10)()
+
+    def scene_iters(self) -> list:
+        """Newton iterations of each scene in the last native frame."""
+        import ctypes
+        if self._sim is None:
+            raise RuntimeError("scene_iters() needs the native driver")
+        out = (ctypes.c_int32 * max(self.meshes.n_scenes, 1))()
+        _lib.call("abd_sim_scene_iters", self._sim, out)
+        return list(out)
 
     def __del__(self):
         if getattr(self, "_sim", None) is not None:

@@ -87,6 +87,18 @@ class Meshes:
         self.body_scene = i32(spec.body_scene)
         self.mass = f64(spec.mass)
         self.I0 = f64(spec.I0)
+        # independent scenes: contiguous body ranges and per-scene pair offsets
+        bs = np.asarray(spec.body_scene, np.int64)
+        if nb and np.any(np.diff(bs) < 0):
+            raise ValueError("bodies of a scene must be contiguous and scenes in increasing order")
+        S = int(bs.max()) + 1 if nb else 0
+        sb = np.searchsorted(bs, np.arange(S + 1)).astype(np.int32)
+        nbs = np.diff(sb).astype(np.int64)
+        sp = np.concatenate([[0], np.cumsum(nbs * (nbs - 1) // 2)]).astype(np.int64)
+        self.n_scenes = S
+        self.scene_boff_host = sb
+        self.scene_boff = i32(sb)
+        self.scene_poff = torch.as_tensor(sp, device=dev)
         self.max_verts = int(vcount.max()) if nb else 0
         self.max_tris = int(fcount.max()) if nb else 0
         self.max_edges = int(ecount.max()) if nb else 0
@@ -95,7 +107,8 @@ class Meshes:
             nb, spec.n_verts, spec.n_tris, spec.n_edges, self.max_verts, self.max_tris, self.max_edges,
             _lib.ptr(self.voff), _lib.ptr(self.foff), _lib.ptr(self.eoff), _lib.ptr(self.vert_body),
             _lib.ptr(self.tri_body), _lib.ptr(self.edge_body), _lib.ptr(self.triangles), _lib.ptr(self.edges),
-            _lib.ptr(self.x0), _lib.ptr(self.body_scene), _lib.ptr(self.mass), _lib.ptr(self.I0))
+            _lib.ptr(self.x0), _lib.ptr(self.body_scene), _lib.ptr(self.mass), _lib.ptr(self.I0),
+            S, _lib.ptr(self.scene_boff), _lib.ptr(self.scene_poff), int(sp[-1]))
         # body pairs of the most recent ij_list(), used by gh()'s slot layout
         self.pairs = None
 

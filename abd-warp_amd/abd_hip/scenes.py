@@ -325,5 +325,13 @@ def subset_scene(spec: SceneSpec, bodies: Sequence[int]) -> SceneSpec:
         F = spec.tris[spec.foff[b]:spec.foff[b + 1]]
         meshes.append((V, F))
     idx = np.asarray(bodies)
+    # scene ids renumbered 0, 1, ... in order (bodies of a scene stay contiguous)
+    _, scene = np.unique(spec.body_scene[idx], return_inverse=True)
     return build_scene(meshes, spec.A[idx], spec.p[idx], spec.Adot[idx], spec.pdot[idx], spec.ground,
-                       mass=spec.mass[idx], I0=spec.I0[idx], body_scene=spec.body_scene[idx], name=spec.name + "-sub")
+                       mass=spec.mass[idx], I0=spec.I0[idx], body_scene=scene.astype(np.int32),
+                       name=spec.name + "-sub")
+
+
+def scene_of(spec: SceneSpec, s: int) -> SceneSpec:
+    """Independent scene `s` of a batched SceneSpec, as a plain one-scene spec."""
+    return subset_scene(spec, np.nonzero(spec.body_scene == s)[0])

@@ -28,29 +28,9 @@ __global__ __launch_bounds__(kBlock) void toi_partial_kernel(AbdScene sc, AbdPar
     const int64_t n = nvg + nee + npt;
     double acc = 1.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        double t = 1.0;
-        if (i < nvg) {
-            const int64_t vid = sc.voff[vg[2 * i]] + vg[2 * i + 1];
-            const double d1 = x1[3 * vid + 1] - P.ground;
-            if (d1 < 0) {
-                const double d0 = x0[3 * vid + 1] - P.ground;
-                t = d0 / (d0 - d1);
-            }
-        } else {
-            const bool is_ee = i < nvg + nee;
-            const int64_t k = is_ee ? i - nvg : i - nvg - nee;
-            int64_t vi[4];
-            if (is_ee)
-                fetch_ee_idx(S, ee + 5 * k, vi);
-            else
-                fetch_pt_idx(S, pt + 5 * k, vi);
-            d3 a[4], b[4];
-            for (int q = 0; q < 4; ++q) {
-                a[q] = ld3(x0 + 3 * vi[q]);
-                b[q] = ld3(x1 + 3 * vi[q]);
-            }
-            t = collision_time(is_ee ? 1 : 0, a, b);
-        }
+        const double t = i < nvg ? cand_toi_vg(sc.voff, x0, x1, vg + 2 * i, P.ground)
+                         : (i < nvg + nee ? cand_toi(1, S, x0, x1, ee + 5 * (i - nvg))
+                                          : cand_toi(0, S, x0, x1, pt + 5 * (i - nvg - nee)));
         if (t < 1.0 && t < acc) acc = t;  // `if t < 1: atomic_min`
     }
     double r = block_min(acc, sh);

@@ -23,6 +23,7 @@
 
 #include "abd_block.h"
 #include "abd_device.h"
+#include "abd_inertia.h"
 #include "abd_internal.h"
 #include "abd_profile.h"
 
@@ -63,29 +64,12 @@ __global__ __launch_bounds__(kBlock) void energy_partial_kernel(AbdScene sc, Abd
     const int64_t n = nee + npt + nvg;
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        double e = 0.0;
-        if (i < nee) {
-            int64_t vi[4];
-            d3 v[4];
-            fetch_ee_idx(S, ee + 5 * i, vi);
-            load4(xk, vi, v);
-            CMap c = cmap_ee(v[0], v[1], v[2], v[3]);
-            double d = signed_distance(c.e0, c.e1p, c.e2p);
-            if (valid_ee(c.beta, c.gamma)) e = barrier(d * d, P.d2hat, P.kappa);
-        } else if (i < nee + npt) {
-            int64_t k = i - nee, vi[4];
-            d3 v[4];
-            fetch_pt_idx(S, pt + 5 * k, vi);
-            load4(xk, vi, v);
-            CMap c = cmap_pt(v[0], v[1], v[2], v[3]);
-            if (valid_pt(c.beta, c.gamma)) e = barrier(dot(c.e2p, c.e2p), P.d2hat, P.kappa);
-        } else {
-            int64_t k = i - nee - npt;
-            int64_t vid = sc.voff[vg[2 * k]] + vg[2 * k + 1];
-            double d = xk[3 * vid + 1] - P.ground;
-            e = barrier(d * d, P.d2hat, P.kappa);
-        }
-        acc += e;
+        if (i < nee)
+            acc += cand_energy_ee(S, xk, ee + 5 * i, P.d2hat, P.kappa);
+        else if (i < nee + npt)
+            acc += cand_energy_pt(S, xk, pt + 5 * (i - nee), P.d2hat, P.kappa);
+        else
+            acc += cand_energy_vg(sc.voff, xk, vg + 2 * (i - nee - npt), P.ground, P.d2hat, P.kappa);
     }
     double r = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
@@ -464,17 +448,6 @@ __global__ __launch_bounds__(kBlock) void pair_gh_kernel(int kind, AbdParams P, 
 }
 
 // -------------------------------------------------------------- inertia
-__device__ __forceinline__ double row_dot(const double* A, int i, int j) {
-    return A[3 * i] * A[3 * j] + A[3 * i + 1] * A[3 * j + 1] + A[3 * i + 2] * A[3 * j + 2];
-}
-// orthogonal_energy.py:129-135
-__device__ __forceinline__ void tilde(const AbdParams& P, const double* A0, const double* Adot, const double* p0,
-                                      const double* pdot, double At[9], double pt[3]) {
-    for (int i = 0; i < 9; ++i) At[i] = A0[i] + P.dt * Adot[i];
-    const double gv[3] = {0.0, P.gravity, 0.0};
-    for (int k = 0; k < 3; ++k) pt[k] = p0[k] + P.dt * pdot[k] + P.dt * P.dt * gv[k];
-}
-
 // orthogonal_energy.py:25-51 (per body), partial sums per workgroup.
 __global__ __launch_bounds__(kBlock) void inertia_energy_kernel(int nb, AbdParams P, const double* __restrict__ mass,
                                                                  const double* __restrict__ I0v,
@@ -487,28 +460,8 @@ __global__ __launch_bounds__(kBlock) void inertia_energy_kernel(int nb, AbdParam
                                                                  double* __restrict__ part) {
     __shared__ double sh[kBlock];
     double acc = 0.0;
-    for (int b = blockIdx.x * kBlock + threadIdx.x; b < nb; b += gridDim.x * kBlock) {
-        double At[9], pt[3];
-        tilde(P, A0 + 9 * b, Adot + 9 * b, p0 + 3 * b, pdot + 3 * b, At, pt);
-        const double* A = Ak + 9 * b;
-        const double m = mass ? mass[b] : P.mass, i0 = I0v ? I0v[b] : P.I0;
-        double dq0[3], dq[9];
-        for (int k = 0; k < 3; ++k) dq0[k] = pk[3 * b + k] - pt[k];
-        for (int k = 0; k < 9; ++k) dq[k] = A[k] - At[k];
-        const double n0 = dq0[0] * dq0[0] + dq0[1] * dq0[1] + dq0[2] * dq0[2];
-        const double n1 = dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2];
-        const double n2 = dq[3] * dq[3] + dq[4] * dq[4] + dq[5] * dq[5];
-        const double n3 = dq[6] * dq[6] + dq[7] * dq[7] + dq[8] * dq[8];
-        const double dqTMdq = n0 * m + (n1 + n2 + n3) * i0;
-        double eo = 0.0;
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
-                double term = row_dot(A, i, j) - (i == j ? 1.0 : 0.0);
-                eo += term * term;
-            }
-        eo = eo * P.stiffness;
-        acc += eo * P.dt * P.dt + 0.5 * dqTMdq;
-    }
+    for (int b = blockIdx.x * kBlock + threadIdx.x; b < nb; b += gridDim.x * kBlock)
+        acc += inertia_energy_body(P, mass, I0v, b, Ak, pk, A0, p0, Adot, pdot);
     double r = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }

@@ -58,8 +58,35 @@ __device__ __forceinline__ bool body_pred(int i, int j, const int32_t* __restric
     return overlap(lo + 3 * i, hi + 3 * i, lo + 3 * j, hi + 3 * j);
 }
 
+// Scene segmentation: pairs enumerated scene by scene (bodies of a scene are
+// contiguous), so the concatenation stays sorted in both orders.
+struct SceneSeg {
+    int n;                 // scenes (<= 1: no segmentation)
+    const int32_t* boff;   // [n+1] body offsets
+    const int64_t* poff;   // [n+1] pair offsets
+};
+
+__device__ __forceinline__ void seg_pair_of(int64_t k, int nb, int order, const SceneSeg& sg, int& i, int& j) {
+    if (sg.n <= 1) {
+        pair_of(k, nb, order, i, j);
+        return;
+    }
+    int lo = 0, hi = sg.n;  // last s with poff[s] <= k
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (sg.poff[mid] <= k)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const int b0 = sg.boff[lo];
+    pair_of(k - sg.poff[lo], sg.boff[lo + 1] - b0, order, i, j);
+    i += b0;
+    j += b0;
+}
+
 template <bool WRITE>
-__global__ __launch_bounds__(kBlock) void body_pairs_kernel(int nb, int64_t npairs, int order,
+__global__ __launch_bounds__(kBlock) void body_pairs_kernel(int nb, int64_t npairs, int order, SceneSeg sg,
                                                              const int32_t* __restrict__ scene,
                                                              const float* __restrict__ lo,
                                                              const float* __restrict__ hi, int64_t* __restrict__ bcnt,
@@ -73,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void body_pairs_kernel(int nb, int64_t npai
         bool pred = false;
         int i = 0, j = 0;
         if (k < npairs) {
-            pair_of(k, nb, order, i, j);
+            seg_pair_of(k, nb, order, sg, i, j);
             pred = body_pred(i, j, scene, lo, hi);
         }
         int tot;
@@ -362,7 +389,12 @@ int abd_cull_bodies(abd_stream_t stream, const AbdScene* sc, const float* lo, co
     if (!sc || !lo || !hi || !count || cap < 0 || (cap > 0 && (!ij || !ji))) return ABD_ERR_ARG;
     hipStream_t s = to_stream(stream);
     const int nb = sc->n_bodies;
-    const int64_t npairs = (int64_t)nb * (nb - 1) / 2;
+    SceneSeg sg{0, nullptr, nullptr};
+    int64_t npairs = (int64_t)nb * (nb - 1) / 2;
+    if (sc->n_scenes > 1 && sc->scene_boff && sc->scene_poff) {
+        sg = SceneSeg{sc->n_scenes, sc->scene_boff, sc->scene_poff};
+        npairs = sc->n_scene_pairs;
+    }
     *count = 0;
     if (npairs <= 0) return ABD_OK;
     const int64_t nblk = ceil_div(npairs, (int64_t)kBlock * kItems);
@@ -373,7 +405,7 @@ int abd_cull_bodies(abd_stream_t stream, const AbdScene* sc, const float* lo, co
     rc = arena_get(kSlotScanB, sizeof(int64_t) * (nblk + 1), (void**)&boff);
     if (rc) return rc;
     int ph = prof_start(kProfBodyPairs, s, 24.0 * nb);
-    hipLaunchKernelGGL(body_pairs_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 0,
+    hipLaunchKernelGGL(body_pairs_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 0, sg,
                        sc->body_scene, lo, hi, bcnt, (const int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s);
@@ -383,15 +415,15 @@ int abd_cull_bodies(abd_stream_t stream, const AbdScene* sc, const float* lo, co
     *count = n;
     if (n > cap) return ABD_ERR_OVERFLOW;
     if (n == 0) return ABD_OK;
-    hipLaunchKernelGGL(body_pairs_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 0,
+    hipLaunchKernelGGL(body_pairs_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 0, sg,
                        sc->body_scene, lo, hi, bcnt, (const int64_t*)boff, ij, cap);
     ABD_CHECK_LAUNCH();
     // (j, i) order: same predicate, second enumeration
-    hipLaunchKernelGGL(body_pairs_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 1,
+    hipLaunchKernelGGL(body_pairs_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 1, sg,
                        sc->body_scene, lo, hi, bcnt, (const int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
     ABD_CHECK_LAUNCH();
     if ((rc = scan_exclusive_i64(s, bcnt, boff, nblk, boff + nblk))) return rc;
-    hipLaunchKernelGGL(body_pairs_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 1,
+    hipLaunchKernelGGL(body_pairs_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 1, sg,
                        sc->body_scene, lo, hi, bcnt, (const int64_t*)boff, ji, cap);
     ABD_CHECK_LAUNCH();
     return ABD_OK;

@@ -351,3 +351,58 @@ __device__ __forceinline__ void fetch_ee_idx(const SceneView& S, const int32_t* 
 }
 
 }  // namespace abd
+
+namespace abd {
+// Per-candidate barrier energy (ipcsk/energy.py:10-48) and TOI (ccd.py:9-40), shared
+// by the whole-list and the per-scene kernels.
+__device__ __forceinline__ double cand_energy_ee(const SceneView& S, const double* __restrict__ xk,
+                                                 const int32_t* t5, double d2hat, double kappa) {
+    int64_t vi[4];
+    fetch_ee_idx(S, t5, vi);
+    d3 v[4];
+    for (int k = 0; k < 4; ++k) v[k] = ld3(xk + 3 * vi[k]);
+    CMap c = cmap_ee(v[0], v[1], v[2], v[3]);
+    double d = signed_distance(c.e0, c.e1p, c.e2p);
+    return valid_ee(c.beta, c.gamma) ? barrier(d * d, d2hat, kappa) : 0.0;
+}
+__device__ __forceinline__ double cand_energy_pt(const SceneView& S, const double* __restrict__ xk,
+                                                 const int32_t* t5, double d2hat, double kappa) {
+    int64_t vi[4];
+    fetch_pt_idx(S, t5, vi);
+    d3 v[4];
+    for (int k = 0; k < 4; ++k) v[k] = ld3(xk + 3 * vi[k]);
+    CMap c = cmap_pt(v[0], v[1], v[2], v[3]);
+    return valid_pt(c.beta, c.gamma) ? barrier(dot(c.e2p, c.e2p), d2hat, kappa) : 0.0;
+}
+__device__ __forceinline__ double cand_energy_vg(const int32_t* __restrict__ voff, const double* __restrict__ xk,
+                                                 const int32_t* t2, double ground, double d2hat, double kappa) {
+    const int64_t vid = voff[t2[0]] + t2[1];
+    const double d = xk[3 * vid + 1] - ground;
+    return barrier(d * d, d2hat, kappa);
+}
+__device__ __forceinline__ double cand_toi(int kind, const SceneView& S, const double* __restrict__ x0,
+                                           const double* __restrict__ x1, const int32_t* t) {
+    int64_t vi[4];
+    if (kind == 0)
+        fetch_pt_idx(S, t, vi);
+    else
+        fetch_ee_idx(S, t, vi);
+    d3 a[4], b[4];
+    for (int q = 0; q < 4; ++q) {
+        a[q] = ld3(x0 + 3 * vi[q]);
+        b[q] = ld3(x1 + 3 * vi[q]);
+    }
+    return collision_time(kind, a, b);
+}
+__device__ __forceinline__ double cand_toi_vg(const int32_t* __restrict__ voff, const double* __restrict__ x0,
+                                              const double* __restrict__ x1, const int32_t* t2, double ground) {
+    const int64_t vid = voff[t2[0]] + t2[1];
+    const double d1 = x1[3 * vid + 1] - ground;
+    double t = 1.0;
+    if (d1 < 0) {
+        const double d0 = x0[3 * vid + 1] - ground;
+        t = d0 / (d0 - d1);
+    }
+    return t;
+}
+}  // namespace abd

@@ -22,7 +22,7 @@ inline hipStream_t to_stream(abd_stream_t s) { return reinterpret_cast<hipStream
 // Per-device scratch arena: `slots` independent growable device buffers plus a
 // small pinned host buffer for count read-back.  Growing synchronizes the
 // device first, so a buffer still in use by queued work is never freed early.
-constexpr int kArenaSlots = 16;
+constexpr int kArenaSlots = 17;
 enum ArenaSlot {
     kSlotScanA = 0,
     kSlotScanB,
@@ -40,6 +40,7 @@ enum ArenaSlot {
     kSlotMisc,
     kSlotSolver,
     kSlotSpare,
+    kSlotSolveBytes,
 };
 int arena_get(int slot, size_t bytes, void** ptr);
 int arena_host(size_t bytes, void** ptr);  // pinned host memory

@@ -30,7 +30,9 @@ bool prof_enabled();
 // Records the start of an instrumented launch; returns a handle (or -1 when off).
 int prof_start(int kernel, hipStream_t s, double bytes);
 // `dev_scale` (nullable): device int32 whose value multiplies `bytes` at query time
-// (iteration counts of the solver).
-void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale = nullptr);
+// (iteration counts); `dev_bytes` (nullable): device double, likewise (byte counts
+// computed on the device, e.g. from per-system iteration counts).  Both are copied
+// on the launch stream, so recording them costs no host synchronization.
+void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale = nullptr, const double* dev_bytes = nullptr);
 
 }  // namespace abd

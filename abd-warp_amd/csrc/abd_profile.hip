@@ -15,10 +15,13 @@ struct Rec {
     hipEvent_t a, b;
     double bytes;
     int scale_slot;  // index into the pinned scale buffer, -1 when unscaled
+    int bytes_slot;  // index into the pinned byte buffer, -1 when unscaled
 };
 constexpr int kScaleSlots = 1 << 16;
 int32_t* g_scale = nullptr;  // pinned host copies of per-launch scale values
+double* g_dbytes = nullptr;  // pinned host copies of per-launch device byte counts
 int g_scale_used = 0;
+int g_bytes_used = 0;
 std::mutex g_mu;
 bool g_on = false;
 std::vector<Rec> g_recs;
@@ -31,7 +34,8 @@ int resolve_locked() {
         ABD_CHECK(hipEventSynchronize(r.b));
         float ms = 0.f;
         ABD_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
-        const double mult = r.scale_slot >= 0 ? (double)g_scale[r.scale_slot] : 1.0;
+        double mult = r.scale_slot >= 0 ? (double)g_scale[r.scale_slot] : 1.0;
+        if (r.bytes_slot >= 0) mult *= g_dbytes[r.bytes_slot];
         g_ms[r.kernel] += ms;
         g_bytes[r.kernel] += r.bytes * mult;
         g_n[r.kernel] += 1;
@@ -40,6 +44,7 @@ int resolve_locked() {
     }
     g_recs.clear();
     g_scale_used = 0;
+    g_bytes_used = 0;
     return ABD_OK;
 }
 }  // namespace
@@ -49,14 +54,14 @@ bool prof_enabled() { return g_on; }
 int prof_start(int kernel, hipStream_t s, double bytes) {
     if (!g_on) return -1;
     std::lock_guard<std::mutex> lock(g_mu);
-    Rec r{kernel, nullptr, nullptr, bytes, -1};
+    Rec r{kernel, nullptr, nullptr, bytes, -1, -1};
     if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
     if (hipEventRecord(r.a, s) != hipSuccess) return -1;
     g_recs.push_back(r);
     return (int)g_recs.size() - 1;
 }
 
-void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale) {
+void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale, const double* dev_bytes) {
     if (handle < 0) return;
     std::lock_guard<std::mutex> lock(g_mu);
     if (handle >= (int)g_recs.size()) return;
@@ -65,6 +70,11 @@ void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale) {
         const int slot = g_scale_used++;
         if (hipMemcpyAsync(&g_scale[slot], dev_scale, sizeof(int32_t), hipMemcpyDeviceToHost, s) == hipSuccess)
             g_recs[handle].scale_slot = slot;
+    }
+    if (dev_bytes && g_dbytes && g_bytes_used < kScaleSlots) {
+        const int slot = g_bytes_used++;
+        if (hipMemcpyAsync(&g_dbytes[slot], dev_bytes, sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess)
+            g_recs[handle].bytes_slot = slot;
     }
     hipEventRecord(g_recs[handle].b, s);
 }
@@ -85,6 +95,8 @@ int abd_profile_begin(void) {
     }
     if (!g_scale && hipHostMalloc((void**)&g_scale, sizeof(int32_t) * kScaleSlots, hipHostMallocDefault) != hipSuccess)
         g_scale = nullptr;
+    if (!g_dbytes && hipHostMalloc((void**)&g_dbytes, sizeof(double) * kScaleSlots, hipHostMallocDefault) != hipSuccess)
+        g_dbytes = nullptr;
     g_on = true;
     return rc;
 }

@@ -64,7 +64,20 @@ struct Sim {
     DevBuf<int32_t> iters;
     double* host = nullptr;  // pinned scalars
     int64_t stats[10] = {};
+    // independent scenes (n_scenes > 1): per-scene device values and pinned host copies
+    int ns = 1, max_scene_bodies = 0;
+    DevBuf<double> sv;        // [kSv][ns]
+    DevBuf<uint8_t> sactive;  // solver mask [ns]
+    double* hsv = nullptr;    // pinned [kSv][ns]
+    double* halpha = nullptr;   // pinned step lengths of the line search [ns]
+    double* hccd = nullptr;     // pinned step lengths of the CCD sweep [ns]
+    uint8_t* hmask = nullptr;   // pinned solver mask [ns]
+    int32_t* hiters = nullptr;  // pinned solver iterations [ns]
+    std::vector<int32_t> scene_it;  // Newton iterations of each scene in the last frame
 };
+
+// per-scene value rows of Sim::sv / Sim::hsv
+enum { kSvEc = 0, kSvEi, kSvToi, kSvQtg, kSvGtg, kSvAlpha, kSvAlphaCcd, kSv };
 
 int get_scalars(hipStream_t s, Sim& S, int first, int count) {
     ABD_CHECK(hipMemcpyAsync(S.host + first, S.scal.p + first, sizeof(double) * count, hipMemcpyDeviceToHost, s));
@@ -266,6 +279,152 @@ int step(hipStream_t s, Sim& S) {
     if ((rc = abd_project_vertices(s, &S.sc, st.A0, st.p0, st.x_view))) return rc;
     S.stats[0] += it;
     S.stats[7] += 1;
+    S.scene_it.assign(1, it);
+    return ABD_OK;
+}
+
+// ---------------------------------------------------------------- batched
+// Many independent scenes advanced together: one launch per stage covers every
+// scene, and each scene keeps its own Newton loop, step length and stopping
+// test, exactly the per-scene sequence of step() above.  A scene that has
+// converged is masked out of the solve and takes zero-length steps.
+
+int upload(hipStream_t s, void* dst, const void* src, size_t bytes) {
+    ABD_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return ABD_OK;
+}
+
+// per-scene energies at q = q_base - alpha_s dq (abd.py:133-141); hsv rows Ec, Ei
+int compute_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
+    int rc;
+    const AbdSimState& st = S.st;
+    const int ns = S.ns;
+    if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha, st.Ak, st.pk))) return rc;
+    if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
+    if ((rc = collision_set(s, S, false))) return rc;
+    if ((rc = abd_scene_contact_energy(s, &S.sc, &S.P, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
+                                       S.sv.p + kSvEc * ns)))
+        return rc;
+    if ((rc = abd_scene_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot,
+                                       S.sv.p + kSvEi * ns)))
+        return rc;
+    ABD_CHECK(hipMemcpyAsync(S.hsv + kSvEc * ns, S.sv.p + kSvEc * ns, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost,
+                             s));
+    ABD_CHECK(hipStreamSynchronize(s));
+    S.stats[5] += 1;
+    return ABD_OK;
+}
+
+int step_scenes(hipStream_t s, Sim& S) {
+    const AbdSimState& st = S.st;
+    const AbdParams& P = S.P;
+    const int nb = S.nb, ns = S.ns;
+    int rc;
+    double* dalpha = S.sv.p + kSvAlpha * ns;
+    double* dccd = S.sv.p + kSvAlphaCcd * ns;
+    ABD_CHECK(hipMemcpyAsync(st.A, st.A0, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+    ABD_CHECK(hipMemcpyAsync(st.p, st.p0, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+    ABD_CHECK(hipMemcpyAsync(st.Ak, st.A0, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+    ABD_CHECK(hipMemcpyAsync(st.pk, st.p0, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+    std::vector<double> E0(ns), E1(ns);
+    std::vector<uint8_t> newton(ns, 1), ls(ns);
+    std::vector<int32_t>& it = S.scene_it;
+    it.assign(ns, 0);
+    for (int k = 0; k < ns; ++k) S.halpha[k] = 0.0;
+    if ((rc = upload(s, dalpha, S.halpha, sizeof(double) * ns))) return rc;
+    if ((rc = compute_energy_scenes(s, S, dalpha))) return rc;
+    for (int k = 0; k < ns; ++k) E0[k] = S.hsv[kSvEc * ns + k] + S.hsv[kSvEi * ns + k];
+    int n_active = ns;
+    while (n_active > 0) {
+        if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
+        if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
+        if ((rc = collision_set(s, S, true))) return rc;
+        const size_t nslot = nb + 2 * (size_t)S.nij;
+        if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
+        ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
+        ABD_CHECK(hipMemsetAsync(st.g, 0, sizeof(double) * 12 * nb, s));
+        if ((rc = abd_inertia_gradient(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g))) return rc;
+        if ((rc = abd_inertia_hessian(s, &S.sc, &P, st.A, S.blocks.p))) return rc;
+        int64_t nact = 0;
+        if ((rc = abd_contact_gh(s, &S.sc, &P, st.x, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.nij,
+                                 S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, &nact)))
+            return rc;
+        S.stats[1] += S.npt;
+        S.stats[2] += S.nee;
+        S.stats[3] += S.nvg;
+        S.stats[4] += nact;
+        for (int k = 0; k < ns; ++k) {
+            S.hmask[k] = newton[k];
+            S.hccd[k] = newton[k] ? 1.0 : 0.0;
+        }
+        if ((rc = upload(s, S.sactive.p, S.hmask, ns))) return rc;
+        if ((rc = abd_bsr_solve_scenes(s, P.solver, &S.sc, S.max_scene_bodies, S.row_ptr.p, S.cols.p, S.slots.p,
+                                       S.blocks.p, st.g, st.dq, P.solver_tol, P.solver_maxiter, 10, S.sactive.p,
+                                       S.iters.p, S.scal.p + 8)))
+            return rc;
+        // CCD over the full steps of the scenes still iterating (abd.py:278-303)
+        if ((rc = upload(s, dccd, S.hccd, sizeof(double) * ns))) return rc;
+        if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dccd, st.Ak, st.pk))) return rc;
+        if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
+        if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
+        if ((rc = trajectory_set(s, S))) return rc;
+        S.stats[8] += S.npt;
+        S.stats[9] += S.nee;
+        if ((rc = abd_scene_toi(s, &S.sc, &P, st.x, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
+                                S.sv.p + kSvToi * ns)))
+            return rc;
+        if ((rc = abd_scene_dot(s, &S.sc, st.dq, st.g, S.sv.p + kSvQtg * ns))) return rc;
+        ABD_CHECK(hipMemcpyAsync(S.hsv + kSvToi * ns, S.sv.p + kSvToi * ns, sizeof(double) * 2 * ns,
+                                 hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipStreamSynchronize(s));
+        for (int k = 0; k < ns; ++k) {
+            ls[k] = newton[k];
+            double t = S.hsv[kSvToi * ns + k];
+            t = std::max(0.0, std::min(1.0, t));
+            if (t < 1.0) t *= 0.9;
+            S.halpha[k] = newton[k] ? t : 0.0;
+        }
+        // line search, each scene on its own (abd.py:146-160)
+        for (int n_ls = n_active; n_ls > 0;) {
+            if ((rc = upload(s, dalpha, S.halpha, sizeof(double) * ns))) return rc;
+            if ((rc = compute_energy_scenes(s, S, dalpha))) return rc;
+            for (int k = 0; k < ns; ++k) {
+                if (!ls[k]) continue;
+                const double e1 = S.hsv[kSvEc * ns + k] + S.hsv[kSvEi * ns + k];
+                const double qTg = S.hsv[kSvQtg * ns + k];
+                const bool wolfe = e1 < E0[k] + P.c1 * S.halpha[k] * qTg;
+                if (wolfe || S.halpha[k] < P.alpha_min) {
+                    ls[k] = 0;
+                    E1[k] = e1;
+                    --n_ls;
+                } else {
+                    S.halpha[k] *= 0.5;
+                }
+            }
+        }
+        // halpha holds the accepted step of every iterating scene and 0 elsewhere
+        if ((rc = upload(s, dalpha, S.halpha, sizeof(double) * ns))) return rc;
+        if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha, st.A, st.p))) return rc;
+        if ((rc = abd_scene_dot(s, &S.sc, st.g, st.g, S.sv.p + kSvGtg * ns))) return rc;
+        ABD_CHECK(hipMemcpyAsync(S.hiters, S.iters.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipMemcpyAsync(S.hsv + kSvGtg * ns, S.sv.p + kSvGtg * ns, sizeof(double) * ns,
+                                 hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipStreamSynchronize(s));
+        for (int k = 0; k < ns; ++k) {
+            if (!newton[k]) continue;
+            E0[k] = E1[k];
+            ++it[k];
+            S.stats[0] += 1;
+            S.stats[6] += S.hiters[k] > 0 ? S.hiters[k] : 0;
+            if (S.hsv[kSvGtg * ns + k] < P.tol || it[k] >= P.max_iter) {
+                newton[k] = 0;
+                --n_active;
+            }
+        }
+    }
+    if ((rc = abd_update_q0qdot(s, nb, P.dt, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot))) return rc;
+    if ((rc = abd_project_vertices(s, &S.sc, st.A0, st.p0, st.x_view))) return rc;
+    S.stats[7] += 1;
     return ABD_OK;
 }
 
@@ -311,8 +470,32 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
     if (!rc) rc = S->scal.reserve(8);
     if (!rc) rc = S->iters.reserve(4);
     if (!rc && hipHostMalloc((void**)&S->host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = ABD_ERR_ARG;
+    if (!rc && scene->n_scenes > 1) {
+        // scene ranges from the host copy of scene_boff
+        const int ns = scene->n_scenes;
+        std::vector<int32_t> sb(ns + 1);
+        if (!scene->scene_boff || !scene->body_scene ||
+            hipMemcpy(sb.data(), scene->scene_boff, sizeof(int32_t) * (ns + 1), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = ABD_ERR_ARG;
+        for (int k = 0; !rc && k < ns; ++k) {
+            if (sb[k + 1] < sb[k]) rc = ABD_ERR_ARG;
+            S->max_scene_bodies = std::max(S->max_scene_bodies, sb[k + 1] - sb[k]);
+        }
+        if (!rc && (sb[0] != 0 || sb[ns] != scene->n_bodies)) rc = ABD_ERR_ARG;
+        S->ns = ns;
+        if (!rc) rc = S->sv.reserve((size_t)kSv * ns);
+        if (!rc) rc = S->sactive.reserve(ns);
+        if (!rc) rc = S->iters.reserve(ns);
+        if (!rc) rc = S->scal.reserve(8 + (size_t)ns);
+        if (!rc && (hipHostMalloc((void**)&S->hsv, sizeof(double) * kSv * ns, hipHostMallocDefault) != hipSuccess ||
+                    hipHostMalloc((void**)&S->halpha, sizeof(double) * ns, hipHostMallocDefault) != hipSuccess ||
+                    hipHostMalloc((void**)&S->hccd, sizeof(double) * ns, hipHostMallocDefault) != hipSuccess ||
+                    hipHostMalloc((void**)&S->hmask, ns, hipHostMallocDefault) != hipSuccess ||
+                    hipHostMalloc((void**)&S->hiters, sizeof(int32_t) * ns, hipHostMallocDefault) != hipSuccess))
+            rc = ABD_ERR_ARG;
+    }
     if (rc) {
-        delete S;
+        abd_sim_destroy(S);
         return rc;
     }
     *out = S;
@@ -322,10 +505,17 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
 int abd_sim_step(void* sim, abd_stream_t stream, int64_t* stats) {
     if (!sim) return ABD_ERR_ARG;
     Sim& S = *static_cast<Sim*>(sim);
-    int rc = step(to_stream(stream), S);
+    int rc = S.ns > 1 ? step_scenes(to_stream(stream), S) : step(to_stream(stream), S);
     if (stats)
         for (int k = 0; k < 10; ++k) stats[k] = S.stats[k];
     return rc;
+}
+
+int abd_sim_scene_iters(void* sim, int32_t* iters) {
+    if (!sim || !iters) return ABD_ERR_ARG;
+    const Sim& S = *static_cast<Sim*>(sim);
+    for (int k = 0; k < S.ns; ++k) iters[k] = k < (int)S.scene_it.size() ? S.scene_it[k] : 0;
+    return ABD_OK;
 }
 
 int abd_sim_destroy(void* sim) {
@@ -347,7 +537,11 @@ int abd_sim_destroy(void* sim) {
     S->blocks.release();
     S->scal.release();
     S->iters.release();
-    if (S->host) hipHostFree(S->host);
+    S->sv.release();
+    S->sactive.release();
+    for (void* h : {(void*)S->host, (void*)S->hsv, (void*)S->halpha, (void*)S->hccd, (void*)S->hmask,
+                    (void*)S->hiters})
+        if (h) hipHostFree(h);
     delete S;
     return ABD_OK;
 }

@@ -27,14 +27,18 @@ struct Bsr {
     const double* blocks;
 };
 
-// out = H in; row idx of body b = idx / 12, sub-row (ii, k) = ((idx % 12) / 3, idx % 3)
-__device__ __forceinline__ void spmv(const Bsr& M, const double* __restrict__ in, double* __restrict__ out, int n) {
+// out = H in over the rows of bodies row0 .. row0 + n/12 (in / out indexed from
+// row0); row idx of body row0 + idx / 12, sub-row (ii, k) = ((idx % 12) / 3, idx % 3)
+__device__ __forceinline__ void spmv(const Bsr& M, int row0, const double* __restrict__ in,
+                                     double* __restrict__ out, int n) {
     for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-        const int b = idx / 12, lr = idx % 12, ii = lr / 3, k = lr % 3;
+        const int b = row0 + idx / 12, lr = idx % 12, ii = lr / 3, k = lr % 3;
         double acc = 0.0;
         for (int e = M.row_ptr[b]; e < M.row_ptr[b + 1]; ++e) {
+            const int lc = M.cols[e] - row0;
+            if ((unsigned)lc >= (unsigned)(n / 12)) continue;  // a block coupling two scenes is not part of either
             const int64_t s = M.slots[e];
-            const double* xin = in + 12 * (int64_t)M.cols[e];
+            const double* xin = in + 12 * (int64_t)lc;
             const double* base = M.blocks + 9 * 16 * s + 3 * k;
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj) {
@@ -62,14 +66,29 @@ __device__ __forceinline__ double* vec_base(double* ws) {
 }
 
 template <bool LDS>
-__global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const double* __restrict__ b,
+__global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const int32_t* __restrict__ sboff,
+                                                                  const uint8_t* __restrict__ active,
+                                                                  const double* __restrict__ b,
                                                                   double* __restrict__ xg, double tol, int maxiter,
-                                                                  int check_every, double* __restrict__ ws,
+                                                                  int check_every, int cap_rows, double* __restrict__ ws,
                                                                   int32_t* __restrict__ iters,
                                                                   double* __restrict__ resid) {
     __shared__ double sh[17];
-    const int n = 12 * M.nb;
-    double* V = vec_base<LDS>(ws);
+    // one system per workgroup: the bodies of scene blockIdx.x (all bodies without scenes)
+    const int sidx = blockIdx.x;
+    if (active && !active[sidx]) return;
+    const int row0 = sboff ? sboff[sidx] : 0;
+    const int n = 12 * ((sboff ? sboff[sidx + 1] : M.nb) - row0);
+    b += 12 * (int64_t)row0;
+    xg += 12 * (int64_t)row0;
+    if (iters) iters += sidx;
+    if (resid) resid += sidx;
+    if (n > cap_rows) {  // larger than the vector storage sized by the caller: report, touch nothing
+        if (threadIdx.x == 0 && iters) *iters = -1;
+        return;
+    }
+    double* V = vec_base<LDS>(ws + (LDS ? 0 : 6 * 12 * (int64_t)row0));
+    if (maxiter <= 0) maxiter = n;  // the system size of this workgroup's system
     double* x = V;
     double* r = V + n;
     double* r0 = V + 2 * n;
@@ -78,7 +97,7 @@ __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const do
     double* t = V + 5 * n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = xg[i];
     __syncthreads();
-    spmv(M, x, t, n);
+    spmv(M, row0, x, t, n);
     for (int i = threadIdx.x; i < n; i += blockDim.x) r[i] = b[i] - t[i];
     __syncthreads();
     double rr = vdot(r, r, n, sh);
@@ -95,7 +114,7 @@ __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const do
         double rho = rr;
         for (;;) {
             for (int c = 0; c < check_every; ++c) {
-                spmv(M, p, v, n);
+                spmv(M, row0, p, v, n);
                 const double r0v = vdot(r0, v, n, sh);
                 const double alpha = r0v != 0.0 ? rho / r0v : 0.0;
                 for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -103,7 +122,7 @@ __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const do
                     r[i] -= alpha * v[i];
                 }
                 __syncthreads();
-                spmv(M, r, t, n);
+                spmv(M, row0, r, t, n);
                 const double tr = vdot(t, r, n, sh);
                 const double tt = vdot(t, t, n, sh);
                 const double omega = tt != 0.0 ? tr / tt : 0.0;
@@ -141,14 +160,29 @@ __device__ __forceinline__ void jacobi(const double* __restrict__ dg, const doub
 }
 
 template <bool LDS>
-__global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const double* __restrict__ b,
+__global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t* __restrict__ sboff,
+                                                                  const uint8_t* __restrict__ active,
+                                                                  const double* __restrict__ b,
                                                             double* __restrict__ xg, double tol, int maxiter,
-                                                            int check_every, double* __restrict__ ws,
+                                                            int check_every, int cap_rows, double* __restrict__ ws,
                                                             int32_t* __restrict__ iters,
                                                             double* __restrict__ resid) {
     __shared__ double sh[17];
-    const int n = 12 * M.nb;
-    double* V = vec_base<LDS>(ws);
+    // one system per workgroup: the bodies of scene blockIdx.x (all bodies without scenes)
+    const int sidx = blockIdx.x;
+    if (active && !active[sidx]) return;
+    const int row0 = sboff ? sboff[sidx] : 0;
+    const int n = 12 * ((sboff ? sboff[sidx + 1] : M.nb) - row0);
+    b += 12 * (int64_t)row0;
+    xg += 12 * (int64_t)row0;
+    if (iters) iters += sidx;
+    if (resid) resid += sidx;
+    if (n > cap_rows) {  // larger than the vector storage sized by the caller: report, touch nothing
+        if (threadIdx.x == 0 && iters) *iters = -1;
+        return;
+    }
+    double* V = vec_base<LDS>(ws + (LDS ? 0 : 6 * 12 * (int64_t)row0));
+    if (maxiter <= 0) maxiter = n;  // the system size of this workgroup's system
     double* x = V;
     double* r = V + n;
     double* p = V + 2 * n;
@@ -157,12 +191,12 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const double* 
     double* dg = V + 5 * n;
     // bsr_get_diag: diagonal of the diagonal 3x3 blocks (sub-block (ii, ii) of slot b)
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int bb = i / 12, ii = (i % 12) / 3, a = i % 3;
+        const int bb = row0 + i / 12, ii = (i % 12) / 3, a = i % 3;
         dg[i] = M.blocks[9 * (16 * (int64_t)bb + ii + 4 * ii) + 4 * a];
         x[i] = xg[i];
     }
     __syncthreads();
-    spmv(M, x, Ap, n);
+    spmv(M, row0, x, Ap, n);
     for (int i = threadIdx.x; i < n; i += blockDim.x) r[i] = b[i] - Ap[i];
     __syncthreads();
     jacobi(dg, r, z, n);
@@ -175,7 +209,7 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const double* 
     if (err > tol_sq) {
         end_iter = maxiter;
         for (int i = 0; i < maxiter; ++i) {
-            spmv(M, p, Ap, n);
+            spmv(M, row0, p, Ap, n);
             const double pAp = vdot(p, Ap, n, sh);
             const double alpha = rz_old / pAp;
             for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -206,31 +240,42 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const double* 
     }
 }
 
+// Algorithmic bytes of one solve launch, summed over its systems: per iteration
+// each SpMV streams the system's blocks (144 doubles), column and slot ids, and
+// row pointers; with vectors outside LDS, ~10 vector passes of 12 n doubles too.
+__global__ void solve_bytes_kernel(const int32_t* __restrict__ row_ptr, int nb, const int32_t* __restrict__ sboff,
+                                   int nsys, const int32_t* __restrict__ iters, int spmv_per_it, int lds,
+                                   double* __restrict__ out) {
+    __shared__ double sh[17];
+    double acc = 0.0;
+    for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
+        const int r0 = sboff ? sboff[s] : 0, r1 = sboff ? sboff[s + 1] : nb;
+        const double nnzb = (double)(row_ptr[r1] - row_ptr[r0]);
+        const double nbs = (double)(r1 - r0);
+        const double per_it = spmv_per_it * (1160.0 * nnzb + 4.0 * (nbs + 1)) + (lds ? 0.0 : 10.0 * 96.0 * nbs);
+        const int it = iters[s] > 0 ? iters[s] : 0;
+        acc += per_it * it;
+    }
+    const double r = block_sum_fast(acc, sh);
+    if (threadIdx.x == 0) *out = r;
+}
+
 }  // namespace abd
 
 using namespace abd;
 
 extern "C" {
 
-int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t* row_ptr, const int32_t* cols,
-                  const int32_t* slots, const double* blocks, const double* b, double* x, double tol, int32_t maxiter,
-                  int32_t check_every, int32_t* iters, double* resid) {
-    if (nb <= 0 || !row_ptr || !cols || !slots || !blocks || !b || !x || check_every <= 0) return ABD_ERR_ARG;
-    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI) return ABD_ERR_ARG;
-    const int n = 12 * nb;
-    if (maxiter <= 0) maxiter = n;
+namespace {
+int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const int32_t* sboff, const uint8_t* active,
+                 int max_rows, const double* b, double* x, double tol, int32_t maxiter, int32_t check_every,
+                 int32_t* iters, double* resid, int threads) {
+    const int n_all = 12 * M.nb;
     double* ws;
-    int rc = arena_get(kSlotSolver, sizeof(double) * 6 * (size_t)n, (void**)&ws);
+    int rc = arena_get(kSlotSolver, sizeof(double) * 6 * (size_t)n_all, (void**)&ws);
     if (rc) return rc;
-    Bsr M{nb, row_ptr, cols, slots, blocks};
-    hipStream_t s = to_stream(stream);
-    // per iteration: two block SpMVs (nnzb 12x12 blocks + column ids + row pointers + in/out
-    // vectors) and ~10 vector passes; scaled by the iteration count read back at query time
-    int32_t nnzb = 0;
-    if (prof_enabled()) ABD_CHECK(hipMemcpy(&nnzb, row_ptr + nb, sizeof(int32_t), hipMemcpyDeviceToHost));
-    const double per_it = 2.0 * (1156.0 * nnzb + 4.0 * (nb + 1) + 2.0 * 96 * nb) + 10.0 * 96 * nb;
-    int ph = prof_start(kProfSolve, s, per_it);
-    const size_t vec_bytes = sizeof(double) * 6 * (size_t)n;
+    int ph = prof_start(kProfSolve, s, 1.0);
+    const size_t vec_bytes = sizeof(double) * 6 * (size_t)max_rows;
     const bool lds = vec_bytes <= kLdsVecBytes;
     if (lds) {
         static bool attr_set = false;
@@ -242,22 +287,58 @@ int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t
             attr_set = true;
         }
         if (method == ABD_SOLVER_BICGSTAB)
-            hipLaunchKernelGGL(bicgstab_kernel<true>, dim3(1), dim3(kSolveThreads), vec_bytes, s, M, b, x, tol,
-                               maxiter, check_every, ws, iters, resid);
+            hipLaunchKernelGGL(bicgstab_kernel<true>, dim3(nsys), dim3(threads), vec_bytes, s, M, sboff, active, b,
+                               x, tol, maxiter, check_every, max_rows, ws, iters, resid);
         else
-            hipLaunchKernelGGL(cg_kernel<true>, dim3(1), dim3(kSolveThreads), vec_bytes, s, M, b, x, tol, maxiter,
-                               check_every, ws, iters, resid);
+            hipLaunchKernelGGL(cg_kernel<true>, dim3(nsys), dim3(threads), vec_bytes, s, M, sboff, active, b, x,
+                               tol, maxiter, check_every, max_rows, ws, iters, resid);
     } else {
         if (method == ABD_SOLVER_BICGSTAB)
-            hipLaunchKernelGGL(bicgstab_kernel<false>, dim3(1), dim3(kSolveThreads), 0, s, M, b, x, tol, maxiter,
-                               check_every, ws, iters, resid);
+            hipLaunchKernelGGL(bicgstab_kernel<false>, dim3(nsys), dim3(threads), 0, s, M, sboff, active, b, x, tol,
+                               maxiter, check_every, max_rows, ws, iters, resid);
         else
-            hipLaunchKernelGGL(cg_kernel<false>, dim3(1), dim3(kSolveThreads), 0, s, M, b, x, tol, maxiter,
-                               check_every, ws, iters, resid);
+            hipLaunchKernelGGL(cg_kernel<false>, dim3(nsys), dim3(threads), 0, s, M, sboff, active, b, x, tol,
+                               maxiter, check_every, max_rows, ws, iters, resid);
     }
     ABD_CHECK_LAUNCH();
-    prof_stop(ph, s, iters);
+    if (ph >= 0 && iters) {
+        // the byte count needs the iteration counts: computed behind the solve, on its stream
+        double* dbytes;
+        if ((rc = arena_get(kSlotSolveBytes, sizeof(double), (void**)&dbytes))) return rc;
+        hipLaunchKernelGGL(solve_bytes_kernel, dim3(1), dim3(256), 0, s, M.row_ptr, M.nb, sboff, nsys, iters,
+                           method == ABD_SOLVER_BICGSTAB ? 2 : 1, lds ? 1 : 0, dbytes);
+        ABD_CHECK_LAUNCH();
+        prof_stop(ph, s, nullptr, dbytes);
+    } else {
+        prof_stop(ph, s);
+    }
     return ABD_OK;
+}
+}  // namespace
+
+int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t* row_ptr, const int32_t* cols,
+                  const int32_t* slots, const double* blocks, const double* b, double* x, double tol, int32_t maxiter,
+                  int32_t check_every, int32_t* iters, double* resid) {
+    if (nb <= 0 || !row_ptr || !cols || !slots || !blocks || !b || !x || check_every <= 0) return ABD_ERR_ARG;
+    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI) return ABD_ERR_ARG;
+    Bsr M{nb, row_ptr, cols, slots, blocks};
+    return launch_solve(to_stream(stream), method, M, 1, nullptr, nullptr, 12 * nb, b, x, tol, maxiter, check_every,
+                        iters, resid, kSolveThreads);
+}
+
+int abd_bsr_solve_scenes(abd_stream_t stream, int32_t method, const AbdScene* sc, int32_t max_scene_bodies,
+                         const int32_t* row_ptr, const int32_t* cols, const int32_t* slots, const double* blocks,
+                         const double* b, double* x, double tol, int32_t maxiter, int32_t check_every,
+                         const uint8_t* active, int32_t* iters, double* resid) {
+    if (!sc || sc->n_scenes <= 0 || !sc->scene_boff || max_scene_bodies <= 0 || !row_ptr || !cols || !slots ||
+        !blocks || !b || !x || check_every <= 0)
+        return ABD_ERR_ARG;
+    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI) return ABD_ERR_ARG;
+    Bsr M{sc->n_bodies, row_ptr, cols, slots, blocks};
+    // a scene system has at most 12 * 64 = 768 rows in the batched workloads: 256 threads
+    const int threads = 12 * max_scene_bodies > 1536 ? kSolveThreads : 256;
+    return launch_solve(to_stream(stream), method, M, sc->n_scenes, sc->scene_boff, active, 12 * max_scene_bodies, b,
+                        x, tol, maxiter, check_every, iters, resid, threads);
 }
 
 }  // extern "C"

@@ -44,31 +44,75 @@ def make_workload(name: str):
         return scenes.config_medium(), "medium: 64 icosphere-3 bodies (642 V / 1280 F / 1920 E), 4x4x4 grid, floor"
     if name == "small":
         return scenes.config_small(), "small: 2 unit cubes (8 V / 12 F / 18 E), B at origin, A at 1.05"
+    if name == "batched":
+        spec = scenes.config_batched()
+        return spec, (f"batched: 256 independent scenes, U{{2..64}} bodies each ({spec.n_bodies} bodies, "
+                      f"{spec.n_verts} V / {spec.n_tris} F in all; cube, icosphere-2 or 6x6x6 box), stacked, "
+                      "speed U[0,2], floor")
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(spec, cp, warmup: int, steps: int, budget_s: float):
-    """The oracle (C FP64 restatement, 1 thread) on the same frames as the timed
-    GPU region, stopped after `budget_s` seconds of timed CPU work."""
-    from oracle import oracle as O
-    ref = O.OracleSim(spec, cp)
-    for _ in range(warmup):
-        ref.step()
-    it0 = int(ref.stats[0])
-    t_cpu = 0.0
-    frames = 0
-    for _ in range(steps):
+GH_KERNELS = ("gh_classify", "gh_contact", "gh_pair_reduce", "gh_body_finalize")
+
+
+def plain_per_scene(spec, cp, sample, warmup: int, steps: int):
+    """The plain (one scene per simulator) GPU path on sampled scenes of a batch:
+    Newton iterations per second of wall time over the same frames as the timed
+    batched region."""
+    import torch
+    from abd_hip import scenes
+    from abd_hip.abd import AffineBodySimulator
+    iters, wall = 0, 0.0
+    for s in sample:
+        sim = AffineBodySimulator(scenes.scene_of(spec, s), cp, "cuda", native=True)
+        for f in range(warmup):
+            sim.step(f)
+        torch.cuda.synchronize()
         t = time.perf_counter()
-        ref.step()
-        t_cpu += time.perf_counter() - t
-        frames += 1
-        log(f"[cpu] frame {warmup + frames - 1} newton {int(ref.stats[0]) - it0} t {t_cpu:.1f}s")
-        if t_cpu >= budget_s:
-            break
-    iters = int(ref.stats[0]) - it0
+        n = 0
+        for f in range(steps):
+            n += sim.step(warmup + f)
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - t
+        iters += n
+        log(f"[plain] scene {s}: {n} Newton iterations")
+        del sim
+    return iters, wall
+
+
+def cpu_baseline(spec, cp, warmup: int, steps: int, budget_s: float, sample=None):
+    """The oracle (C FP64 restatement, 1 thread) on the same frames as the timed
+    GPU region, stopped after `budget_s` seconds of timed CPU work.  With `sample`
+    (scene ids of a batch), the sampled scenes run one after another, each from
+    its own frame 0, sharing the budget."""
+    from oracle import oracle as O
+    from abd_hip import scenes
+    specs = [spec] if sample is None else [scenes.scene_of(spec, s) for s in sample]
+    iters, t_cpu, frames, parts = 0, 0.0, 0, []
+    share = budget_s / len(specs)
+    for k, sp in enumerate(specs):
+        ref = O.OracleSim(sp, cp)
+        for _ in range(warmup):
+            ref.step()
+        it0 = int(ref.stats[0])
+        t_here, f_here = 0.0, 0
+        for _ in range(steps):
+            t = time.perf_counter()
+            ref.step()
+            t_here += time.perf_counter() - t
+            f_here += 1
+            if t_here >= share:
+                break
+        n = int(ref.stats[0]) - it0
+        log(f"[cpu] part {k}: {f_here} frames, {n} Newton iterations, {t_here:.2f}s")
+        iters += n
+        t_cpu += t_here
+        frames += f_here
+        parts.append(f"{'scene ' + str(sample[k]) + ': ' if sample is not None else ''}frames {warmup}.."
+                     f"{warmup + f_here - 1}, {n} it")
     return dict(value=iters / t_cpu if t_cpu > 0 else None, unit="Newton iters/s", cores=1, kind="port",
-                sample=f"oracle (C FP64 restatement, -O2, 1 thread) frames {warmup}..{warmup + frames - 1} "
-                       f"of the same scene and mode: {iters} Newton iterations in {t_cpu:.2f} s")
+                sample=f"oracle (C FP64 restatement, -O2, 1 thread), same mode: " + "; ".join(parts) +
+                       f"; {iters} Newton iterations in {t_cpu:.2f} s")
 
 
 def main():
@@ -81,6 +125,7 @@ def main():
     ap.add_argument("--solver", type=int, default=1, help="0 BiCGSTAB, 1 Jacobi CG")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-sample", type=int, default=8, help="batched: scenes timed on the plain path")
     ap.add_argument("--python-driver", action="store_true", help="run the Python mirror of the driver")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -103,7 +148,6 @@ def main():
         log(f"[gpu] warmup frame {f} newton {it}")
     torch.cuda.synchronize()
     s0 = dict(sim.stats)
-    sim.profile_gh = True
     _lib.profile_begin()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -114,11 +158,12 @@ def main():
     t1 = time.perf_counter()
     _lib.profile_end()
     prof = _lib.profile_query()
-    gh_s = sim.gh_seconds()
     d = {k: sim.stats[k] - s0[k] for k in sim.stats}
     wall = t1 - t0
     iters = d["newton_iters"]
     cands = d["gh_pt"] + d["gh_ee"]
+    # gh time from the profiled gh kernels (both drivers launch the same four)
+    gh_s = sum(prof[k][0] for k in GH_KERNELS if k in prof) * 1e-3
     # dominant kernel with an algorithmic byte count
     ranked = sorted(((v[0], k) for k, v in prof.items() if v[1] > 0 and v[2] > 0), reverse=True)
     dom = ranked[0][1] if ranked else None
@@ -129,9 +174,19 @@ def main():
         roof = dict(kernel=dom, bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=achieved / HBM_PEAK_GBS, traffic=None, launches=n, avg_us=round(1e3 * ms / n, 3),
                     bytes_per_launch=byts / n)
+    batched = spec.n_scenes > 1
+    plain = None
+    sample = None
+    if batched:
+        ns = spec.n_scenes
+        sample = sorted(set(int(round(k * (ns - 1) / (a.plain_sample - 1))) for k in range(a.plain_sample)))
+        p_it, p_wall = plain_per_scene(spec, cp, sample, a.warmup, a.steps)
+        plain = dict(value=p_it / p_wall if p_wall > 0 else None, unit="Newton iters/s",
+                     sample=f"plain native driver, one scene at a time, scenes {sample}, frames "
+                            f"{a.warmup}..{a.warmup + a.steps - 1}: {p_it} Newton iterations in {p_wall:.3f} s")
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(spec, cp, a.warmup, a.steps, a.cpu_budget)
+        cpu = cpu_baseline(spec, cp, a.warmup, a.steps, a.cpu_budget, sample)
     out = {
         "metric": "Newton iters/s + barrier-Hessian pairs/s, batched vs plain, 1 MI355X",
         "value": iters / wall,
@@ -145,13 +200,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": desc, "mode": "plain", "psd_projection": bool(a.psd),
+        "config": {"workload": desc, "mode": "batched" if batched else "plain", "psd_projection": bool(a.psd),
                    "driver": "python" if a.python_driver else "native",
                    "solver": "jacobi-cg" if a.solver == 1 else "bicgstab", "frames_timed": a.steps},
         "gh_pairs_per_s": cands / gh_s if gh_s > 0 else None,
         "gh_candidates": cands,
         "gh_active": d["gh_active"],
         "newton_iters": iters,
+        "plain_per_scene": plain,
+        "batched_over_plain": (iters / wall) / plain["value"] if plain and plain["value"] else None,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms": {k: round(v[0], 3) for k, v in prof.items() if v[1]},

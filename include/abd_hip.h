@@ -90,6 +90,14 @@ typedef struct AbdScene {
     const int32_t* body_scene;               /* [Nb] scene id; bodies of different scenes never pair */
     const double* mass;                      /* [Nb] (NULL: params.mass) */
     const double* I0;                        /* [Nb] (NULL: params.I0) */
+    /* Independent scenes (batching; no reference counterpart): the bodies of scene s are
+     * scene_boff[s]..scene_boff[s+1] (contiguous); scene_poff[s] = number of body pairs
+     * i < j inside scenes before s; n_scene_pairs = scene_poff[n_scenes] (host copy).
+     * n_scenes <= 1 or NULL offsets: one scene (all pairs enumerated). */
+    int32_t n_scenes;
+    const int32_t* scene_boff;               /* [S+1] */
+    const int64_t* scene_poff;               /* [S+1] */
+    int64_t n_scene_pairs;
 } AbdScene;
 
 /* ---- library ---- */
@@ -220,6 +228,31 @@ int abd_cubic_roots(abd_stream_t stream, const double* coef, int64_t n, double x
 int abd_collision_time(abd_stream_t stream, int32_t kind, const double* p_t0, const double* p_t1, int64_t n,
                        double* t);
 
+/* ---- independent scenes (batching; no reference counterpart) ----
+ * Per-scene forms of the reductions above for a scene with n_scenes > 0 and
+ * scene_boff set; outputs are [n_scenes] device arrays, one workgroup per scene. */
+int abd_scene_contact_energy(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* xk,
+                             const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg,
+                             int64_t nvg, double* E);
+int abd_scene_inertia_energy(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* Ak,
+                             const double* pk, const double* A0, const double* p0, const double* Adot,
+                             const double* pdot, double* E);
+int abd_scene_toi(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* x_t0,
+                  const double* x_t1, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
+                  const int32_t* vg, int64_t nvg, double* toi);
+/* out[s] = a . b over the 12 n_s entries of scene s's bodies */
+int abd_scene_dot(abd_stream_t stream, const AbdScene* scene, const double* a, const double* b, double* out);
+/* abd_update_q with alpha[body_scene[b]] (device alpha [n_scenes]) */
+int abd_update_q_scenes(abd_stream_t stream, const AbdScene* scene, const double* A, const double* p,
+                        const double* dq, const double* alpha, double* A_out, double* p_out);
+/* abd_bsr_solve for every scene's block-diagonal system, one workgroup per scene; a
+ * scene with active[s] == 0 (device, nullable) is left untouched; iters/resid [n_scenes].
+ * maxiter 0 -> each scene's system size. */
+int abd_bsr_solve_scenes(abd_stream_t stream, int32_t method, const AbdScene* scene, int32_t max_scene_bodies,
+                         const int32_t* row_ptr, const int32_t* cols, const int32_t* slots, const double* blocks,
+                         const double* b, double* x, double tol, int32_t maxiter, int32_t check_every,
+                         const uint8_t* active, int32_t* iters, double* resid);
+
 /* ---- native frame driver ---- */
 /* Device buffers of one simulation, owned by the caller (all device pointers). */
 typedef struct AbdSimState {
@@ -233,8 +266,13 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
 /* abd.py:212-276 step(): one frame (Newton loop with culling, gh, solve, CCD and line
  * search) on `stream`.  stats (host, nullable) [10] receives running totals: [0] Newton
  * iterations, [1..3] gh PT / EE / VG candidates, [4] active contacts, [5] energy
- * evaluations, [6] solver iterations, [7] frames, [8..9] CCD PT / EE candidates. */
+ * evaluations, [6] solver iterations, [7] frames, [8..9] CCD PT / EE candidates.
+ * With n_scenes > 1 every scene runs its own Newton loop, step length and stopping
+ * test (one launch per stage covers all scenes) and [0] sums the scenes' iterations. */
 int abd_sim_step(void* sim, abd_stream_t stream, int64_t* stats);
+/* Newton iterations of each scene in the last abd_sim_step (iters: host [n_scenes];
+ * with one scene, stats[0] holds the same count summed over frames). */
+int abd_sim_scene_iters(void* sim, int32_t* iters);
 int abd_sim_destroy(void* sim);
 
 /* ---- measurement ---- */

@@ -34,3 +34,22 @@ def test_driver_is_deterministic():
         b.step()
     np.testing.assert_array_equal(a.A, b.A)
     np.testing.assert_array_equal(a.p, b.p)
+
+
+def test_scene_of_splits_a_batch():
+    """scene_of() recovers each independent scene of a batch as a one-scene spec
+    whose arrays are the batch's slices."""
+    from abd_hip import scenes
+    spec = scenes.config_batched(n_scenes=3, seed0=5, nb_range=(2, 4))
+    assert spec.n_scenes == 3
+    b0 = 0
+    for s in range(3):
+        sub = scenes.scene_of(spec, s)
+        nb = sub.n_bodies
+        assert sub.n_scenes == 1 and (sub.body_scene == 0).all()
+        assert np.array_equal(sub.p, spec.p[b0:b0 + nb])
+        v0, v1 = spec.voff[b0], spec.voff[b0 + nb]
+        assert np.array_equal(sub.x0, spec.x0[v0:v1])
+        assert np.array_equal(sub.mass, spec.mass[b0:b0 + nb])
+        b0 += nb
+    assert b0 == spec.n_bodies
