@@ -271,10 +271,10 @@ def _stack_scene(rng: np.random.Generator, n_bodies: int, meshes_lib, gap: float
     return meshes, np.array(ps)
 
 
-def config_batched(n_scenes: int = 256, seed0: int = 1000, nb_range=(2, 64)) -> SceneSpec:
+def config_batched(n_scenes: int = 256, seed0: int = 1000, nb_range=(2, 64), speed: float = 2.0) -> SceneSpec:
     """Batched: n_scenes independent scenes, scene s seeded 1000 + s; each draws
     N_b ~ U{2..64} bodies, each a cube, icosphere-2 or 6x6x6 subdivided box,
-    stacked without overlap, speed U[0, 2] in a uniform direction."""
+    stacked without overlap, speed U[0, speed] (2 by default) in a uniform direction."""
     lib = [cube_mesh(), icosphere_mesh(2, 0.5), subdivided_box_mesh(6)]
     meshes, ps, pd, bs = [], [], [], []
     for s in range(n_scenes):
@@ -285,7 +285,7 @@ def config_batched(n_scenes: int = 256, seed0: int = 1000, nb_range=(2, 64)) -> 
         p = p + np.array([10.0 * s, 0.0, 0.0])
         d = rng.normal(size=(nb, 3))
         d /= np.linalg.norm(d, axis=1, keepdims=True)
-        v = d * rng.uniform(0.0, 2.0, size=(nb, 1))
+        v = d * rng.uniform(0.0, speed, size=(nb, 1))
         meshes += m
         ps.append(p)
         pd.append(v)

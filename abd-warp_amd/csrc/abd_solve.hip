@@ -211,7 +211,8 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
         for (int i = 0; i < maxiter; ++i) {
             spmv(M, row0, p, Ap, n);
             const double pAp = vdot(p, Ap, n, sh);
-            const double alpha = rz_old / pAp;
+            // unguarded in the reference: p = 0 once the residual vanishes exactly between checks
+            const double alpha = pAp != 0.0 ? rz_old / pAp : 0.0;
             for (int k = threadIdx.x; k < n; k += blockDim.x) {
                 x[k] += alpha * p[k];
                 r[k] -= alpha * Ap[k];

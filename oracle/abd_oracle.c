@@ -889,7 +889,9 @@ long orc_bsr_cg(long nslot, const int32_t* rows, const int32_t* cols, const doub
         for (long i = 0; i < maxiter; ++i) {
             trip_mv(nblk, rows, cols, blocks, p, Ap, n);
             double pAp = vdot(p, Ap, n);
-            double alpha = rz_old / pAp;
+            /* the reference divides unguarded: a residual that vanishes exactly between
+             * two checks gives p = 0 and 0/0 here; the step is skipped instead */
+            double alpha = pAp != 0.0 ? rz_old / pAp : 0.0;
             for (long k = 0; k < n; ++k) { x[k] += alpha * p[k]; r[k] -= alpha * Ap[k]; }
             PRECOND();
             rz_new = vdot(r, z, n);
