@@ -133,50 +133,6 @@ int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n
     return ABD_OK;
 }
 
-// -------------------------------------------------------------- reduction
-constexpr int kRedBlocks = 512;
-
-__global__ __launch_bounds__(kBlock) void sum_partial_kernel(const double* __restrict__ in, int64_t n,
-                                                              double* __restrict__ part) {
-    __shared__ double sh[kBlock];
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) acc += in[i];
-    double r = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = r;
-}
-__global__ __launch_bounds__(kBlock) void sum_final_kernel(const double* __restrict__ part, int n,
-                                                            double* __restrict__ out) {
-    __shared__ double sh[kBlock];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += kBlock) acc += part[i];
-    double r = block_sum(acc, sh);
-    if (threadIdx.x == 0) *out = r;
-}
-
-int reduce_sum(hipStream_t s, const double* in, int64_t n, double* out) {
-    int nblk = (int)grid_for(n, kBlock, kRedBlocks);
-    double* part;
-    int rc = arena_get(kSlotReduce, sizeof(double) * kRedBlocks, (void**)&part);
-    if (rc) return rc;
-    hipLaunchKernelGGL(sum_partial_kernel, dim3(nblk), dim3(kBlock), 0, s, in, n, part);
-    ABD_CHECK_LAUNCH();
-    hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(kBlock), 0, s, part, nblk, out);
-    ABD_CHECK_LAUNCH();
-    return ABD_OK;
-}
-
-// ------------------------------------------------------------------- dot
-__global__ __launch_bounds__(kBlock) void dot_partial_kernel(const double* __restrict__ a,
-                                                              const double* __restrict__ b, int64_t n,
-                                                              double* __restrict__ part) {
-    __shared__ double sh[kBlock];
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-        acc += a[i] * b[i];
-    double r = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = r;
-}
-
 int read_i64(hipStream_t s, const int64_t* dev, int64_t* host) {
     int64_t* h;
     int rc = arena_host(sizeof(int64_t), (void**)&h);
@@ -209,19 +165,5 @@ const char* abd_error_string(int code) {
 }
 
 int abd_release(void) { return arena_release_current(); }
-
-int abd_dot(abd_stream_t stream, const double* a, const double* b, int64_t n, double* out) {
-    if (n < 0 || (!a && n) || (!b && n) || !out) return ABD_ERR_ARG;
-    hipStream_t s = to_stream(stream);
-    int nblk = (int)grid_for(n, kBlock, kRedBlocks);
-    double* part;
-    int rc = arena_get(kSlotReduce, sizeof(double) * kRedBlocks, (void**)&part);
-    if (rc) return rc;
-    hipLaunchKernelGGL(dot_partial_kernel, dim3(nblk), dim3(kBlock), 0, s, a, b, n, part);
-    ABD_CHECK_LAUNCH();
-    hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(kBlock), 0, s, part, nblk, out);
-    ABD_CHECK_LAUNCH();
-    return ABD_OK;
-}
 
 }  // extern "C"

@@ -51,38 +51,6 @@ __device__ __forceinline__ void load4(const double* __restrict__ x, const int64_
     for (int k = 0; k < 4; ++k) out[k] = ld3(x + 3 * v[k]);
 }
 
-// -------------------------------------------------------------- energy
-// ipcsk/energy.py:10-48: EE uses l^2, PT |e2p|^2, VG (y - ground)^2 (no validity).
-__global__ __launch_bounds__(kBlock) void energy_partial_kernel(AbdScene sc, AbdParams P,
-                                                                 const double* __restrict__ xk,
-                                                                 const int32_t* __restrict__ pt, int64_t npt,
-                                                                 const int32_t* __restrict__ ee, int64_t nee,
-                                                                 const int32_t* __restrict__ vg, int64_t nvg,
-                                                                 double* __restrict__ part) {
-    __shared__ double sh[kBlock];
-    const SceneView S = view_of(sc);
-    const int64_t n = nee + npt + nvg;
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        if (i < nee)
-            acc += cand_energy_ee(S, xk, ee + 5 * i, P.d2hat, P.kappa);
-        else if (i < nee + npt)
-            acc += cand_energy_pt(S, xk, pt + 5 * (i - nee), P.d2hat, P.kappa);
-        else
-            acc += cand_energy_vg(sc.voff, xk, vg + 2 * (i - nee - npt), P.ground, P.d2hat, P.kappa);
-    }
-    double r = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = r;
-}
-
-__global__ __launch_bounds__(kBlock) void sum_final_kernel2(const double* __restrict__ part, int n,
-                                                             double* __restrict__ out) {
-    __shared__ double sh[kBlock];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += kBlock) acc += part[i];
-    double r = block_sum(acc, sh);
-    if (threadIdx.x == 0) *out = r;
-}
 
 // ------------------------------------------------------------ contact terms
 // One contact: writes coefficients and V_k into out[kContactStride]; returns
@@ -448,24 +416,6 @@ __global__ __launch_bounds__(kBlock) void pair_gh_kernel(int kind, AbdParams P, 
 }
 
 // -------------------------------------------------------------- inertia
-// orthogonal_energy.py:25-51 (per body), partial sums per workgroup.
-__global__ __launch_bounds__(kBlock) void inertia_energy_kernel(int nb, AbdParams P, const double* __restrict__ mass,
-                                                                 const double* __restrict__ I0v,
-                                                                 const double* __restrict__ Ak,
-                                                                 const double* __restrict__ pk,
-                                                                 const double* __restrict__ A0,
-                                                                 const double* __restrict__ p0,
-                                                                 const double* __restrict__ Adot,
-                                                                 const double* __restrict__ pdot,
-                                                                 double* __restrict__ part) {
-    __shared__ double sh[kBlock];
-    double acc = 0.0;
-    for (int b = blockIdx.x * kBlock + threadIdx.x; b < nb; b += gridDim.x * kBlock)
-        acc += inertia_energy_body(P, mass, I0v, b, Ak, pk, A0, p0, Adot, pdot);
-    double r = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = r;
-}
-
 // orthogonal_energy.py:99-112 (overwrites g)
 __global__ void inertia_gradient_kernel(int nb, AbdParams P, const double* __restrict__ mass,
                                         const double* __restrict__ I0v, const double* __restrict__ A,
@@ -528,31 +478,6 @@ __global__ void inertia_hessian_kernel(int nb, AbdParams P, const double* __rest
 using namespace abd;
 
 extern "C" {
-
-int abd_contact_energy(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* xk,
-                       const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
-                       double* E) {
-    if (!sc || !P || !xk || !E || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
-    if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
-    hipStream_t s = to_stream(stream);
-    // the switches of const_params.py:42-44 drop a whole list
-    if (!P->pt) npt = 0;
-    if (!P->ee) nee = 0;
-    if (!P->vg) nvg = 0;
-    const int64_t n = npt + nee + nvg;
-    const int nblk = (int)grid_for(n, kBlock, 512);
-    double* part;
-    int rc = arena_get(kSlotReduce, sizeof(double) * 512, (void**)&part);
-    if (rc) return rc;
-    int ph = prof_start(kProfEnergy, s, 128.0 * npt + 132.0 * nee + 32.0 * nvg);
-    hipLaunchKernelGGL(energy_partial_kernel, dim3(nblk), dim3(kBlock), 0, s, *sc, *P, xk, pt, npt, ee, nee, vg, nvg,
-                       part);
-    ABD_CHECK_LAUNCH();
-    prof_stop(ph, s);
-    hipLaunchKernelGGL(sum_final_kernel2, dim3(1), dim3(kBlock), 0, s, part, nblk, E);
-    ABD_CHECK_LAUNCH();
-    return ABD_OK;
-}
 
 int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x, const int32_t* pt,
                    int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t nij,
@@ -653,24 +578,6 @@ int abd_pair_gh(abd_stream_t stream, int32_t kind, const AbdParams* P, const dou
     if (rc) return rc;
     hipLaunchKernelGGL(pair_gh_kernel, dim3(grid_for(n)), dim3(kBlock), 0, to_stream(stream), kind, *P, x4, r4, n,
                        active, g24, H24, scratch);
-    ABD_CHECK_LAUNCH();
-    return ABD_OK;
-}
-
-int abd_inertia_energy(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* Ak,
-                       const double* pk, const double* A0, const double* p0, const double* Adot, const double* pdot,
-                       double* E) {
-    if (!sc || !P || !Ak || !pk || !A0 || !p0 || !Adot || !pdot || !E) return ABD_ERR_ARG;
-    hipStream_t s = to_stream(stream);
-    const int nb = sc->n_bodies;
-    const int nblk = (int)grid_for(nb, kBlock, 512);
-    double* part;
-    int rc = arena_get(kSlotReduce, sizeof(double) * 512, (void**)&part);
-    if (rc) return rc;
-    hipLaunchKernelGGL(inertia_energy_kernel, dim3(nblk), dim3(kBlock), 0, s, nb, *P, sc->mass, sc->I0, Ak, pk, A0,
-                       p0, Adot, pdot, part);
-    ABD_CHECK_LAUNCH();
-    hipLaunchKernelGGL(sum_final_kernel2, dim3(1), dim3(kBlock), 0, s, part, nblk, E);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }

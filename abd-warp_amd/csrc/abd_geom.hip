@@ -162,6 +162,29 @@ __global__ __launch_bounds__(kBlock) void body_boxes_kernel(const int32_t* __res
     }
 }
 
+// abd.py:337-353 with the step length of the body's scene
+__global__ void update_q_scenes_kernel(int nb, const int32_t* __restrict__ body_scene, const double* __restrict__ A,
+                                       const double* __restrict__ p, const double* __restrict__ dq,
+                                       const double* __restrict__ alpha, double* A_out, double* p_out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 12 * nb) return;
+    const int b = i / 12, r = i % 12;
+    const double al = alpha[body_scene[b]];
+    if (al == 0.0) {  // a scene that does not move keeps its state bit for bit, whatever dq holds
+        if (r < 3)
+            p_out[3 * b + r] = p[3 * b + r];
+        else
+            A_out[9 * b + r - 3] = A[9 * b + r - 3];
+        return;
+    }
+    if (r < 3) {
+        p_out[3 * b + r] = p[3 * b + r] - dq[3 * (4 * b) + r] * al;
+    } else {
+        const int row = (r - 3) / 3, k = (r - 3) % 3;
+        A_out[9 * b + 3 * row + k] = A[9 * b + 3 * row + k] - dq[3 * (4 * b + 1 + row) + k] * al;
+    }
+}
+
 }  // namespace abd
 
 using namespace abd;
@@ -262,6 +285,16 @@ int abd_body_boxes_traj(abd_stream_t stream, const AbdScene* sc, const double* x
     if (!sc->n_bodies) return ABD_OK;
     hipLaunchKernelGGL(body_boxes_kernel, dim3(sc->n_bodies), dim3(kBlock), 0, to_stream(stream), sc->voff, x_end,
                        0.0, 1, lo, hi);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_update_q_scenes(abd_stream_t stream, const AbdScene* sc, const double* A, const double* p, const double* dq,
+                        const double* alpha, double* A_out, double* p_out) {
+    if (!sc || sc->n_scenes <= 0 || !sc->body_scene || !A || !p || !dq || !alpha || !A_out || !p_out) return ABD_ERR_ARG;
+    const int nb = sc->n_bodies;
+    hipLaunchKernelGGL(update_q_scenes_kernel, dim3(grid_for(12 * (int64_t)nb)), dim3(kBlock), 0, to_stream(stream),
+                       nb, sc->body_scene, A, p, dq, alpha, A_out, p_out);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }

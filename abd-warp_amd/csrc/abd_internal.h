@@ -22,7 +22,7 @@ inline hipStream_t to_stream(abd_stream_t s) { return reinterpret_cast<hipStream
 // Per-device scratch arena: `slots` independent growable device buffers plus a
 // small pinned host buffer for count read-back.  Growing synchronizes the
 // device first, so a buffer still in use by queued work is never freed early.
-constexpr int kArenaSlots = 17;
+constexpr int kArenaSlots = 18;
 enum ArenaSlot {
     kSlotScanA = 0,
     kSlotScanB,
@@ -41,6 +41,7 @@ enum ArenaSlot {
     kSlotSolver,
     kSlotSpare,
     kSlotSolveBytes,
+    kSlotPlan,
 };
 int arena_get(int slot, size_t bytes, void** ptr);
 int arena_host(size_t bytes, void** ptr);  // pinned host memory
@@ -50,7 +51,6 @@ int arena_release_current();
 // writing the total to *total (device).  Fixed-order, single workgroup passes.
 int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total);
 // Deterministic sum of n doubles: *out (device) = sum, fixed order.
-int reduce_sum(hipStream_t s, const double* in, int64_t n, double* out);
 // Copies one int64 from device to host (synchronizes `s`).
 int read_i64(hipStream_t s, const int64_t* dev, int64_t* host);
 

@@ -1,0 +1,333 @@
+// abd_reduce.hip — the scalar reductions of the Newton loop (barrier energy,
+// inertia energy, CCD time of impact, dot products), per independent scene.
+//
+// Every reduction is evaluated in an order fixed by the scene's own data: a
+// scene's candidate lists are cut into chunks of kChunkItems counted from the
+// scene's first candidate, each chunk is summed (or min-reduced) by one
+// workgroup in a fixed pattern, and the chunk results are reduced per scene in
+// chunk order.  The same scene therefore gives bit-identical results whether it
+// is evaluated alone (n_scenes = 1: the plain entry points) or inside a batch of
+// any other scenes, and the plain and batched drivers follow the same Newton
+// path to the last bit.
+#include <hip/hip_runtime.h>
+
+#include "abd_block.h"
+#include "abd_device.h"
+#include "abd_inertia.h"
+#include "abd_internal.h"
+#include "abd_profile.h"
+
+namespace abd {
+
+constexpr int kRedThreads = 256;
+constexpr int kChunkItems = 2048;
+constexpr int kLists = 3;
+
+enum { kListPT = 0, kListEE = 1, kListVG = 2 };
+
+struct ListRef {
+    const int32_t* p;
+    int64_t n;
+    int kind;  // kListPT / kListEE / kListVG
+};
+struct Lists {
+    ListRef l[kLists];
+};
+
+__device__ __forceinline__ int width_of(int kind) { return kind == kListVG ? 2 : 5; }
+
+// first k in [0, n) whose tuple's first body belongs to a scene >= s
+__device__ __forceinline__ int64_t scene_lower_bound(const ListRef& L, const int32_t* __restrict__ body_scene,
+                                                     int s) {
+    const int w = width_of(L.kind);
+    int64_t lo = 0, hi = L.n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (body_scene[L.p[w * mid]] < s)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int64_t n_chunks(int64_t len) { return (len + kChunkItems - 1) / kChunkItems; }
+
+// Per scene: the segment [a, b) of each list and the offset of its first chunk.
+// One workgroup; scenes in tiles of blockDim with a running offset.
+__global__ __launch_bounds__(kRedThreads) void plan_kernel(int S, const int32_t* __restrict__ body_scene, Lists L,
+                                                           int64_t* __restrict__ seg, int64_t* __restrict__ choff) {
+    __shared__ int64_t sh[kRedThreads];
+    __shared__ int64_t tile_total;
+    int64_t carry = 0;
+    for (int base = 0; base < S; base += blockDim.x) {
+        const int s = base + threadIdx.x;
+        int64_t cnt = 0;
+        if (s < S) {
+            for (int l = 0; l < kLists; ++l) {
+                int64_t a = 0, b = L.l[l].n;
+                if (S > 1 && L.l[l].n > 0) {
+                    a = scene_lower_bound(L.l[l], body_scene, s);
+                    b = scene_lower_bound(L.l[l], body_scene, s + 1);
+                }
+                seg[2 * (kLists * (int64_t)s + l)] = a;
+                seg[2 * (kLists * (int64_t)s + l) + 1] = b;
+                cnt += n_chunks(b - a);
+            }
+        }
+        const int64_t excl = block_exclusive_scan_i64(cnt, sh);
+        if (s < S) choff[s] = carry + excl;
+        if (threadIdx.x == blockDim.x - 1) tile_total = excl + cnt;
+        __syncthreads();
+        carry += tile_total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) choff[S] = carry;
+}
+
+// the scene whose chunks contain chunk c: the largest s with choff[s] <= c
+__device__ __forceinline__ int scene_of_chunk(const int64_t* __restrict__ choff, int S, int64_t c) {
+    int lo = 0, hi = S - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (choff[mid] <= c)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// ipcsk/energy.py:10-48 per candidate (OP 0) or ccd.py:9-40 TOI (OP 1)
+template <int OP>
+__device__ __forceinline__ double chunk_term(int kind, const AbdScene& sc, const SceneView& V, const AbdParams& P,
+                                             const double* __restrict__ x0, const double* __restrict__ x1,
+                                             const int32_t* t) {
+    if (OP == 0) {
+        if (kind == kListEE) return cand_energy_ee(V, x0, t, P.d2hat, P.kappa);
+        if (kind == kListPT) return cand_energy_pt(V, x0, t, P.d2hat, P.kappa);
+        return cand_energy_vg(sc.voff, x0, t, P.ground, P.d2hat, P.kappa);
+    }
+    if (kind == kListVG) return cand_toi_vg(sc.voff, x0, x1, t, P.ground);
+    return cand_toi(kind == kListPT ? 0 : 1, V, x0, x1, t);
+}
+
+// one chunk per workgroup: OP 0 sums energies, OP 1 takes the minimum TOI below 1
+template <int OP>
+__global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdParams P, const double* __restrict__ x0,
+                                                            const double* __restrict__ x1, Lists L, int S,
+                                                            const int64_t* __restrict__ seg,
+                                                            const int64_t* __restrict__ choff,
+                                                            double* __restrict__ part) {
+    __shared__ double sh[kRedThreads];
+    const int64_t c = blockIdx.x;
+    if (c >= choff[S]) return;
+    const int s = scene_of_chunk(choff, S, c);
+    int64_t k = c - choff[s];
+    int l = 0;
+    int64_t start = 0, end = 0;
+    for (; l < kLists; ++l) {
+        const int64_t a = seg[2 * (kLists * (int64_t)s + l)], b = seg[2 * (kLists * (int64_t)s + l) + 1];
+        const int64_t nch = n_chunks(b - a);
+        if (k < nch) {
+            start = a + k * kChunkItems;
+            end = start + kChunkItems < b ? start + kChunkItems : b;
+            break;
+        }
+        k -= nch;
+    }
+    const SceneView V{sc.voff, sc.foff, sc.eoff, sc.tris, sc.edges, sc.x0};
+    const int kind = l < kLists ? L.l[l].kind : 0;
+    const int w = width_of(kind);
+    double acc = OP == 0 ? 0.0 : 1.0;
+    if (l < kLists) {
+        for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+            const double t = chunk_term<OP>(kind, sc, V, P, x0, x1, L.l[l].p + w * i);
+            if (OP == 0)
+                acc += t;
+            else if (t < 1.0 && t < acc)  // `if t < 1: atomic_min`
+                acc = t;
+        }
+    }
+    const double r = OP == 0 ? block_sum_fast(acc, sh) : block_min(acc, sh);
+    if (threadIdx.x == 0) part[c] = r;
+}
+
+template <int OP>
+__global__ __launch_bounds__(kRedThreads) void scene_final_kernel(const int64_t* __restrict__ choff,
+                                                                  const double* __restrict__ part,
+                                                                  double* __restrict__ out) {
+    __shared__ double sh[kRedThreads];
+    const int s = blockIdx.x;
+    double acc = OP == 0 ? 0.0 : 1.0;
+    for (int64_t i = choff[s] + threadIdx.x; i < choff[s + 1]; i += blockDim.x) {
+        const double v = part[i];
+        if (OP == 0)
+            acc += v;
+        else
+            acc = v < acc ? v : acc;
+    }
+    const double r = OP == 0 ? block_sum_fast(acc, sh) : block_min(acc, sh);
+    if (threadIdx.x == 0) out[s] = r;
+}
+
+// out[s] = sum over the rows [12 sboff[s], 12 sboff[s + 1]) of a * b (one workgroup per scene;
+// sboff == nullptr: one segment [0, n))
+__global__ __launch_bounds__(kRedThreads) void seg_dot_kernel(const int32_t* __restrict__ sboff, int64_t n,
+                                                              const double* __restrict__ a,
+                                                              const double* __restrict__ b, double* __restrict__ out) {
+    __shared__ double sh[17];
+    const int s = blockIdx.x;
+    const int64_t i0 = sboff ? 12 * (int64_t)sboff[s] : 0, i1 = sboff ? 12 * (int64_t)sboff[s + 1] : n;
+    double acc = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) acc += a[i] * b[i];
+    const double r = block_sum_fast(acc, sh);
+    if (threadIdx.x == 0) out[s] = r;
+}
+
+// orthogonal_energy.py:25-51 summed over the bodies of each scene (one workgroup per scene)
+__global__ __launch_bounds__(kRedThreads) void seg_inertia_kernel(const int32_t* __restrict__ sboff, int nb,
+                                                                  AbdParams P, const double* __restrict__ mass,
+                                                                  const double* __restrict__ I0v,
+                                                                  const double* __restrict__ Ak,
+                                                                  const double* __restrict__ pk,
+                                                                  const double* __restrict__ A0,
+                                                                  const double* __restrict__ p0,
+                                                                  const double* __restrict__ Adot,
+                                                                  const double* __restrict__ pdot,
+                                                                  double* __restrict__ E) {
+    __shared__ double sh[17];
+    const int s = blockIdx.x;
+    const int b0 = sboff ? sboff[s] : 0, b1 = sboff ? sboff[s + 1] : nb;
+    double acc = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x)
+        acc += inertia_energy_body(P, mass, I0v, b, Ak, pk, A0, p0, Adot, pdot);
+    const double r = block_sum_fast(acc, sh);
+    if (threadIdx.x == 0) E[s] = r;
+}
+
+// Plan + chunks + per-scene results for OP over the lists; S = 1 treats the lists as one scene.
+template <int OP>
+int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, const double* x0, const double* x1,
+               const Lists& L, double* out, int prof_id, double prof_bytes) {
+    int64_t bound = 3 * (int64_t)S;
+    for (int l = 0; l < kLists; ++l) bound += (L.l[l].n + kChunkItems - 1) / kChunkItems;
+    if (bound > INT32_MAX) return ABD_ERR_ARG;
+    int64_t* plan;
+    double* part;
+    const size_t plan_elems = 2 * kLists * (size_t)S + (size_t)S + 1;
+    int rc = arena_get(kSlotPlan, sizeof(int64_t) * plan_elems, (void**)&plan);
+    if (!rc) rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound, (void**)&part);
+    if (rc) return rc;
+    int64_t* seg = plan;
+    int64_t* choff = plan + 2 * kLists * (size_t)S;
+    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kRedThreads), 0, s, S, sc.body_scene, L, seg, choff);
+    ABD_CHECK_LAUNCH();
+    int ph = prof_start(prof_id, s, prof_bytes);
+    hipLaunchKernelGGL(chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1, L, S, seg,
+                       choff, part);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    hipLaunchKernelGGL(scene_final_kernel<OP>, dim3(S), dim3(kRedThreads), 0, s, choff, part, out);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+bool scenes_ok(const AbdScene* sc) { return sc && sc->n_scenes > 0 && sc->scene_boff && sc->body_scene; }
+
+int energy_impl(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, int S, const double* xk,
+                const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
+                double* E) {
+    if (!P || !xk || !E || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
+    if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
+    // the switches of const_params.py:42-44 drop a whole list
+    if (!P->pt) npt = 0;
+    if (!P->ee) nee = 0;
+    if (!P->vg) nvg = 0;
+    const Lists L{{{ee, nee, kListEE}, {pt, npt, kListPT}, {vg, nvg, kListVG}}};
+    return seg_reduce<0>(to_stream(stream), *sc, *P, S, xk, nullptr, L, E, kProfEnergy,
+                         128.0 * npt + 132.0 * nee + 32.0 * nvg);
+}
+
+int toi_impl(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0,
+             const double* x_t1, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg,
+             int64_t nvg, double* toi) {
+    if (!P || !x_t0 || !x_t1 || !toi || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
+    if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
+    if (!P->pt) npt = 0;  // const_params.py:42-44, abd.py:288-299
+    if (!P->ee) nee = 0;
+    if (!P->vg) nvg = 0;
+    const Lists L{{{vg, nvg, kListVG}, {ee, nee, kListEE}, {pt, npt, kListPT}}};
+    return seg_reduce<1>(to_stream(stream), *sc, *P, S, x_t0, x_t1, L, toi, kProfToi,
+                         224.0 * npt + 228.0 * nee + 56.0 * nvg);
+}
+
+}  // namespace abd
+
+using namespace abd;
+
+extern "C" {
+
+int abd_contact_energy(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* xk,
+                       const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
+                       double* E) {
+    if (!sc) return ABD_ERR_ARG;
+    return energy_impl(stream, sc, P, 1, xk, pt, npt, ee, nee, vg, nvg, E);
+}
+
+int abd_scene_contact_energy(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* xk,
+                             const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg,
+                             int64_t nvg, double* E) {
+    if (!scenes_ok(sc)) return ABD_ERR_ARG;
+    return energy_impl(stream, sc, P, sc->n_scenes, xk, pt, npt, ee, nee, vg, nvg, E);
+}
+
+int abd_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x_t0, const double* x_t1,
+            const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
+            double* toi) {
+    if (!sc) return ABD_ERR_ARG;
+    return toi_impl(stream, sc, P, 1, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, toi);
+}
+
+int abd_scene_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x_t0,
+                  const double* x_t1, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
+                  const int32_t* vg, int64_t nvg, double* toi) {
+    if (!scenes_ok(sc)) return ABD_ERR_ARG;
+    return toi_impl(stream, sc, P, sc->n_scenes, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, toi);
+}
+
+int abd_dot(abd_stream_t stream, const double* a, const double* b, int64_t n, double* out) {
+    if (n < 0 || (!a && n) || (!b && n) || !out) return ABD_ERR_ARG;
+    hipLaunchKernelGGL(seg_dot_kernel, dim3(1), dim3(kRedThreads), 0, to_stream(stream), nullptr, n, a, b, out);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_scene_dot(abd_stream_t stream, const AbdScene* sc, const double* a, const double* b, double* out) {
+    if (!scenes_ok(sc) || !a || !b || !out) return ABD_ERR_ARG;
+    hipLaunchKernelGGL(seg_dot_kernel, dim3(sc->n_scenes), dim3(kRedThreads), 0, to_stream(stream), sc->scene_boff,
+                       (int64_t)0, a, b, out);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_inertia_energy(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* Ak, const double* pk,
+                       const double* A0, const double* p0, const double* Adot, const double* pdot, double* E) {
+    if (!sc || !P || !Ak || !pk || !A0 || !p0 || !Adot || !pdot || !E) return ABD_ERR_ARG;
+    hipLaunchKernelGGL(seg_inertia_kernel, dim3(1), dim3(kRedThreads), 0, to_stream(stream), nullptr, sc->n_bodies,
+                       *P, sc->mass, sc->I0, Ak, pk, A0, p0, Adot, pdot, E);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int abd_scene_inertia_energy(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* Ak,
+                             const double* pk, const double* A0, const double* p0, const double* Adot,
+                             const double* pdot, double* E) {
+    if (!scenes_ok(sc) || !P || !Ak || !pk || !A0 || !p0 || !Adot || !pdot || !E) return ABD_ERR_ARG;
+    hipLaunchKernelGGL(seg_inertia_kernel, dim3(sc->n_scenes), dim3(kRedThreads), 0, to_stream(stream),
+                       sc->scene_boff, sc->n_bodies, *P, sc->mass, sc->I0, Ak, pk, A0, p0, Adot, pdot, E);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+}  // extern "C"

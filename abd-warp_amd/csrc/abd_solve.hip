@@ -268,6 +268,11 @@ using namespace abd;
 extern "C" {
 
 namespace {
+// Workgroup size of a solve, from the largest system's body count alone, so that a
+// scene solved on its own or inside a batch (same rule) reduces in the same order:
+// 256 threads up to 128 bodies (<= 6 rows per thread), 1024 beyond.
+int solve_threads(int max_bodies) { return 12 * max_bodies > 1536 ? kSolveThreads : 256; }
+
 int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const int32_t* sboff, const uint8_t* active,
                  int max_rows, const double* b, double* x, double tol, int32_t maxiter, int32_t check_every,
                  int32_t* iters, double* resid, int threads) {
@@ -324,7 +329,7 @@ int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t
     if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI) return ABD_ERR_ARG;
     Bsr M{nb, row_ptr, cols, slots, blocks};
     return launch_solve(to_stream(stream), method, M, 1, nullptr, nullptr, 12 * nb, b, x, tol, maxiter, check_every,
-                        iters, resid, kSolveThreads);
+                        iters, resid, solve_threads(nb));
 }
 
 int abd_bsr_solve_scenes(abd_stream_t stream, int32_t method, const AbdScene* sc, int32_t max_scene_bodies,
@@ -336,10 +341,8 @@ int abd_bsr_solve_scenes(abd_stream_t stream, int32_t method, const AbdScene* sc
         return ABD_ERR_ARG;
     if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI) return ABD_ERR_ARG;
     Bsr M{sc->n_bodies, row_ptr, cols, slots, blocks};
-    // a scene system has at most 12 * 64 = 768 rows in the batched workloads: 256 threads
-    const int threads = 12 * max_scene_bodies > 1536 ? kSolveThreads : 256;
     return launch_solve(to_stream(stream), method, M, sc->n_scenes, sc->scene_boff, active, 12 * max_scene_bodies, b,
-                        x, tol, maxiter, check_every, iters, resid, threads);
+                        x, tol, maxiter, check_every, iters, resid, solve_threads(max_scene_bodies));
 }
 
 }  // extern "C"
