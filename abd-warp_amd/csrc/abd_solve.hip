@@ -268,10 +268,10 @@ using namespace abd;
 extern "C" {
 
 namespace {
-// Workgroup size of a solve, from the largest system's body count alone, so that a
-// scene solved on its own or inside a batch (same rule) reduces in the same order:
-// 256 threads up to 128 bodies (<= 6 rows per thread), 1024 beyond.
-int solve_threads(int max_bodies) { return 12 * max_bodies > 1536 ? kSolveThreads : 256; }
+// Workgroup size of a solve.  It must not depend on the other systems of a launch,
+// so that a scene solved on its own or inside a batch reduces in the same order;
+// 16 waves per system keep enough loads in flight for the block SpMV.
+int solve_threads(int) { return kSolveThreads; }
 
 int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const int32_t* sboff, const uint8_t* active,
                  int max_rows, const double* b, double* x, double tol, int32_t maxiter, int32_t check_every,

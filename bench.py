@@ -1,22 +1,27 @@
 """Benchmark of the per-Newton-iteration contact path on one MI355X.
 
-Workload (default): the medium plain scene — 64 icosphere-3 bodies (642
-vertices, 1280 triangles, 1920 edges each, radius 0.5) on a 4 x 4 x 4 grid with
-spacing 1.1 over a floor, seeded velocities U[-1,1]^3 and random rotations,
-dhat 1e-3, kappa 1e7, orthogonality stiffness 1e8, dt 0.01, FP64.  A "step" is
-one frame (AffineBodySimulator.step: Newton iterations of culling, barrier
-gradient/Hessian, solve, CCD and line search).  The metric is Newton
-iterations per second over the timed frames; the gh-stage throughput
-(barrier-Hessian candidate pairs per second of gh time) is reported beside it.
+Workload (default, "batched"): 256 independent scenes advanced together, scene
+s seeded 1000 + s; each draws U{2..64} bodies, each a cube (8 V), icosphere-2
+(162 V) or 6x6x6 subdivided box (218 V), stacked in random cells of a 4 x 4
+grid with gap 0.02 over a floor, speed U[0, 2] in a uniform direction
+(8800 bodies, 1.13M vertices, 2.22M triangles in all), dhat 1e-3, kappa 1e7,
+orthogonality stiffness 1e8, dt 0.01, FP64.  A "step" is one frame of every
+scene (AffineBodySimulator.step: per scene, Newton iterations of culling,
+barrier gradient/Hessian, solve, CCD and line search; converged scenes are
+masked).  `value` is the sum over scenes of Newton iterations per second of
+wall time.  Beside it: the plain path (one scene per simulator, the same
+kernels) timed on 8 sampled scenes over the same frames, the ratio of the
+two, and the gh-stage throughput (barrier-Hessian candidate pairs per second
+of gh kernel time).  "--workload medium" runs the 64-icosphere plain scene.
 
 Solver mode: with the reference's contact Hessian (indefinite: b' < 0 times a
-clamped-PSD matrix) the reference's BiCGSTAB diverges on this scene within
-~10 frames and the swept CCD boxes then cover the scene.  The benchmark
-therefore runs the documented stable mode: contact-space PSD projection of the
-barrier Hessian (psd=1) with the reference's Jacobi-preconditioned CG
-(bsr_utils.py:126-235).  The CPU baseline runs the oracle in the same mode.
+clamped-PSD matrix) the reference's BiCGSTAB diverges on colliding scenes and
+the swept CCD boxes then cover the scene.  The benchmark therefore runs the
+documented stable mode: contact-space PSD projection of the barrier Hessian
+(psd=1) with the reference's Jacobi-preconditioned CG (bsr_utils.py:126-235).
+The CPU baseline runs the oracle in the same mode on the sampled scenes.
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload batched|medium|small]
 """
 from __future__ import annotations
 
@@ -118,14 +123,16 @@ def cpu_baseline(spec, cp, warmup: int, steps: int, budget_s: float, sample=None
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="medium")
+    ap.add_argument("--workload", default="batched")
     ap.add_argument("--psd", type=int, default=1)
     ap.add_argument("--solver", type=int, default=1, help="0 BiCGSTAB, 1 Jacobi CG")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=8, help="batched: scenes timed on the plain path")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with one 256 MiB device copy (reference kernel for HBM counter scaling)")
     ap.add_argument("--python-driver", action="store_true", help="run the Python mirror of the driver")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -158,6 +165,14 @@ def main():
     t1 = time.perf_counter()
     _lib.profile_end()
     prof = _lib.profile_query()
+    if a.calibrate_copy:
+        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[gpu] calibration copy: 268435456 bytes read, 268435456 bytes written")
+        del src, dst
     d = {k: sim.stats[k] - s0[k] for k in sim.stats}
     wall = t1 - t0
     iters = d["newton_iters"]
@@ -207,6 +222,7 @@ def main():
         "gh_candidates": cands,
         "gh_active": d["gh_active"],
         "newton_iters": iters,
+        "ccd_candidates": d["ccd_pt"] + d["ccd_ee"],
         "plain_per_scene": plain,
         "batched_over_plain": (iters / wall) / plain["value"] if plain and plain["value"] else None,
         "roofline": roof,
