@@ -92,6 +92,7 @@ def load():
         "abd_sim_create": [S, P, C.c_void_p, C.POINTER(C.c_void_p)],
         "abd_sim_step": [p, p, C.POINTER(C.c_int64)],
         "abd_sim_scene_iters": [p, C.POINTER(C.c_int32)],
+        "abd_sim_hessian_solve": [p, p, i32, d, i32, p, p, C.POINTER(C.c_int32), C.POINTER(C.c_double)],
         "abd_sim_destroy": [p],
         "abd_profile_begin": [],
         "abd_profile_end": [],

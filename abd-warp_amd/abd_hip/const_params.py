@@ -46,7 +46,8 @@ class ContactParams:
     ee: bool = True
     vg: bool = True
     solver: int = SOLVER_BICGSTAB
-    # 0: the reference's contact Hessian (indefinite); 1: contact-space PSD projection
+    # 0: the reference's contact Hessian (indefinite); 1: contact-space PSD projection;
+    # 2: exact Hessian of the barrier energy (1 and 2 go beyond the reference)
     psd: int = 0
 
     @property

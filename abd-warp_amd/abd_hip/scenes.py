@@ -295,6 +295,28 @@ def config_batched(n_scenes: int = 256, seed0: int = 1000, nb_range=(2, 64), spe
                        np.concatenate(pd), ground=0.0 - 0.5 * DHAT, body_scene=np.array(bs), name="batched")
 
 
+def config_diff(seed: int = 0, n_side: int = 4) -> SceneSpec:
+    """Differentiation: 16 icosphere-2 bodies (162 V, 320 F, radius 0.5) on a
+    4 x 4 grid with spacing 1.5, centres 0.5 + U[0.02, 0.1] above the floor
+    (y = 0), identity orientation, seeded velocities U[-0.5, 0.5]^3."""
+    rng = np.random.default_rng(seed)
+    V, F = icosphere_mesh(2, 0.5)
+    n = n_side * n_side
+    ps = []
+    for i in range(n):
+        ps.append([1.5 * (i % n_side), 0.5 - V[:, 1].min() - 0.5 + rng.uniform(0.02, 0.1), 1.5 * (i // n_side)])
+    pdot = rng.uniform(-0.5, 0.5, size=(n, 3))
+    return build_scene([(V, F)] * n, np.stack([np.eye(3)] * n), np.array(ps), np.zeros((n, 3, 3)), pdot,
+                       ground=0.0, name="diff")
+
+
+def diff_targets(spec: SceneSpec, seed: int = 1) -> np.ndarray:
+    """Seeded targets for the differentiation loss: each body's start position
+    moved by U[-0.3, 0.3]^3."""
+    rng = np.random.default_rng(seed)
+    return spec.p + rng.uniform(-0.3, 0.3, size=spec.p.shape)
+
+
 def config_stress(seed: int = 0, n_bodies: int = 1024) -> SceneSpec:
     """Stress: 1024 bodies (equal thirds cube / icosphere-3 / icosphere-4
     ellipsoid with radii U[0.3,0.8]^3 and a radial perturbation) on a jittered
@@ -330,6 +352,27 @@ def subset_scene(spec: SceneSpec, bodies: Sequence[int]) -> SceneSpec:
     return build_scene(meshes, spec.A[idx], spec.p[idx], spec.Adot[idx], spec.pdot[idx], spec.ground,
                        mass=spec.mass[idx], I0=spec.I0[idx], body_scene=scene.astype(np.int32),
                        name=spec.name + "-sub")
+
+
+def concat_scenes(specs: Sequence[SceneSpec], name: str = "batch") -> SceneSpec:
+    """Independent scenes side by side in one SceneSpec (scene k = specs[k]; the
+    scenes never interact, whatever their positions).  All must share the ground."""
+    meshes, A, p, Adot, pdot, mass, I0, bs = [], [], [], [], [], [], [], []
+    for k, sp in enumerate(specs):
+        if sp.ground != specs[0].ground:
+            raise ValueError("scenes of a batch share the ground height")
+        for b in range(sp.n_bodies):
+            meshes.append((sp.x0[sp.voff[b]:sp.voff[b + 1]], sp.tris[sp.foff[b]:sp.foff[b + 1]]))
+        A.append(sp.A)
+        p.append(sp.p)
+        Adot.append(sp.Adot)
+        pdot.append(sp.pdot)
+        mass.append(sp.mass)
+        I0.append(sp.I0)
+        bs += [k] * sp.n_bodies
+    return build_scene(meshes, np.concatenate(A), np.concatenate(p), np.concatenate(Adot), np.concatenate(pdot),
+                       specs[0].ground, mass=np.concatenate(mass), I0=np.concatenate(I0),
+                       body_scene=np.array(bs, np.int32), name=name)
 
 
 def scene_of(spec: SceneSpec, s: int) -> SceneSpec:

@@ -145,6 +145,74 @@ __device__ bool contact_terms(int kind, const d3 v[4], const d3 r[4], double d2h
     return true;
 }
 
+// Exact mode (psd = 2, beyond the reference): the dense 24x24 block
+// J^T (b' d2s/dx2 + b'' ds/dx ds/dx^T) J and the gradient b' J^T ds/dx of one
+// contact, written as out[0..575] (row-major, bodies [I | J]) and out[576..599].
+constexpr int kDenseStride = 600;
+constexpr int kDenseGOff = 576;
+__device__ bool contact_terms_exact(int kind, const d3 v[4], const d3 r[4], double d2hat, double kappa,
+                                    double* __restrict__ out) {
+    CMap c = kind == 0 ? cmap_pt(v[0], v[1], v[2], v[3]) : cmap_ee(v[0], v[1], v[2], v[3]);
+    bool valid = kind == 0 ? valid_pt(c.beta, c.gamma) : valid_ee(c.beta, c.gamma);
+    double l = signed_distance(c.e0, c.e1p, c.e2p);
+    double s = kind == 0 ? l * l : dot(c.e2p, c.e2p);
+    if (!(s < d2hat && valid)) return false;
+    const double b1 = barrier_d1(s, d2hat, kappa);
+    const double b2 = barrier_d2(s, d2hat, kappa);
+    // (u, v, w) = L x over the contact's vertices (the edge vectors of cmap_pt / cmap_ee)
+    const double Lpt[12] = {0, 1, -1, 0, 0, 0, -1, 1, 1, 0, -1, 0};
+    const double Lee[12] = {-1, 1, 0, 0, 0, 0, -1, 1, -1, 0, 1, 0};
+    const double* L = kind == 0 ? Lpt : Lee;
+    const d3 u = kind == 0 ? v[1] - v[2] : v[1] - v[0];
+    const d3 vv = v[3] - v[2];
+    const d3 w = kind == 0 ? v[0] - v[2] : v[2] - v[0];
+    double ge[9], He[81];
+    plane_dist2_derivs(u, vv, w, ge, He);
+    double gx[12], Hx[144];
+    for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 3; ++k) {
+            double acc = 0.0;
+            for (int e = 0; e < 3; ++e) acc += L[4 * e + a] * ge[3 * e + k];
+            gx[3 * a + k] = acc;
+        }
+    for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 3; ++k)
+            for (int b = 0; b < 4; ++b)
+                for (int m = 0; m < 3; ++m) {
+                    double acc = 0.0;
+                    for (int e = 0; e < 3; ++e)
+                        for (int f = 0; f < 3; ++f) acc += L[4 * e + a] * L[4 * f + b] * He[9 * (3 * e + k) + 3 * f + m];
+                    Hx[12 * (3 * a + k) + 3 * b + m] = acc * b1 + (gx[3 * a + k] * gx[3 * b + m]) * b2;
+                }
+    // J: vertex t of body half h(t) maps through [1, r_t^T] (x) I3
+    const int nI = kind == 0 ? 1 : 2;
+    for (int i = 0; i < 24; ++i) {
+        const int hi = i / 12, ai = (i % 12) / 3, ki = i % 3;
+        double gacc = 0.0;
+        for (int t = 0; t < 4; ++t) {
+            if ((t < nI ? 0 : 1) != hi) continue;
+            const double Ji = ai == 0 ? 1.0 : comp(r[t], ai - 1);
+            gacc += Ji * gx[3 * t + ki];
+        }
+        out[kDenseGOff + i] = b1 * gacc;
+        for (int j = 0; j < 24; ++j) {
+            const int hj = j / 12, aj = (j % 12) / 3, kj = j % 3;
+            double acc = 0.0;
+            for (int t = 0; t < 4; ++t) {
+                if ((t < nI ? 0 : 1) != hi) continue;
+                const double Ji = ai == 0 ? 1.0 : comp(r[t], ai - 1);
+                for (int q = 0; q < 4; ++q) {
+                    if ((q < nI ? 0 : 1) != hj) continue;
+                    const double Jj = aj == 0 ? 1.0 : comp(r[q], aj - 1);
+                    acc += Ji * Jj * Hx[12 * (3 * t + ki) + 3 * q + kj];
+                }
+            }
+            out[24 * i + j] = acc;
+        }
+    }
+    return true;
+}
+
 // stage A: active flags over [PT | EE], plus ordering checks.
 __global__ __launch_bounds__(kBlock) void classify_kernel(AbdScene sc, AbdParams P, const double* __restrict__ x,
                                                            const int32_t* __restrict__ pt, int64_t npt,
@@ -224,7 +292,10 @@ __global__ __launch_bounds__(kBlock) void contact_kernel(AbdScene sc, AbdParams 
         d3 v[4], r[4];
         load4(x, vi, v);
         load4(sc.x0, vi, r);
-        contact_terms(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, P.psd, cbuf + e * kContactStride);
+        if (P.psd == 2)
+            contact_terms_exact(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, cbuf + e * kDenseStride);
+        else
+            contact_terms(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, P.psd, cbuf + e * kContactStride);
         ckey[e] = ((int64_t)(is_ee ? 1 : 0) << 32) | (int64_t)t5[2];
         swap[e] = t5[0] > t5[1] ? 1 : 0;  // body I is ij1 (PT second order)
         if (is_ee)
@@ -250,15 +321,18 @@ constexpr int kPairThreads = 576;  // one thread per entry of the 24x24 body-pai
 constexpr int kPairBatch = 8;
 constexpr int kPairBufStride = 2 * 144 + 24;  // diag(ij0), diag(ij1), g(ij0), g(ij1)
 
-// stage D: one workgroup per body pair ii.
+// stage D: one workgroup per body pair ii.  DENSE: contacts stored by contact_terms_exact.
+template <bool DENSE>
 __global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64_t nij,
                                                                     const double* __restrict__ cbuf,
                                                                     const int64_t* __restrict__ ckey,
                                                                     const uint8_t* __restrict__ swap, int64_t nact,
                                                                     double* __restrict__ pbuf,
                                                                     double* __restrict__ blocks) {
-    __shared__ double sc[kPairBatch][kContactStride];
-    __shared__ uint8_t ssw[kPairBatch];
+    constexpr int kStride = DENSE ? kDenseStride : kContactStride;
+    constexpr int kBatch = DENSE ? 4 : kPairBatch;
+    __shared__ double sc[kBatch][kStride];
+    __shared__ uint8_t ssw[kBatch];
     const int64_t ii = blockIdx.x;
     const int t = threadIdx.x;
     int64_t seg[2][2];
@@ -271,23 +345,27 @@ __global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64
     double h = 0.0, gacc = 0.0;
     bool any = false;
     for (int typ = 0; typ < 2; ++typ) {
-        for (int64_t e0 = seg[typ][0]; e0 < seg[typ][1]; e0 += kPairBatch) {
-            int nbat = (int)((seg[typ][1] - e0) < kPairBatch ? (seg[typ][1] - e0) : kPairBatch);
-            for (int q = t; q < nbat * kContactStride; q += kPairThreads)
-                sc[q / kContactStride][q % kContactStride] = cbuf[(e0 + q / kContactStride) * kContactStride +
-                                                                  q % kContactStride];
+        for (int64_t e0 = seg[typ][0]; e0 < seg[typ][1]; e0 += kBatch) {
+            int nbat = (int)((seg[typ][1] - e0) < kBatch ? (seg[typ][1] - e0) : kBatch);
+            for (int q = t; q < nbat * kStride; q += kPairThreads)
+                sc[q / kStride][q % kStride] = cbuf[(e0 + q / kStride) * kStride + q % kStride];
             if (t < nbat) ssw[t] = swap[e0 + t];
             __syncthreads();
             for (int b = 0; b < nbat; ++b) {
                 // body order [ij0 | ij1]: swap halves when body I is ij1
                 const int sw = ssw[b] ? 12 : 0;
                 const int r = (R + sw) % 24, cc = (C + sw) % 24;
+                if (DENSE) {
+                    h += sc[b][24 * r + cc];
+                    if (t < 24) gacc += sc[b][kDenseGOff + (t + sw) % 24];
+                } else {
 #pragma unroll
-                for (int k = 0; k < 5; ++k) {
-                    const double* V = &sc[b][kVOff + 24 * k];
-                    h += sc[b][kCoefOff + k] * (V[r] * V[cc]);
+                    for (int k = 0; k < 5; ++k) {
+                        const double* V = &sc[b][kVOff + 24 * k];
+                        h += sc[b][kCoefOff + k] * (V[r] * V[cc]);
+                    }
+                    if (t < 24) gacc += sc[b][kB1Off] * sc[b][kVOff + 24 * 4 + (t + sw) % 24];
                 }
-                if (t < 24) gacc += sc[b][kB1Off] * sc[b][kVOff + 24 * 4 + (t + sw) % 24];
             }
             any = true;
             __syncthreads();
@@ -365,7 +443,8 @@ __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdP
                 acc += (((d * 2.0) * nr) * th_i) * db;
             } else {
                 const double th_j = ja == 0 ? 1.0 : comp(vt, ja - 1);
-                const double scl = th_i * th_j * d2b * 4.0 * d * d;
+                double scl = th_i * th_j * d2b * 4.0 * d * d;
+                if (P.psd == 2) scl += th_i * th_j * 2.0 * db;  // exact mode: the b' term
                 const double nn = (ra == 1 ? 1.0 : 0.0) * (ca == 1 ? 1.0 : 0.0);
                 acc += nn * scl;
             }
@@ -396,6 +475,16 @@ __global__ __launch_bounds__(kBlock) void pair_gh_kernel(int kind, AbdParams P, 
         for (int k = 0; k < 4; ++k) {
             v[k] = ld3(x4 + 12 * i + 3 * k);
             r[k] = ld3(r4 + 12 * i + 3 * k);
+        }
+        if (P.psd == 2) {
+            double* buf = scratch + i * kDenseStride;
+            bool act = contact_terms_exact(kind, v, r, P.d2hat, P.kappa, buf);
+            active[i] = act ? 1 : 0;
+            for (int a = 0; a < 24; ++a) {
+                g24[24 * i + a] = act ? buf[kDenseGOff + a] : 0.0;
+                for (int c = 0; c < 24; ++c) H24[576 * i + 24 * a + c] = act ? buf[24 * a + c] : 0.0;
+            }
+            continue;
         }
         double* buf = scratch + i * kContactStride;
         bool act = contact_terms(kind, v, r, P.d2hat, P.kappa, P.psd, buf);
@@ -541,19 +630,25 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
         double* cbuf;
         int64_t* ckey;
         uint8_t* swp;
-        if ((rc = arena_get(kSlotContact, (sizeof(double) * kContactStride + sizeof(int64_t) + 1) * (size_t)nact,
+        const bool dense = P->psd == 2;
+        const int stride = dense ? kDenseStride : kContactStride;
+        if ((rc = arena_get(kSlotContact, (sizeof(double) * stride + sizeof(int64_t) + 1) * (size_t)nact,
                             (void**)&cbuf)))
             return rc;
-        ckey = reinterpret_cast<int64_t*>(cbuf + kContactStride * nact);
+        ckey = reinterpret_cast<int64_t*>(cbuf + stride * nact);
         swp = reinterpret_cast<uint8_t*>(ckey + nact);
-        int ph = prof_start(kProfContact, s, (4.0 + 20 + 16 + 192 + 8.0 * kContactStride + 9) * nact);
+        int ph = prof_start(kProfContact, s, (4.0 + 20 + 16 + 192 + 8.0 * stride + 9) * nact);
         hipLaunchKernelGGL(contact_kernel, dim3(grid_for(nact, kBlock, 65535)), dim3(kBlock), 0, s, *sc, *P, x, pt,
                            npt, ee, (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s);
-        ph = prof_start(kProfPairReduce, s, (8.0 * kContactStride + 9) * nact + 8.0 * kPairBufStride * nij);
-        hipLaunchKernelGGL(pair_reduce_kernel, dim3((unsigned)nij), dim3(kPairThreads), 0, s, nb, nij,
-                           (const double*)cbuf, (const int64_t*)ckey, (const uint8_t*)swp, nact, pbuf, blocks);
+        ph = prof_start(kProfPairReduce, s, (8.0 * stride + 9) * nact + 8.0 * kPairBufStride * nij);
+        if (dense)
+            hipLaunchKernelGGL(pair_reduce_kernel<true>, dim3((unsigned)nij), dim3(kPairThreads), 0, s, nb, nij,
+                               (const double*)cbuf, (const int64_t*)ckey, (const uint8_t*)swp, nact, pbuf, blocks);
+        else
+            hipLaunchKernelGGL(pair_reduce_kernel<false>, dim3((unsigned)nij), dim3(kPairThreads), 0, s, nb, nij,
+                               (const double*)cbuf, (const int64_t*)ckey, (const uint8_t*)swp, nact, pbuf, blocks);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s);
     } else if (nij > 0) {
@@ -574,7 +669,8 @@ int abd_pair_gh(abd_stream_t stream, int32_t kind, const AbdParams* P, const dou
     if (!P || n < 0 || (n && (!x4 || !r4 || !active || !g24 || !H24)) || (kind != 0 && kind != 1)) return ABD_ERR_ARG;
     if (n == 0) return ABD_OK;
     double* scratch;
-    int rc = arena_get(kSlotContact, sizeof(double) * kContactStride * (size_t)n, (void**)&scratch);
+    const int stride = P->psd == 2 ? kDenseStride : kContactStride;
+    int rc = arena_get(kSlotContact, sizeof(double) * stride * (size_t)n, (void**)&scratch);
     if (rc) return rc;
     hipLaunchKernelGGL(pair_gh_kernel, dim3(grid_for(n)), dim3(kBlock), 0, to_stream(stream), kind, *P, x4, r4, n,
                        active, g24, H24, scratch);

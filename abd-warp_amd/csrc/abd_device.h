@@ -406,3 +406,60 @@ __device__ __forceinline__ double cand_toi_vg(const int32_t* __restrict__ voff, 
     return t;
 }
 }  // namespace abd
+
+namespace abd {
+// Exact derivatives of the squared plane / line distance used by the barrier,
+// s = T^2 / N with T = w.(u x v) and N = |u x v|^2 (u, v, w the three edge
+// vectors of cmap_from_edges).  g: ds/d(u, v, w) (9), H: d2s/d(u, v, w)^2 (9x9,
+// row-major).  With
+//   dT = (v x w, w x u, u x v),  d2T: uv = -[w]x, uw = [v]x, vw = -[u]x (transposes below),
+//   dN = (2 v x c, 2 c x u, 0),  d2N: uu = 2(|v|^2 I - v v^T), vv = 2(|u|^2 I - u u^T),
+//                                  uv = 2(u v^T - (u.v) I) - 2[c]x   (c = u x v),
+//   ds  = (2T/N) dT - (T^2/N^2) dN,
+//   d2s = (2/N) dT dT^T + (2T/N) d2T - (2T/N^2)(dT dN^T + dN dT^T) - (T^2/N^2) d2N
+//         + (2T^2/N^3) dN dN^T.
+__device__ inline void skew_add(double* H, int r0, int c0, d3 a, double s) {
+    // H[r0.., c0..] += s [a]x   ([a]x b = a x b)
+    H[9 * (r0 + 0) + c0 + 1] += -a.z * s;
+    H[9 * (r0 + 0) + c0 + 2] += a.y * s;
+    H[9 * (r0 + 1) + c0 + 0] += a.z * s;
+    H[9 * (r0 + 1) + c0 + 2] += -a.x * s;
+    H[9 * (r0 + 2) + c0 + 0] += -a.y * s;
+    H[9 * (r0 + 2) + c0 + 1] += a.x * s;
+}
+__device__ inline void plane_dist2_derivs(d3 u, d3 v, d3 w, double g[9], double H[81]) {
+    const d3 c = cross(u, v);
+    const double T = dot(w, c), N = dot(c, c);
+    const d3 gT[3] = {cross(v, w), cross(w, u), c};
+    const d3 gN[3] = {cross(v, c) * 2.0, cross(c, u) * 2.0, mk(0.0, 0.0, 0.0)};
+    const double a1 = 2.0 * T / N, a2 = T * T / (N * N);
+    for (int e = 0; e < 3; ++e)
+        for (int k = 0; k < 3; ++k) g[3 * e + k] = a1 * comp(gT[e], k) - a2 * comp(gN[e], k);
+    const double b0 = 2.0 / N, b1 = 2.0 * T / (N * N), b2 = 2.0 * T * T / (N * N * N);
+    for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j) {
+            const double ti = comp(gT[i / 3], i % 3), tj = comp(gT[j / 3], j % 3);
+            const double ni = comp(gN[i / 3], i % 3), nj = comp(gN[j / 3], j % 3);
+            H[9 * i + j] = b0 * (ti * tj) - b1 * (ti * nj + ni * tj) + b2 * (ni * nj);
+        }
+    // (2T/N) d2T
+    skew_add(H, 0, 3, w, -a1);
+    skew_add(H, 3, 0, w, a1);
+    skew_add(H, 0, 6, v, a1);
+    skew_add(H, 6, 0, v, -a1);
+    skew_add(H, 3, 6, u, -a1);
+    skew_add(H, 6, 3, u, a1);
+    // -(T^2/N^2) d2N
+    const double uu = dot(u, u), vv = dot(v, v), uv = dot(u, v);
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) {
+            const double I = r == k ? 1.0 : 0.0;
+            H[9 * r + k] -= a2 * (2.0 * (vv * I - comp(v, r) * comp(v, k)));
+            H[9 * (3 + r) + 3 + k] -= a2 * (2.0 * (uu * I - comp(u, r) * comp(u, k)));
+            H[9 * r + 3 + k] -= a2 * (2.0 * (comp(u, r) * comp(v, k) - uv * I));
+            H[9 * (3 + r) + k] -= a2 * (2.0 * (comp(v, r) * comp(u, k) - uv * I));
+        }
+    skew_add(H, 0, 3, c, 2.0 * a2);
+    skew_add(H, 3, 0, c, -2.0 * a2);
+}
+}  // namespace abd

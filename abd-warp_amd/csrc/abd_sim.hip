@@ -511,6 +511,38 @@ int abd_sim_step(void* sim, abd_stream_t stream, int64_t* stats) {
     return rc;
 }
 
+int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double tol, int32_t maxiter,
+                          const double* rhs, double* y, int32_t* iters, double* resid) {
+    if (!sim || !rhs || !y) return ABD_ERR_ARG;
+    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI) return ABD_ERR_ARG;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (S.ns > 1) return ABD_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    const AbdSimState& st = S.st;
+    int rc;
+    // the Newton matrix of abd.py:230-243 at q = (A, p), contacts from x(q)
+    if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
+    if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.xk))) return rc;
+    if ((rc = collision_set(s, S, true))) return rc;
+    const size_t nslot = S.nb + 2 * (size_t)S.nij;
+    if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
+    ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
+    ABD_CHECK(hipMemsetAsync(st.g, 0, sizeof(double) * 12 * S.nb, s));
+    if ((rc = abd_inertia_hessian(s, &S.sc, &S.P, st.A, S.blocks.p))) return rc;
+    int64_t nact = 0;
+    if ((rc = abd_contact_gh(s, &S.sc, &S.P, st.x, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.nij,
+                             S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, &nact)))
+        return rc;
+    ABD_CHECK(hipMemsetAsync(y, 0, sizeof(double) * 12 * S.nb, s));
+    if ((rc = abd_bsr_solve(s, method, S.nb, S.row_ptr.p, S.cols.p, S.slots.p, S.blocks.p, rhs, y, tol, maxiter, 10,
+                            S.iters.p, S.scal.p + 4)))
+        return rc;
+    if (iters) ABD_CHECK(hipMemcpyAsync(iters, S.iters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (resid) ABD_CHECK(hipMemcpyAsync(resid, S.scal.p + 4, sizeof(double), hipMemcpyDeviceToHost, s));
+    ABD_CHECK(hipStreamSynchronize(s));
+    return ABD_OK;
+}
+
 int abd_sim_scene_iters(void* sim, int32_t* iters) {
     if (!sim || !iters) return ABD_ERR_ARG;
     const Sim& S = *static_cast<Sim*>(sim);

@@ -66,7 +66,10 @@ typedef void* abd_stream_t; /* hipStream_t */
 /* psd: 0 = the reference's contact Hessian (PT eigenvalues of d2(l^2)/dc2 clamped, EE
  * not, then b' H + b'' g g^T: indefinite since b' < 0); 1 = the whole barrier Hessian
  * projected in contact space (eigenvalues b' L_k and b' L_4 + b'' |q_4|^2 clamped at 0),
- * which keeps the Newton matrix positive semi-definite.  No reference counterpart. */
+ * which keeps the Newton matrix positive semi-definite; 2 = the exact Hessian of the
+ * barrier energy (b' d2s/dx2 + b'' ds/dx ds/dx^T with the curvature of the plane / line
+ * projection, and the b' term of the vertex-ground block), which the adjoint needs.
+ * 1 and 2 have no reference counterpart. */
 typedef struct AbdParams {
     double dt, dhat, d2hat, kappa, stiffness, gravity, ground, mass, I0, c1, alpha_min, tol;
     double solver_tol;
@@ -270,6 +273,14 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
  * With n_scenes > 1 every scene runs its own Newton loop, step length and stopping
  * test (one launch per stage covers all scenes) and [0] sums the scenes' iterations. */
 int abd_sim_step(void* sim, abd_stream_t stream, int64_t* stats);
+/* Adjoint support (no reference counterpart): assembles the Newton matrix H of
+ * abd.py:230-243 (inertia + orthogonality + contact terms, contacts culled at x(q))
+ * at the state's current (A, p), and solves H y = rhs (device [4 Nb][3]; y from zero)
+ * with `method` to relative tolerance `tol`.  With params.psd = 2, H is the exact
+ * Hessian of the frame energy, so y = H^-1 rhs is the implicit-function-theorem
+ * adjoint of one converged step.  iters / resid: host, nullable.  One scene only. */
+int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double tol, int32_t maxiter,
+                          const double* rhs, double* y, int32_t* iters, double* resid);
 /* Newton iterations of each scene in the last abd_sim_step (iters: host [n_scenes];
  * with one scene, stats[0] holds the same count summed over frames). */
 int abd_sim_scene_iters(void* sim, int32_t* iters);

@@ -478,6 +478,50 @@ static void transpose(const double* A, int m, int n, double* T) {
  * (:237-246 / :278-287).  npI: number of contact vertices on body I (PT 1, EE 2).
  * JI: npI x 4, JJ: (4-npI) x 4 rows [1, x0^T].  Outputs g24 (24),
  * Hi/Hj/Hij (12x12 row-major). */
+static void map_pair(const double g12[12], const double Hipc[144], int npI, const double* JI, const double* JJ,
+                     double B1, double g24[24], double Hi[144], double Hj[144], double Hij[144]);
+
+/* Beyond the reference (psd = 2): the exact derivatives of the squared plane /
+ * line distance s = T^2/N, T = det[u v w] = w.(u x v), N = |u|^2|v|^2 - (u.v)^2,
+ * over the edge vectors (u, v, w) of C_vf / C_ee (the reference's dc/dx treats
+ * alpha, beta, gamma as constants and so drops the curvature of the projection).
+ * g: ds/d(u,v,w) (9); H: 9x9 row-major. */
+static void skew3(v3 a, double sgn, double* H, int r0, int c0) {
+    H[9 * r0 + c0 + 1] -= sgn * a.z; H[9 * r0 + c0 + 2] += sgn * a.y;
+    H[9 * (r0 + 1) + c0] += sgn * a.z; H[9 * (r0 + 1) + c0 + 2] -= sgn * a.x;
+    H[9 * (r0 + 2) + c0] -= sgn * a.y; H[9 * (r0 + 2) + c0 + 1] += sgn * a.x;
+}
+static void exact_dist2(v3 u, v3 v, v3 w, double g[9], double H[81]) {
+    double uu = dot(u, u), vv = dot(v, v), uv = dot(u, v);
+    double T = dot(w, cross(u, v));
+    double N = uu * vv - uv * uv;
+    double dT[9], dN[9] = {0}, d2N[81] = {0}, d2T[81] = {0};
+    v3 t0 = cross(v, w), t1 = cross(w, u), t2 = cross(u, v);
+    for (int k = 0; k < 3; ++k) {
+        dT[k] = comp(t0, k); dT[3 + k] = comp(t1, k); dT[6 + k] = comp(t2, k);
+        dN[k] = 2.0 * vv * comp(u, k) - 2.0 * uv * comp(v, k);
+        dN[3 + k] = 2.0 * uu * comp(v, k) - 2.0 * uv * comp(u, k);
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double I = r == c ? 1.0 : 0.0;
+            d2N[9 * r + c] = 2.0 * vv * I - 2.0 * comp(v, r) * comp(v, c);
+            d2N[9 * (3 + r) + 3 + c] = 2.0 * uu * I - 2.0 * comp(u, r) * comp(u, c);
+            d2N[9 * r + 3 + c] = 4.0 * comp(u, r) * comp(v, c) - 2.0 * comp(v, r) * comp(u, c) - 2.0 * uv * I;
+            d2N[9 * (3 + c) + r] = d2N[9 * r + 3 + c];
+        }
+    /* T = u.(v x w): d2T/dudv = -[w]x, dudw = [v]x, dvdw = -[u]x, and transposes */
+    skew3(w, -1.0, d2T, 0, 3); skew3(w, 1.0, d2T, 3, 0);
+    skew3(v, 1.0, d2T, 0, 6); skew3(v, -1.0, d2T, 6, 0);
+    skew3(u, -1.0, d2T, 3, 6); skew3(u, 1.0, d2T, 6, 3);
+    for (int i = 0; i < 9; ++i) g[i] = 2.0 * T / N * dT[i] - T * T / (N * N) * dN[i];
+    for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j)
+            H[9 * i + j] = 2.0 / N * dT[i] * dT[j] + 2.0 * T / N * d2T[9 * i + j]
+                           - 2.0 * T / (N * N) * (dT[i] * dN[j] + dN[i] * dT[j]) - T * T / (N * N) * d2N[9 * i + j]
+                           + 2.0 * T * T / (N * N * N) * dN[i] * dN[j];
+}
+
 static void pair_gh_dense(const double Q[45], const double lam[5], const double D[12], int npI, const double* JI,
                           const double* JJ, double B1, double B2, int psd, double g24[24], double Hi[144],
                           double Hj[144], double Hij[144]) {
@@ -522,6 +566,13 @@ static void pair_gh_dense(const double Q[45], const double lam[5], const double 
     for (int i = 0; i < 12; ++i)
         for (int j = 0; j < 12; ++j)
             Hipc[i * 12 + j] = psd ? H12[i * 12 + j] : H12[i * 12 + j] * B1 + (g12[i] * g12[j]) * B2;
+    map_pair(g12, Hipc, npI, JI, JJ, B1, g24, Hi, Hj, Hij);
+}
+
+/* JTg and JTH12J of pt_ipc.py:170-172,237-246 (ee_ipc.py:278-287): the 12-entry
+ * vertex-space gradient g12 and Hessian Hipc to the two bodies' affine coordinates. */
+static void map_pair(const double g12[12], const double Hipc[144], int npI, const double* JI, const double* JJ,
+                     double B1, double g24[24], double Hi[144], double Hj[144], double Hij[144]) {
     /* JTg: [kron(JI^T, I3) @ g12[:3npI]; kron(JJ^T, I3) @ g12[3npI:]] * B' */
     int nJ = 4 - npI;
     double JIT[16], JJT[16], KI[144], KJ[144];
@@ -607,10 +658,6 @@ int orc_pair_gh(int kind, const orc_params* P, const v3 v[4], const v3 r[4], dou
     }
     if (s_out) *s_out = s;
     if (!(s < P->d2hat && valid)) return 0;
-    /* psd mode projects the whole contact-space Hessian, so it starts from the
-     * unclamped eigenvalues */
-    Q_lambda(e0p, e1p, e2p, kind == 0 && !P->psd, Q, lam);
-    if (kind == 0) dcdx_pt(v[0], v[1], v[2], v[3], D); else dcdx_ee(v[0], v[1], v[2], v[3], D);
     double B1 = orc_barrier_d1(s, P->d2hat, P->kappa);
     double B2 = orc_barrier_d2(s, P->d2hat, P->kappa);
     /* pt_ipc.py:249-255 extract_JpJt / ee_ipc.py:301-311 extract_JeiJej */
@@ -621,6 +668,33 @@ int orc_pair_gh(int kind, const orc_params* P, const v3 v[4], const v3 r[4], dou
         v3 q = r[npI + i];
         JJ[4 * i] = 1.0; JJ[4 * i + 1] = q.x; JJ[4 * i + 2] = q.y; JJ[4 * i + 3] = q.z;
     }
+    if (P->psd == 2) {
+        /* exact: b' d2s/dx2 + b'' ds/dx ds/dx^T, with (u, v, w) = L x (rows over the
+         * contact's four vertices) as in C_vf (x1-x2, x3-x2, x0-x2) / C_ee (x1-x0, x3-x2, x2-x0) */
+        static const double Lpt[12] = {0, 1, -1, 0, 0, 0, -1, 1, 1, 0, -1, 0};
+        static const double Lee[12] = {-1, 1, 0, 0, 0, 0, -1, 1, -1, 0, 1, 0};
+        const double* L = kind == 0 ? Lpt : Lee;
+        v3 e[3];
+        for (int a = 0; a < 3; ++a) {
+            e[a] = mk(0, 0, 0);
+            for (int t = 0; t < 4; ++t) e[a] = add(e[a], scl(v[t], L[4 * a + t]));
+        }
+        double ge[9], He[81], K[108], KT[108], tmp[108], Hx[144], g12[12], Hipc[144];
+        exact_dist2(e[0], e[1], e[2], ge, He);
+        kron3(L, 3, 4, K);  /* 9 x 12 */
+        transpose(K, 9, 12, KT);
+        matmul(KT, ge, g12, 12, 9, 1);
+        matmul(KT, He, tmp, 12, 9, 9);
+        matmul(tmp, K, Hx, 12, 9, 12);
+        for (int i = 0; i < 12; ++i)
+            for (int j = 0; j < 12; ++j) Hipc[i * 12 + j] = Hx[i * 12 + j] * B1 + (g12[i] * g12[j]) * B2;
+        map_pair(g12, Hipc, npI, JI, JJ, B1, g24, Hi, Hj, Hij);
+        return 1;
+    }
+    /* psd mode projects the whole contact-space Hessian, so it starts from the
+     * unclamped eigenvalues */
+    Q_lambda(e0p, e1p, e2p, kind == 0 && !P->psd, Q, lam);
+    if (kind == 0) dcdx_pt(v[0], v[1], v[2], v[3], D); else dcdx_ee(v[0], v[1], v[2], v[3], D);
     pair_gh_dense(Q, lam, D, npI, JI, JJ, B1, B2, P->psd, g24, Hi, Hj, Hij);
     return 1;
 }
@@ -670,6 +744,7 @@ void orc_contact_gh(const orc_scene* S, const orc_params* P, const double* x, co
                 for (int jj = 0; jj < 4; ++jj) {
                     double th_j = jj == 0 ? 1.0 : comp(vt, jj - 1);
                     double sc = th_i * th_j * d2b * 4.0 * d * d;
+                    if (P->psd == 2) sc += th_i * th_j * 2.0 * db;  /* exact: the b' term of D11 */
                     double* B = blocks + 9 * (16 * (long)b + ii + jj * 4);
                     for (int rr = 0; rr < 3; ++rr)
                         for (int cc = 0; cc < 3; ++cc) B[3 * rr + cc] += (comp(nd, rr) * comp(nd, cc)) * sc;

@@ -163,10 +163,11 @@ def test_trajectory_boxes_bit_exact(lib):
 
 
 # --------------------------------------------------------------- energy and gh
-def test_energy_and_gh_match_oracle(lib):
+@pytest.mark.parametrize("psd", [0, 1, 2])
+def test_energy_and_gh_match_oracle(lib, psd):
     from abd_hip.ipc import IPCContactEnergy
     spec = contact_scene()
-    cp = params(ground=spec.ground)
+    cp = params(ground=spec.ground, psd=psd)
     m = _gpu_scene(spec)
     sc = O.OracleScene(spec)
     x = O.project(sc, spec.A, spec.p)
@@ -431,11 +432,13 @@ def test_gpu_matches_golden_fixtures(lib):
             assert rel_err(H24[i], f[f"{tag}_act_H24"][i - 1]) <= 1e-9
 
 
+@pytest.mark.parametrize("psd", [1, 2])
 @pytest.mark.parametrize("kind", [0, 1])
-def test_pair_gh_psd_mode_matches_oracle(lib, kind):
+def test_pair_gh_psd_mode_matches_oracle(lib, kind, psd):
+    """psd=1 (contact-space PSD projection) and psd=2 (exact Hessian) against the oracle."""
     from abd_hip.ipc import pair_gh
     rng = np.random.default_rng(20 + kind)
-    cp = params(psd=1)
+    cp = params(psd=psd)
     X, R = [], []
     for i in range(128):
         d = rng.uniform(1e-5, 9.9e-4) * (1 if i % 2 else -1)

@@ -239,3 +239,70 @@ def test_psd_mode_is_projection_of_contact_space_hessian(kind):
         H = np.block([[Hi, Hij], [Hij.T, Hj]])
         assert rel_err(H, Href) <= 1e-9
         assert np.linalg.eigvalsh(0.5 * (H + H.T)).min() >= -1e-9 * np.abs(H).max()
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_exact_mode_hessian_is_derivative_of_gradient(kind):
+    """psd=2 (beyond the reference): the 24x24 body-pair block is the exact Hessian of
+    b(s(x(q))) — central differences of the exact gradient g24 in the 24 affine
+    coordinates — and the gradient is the reference's."""
+    rng = np.random.default_rng(12 + kind)
+    cp = params(psd=2)
+    prm, prm0 = O.params(cp), O.params(params())
+    nI = 1 if kind == 0 else 2
+    checked = 0
+    for _ in range(40):
+        d = rng.uniform(2e-4, 9e-4)
+        xt = near_contact_pt(rng, d) if kind == 0 else near_contact_ee(rng, d)
+        q = np.zeros(24)
+        r4 = np.zeros((4, 3))
+        for half, verts in ((0, range(nI)), (12, range(nI, 4))):
+            A = np.eye(3) + 0.2 * rng.uniform(-1, 1, size=(3, 3))
+            p = rng.uniform(-1, 1, 3)
+            q[half:half + 3] = p
+            q[half + 3:half + 12] = A.reshape(-1)
+            for v in verts:
+                r4[v] = np.linalg.solve(A.T, xt[v] - p)
+        x = affine_map(q, r4, nI)
+        act, s, g24, Hi, Hj, Hij = O.pair_gh(kind, prm, x, r4)
+        if not act:
+            continue
+        _, _, g_ref, *_ = O.pair_gh(kind, prm0, x, r4)
+        assert rel_err(g24, g_ref) <= 1e-9
+        H = np.block([[Hi, Hij], [Hij.T, Hj]])
+        assert rel_err(H, H.T) <= 1e-12
+        h = 1e-7
+        fd = np.zeros((24, 24))
+        for i in range(24):
+            e = np.eye(24)[i] * h
+            gp = O.pair_gh(kind, prm, affine_map(q + e, r4, nI), r4)[2]
+            gm = O.pair_gh(kind, prm, affine_map(q - e, r4, nI), r4)[2]
+            fd[:, i] = (gp - gm) / (2 * h)
+        assert rel_err(H, fd) <= 1e-5, rel_err(H, fd)
+        checked += 1
+    assert checked >= 20
+
+
+def test_exact_mode_vg_hessian_includes_first_derivative_term():
+    """psd=2: the vertex-ground block is (2 b' + 4 d^2 b'') theta theta^T e_y e_y^T (the
+    exact second derivative of b(d^2)); the reference keeps only the b'' term (D11)."""
+    from abd_hip import scenes
+    cp = params(psd=2)
+    spec = scenes.config_small()
+    sc = O.OracleScene(spec)
+    x = O.project(sc, spec.A, spec.p)
+    x = x.copy()
+    d = 4e-4
+    x[0, 1] = cp.ground + d
+    vg = np.array([[0, 0]], np.int32)
+    g0, B0 = O.contact_gh(sc, O.params(params()), x, None, None, vg, 0)[:2]
+    g2, B2 = O.contact_gh(sc, O.params(cp), x, None, None, vg, 0)[:2]
+    assert np.array_equal(g0, g2)
+    b1 = O.barrier_d1(d * d, cp.d2hat, cp.kappa)
+    b2 = O.barrier_d2(d * d, cp.d2hat, cp.kappa)
+    th = np.concatenate([[1.0], spec.x0[0]])
+    # (ii, jj) sub-block at 16 b + ii + 4 jj, entry (1, 1)
+    for ii in range(4):
+        for jj in range(4):
+            got = B2[ii + 4 * jj][1, 1] - B0[ii + 4 * jj][1, 1]
+            assert abs(got - th[ii] * th[jj] * 2.0 * b1) <= 1e-9 * abs(th[ii] * th[jj] * 2.0 * b1) + 1e-300
