@@ -90,6 +90,16 @@ class AffineBodySimulator:
         self._sim = h
         self._native_stats = (ctypes.c_int64 * 10)()
 
+    @classmethod
+    def from_config(cls, config_file: str = "config.json", device="cuda", native=False) -> "AffineBodySimulator":
+        """abd.py:21 AffineBodySimulator(config_file): scene and parameters from the
+        reference's config.json / scenes/*.json schema (abd_hip.scene_io)."""
+        from .scene_io import load_config
+        cfg = load_config(config_file)
+        sim = cls(cfg.scene, cfg.params, device, native=native)
+        sim.frame_in, sim.frame_out = cfg.frame_in, cfg.frame_out
+        return sim
+
     def scene_iters(self) -> list:
         """Newton iterations of each scene in the last native frame."""
         import ctypes

@@ -90,6 +90,36 @@ __device__ __forceinline__ double block_sum_fast(double v, double* sh) {
     return sh[16];
 }
 
+// Two independent sums with the reduction pattern of block_sum_fast (each result
+// bit-identical to a block_sum_fast of that value alone) and the same two
+// barriers.  sh must hold 34 doubles.
+__device__ __forceinline__ void block_sum2_fast(double& a, double& b, double* sh) {
+    const int lane = lane_id(), w = wave_id(), nw = n_waves();
+    a = wave_sum_down(a);
+    b = wave_sum_down(b);
+    if (lane == 0) {
+        sh[w] = a;
+        sh[17 + w] = b;
+    }
+    __syncthreads();
+    if (w == 0) {
+        double ta = lane < nw ? sh[lane] : 0.0;
+        double tb = lane < nw ? sh[17 + lane] : 0.0;
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) {
+            ta += __shfl_down(ta, off, kWave);
+            tb += __shfl_down(tb, off, kWave);
+        }
+        if (lane == 0) {
+            sh[16] = ta;
+            sh[33] = tb;
+        }
+    }
+    __syncthreads();
+    a = sh[16];
+    b = sh[33];
+}
+
 // Exclusive prefix sum of one int64 per thread (sh: blockDim.x entries).
 __device__ __forceinline__ int64_t block_exclusive_scan_i64(int64_t v, int64_t* sh) {
     const int t = threadIdx.x;

@@ -51,6 +51,33 @@ __device__ __forceinline__ void spmv(const Bsr& M, int row0, const double* __res
     __syncthreads();
 }
 
+// out = H in over this thread's rows (the rows spmv() gives it), returning the
+// thread's partial sum of w[idx] * out[idx] in row order; no barrier: the caller
+// reduces the partials (which also orders the writes before any cross-thread read).
+__device__ __forceinline__ double spmv_rows_dot(const Bsr& M, int row0, const double* __restrict__ in,
+                                                double* __restrict__ out, int n, const double* __restrict__ w) {
+    double part = 0.0;
+    for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+        const int b = row0 + idx / 12, lr = idx % 12, ii = lr / 3, k = lr % 3;
+        double acc = 0.0;
+        for (int e = M.row_ptr[b]; e < M.row_ptr[b + 1]; ++e) {
+            const int lc = M.cols[e] - row0;
+            if ((unsigned)lc >= (unsigned)(n / 12)) continue;  // a block coupling two scenes is not part of either
+            const int64_t s = M.slots[e];
+            const double* xin = in + 12 * (int64_t)lc;
+            const double* base = M.blocks + 9 * 16 * s + 3 * k;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const double* B = base + 9 * (ii + 4 * jj);
+                acc += B[0] * xin[3 * jj] + B[1] * xin[3 * jj + 1] + B[2] * xin[3 * jj + 2];
+            }
+        }
+        out[idx] = acc;
+        part += w[idx] * acc;
+    }
+    return part;
+}
+
 __device__ __forceinline__ double vdot(const double* __restrict__ a, const double* __restrict__ b, int n,
                                        double* sh) {
     double acc = 0.0;
@@ -74,6 +101,7 @@ __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const in
                                                                   int32_t* __restrict__ iters,
                                                                   double* __restrict__ resid) {
     __shared__ double sh[17];
+    __shared__ double sh2[34];
     // one system per workgroup: the bodies of scene blockIdx.x (all bodies without scenes)
     const int sidx = blockIdx.x;
     if (active && !active[sidx]) return;
@@ -113,27 +141,50 @@ __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const in
         __syncthreads();
         double rho = rr;
         for (;;) {
+            // fused passes: each dot is accumulated where its operand is produced, in
+            // the row order of vdot(), so the iterates match the unfused sequence bit for bit
             for (int c = 0; c < check_every; ++c) {
-                spmv(M, row0, p, v, n);
-                const double r0v = vdot(r0, v, n, sh);
+                const double r0v = block_sum_fast(spmv_rows_dot(M, row0, p, v, n, r0), sh);
                 const double alpha = r0v != 0.0 ? rho / r0v : 0.0;
                 for (int i = threadIdx.x; i < n; i += blockDim.x) {
                     x[i] += alpha * p[i];
                     r[i] -= alpha * v[i];
                 }
                 __syncthreads();
-                spmv(M, row0, r, t, n);
-                const double tr = vdot(t, r, n, sh);
-                const double tt = vdot(t, t, n, sh);
+                double tr = 0.0, tt = 0.0;
+                for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+                    // t = H r for this thread's rows (as spmv_rows_dot), then t.r and t.t
+                    const int b = row0 + idx / 12, lr = idx % 12, ii = lr / 3, k = lr % 3;
+                    double acc = 0.0;
+                    for (int e = M.row_ptr[b]; e < M.row_ptr[b + 1]; ++e) {
+                        const int lc = M.cols[e] - row0;
+                        if ((unsigned)lc >= (unsigned)(n / 12)) continue;
+                        const int64_t sl = M.slots[e];
+                        const double* xin = r + 12 * (int64_t)lc;
+                        const double* base = M.blocks + 9 * 16 * sl + 3 * k;
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) {
+                            const double* B = base + 9 * (ii + 4 * jj);
+                            acc += B[0] * xin[3 * jj] + B[1] * xin[3 * jj + 1] + B[2] * xin[3 * jj + 2];
+                        }
+                    }
+                    t[idx] = acc;
+                    tr += acc * r[idx];
+                    tt += acc * acc;
+                }
+                block_sum2_fast(tr, tt, sh2);
                 const double omega = tt != 0.0 ? tr / tt : 0.0;
+                double rr_p = 0.0, rho_p = 0.0;
                 for (int i = threadIdx.x; i < n; i += blockDim.x) {
                     x[i] += omega * r[i];
                     r[i] -= omega * t[i];
+                    rr_p += r[i] * r[i];
+                    rho_p += r0[i] * r[i];
                 }
-                __syncthreads();
-                rr = vdot(r, r, n, sh);
+                block_sum2_fast(rr_p, rho_p, sh2);
+                rr = rr_p;
                 const double rho_old = rho;
-                rho = vdot(r0, r, n, sh);
+                rho = rho_p;
                 const double beta = (rho_old != 0.0 && omega != 0.0) ? (rho / rho_old) * (alpha / omega) : 0.0;
                 for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = r[i] + beta * (p[i] - omega * v[i]);
                 __syncthreads();
@@ -208,18 +259,22 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
     int end_iter = 0;
     if (err > tol_sq) {
         end_iter = maxiter;
+        // fused passes (dots accumulated where their operands are produced, in vdot()'s
+        // row order: the iterates match the unfused sequence bit for bit)
         for (int i = 0; i < maxiter; ++i) {
-            spmv(M, row0, p, Ap, n);
-            const double pAp = vdot(p, Ap, n, sh);
+            const double pAp = block_sum_fast(spmv_rows_dot(M, row0, p, Ap, n, p), sh);
             // unguarded in the reference: p = 0 once the residual vanishes exactly between checks
             const double alpha = pAp != 0.0 ? rz_old / pAp : 0.0;
+            double rz_p = 0.0;
             for (int k = threadIdx.x; k < n; k += blockDim.x) {
                 x[k] += alpha * p[k];
                 r[k] -= alpha * Ap[k];
+                const int k3 = k - k % 3;
+                const double d0 = dg[k3], d1 = dg[k3 + 1], d2 = dg[k3 + 2];
+                z[k] = (d0 * d0 + d1 * d1 + d2 * d2 == 0.0) ? r[k] : r[k] / fabs(dg[k]);
+                rz_p += r[k] * z[k];
             }
-            __syncthreads();
-            jacobi(dg, r, z, n);
-            rz_new = vdot(r, z, n, sh);
+            rz_new = block_sum_fast(rz_p, sh);
             if (((i + 1) % check_every) == 0 && rz_new <= tol_sq) {
                 end_iter = i;
                 err = rz_new;
