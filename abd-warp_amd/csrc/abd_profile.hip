@@ -16,6 +16,7 @@ struct Rec {
     double bytes;
     int scale_slot;  // index into the pinned scale buffer, -1 when unscaled
     int bytes_slot;  // index into the pinned byte buffer, -1 when unscaled
+    bool stopped;    // prof_stop recorded the end event
 };
 constexpr int kScaleSlots = 1 << 16;
 int32_t* g_scale = nullptr;  // pinned host copies of per-launch scale values
@@ -25,24 +26,59 @@ int g_bytes_used = 0;
 std::mutex g_mu;
 bool g_on = false;
 std::vector<Rec> g_recs;
+// events are created once and recycled: creating two events per launch would cost
+// more host time than many of the kernels being timed
+std::vector<hipEvent_t> g_pool;
+
+hipEvent_t take_event() {
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
 double g_ms[kProfCount] = {};
 double g_bytes[kProfCount] = {};
 int64_t g_n[kProfCount] = {};
 
+size_t g_done = 0;  // records before this index are resolved (their events recycled)
+
+void account(const Rec& r, float ms) {
+    double mult = r.scale_slot >= 0 ? (double)g_scale[r.scale_slot] : 1.0;
+    if (r.bytes_slot >= 0) mult *= g_dbytes[r.bytes_slot];
+    g_ms[r.kernel] += ms;
+    g_bytes[r.kernel] += r.bytes * mult;
+    g_n[r.kernel] += 1;
+    g_pool.push_back(r.a);
+    g_pool.push_back(r.b);
+}
+
+// Resolves the records whose end event has completed, in order, without waiting.
+void resolve_ready_locked() {
+    while (g_done < g_recs.size()) {
+        Rec& r = g_recs[g_done];
+        if (!r.stopped || hipEventQuery(r.b) != hipSuccess) break;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) break;
+        account(r, ms);
+        ++g_done;
+    }
+}
+
 int resolve_locked() {
-    for (Rec& r : g_recs) {
+    for (; g_done < g_recs.size(); ++g_done) {
+        Rec& r = g_recs[g_done];
+        if (!r.stopped) continue;
         ABD_CHECK(hipEventSynchronize(r.b));
         float ms = 0.f;
         ABD_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
-        double mult = r.scale_slot >= 0 ? (double)g_scale[r.scale_slot] : 1.0;
-        if (r.bytes_slot >= 0) mult *= g_dbytes[r.bytes_slot];
-        g_ms[r.kernel] += ms;
-        g_bytes[r.kernel] += r.bytes * mult;
-        g_n[r.kernel] += 1;
-        hipEventDestroy(r.a);
-        hipEventDestroy(r.b);
+        account(r, ms);
     }
     g_recs.clear();
+    g_done = 0;
     g_scale_used = 0;
     g_bytes_used = 0;
     return ABD_OK;
@@ -54,9 +90,18 @@ bool prof_enabled() { return g_on; }
 int prof_start(int kernel, hipStream_t s, double bytes) {
     if (!g_on) return -1;
     std::lock_guard<std::mutex> lock(g_mu);
-    Rec r{kernel, nullptr, nullptr, bytes, -1, -1};
-    if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
-    if (hipEventRecord(r.a, s) != hipSuccess) return -1;
+    if (g_pool.size() < 2 && g_recs.size() - g_done > 1024) resolve_ready_locked();
+    Rec r{kernel, take_event(), take_event(), bytes, -1, -1, false};
+    if (!r.a || !r.b) {
+        if (r.a) g_pool.push_back(r.a);
+        if (r.b) g_pool.push_back(r.b);
+        return -1;
+    }
+    if (hipEventRecord(r.a, s) != hipSuccess) {
+        g_pool.push_back(r.a);
+        g_pool.push_back(r.b);
+        return -1;
+    }
     g_recs.push_back(r);
     return (int)g_recs.size() - 1;
 }
@@ -77,6 +122,7 @@ void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale, const double
             g_recs[handle].bytes_slot = slot;
     }
     hipEventRecord(g_recs[handle].b, s);
+    g_recs[handle].stopped = true;
 }
 
 }  // namespace abd
@@ -97,6 +143,12 @@ int abd_profile_begin(void) {
         g_scale = nullptr;
     if (!g_dbytes && hipHostMalloc((void**)&g_dbytes, sizeof(double) * kScaleSlots, hipHostMallocDefault) != hipSuccess)
         g_dbytes = nullptr;
+    // create the events a profiled region recycles before it starts
+    while (g_pool.size() < 4096) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) break;
+        g_pool.push_back(e);
+    }
     g_on = true;
     return rc;
 }

@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=8, help="batched: scenes timed on the plain path")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="time without the per-kernel HIP events (no roofline / kernel breakdown)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with one 256 MiB device copy (reference kernel for HBM counter scaling)")
     ap.add_argument("--python-driver", action="store_true", help="run the Python mirror of the driver")
@@ -155,7 +157,8 @@ def main():
         log(f"[gpu] warmup frame {f} newton {it}")
     torch.cuda.synchronize()
     s0 = dict(sim.stats)
-    _lib.profile_begin()
+    if not a.no_profile:
+        _lib.profile_begin()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for f in range(a.steps):
