@@ -316,6 +316,170 @@ __global__ void solve_bytes_kernel(const int32_t* __restrict__ row_ptr, int nb, 
     if (threadIdx.x == 0) *out = r;
 }
 
+// ---------------------------------------------------------------------------
+// Jacobi CG of bsr_utils.py:126-235 for one system too large for one workgroup's
+// LDS (more than kLargeBodies bodies): the rows are split over workgroups of
+// kMwThreads (one row per thread), and each iteration is three launches —
+// [beta, p update], [SpMV + partial p.Ap], [alpha, x / r / z update + partial r.z].
+// Every workgroup reduces the per-workgroup partials itself, in workgroup order,
+// so all of them hold the same scalars and the sequence is deterministic.  The
+// stopping test (every check_every iterations) sets a device flag that ends the
+// remaining launches of the batch; the host reads it once per batch.
+constexpr int kMwThreads = 256;
+constexpr int kLargeBodies = 170;  // 6 vectors of 12 nb doubles no longer fit kLdsVecBytes
+
+struct MwState {
+    double* x;
+    double* r;
+    double* p;
+    double* Ap;
+    double* z;
+    double* dg;
+    double* part;   // [2][nblk]: partials of p.Ap and r.z
+    double* rz;     // [2]: r.z of the current iteration (slot i % 2) and the next
+    double* err;    // final r.z
+    int32_t* done;  // 1 once stopped
+    int32_t* it;    // end_iter
+};
+
+__device__ __forceinline__ double mw_sum(const double* __restrict__ part, int nblk, double* sh) {
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += blockDim.x) acc += part[k];
+    return block_sum_fast(acc, sh);
+}
+
+__device__ __forceinline__ double mw_row(const Bsr& M, const double* __restrict__ in, int idx) {
+    const int b = idx / 12, lr = idx % 12, ii = lr / 3, k = lr % 3;
+    double acc = 0.0;
+    for (int e = M.row_ptr[b]; e < M.row_ptr[b + 1]; ++e) {
+        const double* xin = in + 12 * (int64_t)M.cols[e];
+        const double* base = M.blocks + 9 * 16 * (int64_t)M.slots[e] + 3 * k;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const double* B = base + 9 * (ii + 4 * jj);
+            acc += B[0] * xin[3 * jj] + B[1] * xin[3 * jj + 1] + B[2] * xin[3 * jj + 2];
+        }
+    }
+    return acc;
+}
+
+__device__ __forceinline__ double mw_jacobi(const double* __restrict__ dg, const double r, int i) {
+    const int i3 = i - i % 3;
+    const double d0 = dg[i3], d1 = dg[i3 + 1], d2 = dg[i3 + 2];
+    return (d0 * d0 + d1 * d1 + d2 * d2 == 0.0) ? r : r / fabs(dg[i]);
+}
+
+// setup: diagonal, x = x0, r = b - H x, z, p = z, partial r.z
+__global__ __launch_bounds__(kMwThreads) void mw_init_kernel(Bsr M, const double* __restrict__ b,
+                                                             const double* __restrict__ xg, MwState W) {
+    __shared__ double sh[17];
+    const int n = 12 * M.nb;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (i < n) {
+        // bsr_get_diag: the three diagonal entries of this row's 3x3 diagonal sub-block (a
+        // 3-row group may straddle two workgroups, so each thread reads all three itself)
+        const int bb = i / 12, ii = (i % 12) / 3, a = i % 3;
+        const double* D = M.blocks + 9 * (16 * (int64_t)bb + ii + 4 * ii);
+        const double d[3] = {D[0], D[4], D[8]};
+        W.dg[i] = d[a];
+        W.x[i] = xg[i];
+        const double r = b[i] - mw_row(M, xg, i);
+        W.r[i] = r;
+        const double z = (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] == 0.0) ? r : r / fabs(d[a]);
+        W.z[i] = z;
+        W.p[i] = z;
+        part = r * z;
+    }
+    const double t = block_sum_fast(part, sh);
+    if (threadIdx.x == 0) W.part[gridDim.x + blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kMwThreads) void mw_start_kernel(int nblk, double tol_sq, int maxiter, MwState W) {
+    __shared__ double sh[17];
+    const double rz = mw_sum(W.part + nblk, nblk, sh);
+    if (threadIdx.x == 0) {
+        W.rz[0] = rz;
+        *W.err = rz;
+        *W.it = rz > tol_sq ? maxiter : 0;
+        *W.done = rz > tol_sq ? 0 : 1;
+    }
+}
+
+// iteration i, part 1: SpMV of p (complete after the previous launch) and partial p.Ap
+__global__ __launch_bounds__(kMwThreads) void mw_spmv_kernel(Bsr M, MwState W) {
+    __shared__ double sh[17];
+    if (*W.done) return;
+    const int n = 12 * M.nb;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (i < n) {
+        const double a = mw_row(M, W.p, i);
+        W.Ap[i] = a;
+        part = W.p[i] * a;
+    }
+    const double t = block_sum_fast(part, sh);
+    if (threadIdx.x == 0) W.part[blockIdx.x] = t;
+}
+
+// part 2: alpha, x / r / z update, partial r.z
+__global__ __launch_bounds__(kMwThreads) void mw_update_kernel(int n, int iter, MwState W) {
+    __shared__ double sh[17];
+    if (*W.done) return;
+    const int nblk = gridDim.x;
+    const double pAp = mw_sum(W.part, nblk, sh);
+    const double rz_old = W.rz[iter & 1];
+    // unguarded in the reference: p = 0 once the residual vanishes exactly between checks
+    const double alpha = pAp != 0.0 ? rz_old / pAp : 0.0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (i < n) {
+        W.x[i] += alpha * W.p[i];
+        const double r = W.r[i] - alpha * W.Ap[i];
+        W.r[i] = r;
+        const double z = mw_jacobi(W.dg, r, i);
+        W.z[i] = z;
+        part = r * z;
+    }
+    const double t = block_sum_fast(part, sh);
+    if (threadIdx.x == 0) W.part[nblk + blockIdx.x] = t;
+}
+
+// part 3: stopping test, beta, p = z + beta p
+__global__ __launch_bounds__(kMwThreads) void mw_direction_kernel(int n, int iter, int check_every, double tol_sq,
+                                                                  MwState W) {
+    __shared__ double sh[17];
+    if (*W.done) return;
+    const int nblk = gridDim.x;
+    const double rz_new = mw_sum(W.part + nblk, nblk, sh);
+    const double rz_old = W.rz[iter & 1];
+    if (((iter + 1) % check_every) == 0 && rz_new <= tol_sq) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            *W.done = 1;
+            *W.it = iter;
+            *W.err = rz_new;
+        }
+        return;
+    }
+    const double beta = rz_new > tol_sq ? rz_new / rz_old : 0.0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) W.p[i] = W.z[i] + beta * W.p[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        W.rz[(iter + 1) & 1] = rz_new;
+        *W.err = rz_new;
+    }
+}
+
+__global__ void mw_finish_kernel(int n, MwState W, double* __restrict__ xg, int32_t* __restrict__ iters,
+                                 double* __restrict__ resid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) xg[i] = W.x[i];
+    if (i == 0) {
+        if (iters) *iters = *W.it;
+        if (resid) *resid = *W.err;
+    }
+}
+
 }  // namespace abd
 
 using namespace abd;
@@ -377,12 +541,63 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
 }
 }  // namespace
 
+// Jacobi CG over several workgroups (systems of more than kLargeBodies bodies).
+int solve_cg_multi(hipStream_t s, const Bsr& M, const double* b, double* x, double tol, int32_t maxiter,
+                   int32_t check_every, int32_t* iters, double* resid) {
+    const int n = 12 * M.nb;
+    const int nblk = (n + kMwThreads - 1) / kMwThreads;
+    if (maxiter <= 0) maxiter = n;
+    double* ws;
+    const size_t nd = 6 * (size_t)n + 2 * (size_t)nblk + 3;
+    int rc = arena_get(kSlotSolver, sizeof(double) * nd + 2 * sizeof(int32_t) + 16, (void**)&ws);
+    if (rc) return rc;
+    MwState W;
+    W.x = ws;
+    W.r = ws + n;
+    W.p = ws + 2 * (size_t)n;
+    W.Ap = ws + 3 * (size_t)n;
+    W.z = ws + 4 * (size_t)n;
+    W.dg = ws + 5 * (size_t)n;
+    W.part = ws + 6 * (size_t)n;
+    W.rz = W.part + 2 * nblk;
+    W.err = W.rz + 2;
+    W.done = reinterpret_cast<int32_t*>(W.err + 1);
+    W.it = W.done + 1;
+    int32_t* hdone;
+    if ((rc = arena_host(sizeof(int32_t), (void**)&hdone))) return rc;
+    const double tol_sq = tol * tol * (double)n;
+    int ph = prof_start(kProfSolve, s, 0.0);
+    hipLaunchKernelGGL(mw_init_kernel, dim3(nblk), dim3(kMwThreads), 0, s, M, b, (const double*)x, W);
+    ABD_CHECK_LAUNCH();
+    hipLaunchKernelGGL(mw_start_kernel, dim3(1), dim3(kMwThreads), 0, s, nblk, tol_sq, maxiter, W);
+    ABD_CHECK_LAUNCH();
+    for (int i0 = 0; i0 < maxiter; i0 += check_every) {
+        const int i1 = i0 + check_every < maxiter ? i0 + check_every : maxiter;
+        for (int i = i0; i < i1; ++i) {
+            hipLaunchKernelGGL(mw_spmv_kernel, dim3(nblk), dim3(kMwThreads), 0, s, M, W);
+            hipLaunchKernelGGL(mw_update_kernel, dim3(nblk), dim3(kMwThreads), 0, s, n, i, W);
+            hipLaunchKernelGGL(mw_direction_kernel, dim3(nblk), dim3(kMwThreads), 0, s, n, i, (int)check_every,
+                               tol_sq, W);
+        }
+        ABD_CHECK_LAUNCH();
+        ABD_CHECK(hipMemcpyAsync(hdone, W.done, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipStreamSynchronize(s));
+        if (*hdone) break;
+    }
+    hipLaunchKernelGGL(mw_finish_kernel, dim3(nblk), dim3(kMwThreads), 0, s, n, W, x, iters, resid);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    return ABD_OK;
+}
+
 int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t* row_ptr, const int32_t* cols,
                   const int32_t* slots, const double* blocks, const double* b, double* x, double tol, int32_t maxiter,
                   int32_t check_every, int32_t* iters, double* resid) {
     if (nb <= 0 || !row_ptr || !cols || !slots || !blocks || !b || !x || check_every <= 0) return ABD_ERR_ARG;
     if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI) return ABD_ERR_ARG;
     Bsr M{nb, row_ptr, cols, slots, blocks};
+    if (method == ABD_SOLVER_CG_JACOBI && nb > kLargeBodies)
+        return solve_cg_multi(to_stream(stream), M, b, x, tol, maxiter, check_every, iters, resid);
     return launch_solve(to_stream(stream), method, M, 1, nullptr, nullptr, 12 * nb, b, x, tol, maxiter, check_every,
                         iters, resid, solve_threads(nb));
 }

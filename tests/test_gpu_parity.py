@@ -277,12 +277,14 @@ def _random_system(nb, ij, rng, spd=True):
 
 
 @pytest.mark.parametrize("method", [0, 1])
-def test_bsr_solve_matches_oracle(lib, method):
+@pytest.mark.parametrize("nb", [40, 300])
+def test_bsr_solve_matches_oracle(lib, method, nb):
+    """nb = 300 exercises the large-system paths (CG over several workgroups; BiCGSTAB
+    with its vectors outside LDS)."""
     from abd_hip import _lib
     from abd_hip.sparse import BsrMatrix, SolveInfo, bicgstab, bsr_cg
-    rng = np.random.default_rng(3 + method)
-    nb = 40
-    pairs = sorted({tuple(sorted(rng.choice(nb, 2, replace=False))) for _ in range(70)})
+    rng = np.random.default_rng(3 + method + nb)
+    pairs = sorted({tuple(sorted(rng.choice(nb, 2, replace=False))) for _ in range(7 * nb // 4)})
     ij = np.array(pairs, np.int32)
     blocks, dense = _random_system(nb, ij, rng, spd=True)
     b = rng.normal(size=(4 * nb, 3))
