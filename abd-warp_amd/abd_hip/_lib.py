@@ -45,6 +45,11 @@ class AbdError(RuntimeError):
 _lib = None
 
 
+class AbdTuning(C.Structure):
+    """Mirror of AbdTuning (include/abd_hip.h)."""
+    _fields_ = [("cull_chunk", C.c_int32), ("cull_grid", C.c_int32)]
+
+
 def load():
     global _lib
     if _lib is not None:
@@ -92,6 +97,9 @@ def load():
         "abd_sim_create": [S, P, C.c_void_p, C.POINTER(C.c_void_p)],
         "abd_sim_step": [p, p, C.POINTER(C.c_int64)],
         "abd_sim_scene_iters": [p, C.POINTER(C.c_int32)],
+        "abd_sim_autotune": [p, p, C.POINTER(AbdTuning)],
+        "abd_get_tuning": [C.POINTER(AbdTuning)],
+        "abd_set_tuning": [C.POINTER(AbdTuning)],
         "abd_sim_hessian_solve": [p, p, i32, d, i32, p, p, C.POINTER(C.c_int32), C.POINTER(C.c_double)],
         "abd_sim_destroy": [p],
         "abd_profile_begin": [],

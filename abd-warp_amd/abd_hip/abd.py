@@ -15,6 +15,8 @@ The scene is a SceneSpec (procedural meshes) instead of config.json + bgeo asset
 """
 from __future__ import annotations
 
+import ctypes
+
 from typing import Optional
 
 import torch
@@ -99,6 +101,15 @@ class AffineBodySimulator:
         sim = cls(cfg.scene, cfg.params, device, native=native)
         sim.frame_in, sim.frame_out = cfg.frame_in, cfg.frame_out
         return sim
+
+    def autotune(self) -> dict:
+        """Picks the culling launch shapes (abd_sim_autotune) at the current state; the
+        results of every later frame are unchanged by the choice."""
+        if self._sim is None:
+            raise RuntimeError("autotune() needs the native driver")
+        t = _lib.AbdTuning()
+        _lib.call("abd_sim_autotune", self._sim, _lib.stream_handle(), ctypes.byref(t))
+        return {"cull_chunk": t.cull_chunk, "cull_grid": t.cull_grid}
 
     def scene_iters(self) -> list:
         """Newton iterations of each scene in the last native frame."""

@@ -133,6 +133,11 @@ int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n
     return ABD_OK;
 }
 
+namespace {
+AbdTuning g_tuning{4096, 16384};
+}
+AbdTuning tuning() { return g_tuning; }
+
 int read_i64(hipStream_t s, const int64_t* dev, int64_t* host) {
     int64_t* h;
     int rc = arena_host(sizeof(int64_t), (void**)&h);
@@ -165,5 +170,19 @@ const char* abd_error_string(int code) {
 }
 
 int abd_release(void) { return arena_release_current(); }
+
+int abd_get_tuning(AbdTuning* t) {
+    if (!t) return ABD_ERR_ARG;
+    *t = g_tuning;
+    return ABD_OK;
+}
+
+int abd_set_tuning(const AbdTuning* t) {
+    if (!t || t->cull_chunk < 256 || t->cull_chunk > 65536 || (t->cull_chunk & (t->cull_chunk - 1)) ||
+        t->cull_grid < 64 || t->cull_grid > (1 << 20))
+        return ABD_ERR_ARG;
+    g_tuning = *t;
+    return ABD_OK;
+}
 
 }  // extern "C"

@@ -7,7 +7,8 @@
 //   2. per body pair and side, the primitives whose box overlaps the hull of the
 //      OTHER side's primitive boxes are kept (a primitive box that misses that
 //      hull cannot overlap any box of the other side, so nothing is lost);
-//   3. the filtered lists are tested exhaustively in chunks of kChunk tests.
+//   3. the filtered lists are tested exhaustively in chunks of `chunk` tests (a tunable
+//      launch shape: the emitted list is the same for every chunk size).
 // Every list is produced by a count -> exclusive scan -> write sequence with
 // wave-ballot ranks, so outputs are emitted in sorted order (ii, order, a, b)
 // and are identical from run to run.
@@ -118,7 +119,6 @@ __global__ __launch_bounds__(kBlock) void body_pairs_kernel(int nb, int64_t npai
 }
 
 // ------------------------------------------------------------- primitives
-constexpr int kChunk = kBlock * 16;
 
 // Hull of the FP32 boxes of each body's primitives (one workgroup per body).
 __global__ __launch_bounds__(kBlock) void hull_kernel(const int32_t* __restrict__ offs, const float* __restrict__ lo,
@@ -196,11 +196,12 @@ __global__ __launch_bounds__(kBlock) void filter_kernel(int pt, const int32_t* _
     if (threadIdx.x == 0) fcnt[blockIdx.x] = run;
 }
 
-__global__ void unit_chunks_kernel(int64_t nunits, const int64_t* __restrict__ fcnt, int64_t* __restrict__ nchunk) {
+__global__ void unit_chunks_kernel(int64_t nunits, int chunk, const int64_t* __restrict__ fcnt,
+                                   int64_t* __restrict__ nchunk) {
     int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nunits) return;
     int64_t nt = fcnt[2 * u] * fcnt[2 * u + 1];
-    nchunk[u] = (nt + kChunk - 1) / kChunk;
+    nchunk[u] = (nt + chunk - 1) / chunk;
 }
 
 // last u with choff[u] <= c (choff non-decreasing, choff[0] = 0)
@@ -217,8 +218,8 @@ __device__ __forceinline__ int64_t find_unit(const int64_t* __restrict__ choff, 
 }
 
 template <bool WRITE>
-__global__ __launch_bounds__(kBlock) void prim_pairs_kernel(int pt, const int32_t* __restrict__ ij, PrimSets S,
-                                                             int64_t maxs, const int32_t* __restrict__ flist,
+__global__ __launch_bounds__(kBlock) void prim_pairs_kernel(int pt, int chunk, const int32_t* __restrict__ ij,
+                                                             PrimSets S, int64_t maxs, const int32_t* __restrict__ flist,
                                                              const int64_t* __restrict__ fcnt,
                                                              const int64_t* __restrict__ choff, int64_t nunits,
                                                              const int64_t* __restrict__ total_chunks,
@@ -237,8 +238,8 @@ __global__ __launch_bounds__(kBlock) void prim_pairs_kernel(int pt, const int32_
         const int32_t* LA = flist + (2 * u) * maxs;
         const int32_t* LB = flist + (2 * u + 1) * maxs;
         const int oA = S.off1[I], oB = S.off2[J];
-        const uint32_t f0 = (uint32_t)(c - choff[u]) * kChunk;
-        const uint32_t f1 = f0 + kChunk < nt ? f0 + kChunk : nt;
+        const uint32_t f0 = (uint32_t)(c - choff[u]) * (uint32_t)chunk;
+        const uint32_t f1 = f0 + (uint32_t)chunk < nt ? f0 + (uint32_t)chunk : nt;
         int64_t run = WRITE ? ccoff[c] : 0;
         for (uint32_t base = f0; base < f1; base += kBlock) {
             uint32_t f = base + threadIdx.x;
@@ -476,7 +477,9 @@ int abd_cull_prims(abd_stream_t stream, const AbdScene* sc, int32_t mode, const 
     hipLaunchKernelGGL(filter_kernel, dim3((unsigned)(2 * nunits)), dim3(kBlock), 0, s, pt, ij, S, maxs, flist, fcnt);
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s);
-    hipLaunchKernelGGL(unit_chunks_kernel, dim3(grid_for(nunits)), dim3(kBlock), 0, s, nunits, fcnt, nchunk);
+    const AbdTuning tune = tuning();
+    hipLaunchKernelGGL(unit_chunks_kernel, dim3(grid_for(nunits)), dim3(kBlock), 0, s, nunits, tune.cull_chunk, fcnt,
+                       nchunk);
     ABD_CHECK_LAUNCH();
     if ((rc = scan_exclusive_i64(s, nchunk, choff, nunits, choff + nunits))) return rc;
     int64_t T;
@@ -485,9 +488,9 @@ int abd_cull_prims(abd_stream_t stream, const AbdScene* sc, int32_t mode, const 
     int64_t *ccnt, *ccoff;
     if ((rc = arena_get(kSlotChunkCount, sizeof(int64_t) * (size_t)(T + 1), (void**)&ccnt))) return rc;
     if ((rc = arena_get(kSlotChunk, sizeof(int64_t) * (size_t)(T + 1), (void**)&ccoff))) return rc;
-    const unsigned grid = (unsigned)(T < 16384 ? T : 16384);
+    const unsigned grid = (unsigned)(T < tune.cull_grid ? T : tune.cull_grid);
     ph = prof_start(kProfPrimCount, s, 0.0);
-    hipLaunchKernelGGL(prim_pairs_kernel<false>, dim3(grid), dim3(kBlock), 0, s, pt, ij, S, maxs,
+    hipLaunchKernelGGL(prim_pairs_kernel<false>, dim3(grid), dim3(kBlock), 0, s, pt, tune.cull_chunk, ij, S, maxs,
                        (const int32_t*)flist, (const int64_t*)fcnt, (const int64_t*)choff, nunits,
                        (const int64_t*)(choff + nunits), ccnt, (const int64_t*)nullptr, (int32_t*)nullptr,
                        (int64_t)0);
@@ -500,7 +503,7 @@ int abd_cull_prims(abd_stream_t stream, const AbdScene* sc, int32_t mode, const 
     if (n > cap) return ABD_ERR_OVERFLOW;
     if (n == 0) return ABD_OK;
     ph = prof_start(kProfPrimFill, s, 20.0 * n);
-    hipLaunchKernelGGL(prim_pairs_kernel<true>, dim3(grid), dim3(kBlock), 0, s, pt, ij, S, maxs,
+    hipLaunchKernelGGL(prim_pairs_kernel<true>, dim3(grid), dim3(kBlock), 0, s, pt, tune.cull_chunk, ij, S, maxs,
                        (const int32_t*)flist, (const int64_t*)fcnt, (const int64_t*)choff, nunits,
                        (const int64_t*)(choff + nunits), ccnt, (const int64_t*)ccoff, out, cap);
     ABD_CHECK_LAUNCH();

@@ -43,6 +43,8 @@ enum ArenaSlot {
     kSlotSolveBytes,
     kSlotPlan,
 };
+// Launch shapes that change no result (abd_set_tuning); process-wide.
+AbdTuning tuning();
 int arena_get(int slot, size_t bytes, void** ptr);
 int arena_host(size_t bytes, void** ptr);  // pinned host memory
 int arena_release_current();

@@ -543,6 +543,50 @@ int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double
     return ABD_OK;
 }
 
+int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen) {
+    if (!sim) return ABD_ERR_ARG;
+    Sim& S = *static_cast<Sim*>(sim);
+    hipStream_t s = to_stream(stream);
+    const AbdTuning saved = tuning();
+    int rc;
+    // the candidate lists at the current state (xk) are what every collision_set of a frame culls
+    if ((rc = abd_project_vertices(s, &S.sc, S.st.Ak, S.st.pk, S.st.xk))) return rc;
+    hipEvent_t e0, e1;
+    ABD_CHECK(hipEventCreate(&e0));
+    ABD_CHECK(hipEventCreate(&e1));
+    AbdTuning best = saved;
+    float best_ms = 1e30f;
+    const int chunks[] = {1024, 2048, 4096, 8192, 16384};
+    const int grids[] = {4096, 16384, 65536};
+    for (int c : chunks)
+        for (int g : grids) {
+            AbdTuning t{c, g};
+            if ((rc = abd_set_tuning(&t))) break;
+            float ms_min = 1e30f;
+            for (int rep = 0; rep < 3 && !rc; ++rep) {
+                hipEventRecord(e0, s);
+                rc = collision_set(s, S, false);
+                hipEventRecord(e1, s);
+                if (!rc && hipEventSynchronize(e1) == hipSuccess) {
+                    float ms = 0.f;
+                    hipEventElapsedTime(&ms, e0, e1);
+                    ms_min = ms < ms_min ? ms : ms_min;
+                }
+            }
+            if (rc) break;
+            if (ms_min < best_ms) {
+                best_ms = ms_min;
+                best = t;
+            }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    abd_set_tuning(rc ? &saved : &best);
+    if (rc) return rc;
+    if (chosen) *chosen = best;
+    return ABD_OK;
+}
+
 int abd_sim_scene_iters(void* sim, int32_t* iters) {
     if (!sim || !iters) return ABD_ERR_ARG;
     const Sim& S = *static_cast<Sim*>(sim);

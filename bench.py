@@ -160,6 +160,11 @@ def main():
     for f in range(a.warmup):
         it = sim.step(f)
         log(f"[gpu] warmup frame {f} newton {it}")
+    tiles = None
+    if not a.python_driver:
+        # launch shapes of the culling chunk passes, timed at the warmed-up state (no effect on results)
+        tiles = sim.autotune()
+        log(f"[gpu] autotuned culling tiles: {tiles}")
     torch.cuda.synchronize()
     s0 = dict(sim.stats)
     if not a.no_profile:
@@ -234,7 +239,8 @@ def main():
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": desc, "mode": "batched" if batched else "plain", "psd_projection": bool(a.psd),
                    "driver": "python" if a.python_driver else "native",
-                   "solver": "jacobi-cg" if a.solver == 1 else "bicgstab", "frames_timed": a.steps},
+                   "solver": "jacobi-cg" if a.solver == 1 else "bicgstab", "frames_timed": a.steps,
+                   "tiles": tiles},
         "gh_pairs_per_s": cands / gh_s if gh_s > 0 else None,
         "gh_candidates": cands,
         "gh_active": d["gh_active"],

@@ -62,6 +62,16 @@ typedef void* abd_stream_t; /* hipStream_t */
 #define ABD_SOLVER_BICGSTAB 0
 #define ABD_SOLVER_CG_JACOBI 1
 
+/* Launch shapes that change no result (no reference counterpart): candidate lists are
+ * exact and emitted in sorted order whatever the tiling.  cull_chunk: primitive-pair tests
+ * per chunk of abd_cull_prims (power of two, 256..65536; default 4096); cull_grid: most
+ * workgroups of its chunk passes (64..2^20; default 16384).  Process-wide. */
+typedef struct AbdTuning {
+    int32_t cull_chunk, cull_grid;
+} AbdTuning;
+int abd_get_tuning(AbdTuning* tuning);
+int abd_set_tuning(const AbdTuning* tuning);
+
 /* Runtime parameters (the reference's module constants, const_params.py:24-39). */
 /* psd: 0 = the reference's contact Hessian (PT eigenvalues of d2(l^2)/dc2 clamped, EE
  * not, then b' H + b'' g g^T: indefinite since b' < 0); 1 = the whole barrier Hessian
@@ -281,6 +291,11 @@ int abd_sim_step(void* sim, abd_stream_t stream, int64_t* stats);
  * adjoint of one converged step.  iters / resid: host, nullable.  One scene only. */
 int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double tol, int32_t maxiter,
                           const double* rhs, double* y, int32_t* iters, double* resid);
+/* Autotuner (no reference counterpart): times the collision-set culling at the driver's
+ * current state for a small set of AbdTuning shapes (chunk 1024..16384, grid 4096..65536,
+ * best of three each), keeps the fastest with abd_set_tuning and returns it (nullable).
+ * Results are unchanged by the choice; call it once the scene has representative contacts. */
+int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen);
 /* Newton iterations of each scene in the last abd_sim_step (iters: host [n_scenes];
  * with one scene, stats[0] holds the same count summed over frames). */
 int abd_sim_scene_iters(void* sim, int32_t* iters);

@@ -67,3 +67,41 @@ def test_product_does_not_import_the_oracle():
         if fn.endswith(".py"):
             text = open(os.path.join(pkg, fn)).read()
             assert not re.search(r"^\s*(from|import)\s+oracle", text, re.M), fn
+
+
+def test_sim_state_and_tuning_layouts_match_header(tmp_path):
+    from abd_hip import _lib
+    structs = [("AbdSimState", _lib.AbdSimState), ("AbdTuning", _lib.AbdTuning)]
+    src = tmp_path / "layout2.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {']
+    for cname, py in structs:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        lines += [f'printf("%zu\\n", offsetof({cname}, {f}));' for f, _ in py._fields_]
+    lines += ['return 0; }']
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout2"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    k = 0
+    for cname, py in structs:
+        assert vals[k] == ctypes.sizeof(py), cname
+        k += 1
+        for f, _ in py._fields_:
+            assert vals[k] == getattr(py, f).offset, (cname, f)
+            k += 1
+
+
+def test_tuning_round_trip_and_bounds():
+    from abd_hip import _lib
+    L = _lib.load()
+    t0 = _lib.AbdTuning()
+    assert L.abd_get_tuning(ctypes.byref(t0)) == 0
+    assert (t0.cull_chunk, t0.cull_grid) == (4096, 16384)
+    bad = _lib.AbdTuning(3000, 16384)   # not a power of two
+    assert L.abd_set_tuning(ctypes.byref(bad)) != 0
+    good = _lib.AbdTuning(8192, 4096)
+    assert L.abd_set_tuning(ctypes.byref(good)) == 0
+    t1 = _lib.AbdTuning()
+    L.abd_get_tuning(ctypes.byref(t1))
+    assert (t1.cull_chunk, t1.cull_grid) == (8192, 4096)
+    assert L.abd_set_tuning(ctypes.byref(t0)) == 0

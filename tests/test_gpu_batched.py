@@ -136,3 +136,29 @@ def test_full_batched_config_sample_matches_plain():
             assert np.array_equal(A, ps.states.A.cpu().numpy()), (f, s)
             assert np.array_equal(p, ps.states.p.cpu().numpy()), (f, s)
     assert torch.isfinite(sim.states.A).all() and torch.isfinite(sim.states.p).all()
+
+
+def test_autotuned_tiles_change_no_result():
+    """Different culling launch shapes (the autotuner's choices) give bit-identical frames."""
+    from abd_hip import _lib, scenes
+    from abd_hip.abd import AffineBodySimulator
+    from abd_hip.const_params import ContactParams
+    import ctypes
+    spec = scenes.config_batched(n_scenes=8, seed0=11, nb_range=(2, 12))
+    cp = ContactParams(ground=spec.ground, psd=1, solver=1)
+    L = _lib.load()
+    t0 = _lib.AbdTuning()
+    L.abd_get_tuning(ctypes.byref(t0))
+    out = []
+    try:
+        for chunk, grid in ((4096, 16384), (1024, 64), (65536, 4096)):
+            assert L.abd_set_tuning(ctypes.byref(_lib.AbdTuning(chunk, grid))) == 0
+            sim = AffineBodySimulator(spec, cp, "cuda", native=True)
+            for f in range(5):
+                sim.step(f)
+            out.append((sim.states.A.cpu().numpy(), sim.states.p.cpu().numpy(), dict(sim.stats)))
+        sim.autotune()
+    finally:
+        L.abd_set_tuning(ctypes.byref(t0))
+    for A, p, st in out[1:]:
+        assert np.array_equal(A, out[0][0]) and np.array_equal(p, out[0][1]) and st == out[0][2]
