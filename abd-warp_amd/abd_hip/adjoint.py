@@ -27,7 +27,7 @@ import torch
 
 from . import _lib
 from .abd import AffineBodySimulator
-from .const_params import SOLVER_BICGSTAB, ContactParams
+from .const_params import SOLVER_CG_JACOBI, ContactParams
 from .scenes import SceneSpec
 
 
@@ -48,7 +48,8 @@ class Rollout:
     a tight tolerance (`params.tol`, `params.max_iter`), since the adjoint is
     exact for the minimiser of each frame."""
 
-    def __init__(self, spec: SceneSpec, params: ContactParams, device="cuda", solve_tol: float = 1e-13):
+    def __init__(self, spec: SceneSpec, params: ContactParams, device="cuda", solve_tol: float = 1e-12,
+                 solve_method: int = SOLVER_CG_JACOBI, solve_maxiter: int = 20000):
         if params.psd != 2:
             raise ValueError("the adjoint needs the exact Hessian: ContactParams(psd=2)")
         if spec.n_scenes > 1:
@@ -60,7 +61,11 @@ class Rollout:
         self.device = self.sim.device
         st = self.sim.states
         self.mass_q = torch.stack([st.mass, st.I0, st.I0, st.I0], dim=1).reshape(-1, 1)  # [4 nb, 1]
+        # the Hessian of a frame with vertex-ground contacts only is SPD: Jacobi CG; BiCGSTAB
+        # (solve_method = 0) for scenes whose exact contact Hessian is indefinite
         self.solve_tol = solve_tol
+        self.solve_method = solve_method
+        self.solve_maxiter = solve_maxiter
         self.checkpoints: List[torch.Tensor] = []
         self.solver_iters: List[int] = []
         self.newton_iters: List[int] = []
@@ -104,8 +109,8 @@ class Rollout:
             st.A.copy_(ck[:, :9].reshape(nb, 3, 3))
             st.p.copy_(ck[:, 9:12])
             lam = (gq + gv / dt).contiguous()
-            _lib.call("abd_sim_hessian_solve", self.sim._sim, stream, SOLVER_BICGSTAB, self.solve_tol, 0,
-                      _lib.ptr(lam), _lib.ptr(y), ctypes.byref(it), ctypes.byref(res))
+            _lib.call("abd_sim_hessian_solve", self.sim._sim, stream, self.solve_method, self.solve_tol,
+                      self.solve_maxiter, _lib.ptr(lam), _lib.ptr(y), ctypes.byref(it), ctypes.byref(res))
             self.solver_iters.append(int(it.value))
             qt_bar = self.mass_q * y
             gq = qt_bar - gv / dt

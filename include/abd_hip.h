@@ -286,7 +286,7 @@ int abd_sim_step(void* sim, abd_stream_t stream, int64_t* stats);
 /* Adjoint support (no reference counterpart): assembles the Newton matrix H of
  * abd.py:230-243 (inertia + orthogonality + contact terms, contacts culled at x(q))
  * at the state's current (A, p), and solves H y = rhs (device [4 Nb][3]; y from zero)
- * with `method` to relative tolerance `tol`.  With params.psd = 2, H is the exact
+ * with `method` and `tol` as abd_bsr_solve does.  With params.psd = 2, H is the exact
  * Hessian of the frame energy, so y = H^-1 rhs is the implicit-function-theorem
  * adjoint of one converged step.  iters / resid: host, nullable.  One scene only. */
 int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double tol, int32_t maxiter,
