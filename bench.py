@@ -63,6 +63,23 @@ def make_workload(name: str):
 
 
 GH_KERNELS = ("gh_classify", "gh_contact", "gh_pair_reduce", "gh_body_finalize")
+# measured HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy of known
+# size) of the profiled kernels, kept under profiles/ as "<kernel>_<workload>_pmc.md"
+PMC_FILES = {"solve": "cg_kernel"}
+
+
+def measured_traffic(kernel: str, workload: str):
+    name = PMC_FILES.get(kernel)
+    if not name:
+        return None
+    path = os.path.join(ROOT, "profiles", f"{name}_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def plain_per_scene(spec, cp, sample, warmup: int, steps: int):
@@ -200,8 +217,8 @@ def main():
         ms, n, byts = prof[dom]
         achieved = byts / (ms * 1e-3) / 1e9
         roof = dict(kernel=dom, bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=None, launches=n, avg_us=round(1e3 * ms / n, 3),
-                    bytes_per_launch=byts / n)
+                    frac=achieved / HBM_PEAK_GBS, traffic=measured_traffic(dom, a.workload), launches=n,
+                    avg_us=round(1e3 * ms / n, 3), bytes_per_launch=byts / n)
     batched = spec.n_scenes > 1
     plain = None
     sample = None
