@@ -285,8 +285,9 @@ class AffineBodySimulator:
             self.stats["gh_vg"] += int(vg_list.shape[0])
             self.stats["gh_active"] += nact
             hess = bsr_from_pairs(self.n_bodies, pairs, blocks)
-            if p.solver == 1:
-                bsr_cg(hess, self.dq, self.g, p.solver_maxiter, p.solver_tol, info=self.solve_info)
+            if p.solver in (1, 2):
+                bsr_cg(hess, self.dq, self.g, p.solver_maxiter, p.solver_tol, info=self.solve_info,
+                       block_precond=p.solver == 2)
             else:
                 bicgstab(hess, self.g, self.dq, p.solver_tol, p.solver_maxiter, info=self.solve_info)
             alpha_cap = self.ccd(states, self.dq)

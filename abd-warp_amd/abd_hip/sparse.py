@@ -52,6 +52,8 @@ def bicgstab(A: BsrMatrix, b, x, tol=1e-4, maxiter=0, check_every=10, info=None)
     return _solve(_lib.SOLVER_BICGSTAB, A, b, x, tol, maxiter, check_every, info)
 
 
-def bsr_cg(A: BsrMatrix, x, b, max_iters=0, tol=1e-4, check_every=10, info=None):
-    """bsr_utils.py:126-235 (diagonal preconditioner; absolute tol^2 n on r.z)."""
-    return _solve(_lib.SOLVER_CG_JACOBI, A, b, x, tol, max_iters, check_every, info)
+def bsr_cg(A: BsrMatrix, x, b, max_iters=0, tol=1e-4, check_every=10, info=None, block_precond=False):
+    """bsr_utils.py:126-235 (diagonal preconditioner; absolute tol^2 n on r.z).  block_precond
+    (beyond the reference) preconditions with each body's 12x12 diagonal block instead."""
+    method = _lib.SOLVER_CG_BLOCK if block_precond else _lib.SOLVER_CG_JACOBI
+    return _solve(method, A, b, x, tol, max_iters, check_every, info)

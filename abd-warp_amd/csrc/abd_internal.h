@@ -22,7 +22,7 @@ inline hipStream_t to_stream(abd_stream_t s) { return reinterpret_cast<hipStream
 // Per-device scratch arena: `slots` independent growable device buffers plus a
 // small pinned host buffer for count read-back.  Growing synchronizes the
 // device first, so a buffer still in use by queued work is never freed early.
-constexpr int kArenaSlots = 18;
+constexpr int kArenaSlots = 19;
 enum ArenaSlot {
     kSlotScanA = 0,
     kSlotScanB,
@@ -42,6 +42,7 @@ enum ArenaSlot {
     kSlotSpare,
     kSlotSolveBytes,
     kSlotPlan,
+    kSlotPrecond,
 };
 // Launch shapes that change no result (abd_set_tuning); process-wide.
 AbdTuning tuning();

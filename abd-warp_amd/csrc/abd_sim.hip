@@ -514,7 +514,8 @@ int abd_sim_step(void* sim, abd_stream_t stream, int64_t* stats) {
 int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double tol, int32_t maxiter,
                           const double* rhs, double* y, int32_t* iters, double* resid) {
     if (!sim || !rhs || !y) return ABD_ERR_ARG;
-    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI) return ABD_ERR_ARG;
+    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI && method != ABD_SOLVER_CG_BLOCK)
+        return ABD_ERR_ARG;
     Sim& S = *static_cast<Sim*>(sim);
     if (S.ns > 1) return ABD_ERR_ARG;
     hipStream_t s = to_stream(stream);

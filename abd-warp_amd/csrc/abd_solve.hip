@@ -210,12 +210,71 @@ __device__ __forceinline__ void jacobi(const double* __restrict__ dg, const doub
     __syncthreads();
 }
 
-template <bool LDS>
+// Inverse of body b's 12x12 diagonal block (slot b) by Gauss-Jordan elimination with
+// partial pivoting, written to out[144]; a singular block gives the identity (that body
+// stays unpreconditioned).
+__device__ void invert_diag_block(const Bsr& M, int b, double* __restrict__ out) {
+    double a[12][24];
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) {
+            const int ii = i / 3, u = i % 3, jj = j / 3, v = j % 3;
+            a[i][j] = M.blocks[9 * (16 * (int64_t)b + ii + 4 * jj) + 3 * u + v];
+            a[i][12 + j] = i == j ? 1.0 : 0.0;
+        }
+    bool ok = true;
+    for (int c = 0; c < 12 && ok; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 12; ++r)
+            if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
+        if (a[piv][c] == 0.0) {
+            ok = false;
+            break;
+        }
+        if (piv != c)
+            for (int j = 0; j < 24; ++j) {
+                const double t = a[c][j];
+                a[c][j] = a[piv][j];
+                a[piv][j] = t;
+            }
+        const double inv = 1.0 / a[c][c];
+        for (int j = 0; j < 24; ++j) a[c][j] *= inv;
+        for (int r = 0; r < 12; ++r) {
+            if (r == c) continue;
+            const double f = a[r][c];
+            if (f != 0.0)
+                for (int j = 0; j < 24; ++j) a[r][j] -= f * a[c][j];
+        }
+    }
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) out[12 * i + j] = ok ? a[i][12 + j] : (i == j ? 1.0 : 0.0);
+}
+
+__global__ void invert_blocks_kernel(Bsr M, double* __restrict__ dinv) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < M.nb) invert_diag_block(M, b, dinv + 144 * (int64_t)b);
+}
+
+// z = D_b^-1 r for row k of a system whose body-diagonal inverses are in dinv (global body ids)
+__device__ __forceinline__ double block_precond(const double* __restrict__ dinv, const double* __restrict__ r,
+                                                int row0, int k) {
+    const int lb = k / 12, l = k % 12;
+    const double* Di = dinv + 144 * (int64_t)(row0 + lb) + 12 * l;
+    const double* rb = r + 12 * lb;
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) acc += Di[q] * rb[q];
+    return acc;
+}
+
+// BJ: the per-body 12x12 block-Jacobi preconditioner (beyond the reference) in place of
+// the diagonal one; dinv: [nb][144] workspace of the inverses.
+template <bool LDS, bool BJ>
 __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t* __restrict__ sboff,
                                                                   const uint8_t* __restrict__ active,
                                                                   const double* __restrict__ b,
                                                             double* __restrict__ xg, double tol, int maxiter,
                                                             int check_every, int cap_rows, double* __restrict__ ws,
+                                                            double* __restrict__ dinv,
                                                             int32_t* __restrict__ iters,
                                                             double* __restrict__ resid) {
     __shared__ double sh[17];
@@ -250,7 +309,12 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
     spmv(M, row0, x, Ap, n);
     for (int i = threadIdx.x; i < n; i += blockDim.x) r[i] = b[i] - Ap[i];
     __syncthreads();
-    jacobi(dg, r, z, n);
+    if (BJ) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) z[i] = block_precond(dinv, r, row0, i);
+        __syncthreads();
+    } else {
+        jacobi(dg, r, z, n);
+    }
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = z[i];
     __syncthreads();
     double rz_old = vdot(r, z, n, sh), rz_new = 0.0;
@@ -266,13 +330,26 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
             // unguarded in the reference: p = 0 once the residual vanishes exactly between checks
             const double alpha = pAp != 0.0 ? rz_old / pAp : 0.0;
             double rz_p = 0.0;
-            for (int k = threadIdx.x; k < n; k += blockDim.x) {
-                x[k] += alpha * p[k];
-                r[k] -= alpha * Ap[k];
-                const int k3 = k - k % 3;
-                const double d0 = dg[k3], d1 = dg[k3 + 1], d2 = dg[k3 + 2];
-                z[k] = (d0 * d0 + d1 * d1 + d2 * d2 == 0.0) ? r[k] : r[k] / fabs(dg[k]);
-                rz_p += r[k] * z[k];
+            if (BJ) {
+                // the body's whole residual enters z: update r, then precondition
+                for (int k = threadIdx.x; k < n; k += blockDim.x) {
+                    x[k] += alpha * p[k];
+                    r[k] -= alpha * Ap[k];
+                }
+                __syncthreads();
+                for (int k = threadIdx.x; k < n; k += blockDim.x) {
+                    z[k] = block_precond(dinv, r, row0, k);
+                    rz_p += r[k] * z[k];
+                }
+            } else {
+                for (int k = threadIdx.x; k < n; k += blockDim.x) {
+                    x[k] += alpha * p[k];
+                    r[k] -= alpha * Ap[k];
+                    const int k3 = k - k % 3;
+                    const double d0 = dg[k3], d1 = dg[k3 + 1], d2 = dg[k3 + 2];
+                    z[k] = (d0 * d0 + d1 * d1 + d2 * d2 == 0.0) ? r[k] : r[k] / fabs(dg[k]);
+                    rz_p += r[k] * z[k];
+                }
             }
             rz_new = block_sum_fast(rz_p, sh);
             if (((i + 1) % check_every) == 0 && rz_new <= tol_sq) {
@@ -301,14 +378,15 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
 // row pointers; with vectors outside LDS, ~10 vector passes of 12 n doubles too.
 __global__ void solve_bytes_kernel(const int32_t* __restrict__ row_ptr, int nb, const int32_t* __restrict__ sboff,
                                    int nsys, const int32_t* __restrict__ iters, int spmv_per_it, int lds,
-                                   double* __restrict__ out) {
+                                   double precond_per_body, double* __restrict__ out) {
     __shared__ double sh[17];
     double acc = 0.0;
     for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
         const int r0 = sboff ? sboff[s] : 0, r1 = sboff ? sboff[s + 1] : nb;
         const double nnzb = (double)(row_ptr[r1] - row_ptr[r0]);
         const double nbs = (double)(r1 - r0);
-        const double per_it = spmv_per_it * (1160.0 * nnzb + 4.0 * (nbs + 1)) + (lds ? 0.0 : 10.0 * 96.0 * nbs);
+        const double per_it = spmv_per_it * (1160.0 * nnzb + 4.0 * (nbs + 1)) + (lds ? 0.0 : 10.0 * 96.0 * nbs) +
+                              precond_per_body * nbs;
         const int it = iters[s] > 0 ? iters[s] : 0;
         acc += per_it * it;
     }
@@ -499,6 +577,14 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
     double* ws;
     int rc = arena_get(kSlotSolver, sizeof(double) * 6 * (size_t)n_all, (void**)&ws);
     if (rc) return rc;
+    const bool bj = method == ABD_SOLVER_CG_BLOCK;
+    double* dinv = nullptr;
+    if (bj && (rc = arena_get(kSlotPrecond, sizeof(double) * 144 * (size_t)M.nb, (void**)&dinv))) return rc;
+    if (bj) {
+        // every body's diagonal-block inverse, one thread per body (inactive scenes included: cheap)
+        hipLaunchKernelGGL(invert_blocks_kernel, dim3((M.nb + 127) / 128), dim3(128), 0, s, M, dinv);
+        ABD_CHECK_LAUNCH();
+    }
     int ph = prof_start(kProfSolve, s, 1.0);
     const size_t vec_bytes = sizeof(double) * 6 * (size_t)max_rows;
     const bool lds = vec_bytes <= kLdsVecBytes;
@@ -507,23 +593,31 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
         if (!attr_set) {
             hipFuncSetAttribute((const void*)bicgstab_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)kLdsVecBytes);
-            hipFuncSetAttribute((const void*)cg_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+            hipFuncSetAttribute((const void*)cg_kernel<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kLdsVecBytes);
+            hipFuncSetAttribute((const void*)cg_kernel<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)kLdsVecBytes);
             attr_set = true;
         }
         if (method == ABD_SOLVER_BICGSTAB)
             hipLaunchKernelGGL(bicgstab_kernel<true>, dim3(nsys), dim3(threads), vec_bytes, s, M, sboff, active, b,
                                x, tol, maxiter, check_every, max_rows, ws, iters, resid);
+        else if (bj)
+            hipLaunchKernelGGL((cg_kernel<true, true>), dim3(nsys), dim3(threads), vec_bytes, s, M, sboff, active, b,
+                               x, tol, maxiter, check_every, max_rows, ws, dinv, iters, resid);
         else
-            hipLaunchKernelGGL(cg_kernel<true>, dim3(nsys), dim3(threads), vec_bytes, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, ws, iters, resid);
+            hipLaunchKernelGGL((cg_kernel<true, false>), dim3(nsys), dim3(threads), vec_bytes, s, M, sboff, active,
+                               b, x, tol, maxiter, check_every, max_rows, ws, dinv, iters, resid);
     } else {
         if (method == ABD_SOLVER_BICGSTAB)
             hipLaunchKernelGGL(bicgstab_kernel<false>, dim3(nsys), dim3(threads), 0, s, M, sboff, active, b, x, tol,
                                maxiter, check_every, max_rows, ws, iters, resid);
+        else if (bj)
+            hipLaunchKernelGGL((cg_kernel<false, true>), dim3(nsys), dim3(threads), 0, s, M, sboff, active, b, x,
+                               tol, maxiter, check_every, max_rows, ws, dinv, iters, resid);
         else
-            hipLaunchKernelGGL(cg_kernel<false>, dim3(nsys), dim3(threads), 0, s, M, sboff, active, b, x, tol,
-                               maxiter, check_every, max_rows, ws, iters, resid);
+            hipLaunchKernelGGL((cg_kernel<false, false>), dim3(nsys), dim3(threads), 0, s, M, sboff, active, b, x,
+                               tol, maxiter, check_every, max_rows, ws, dinv, iters, resid);
     }
     ABD_CHECK_LAUNCH();
     if (ph >= 0 && iters) {
@@ -531,7 +625,7 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
         double* dbytes;
         if ((rc = arena_get(kSlotSolveBytes, sizeof(double), (void**)&dbytes))) return rc;
         hipLaunchKernelGGL(solve_bytes_kernel, dim3(1), dim3(256), 0, s, M.row_ptr, M.nb, sboff, nsys, iters,
-                           method == ABD_SOLVER_BICGSTAB ? 2 : 1, lds ? 1 : 0, dbytes);
+                           method == ABD_SOLVER_BICGSTAB ? 2 : 1, lds ? 1 : 0, bj ? 1152.0 : 0.0, dbytes);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s, nullptr, dbytes);
     } else {
@@ -594,7 +688,8 @@ int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t
                   const int32_t* slots, const double* blocks, const double* b, double* x, double tol, int32_t maxiter,
                   int32_t check_every, int32_t* iters, double* resid) {
     if (nb <= 0 || !row_ptr || !cols || !slots || !blocks || !b || !x || check_every <= 0) return ABD_ERR_ARG;
-    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI) return ABD_ERR_ARG;
+    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI && method != ABD_SOLVER_CG_BLOCK)
+        return ABD_ERR_ARG;
     Bsr M{nb, row_ptr, cols, slots, blocks};
     if (method == ABD_SOLVER_CG_JACOBI && nb > kLargeBodies)
         return solve_cg_multi(to_stream(stream), M, b, x, tol, maxiter, check_every, iters, resid);
@@ -609,7 +704,8 @@ int abd_bsr_solve_scenes(abd_stream_t stream, int32_t method, const AbdScene* sc
     if (!sc || sc->n_scenes <= 0 || !sc->scene_boff || max_scene_bodies <= 0 || !row_ptr || !cols || !slots ||
         !blocks || !b || !x || check_every <= 0)
         return ABD_ERR_ARG;
-    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI) return ABD_ERR_ARG;
+    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI && method != ABD_SOLVER_CG_BLOCK)
+        return ABD_ERR_ARG;
     Bsr M{sc->n_bodies, row_ptr, cols, slots, blocks};
     return launch_solve(to_stream(stream), method, M, sc->n_scenes, sc->scene_boff, active, 12 * max_scene_bodies, b,
                         x, tol, maxiter, check_every, iters, resid, solve_threads(max_scene_bodies));

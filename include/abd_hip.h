@@ -61,6 +61,7 @@ typedef void* abd_stream_t; /* hipStream_t */
 #define ABD_CULL_PT 1
 #define ABD_SOLVER_BICGSTAB 0
 #define ABD_SOLVER_CG_JACOBI 1
+#define ABD_SOLVER_CG_BLOCK 2 /* CG, 12x12 block-Jacobi preconditioner (no reference counterpart) */
 
 /* Launch shapes that change no result (no reference counterpart): candidate lists are
  * exact and emitted in sorted order whatever the tiling.  cull_chunk: primitive-pair tests

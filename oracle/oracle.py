@@ -80,6 +80,8 @@ def lib():
         L.orc_bicgstab.argtypes = [l, p, p, p, p, p, l, d, l, i, p]
         L.orc_bsr_cg.restype = l
         L.orc_bsr_cg.argtypes = [l, p, p, p, p, p, l, d, l, i, p]
+        L.orc_bsr_cg_block.restype = l
+        L.orc_bsr_cg_block.argtypes = [l, p, p, p, p, p, l, d, l, i, p]
         L.orc_cubic_roots.restype = i
         L.orc_cubic_roots.argtypes = [p, d, d, p]
         L.orc_coplanar_poly.restype = i
@@ -325,6 +327,18 @@ def bsr_cg(nb: int, ij, blocks, b, x, tol, maxiter=0, check_every=10):
     res = np.zeros(1)
     it = lib().orc_bsr_cg(blocks.shape[0] // 16, _ptr(rows), _ptr(cols), _ptr(blocks), _ptr(b), _ptr(x), b.size,
                           float(tol), int(maxiter), int(check_every), _ptr(res))
+    return x, int(it), float(res[0])
+
+
+def bsr_cg_block(nb: int, ij, blocks, b, x, tol, maxiter=0, check_every=10):
+    """CG with the per-body 12x12 diagonal block as preconditioner (beyond the reference)."""
+    rows, cols = set_triplets(nb, ij)
+    blocks = _c(blocks, np.float64)
+    b = _c(b, np.float64)
+    x = np.array(x, np.float64, copy=True)
+    res = np.zeros(1)
+    it = lib().orc_bsr_cg_block(blocks.shape[0] // 16, _ptr(rows), _ptr(cols), _ptr(blocks), _ptr(b), _ptr(x),
+                                b.size, float(tol), int(maxiter), int(check_every), _ptr(res))
     return x, int(it), float(res[0])
 
 

@@ -76,3 +76,15 @@ def test_medium_subset_psd_cg_frames():
     cp = ContactParams(ground=spec.ground, psd=1, solver=1, solver_tol=1e-10, solver_maxiter=20000)
     sim, ref, worst = _run(spec, cp, 10, native=True)
     assert sim.stats["gh_active"] > 0
+
+
+def test_medium_subset_block_jacobi_cg_frames():
+    """The stable mode with the block-Jacobi CG (beyond the reference) against the oracle in
+    the same mode."""
+    from abd_hip import scenes
+    from abd_hip.const_params import ContactParams
+    spec = scenes.config_medium(seed=0, n_side=2, subdiv=2)
+    spec.p[:] = spec.p * 0.93
+    cp = ContactParams(ground=spec.ground, psd=1, solver=2, solver_tol=1e-10, solver_maxiter=20000)
+    sim, ref, worst = _run(spec, cp, 10, native=True)
+    assert sim.stats["gh_active"] > 0

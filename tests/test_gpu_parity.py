@@ -276,9 +276,10 @@ def _random_system(nb, ij, rng, spd=True):
     return blocks, dense
 
 
-@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("method", [0, 1, 2])
 @pytest.mark.parametrize("nb", [40, 300])
 def test_bsr_solve_matches_oracle(lib, method, nb):
+    """method 2 is CG with the 12x12 block-Jacobi preconditioner (beyond the reference)."""
     """nb = 300 exercises the large-system paths (CG over several workgroups; BiCGSTAB
     with its vectors outside LDS)."""
     from abd_hip import _lib
@@ -309,9 +310,12 @@ def test_bsr_solve_matches_oracle(lib, method, nb):
     if method == 0:
         bicgstab(A, T(b), x, tol, 0, 10, info)
         x_ref, it_ref, _ = O.bicgstab(nb, ij, blocks, b, x0, tol)
-    else:
+    elif method == 1:
         bsr_cg(A, x, T(b), 0, tol, 10, info)
         x_ref, it_ref, _ = O.bsr_cg(nb, ij, blocks, b, x0, tol)
+    else:
+        bsr_cg(A, x, T(b), 0, tol, 10, info, block_precond=True)
+        x_ref, it_ref, _ = O.bsr_cg_block(nb, ij, blocks, b, x0, tol)
     xs = np.linalg.solve(dense, b.reshape(-1)).reshape(-1, 3)
     assert rel_err(N(x), xs) <= 1e-7
     assert rel_err(N(x), x_ref) <= 1e-7

@@ -262,14 +262,14 @@ def test_triplets_assemble_the_dense_matrix():
     np.testing.assert_array_equal(Hd, dense)
 
 
-@pytest.mark.parametrize("which", ["bicgstab", "cg"])
+@pytest.mark.parametrize("which", ["bicgstab", "cg", "cg_block"])
 def test_solvers_against_numpy(which):
     rng = np.random.default_rng(6)
     nb = 12
     ij, blocks, dense = _system(rng, nb)
     b = rng.normal(size=(4 * nb, 3))
     x0 = np.zeros_like(b)
-    fn = O.bicgstab if which == "bicgstab" else O.bsr_cg
+    fn = {"bicgstab": O.bicgstab, "cg": O.bsr_cg, "cg_block": O.bsr_cg_block}[which]
     x, it, res = fn(nb, ij, blocks, b, x0, 1e-12)
     xs = np.linalg.solve(dense, b.reshape(-1)).reshape(-1, 3)
     assert rel_err(x, xs) <= 1e-9
@@ -278,3 +278,41 @@ def test_solvers_against_numpy(which):
     x, it, res = O.bicgstab(nb, ij, blocks, b, x0, 1e-3)
     r = b.reshape(-1) - dense @ x.reshape(-1)
     assert np.linalg.norm(r) <= 1e-3 * np.linalg.norm(b)
+
+
+def test_block_jacobi_cg_is_preconditioned_cg():
+    """The block-Jacobi CG (beyond the reference) equals a numpy restatement of PCG with
+    M = blockdiag(12x12 diagonal blocks), iterate for iterate, and takes fewer iterations
+    than the diagonal preconditioner when bodies are strongly coupled inside their block."""
+    rng = np.random.default_rng(8)
+    nb = 10
+    ij, blocks, dense = _system(rng, nb)
+    n = 12 * nb
+    # make the 12x12 diagonal blocks strongly non-diagonal (what contacts do to a body)
+    for q in range(nb):
+        C = rng.normal(size=(12, 12))
+        C = 40.0 * (C @ C.T)
+        for ii in range(4):
+            for jj in range(4):
+                blocks[16 * q + ii + 4 * jj] += C[3 * ii:3 * ii + 3, 3 * jj:3 * jj + 3]
+        dense[12 * q:12 * q + 12, 12 * q:12 * q + 12] += C
+    b = rng.normal(size=(4 * nb, 3))
+    x0 = np.zeros_like(b)
+    Minv = np.zeros((n, n))
+    for q in range(nb):
+        Minv[12 * q:12 * q + 12, 12 * q:12 * q + 12] = np.linalg.inv(dense[12 * q:12 * q + 12, 12 * q:12 * q + 12])
+    # numpy PCG with the same stopping rule (tol^2 n on r.z, every 10 iterations)
+    tol, bb = 1e-10, b.reshape(-1)
+    x = np.zeros(n); r = bb - dense @ x; z = Minv @ r; p = z.copy(); rz = r @ z
+    it_ref = 0
+    for i in range(n):
+        Ap = dense @ p; a = rz / (p @ Ap); x += a * p; r -= a * Ap; z = Minv @ r; rz_new = r @ z
+        if (i + 1) % 10 == 0 and rz_new <= tol * tol * n:
+            it_ref = i
+            break
+        p = z + (rz_new / rz) * p; rz = rz_new
+    xo, it, _ = O.bsr_cg_block(nb, ij, blocks, b, x0, tol)
+    assert it == it_ref
+    assert rel_err(xo.reshape(-1), x) <= 1e-9
+    _, it_diag, _ = O.bsr_cg(nb, ij, blocks, b, x0, tol)
+    assert it < it_diag
