@@ -226,9 +226,11 @@ int abd_prim_boxes(abd_stream_t stream, const AbdScene* sc, int32_t kind, const 
                    float* hi) {
     if (!sc || !x || !lo || !hi) return ABD_ERR_ARG;
     hipStream_t s = to_stream(stream);
+    // each vertex read once (24 B) + per primitive its indices and body id and the box written
+    // (24 B): the compulsory traffic; a vertex shared by several primitives is re-read from cache
     const double bytes = kind == ABD_KIND_POINT ? 48.0 * sc->n_verts
-                         : kind == ABD_KIND_EDGE ? (12.0 + 48 + 24) * sc->n_edges
-                                                 : (16.0 + 72 + 24) * sc->n_tris;
+                         : kind == ABD_KIND_EDGE ? 24.0 * sc->n_verts + (12.0 + 24) * sc->n_edges
+                                                 : 24.0 * sc->n_verts + (16.0 + 24) * sc->n_tris;
     const int64_t nprim = kind == ABD_KIND_POINT ? sc->n_verts : kind == ABD_KIND_EDGE ? sc->n_edges : sc->n_tris;
     if (kind < ABD_KIND_POINT || kind > ABD_KIND_TRI) return ABD_ERR_ARG;
     if (nprim == 0) return ABD_OK;

@@ -18,8 +18,10 @@ Solver mode: with the reference's contact Hessian (indefinite: b' < 0 times a
 clamped-PSD matrix) the reference's BiCGSTAB diverges on colliding scenes and
 the swept CCD boxes then cover the scene.  The benchmark therefore runs the
 documented stable mode: contact-space PSD projection of the barrier Hessian
-(psd=1) with the reference's Jacobi-preconditioned CG (bsr_utils.py:126-235).
-The CPU baseline runs the oracle in the same mode on the sampled scenes.
+(psd=1) with the reference's CG (bsr_utils.py:126-235, same iteration and
+stopping rule) preconditioned by each body's 12x12 diagonal block instead of
+its scalar diagonal (--solver 1 restores the diagonal).  The CPU baseline runs
+the oracle in the same mode on the sampled scenes.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload batched|medium|small]
 """
@@ -65,7 +67,7 @@ def make_workload(name: str):
 GH_KERNELS = ("gh_classify", "gh_contact", "gh_pair_reduce", "gh_body_finalize")
 # measured HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy of known
 # size) of the profiled kernels, kept under profiles/ as "<kernel>_<workload>_pmc.md"
-PMC_FILES = {"solve": "cg_kernel"}
+PMC_FILES = {"solve": "cg_kernel", "boxes": "prim_boxes"}
 
 
 def measured_traffic(kernel: str, workload: str):
@@ -149,7 +151,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="batched")
     ap.add_argument("--psd", type=int, default=1)
-    ap.add_argument("--solver", type=int, default=1, help="0 BiCGSTAB, 1 Jacobi CG")
+    ap.add_argument("--solver", type=int, default=2, help="0 BiCGSTAB, 1 Jacobi CG, 2 block-Jacobi CG")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=8, help="batched: scenes timed on the plain path")
@@ -256,7 +258,7 @@ def main():
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": desc, "mode": "batched" if batched else "plain", "psd_projection": bool(a.psd),
                    "driver": "python" if a.python_driver else "native",
-                   "solver": "jacobi-cg" if a.solver == 1 else "bicgstab", "frames_timed": a.steps,
+                   "solver": {0: "bicgstab", 1: "jacobi-cg", 2: "block-jacobi-cg"}[a.solver], "frames_timed": a.steps,
                    "tiles": tiles},
         "gh_pairs_per_s": cands / gh_s if gh_s > 0 else None,
         "gh_candidates": cands,
