@@ -162,6 +162,74 @@ __global__ __launch_bounds__(kBlock) void body_boxes_kernel(const int32_t* __res
     }
 }
 
+// Swept boxes with the collision-set dilation at both ends: the union of a primitive's
+// dilated box at x0 and at x1 (same R3 arithmetic as the boxes above).  Every pair whose
+// dilated boxes overlap at some x0 + t (x1 - x0), t in [0, 1], overlaps here, so the
+// lists culled from them hold every candidate of any collision set along the step.
+__global__ __launch_bounds__(kBlock) void swept_point_boxes_kernel(const double* __restrict__ x0,
+                                                                    const double* __restrict__ x1, int64_t n,
+                                                                    float dil, float* __restrict__ lo,
+                                                                    float* __restrict__ hi) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < 3 * n; i += (int64_t)gridDim.x * kBlock) {
+        const float c0 = (float)x0[i], c1 = (float)x1[i];
+        lo[i] = fminf(c0 - dil, c1 - dil);
+        hi[i] = fmaxf(c0 + dil, c1 + dil);
+    }
+}
+template <int NV>
+__global__ __launch_bounds__(kBlock) void swept_prim_boxes_kernel(const int32_t* __restrict__ prim_body,
+                                                                   const int32_t* __restrict__ voff,
+                                                                   const int32_t* __restrict__ topo,
+                                                                   const double* __restrict__ x0,
+                                                                   const double* __restrict__ x1, int64_t n,
+                                                                   double dil, float* __restrict__ lo,
+                                                                   float* __restrict__ hi) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        int vb = voff[prim_body[i]];
+        int64_t vi[NV];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) vi[j] = vb + topo[NV * i + j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double mn0 = x0[3 * vi[0] + k], mx0 = mn0, mn1 = x1[3 * vi[0] + k], mx1 = mn1;
+#pragma unroll
+            for (int j = 1; j < NV; ++j) {
+                const double c0 = x0[3 * vi[j] + k], c1 = x1[3 * vi[j] + k];
+                mn0 = c0 < mn0 ? c0 : mn0;
+                mx0 = c0 > mx0 ? c0 : mx0;
+                mn1 = c1 < mn1 ? c1 : mn1;
+                mx1 = c1 > mx1 ? c1 : mx1;
+            }
+            lo[3 * i + k] = fminf((float)(mn0 - dil), (float)(mn1 - dil));
+            hi[3 * i + k] = fmaxf((float)(mx0 + dil), (float)(mx1 + dil));
+        }
+    }
+}
+__global__ __launch_bounds__(kBlock) void swept_body_boxes_kernel(const int32_t* __restrict__ voff,
+                                                                   const double* __restrict__ x0,
+                                                                   const double* __restrict__ x1, double dil,
+                                                                   float* __restrict__ lo, float* __restrict__ hi) {
+    __shared__ double sh[kBlock];
+    const int b = blockIdx.x;
+    const int v0 = voff[b], v1 = voff[b + 1];
+    for (int k = 0; k < 3; ++k) {
+        double mn0 = 1.0 / 0.0, mx0 = -1.0 / 0.0, mn1 = 1.0 / 0.0, mx1 = -1.0 / 0.0;
+        for (int v = v0 + threadIdx.x; v < v1; v += kBlock) {
+            const double c0 = x0[3 * (int64_t)v + k], c1 = x1[3 * (int64_t)v + k];
+            mn0 = c0 < mn0 ? c0 : mn0;
+            mx0 = c0 > mx0 ? c0 : mx0;
+            mn1 = c1 < mn1 ? c1 : mn1;
+            mx1 = c1 > mx1 ? c1 : mx1;
+        }
+        const double gmn0 = block_min(mn0, sh), gmx0 = -block_min(-mx0, sh);
+        const double gmn1 = block_min(mn1, sh), gmx1 = -block_min(-mx1, sh);
+        if (threadIdx.x == 0) {
+            lo[3 * b + k] = fminf((float)(gmn0 - dil), (float)(gmn1 - dil));
+            hi[3 * b + k] = fmaxf((float)(gmx0 + dil), (float)(gmx1 + dil));
+        }
+    }
+}
+
 // abd.py:337-353 with the step length of the body's scene
 __global__ void update_q_scenes_kernel(int nb, const int32_t* __restrict__ body_scene, const double* __restrict__ A,
                                        const double* __restrict__ p, const double* __restrict__ dq,
@@ -183,6 +251,31 @@ __global__ void update_q_scenes_kernel(int nb, const int32_t* __restrict__ body_
         const int row = (r - 3) / 3, k = (r - 3) % 3;
         A_out[9 * b + 3 * row + k] = A[9 * b + 3 * row + k] - dq[3 * (4 * b + 1 + row) + k] * al;
     }
+}
+
+int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,
+                float* lo, float* hi) {
+    if (kind == ABD_KIND_POINT) {
+        if (!sc->n_verts) return ABD_OK;
+        hipLaunchKernelGGL(swept_point_boxes_kernel, dim3(grid_for(3 * (int64_t)sc->n_verts)), dim3(kBlock), 0, s, x0,
+                           x1, (int64_t)sc->n_verts, (float)dil, lo, hi);
+    } else if (kind == ABD_KIND_EDGE) {
+        if (!sc->n_edges) return ABD_OK;
+        hipLaunchKernelGGL(swept_prim_boxes_kernel<2>, dim3(grid_for(sc->n_edges)), dim3(kBlock), 0, s,
+                           sc->edge_body, sc->voff, sc->edges, x0, x1, (int64_t)sc->n_edges, dil, lo, hi);
+    } else if (kind == ABD_KIND_TRI) {
+        if (!sc->n_tris) return ABD_OK;
+        hipLaunchKernelGGL(swept_prim_boxes_kernel<3>, dim3(grid_for(sc->n_tris)), dim3(kBlock), 0, s, sc->tri_body,
+                           sc->voff, sc->tris, x0, x1, (int64_t)sc->n_tris, dil, lo, hi);
+    } else if (kind == -1) {
+        if (!sc->n_bodies) return ABD_OK;
+        hipLaunchKernelGGL(swept_body_boxes_kernel, dim3(sc->n_bodies), dim3(kBlock), 0, s, sc->voff, x0, x1, dil, lo,
+                           hi);
+    } else {
+        return ABD_ERR_ARG;
+    }
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
 }
 
 }  // namespace abd

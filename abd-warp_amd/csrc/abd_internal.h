@@ -46,6 +46,10 @@ enum ArenaSlot {
 };
 // Launch shapes that change no result (abd_set_tuning); process-wide.
 AbdTuning tuning();
+// Swept boxes dilated at both ends (kind ABD_KIND_*, or -1 for body boxes): the union of
+// the dilated boxes at x0 and at x1.
+int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,
+                float* lo, float* hi);
 int arena_get(int slot, size_t bytes, void** ptr);
 int arena_host(size_t bytes, void** ptr);  // pinned host memory
 int arena_release_current();

@@ -150,23 +150,23 @@ int collision_set(hipStream_t s, Sim& S, bool structure) {
     return cull_vg_list(s, S, S.body, S.pt);
 }
 
-// abd.py:170-185
-int trajectory_set(hipStream_t s, Sim& S) {
+// abd.py:170-185, with both ends of every box dilated as in collision_set: one candidate
+// set per Newton iteration that holds both the CCD candidates of the step (a pair that
+// collides at some t has overlapping boxes at t, inside these) and every candidate of a
+// collision set at any point of the step (x - alpha dq, alpha in [0, 1]).  The CCD then
+// finds the same time of impact as the reference's trajectory lists (extra pairs have no
+// verified root), and the line search's energies need no re-culling: a pair missing from
+// the collision set at x - alpha dq has boxes apart, hence distance >= dhat and a zero
+// barrier, so only the order of the nonzero terms' partial sums changes.
+int swept_set(hipStream_t s, Sim& S) {
+    const double dh = S.P.dhat * 0.5;
+    const AbdSimState& st = S.st;
     int rc;
-    auto copy = [&](BoxBuf& dst, const BoxBuf& src, size_t n) -> int {
-        ABD_CHECK(hipMemcpyAsync(dst.lo.p, src.lo.p, 12 * n, hipMemcpyDeviceToDevice, s));
-        ABD_CHECK(hipMemcpyAsync(dst.hi.p, src.hi.p, 12 * n, hipMemcpyDeviceToDevice, s));
-        return ABD_OK;
-    };
-    if ((rc = copy(S.tbody, S.body, S.nb))) return rc;
-    if ((rc = abd_body_boxes_traj(s, &S.sc, S.st.xk, S.tbody.lo.p, S.tbody.hi.p))) return rc;
+    if ((rc = swept_boxes(s, &S.sc, -1, st.x, st.xk, dh, S.tbody.lo.p, S.tbody.hi.p))) return rc;
     if ((rc = body_pairs(s, S, S.tbody, false))) return rc;
-    if ((rc = copy(S.ttri, S.tri, S.nf))) return rc;
-    if ((rc = copy(S.tedge, S.edge, S.ne))) return rc;
-    if ((rc = copy(S.tpt, S.pt, S.nv))) return rc;
-    if ((rc = abd_prim_boxes_traj(s, &S.sc, ABD_KIND_TRI, S.st.xk, S.ttri.lo.p, S.ttri.hi.p))) return rc;
-    if ((rc = abd_prim_boxes_traj(s, &S.sc, ABD_KIND_EDGE, S.st.xk, S.tedge.lo.p, S.tedge.hi.p))) return rc;
-    if ((rc = abd_prim_boxes_traj(s, &S.sc, ABD_KIND_POINT, S.st.xk, S.tpt.lo.p, S.tpt.hi.p))) return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, st.x, st.xk, 0.0, S.ttri.lo.p, S.ttri.hi.p))) return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, st.x, st.xk, dh, S.tedge.lo.p, S.tedge.hi.p))) return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, st.x, st.xk, S.P.dhat, S.tpt.lo.p, S.tpt.hi.p))) return rc;
     if ((rc = cull_list(s, S, ABD_CULL_EE, S.tedge, S.tedge, S.eel, &S.nee))) return rc;
     if ((rc = cull_list(s, S, ABD_CULL_PT, S.tpt, S.ttri, S.ptl, &S.npt))) return rc;
     return cull_vg_list(s, S, S.tbody, S.tpt);
@@ -179,6 +179,23 @@ int compute_energy(hipStream_t s, Sim& S, double alpha, double* E) {
     if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, alpha, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
     if ((rc = collision_set(s, S, false))) return rc;
+    if ((rc = abd_contact_energy(s, &S.sc, &S.P, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.scal.p)))
+        return rc;
+    if ((rc = abd_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot, S.scal.p + 1)))
+        return rc;
+    if ((rc = get_scalars(s, S, 0, 2))) return rc;
+    S.stats[5] += 1;
+    *E = S.host[0] + S.host[1];
+    return ABD_OK;
+}
+
+// abd.py:133-141 inside the line search: the energy at q - alpha dq over the swept
+// candidate set of this iteration (swept_set), without re-culling
+int line_energy(hipStream_t s, Sim& S, double alpha, double* E) {
+    int rc;
+    const AbdSimState& st = S.st;
+    if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, alpha, st.Ak, st.pk))) return rc;
+    if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
     if ((rc = abd_contact_energy(s, &S.sc, &S.P, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.scal.p)))
         return rc;
     if ((rc = abd_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot, S.scal.p + 1)))
@@ -203,7 +220,7 @@ int ccd(hipStream_t s, Sim& S, double* alpha_cap) {
     if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, 1.0, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
-    if ((rc = trajectory_set(s, S))) return rc;
+    if ((rc = swept_set(s, S))) return rc;
     S.stats[8] += S.npt;
     S.stats[9] += S.nee;
     if ((rc = abd_toi(s, &S.sc, &S.P, st.x, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.scal.p + 2)))
@@ -257,7 +274,7 @@ int step(hipStream_t s, Sim& S) {
         if (rc) return rc;
         double alpha = alpha_cap, E1;
         for (;;) {
-            if ((rc = compute_energy(s, S, alpha, &E1))) return rc;
+            if ((rc = line_energy(s, S, alpha, &E1))) return rc;
             const bool wolfe = E1 < E0 + P.c1 * alpha * qTg;
             if (wolfe || alpha < P.alpha_min) break;
             alpha *= 0.5;
@@ -302,6 +319,26 @@ int compute_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
     if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
     if ((rc = collision_set(s, S, false))) return rc;
+    if ((rc = abd_scene_contact_energy(s, &S.sc, &S.P, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
+                                       S.sv.p + kSvEc * ns)))
+        return rc;
+    if ((rc = abd_scene_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot,
+                                       S.sv.p + kSvEi * ns)))
+        return rc;
+    ABD_CHECK(hipMemcpyAsync(S.hsv + kSvEc * ns, S.sv.p + kSvEc * ns, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost,
+                             s));
+    ABD_CHECK(hipStreamSynchronize(s));
+    S.stats[5] += 1;
+    return ABD_OK;
+}
+
+// the per-scene energies at q - alpha_s dq over the swept candidate set (line_energy)
+int line_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
+    int rc;
+    const AbdSimState& st = S.st;
+    const int ns = S.ns;
+    if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha, st.Ak, st.pk))) return rc;
+    if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
     if ((rc = abd_scene_contact_energy(s, &S.sc, &S.P, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
                                        S.sv.p + kSvEc * ns)))
         return rc;
@@ -367,7 +404,7 @@ int step_scenes(hipStream_t s, Sim& S) {
         if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dccd, st.Ak, st.pk))) return rc;
         if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
         if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
-        if ((rc = trajectory_set(s, S))) return rc;
+        if ((rc = swept_set(s, S))) return rc;
         S.stats[8] += S.npt;
         S.stats[9] += S.nee;
         if ((rc = abd_scene_toi(s, &S.sc, &P, st.x, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
@@ -387,7 +424,7 @@ int step_scenes(hipStream_t s, Sim& S) {
         // line search, each scene on its own (abd.py:146-160)
         for (int n_ls = n_active; n_ls > 0;) {
             if ((rc = upload(s, dalpha, S.halpha, sizeof(double) * ns))) return rc;
-            if ((rc = compute_energy_scenes(s, S, dalpha))) return rc;
+            if ((rc = line_energy_scenes(s, S, dalpha))) return rc;
             for (int k = 0; k < ns; ++k) {
                 if (!ls[k]) continue;
                 const double e1 = S.hsv[kSvEc * ns + k] + S.hsv[kSvEi * ns + k];
