@@ -74,7 +74,17 @@ struct Sim {
     uint8_t* hmask = nullptr;   // pinned solver mask [ns]
     int32_t* hiters = nullptr;  // pinned solver iterations [ns]
     std::vector<int32_t> scene_it;  // Newton iterations of each scene in the last frame
+    // line-search candidates evaluated together: states, vertices and per-candidate scalars
+    DevBuf<double> cand_A, cand_p, cand_x, cand_E;  // [kLsSpec] x (9 nb | 3 nb | 3 V | 2 ns)
+    DevBuf<double> cand_alpha;                      // [kLsSpec][ns]
+    double* hcand = nullptr;                        // pinned [kLsSpec][ns] alphas, then [kLsSpec][2 ns] energies
 };
+
+// Step lengths evaluated per line-search pass after the first: the trials alpha,
+// alpha/2, ..., alpha/2^(kLsSpec-1) of the backtracking loop (abd.py:146-160), whose
+// accept test depends only on each trial's own energy, so taking the first accepted
+// trial of a pass gives exactly the sequential loop's step.
+constexpr int kLsSpec = 4;
 
 // per-scene value rows of Sim::sv / Sim::hsv
 enum { kSvEc = 0, kSvEi, kSvToi, kSvQtg, kSvGtg, kSvAlpha, kSvAlphaCcd, kSv };
@@ -189,20 +199,29 @@ int compute_energy(hipStream_t s, Sim& S, double alpha, double* E) {
     return ABD_OK;
 }
 
-// abd.py:133-141 inside the line search: the energy at q - alpha dq over the swept
-// candidate set of this iteration (swept_set), without re-culling
-int line_energy(hipStream_t s, Sim& S, double alpha, double* E) {
+// abd.py:133-141 inside the line search, for K trial step lengths at once (energies to
+// E[K], one host round trip): the energy at q - alpha dq over the swept candidate set of
+// this iteration (swept_set), without re-culling
+int line_energy_multi(hipStream_t s, Sim& S, const double* alphas, int K, double* E) {
     int rc;
     const AbdSimState& st = S.st;
-    if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, alpha, st.Ak, st.pk))) return rc;
-    if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-    if ((rc = abd_contact_energy(s, &S.sc, &S.P, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.scal.p)))
-        return rc;
-    if ((rc = abd_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot, S.scal.p + 1)))
-        return rc;
-    if ((rc = get_scalars(s, S, 0, 2))) return rc;
-    S.stats[5] += 1;
-    *E = S.host[0] + S.host[1];
+    const size_t nb = S.nb, nv = S.nv;
+    for (int k = 0; k < K; ++k) {
+        double* Ak = S.cand_A.p + 9 * nb * k;
+        double* pk = S.cand_p.p + 3 * nb * k;
+        double* xk = S.cand_x.p + 3 * nv * k;
+        if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, alphas[k], Ak, pk))) return rc;
+        if ((rc = abd_project_vertices(s, &S.sc, Ak, pk, xk))) return rc;
+        if ((rc = abd_contact_energy(s, &S.sc, &S.P, xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
+                                     S.cand_E.p + 2 * k)))
+            return rc;
+        if ((rc = abd_inertia_energy(s, &S.sc, &S.P, Ak, pk, st.A0, st.p0, st.Adot, st.pdot, S.cand_E.p + 2 * k + 1)))
+            return rc;
+    }
+    ABD_CHECK(hipMemcpyAsync(S.hcand, S.cand_E.p, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, s));
+    ABD_CHECK(hipStreamSynchronize(s));
+    for (int k = 0; k < K; ++k) E[k] = S.hcand[2 * k] + S.hcand[2 * k + 1];
+    S.stats[5] += K;
     return ABD_OK;
 }
 
@@ -272,15 +291,30 @@ int step(hipStream_t s, Sim& S) {
         // abd.py:146-160 line search (Armijo test as written)
         const double qTg = dot_host(s, S, st.dq, st.g, &rc);
         if (rc) return rc;
-        double alpha = alpha_cap, E1;
-        for (;;) {
-            if ((rc = line_energy(s, S, alpha, &E1))) return rc;
-            const bool wolfe = E1 < E0 + P.c1 * alpha * qTg;
-            if (wolfe || alpha < P.alpha_min) break;
-            alpha *= 0.5;
+        double alpha = alpha_cap, E1 = 0.0;
+        for (int pass = 0;; ++pass) {
+            // the first trial alone (usually accepted), then kLsSpec trials per round trip
+            const int K = pass == 0 ? 1 : kLsSpec;
+            double trial[kLsSpec], Ek[kLsSpec];
+            for (int k = 0; k < K; ++k) trial[k] = k == 0 ? alpha : trial[k - 1] * 0.5;
+            if ((rc = line_energy_multi(s, S, trial, K, Ek))) return rc;
+            int acc = -1;
+            for (int k = 0; k < K && acc < 0; ++k) {
+                const bool wolfe = Ek[k] < E0 + P.c1 * trial[k] * qTg;
+                if (wolfe || trial[k] < P.alpha_min) acc = k;
+            }
+            if (acc >= 0) {
+                alpha = trial[acc];
+                E1 = Ek[acc];
+                break;
+            }
+            alpha = trial[K - 1] * 0.5;
         }
         E0 = E1;
         if ((rc = abd_update_q(s, nb, st.A, st.p, st.dq, alpha, st.A, st.p))) return rc;
+        // (Ak, pk) = q - alpha dq, the accepted trial (the same expression as the update above)
+        ABD_CHECK(hipMemcpyAsync(st.Ak, st.A, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+        ABD_CHECK(hipMemcpyAsync(st.pk, st.p, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
         ++it;
         // g.g (abd.py:265) and the solver's iteration count in one read-back
         if ((rc = abd_dot(s, st.g, st.g, 12 * (int64_t)nb, S.scal.p + 3))) return rc;
@@ -332,23 +366,30 @@ int compute_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
     return ABD_OK;
 }
 
-// the per-scene energies at q - alpha_s dq over the swept candidate set (line_energy)
-int line_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
+// line_energy_multi per scene, for K trial step lengths each (dalpha [K][ns] on the device):
+// energies to hcand[kLsSpec * ns + (2 k) ns + s] (contact) and [... + (2 k + 1) ns + s] (inertia)
+int line_energy_scenes_multi(hipStream_t s, Sim& S, const double* dalpha, int K) {
     int rc;
     const AbdSimState& st = S.st;
     const int ns = S.ns;
-    if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha, st.Ak, st.pk))) return rc;
-    if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-    if ((rc = abd_scene_contact_energy(s, &S.sc, &S.P, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
-                                       S.sv.p + kSvEc * ns)))
-        return rc;
-    if ((rc = abd_scene_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot,
-                                       S.sv.p + kSvEi * ns)))
-        return rc;
-    ABD_CHECK(hipMemcpyAsync(S.hsv + kSvEc * ns, S.sv.p + kSvEc * ns, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost,
-                             s));
+    const size_t nb = S.nb, nv = S.nv;
+    for (int k = 0; k < K; ++k) {
+        double* Ak = S.cand_A.p + 9 * nb * k;
+        double* pk = S.cand_p.p + 3 * nb * k;
+        double* xk = S.cand_x.p + 3 * nv * k;
+        if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha + (size_t)ns * k, Ak, pk))) return rc;
+        if ((rc = abd_project_vertices(s, &S.sc, Ak, pk, xk))) return rc;
+        if ((rc = abd_scene_contact_energy(s, &S.sc, &S.P, xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
+                                           S.cand_E.p + (size_t)2 * ns * k)))
+            return rc;
+        if ((rc = abd_scene_inertia_energy(s, &S.sc, &S.P, Ak, pk, st.A0, st.p0, st.Adot, st.pdot,
+                                           S.cand_E.p + (size_t)2 * ns * k + ns)))
+            return rc;
+    }
+    ABD_CHECK(hipMemcpyAsync(S.hcand + (size_t)kLsSpec * ns, S.cand_E.p, sizeof(double) * 2 * ns * K,
+                             hipMemcpyDeviceToHost, s));
     ABD_CHECK(hipStreamSynchronize(s));
-    S.stats[5] += 1;
+    S.stats[5] += K;
     return ABD_OK;
 }
 
@@ -421,27 +462,47 @@ int step_scenes(hipStream_t s, Sim& S) {
             if (t < 1.0) t *= 0.9;
             S.halpha[k] = newton[k] ? t : 0.0;
         }
-        // line search, each scene on its own (abd.py:146-160)
-        for (int n_ls = n_active; n_ls > 0;) {
-            if ((rc = upload(s, dalpha, S.halpha, sizeof(double) * ns))) return rc;
-            if ((rc = line_energy_scenes(s, S, dalpha))) return rc;
-            for (int k = 0; k < ns; ++k) {
-                if (!ls[k]) continue;
-                const double e1 = S.hsv[kSvEc * ns + k] + S.hsv[kSvEi * ns + k];
-                const double qTg = S.hsv[kSvQtg * ns + k];
-                const bool wolfe = e1 < E0[k] + P.c1 * S.halpha[k] * qTg;
-                if (wolfe || S.halpha[k] < P.alpha_min) {
-                    ls[k] = 0;
-                    E1[k] = e1;
+        // line search, each scene on its own (abd.py:146-160): the first trial, then kLsSpec
+        // trials (alpha, alpha/2, ...) per pass for the scenes still searching
+        for (int n_ls = n_active, pass = 0; n_ls > 0; ++pass) {
+            const int K = pass == 0 ? 1 : kLsSpec;
+            double* ha = S.hcand;  // [K][ns] trial step lengths (0: the scene does not move)
+            for (int k = 0; k < K; ++k)
+                for (int q = 0; q < ns; ++q)
+                    ha[(size_t)k * ns + q] = !ls[q] ? 0.0 : (k == 0 ? S.halpha[q] : ha[(size_t)(k - 1) * ns + q] * 0.5);
+            if ((rc = upload(s, S.cand_alpha.p, ha, sizeof(double) * ns * K))) return rc;
+            if ((rc = line_energy_scenes_multi(s, S, S.cand_alpha.p, K))) return rc;
+            const double* he = S.hcand + (size_t)kLsSpec * ns;
+            for (int q = 0; q < ns; ++q) {
+                if (!ls[q]) continue;
+                const double qTg = S.hsv[kSvQtg * ns + q];
+                int acc = -1;
+                for (int k = 0; k < K && acc < 0; ++k) {
+                    const double a = ha[(size_t)k * ns + q];
+                    const double e1 = he[(size_t)2 * ns * k + q] + he[(size_t)2 * ns * k + ns + q];
+                    const bool wolfe = e1 < E0[q] + P.c1 * a * qTg;
+                    if (wolfe || a < P.alpha_min) {
+                        acc = k;
+                        S.halpha[q] = a;
+                        E1[q] = e1;
+                    }
+                }
+                if (acc >= 0) {
+                    ls[q] = 0;
                     --n_ls;
                 } else {
-                    S.halpha[k] *= 0.5;
+                    S.halpha[q] = ha[(size_t)(K - 1) * ns + q] * 0.5;
                 }
             }
         }
         // halpha holds the accepted step of every iterating scene and 0 elsewhere
+        for (int q = 0; q < ns; ++q)
+            if (!newton[q]) S.halpha[q] = 0.0;
         if ((rc = upload(s, dalpha, S.halpha, sizeof(double) * ns))) return rc;
         if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha, st.A, st.p))) return rc;
+        // (Ak, pk) = the accepted trial of every scene (the same expression as the update above)
+        ABD_CHECK(hipMemcpyAsync(st.Ak, st.A, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+        ABD_CHECK(hipMemcpyAsync(st.pk, st.p, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
         if ((rc = abd_scene_dot(s, &S.sc, st.g, st.g, S.sv.p + kSvGtg * ns))) return rc;
         ABD_CHECK(hipMemcpyAsync(S.hiters, S.iters.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
         ABD_CHECK(hipMemcpyAsync(S.hsv + kSvGtg * ns, S.sv.p + kSvGtg * ns, sizeof(double) * ns,
@@ -507,6 +568,17 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
     if (!rc) rc = S->scal.reserve(8);
     if (!rc) rc = S->iters.reserve(4);
     if (!rc && hipHostMalloc((void**)&S->host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = ABD_ERR_ARG;
+    {
+        const int nsc = scene->n_scenes > 1 ? scene->n_scenes : 1;
+        if (!rc) rc = S->cand_A.reserve(9 * nb * kLsSpec);
+        if (!rc) rc = S->cand_p.reserve(3 * nb * kLsSpec);
+        if (!rc) rc = S->cand_x.reserve(3 * (size_t)S->nv * kLsSpec);
+        if (!rc) rc = S->cand_E.reserve(2 * (size_t)nsc * kLsSpec);
+        if (!rc) rc = S->cand_alpha.reserve((size_t)nsc * kLsSpec);
+        if (!rc && hipHostMalloc((void**)&S->hcand, sizeof(double) * 3 * nsc * kLsSpec, hipHostMallocDefault) !=
+                       hipSuccess)
+            rc = ABD_ERR_ARG;
+    }
     if (!rc && scene->n_scenes > 1) {
         // scene ranges from the host copy of scene_boff
         const int ns = scene->n_scenes;
@@ -653,6 +725,12 @@ int abd_sim_destroy(void* sim) {
     S->iters.release();
     S->sv.release();
     S->sactive.release();
+    S->cand_A.release();
+    S->cand_p.release();
+    S->cand_x.release();
+    S->cand_E.release();
+    S->cand_alpha.release();
+    if (S->hcand) hipHostFree(S->hcand);
     for (void* h : {(void*)S->host, (void*)S->hsv, (void*)S->halpha, (void*)S->hccd, (void*)S->hmask,
                     (void*)S->hiters})
         if (h) hipHostFree(h);
