@@ -21,6 +21,8 @@
 //            + its vertex-ground terms.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "abd_block.h"
 #include "abd_device.h"
 #include "abd_inertia.h"
@@ -594,6 +596,7 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
     ABD_CHECK_LAUNCH();
     int64_t nact = 0;
     int32_t* act = nullptr;
+    const int64_t* dev_nact = nullptr;
     if (n + nvg > 0) {
         uint8_t* flag;
         if ((rc = arena_get(kSlotFlags, (size_t)n + 1, (void**)&flag))) return rc;
@@ -615,12 +618,19 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
             hipLaunchKernelGGL(compact_flags_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s,
                                (const uint8_t*)flag, n, bcnt, (const int64_t*)boff, act);
             ABD_CHECK_LAUNCH();
-            if ((rc = read_i64(s, boff + nblk, &nact))) return rc;
+            dev_nact = boff + nblk;
         }
     }
-    int herr = 0;
-    ABD_CHECK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    // the active count and the ordering flag in one read-back
+    int64_t* hv;
+    if ((rc = arena_host(2 * sizeof(int64_t), (void**)&hv))) return rc;
+    hv[0] = 0;
+    if (dev_nact) ABD_CHECK(hipMemcpyAsync(hv, dev_nact, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    ABD_CHECK(hipMemcpyAsync(hv + 1, err, sizeof(int), hipMemcpyDeviceToHost, s));
     ABD_CHECK(hipStreamSynchronize(s));
+    nact = hv[0];
+    int herr = 0;
+    memcpy(&herr, hv + 1, sizeof(int));
     if (herr) return ABD_ERR_UNSORTED;
     double* pbuf = nullptr;
     if (nij > 0) {

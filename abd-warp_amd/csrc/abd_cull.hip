@@ -142,12 +142,6 @@ __global__ __launch_bounds__(kBlock) void hull_kernel(const int32_t* __restrict_
     }
 }
 
-struct PrimSets {
-    const int32_t* off1;  // per-body offsets of set 1 (PT points / EE edges)
-    const int32_t* off2;  // set 2 (PT triangles / EE edges)
-    const float *lo1, *hi1, *lo2, *hi2;
-    const float *hlo1, *hhi1, *hlo2, *hhi2;  // per-body hulls
-};
 
 // unit -> (ii, I (set-1 body), J (set-2 body))
 __device__ __forceinline__ void unit_bodies(int pt, int64_t u, const int32_t* __restrict__ ij, int64_t& ii, int& I,
@@ -379,6 +373,133 @@ __global__ void rowfill_kernel(int nb, int64_t nij, const int32_t* __restrict__ 
     }
 }
 
+// ------------------------------------------------------- staged culling
+// The primitive-pair culling in three stages that leave their counts on the device,
+// so that a caller can run several cullings side by side and read all their counts in
+// one host synchronization.  `set` selects one of two independent workspaces.
+namespace {
+const int kCullSlots[2][8] = {
+    {kSlotHull1, kSlotHull2, kSlotFilter, kSlotFilterCount, kSlotScanA, kSlotScanB, kSlotChunkCount, kSlotChunk},
+    {kSlotHull1B, kSlotHull2B, kSlotFilterB, kSlotFilterCountB, kSlotScanAB, kSlotScanBB, kSlotChunkCountB,
+     kSlotChunkB}};
+}  // namespace
+
+int prim_cull_begin(hipStream_t s, const AbdScene* sc, int mode, const int32_t* ij, int64_t nij, const float* lo1,
+                    const float* hi1, const float* lo2, const float* hi2, int set, PrimCull& C) {
+    const int* slot = kCullSlots[set];
+    C.set = set;
+    C.pt = mode == ABD_CULL_PT;
+    C.ij = ij;
+    const int nb = sc->n_bodies;
+    PrimSets& S = C.S;
+    S.off1 = C.pt ? sc->voff : sc->eoff;
+    S.off2 = C.pt ? sc->foff : sc->eoff;
+    S.lo1 = lo1;
+    S.hi1 = hi1;
+    S.lo2 = lo2;
+    S.hi2 = hi2;
+    const int64_t max1 = C.pt ? sc->max_verts : sc->max_edges;
+    const int64_t max2 = C.pt ? sc->max_tris : sc->max_edges;
+    C.maxs = max1 > max2 ? max1 : max2;
+    if (C.maxs > 65535) return ABD_ERR_ARG;  // n1 * n2 tests per body pair must fit in 32 bits
+    C.nunits = C.pt ? 2 * nij : nij;
+    if (2 * C.nunits > INT32_MAX) return ABD_ERR_ARG;
+    int rc;
+    float *h1, *h2;
+    if ((rc = arena_get(slot[0], sizeof(float) * 6 * (size_t)nb, (void**)&h1))) return rc;
+    if ((rc = arena_get(slot[1], sizeof(float) * 6 * (size_t)nb, (void**)&h2))) return rc;
+    S.hlo1 = h1;
+    S.hhi1 = h1 + 3 * nb;
+    S.hlo2 = h2;
+    S.hhi2 = h2 + 3 * nb;
+    hipLaunchKernelGGL(hull_kernel, dim3(nb), dim3(kBlock), 0, s, S.off1, lo1, hi1, h1, h1 + 3 * nb);
+    ABD_CHECK_LAUNCH();
+    hipLaunchKernelGGL(hull_kernel, dim3(nb), dim3(kBlock), 0, s, S.off2, lo2, hi2, h2, h2 + 3 * nb);
+    ABD_CHECK_LAUNCH();
+    if ((rc = arena_get(slot[2], sizeof(int32_t) * (size_t)(2 * C.nunits * C.maxs), (void**)&C.flist))) return rc;
+    if ((rc = arena_get(slot[3], sizeof(int64_t) * (size_t)(2 * C.nunits), (void**)&C.fcnt))) return rc;
+    if ((rc = arena_get(slot[4], sizeof(int64_t) * (size_t)(C.nunits + 1), (void**)&C.nchunk))) return rc;
+    if ((rc = arena_get(slot[5], sizeof(int64_t) * (size_t)(C.nunits + 1), (void**)&C.choff))) return rc;
+    int ph = prof_start(kProfFilter, s, 0.0);
+    hipLaunchKernelGGL(filter_kernel, dim3((unsigned)(2 * C.nunits)), dim3(kBlock), 0, s, C.pt, ij, S, C.maxs,
+                       C.flist, C.fcnt);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    C.tune = tuning();
+    hipLaunchKernelGGL(unit_chunks_kernel, dim3(grid_for(C.nunits)), dim3(kBlock), 0, s, C.nunits, C.tune.cull_chunk,
+                       C.fcnt, C.nchunk);
+    ABD_CHECK_LAUNCH();
+    if ((rc = scan_exclusive_i64(s, C.nchunk, C.choff, C.nunits, C.choff + C.nunits))) return rc;
+    C.total_chunks = C.choff + C.nunits;
+    C.total_pairs = nullptr;
+    return ABD_OK;
+}
+
+int prim_cull_count(hipStream_t s, PrimCull& C, int64_t T) {
+    const int* slot = kCullSlots[C.set];
+    C.T = T;
+    if (T == 0) return ABD_OK;
+    int rc;
+    if ((rc = arena_get(slot[6], sizeof(int64_t) * (size_t)(T + 1), (void**)&C.ccnt))) return rc;
+    if ((rc = arena_get(slot[7], sizeof(int64_t) * (size_t)(T + 1), (void**)&C.ccoff))) return rc;
+    C.grid = (unsigned)(T < C.tune.cull_grid ? T : C.tune.cull_grid);
+    int ph = prof_start(kProfPrimCount, s, 0.0);
+    hipLaunchKernelGGL(prim_pairs_kernel<false>, dim3(C.grid), dim3(kBlock), 0, s, C.pt, C.tune.cull_chunk, C.ij, C.S,
+                       C.maxs, (const int32_t*)C.flist, (const int64_t*)C.fcnt, (const int64_t*)C.choff, C.nunits,
+                       (const int64_t*)C.total_chunks, C.ccnt, (const int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    if ((rc = scan_exclusive_i64(s, C.ccnt, C.ccoff, T, C.ccoff + T))) return rc;
+    C.total_pairs = C.ccoff + T;
+    return ABD_OK;
+}
+
+int prim_cull_fill(hipStream_t s, PrimCull& C, int32_t* out, int64_t cap, int64_t n) {
+    if (C.T == 0 || n == 0) return ABD_OK;
+    int ph = prof_start(kProfPrimFill, s, 20.0 * n);
+    hipLaunchKernelGGL(prim_pairs_kernel<true>, dim3(C.grid), dim3(kBlock), 0, s, C.pt, C.tune.cull_chunk, C.ij, C.S,
+                       C.maxs, (const int32_t*)C.flist, (const int64_t*)C.fcnt, (const int64_t*)C.choff, C.nunits,
+                       (const int64_t*)C.total_chunks, C.ccnt, (const int64_t*)C.ccoff, out, cap);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    return ABD_OK;
+}
+
+int vg_cull_begin(hipStream_t s, const AbdScene* sc, const float* blo, const float* plo, const float* phi, double dhat,
+                  double ground, VgCull& C) {
+    C.sc = sc;
+    C.blo = blo;
+    C.plo = plo;
+    C.phi = phi;
+    C.yth = 0.5 * dhat + ground;
+    C.fg = (float)ground;
+    const int64_t nv = sc->n_verts;
+    C.nblk = ceil_div(nv, (int64_t)kBlock * kItems);
+    C.total = nullptr;
+    if (nv == 0) return ABD_OK;
+    int rc;
+    if ((rc = arena_get(kSlotVgA, sizeof(int64_t) * (C.nblk + 1), (void**)&C.bcnt))) return rc;
+    if ((rc = arena_get(kSlotVgB, sizeof(int64_t) * (C.nblk + 1), (void**)&C.boff))) return rc;
+    int ph = prof_start(kProfVg, s, 28.0 * nv);
+    hipLaunchKernelGGL(vg_kernel<false>, dim3((unsigned)C.nblk), dim3(kBlock), 0, s, nv, sc->vert_body, sc->voff, blo,
+                       plo, phi, C.yth, C.fg, C.bcnt, (const int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    if ((rc = scan_exclusive_i64(s, C.bcnt, C.boff, C.nblk, C.boff + C.nblk))) return rc;
+    C.total = C.boff + C.nblk;
+    return ABD_OK;
+}
+
+int vg_cull_fill(hipStream_t s, VgCull& C, int32_t* out, int64_t cap, int64_t n) {
+    if (n == 0 || !C.total) return ABD_OK;
+    const AbdScene* sc = C.sc;
+    hipLaunchKernelGGL(vg_kernel<true>, dim3((unsigned)C.nblk), dim3(kBlock), 0, s, (int64_t)sc->n_verts,
+                       sc->vert_body, sc->voff, C.blo, C.plo, C.phi, C.yth, C.fg, C.bcnt, (const int64_t*)C.boff, out,
+                       cap);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
 }  // namespace abd
 
 using namespace abd;
@@ -440,75 +561,19 @@ int abd_cull_prims(abd_stream_t stream, const AbdScene* sc, int32_t mode, const 
     *count = 0;
     if (nij == 0) return ABD_OK;
     hipStream_t s = to_stream(stream);
-    const int pt = mode == ABD_CULL_PT;
-    const int nb = sc->n_bodies;
-    PrimSets S;
-    S.off1 = pt ? sc->voff : sc->eoff;
-    S.off2 = pt ? sc->foff : sc->eoff;
-    S.lo1 = lo1;
-    S.hi1 = hi1;
-    S.lo2 = lo2;
-    S.hi2 = hi2;
-    const int64_t max1 = pt ? sc->max_verts : sc->max_edges;
-    const int64_t max2 = pt ? sc->max_tris : sc->max_edges;
-    const int64_t maxs = max1 > max2 ? max1 : max2;
-    if (maxs > 65535) return ABD_ERR_ARG;  // n1 * n2 tests per body pair must fit in 32 bits
-    const int64_t nunits = pt ? 2 * nij : nij;
-    int rc;
-    float *h1, *h2;
-    if ((rc = arena_get(kSlotHull1, sizeof(float) * 6 * (size_t)nb, (void**)&h1))) return rc;
-    if ((rc = arena_get(kSlotHull2, sizeof(float) * 6 * (size_t)nb, (void**)&h2))) return rc;
-    S.hlo1 = h1;
-    S.hhi1 = h1 + 3 * nb;
-    S.hlo2 = h2;
-    S.hhi2 = h2 + 3 * nb;
-    hipLaunchKernelGGL(hull_kernel, dim3(nb), dim3(kBlock), 0, s, S.off1, lo1, hi1, h1, h1 + 3 * nb);
-    ABD_CHECK_LAUNCH();
-    hipLaunchKernelGGL(hull_kernel, dim3(nb), dim3(kBlock), 0, s, S.off2, lo2, hi2, h2, h2 + 3 * nb);
-    ABD_CHECK_LAUNCH();
-    int32_t* flist;
-    int64_t *fcnt, *nchunk, *choff;
-    if ((rc = arena_get(kSlotFilter, sizeof(int32_t) * (size_t)(2 * nunits * maxs), (void**)&flist))) return rc;
-    if ((rc = arena_get(kSlotFilterCount, sizeof(int64_t) * (size_t)(2 * nunits), (void**)&fcnt))) return rc;
-    if ((rc = arena_get(kSlotScanA, sizeof(int64_t) * (size_t)(nunits + 1), (void**)&nchunk))) return rc;
-    if ((rc = arena_get(kSlotScanB, sizeof(int64_t) * (size_t)(nunits + 1), (void**)&choff))) return rc;
-    if (2 * nunits > INT32_MAX) return ABD_ERR_ARG;
-    int ph = prof_start(kProfFilter, s, 0.0);
-    hipLaunchKernelGGL(filter_kernel, dim3((unsigned)(2 * nunits)), dim3(kBlock), 0, s, pt, ij, S, maxs, flist, fcnt);
-    ABD_CHECK_LAUNCH();
-    prof_stop(ph, s);
-    const AbdTuning tune = tuning();
-    hipLaunchKernelGGL(unit_chunks_kernel, dim3(grid_for(nunits)), dim3(kBlock), 0, s, nunits, tune.cull_chunk, fcnt,
-                       nchunk);
-    ABD_CHECK_LAUNCH();
-    if ((rc = scan_exclusive_i64(s, nchunk, choff, nunits, choff + nunits))) return rc;
+    PrimCull C;
+    int rc = prim_cull_begin(s, sc, mode, ij, nij, lo1, hi1, lo2, hi2, 0, C);
+    if (rc) return rc;
     int64_t T;
-    if ((rc = read_i64(s, choff + nunits, &T))) return rc;
+    if ((rc = read_i64(s, C.total_chunks, &T))) return rc;
+    if ((rc = prim_cull_count(s, C, T))) return rc;
     if (T == 0) return ABD_OK;
-    int64_t *ccnt, *ccoff;
-    if ((rc = arena_get(kSlotChunkCount, sizeof(int64_t) * (size_t)(T + 1), (void**)&ccnt))) return rc;
-    if ((rc = arena_get(kSlotChunk, sizeof(int64_t) * (size_t)(T + 1), (void**)&ccoff))) return rc;
-    const unsigned grid = (unsigned)(T < tune.cull_grid ? T : tune.cull_grid);
-    ph = prof_start(kProfPrimCount, s, 0.0);
-    hipLaunchKernelGGL(prim_pairs_kernel<false>, dim3(grid), dim3(kBlock), 0, s, pt, tune.cull_chunk, ij, S, maxs,
-                       (const int32_t*)flist, (const int64_t*)fcnt, (const int64_t*)choff, nunits,
-                       (const int64_t*)(choff + nunits), ccnt, (const int64_t*)nullptr, (int32_t*)nullptr,
-                       (int64_t)0);
-    ABD_CHECK_LAUNCH();
-    prof_stop(ph, s);
-    if ((rc = scan_exclusive_i64(s, ccnt, ccoff, T, ccoff + T))) return rc;
     int64_t n;
-    if ((rc = read_i64(s, ccoff + T, &n))) return rc;
+    if ((rc = read_i64(s, C.total_pairs, &n))) return rc;
     *count = n;
     if (n > cap) return ABD_ERR_OVERFLOW;
     if (n == 0) return ABD_OK;
-    ph = prof_start(kProfPrimFill, s, 20.0 * n);
-    hipLaunchKernelGGL(prim_pairs_kernel<true>, dim3(grid), dim3(kBlock), 0, s, pt, tune.cull_chunk, ij, S, maxs,
-                       (const int32_t*)flist, (const int64_t*)fcnt, (const int64_t*)choff, nunits,
-                       (const int64_t*)(choff + nunits), ccnt, (const int64_t*)ccoff, out, cap);
-    ABD_CHECK_LAUNCH();
-    prof_stop(ph, s);
-    return ABD_OK;
+    return prim_cull_fill(s, C, out, cap, n);
 }
 
 int abd_cull_vg(abd_stream_t stream, const AbdScene* sc, const float* blo, const float* plo, const float* phi,
@@ -516,30 +581,15 @@ int abd_cull_vg(abd_stream_t stream, const AbdScene* sc, const float* blo, const
     if (!sc || !blo || !plo || !phi || !count || cap < 0 || (cap > 0 && !out)) return ABD_ERR_ARG;
     hipStream_t s = to_stream(stream);
     *count = 0;
-    const int64_t nv = sc->n_verts;
-    if (nv == 0) return ABD_OK;
-    const int64_t nblk = ceil_div(nv, (int64_t)kBlock * kItems);
-    int64_t *bcnt, *boff;
-    int rc;
-    if ((rc = arena_get(kSlotScanA, sizeof(int64_t) * (nblk + 1), (void**)&bcnt))) return rc;
-    if ((rc = arena_get(kSlotScanB, sizeof(int64_t) * (nblk + 1), (void**)&boff))) return rc;
-    const double yth = 0.5 * dhat + ground;
-    const float fg = (float)ground;
-    int ph = prof_start(kProfVg, s, 28.0 * nv);
-    hipLaunchKernelGGL(vg_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nv, sc->vert_body, sc->voff, blo,
-                       plo, phi, yth, fg, bcnt, (const int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
-    ABD_CHECK_LAUNCH();
-    prof_stop(ph, s);
-    if ((rc = scan_exclusive_i64(s, bcnt, boff, nblk, boff + nblk))) return rc;
+    if (sc->n_verts == 0) return ABD_OK;
+    VgCull C;
+    int rc = vg_cull_begin(s, sc, blo, plo, phi, dhat, ground, C);
+    if (rc) return rc;
     int64_t n;
-    if ((rc = read_i64(s, boff + nblk, &n))) return rc;
+    if ((rc = read_i64(s, C.total, &n))) return rc;
     *count = n;
     if (n > cap) return ABD_ERR_OVERFLOW;
-    if (n == 0) return ABD_OK;
-    hipLaunchKernelGGL(vg_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nv, sc->vert_body, sc->voff, blo,
-                       plo, phi, yth, fg, bcnt, (const int64_t*)boff, out, cap);
-    ABD_CHECK_LAUNCH();
-    return ABD_OK;
+    return vg_cull_fill(s, C, out, cap, n);
 }
 
 int abd_set_triplets(abd_stream_t stream, int32_t nb, const int32_t* ij, int64_t nij, int32_t* rows,

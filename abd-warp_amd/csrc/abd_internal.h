@@ -22,7 +22,7 @@ inline hipStream_t to_stream(abd_stream_t s) { return reinterpret_cast<hipStream
 // Per-device scratch arena: `slots` independent growable device buffers plus a
 // small pinned host buffer for count read-back.  Growing synchronizes the
 // device first, so a buffer still in use by queued work is never freed early.
-constexpr int kArenaSlots = 19;
+constexpr int kArenaSlots = 29;
 enum ArenaSlot {
     kSlotScanA = 0,
     kSlotScanB,
@@ -43,9 +43,58 @@ enum ArenaSlot {
     kSlotSolveBytes,
     kSlotPlan,
     kSlotPrecond,
+    // a second primitive-culling workspace (culls run side by side) and the VG culling's
+    kSlotHull1B,
+    kSlotHull2B,
+    kSlotFilterB,
+    kSlotFilterCountB,
+    kSlotScanAB,
+    kSlotScanBB,
+    kSlotChunkCountB,
+    kSlotChunkB,
+    kSlotVgA,
+    kSlotVgB,
 };
 // Launch shapes that change no result (abd_set_tuning); process-wide.
 AbdTuning tuning();
+// Staged culling (abd_cull.hip): begin / count / fill leave their counts on the device
+// (total_chunks, total_pairs, total) so several cullings share one host synchronization;
+// `set` 0 or 1 picks one of two independent workspaces.
+struct PrimSets {
+    const int32_t* off1;  // per-body offsets of set 1 (PT points / EE edges)
+    const int32_t* off2;  // set 2 (PT triangles / EE edges)
+    const float *lo1, *hi1, *lo2, *hi2;
+    const float *hlo1, *hhi1, *hlo2, *hhi2;  // per-body hulls
+};
+struct PrimCull {
+    int set = 0, pt = 0;
+    const int32_t* ij = nullptr;
+    PrimSets S{};
+    int64_t nunits = 0, maxs = 0, T = 0;
+    int32_t* flist = nullptr;
+    int64_t *fcnt = nullptr, *nchunk = nullptr, *choff = nullptr, *ccnt = nullptr, *ccoff = nullptr;
+    const int64_t* total_chunks = nullptr;  // device, after begin
+    const int64_t* total_pairs = nullptr;   // device, after count
+    AbdTuning tune{};
+    unsigned grid = 0;
+};
+struct VgCull {
+    const AbdScene* sc = nullptr;
+    const float *blo = nullptr, *plo = nullptr, *phi = nullptr;
+    double yth = 0.0;
+    float fg = 0.f;
+    int64_t nblk = 0;
+    int64_t *bcnt = nullptr, *boff = nullptr;
+    const int64_t* total = nullptr;  // device, after begin
+};
+int prim_cull_begin(hipStream_t s, const AbdScene* sc, int mode, const int32_t* ij, int64_t nij, const float* lo1,
+                    const float* hi1, const float* lo2, const float* hi2, int set, PrimCull& C);
+int prim_cull_count(hipStream_t s, PrimCull& C, int64_t T);
+int prim_cull_fill(hipStream_t s, PrimCull& C, int32_t* out, int64_t cap, int64_t n);
+int vg_cull_begin(hipStream_t s, const AbdScene* sc, const float* blo, const float* plo, const float* phi, double dhat,
+                  double ground, VgCull& C);
+int vg_cull_fill(hipStream_t s, VgCull& C, int32_t* out, int64_t cap, int64_t n);
+
 // Swept boxes dilated at both ends (kind ABD_KIND_*, or -1 for body boxes): the union of
 // the dilated boxes at x0 and at x1.
 int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,

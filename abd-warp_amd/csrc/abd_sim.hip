@@ -62,7 +62,8 @@ struct Sim {
     DevBuf<double> blocks;
     DevBuf<double> scal;  // [0] E contact, [1] E inertia, [2] toi, [3] dot, [4] resid
     DevBuf<int32_t> iters;
-    double* host = nullptr;  // pinned scalars
+    double* host = nullptr;     // pinned scalars
+    int64_t* hcount = nullptr;  // pinned list counts
     int64_t stats[10] = {};
     // independent scenes (n_scenes > 1): per-scene device values and pinned host copies
     int ns = 1, max_scene_bodies = 0;
@@ -95,35 +96,49 @@ int get_scalars(hipStream_t s, Sim& S, int first, int count) {
     return ABD_OK;
 }
 
-int cull_list(hipStream_t s, Sim& S, int mode, const BoxBuf& b1, const BoxBuf& b2, DevBuf<int32_t>& out,
-              int64_t* n) {
-    for (;;) {
-        int64_t cnt = 0;
-        int rc = abd_cull_prims(s, &S.sc, mode, S.ij.p, S.nij, b1.lo.p, b1.hi.p, b2.lo.p, b2.hi.p, out.p,
-                                (int64_t)(out.cap / 5), &cnt);
-        if (rc == ABD_ERR_OVERFLOW) {
-            if ((rc = out.reserve(5 * (size_t)cnt))) return rc;
-            continue;
-        }
-        if (rc) return rc;
-        *n = cnt;
-        return ABD_OK;
+// The EE, PT and VG candidate lists of one set of boxes (culling.py:32-76), culled side
+// by side so that their counts come back in two host synchronizations instead of five.
+int cull_lists(hipStream_t s, Sim& S, const BoxBuf& body, const BoxBuf& pts, const BoxBuf& tris,
+               const BoxBuf& edges) {
+    PrimCull ce, cp;
+    VgCull cv;
+    const bool pairs = S.nij > 0;
+    int rc;
+    if (pairs) {
+        if ((rc = prim_cull_begin(s, &S.sc, ABD_CULL_EE, S.ij.p, S.nij, edges.lo.p, edges.hi.p, edges.lo.p,
+                                  edges.hi.p, 0, ce)))
+            return rc;
+        if ((rc = prim_cull_begin(s, &S.sc, ABD_CULL_PT, S.ij.p, S.nij, pts.lo.p, pts.hi.p, tris.lo.p, tris.hi.p, 1,
+                                  cp)))
+            return rc;
     }
-}
-
-int cull_vg_list(hipStream_t s, Sim& S, const BoxBuf& body, const BoxBuf& pts) {
-    for (;;) {
-        int64_t cnt = 0;
-        int rc = abd_cull_vg(s, &S.sc, body.lo.p, pts.lo.p, pts.hi.p, S.P.dhat, S.P.ground, S.vgl.p,
-                             (int64_t)(S.vgl.cap / 2), &cnt);
-        if (rc == ABD_ERR_OVERFLOW) {
-            if ((rc = S.vgl.reserve(2 * (size_t)cnt))) return rc;
-            continue;
-        }
-        if (rc) return rc;
-        S.nvg = cnt;
-        return ABD_OK;
+    if ((rc = vg_cull_begin(s, &S.sc, body.lo.p, pts.lo.p, pts.hi.p, S.P.dhat, S.P.ground, cv))) return rc;
+    int64_t* hv = S.hcount;
+    hv[0] = hv[1] = hv[2] = 0;
+    if (pairs) {
+        ABD_CHECK(hipMemcpyAsync(hv + 0, ce.total_chunks, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipMemcpyAsync(hv + 1, cp.total_chunks, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     }
+    if (cv.total) ABD_CHECK(hipMemcpyAsync(hv + 2, cv.total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    ABD_CHECK(hipStreamSynchronize(s));
+    const int64_t T_ee = hv[0], T_pt = hv[1];
+    S.nvg = hv[2];
+    if ((rc = S.vgl.reserve(2 * (size_t)S.nvg + 2))) return rc;
+    if ((rc = vg_cull_fill(s, cv, S.vgl.p, (int64_t)(S.vgl.cap / 2), S.nvg))) return rc;
+    S.nee = S.npt = 0;
+    if (!pairs) return ABD_OK;
+    if ((rc = prim_cull_count(s, ce, T_ee))) return rc;
+    if ((rc = prim_cull_count(s, cp, T_pt))) return rc;
+    hv[3] = hv[4] = 0;
+    if (T_ee) ABD_CHECK(hipMemcpyAsync(hv + 3, ce.total_pairs, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (T_pt) ABD_CHECK(hipMemcpyAsync(hv + 4, cp.total_pairs, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (T_ee || T_pt) ABD_CHECK(hipStreamSynchronize(s));
+    S.nee = hv[3];
+    S.npt = hv[4];
+    if ((rc = S.eel.reserve(5 * (size_t)S.nee + 5))) return rc;
+    if ((rc = S.ptl.reserve(5 * (size_t)S.npt + 5))) return rc;
+    if ((rc = prim_cull_fill(s, ce, S.eel.p, (int64_t)(S.eel.cap / 5), S.nee))) return rc;
+    return prim_cull_fill(s, cp, S.ptl.p, (int64_t)(S.ptl.cap / 5), S.npt);
 }
 
 int body_pairs(hipStream_t s, Sim& S, const BoxBuf& body, bool structure) {
@@ -155,9 +170,7 @@ int collision_set(hipStream_t s, Sim& S, bool structure) {
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_TRI, S.st.xk, 0.0, S.tri.lo.p, S.tri.hi.p))) return rc;
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_EDGE, S.st.xk, dh, S.edge.lo.p, S.edge.hi.p))) return rc;
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_POINT, S.st.xk, S.P.dhat, S.pt.lo.p, S.pt.hi.p))) return rc;
-    if ((rc = cull_list(s, S, ABD_CULL_EE, S.edge, S.edge, S.eel, &S.nee))) return rc;
-    if ((rc = cull_list(s, S, ABD_CULL_PT, S.pt, S.tri, S.ptl, &S.npt))) return rc;
-    return cull_vg_list(s, S, S.body, S.pt);
+    return cull_lists(s, S, S.body, S.pt, S.tri, S.edge);
 }
 
 // abd.py:170-185, with both ends of every box dilated as in collision_set: one candidate
@@ -177,9 +190,7 @@ int swept_set(hipStream_t s, Sim& S) {
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, st.x, st.xk, 0.0, S.ttri.lo.p, S.ttri.hi.p))) return rc;
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, st.x, st.xk, dh, S.tedge.lo.p, S.tedge.hi.p))) return rc;
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, st.x, st.xk, S.P.dhat, S.tpt.lo.p, S.tpt.hi.p))) return rc;
-    if ((rc = cull_list(s, S, ABD_CULL_EE, S.tedge, S.tedge, S.eel, &S.nee))) return rc;
-    if ((rc = cull_list(s, S, ABD_CULL_PT, S.tpt, S.ttri, S.ptl, &S.npt))) return rc;
-    return cull_vg_list(s, S, S.tbody, S.tpt);
+    return cull_lists(s, S, S.tbody, S.tpt, S.ttri, S.tedge);
 }
 
 // abd.py:133-141
@@ -568,6 +579,8 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
     if (!rc) rc = S->scal.reserve(8);
     if (!rc) rc = S->iters.reserve(4);
     if (!rc && hipHostMalloc((void**)&S->host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = ABD_ERR_ARG;
+    if (!rc && hipHostMalloc((void**)&S->hcount, 8 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess)
+        rc = ABD_ERR_ARG;
     {
         const int nsc = scene->n_scenes > 1 ? scene->n_scenes : 1;
         if (!rc) rc = S->cand_A.reserve(9 * nb * kLsSpec);
@@ -731,7 +744,7 @@ int abd_sim_destroy(void* sim) {
     S->cand_E.release();
     S->cand_alpha.release();
     if (S->hcand) hipHostFree(S->hcand);
-    for (void* h : {(void*)S->host, (void*)S->hsv, (void*)S->halpha, (void*)S->hccd, (void*)S->hmask,
+    for (void* h : {(void*)S->host, (void*)S->hcount, (void*)S->hsv, (void*)S->halpha, (void*)S->hccd, (void*)S->hmask,
                     (void*)S->hiters})
         if (h) hipHostFree(h);
     delete S;
