@@ -17,19 +17,17 @@ __device__ __forceinline__ void tilde(const AbdParams& P, const double* A0, cons
     const double gv[3] = {0.0, P.gravity, 0.0};
     for (int k = 0; k < 3; ++k) pt[k] = p0[k] + P.dt * pdot[k] + P.dt * P.dt * gv[k];
 }
-// orthogonal_energy.py:25-51, energy of body b
-__device__ __forceinline__ double inertia_energy_body(const AbdParams& P, const double* __restrict__ mass,
-                                                      const double* __restrict__ I0v, int b,
-                                                      const double* __restrict__ Ak, const double* __restrict__ pk,
-                                                      const double* __restrict__ A0, const double* __restrict__ p0,
-                                                      const double* __restrict__ Adot,
-                                                      const double* __restrict__ pdot) {
+// orthogonal_energy.py:25-51, energy of body b at the state (A, p) (A[9], p[3] of body b)
+__device__ __forceinline__ double inertia_energy_q(const AbdParams& P, const double* __restrict__ mass,
+                                                   const double* __restrict__ I0v, int b, const double* A,
+                                                   const double* p, const double* __restrict__ A0,
+                                                   const double* __restrict__ p0, const double* __restrict__ Adot,
+                                                   const double* __restrict__ pdot) {
     double At[9], pt[3];
     tilde(P, A0 + 9 * b, Adot + 9 * b, p0 + 3 * b, pdot + 3 * b, At, pt);
-    const double* A = Ak + 9 * b;
     const double m = mass ? mass[b] : P.mass, i0 = I0v ? I0v[b] : P.I0;
     double dq0[3], dq[9];
-    for (int k = 0; k < 3; ++k) dq0[k] = pk[3 * b + k] - pt[k];
+    for (int k = 0; k < 3; ++k) dq0[k] = p[k] - pt[k];
     for (int k = 0; k < 9; ++k) dq[k] = A[k] - At[k];
     const double n0 = dq0[0] * dq0[0] + dq0[1] * dq0[1] + dq0[2] * dq0[2];
     const double n1 = dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2];
@@ -44,6 +42,25 @@ __device__ __forceinline__ double inertia_energy_body(const AbdParams& P, const 
         }
     eo = eo * P.stiffness;
     return eo * P.dt * P.dt + 0.5 * dqTMdq;
+}
+// the same at the stored state (Ak, pk) of body b
+__device__ __forceinline__ double inertia_energy_body(const AbdParams& P, const double* __restrict__ mass,
+                                                      const double* __restrict__ I0v, int b,
+                                                      const double* __restrict__ Ak, const double* __restrict__ pk,
+                                                      const double* __restrict__ A0, const double* __restrict__ p0,
+                                                      const double* __restrict__ Adot,
+                                                      const double* __restrict__ pdot) {
+    return inertia_energy_q(P, mass, I0v, b, Ak + 9 * b, pk + 3 * b, A0, p0, Adot, pdot);
+}
+// body b's state q - alpha dq (abd.py:337-353; alpha = 0 keeps q exactly)
+__device__ __forceinline__ void trial_state(int b, double alpha, const double* __restrict__ A,
+                                            const double* __restrict__ p, const double* __restrict__ dq, double Ak[9],
+                                            double pk[3]) {
+    for (int r = 0; r < 3; ++r) pk[r] = alpha == 0.0 ? p[3 * b + r] : p[3 * b + r] - dq[3 * (4 * b) + r] * alpha;
+    for (int row = 0; row < 3; ++row)
+        for (int k = 0; k < 3; ++k)
+            Ak[3 * row + k] = alpha == 0.0 ? A[9 * b + 3 * row + k]
+                                           : A[9 * b + 3 * row + k] - dq[3 * (4 * b + 1 + row) + k] * alpha;
 }
 
 }  // namespace abd

@@ -99,6 +99,13 @@ int vg_cull_fill(hipStream_t s, VgCull& C, int32_t* out, int64_t cap, int64_t n)
 // the dilated boxes at x0 and at x1.
 int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,
                 float* lo, float* hi);
+// K line-search trials q - alpha dq at once (alpha [K][S] on the device, S = 1: the
+// whole system as one scene): trial vertices to xK [K][3 V], per-scene contact
+// energies to E[2 S k + s] and inertia energies to E[2 S k + S + s].
+int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
+                const double* A, const double* p, const double* dq, const double* A0, const double* p0,
+                const double* Adot, const double* pdot, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
+                const int32_t* vg, int64_t nvg, double* xK, double* E);
 int arena_get(int slot, size_t bytes, void** ptr);
 int arena_host(size_t bytes, void** ptr);  // pinned host memory
 int arena_release_current();

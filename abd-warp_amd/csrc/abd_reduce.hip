@@ -113,13 +113,17 @@ __device__ __forceinline__ double chunk_term(int kind, const AbdScene& sc, const
 }
 
 // one chunk per workgroup: OP 0 sums energies, OP 1 takes the minimum TOI below 1
+// blockIdx.y: the trial (positions x0 + x_stride y, partials part + part_stride y)
 template <int OP>
 __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdParams P, const double* __restrict__ x0,
                                                             const double* __restrict__ x1, Lists L, int S,
                                                             const int64_t* __restrict__ seg,
                                                             const int64_t* __restrict__ choff,
-                                                            double* __restrict__ part) {
+                                                            double* __restrict__ part, int64_t x_stride,
+                                                            int64_t part_stride) {
     __shared__ double sh[kRedThreads];
+    x0 += x_stride * blockIdx.y;
+    part += part_stride * blockIdx.y;
     const int64_t c = blockIdx.x;
     if (c >= choff[S]) return;
     const int s = scene_of_chunk(choff, S, c);
@@ -156,8 +160,11 @@ __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdPara
 template <int OP>
 __global__ __launch_bounds__(kRedThreads) void scene_final_kernel(const int64_t* __restrict__ choff,
                                                                   const double* __restrict__ part,
-                                                                  double* __restrict__ out) {
+                                                                  double* __restrict__ out, int64_t part_stride,
+                                                                  int64_t out_stride) {
     __shared__ double sh[kRedThreads];
+    part += part_stride * blockIdx.y;
+    out += out_stride * blockIdx.y;
     const int s = blockIdx.x;
     double acc = OP == 0 ? 0.0 : 1.0;
     for (int64_t i = choff[s] + threadIdx.x; i < choff[s + 1]; i += blockDim.x) {
@@ -225,10 +232,110 @@ int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
     ABD_CHECK_LAUNCH();
     int ph = prof_start(prof_id, s, prof_bytes);
     hipLaunchKernelGGL(chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1, L, S, seg,
-                       choff, part);
+                       choff, part, (int64_t)0, (int64_t)0);
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s);
-    hipLaunchKernelGGL(scene_final_kernel<OP>, dim3(S), dim3(kRedThreads), 0, s, choff, part, out);
+    hipLaunchKernelGGL(scene_final_kernel<OP>, dim3(S), dim3(kRedThreads), 0, s, choff, part, out, (int64_t)0,
+                       (int64_t)0);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+// ---------------------------------------------------------- line-search trials
+// K trial states q - alpha dq evaluated together (alpha: [K][S] on the device; S = 1
+// treats the scene as one): the vertices of trial k (abd.py:305-324 on the trial state,
+// the same arithmetic as abd_update_q + abd_project_vertices), then per scene the contact
+// and inertia energies of every trial, with one plan of the candidate lists for all.
+// E[2S k + s] is the contact energy and E[2S k + S + s] the inertia energy of scene s
+// at trial k, each bit-identical to the single-state reductions at that state.
+__global__ __launch_bounds__(kRedThreads) void project_trials_kernel(int S, const double* __restrict__ alpha,
+                                                                     const int32_t* __restrict__ body_scene,
+                                                                     const int32_t* __restrict__ vert_body,
+                                                                     const double* __restrict__ x0,
+                                                                     const double* __restrict__ A,
+                                                                     const double* __restrict__ p,
+                                                                     const double* __restrict__ dq, int64_t nv,
+                                                                     double* __restrict__ xK) {
+    const int k = blockIdx.y;
+    double* x = xK + 3 * nv * k;
+    for (int64_t v = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kRedThreads) {
+        const int b = vert_body[v];
+        const double al = alpha[(int64_t)S * k + (S > 1 ? body_scene[b] : 0)];
+        double Ab[9], pb[3];
+        trial_state(b, al, A, p, dq, Ab, pb);
+        const double r0 = x0[3 * v], r1 = x0[3 * v + 1], r2 = x0[3 * v + 2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double r = pb[c] + r0 * Ab[c];
+            r = r + r1 * Ab[3 + c];
+            r = r + r2 * Ab[6 + c];
+            x[3 * v + c] = r;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void inertia_trials_kernel(int S, const double* __restrict__ alpha,
+                                                                     const int32_t* __restrict__ sboff, int nb,
+                                                                     AbdParams P, const double* __restrict__ mass,
+                                                                     const double* __restrict__ I0v,
+                                                                     const double* __restrict__ A,
+                                                                     const double* __restrict__ p,
+                                                                     const double* __restrict__ dq,
+                                                                     const double* __restrict__ A0,
+                                                                     const double* __restrict__ p0,
+                                                                     const double* __restrict__ Adot,
+                                                                     const double* __restrict__ pdot,
+                                                                     double* __restrict__ E) {
+    __shared__ double sh[17];
+    const int s = blockIdx.x, k = blockIdx.y;
+    const double al = alpha[(int64_t)S * k + s];
+    const int b0 = S > 1 ? sboff[s] : 0, b1 = S > 1 ? sboff[s + 1] : nb;
+    double acc = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+        double Ab[9], pb[3];
+        trial_state(b, al, A, p, dq, Ab, pb);
+        acc += inertia_energy_q(P, mass, I0v, b, Ab, pb, A0, p0, Adot, pdot);
+    }
+    const double r = block_sum_fast(acc, sh);
+    if (threadIdx.x == 0) E[(int64_t)2 * S * k + s] = r;
+}
+
+int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
+                const double* A, const double* p, const double* dq, const double* A0, const double* p0,
+                const double* Adot, const double* pdot, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
+                const int32_t* vg, int64_t nvg, double* xK, double* E) {
+    const int64_t nv = sc->n_verts;
+    int rc;
+    hipLaunchKernelGGL(project_trials_kernel, dim3((unsigned)grid_for(nv, kRedThreads, 4096), K), dim3(kRedThreads), 0,
+                       s, S, alpha, sc->body_scene, sc->vert_body, sc->x0, A, p, dq, nv, xK);
+    ABD_CHECK_LAUNCH();
+    if (!P->pt) npt = 0;
+    if (!P->ee) nee = 0;
+    if (!P->vg) nvg = 0;
+    const Lists L{{{ee, nee, kListEE}, {pt, npt, kListPT}, {vg, nvg, kListVG}}};
+    int64_t bound = 3 * (int64_t)S;
+    for (int l = 0; l < kLists; ++l) bound += (L.l[l].n + kChunkItems - 1) / kChunkItems;
+    if (bound > INT32_MAX) return ABD_ERR_ARG;
+    int64_t* plan;
+    double* part;
+    const size_t plan_elems = 2 * kLists * (size_t)S + (size_t)S + 1;
+    if ((rc = arena_get(kSlotPlan, sizeof(int64_t) * plan_elems, (void**)&plan))) return rc;
+    if ((rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound * K, (void**)&part))) return rc;
+    int64_t* seg = plan;
+    int64_t* choff = plan + 2 * kLists * (size_t)S;
+    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kRedThreads), 0, s, S, sc->body_scene, L, seg, choff);
+    ABD_CHECK_LAUNCH();
+    int ph = prof_start(kProfEnergy, s, K * (128.0 * npt + 132.0 * nee + 32.0 * nvg));
+    hipLaunchKernelGGL(chunk_kernel<0>, dim3((unsigned)bound, K), dim3(kRedThreads), 0, s, *sc, *P,
+                       (const double*)xK, (const double*)nullptr, L, S, (const int64_t*)seg, (const int64_t*)choff,
+                       part, 3 * nv, bound);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    hipLaunchKernelGGL(scene_final_kernel<0>, dim3(S, K), dim3(kRedThreads), 0, s, (const int64_t*)choff,
+                       (const double*)part, E, bound, (int64_t)2 * S);
+    ABD_CHECK_LAUNCH();
+    hipLaunchKernelGGL(inertia_trials_kernel, dim3(S, K), dim3(kRedThreads), 0, s, S, alpha, sc->scene_boff,
+                       sc->n_bodies, *P, sc->mass, sc->I0, A, p, dq, A0, p0, Adot, pdot, E + S);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }

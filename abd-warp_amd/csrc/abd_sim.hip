@@ -76,9 +76,9 @@ struct Sim {
     int32_t* hiters = nullptr;  // pinned solver iterations [ns]
     std::vector<int32_t> scene_it;  // Newton iterations of each scene in the last frame
     // line-search candidates evaluated together: states, vertices and per-candidate scalars
-    DevBuf<double> cand_A, cand_p, cand_x, cand_E;  // [kLsSpec] x (9 nb | 3 nb | 3 V | 2 ns)
-    DevBuf<double> cand_alpha;                      // [kLsSpec][ns]
-    double* hcand = nullptr;                        // pinned [kLsSpec][ns] alphas, then [kLsSpec][2 ns] energies
+    DevBuf<double> cand_x, cand_E;  // [kLsSpec] x (3 V | 2 ns): trial vertices and energies
+    DevBuf<double> cand_alpha;      // [kLsSpec][ns]
+    double* hcand = nullptr;        // pinned [kLsSpec][ns] alphas, then [kLsSpec][2 ns] energies
 };
 
 // Step lengths evaluated per line-search pass after the first: the trials alpha,
@@ -216,22 +216,16 @@ int compute_energy(hipStream_t s, Sim& S, double alpha, double* E) {
 int line_energy_multi(hipStream_t s, Sim& S, const double* alphas, int K, double* E) {
     int rc;
     const AbdSimState& st = S.st;
-    const size_t nb = S.nb, nv = S.nv;
-    for (int k = 0; k < K; ++k) {
-        double* Ak = S.cand_A.p + 9 * nb * k;
-        double* pk = S.cand_p.p + 3 * nb * k;
-        double* xk = S.cand_x.p + 3 * nv * k;
-        if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, alphas[k], Ak, pk))) return rc;
-        if ((rc = abd_project_vertices(s, &S.sc, Ak, pk, xk))) return rc;
-        if ((rc = abd_contact_energy(s, &S.sc, &S.P, xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
-                                     S.cand_E.p + 2 * k)))
-            return rc;
-        if ((rc = abd_inertia_energy(s, &S.sc, &S.P, Ak, pk, st.A0, st.p0, st.Adot, st.pdot, S.cand_E.p + 2 * k + 1)))
-            return rc;
-    }
-    ABD_CHECK(hipMemcpyAsync(S.hcand, S.cand_E.p, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, s));
+    double* ha = S.hcand;
+    double* he = S.hcand + kLsSpec;
+    for (int k = 0; k < K; ++k) ha[k] = alphas[k];
+    ABD_CHECK(hipMemcpyAsync(S.cand_alpha.p, ha, sizeof(double) * K, hipMemcpyHostToDevice, s));
+    if ((rc = line_trials(s, &S.sc, &S.P, 1, K, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
+                          S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.cand_x.p, S.cand_E.p)))
+        return rc;
+    ABD_CHECK(hipMemcpyAsync(he, S.cand_E.p, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, s));
     ABD_CHECK(hipStreamSynchronize(s));
-    for (int k = 0; k < K; ++k) E[k] = S.hcand[2 * k] + S.hcand[2 * k + 1];
+    for (int k = 0; k < K; ++k) E[k] = he[2 * k] + he[2 * k + 1];
     S.stats[5] += K;
     return ABD_OK;
 }
@@ -383,20 +377,9 @@ int line_energy_scenes_multi(hipStream_t s, Sim& S, const double* dalpha, int K)
     int rc;
     const AbdSimState& st = S.st;
     const int ns = S.ns;
-    const size_t nb = S.nb, nv = S.nv;
-    for (int k = 0; k < K; ++k) {
-        double* Ak = S.cand_A.p + 9 * nb * k;
-        double* pk = S.cand_p.p + 3 * nb * k;
-        double* xk = S.cand_x.p + 3 * nv * k;
-        if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha + (size_t)ns * k, Ak, pk))) return rc;
-        if ((rc = abd_project_vertices(s, &S.sc, Ak, pk, xk))) return rc;
-        if ((rc = abd_scene_contact_energy(s, &S.sc, &S.P, xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
-                                           S.cand_E.p + (size_t)2 * ns * k)))
-            return rc;
-        if ((rc = abd_scene_inertia_energy(s, &S.sc, &S.P, Ak, pk, st.A0, st.p0, st.Adot, st.pdot,
-                                           S.cand_E.p + (size_t)2 * ns * k + ns)))
-            return rc;
-    }
+    if ((rc = line_trials(s, &S.sc, &S.P, ns, K, dalpha, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot, S.ptl.p,
+                          S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.cand_x.p, S.cand_E.p)))
+        return rc;
     ABD_CHECK(hipMemcpyAsync(S.hcand + (size_t)kLsSpec * ns, S.cand_E.p, sizeof(double) * 2 * ns * K,
                              hipMemcpyDeviceToHost, s));
     ABD_CHECK(hipStreamSynchronize(s));
@@ -583,8 +566,6 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
         rc = ABD_ERR_ARG;
     {
         const int nsc = scene->n_scenes > 1 ? scene->n_scenes : 1;
-        if (!rc) rc = S->cand_A.reserve(9 * nb * kLsSpec);
-        if (!rc) rc = S->cand_p.reserve(3 * nb * kLsSpec);
         if (!rc) rc = S->cand_x.reserve(3 * (size_t)S->nv * kLsSpec);
         if (!rc) rc = S->cand_E.reserve(2 * (size_t)nsc * kLsSpec);
         if (!rc) rc = S->cand_alpha.reserve((size_t)nsc * kLsSpec);
@@ -738,8 +719,6 @@ int abd_sim_destroy(void* sim) {
     S->iters.release();
     S->sv.release();
     S->sactive.release();
-    S->cand_A.release();
-    S->cand_p.release();
     S->cand_x.release();
     S->cand_E.release();
     S->cand_alpha.release();

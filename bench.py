@@ -226,7 +226,8 @@ def main():
     sample = None
     if batched:
         ns = spec.n_scenes
-        sample = sorted(set(int(round(k * (ns - 1) / (a.plain_sample - 1))) for k in range(a.plain_sample)))
+        k_n = max(a.plain_sample, 1)
+        sample = [0] if k_n == 1 else sorted(set(int(round(k * (ns - 1) / (k_n - 1))) for k in range(k_n)))
         p_it, p_wall = plain_per_scene(spec, cp, sample, a.warmup, a.steps)
         plain = dict(value=p_it / p_wall if p_wall > 0 else None, unit="Newton iters/s",
                      sample=f"plain native driver, one scene at a time, scenes {sample}, frames "
