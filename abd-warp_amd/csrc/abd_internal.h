@@ -105,7 +105,17 @@ int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, c
 int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
                 const double* A, const double* p, const double* dq, const double* A0, const double* p0,
                 const double* Adot, const double* pdot, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
-                const int32_t* vg, int64_t nvg, double* xK, double* E);
+                const int32_t* vg, int64_t nvg, const int64_t* plan, double* xK, double* E);
+// The per-scene chunk plan of the energy / TOI reductions over one set of candidate lists
+// (plan: reduce_plan_elems(S) int64 on the device), made once and passed as `plan` to every
+// reduction over those lists (nullptr: each makes its own).
+size_t reduce_plan_elems(int S);
+int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const int32_t* pt, int64_t npt,
+                const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t* plan);
+// Per-scene TOI (abd_scene_toi; S = 1: the whole system as one scene) with an optional plan.
+int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
+              const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
+              const int64_t* plan, double* toi);
 int arena_get(int slot, size_t bytes, void** ptr);
 int arena_host(size_t bytes, void** ptr);  // pinned host memory
 int arena_release_current();

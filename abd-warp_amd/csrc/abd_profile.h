@@ -23,6 +23,7 @@ enum ProfKernel {
     kProfSolve,         // Krylov solve (one workgroup)
     kProfProject,       // vertex projection
     kProfBoxes,         // primitive boxes
+    kProfPrecond,       // block-Jacobi: diagonal-block inverses
     kProfCount
 };
 

@@ -213,23 +213,45 @@ __global__ __launch_bounds__(kRedThreads) void seg_inertia_kernel(const int32_t*
     if (threadIdx.x == 0) E[s] = r;
 }
 
+// Upper bound on the chunks of the lists over S scenes (each scene's lists may end in a
+// partial chunk).
+int64_t chunk_bound(const Lists& L, int S) {
+    int64_t bound = 3 * (int64_t)S;
+    for (int l = 0; l < kLists; ++l) bound += (L.l[l].n + kChunkItems - 1) / kChunkItems;
+    return bound;
+}
+
+// The per-scene segments and chunk offsets of the lists: plan_in when given (made by
+// reduce_plan for these lists), else computed into the arena.
+int get_plan(hipStream_t s, const AbdScene& sc, int S, const Lists& L, const int64_t* plan_in,
+             const int64_t** seg, const int64_t** choff) {
+    if (plan_in) {
+        *seg = plan_in;
+        *choff = plan_in + 2 * kLists * (size_t)S;
+        return ABD_OK;
+    }
+    int64_t* plan;
+    int rc = arena_get(kSlotPlan, sizeof(int64_t) * reduce_plan_elems(S), (void**)&plan);
+    if (rc) return rc;
+    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kRedThreads), 0, s, S, sc.body_scene, L, plan,
+                       plan + 2 * kLists * (size_t)S);
+    ABD_CHECK_LAUNCH();
+    *seg = plan;
+    *choff = plan + 2 * kLists * (size_t)S;
+    return ABD_OK;
+}
+
 // Plan + chunks + per-scene results for OP over the lists; S = 1 treats the lists as one scene.
 template <int OP>
 int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, const double* x0, const double* x1,
-               const Lists& L, double* out, int prof_id, double prof_bytes) {
-    int64_t bound = 3 * (int64_t)S;
-    for (int l = 0; l < kLists; ++l) bound += (L.l[l].n + kChunkItems - 1) / kChunkItems;
+               const Lists& L, const int64_t* plan_in, double* out, int prof_id, double prof_bytes) {
+    const int64_t bound = chunk_bound(L, S);
     if (bound > INT32_MAX) return ABD_ERR_ARG;
-    int64_t* plan;
+    const int64_t *seg, *choff;
     double* part;
-    const size_t plan_elems = 2 * kLists * (size_t)S + (size_t)S + 1;
-    int rc = arena_get(kSlotPlan, sizeof(int64_t) * plan_elems, (void**)&plan);
+    int rc = get_plan(s, sc, S, L, plan_in, &seg, &choff);
     if (!rc) rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound, (void**)&part);
     if (rc) return rc;
-    int64_t* seg = plan;
-    int64_t* choff = plan + 2 * kLists * (size_t)S;
-    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kRedThreads), 0, s, S, sc.body_scene, L, seg, choff);
-    ABD_CHECK_LAUNCH();
     int ph = prof_start(prof_id, s, prof_bytes);
     hipLaunchKernelGGL(chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1, L, S, seg,
                        choff, part, (int64_t)0, (int64_t)0);
@@ -237,6 +259,28 @@ int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
     prof_stop(ph, s);
     hipLaunchKernelGGL(scene_final_kernel<OP>, dim3(S), dim3(kRedThreads), 0, s, choff, part, out, (int64_t)0,
                        (int64_t)0);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+// The candidate lists of energies and TOIs, in the one order every reduction uses (the
+// sum order of the energy; a minimum does not depend on it).  The switches of
+// const_params.py:42-44 drop a whole list.
+Lists contact_lists(const AbdParams& P, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
+                    const int32_t* vg, int64_t nvg) {
+    if (!P.pt) npt = 0;
+    if (!P.ee) nee = 0;
+    if (!P.vg) nvg = 0;
+    return Lists{{{ee, nee, kListEE}, {pt, npt, kListPT}, {vg, nvg, kListVG}}};
+}
+
+size_t reduce_plan_elems(int S) { return 2 * kLists * (size_t)S + (size_t)S + 1; }
+
+int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const int32_t* pt, int64_t npt,
+                const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t* plan) {
+    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);
+    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kRedThreads), 0, s, S, sc->body_scene, L, plan,
+                       plan + 2 * kLists * (size_t)S);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }
@@ -303,42 +347,37 @@ __global__ __launch_bounds__(kRedThreads) void inertia_trials_kernel(int S, cons
 int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
                 const double* A, const double* p, const double* dq, const double* A0, const double* p0,
                 const double* Adot, const double* pdot, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
-                const int32_t* vg, int64_t nvg, double* xK, double* E) {
+                const int32_t* vg, int64_t nvg, const int64_t* plan_in, double* xK, double* E) {
     const int64_t nv = sc->n_verts;
     int rc;
     hipLaunchKernelGGL(project_trials_kernel, dim3((unsigned)grid_for(nv, kRedThreads, 4096), K), dim3(kRedThreads), 0,
                        s, S, alpha, sc->body_scene, sc->vert_body, sc->x0, A, p, dq, nv, xK);
     ABD_CHECK_LAUNCH();
-    if (!P->pt) npt = 0;
-    if (!P->ee) nee = 0;
-    if (!P->vg) nvg = 0;
-    const Lists L{{{ee, nee, kListEE}, {pt, npt, kListPT}, {vg, nvg, kListVG}}};
-    int64_t bound = 3 * (int64_t)S;
-    for (int l = 0; l < kLists; ++l) bound += (L.l[l].n + kChunkItems - 1) / kChunkItems;
+    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);
+    npt = L.l[1].n;
+    nee = L.l[0].n;
+    nvg = L.l[2].n;
+    const int64_t bound = chunk_bound(L, S);
     if (bound > INT32_MAX) return ABD_ERR_ARG;
-    int64_t* plan;
+    const int64_t *seg, *choff;
     double* part;
-    const size_t plan_elems = 2 * kLists * (size_t)S + (size_t)S + 1;
-    if ((rc = arena_get(kSlotPlan, sizeof(int64_t) * plan_elems, (void**)&plan))) return rc;
+    if ((rc = get_plan(s, *sc, S, L, plan_in, &seg, &choff))) return rc;
     if ((rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound * K, (void**)&part))) return rc;
-    int64_t* seg = plan;
-    int64_t* choff = plan + 2 * kLists * (size_t)S;
-    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kRedThreads), 0, s, S, sc->body_scene, L, seg, choff);
-    ABD_CHECK_LAUNCH();
     int ph = prof_start(kProfEnergy, s, K * (128.0 * npt + 132.0 * nee + 32.0 * nvg));
     hipLaunchKernelGGL(chunk_kernel<0>, dim3((unsigned)bound, K), dim3(kRedThreads), 0, s, *sc, *P,
-                       (const double*)xK, (const double*)nullptr, L, S, (const int64_t*)seg, (const int64_t*)choff,
-                       part, 3 * nv, bound);
+                       (const double*)xK, (const double*)nullptr, L, S, seg, choff, part, 3 * nv, bound);
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s);
-    hipLaunchKernelGGL(scene_final_kernel<0>, dim3(S, K), dim3(kRedThreads), 0, s, (const int64_t*)choff,
-                       (const double*)part, E, bound, (int64_t)2 * S);
+    hipLaunchKernelGGL(scene_final_kernel<0>, dim3(S, K), dim3(kRedThreads), 0, s, choff, (const double*)part, E,
+                       bound, (int64_t)2 * S);
     ABD_CHECK_LAUNCH();
     hipLaunchKernelGGL(inertia_trials_kernel, dim3(S, K), dim3(kRedThreads), 0, s, S, alpha, sc->scene_boff,
                        sc->n_bodies, *P, sc->mass, sc->I0, A, p, dq, A0, p0, Adot, pdot, E + S);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }
+
+namespace {
 
 bool scenes_ok(const AbdScene* sc) { return sc && sc->n_scenes > 0 && sc->scene_boff && sc->body_scene; }
 
@@ -347,26 +386,21 @@ int energy_impl(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, int
                 double* E) {
     if (!P || !xk || !E || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
-    // the switches of const_params.py:42-44 drop a whole list
-    if (!P->pt) npt = 0;
-    if (!P->ee) nee = 0;
-    if (!P->vg) nvg = 0;
-    const Lists L{{{ee, nee, kListEE}, {pt, npt, kListPT}, {vg, nvg, kListVG}}};
-    return seg_reduce<0>(to_stream(stream), *sc, *P, S, xk, nullptr, L, E, kProfEnergy,
-                         128.0 * npt + 132.0 * nee + 32.0 * nvg);
+    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);
+    return seg_reduce<0>(to_stream(stream), *sc, *P, S, xk, nullptr, L, nullptr, E, kProfEnergy,
+                         128.0 * L.l[1].n + 132.0 * L.l[0].n + 32.0 * L.l[2].n);
 }
 
-int toi_impl(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0,
-             const double* x_t1, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg,
-             int64_t nvg, double* toi) {
+}  // namespace
+
+int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
+              const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
+              const int64_t* plan, double* toi) {
     if (!P || !x_t0 || !x_t1 || !toi || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
-    if (!P->pt) npt = 0;  // const_params.py:42-44, abd.py:288-299
-    if (!P->ee) nee = 0;
-    if (!P->vg) nvg = 0;
-    const Lists L{{{vg, nvg, kListVG}, {ee, nee, kListEE}, {pt, npt, kListPT}}};
-    return seg_reduce<1>(to_stream(stream), *sc, *P, S, x_t0, x_t1, L, toi, kProfToi,
-                         224.0 * npt + 228.0 * nee + 56.0 * nvg);
+    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);  // abd.py:288-299
+    return seg_reduce<1>(s, *sc, *P, S, x_t0, x_t1, L, plan, toi, kProfToi,
+                         224.0 * L.l[1].n + 228.0 * L.l[0].n + 56.0 * L.l[2].n);
 }
 
 }  // namespace abd
@@ -393,14 +427,15 @@ int abd_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const d
             const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
             double* toi) {
     if (!sc) return ABD_ERR_ARG;
-    return toi_impl(stream, sc, P, 1, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, toi);
+    return scene_toi(to_stream(stream), sc, P, 1, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, nullptr, toi);
 }
 
 int abd_scene_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x_t0,
                   const double* x_t1, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
                   const int32_t* vg, int64_t nvg, double* toi) {
     if (!scenes_ok(sc)) return ABD_ERR_ARG;
-    return toi_impl(stream, sc, P, sc->n_scenes, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, toi);
+    return scene_toi(to_stream(stream), sc, P, sc->n_scenes, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, nullptr,
+                     toi);
 }
 
 int abd_dot(abd_stream_t stream, const double* a, const double* b, int64_t n, double* out) {

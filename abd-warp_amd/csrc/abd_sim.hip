@@ -78,6 +78,7 @@ struct Sim {
     // line-search candidates evaluated together: states, vertices and per-candidate scalars
     DevBuf<double> cand_x, cand_E;  // [kLsSpec] x (3 V | 2 ns): trial vertices and energies
     DevBuf<double> cand_alpha;      // [kLsSpec][ns]
+    DevBuf<int64_t> plan;           // reduction plan of the swept candidate set (CCD + line search)
     double* hcand = nullptr;        // pinned [kLsSpec][ns] alphas, then [kLsSpec][2 ns] energies
 };
 
@@ -221,7 +222,7 @@ int line_energy_multi(hipStream_t s, Sim& S, const double* alphas, int K, double
     for (int k = 0; k < K; ++k) ha[k] = alphas[k];
     ABD_CHECK(hipMemcpyAsync(S.cand_alpha.p, ha, sizeof(double) * K, hipMemcpyHostToDevice, s));
     if ((rc = line_trials(s, &S.sc, &S.P, 1, K, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
-                          S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.cand_x.p, S.cand_E.p)))
+                          S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p, S.cand_x.p, S.cand_E.p)))
         return rc;
     ABD_CHECK(hipMemcpyAsync(he, S.cand_E.p, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, s));
     ABD_CHECK(hipStreamSynchronize(s));
@@ -247,7 +248,10 @@ int ccd(hipStream_t s, Sim& S, double* alpha_cap) {
     if ((rc = swept_set(s, S))) return rc;
     S.stats[8] += S.npt;
     S.stats[9] += S.nee;
-    if ((rc = abd_toi(s, &S.sc, &S.P, st.x, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.scal.p + 2)))
+    // one reduction plan of the swept set serves the TOI and every line-search trial
+    if ((rc = reduce_plan(s, &S.sc, &S.P, 1, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p))) return rc;
+    if ((rc = scene_toi(s, &S.sc, &S.P, 1, st.x, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p,
+                        S.scal.p + 2)))
         return rc;
     if ((rc = get_scalars(s, S, 2, 1))) return rc;
     double t = S.host[2];
@@ -378,7 +382,7 @@ int line_energy_scenes_multi(hipStream_t s, Sim& S, const double* dalpha, int K)
     const AbdSimState& st = S.st;
     const int ns = S.ns;
     if ((rc = line_trials(s, &S.sc, &S.P, ns, K, dalpha, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot, S.ptl.p,
-                          S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.cand_x.p, S.cand_E.p)))
+                          S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p, S.cand_x.p, S.cand_E.p)))
         return rc;
     ABD_CHECK(hipMemcpyAsync(S.hcand + (size_t)kLsSpec * ns, S.cand_E.p, sizeof(double) * 2 * ns * K,
                              hipMemcpyDeviceToHost, s));
@@ -442,8 +446,10 @@ int step_scenes(hipStream_t s, Sim& S) {
         if ((rc = swept_set(s, S))) return rc;
         S.stats[8] += S.npt;
         S.stats[9] += S.nee;
-        if ((rc = abd_scene_toi(s, &S.sc, &P, st.x, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
-                                S.sv.p + kSvToi * ns)))
+        if ((rc = reduce_plan(s, &S.sc, &P, ns, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p)))
+            return rc;
+        if ((rc = scene_toi(s, &S.sc, &P, ns, st.x, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p,
+                            S.sv.p + kSvToi * ns)))
             return rc;
         if ((rc = abd_scene_dot(s, &S.sc, st.dq, st.g, S.sv.p + kSvQtg * ns))) return rc;
         ABD_CHECK(hipMemcpyAsync(S.hsv + kSvToi * ns, S.sv.p + kSvToi * ns, sizeof(double) * 2 * ns,
@@ -569,6 +575,7 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
         if (!rc) rc = S->cand_x.reserve(3 * (size_t)S->nv * kLsSpec);
         if (!rc) rc = S->cand_E.reserve(2 * (size_t)nsc * kLsSpec);
         if (!rc) rc = S->cand_alpha.reserve((size_t)nsc * kLsSpec);
+        if (!rc) rc = S->plan.reserve(reduce_plan_elems(nsc));
         if (!rc && hipHostMalloc((void**)&S->hcand, sizeof(double) * 3 * nsc * kLsSpec, hipHostMallocDefault) !=
                        hipSuccess)
             rc = ABD_ERR_ARG;
@@ -722,6 +729,7 @@ int abd_sim_destroy(void* sim) {
     S->cand_x.release();
     S->cand_E.release();
     S->cand_alpha.release();
+    S->plan.release();
     if (S->hcand) hipHostFree(S->hcand);
     for (void* h : {(void*)S->host, (void*)S->hcount, (void*)S->hsv, (void*)S->halpha, (void*)S->hccd, (void*)S->hmask,
                     (void*)S->hiters})

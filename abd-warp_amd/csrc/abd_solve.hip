@@ -9,6 +9,8 @@
 // with no host round trips.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "abd_block.h"
 #include "abd_internal.h"
 #include "abd_profile.h"
@@ -18,6 +20,8 @@ namespace abd {
 constexpr int kSolveThreads = 1024;
 // Systems with 6 n doubles up to this size keep every Krylov vector in LDS.
 constexpr size_t kLdsVecBytes = 96 * 1024;
+// LDS for a system's staged block structure (row offsets and (column, slot) entries).
+constexpr size_t kLdsStructBytes = 24 * 1024;
 
 struct Bsr {
     int nb;
@@ -27,26 +31,108 @@ struct Bsr {
     const double* blocks;
 };
 
-// out = H in over the rows of bodies row0 .. row0 + n/12 (in / out indexed from
-// row0); row idx of body row0 + idx / 12, sub-row (ii, k) = ((idx % 12) / 3, idx % 3)
-__device__ __forceinline__ void spmv(const Bsr& M, int row0, const double* __restrict__ in,
+// One 12x12 block's row (ii, k) against 12 entries of a vector: the three-term sums of
+// sub-blocks (ii, jj), jj = 0..3, added to acc in jj order.
+__device__ __forceinline__ double block_row_add(double acc, const double (&B)[12], const double* __restrict__ xin) {
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj)
+        acc += B[3 * jj] * xin[3 * jj] + B[3 * jj + 1] * xin[3 * jj + 1] + B[3 * jj + 2] * xin[3 * jj + 2];
+    return acc;
+}
+
+__device__ __forceinline__ void load_block_row(const double* __restrict__ blocks, int64_t slot, int ii, int k,
+                                               double (&B)[12]) {
+    const double* base = blocks + 9 * 16 * slot + 3 * k;
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+        const double* Bj = base + 9 * (ii + 4 * jj);
+        B[3 * jj] = Bj[0];
+        B[3 * jj + 1] = Bj[1];
+        B[3 * jj + 2] = Bj[2];
+    }
+}
+
+// A system's block structure as its SpMV reads it: from the global CSR (columns outside
+// the system filtered on the fly) or, with ST, from an LDS copy made once per solve by
+// stage_rows (local row offsets lrp, entries ent = (local column or -1, slot)), which
+// takes two dependent global loads off every block the SpMV visits.
+struct Rows {
+    int row0, nbs;
+    const int* lrp;
+    const int2* ent;
+};
+
+template <bool ST>
+__device__ __forceinline__ void row_entry(const Bsr& M, const Rows& R, int e, int& lc, int64_t& slot) {
+    if (ST) {
+        const int2 v = R.ent[e];
+        lc = v.x;
+        slot = v.y;
+    } else {
+        lc = M.cols[e] - R.row0;
+        if ((unsigned)lc >= (unsigned)R.nbs) lc = -1;  // a block coupling two scenes is part of neither
+        slot = M.slots[e];
+    }
+}
+
+// (H in)[row (lb, ii, k)] of the system (local body lb), summed over the row's blocks in
+// column order.  Two blocks are loaded per step so that their memory latencies overlap;
+// the sum order is that of a one-block loop.
+template <bool ST>
+__device__ __forceinline__ double row_product(const Bsr& M, const Rows& R, int lb, int ii, int k,
+                                              const double* __restrict__ in) {
+    double acc = 0.0;
+    int e = ST ? R.lrp[lb] : M.row_ptr[R.row0 + lb];
+    const int e1 = ST ? R.lrp[lb + 1] : M.row_ptr[R.row0 + lb + 1];
+    for (; e + 1 < e1; e += 2) {
+        int lc0, lc1;
+        int64_t s0, s1;
+        row_entry<ST>(M, R, e, lc0, s0);
+        row_entry<ST>(M, R, e + 1, lc1, s1);
+        double B0[12], B1[12];
+        load_block_row(M.blocks, s0, ii, k, B0);
+        load_block_row(M.blocks, s1, ii, k, B1);
+        if (lc0 >= 0) acc = block_row_add(acc, B0, in + 12 * (int64_t)lc0);
+        if (lc1 >= 0) acc = block_row_add(acc, B1, in + 12 * (int64_t)lc1);
+    }
+    if (e < e1) {
+        int lc;
+        int64_t sl;
+        row_entry<ST>(M, R, e, lc, sl);
+        if (lc >= 0) {
+            double B[12];
+            load_block_row(M.blocks, sl, ii, k, B);
+            acc = block_row_add(acc, B, in + 12 * (int64_t)lc);
+        }
+    }
+    return acc;
+}
+
+// Copies the structure of the system's rows into LDS (lrp: nbs + 1 ints, ent: cap_ent
+// entries) when it fits; returns false (nothing staged) otherwise.  Uniform over the
+// workgroup; ends with a barrier when it stages.
+__device__ __forceinline__ bool stage_rows(const Bsr& M, Rows& R, int* lrp, int2* ent, int cap_ent) {
+    const int e0 = M.row_ptr[R.row0], nnz = M.row_ptr[R.row0 + R.nbs] - e0;
+    if (cap_ent < 0 || nnz > cap_ent) return false;
+    for (int r = threadIdx.x; r <= R.nbs; r += blockDim.x) lrp[r] = M.row_ptr[R.row0 + r] - e0;
+    for (int e = threadIdx.x; e < nnz; e += blockDim.x) {
+        const int lc = M.cols[e0 + e] - R.row0;
+        ent[e] = make_int2((unsigned)lc < (unsigned)R.nbs ? lc : -1, M.slots[e0 + e]);
+    }
+    __syncthreads();
+    R.lrp = lrp;
+    R.ent = ent;
+    return true;
+}
+
+// out = H in over the system's rows (in / out indexed from its first row); row idx of
+// local body idx / 12, sub-row (ii, k) = ((idx % 12) / 3, idx % 3)
+template <bool ST>
+__device__ __forceinline__ void spmv(const Bsr& M, const Rows& R, const double* __restrict__ in,
                                      double* __restrict__ out, int n) {
     for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-        const int b = row0 + idx / 12, lr = idx % 12, ii = lr / 3, k = lr % 3;
-        double acc = 0.0;
-        for (int e = M.row_ptr[b]; e < M.row_ptr[b + 1]; ++e) {
-            const int lc = M.cols[e] - row0;
-            if ((unsigned)lc >= (unsigned)(n / 12)) continue;  // a block coupling two scenes is not part of either
-            const int64_t s = M.slots[e];
-            const double* xin = in + 12 * (int64_t)lc;
-            const double* base = M.blocks + 9 * 16 * s + 3 * k;
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                const double* B = base + 9 * (ii + 4 * jj);
-                acc += B[0] * xin[3 * jj] + B[1] * xin[3 * jj + 1] + B[2] * xin[3 * jj + 2];
-            }
-        }
-        out[idx] = acc;
+        const int lr = idx % 12;
+        out[idx] = row_product<ST>(M, R, idx / 12, lr / 3, lr % 3, in);
     }
     __syncthreads();
 }
@@ -54,24 +140,13 @@ __device__ __forceinline__ void spmv(const Bsr& M, int row0, const double* __res
 // out = H in over this thread's rows (the rows spmv() gives it), returning the
 // thread's partial sum of w[idx] * out[idx] in row order; no barrier: the caller
 // reduces the partials (which also orders the writes before any cross-thread read).
-__device__ __forceinline__ double spmv_rows_dot(const Bsr& M, int row0, const double* __restrict__ in,
+template <bool ST>
+__device__ __forceinline__ double spmv_rows_dot(const Bsr& M, const Rows& R, const double* __restrict__ in,
                                                 double* __restrict__ out, int n, const double* __restrict__ w) {
     double part = 0.0;
     for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-        const int b = row0 + idx / 12, lr = idx % 12, ii = lr / 3, k = lr % 3;
-        double acc = 0.0;
-        for (int e = M.row_ptr[b]; e < M.row_ptr[b + 1]; ++e) {
-            const int lc = M.cols[e] - row0;
-            if ((unsigned)lc >= (unsigned)(n / 12)) continue;  // a block coupling two scenes is not part of either
-            const int64_t s = M.slots[e];
-            const double* xin = in + 12 * (int64_t)lc;
-            const double* base = M.blocks + 9 * 16 * s + 3 * k;
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                const double* B = base + 9 * (ii + 4 * jj);
-                acc += B[0] * xin[3 * jj] + B[1] * xin[3 * jj + 1] + B[2] * xin[3 * jj + 2];
-            }
-        }
+        const int lr = idx % 12;
+        const double acc = row_product<ST>(M, R, idx / 12, lr / 3, lr % 3, in);
         out[idx] = acc;
         part += w[idx] * acc;
     }
@@ -85,38 +160,27 @@ __device__ __forceinline__ double vdot(const double* __restrict__ a, const doubl
     return block_sum_fast(acc, sh);
 }
 
-// Vector storage: LDS (dynamic shared memory) or the scratch arena.
+// Dynamic LDS of a solve workgroup: the six Krylov vectors (when LDS; else in the
+// scratch arena), then the staged structure (lrp rounded up to 8 bytes, then ent).
 template <bool LDS>
-__device__ __forceinline__ double* vec_base(double* ws) {
-    extern __shared__ double lds_vec[];
-    return LDS ? lds_vec : ws;
-}
-
-template <bool LDS>
-__global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const int32_t* __restrict__ sboff,
-                                                                  const uint8_t* __restrict__ active,
-                                                                  const double* __restrict__ b,
-                                                                  double* __restrict__ xg, double tol, int maxiter,
-                                                                  int check_every, int cap_rows, double* __restrict__ ws,
-                                                                  int32_t* __restrict__ iters,
-                                                                  double* __restrict__ resid) {
-    __shared__ double sh[17];
-    __shared__ double sh2[34];
-    // one system per workgroup: the bodies of scene blockIdx.x (all bodies without scenes)
-    const int sidx = blockIdx.x;
-    if (active && !active[sidx]) return;
-    const int row0 = sboff ? sboff[sidx] : 0;
-    const int n = 12 * ((sboff ? sboff[sidx + 1] : M.nb) - row0);
-    b += 12 * (int64_t)row0;
-    xg += 12 * (int64_t)row0;
-    if (iters) iters += sidx;
-    if (resid) resid += sidx;
-    if (n > cap_rows) {  // larger than the vector storage sized by the caller: report, touch nothing
-        if (threadIdx.x == 0 && iters) *iters = -1;
-        return;
+struct SolveLds {
+    double* V;
+    int* lrp;
+    int2* ent;
+    __device__ SolveLds(double* ws, int cap_rows) {
+        extern __shared__ double lds_dyn[];
+        V = LDS ? lds_dyn : ws;
+        char* st = (char*)(lds_dyn + (LDS ? 6 * cap_rows : 0));
+        lrp = (int*)st;
+        ent = (int2*)(st + 8 * ((cap_rows / 12 + 2) / 2));
     }
-    double* V = vec_base<LDS>(ws + (LDS ? 0 : 6 * 12 * (int64_t)row0));
-    if (maxiter <= 0) maxiter = n;  // the system size of this workgroup's system
+};
+
+template <bool ST>
+__device__ __forceinline__ void bicgstab_body(const Bsr& M, const Rows& R, int n, const double* __restrict__ b,
+                                              double* __restrict__ xg, double tol, int maxiter, int check_every,
+                                              double* V, int32_t* __restrict__ iters, double* __restrict__ resid,
+                                              double* sh, double* sh2) {
     double* x = V;
     double* r = V + n;
     double* r0 = V + 2 * n;
@@ -125,7 +189,7 @@ __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const in
     double* t = V + 5 * n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = xg[i];
     __syncthreads();
-    spmv(M, row0, x, t, n);
+    spmv<ST>(M, R, x, t, n);
     for (int i = threadIdx.x; i < n; i += blockDim.x) r[i] = b[i] - t[i];
     __syncthreads();
     double rr = vdot(r, r, n, sh);
@@ -144,7 +208,7 @@ __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const in
             // fused passes: each dot is accumulated where its operand is produced, in
             // the row order of vdot(), so the iterates match the unfused sequence bit for bit
             for (int c = 0; c < check_every; ++c) {
-                const double r0v = block_sum_fast(spmv_rows_dot(M, row0, p, v, n, r0), sh);
+                const double r0v = block_sum_fast(spmv_rows_dot<ST>(M, R, p, v, n, r0), sh);
                 const double alpha = r0v != 0.0 ? rho / r0v : 0.0;
                 for (int i = threadIdx.x; i < n; i += blockDim.x) {
                     x[i] += alpha * p[i];
@@ -154,20 +218,8 @@ __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const in
                 double tr = 0.0, tt = 0.0;
                 for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
                     // t = H r for this thread's rows (as spmv_rows_dot), then t.r and t.t
-                    const int b = row0 + idx / 12, lr = idx % 12, ii = lr / 3, k = lr % 3;
-                    double acc = 0.0;
-                    for (int e = M.row_ptr[b]; e < M.row_ptr[b + 1]; ++e) {
-                        const int lc = M.cols[e] - row0;
-                        if ((unsigned)lc >= (unsigned)(n / 12)) continue;
-                        const int64_t sl = M.slots[e];
-                        const double* xin = r + 12 * (int64_t)lc;
-                        const double* base = M.blocks + 9 * 16 * sl + 3 * k;
-#pragma unroll
-                        for (int jj = 0; jj < 4; ++jj) {
-                            const double* B = base + 9 * (ii + 4 * jj);
-                            acc += B[0] * xin[3 * jj] + B[1] * xin[3 * jj + 1] + B[2] * xin[3 * jj + 2];
-                        }
-                    }
+                    const int lr = idx % 12;
+                    const double acc = row_product<ST>(M, R, idx / 12, lr / 3, lr % 3, r);
                     t[idx] = acc;
                     tr += acc * r[idx];
                     tt += acc * acc;
@@ -200,6 +252,41 @@ __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const in
     }
 }
 
+// cap_ent: capacity of the staged structure (entries); a system with more blocks reads
+// its structure from the global CSR.
+template <bool LDS>
+__global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const int32_t* __restrict__ sboff,
+                                                                  const uint8_t* __restrict__ active,
+                                                                  const double* __restrict__ b,
+                                                                  double* __restrict__ xg, double tol, int maxiter,
+                                                                  int check_every, int cap_rows, int cap_ent,
+                                                                  double* __restrict__ ws,
+                                                                  int32_t* __restrict__ iters,
+                                                                  double* __restrict__ resid) {
+    __shared__ double sh[17];
+    __shared__ double sh2[34];
+    // one system per workgroup: the bodies of scene blockIdx.x (all bodies without scenes)
+    const int sidx = blockIdx.x;
+    if (active && !active[sidx]) return;
+    const int row0 = sboff ? sboff[sidx] : 0;
+    const int n = 12 * ((sboff ? sboff[sidx + 1] : M.nb) - row0);
+    b += 12 * (int64_t)row0;
+    xg += 12 * (int64_t)row0;
+    if (iters) iters += sidx;
+    if (resid) resid += sidx;
+    if (n > cap_rows) {  // larger than the vector storage sized by the caller: report, touch nothing
+        if (threadIdx.x == 0 && iters) *iters = -1;
+        return;
+    }
+    const SolveLds<LDS> S(ws + (LDS ? 0 : 6 * 12 * (int64_t)row0), cap_rows);
+    if (maxiter <= 0) maxiter = n;  // the system size of this workgroup's system
+    Rows R{row0, n / 12, nullptr, nullptr};
+    if (stage_rows(M, R, S.lrp, S.ent, cap_ent))
+        bicgstab_body<true>(M, R, n, b, xg, tol, maxiter, check_every, S.V, iters, resid, sh, sh2);
+    else
+        bicgstab_body<false>(M, R, n, b, xg, tol, maxiter, check_every, S.V, iters, resid, sh, sh2);
+}
+
 __device__ __forceinline__ void jacobi(const double* __restrict__ dg, const double* __restrict__ r,
                                        double* __restrict__ z, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -210,48 +297,58 @@ __device__ __forceinline__ void jacobi(const double* __restrict__ dg, const doub
     __syncthreads();
 }
 
-// Inverse of body b's 12x12 diagonal block (slot b) by Gauss-Jordan elimination with
-// partial pivoting, written to out[144]; a singular block gives the identity (that body
-// stays unpreconditioned).
-__device__ void invert_diag_block(const Bsr& M, int b, double* __restrict__ out) {
-    double a[12][24];
-    for (int i = 0; i < 12; ++i)
-        for (int j = 0; j < 12; ++j) {
-            const int ii = i / 3, u = i % 3, jj = j / 3, v = j % 3;
-            a[i][j] = M.blocks[9 * (16 * (int64_t)b + ii + 4 * jj) + 3 * u + v];
-            a[i][12 + j] = i == j ? 1.0 : 0.0;
-        }
+// Inverse of each body's 12x12 diagonal block (slot b) by Gauss-Jordan elimination with
+// partial pivoting, written to dinv + 144 b; a singular block gives the identity (that
+// body stays unpreconditioned).  One 64-thread workgroup per body: the augmented 12x24
+// matrix in LDS, the 288 entries of each elimination step updated in parallel, each with
+// the arithmetic of the sequential elimination (row c scaled by 1 / pivot, then
+// a[r][j] -= a[r][c] a[c][j] for every other row with a[r][c] != 0).
+constexpr int kInvThreads = 64;
+__global__ __launch_bounds__(kInvThreads) void invert_blocks_kernel(Bsr M, double* __restrict__ dinv) {
+    __shared__ double a[12][24];
+    __shared__ double srow[24];
+    __shared__ double f[12];
+    const int b = blockIdx.x, t = threadIdx.x;
+    for (int e = t; e < 288; e += kInvThreads) {
+        const int i = e / 24, j = e % 24;
+        a[i][j] = j < 12 ? M.blocks[9 * (16 * (int64_t)b + i / 3 + 4 * (j / 3)) + 3 * (i % 3) + j % 3]
+                         : (i == j - 12 ? 1.0 : 0.0);
+    }
+    __syncthreads();
     bool ok = true;
-    for (int c = 0; c < 12 && ok; ++c) {
-        int piv = c;
+    for (int c = 0; c < 12; ++c) {
+        int piv = c;  // the first row of largest |a[r][c]|, r >= c (every thread alike)
         for (int r = c + 1; r < 12; ++r)
             if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
         if (a[piv][c] == 0.0) {
             ok = false;
             break;
         }
-        if (piv != c)
-            for (int j = 0; j < 24; ++j) {
-                const double t = a[c][j];
-                a[c][j] = a[piv][j];
-                a[piv][j] = t;
-            }
-        const double inv = 1.0 / a[c][c];
-        for (int j = 0; j < 24; ++j) a[c][j] *= inv;
-        for (int r = 0; r < 12; ++r) {
-            if (r == c) continue;
-            const double f = a[r][c];
-            if (f != 0.0)
-                for (int j = 0; j < 24; ++j) a[r][j] -= f * a[c][j];
+        __syncthreads();
+        if (piv != c && t < 24) {
+            const double u = a[c][t];
+            a[c][t] = a[piv][t];
+            a[piv][t] = u;
         }
+        __syncthreads();
+        const double inv = 1.0 / a[c][c];
+        if (t < 24) srow[t] = a[c][t] * inv;
+        if (t < 12) f[t] = a[t][c];
+        __syncthreads();
+        for (int e = t; e < 288; e += kInvThreads) {
+            const int i = e / 24, j = e % 24;
+            if (i == c)
+                a[i][j] = srow[j];
+            else if (f[i] != 0.0)
+                a[i][j] -= f[i] * srow[j];
+        }
+        __syncthreads();
     }
-    for (int i = 0; i < 12; ++i)
-        for (int j = 0; j < 12; ++j) out[12 * i + j] = ok ? a[i][12 + j] : (i == j ? 1.0 : 0.0);
-}
-
-__global__ void invert_blocks_kernel(Bsr M, double* __restrict__ dinv) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < M.nb) invert_diag_block(M, b, dinv + 144 * (int64_t)b);
+    double* out = dinv + 144 * (int64_t)b;
+    for (int e = t; e < 144; e += kInvThreads) {
+        const int i = e / 12, j = e % 12;
+        out[e] = ok ? a[i][12 + j] : (i == j ? 1.0 : 0.0);
+    }
 }
 
 // z = D_b^-1 r for row k of a system whose body-diagonal inverses are in dinv (global body ids)
@@ -266,33 +363,12 @@ __device__ __forceinline__ double block_precond(const double* __restrict__ dinv,
     return acc;
 }
 
-// BJ: the per-body 12x12 block-Jacobi preconditioner (beyond the reference) in place of
-// the diagonal one; dinv: [nb][144] workspace of the inverses.
-template <bool LDS, bool BJ>
-__global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t* __restrict__ sboff,
-                                                                  const uint8_t* __restrict__ active,
-                                                                  const double* __restrict__ b,
-                                                            double* __restrict__ xg, double tol, int maxiter,
-                                                            int check_every, int cap_rows, double* __restrict__ ws,
-                                                            double* __restrict__ dinv,
-                                                            int32_t* __restrict__ iters,
-                                                            double* __restrict__ resid) {
-    __shared__ double sh[17];
-    // one system per workgroup: the bodies of scene blockIdx.x (all bodies without scenes)
-    const int sidx = blockIdx.x;
-    if (active && !active[sidx]) return;
-    const int row0 = sboff ? sboff[sidx] : 0;
-    const int n = 12 * ((sboff ? sboff[sidx + 1] : M.nb) - row0);
-    b += 12 * (int64_t)row0;
-    xg += 12 * (int64_t)row0;
-    if (iters) iters += sidx;
-    if (resid) resid += sidx;
-    if (n > cap_rows) {  // larger than the vector storage sized by the caller: report, touch nothing
-        if (threadIdx.x == 0 && iters) *iters = -1;
-        return;
-    }
-    double* V = vec_base<LDS>(ws + (LDS ? 0 : 6 * 12 * (int64_t)row0));
-    if (maxiter <= 0) maxiter = n;  // the system size of this workgroup's system
+template <bool BJ, bool ST>
+__device__ __forceinline__ void cg_body(const Bsr& M, const Rows& R, int n, const double* __restrict__ b,
+                                        double* __restrict__ xg, double tol, int maxiter, int check_every,
+                                        double* V, const double* __restrict__ dinv, int32_t* __restrict__ iters,
+                                        double* __restrict__ resid, double* sh) {
+    const int row0 = R.row0;
     double* x = V;
     double* r = V + n;
     double* p = V + 2 * n;
@@ -306,7 +382,7 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
         x[i] = xg[i];
     }
     __syncthreads();
-    spmv(M, row0, x, Ap, n);
+    spmv<ST>(M, R, x, Ap, n);
     for (int i = threadIdx.x; i < n; i += blockDim.x) r[i] = b[i] - Ap[i];
     __syncthreads();
     if (BJ) {
@@ -326,7 +402,7 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
         // fused passes (dots accumulated where their operands are produced, in vdot()'s
         // row order: the iterates match the unfused sequence bit for bit)
         for (int i = 0; i < maxiter; ++i) {
-            const double pAp = block_sum_fast(spmv_rows_dot(M, row0, p, Ap, n, p), sh);
+            const double pAp = block_sum_fast(spmv_rows_dot<ST>(M, R, p, Ap, n, p), sh);
             // unguarded in the reference: p = 0 once the residual vanishes exactly between checks
             const double alpha = pAp != 0.0 ? rz_old / pAp : 0.0;
             double rz_p = 0.0;
@@ -371,6 +447,39 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
         if (iters) *iters = end_iter;
         if (resid) *resid = err;
     }
+}
+
+// BJ: the per-body 12x12 block-Jacobi preconditioner (beyond the reference) in place of
+// the diagonal one; dinv: [nb][144] workspace of the inverses.  cap_ent as bicgstab_kernel.
+template <bool LDS, bool BJ>
+__global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t* __restrict__ sboff,
+                                                           const uint8_t* __restrict__ active,
+                                                           const double* __restrict__ b, double* __restrict__ xg,
+                                                           double tol, int maxiter, int check_every, int cap_rows,
+                                                           int cap_ent, double* __restrict__ ws,
+                                                           double* __restrict__ dinv, int32_t* __restrict__ iters,
+                                                           double* __restrict__ resid) {
+    __shared__ double sh[17];
+    // one system per workgroup: the bodies of scene blockIdx.x (all bodies without scenes)
+    const int sidx = blockIdx.x;
+    if (active && !active[sidx]) return;
+    const int row0 = sboff ? sboff[sidx] : 0;
+    const int n = 12 * ((sboff ? sboff[sidx + 1] : M.nb) - row0);
+    b += 12 * (int64_t)row0;
+    xg += 12 * (int64_t)row0;
+    if (iters) iters += sidx;
+    if (resid) resid += sidx;
+    if (n > cap_rows) {  // larger than the vector storage sized by the caller: report, touch nothing
+        if (threadIdx.x == 0 && iters) *iters = -1;
+        return;
+    }
+    const SolveLds<LDS> S(ws + (LDS ? 0 : 6 * 12 * (int64_t)row0), cap_rows);
+    if (maxiter <= 0) maxiter = n;  // the system size of this workgroup's system
+    Rows R{row0, n / 12, nullptr, nullptr};
+    if (stage_rows(M, R, S.lrp, S.ent, cap_ent))
+        cg_body<BJ, true>(M, R, n, b, xg, tol, maxiter, check_every, S.V, dinv, iters, resid, sh);
+    else
+        cg_body<BJ, false>(M, R, n, b, xg, tol, maxiter, check_every, S.V, dinv, iters, resid, sh);
 }
 
 // Algorithmic bytes of one solve launch, summed over its systems: per iteration
@@ -582,42 +691,61 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
     if (bj && (rc = arena_get(kSlotPrecond, sizeof(double) * 144 * (size_t)M.nb, (void**)&dinv))) return rc;
     if (bj) {
         // every body's diagonal-block inverse, one thread per body (inactive scenes included: cheap)
-        hipLaunchKernelGGL(invert_blocks_kernel, dim3((M.nb + 127) / 128), dim3(128), 0, s, M, dinv);
-        ABD_CHECK_LAUNCH();
+        if (M.nb > 0) {
+            // algorithmic bytes: each diagonal block read, its inverse written
+            const int ph_inv = prof_start(kProfPrecond, s, 2304.0 * M.nb);
+            hipLaunchKernelGGL(invert_blocks_kernel, dim3(M.nb), dim3(kInvThreads), 0, s, M, dinv);
+            ABD_CHECK_LAUNCH();
+            prof_stop(ph_inv, s);
+        }
     }
     int ph = prof_start(kProfSolve, s, 1.0);
     const size_t vec_bytes = sizeof(double) * 6 * (size_t)max_rows;
     const bool lds = vec_bytes <= kLdsVecBytes;
+    // dynamic LDS: the vectors (lds), then the structure region within the device's limit
+    int dev = 0, max_lds = 0;
+    ABD_CHECK(hipGetDevice(&dev));
+    ABD_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const size_t vec_lds = lds ? vec_bytes : 0;
+    const size_t room = (size_t)max_lds > vec_lds + 1024 ? (size_t)max_lds - vec_lds - 1024 : 0;
+    size_t st_bytes = std::min(kLdsStructBytes, room);
+    const size_t lrp_bytes = 8 * (size_t)((max_rows / 12 + 2) / 2);
+    int cap_ent = -1;
+    if (st_bytes >= lrp_bytes)
+        cap_ent = (int)((st_bytes - lrp_bytes) / 8);
+    else
+        st_bytes = 0;
+    const size_t dyn = vec_lds + st_bytes;
+    static bool attr_set = false;
+    if (!attr_set) {
+        const int cap = (int)std::max(kLdsVecBytes, std::min((size_t)max_lds > 1024 ? (size_t)max_lds - 1024 : 0,
+                                                             kLdsVecBytes + kLdsStructBytes));
+        const void* fns[] = {(const void*)bicgstab_kernel<true>,     (const void*)bicgstab_kernel<false>,
+                             (const void*)cg_kernel<true, false>,    (const void*)cg_kernel<true, true>,
+                             (const void*)cg_kernel<false, false>,   (const void*)cg_kernel<false, true>};
+        for (const void* f : fns) ABD_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, cap));
+        attr_set = true;
+    }
     if (lds) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            hipFuncSetAttribute((const void*)bicgstab_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)kLdsVecBytes);
-            hipFuncSetAttribute((const void*)cg_kernel<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)kLdsVecBytes);
-            hipFuncSetAttribute((const void*)cg_kernel<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)kLdsVecBytes);
-            attr_set = true;
-        }
         if (method == ABD_SOLVER_BICGSTAB)
-            hipLaunchKernelGGL(bicgstab_kernel<true>, dim3(nsys), dim3(threads), vec_bytes, s, M, sboff, active, b,
-                               x, tol, maxiter, check_every, max_rows, ws, iters, resid);
+            hipLaunchKernelGGL(bicgstab_kernel<true>, dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x, tol,
+                               maxiter, check_every, max_rows, cap_ent, ws, iters, resid);
         else if (bj)
-            hipLaunchKernelGGL((cg_kernel<true, true>), dim3(nsys), dim3(threads), vec_bytes, s, M, sboff, active, b,
-                               x, tol, maxiter, check_every, max_rows, ws, dinv, iters, resid);
+            hipLaunchKernelGGL((cg_kernel<true, true>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
+                               tol, maxiter, check_every, max_rows, cap_ent, ws, dinv, iters, resid);
         else
-            hipLaunchKernelGGL((cg_kernel<true, false>), dim3(nsys), dim3(threads), vec_bytes, s, M, sboff, active,
-                               b, x, tol, maxiter, check_every, max_rows, ws, dinv, iters, resid);
+            hipLaunchKernelGGL((cg_kernel<true, false>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
+                               tol, maxiter, check_every, max_rows, cap_ent, ws, dinv, iters, resid);
     } else {
         if (method == ABD_SOLVER_BICGSTAB)
-            hipLaunchKernelGGL(bicgstab_kernel<false>, dim3(nsys), dim3(threads), 0, s, M, sboff, active, b, x, tol,
-                               maxiter, check_every, max_rows, ws, iters, resid);
+            hipLaunchKernelGGL(bicgstab_kernel<false>, dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
+                               tol, maxiter, check_every, max_rows, cap_ent, ws, iters, resid);
         else if (bj)
-            hipLaunchKernelGGL((cg_kernel<false, true>), dim3(nsys), dim3(threads), 0, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, ws, dinv, iters, resid);
+            hipLaunchKernelGGL((cg_kernel<false, true>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
+                               tol, maxiter, check_every, max_rows, cap_ent, ws, dinv, iters, resid);
         else
-            hipLaunchKernelGGL((cg_kernel<false, false>), dim3(nsys), dim3(threads), 0, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, ws, dinv, iters, resid);
+            hipLaunchKernelGGL((cg_kernel<false, false>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
+                               tol, maxiter, check_every, max_rows, cap_ent, ws, dinv, iters, resid);
     }
     ABD_CHECK_LAUNCH();
     if (ph >= 0 && iters) {
