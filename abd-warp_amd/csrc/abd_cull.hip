@@ -120,28 +120,41 @@ __global__ __launch_bounds__(kBlock) void body_pairs_kernel(int nb, int64_t npai
 
 // ------------------------------------------------------------- primitives
 
-// Hull of the FP32 boxes of each body's primitives (one workgroup per body).
-__global__ __launch_bounds__(kBlock) void hull_kernel(const int32_t* __restrict__ offs, const float* __restrict__ lo,
-                                                       const float* __restrict__ hi, float* __restrict__ hlo,
-                                                       float* __restrict__ hhi) {
-    __shared__ float sh[kBlock];
-    const int b = blockIdx.x;
-    const int p0 = offs[b], p1 = offs[b + 1];
-    for (int k = 0; k < 3; ++k) {
-        float mn = INFINITY, mx = -INFINITY;
-        for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
-            mn = fminf(mn, lo[3 * (int64_t)p + k]);
-            mx = fmaxf(mx, hi[3 * (int64_t)p + k]);
-        }
-        float a = block_minf(mn, sh);
-        float c = block_maxf(mx, sh);
-        if (threadIdx.x == 0) {
-            hlo[3 * b + k] = a;
-            hhi[3 * b + k] = c;
+// Hull of the FP32 boxes of each body's primitives: one wave per body (kBlock / 64 bodies
+// per workgroup), the six extrema reduced by lane shuffles (a minimum or maximum does not
+// depend on the order).  blockIdx.y selects one of two primitive sets (set 1 only when
+// nsets == 1).
+struct HullSet {
+    const int32_t* offs;
+    const float *lo, *hi;
+    float *hlo, *hhi;
+};
+__global__ __launch_bounds__(kBlock) void hull_kernel(int nb, HullSet H1, HullSet H2) {
+    const HullSet& H = blockIdx.y ? H2 : H1;
+    const int b = blockIdx.x * (kBlock / kWave) + wave_id();
+    if (b >= nb) return;  // the whole wave (no barrier below)
+    const int p0 = H.offs[b], p1 = H.offs[b + 1];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int p = p0 + lane_id(); p < p1; p += kWave) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fminf(mn[k], H.lo[3 * (int64_t)p + k]);
+            mx[k] = fmaxf(mx[k], H.hi[3 * (int64_t)p + k]);
         }
     }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fminf(mn[k], __shfl_xor(mn[k], off, kWave));
+            mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], off, kWave));
+        }
+    if (lane_id() < 3) {
+        const int k = lane_id();
+        H.hlo[3 * b + k] = k == 0 ? mn[0] : (k == 1 ? mn[1] : mn[2]);
+        H.hhi[3 * b + k] = k == 0 ? mx[0] : (k == 1 ? mx[1] : mx[2]);
+    }
 }
-
 
 // unit -> (ii, I (set-1 body), J (set-2 body))
 __device__ __forceinline__ void unit_bodies(int pt, int64_t u, const int32_t* __restrict__ ij, int64_t& ii, int& I,
@@ -412,10 +425,18 @@ int prim_cull_begin(hipStream_t s, const AbdScene* sc, int mode, const int32_t* 
     S.hhi1 = h1 + 3 * nb;
     S.hlo2 = h2;
     S.hhi2 = h2 + 3 * nb;
-    hipLaunchKernelGGL(hull_kernel, dim3(nb), dim3(kBlock), 0, s, S.off1, lo1, hi1, h1, h1 + 3 * nb);
-    ABD_CHECK_LAUNCH();
-    hipLaunchKernelGGL(hull_kernel, dim3(nb), dim3(kBlock), 0, s, S.off2, lo2, hi2, h2, h2 + 3 * nb);
-    ABD_CHECK_LAUNCH();
+    // both hulls in one launch; the EE culling compares one set of boxes with itself
+    const bool same = S.off1 == S.off2 && lo1 == lo2 && hi1 == hi2;
+    if (same) {
+        S.hlo2 = S.hlo1;
+        S.hhi2 = S.hhi1;
+    }
+    if (nb > 0) {
+        const HullSet H1{S.off1, lo1, hi1, h1, h1 + 3 * nb}, H2{S.off2, lo2, hi2, h2, h2 + 3 * nb};
+        hipLaunchKernelGGL(hull_kernel, dim3((unsigned)ceil_div(nb, kBlock / kWave), same ? 1 : 2), dim3(kBlock), 0,
+                           s, nb, H1, H2);
+        ABD_CHECK_LAUNCH();
+    }
     if ((rc = arena_get(slot[2], sizeof(int32_t) * (size_t)(2 * C.nunits * C.maxs), (void**)&C.flist))) return rc;
     if ((rc = arena_get(slot[3], sizeof(int64_t) * (size_t)(2 * C.nunits), (void**)&C.fcnt))) return rc;
     if ((rc = arena_get(slot[4], sizeof(int64_t) * (size_t)(C.nunits + 1), (void**)&C.nchunk))) return rc;
@@ -500,16 +521,9 @@ int vg_cull_fill(hipStream_t s, VgCull& C, int32_t* out, int64_t cap, int64_t n)
     return ABD_OK;
 }
 
-}  // namespace abd
-
-using namespace abd;
-
-extern "C" {
-
-int abd_cull_bodies(abd_stream_t stream, const AbdScene* sc, const float* lo, const float* hi, int32_t* ij,
-                    int32_t* ji, int64_t cap, int64_t* count) {
-    if (!sc || !lo || !hi || !count || cap < 0 || (cap > 0 && (!ij || !ji))) return ABD_ERR_ARG;
-    hipStream_t s = to_stream(stream);
+// abd_cull_bodies; ji == nullptr skips the (j, i) order.
+int cull_bodies(hipStream_t s, const AbdScene* sc, const float* lo, const float* hi, int32_t* ij, int32_t* ji,
+                int64_t cap, int64_t* count) {
     const int nb = sc->n_bodies;
     SceneSeg sg{0, nullptr, nullptr};
     int64_t npairs = (int64_t)nb * (nb - 1) / 2;
@@ -540,6 +554,7 @@ int abd_cull_bodies(abd_stream_t stream, const AbdScene* sc, const float* lo, co
     hipLaunchKernelGGL(body_pairs_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 0, sg,
                        sc->body_scene, lo, hi, bcnt, (const int64_t*)boff, ij, cap);
     ABD_CHECK_LAUNCH();
+    if (!ji) return ABD_OK;
     // (j, i) order: same predicate, second enumeration
     hipLaunchKernelGGL(body_pairs_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s, nb, npairs, 1, sg,
                        sc->body_scene, lo, hi, bcnt, (const int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
@@ -549,6 +564,19 @@ int abd_cull_bodies(abd_stream_t stream, const AbdScene* sc, const float* lo, co
                        sc->body_scene, lo, hi, bcnt, (const int64_t*)boff, ji, cap);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
+}
+
+
+}  // namespace abd
+
+using namespace abd;
+
+extern "C" {
+
+int abd_cull_bodies(abd_stream_t stream, const AbdScene* sc, const float* lo, const float* hi, int32_t* ij,
+                    int32_t* ji, int64_t cap, int64_t* count) {
+    if (!sc || !lo || !hi || !count || cap < 0 || (cap > 0 && (!ij || !ji))) return ABD_ERR_ARG;
+    return cull_bodies(to_stream(stream), sc, lo, hi, ij, ji, cap, count);
 }
 
 int abd_cull_prims(abd_stream_t stream, const AbdScene* sc, int32_t mode, const int32_t* ij, int64_t nij,

@@ -355,29 +355,43 @@ __device__ __forceinline__ void fetch_ee_idx(const SceneView& S, const int32_t* 
 namespace abd {
 // Per-candidate barrier energy (ipcsk/energy.py:10-48) and TOI (ccd.py:9-40), shared
 // by the whole-list and the per-scene kernels.
-__device__ __forceinline__ double cand_energy_ee(const SceneView& S, const double* __restrict__ xk,
-                                                 const int32_t* t5, double d2hat, double kappa) {
+// Vertex positions of the candidate functions: read from an array (Verts), or computed
+// on the fly by another source with the same interface (load4: vertices vi[0..nI) of
+// body I, the rest of body J; y: the y coordinate of vertex vid of body b).
+struct Verts {
+    const double* x;
+    __device__ __forceinline__ void load4(int, int, const int64_t vi[4], int, d3 v[4]) const {
+        for (int k = 0; k < 4; ++k) v[k] = ld3(x + 3 * vi[k]);
+    }
+    __device__ __forceinline__ double y(int, int64_t vid) const { return x[3 * vid + 1]; }
+};
+
+template <class Src>
+__device__ __forceinline__ double cand_energy_ee(const SceneView& S, const Src& X, const int32_t* t5, double d2hat,
+                                                 double kappa) {
     int64_t vi[4];
     fetch_ee_idx(S, t5, vi);
     d3 v[4];
-    for (int k = 0; k < 4; ++k) v[k] = ld3(xk + 3 * vi[k]);
+    X.load4(t5[0], t5[1], vi, 2, v);
     CMap c = cmap_ee(v[0], v[1], v[2], v[3]);
     double d = signed_distance(c.e0, c.e1p, c.e2p);
     return valid_ee(c.beta, c.gamma) ? barrier(d * d, d2hat, kappa) : 0.0;
 }
-__device__ __forceinline__ double cand_energy_pt(const SceneView& S, const double* __restrict__ xk,
-                                                 const int32_t* t5, double d2hat, double kappa) {
+template <class Src>
+__device__ __forceinline__ double cand_energy_pt(const SceneView& S, const Src& X, const int32_t* t5, double d2hat,
+                                                 double kappa) {
     int64_t vi[4];
     fetch_pt_idx(S, t5, vi);
     d3 v[4];
-    for (int k = 0; k < 4; ++k) v[k] = ld3(xk + 3 * vi[k]);
+    X.load4(t5[0], t5[1], vi, 1, v);
     CMap c = cmap_pt(v[0], v[1], v[2], v[3]);
     return valid_pt(c.beta, c.gamma) ? barrier(dot(c.e2p, c.e2p), d2hat, kappa) : 0.0;
 }
-__device__ __forceinline__ double cand_energy_vg(const int32_t* __restrict__ voff, const double* __restrict__ xk,
-                                                 const int32_t* t2, double ground, double d2hat, double kappa) {
+template <class Src>
+__device__ __forceinline__ double cand_energy_vg(const int32_t* __restrict__ voff, const Src& X, const int32_t* t2,
+                                                 double ground, double d2hat, double kappa) {
     const int64_t vid = voff[t2[0]] + t2[1];
-    const double d = xk[3 * vid + 1] - ground;
+    const double d = X.y(t2[0], vid) - ground;
     return barrier(d * d, d2hat, kappa);
 }
 __device__ __forceinline__ double cand_toi(int kind, const SceneView& S, const double* __restrict__ x0,

@@ -33,15 +33,20 @@ __global__ __launch_bounds__(kBlock) void project_kernel(const int32_t* __restri
 
 // abd.py:337-353
 __global__ void update_q_kernel(int nb, const double* __restrict__ A, const double* __restrict__ p,
-                                const double* __restrict__ dq, double alpha, double* A_out, double* p_out) {
+                                const double* __restrict__ dq, double alpha, double* A_out, double* p_out,
+                                double* A_out2, double* p_out2) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 12 * nb) return;
     int b = i / 12, r = i % 12;
     if (r < 3) {
-        p_out[3 * b + r] = p[3 * b + r] - dq[3 * (4 * b) + r] * alpha;
+        const double v = p[3 * b + r] - dq[3 * (4 * b) + r] * alpha;
+        p_out[3 * b + r] = v;
+        if (p_out2) p_out2[3 * b + r] = v;
     } else {
         int row = (r - 3) / 3, k = (r - 3) % 3;
-        A_out[9 * b + 3 * row + k] = A[9 * b + 3 * row + k] - dq[3 * (4 * b + 1 + row) + k] * alpha;
+        const double v = A[9 * b + 3 * row + k] - dq[3 * (4 * b + 1 + row) + k] * alpha;
+        A_out[9 * b + 3 * row + k] = v;
+        if (A_out2) A_out2[9 * b + 3 * row + k] = v;
     }
 }
 
@@ -233,24 +238,44 @@ __global__ __launch_bounds__(kBlock) void swept_body_boxes_kernel(const int32_t*
 // abd.py:337-353 with the step length of the body's scene
 __global__ void update_q_scenes_kernel(int nb, const int32_t* __restrict__ body_scene, const double* __restrict__ A,
                                        const double* __restrict__ p, const double* __restrict__ dq,
-                                       const double* __restrict__ alpha, double* A_out, double* p_out) {
+                                       const double* __restrict__ alpha, double* A_out, double* p_out,
+                                       double* A_out2, double* p_out2) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 12 * nb) return;
     const int b = i / 12, r = i % 12;
     const double al = alpha[body_scene[b]];
-    if (al == 0.0) {  // a scene that does not move keeps its state bit for bit, whatever dq holds
-        if (r < 3)
-            p_out[3 * b + r] = p[3 * b + r];
-        else
-            A_out[9 * b + r - 3] = A[9 * b + r - 3];
-        return;
-    }
-    if (r < 3) {
-        p_out[3 * b + r] = p[3 * b + r] - dq[3 * (4 * b) + r] * al;
-    } else {
+    // a scene that does not move keeps its state bit for bit, whatever dq holds
+    double* out = r < 3 ? p_out + 3 * b + r : A_out + 9 * b + r - 3;
+    double* out2 = r < 3 ? (p_out2 ? p_out2 + 3 * b + r : nullptr) : (A_out2 ? A_out2 + 9 * b + r - 3 : nullptr);
+    double v;
+    if (r < 3)
+        v = al == 0.0 ? p[3 * b + r] : p[3 * b + r] - dq[3 * (4 * b) + r] * al;
+    else {
         const int row = (r - 3) / 3, k = (r - 3) % 3;
-        A_out[9 * b + 3 * row + k] = A[9 * b + 3 * row + k] - dq[3 * (4 * b + 1 + row) + k] * al;
+        v = al == 0.0 ? A[9 * b + 3 * row + k] : A[9 * b + 3 * row + k] - dq[3 * (4 * b + 1 + row) + k] * al;
     }
+    *out = v;
+    if (out2) *out2 = v;
+}
+
+// q_out = q - alpha dq, also written to (A_out2, p_out2) when they are given
+int update_q_dual(hipStream_t s, int nb, const double* A, const double* p, const double* dq, double alpha,
+                  double* A_out, double* p_out, double* A_out2, double* p_out2) {
+    if (nb == 0) return ABD_OK;
+    hipLaunchKernelGGL(update_q_kernel, dim3(grid_for(12 * (int64_t)nb)), dim3(kBlock), 0, s, nb, A, p, dq, alpha,
+                       A_out, p_out, A_out2, p_out2);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int update_q_scenes_dual(hipStream_t s, const AbdScene* sc, const double* A, const double* p, const double* dq,
+                         const double* alpha, double* A_out, double* p_out, double* A_out2, double* p_out2) {
+    const int nb = sc->n_bodies;
+    if (nb == 0) return ABD_OK;
+    hipLaunchKernelGGL(update_q_scenes_kernel, dim3(grid_for(12 * (int64_t)nb)), dim3(kBlock), 0, s, nb,
+                       sc->body_scene, A, p, dq, alpha, A_out, p_out, A_out2, p_out2);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
 }
 
 int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,
@@ -299,10 +324,7 @@ int abd_update_q(abd_stream_t stream, int32_t nb, const double* A, const double*
                  double* A_out, double* p_out) {
     if (nb < 0 || !A || !p || !dq || !A_out || !p_out) return ABD_ERR_ARG;
     if (nb == 0) return ABD_OK;
-    hipLaunchKernelGGL(update_q_kernel, dim3(grid_for(12 * (int64_t)nb)), dim3(kBlock), 0, to_stream(stream), nb, A,
-                       p, dq, alpha, A_out, p_out);
-    ABD_CHECK_LAUNCH();
-    return ABD_OK;
+    return update_q_dual(to_stream(stream), nb, A, p, dq, alpha, A_out, p_out, nullptr, nullptr);
 }
 
 int abd_update_q0qdot(abd_stream_t stream, int32_t nb, double dt, const double* A, const double* p, double* A0,
@@ -387,11 +409,7 @@ int abd_body_boxes_traj(abd_stream_t stream, const AbdScene* sc, const double* x
 int abd_update_q_scenes(abd_stream_t stream, const AbdScene* sc, const double* A, const double* p, const double* dq,
                         const double* alpha, double* A_out, double* p_out) {
     if (!sc || sc->n_scenes <= 0 || !sc->body_scene || !A || !p || !dq || !alpha || !A_out || !p_out) return ABD_ERR_ARG;
-    const int nb = sc->n_bodies;
-    hipLaunchKernelGGL(update_q_scenes_kernel, dim3(grid_for(12 * (int64_t)nb)), dim3(kBlock), 0, to_stream(stream),
-                       nb, sc->body_scene, A, p, dq, alpha, A_out, p_out);
-    ABD_CHECK_LAUNCH();
-    return ABD_OK;
+    return update_q_scenes_dual(to_stream(stream), sc, A, p, dq, alpha, A_out, p_out, nullptr, nullptr);
 }
 
 }  // extern "C"

@@ -100,12 +100,12 @@ int vg_cull_fill(hipStream_t s, VgCull& C, int32_t* out, int64_t cap, int64_t n)
 int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,
                 float* lo, float* hi);
 // K line-search trials q - alpha dq at once (alpha [K][S] on the device, S = 1: the
-// whole system as one scene): trial vertices to xK [K][3 V], per-scene contact
-// energies to E[2 S k + s] and inertia energies to E[2 S k + S + s].
+// whole system as one scene): per-scene contact energies to E[2 S k + s] and inertia
+// energies to E[2 S k + S + s].
 int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
                 const double* A, const double* p, const double* dq, const double* A0, const double* p0,
                 const double* Adot, const double* pdot, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
-                const int32_t* vg, int64_t nvg, const int64_t* plan, double* xK, double* E);
+                const int32_t* vg, int64_t nvg, const int64_t* plan, double* E);
 // The per-scene chunk plan of the energy / TOI reductions over one set of candidate lists
 // (plan: reduce_plan_elems(S) int64 on the device), made once and passed as `plan` to every
 // reduction over those lists (nullptr: each makes its own).
@@ -116,6 +116,15 @@ int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, co
 int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
               const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
               const int64_t* plan, double* toi);
+// abd_update_q / abd_update_q_scenes writing the result to a second state as well (when
+// A_out2, p_out2 are given).
+int update_q_dual(hipStream_t s, int nb, const double* A, const double* p, const double* dq, double alpha,
+                  double* A_out, double* p_out, double* A_out2, double* p_out2);
+int update_q_scenes_dual(hipStream_t s, const AbdScene* sc, const double* A, const double* p, const double* dq,
+                         const double* alpha, double* A_out, double* p_out, double* A_out2, double* p_out2);
+// abd_cull_bodies without the (j, i) order when ji == nullptr.
+int cull_bodies(hipStream_t s, const AbdScene* sc, const float* lo, const float* hi, int32_t* ij, int32_t* ji,
+                int64_t cap, int64_t* count);
 int arena_get(int slot, size_t bytes, void** ptr);
 int arena_host(size_t bytes, void** ptr);  // pinned host memory
 int arena_release_current();

@@ -98,31 +98,70 @@ __device__ __forceinline__ int scene_of_chunk(const int64_t* __restrict__ choff,
     return lo;
 }
 
-// ipcsk/energy.py:10-48 per candidate (OP 0) or ccd.py:9-40 TOI (OP 1)
-template <int OP>
-__device__ __forceinline__ double chunk_term(int kind, const AbdScene& sc, const SceneView& V, const AbdParams& P,
-                                             const double* __restrict__ x0, const double* __restrict__ x1,
-                                             const int32_t* t) {
-    if (OP == 0) {
-        if (kind == kListEE) return cand_energy_ee(V, x0, t, P.d2hat, P.kappa);
-        if (kind == kListPT) return cand_energy_pt(V, x0, t, P.d2hat, P.kappa);
-        return cand_energy_vg(sc.voff, x0, t, P.ground, P.d2hat, P.kappa);
+// Vertex positions at the line-search trial state q - alpha dq of one scene, computed on
+// the fly with the arithmetic of abd_update_q + abd_project_vertices (abd.py:305-353), so
+// that they equal the projected vertices of the updated state bit for bit.
+struct VertsTrial {
+    const double *xr, *A, *p, *dq;  // rest positions, state, step
+    double alpha;
+    __device__ __forceinline__ static d3 place(const double (&Ab)[9], const double (&pb)[3],
+                                               const double* __restrict__ r) {
+        double x[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double v = pb[c] + r[0] * Ab[c];
+            v = v + r[1] * Ab[3 + c];
+            v = v + r[2] * Ab[6 + c];
+            x[c] = v;
+        }
+        return mk(x[0], x[1], x[2]);
     }
+    __device__ __forceinline__ void load4(int I, int J, const int64_t vi[4], int nI, d3 v[4]) const {
+        double AI[9], pI[3], AJ[9], pJ[3];
+        trial_state(I, alpha, A, p, dq, AI, pI);
+        trial_state(J, alpha, A, p, dq, AJ, pJ);
+        for (int k = 0; k < 4; ++k)
+            v[k] = k < nI ? place(AI, pI, xr + 3 * vi[k]) : place(AJ, pJ, xr + 3 * vi[k]);
+    }
+    __device__ __forceinline__ double y(int b, int64_t vid) const {
+        double Ab[9], pb[3];
+        trial_state(b, alpha, A, p, dq, Ab, pb);
+        return place(Ab, pb, xr + 3 * vid).y;
+    }
+};
+
+// ipcsk/energy.py:10-48 per candidate
+template <class Src>
+__device__ __forceinline__ double energy_term(int kind, const AbdScene& sc, const SceneView& V, const AbdParams& P,
+                                              const Src& X, const int32_t* t) {
+    if (kind == kListEE) return cand_energy_ee(V, X, t, P.d2hat, P.kappa);
+    if (kind == kListPT) return cand_energy_pt(V, X, t, P.d2hat, P.kappa);
+    return cand_energy_vg(sc.voff, X, t, P.ground, P.d2hat, P.kappa);
+}
+
+// ccd.py:9-40 TOI per candidate
+__device__ __forceinline__ double toi_term(int kind, const AbdScene& sc, const SceneView& V, const AbdParams& P,
+                                           const double* __restrict__ x0, const double* __restrict__ x1,
+                                           const int32_t* t) {
     if (kind == kListVG) return cand_toi_vg(sc.voff, x0, x1, t, P.ground);
     return cand_toi(kind == kListPT ? 0 : 1, V, x0, x1, t);
 }
 
-// one chunk per workgroup: OP 0 sums energies, OP 1 takes the minimum TOI below 1
-// blockIdx.y: the trial (positions x0 + x_stride y, partials part + part_stride y)
+// The line-search trials of chunk_kernel<2>: step lengths alpha [trial][S] and the state.
+struct TrialArgs {
+    const double *alpha, *A, *p, *dq;
+};
+
+// One chunk per workgroup.  OP 0 sums energies at positions x0; OP 1 takes the minimum
+// TOI below 1 over x0 -> x1; OP 2 sums energies at the trial state blockIdx.y of T
+// (partials part + part_stride blockIdx.y).
 template <int OP>
 __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdParams P, const double* __restrict__ x0,
-                                                            const double* __restrict__ x1, Lists L, int S,
-                                                            const int64_t* __restrict__ seg,
+                                                            const double* __restrict__ x1, TrialArgs T, Lists L,
+                                                            int S, const int64_t* __restrict__ seg,
                                                             const int64_t* __restrict__ choff,
-                                                            double* __restrict__ part, int64_t x_stride,
-                                                            int64_t part_stride) {
+                                                            double* __restrict__ part, int64_t part_stride) {
     __shared__ double sh[kRedThreads];
-    x0 += x_stride * blockIdx.y;
     part += part_stride * blockIdx.y;
     const int64_t c = blockIdx.x;
     if (c >= choff[S]) return;
@@ -143,17 +182,24 @@ __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdPara
     const SceneView V{sc.voff, sc.foff, sc.eoff, sc.tris, sc.edges, sc.x0};
     const int kind = l < kLists ? L.l[l].kind : 0;
     const int w = width_of(kind);
-    double acc = OP == 0 ? 0.0 : 1.0;
+    double acc = OP == 1 ? 1.0 : 0.0;
     if (l < kLists) {
-        for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x) {
-            const double t = chunk_term<OP>(kind, sc, V, P, x0, x1, L.l[l].p + w * i);
-            if (OP == 0)
-                acc += t;
-            else if (t < 1.0 && t < acc)  // `if t < 1: atomic_min`
-                acc = t;
+        if (OP == 0) {
+            const Verts X{x0};
+            for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x)
+                acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i);
+        } else if (OP == 2) {
+            const VertsTrial X{sc.x0, T.A, T.p, T.dq, T.alpha[(int64_t)S * blockIdx.y + s]};
+            for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x)
+                acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i);
+        } else {
+            for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+                const double t = toi_term(kind, sc, V, P, x0, x1, L.l[l].p + w * i);
+                if (t < 1.0 && t < acc) acc = t;  // `if t < 1: atomic_min`
+            }
         }
     }
-    const double r = OP == 0 ? block_sum_fast(acc, sh) : block_min(acc, sh);
+    const double r = OP == 1 ? block_min(acc, sh) : block_sum_fast(acc, sh);
     if (threadIdx.x == 0) part[c] = r;
 }
 
@@ -166,15 +212,15 @@ __global__ __launch_bounds__(kRedThreads) void scene_final_kernel(const int64_t*
     part += part_stride * blockIdx.y;
     out += out_stride * blockIdx.y;
     const int s = blockIdx.x;
-    double acc = OP == 0 ? 0.0 : 1.0;
+    double acc = OP == 1 ? 1.0 : 0.0;
     for (int64_t i = choff[s] + threadIdx.x; i < choff[s + 1]; i += blockDim.x) {
         const double v = part[i];
-        if (OP == 0)
-            acc += v;
-        else
+        if (OP == 1)
             acc = v < acc ? v : acc;
+        else
+            acc += v;
     }
-    const double r = OP == 0 ? block_sum_fast(acc, sh) : block_min(acc, sh);
+    const double r = OP == 1 ? block_min(acc, sh) : block_sum_fast(acc, sh);
     if (threadIdx.x == 0) out[s] = r;
 }
 
@@ -253,8 +299,8 @@ int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
     if (!rc) rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound, (void**)&part);
     if (rc) return rc;
     int ph = prof_start(prof_id, s, prof_bytes);
-    hipLaunchKernelGGL(chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1, L, S, seg,
-                       choff, part, (int64_t)0, (int64_t)0);
+    hipLaunchKernelGGL(chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1, TrialArgs{},
+                       L, S, seg, choff, part, (int64_t)0);
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s);
     hipLaunchKernelGGL(scene_final_kernel<OP>, dim3(S), dim3(kRedThreads), 0, s, choff, part, out, (int64_t)0,
@@ -287,37 +333,11 @@ int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, co
 
 // ---------------------------------------------------------- line-search trials
 // K trial states q - alpha dq evaluated together (alpha: [K][S] on the device; S = 1
-// treats the scene as one): the vertices of trial k (abd.py:305-324 on the trial state,
-// the same arithmetic as abd_update_q + abd_project_vertices), then per scene the contact
-// and inertia energies of every trial, with one plan of the candidate lists for all.
+// treats the scene as one): per scene the contact energy of every trial over the
+// candidate lists, each candidate's vertices placed at the trial state on the fly
+// (VertsTrial), and the inertia energy of every trial, with one plan of the lists for all.
 // E[2S k + s] is the contact energy and E[2S k + S + s] the inertia energy of scene s
 // at trial k, each bit-identical to the single-state reductions at that state.
-__global__ __launch_bounds__(kRedThreads) void project_trials_kernel(int S, const double* __restrict__ alpha,
-                                                                     const int32_t* __restrict__ body_scene,
-                                                                     const int32_t* __restrict__ vert_body,
-                                                                     const double* __restrict__ x0,
-                                                                     const double* __restrict__ A,
-                                                                     const double* __restrict__ p,
-                                                                     const double* __restrict__ dq, int64_t nv,
-                                                                     double* __restrict__ xK) {
-    const int k = blockIdx.y;
-    double* x = xK + 3 * nv * k;
-    for (int64_t v = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kRedThreads) {
-        const int b = vert_body[v];
-        const double al = alpha[(int64_t)S * k + (S > 1 ? body_scene[b] : 0)];
-        double Ab[9], pb[3];
-        trial_state(b, al, A, p, dq, Ab, pb);
-        const double r0 = x0[3 * v], r1 = x0[3 * v + 1], r2 = x0[3 * v + 2];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            double r = pb[c] + r0 * Ab[c];
-            r = r + r1 * Ab[3 + c];
-            r = r + r2 * Ab[6 + c];
-            x[3 * v + c] = r;
-        }
-    }
-}
-
 __global__ __launch_bounds__(kRedThreads) void inertia_trials_kernel(int S, const double* __restrict__ alpha,
                                                                      const int32_t* __restrict__ sboff, int nb,
                                                                      AbdParams P, const double* __restrict__ mass,
@@ -347,12 +367,8 @@ __global__ __launch_bounds__(kRedThreads) void inertia_trials_kernel(int S, cons
 int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
                 const double* A, const double* p, const double* dq, const double* A0, const double* p0,
                 const double* Adot, const double* pdot, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
-                const int32_t* vg, int64_t nvg, const int64_t* plan_in, double* xK, double* E) {
-    const int64_t nv = sc->n_verts;
+                const int32_t* vg, int64_t nvg, const int64_t* plan_in, double* E) {
     int rc;
-    hipLaunchKernelGGL(project_trials_kernel, dim3((unsigned)grid_for(nv, kRedThreads, 4096), K), dim3(kRedThreads), 0,
-                       s, S, alpha, sc->body_scene, sc->vert_body, sc->x0, A, p, dq, nv, xK);
-    ABD_CHECK_LAUNCH();
     const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);
     npt = L.l[1].n;
     nee = L.l[0].n;
@@ -364,8 +380,9 @@ int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, in
     if ((rc = get_plan(s, *sc, S, L, plan_in, &seg, &choff))) return rc;
     if ((rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound * K, (void**)&part))) return rc;
     int ph = prof_start(kProfEnergy, s, K * (128.0 * npt + 132.0 * nee + 32.0 * nvg));
-    hipLaunchKernelGGL(chunk_kernel<0>, dim3((unsigned)bound, K), dim3(kRedThreads), 0, s, *sc, *P,
-                       (const double*)xK, (const double*)nullptr, L, S, seg, choff, part, 3 * nv, bound);
+    hipLaunchKernelGGL(chunk_kernel<2>, dim3((unsigned)bound, K), dim3(kRedThreads), 0, s, *sc, *P,
+                       (const double*)nullptr, (const double*)nullptr, TrialArgs{alpha, A, p, dq}, L, S, seg, choff,
+                       part, bound);
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s);
     hipLaunchKernelGGL(scene_final_kernel<0>, dim3(S, K), dim3(kRedThreads), 0, s, choff, (const double*)part, E,

@@ -76,7 +76,7 @@ struct Sim {
     int32_t* hiters = nullptr;  // pinned solver iterations [ns]
     std::vector<int32_t> scene_it;  // Newton iterations of each scene in the last frame
     // line-search candidates evaluated together: states, vertices and per-candidate scalars
-    DevBuf<double> cand_x, cand_E;  // [kLsSpec] x (3 V | 2 ns): trial vertices and energies
+    DevBuf<double> cand_E;          // [kLsSpec][2 ns] trial energies
     DevBuf<double> cand_alpha;      // [kLsSpec][ns]
     DevBuf<int64_t> plan;           // reduction plan of the swept candidate set (CCD + line search)
     double* hcand = nullptr;        // pinned [kLsSpec][ns] alphas, then [kLsSpec][2 ns] energies
@@ -145,7 +145,9 @@ int cull_lists(hipStream_t s, Sim& S, const BoxBuf& body, const BoxBuf& pts, con
 int body_pairs(hipStream_t s, Sim& S, const BoxBuf& body, bool structure) {
     for (;;) {
         int64_t cnt = 0;
-        int rc = abd_cull_bodies(s, &S.sc, body.lo.p, body.hi.p, S.ij.p, S.ji.p, (int64_t)(S.ij.cap / 2), &cnt);
+        // the (j, i) order only feeds the BSR structure
+        int rc = cull_bodies(s, &S.sc, body.lo.p, body.hi.p, S.ij.p, structure ? S.ji.p : nullptr,
+                             (int64_t)(S.ij.cap / 2), &cnt);
         if (rc == ABD_ERR_OVERFLOW) {
             if ((rc = S.ij.reserve(2 * (size_t)cnt))) return rc;
             if ((rc = S.ji.reserve(2 * (size_t)cnt))) return rc;
@@ -222,7 +224,7 @@ int line_energy_multi(hipStream_t s, Sim& S, const double* alphas, int K, double
     for (int k = 0; k < K; ++k) ha[k] = alphas[k];
     ABD_CHECK(hipMemcpyAsync(S.cand_alpha.p, ha, sizeof(double) * K, hipMemcpyHostToDevice, s));
     if ((rc = line_trials(s, &S.sc, &S.P, 1, K, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
-                          S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p, S.cand_x.p, S.cand_E.p)))
+                          S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p, S.cand_E.p)))
         return rc;
     ABD_CHECK(hipMemcpyAsync(he, S.cand_E.p, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, s));
     ABD_CHECK(hipStreamSynchronize(s));
@@ -320,10 +322,8 @@ int step(hipStream_t s, Sim& S) {
             alpha = trial[K - 1] * 0.5;
         }
         E0 = E1;
-        if ((rc = abd_update_q(s, nb, st.A, st.p, st.dq, alpha, st.A, st.p))) return rc;
-        // (Ak, pk) = q - alpha dq, the accepted trial (the same expression as the update above)
-        ABD_CHECK(hipMemcpyAsync(st.Ak, st.A, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
-        ABD_CHECK(hipMemcpyAsync(st.pk, st.p, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+        // q and (Ak, pk) = q - alpha dq, the accepted trial
+        if ((rc = update_q_dual(s, nb, st.A, st.p, st.dq, alpha, st.A, st.p, st.Ak, st.pk))) return rc;
         ++it;
         // g.g (abd.py:265) and the solver's iteration count in one read-back
         if ((rc = abd_dot(s, st.g, st.g, 12 * (int64_t)nb, S.scal.p + 3))) return rc;
@@ -382,7 +382,7 @@ int line_energy_scenes_multi(hipStream_t s, Sim& S, const double* dalpha, int K)
     const AbdSimState& st = S.st;
     const int ns = S.ns;
     if ((rc = line_trials(s, &S.sc, &S.P, ns, K, dalpha, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot, S.ptl.p,
-                          S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p, S.cand_x.p, S.cand_E.p)))
+                          S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p, S.cand_E.p)))
         return rc;
     ABD_CHECK(hipMemcpyAsync(S.hcand + (size_t)kLsSpec * ns, S.cand_E.p, sizeof(double) * 2 * ns * K,
                              hipMemcpyDeviceToHost, s));
@@ -499,10 +499,8 @@ int step_scenes(hipStream_t s, Sim& S) {
         for (int q = 0; q < ns; ++q)
             if (!newton[q]) S.halpha[q] = 0.0;
         if ((rc = upload(s, dalpha, S.halpha, sizeof(double) * ns))) return rc;
-        if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha, st.A, st.p))) return rc;
-        // (Ak, pk) = the accepted trial of every scene (the same expression as the update above)
-        ABD_CHECK(hipMemcpyAsync(st.Ak, st.A, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
-        ABD_CHECK(hipMemcpyAsync(st.pk, st.p, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
+        // q and (Ak, pk) = the accepted trial of every scene
+        if ((rc = update_q_scenes_dual(s, &S.sc, st.A, st.p, st.dq, dalpha, st.A, st.p, st.Ak, st.pk))) return rc;
         if ((rc = abd_scene_dot(s, &S.sc, st.g, st.g, S.sv.p + kSvGtg * ns))) return rc;
         ABD_CHECK(hipMemcpyAsync(S.hiters, S.iters.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
         ABD_CHECK(hipMemcpyAsync(S.hsv + kSvGtg * ns, S.sv.p + kSvGtg * ns, sizeof(double) * ns,
@@ -572,7 +570,6 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
         rc = ABD_ERR_ARG;
     {
         const int nsc = scene->n_scenes > 1 ? scene->n_scenes : 1;
-        if (!rc) rc = S->cand_x.reserve(3 * (size_t)S->nv * kLsSpec);
         if (!rc) rc = S->cand_E.reserve(2 * (size_t)nsc * kLsSpec);
         if (!rc) rc = S->cand_alpha.reserve((size_t)nsc * kLsSpec);
         if (!rc) rc = S->plan.reserve(reduce_plan_elems(nsc));
@@ -726,7 +723,6 @@ int abd_sim_destroy(void* sim) {
     S->iters.release();
     S->sv.release();
     S->sactive.release();
-    S->cand_x.release();
     S->cand_E.release();
     S->cand_alpha.release();
     S->plan.release();
