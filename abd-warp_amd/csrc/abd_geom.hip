@@ -138,33 +138,52 @@ __global__ __launch_bounds__(kBlock) void prim_traj_kernel(const int32_t* __rest
     }
 }
 
-// culling2.py:171-210: one workgroup per body reduces its vertices in FP64.
-// traj = 0: lo/hi = f32(min/max -+ dil); traj = 1: lo/hi = min/max(lo/hi, f32(min/max)).
-__global__ __launch_bounds__(kBlock) void body_boxes_kernel(const int32_t* __restrict__ voff,
+// FP64 extrema of each coordinate over a body's vertices [v0, v1), one wave per body:
+// lanes stride the vertices, then a shuffle tree (a minimum or maximum does not depend
+// on the order).  Every lane receives the result.
+__device__ __forceinline__ void wave_extrema(const double* __restrict__ x, int v0, int v1, double (&mn)[3],
+                                             double (&mx)[3]) {
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = 1.0 / 0.0;
+        mx[k] = -1.0 / 0.0;
+    }
+    for (int v = v0 + lane_id(); v < v1; v += kWave)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double c = x[3 * (int64_t)v + k];
+            mn[k] = c < mn[k] ? c : mn[k];
+            mx[k] = c > mx[k] ? c : mx[k];
+        }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double a = __shfl_xor(mn[k], off, kWave), b = __shfl_xor(mx[k], off, kWave);
+            mn[k] = a < mn[k] ? a : mn[k];
+            mx[k] = b > mx[k] ? b : mx[k];
+        }
+}
+
+// culling2.py:171-210: the FP64 extrema of each body's vertices, one wave per body
+// (kBlock / 64 bodies per workgroup).  traj = 0: lo/hi = f32(min/max -+ dil);
+// traj = 1: lo/hi = min/max(lo/hi, f32(min/max)).
+__global__ __launch_bounds__(kBlock) void body_boxes_kernel(int nb, const int32_t* __restrict__ voff,
                                                              const double* __restrict__ x, double dil, int traj,
                                                              float* __restrict__ lo, float* __restrict__ hi) {
-    __shared__ double sh[kBlock];
-    const int b = blockIdx.x;
-    const int v0 = voff[b], v1 = voff[b + 1];
-    for (int k = 0; k < 3; ++k) {
-        double mn = 1.0 / 0.0, mx = -1.0 / 0.0;
-        for (int v = v0 + threadIdx.x; v < v1; v += kBlock) {
-            double c = x[3 * (int64_t)v + k];
-            mn = c < mn ? c : mn;
-            mx = c > mx ? c : mx;
-        }
-        double gmn = block_min(mn, sh);
-        double gmx = -block_min(-mx, sh);
-        if (threadIdx.x == 0) {
+    const int b = blockIdx.x * (kBlock / kWave) + wave_id();
+    if (b >= nb) return;  // the whole wave (no barrier)
+    double mn[3], mx[3];
+    wave_extrema(x, voff[b], voff[b + 1], mn, mx);
+    if (lane_id() == 0)
+        for (int k = 0; k < 3; ++k) {
             if (traj) {
-                hi[3 * b + k] = fmaxf(hi[3 * b + k], (float)gmx);
-                lo[3 * b + k] = fminf(lo[3 * b + k], (float)gmn);
+                hi[3 * b + k] = fmaxf(hi[3 * b + k], (float)mx[k]);
+                lo[3 * b + k] = fminf(lo[3 * b + k], (float)mn[k]);
             } else {
-                lo[3 * b + k] = (float)(gmn - dil);
-                hi[3 * b + k] = (float)(gmx + dil);
+                lo[3 * b + k] = (float)(mn[k] - dil);
+                hi[3 * b + k] = (float)(mx[k] + dil);
             }
         }
-    }
 }
 
 // Swept boxes with the collision-set dilation at both ends: the union of a primitive's
@@ -210,29 +229,20 @@ __global__ __launch_bounds__(kBlock) void swept_prim_boxes_kernel(const int32_t*
         }
     }
 }
-__global__ __launch_bounds__(kBlock) void swept_body_boxes_kernel(const int32_t* __restrict__ voff,
+__global__ __launch_bounds__(kBlock) void swept_body_boxes_kernel(int nb, const int32_t* __restrict__ voff,
                                                                    const double* __restrict__ x0,
                                                                    const double* __restrict__ x1, double dil,
                                                                    float* __restrict__ lo, float* __restrict__ hi) {
-    __shared__ double sh[kBlock];
-    const int b = blockIdx.x;
-    const int v0 = voff[b], v1 = voff[b + 1];
-    for (int k = 0; k < 3; ++k) {
-        double mn0 = 1.0 / 0.0, mx0 = -1.0 / 0.0, mn1 = 1.0 / 0.0, mx1 = -1.0 / 0.0;
-        for (int v = v0 + threadIdx.x; v < v1; v += kBlock) {
-            const double c0 = x0[3 * (int64_t)v + k], c1 = x1[3 * (int64_t)v + k];
-            mn0 = c0 < mn0 ? c0 : mn0;
-            mx0 = c0 > mx0 ? c0 : mx0;
-            mn1 = c1 < mn1 ? c1 : mn1;
-            mx1 = c1 > mx1 ? c1 : mx1;
+    const int b = blockIdx.x * (kBlock / kWave) + wave_id();
+    if (b >= nb) return;  // the whole wave (no barrier)
+    double mn0[3], mx0[3], mn1[3], mx1[3];
+    wave_extrema(x0, voff[b], voff[b + 1], mn0, mx0);
+    wave_extrema(x1, voff[b], voff[b + 1], mn1, mx1);
+    if (lane_id() == 0)
+        for (int k = 0; k < 3; ++k) {
+            lo[3 * b + k] = fminf((float)(mn0[k] - dil), (float)(mn1[k] - dil));
+            hi[3 * b + k] = fmaxf((float)(mx0[k] + dil), (float)(mx1[k] + dil));
         }
-        const double gmn0 = block_min(mn0, sh), gmx0 = -block_min(-mx0, sh);
-        const double gmn1 = block_min(mn1, sh), gmx1 = -block_min(-mx1, sh);
-        if (threadIdx.x == 0) {
-            lo[3 * b + k] = fminf((float)(gmn0 - dil), (float)(gmn1 - dil));
-            hi[3 * b + k] = fmaxf((float)(gmx0 + dil), (float)(gmx1 + dil));
-        }
-    }
 }
 
 // abd.py:337-353 with the step length of the body's scene
@@ -294,8 +304,8 @@ int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, c
                            sc->voff, sc->tris, x0, x1, (int64_t)sc->n_tris, dil, lo, hi);
     } else if (kind == -1) {
         if (!sc->n_bodies) return ABD_OK;
-        hipLaunchKernelGGL(swept_body_boxes_kernel, dim3(sc->n_bodies), dim3(kBlock), 0, s, sc->voff, x0, x1, dil, lo,
-                           hi);
+        hipLaunchKernelGGL(swept_body_boxes_kernel, dim3((unsigned)ceil_div(sc->n_bodies, kBlock / kWave)),
+                           dim3(kBlock), 0, s, sc->n_bodies, sc->voff, x0, x1, dil, lo, hi);
     } else {
         return ABD_ERR_ARG;
     }
@@ -391,8 +401,8 @@ int abd_prim_boxes_traj(abd_stream_t stream, const AbdScene* sc, int32_t kind, c
 int abd_body_boxes(abd_stream_t stream, const AbdScene* sc, const double* x, double dil, float* lo, float* hi) {
     if (!sc || !x || !lo || !hi) return ABD_ERR_ARG;
     if (!sc->n_bodies) return ABD_OK;
-    hipLaunchKernelGGL(body_boxes_kernel, dim3(sc->n_bodies), dim3(kBlock), 0, to_stream(stream), sc->voff, x, dil, 0,
-                       lo, hi);
+    hipLaunchKernelGGL(body_boxes_kernel, dim3((unsigned)ceil_div(sc->n_bodies, kBlock / kWave)), dim3(kBlock), 0,
+                       to_stream(stream), sc->n_bodies, sc->voff, x, dil, 0, lo, hi);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }
@@ -400,8 +410,8 @@ int abd_body_boxes(abd_stream_t stream, const AbdScene* sc, const double* x, dou
 int abd_body_boxes_traj(abd_stream_t stream, const AbdScene* sc, const double* x_end, float* lo, float* hi) {
     if (!sc || !x_end || !lo || !hi) return ABD_ERR_ARG;
     if (!sc->n_bodies) return ABD_OK;
-    hipLaunchKernelGGL(body_boxes_kernel, dim3(sc->n_bodies), dim3(kBlock), 0, to_stream(stream), sc->voff, x_end,
-                       0.0, 1, lo, hi);
+    hipLaunchKernelGGL(body_boxes_kernel, dim3((unsigned)ceil_div(sc->n_bodies, kBlock / kWave)), dim3(kBlock), 0,
+                       to_stream(stream), sc->n_bodies, sc->voff, x_end, 0.0, 1, lo, hi);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }

@@ -53,28 +53,33 @@ __device__ __forceinline__ int64_t scene_lower_bound(const ListRef& L, const int
 
 __device__ __forceinline__ int64_t n_chunks(int64_t len) { return (len + kChunkItems - 1) / kChunkItems; }
 
-// Per scene: the segment [a, b) of each list and the offset of its first chunk.
-// One workgroup; scenes in tiles of blockDim with a running offset.
-__global__ __launch_bounds__(kRedThreads) void plan_kernel(int S, const int32_t* __restrict__ body_scene, Lists L,
-                                                           int64_t* __restrict__ seg, int64_t* __restrict__ choff) {
-    __shared__ int64_t sh[kRedThreads];
+// Per scene: the segment [a, b) of each list and the offset of its first chunk.  One
+// workgroup: first the S + 1 scene boundaries of every list, one binary search per
+// thread (scene s ends where scene s + 1 begins), then the chunk counts of the scenes in
+// tiles of blockDim with a running offset.
+constexpr int kPlanThreads = 1024;
+__global__ __launch_bounds__(kPlanThreads) void plan_kernel(int S, const int32_t* __restrict__ body_scene, Lists L,
+                                                            int64_t* __restrict__ seg, int64_t* __restrict__ choff) {
+    __shared__ int64_t sh[kPlanThreads];
     __shared__ int64_t tile_total;
+    for (int idx = threadIdx.x; idx < kLists * (S + 1); idx += blockDim.x) {
+        const int l = idx / (S + 1), s = idx % (S + 1);
+        int64_t v;
+        if (S == 1)
+            v = s == 0 ? 0 : L.l[l].n;
+        else
+            v = L.l[l].n > 0 ? scene_lower_bound(L.l[l], body_scene, s) : 0;
+        if (s < S) seg[2 * (kLists * (int64_t)s + l)] = v;
+        if (s > 0) seg[2 * (kLists * (int64_t)(s - 1) + l) + 1] = v;
+    }
+    __syncthreads();  // the boundaries written above are read by other threads below
     int64_t carry = 0;
     for (int base = 0; base < S; base += blockDim.x) {
         const int s = base + threadIdx.x;
         int64_t cnt = 0;
-        if (s < S) {
-            for (int l = 0; l < kLists; ++l) {
-                int64_t a = 0, b = L.l[l].n;
-                if (S > 1 && L.l[l].n > 0) {
-                    a = scene_lower_bound(L.l[l], body_scene, s);
-                    b = scene_lower_bound(L.l[l], body_scene, s + 1);
-                }
-                seg[2 * (kLists * (int64_t)s + l)] = a;
-                seg[2 * (kLists * (int64_t)s + l) + 1] = b;
-                cnt += n_chunks(b - a);
-            }
-        }
+        if (s < S)
+            for (int l = 0; l < kLists; ++l)
+                cnt += n_chunks(seg[2 * (kLists * (int64_t)s + l) + 1] - seg[2 * (kLists * (int64_t)s + l)]);
         const int64_t excl = block_exclusive_scan_i64(cnt, sh);
         if (s < S) choff[s] = carry + excl;
         if (threadIdx.x == blockDim.x - 1) tile_total = excl + cnt;
@@ -129,6 +134,35 @@ struct VertsTrial {
         return place(Ab, pb, xr + 3 * vid).y;
     }
 };
+
+// VertsTrial with the trial states of the bodies [b0, b0 + n) computed once per workgroup
+// into LDS (st: 12 doubles per body, A then p).
+struct VertsTrialLds {
+    const double* xr;
+    const double* st;
+    int b0;
+    __device__ __forceinline__ void state(int b, double (&Ab)[9], double (&pb)[3]) const {
+        const double* q = st + 12 * (b - b0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Ab[i] = q[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) pb[i] = q[9 + i];
+    }
+    __device__ __forceinline__ void load4(int I, int J, const int64_t vi[4], int nI, d3 v[4]) const {
+        double AI[9], pI[3], AJ[9], pJ[3];
+        state(I, AI, pI);
+        state(J, AJ, pJ);
+        for (int k = 0; k < 4; ++k)
+            v[k] = k < nI ? VertsTrial::place(AI, pI, xr + 3 * vi[k]) : VertsTrial::place(AJ, pJ, xr + 3 * vi[k]);
+    }
+    __device__ __forceinline__ double y(int b, int64_t vid) const {
+        double Ab[9], pb[3];
+        state(b, Ab, pb);
+        return VertsTrial::place(Ab, pb, xr + 3 * vid).y;
+    }
+};
+// bodies of a scene whose trial states a chunk workgroup keeps in LDS
+constexpr int kTrialLdsBodies = 128;
 
 // ipcsk/energy.py:10-48 per candidate
 template <class Src>
@@ -189,9 +223,26 @@ __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdPara
             for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x)
                 acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i);
         } else if (OP == 2) {
-            const VertsTrial X{sc.x0, T.A, T.p, T.dq, T.alpha[(int64_t)S * blockIdx.y + s]};
-            for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x)
-                acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i);
+            const double al = T.alpha[(int64_t)S * blockIdx.y + s];
+            const int b0 = S > 1 ? sc.scene_boff[s] : 0, b1 = S > 1 ? sc.scene_boff[s + 1] : sc.n_bodies;
+            if (b1 - b0 <= kTrialLdsBodies) {  // uniform over the workgroup
+                __shared__ double st[12 * kTrialLdsBodies];
+                for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+                    double Ab[9], pb[3];
+                    trial_state(b, al, T.A, T.p, T.dq, Ab, pb);
+                    double* q = st + 12 * (b - b0);
+                    for (int i = 0; i < 9; ++i) q[i] = Ab[i];
+                    for (int i = 0; i < 3; ++i) q[9 + i] = pb[i];
+                }
+                __syncthreads();
+                const VertsTrialLds X{sc.x0, st, b0};
+                for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x)
+                    acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i);
+            } else {
+                const VertsTrial X{sc.x0, T.A, T.p, T.dq, al};
+                for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x)
+                    acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i);
+            }
         } else {
             for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x) {
                 const double t = toi_term(kind, sc, V, P, x0, x1, L.l[l].p + w * i);
@@ -279,7 +330,7 @@ int get_plan(hipStream_t s, const AbdScene& sc, int S, const Lists& L, const int
     int64_t* plan;
     int rc = arena_get(kSlotPlan, sizeof(int64_t) * reduce_plan_elems(S), (void**)&plan);
     if (rc) return rc;
-    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kRedThreads), 0, s, S, sc.body_scene, L, plan,
+    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kPlanThreads), 0, s, S, sc.body_scene, L, plan,
                        plan + 2 * kLists * (size_t)S);
     ABD_CHECK_LAUNCH();
     *seg = plan;
@@ -325,7 +376,7 @@ size_t reduce_plan_elems(int S) { return 2 * kLists * (size_t)S + (size_t)S + 1;
 int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const int32_t* pt, int64_t npt,
                 const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t* plan) {
     const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);
-    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kRedThreads), 0, s, S, sc->body_scene, L, plan,
+    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kPlanThreads), 0, s, S, sc->body_scene, L, plan,
                        plan + 2 * kLists * (size_t)S);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
