@@ -157,6 +157,12 @@ def profile_end():
     call("abd_profile_end")
 
 
+def profile_select(names=None):
+    """Times only the kernels named (all of PROF_NAMES when None) while profiling."""
+    mask = 0xFFFFFFFF if names is None else sum(1 << PROF_NAMES.index(n) for n in names)
+    call("abd_profile_select", C.c_uint32(mask))
+
+
 def profile_query() -> dict:
     """{kernel name: (total ms, launches, algorithmic bytes)} since profile_begin()."""
     out = {}

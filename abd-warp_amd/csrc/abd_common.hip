@@ -96,7 +96,8 @@ constexpr int kScanItems = 8;
 
 __global__ __launch_bounds__(kScanThreads) void scan_i64_kernel(const int64_t* __restrict__ in,
                                                                  int64_t* __restrict__ out, int64_t n,
-                                                                 int64_t* __restrict__ total) {
+                                                                 int64_t* __restrict__ total,
+                                                                 int64_t* __restrict__ total2) {
     __shared__ int64_t sh[kScanThreads];
     int64_t carry = 0;
     const int t = threadIdx.x;
@@ -125,10 +126,11 @@ __global__ __launch_bounds__(kScanThreads) void scan_i64_kernel(const int64_t* _
         __syncthreads();
     }
     if (t == 0 && total) *total = carry;
+    if (t == 0 && total2) *total2 = carry;
 }
 
-int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total) {
-    hipLaunchKernelGGL(scan_i64_kernel, dim3(1), dim3(kScanThreads), 0, s, in, out, n, total);
+int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int64_t* total2) {
+    hipLaunchKernelGGL(scan_i64_kernel, dim3(1), dim3(kScanThreads), 0, s, in, out, n, total, total2);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }

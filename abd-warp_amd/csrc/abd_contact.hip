@@ -414,11 +414,13 @@ __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdP
     const int b = blockIdx.x;
     const int t = threadIdx.x;
     const int nb = sc.n_bodies;
-    if (t >= 156) return;
+    // threads 0..143: the entries of the diagonal block; 144..155: the gradient; the rest
+    // only help to evaluate the vertex-ground terms
+    const bool act = t < 156;
     const bool isg = t >= 144;
     const int R = isg ? t - 144 : t / 12, C = isg ? 0 : t % 12;
     double acc = 0.0;
-    if (nij > 0) {
+    if (act && nij > 0) {
         for (int e = row_ptr[b]; e < row_ptr[b + 1]; ++e) {
             const int s = slots[e];
             if (s < nb) continue;
@@ -429,29 +431,44 @@ __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdP
         }
     }
     if (P.vg && nvg > 0) {
-        // ipcsk/vg_ipc.py:5-34 (no 2 b' grad d grad d^T term, D11)
+        // ipcsk/vg_ipc.py:5-34 (no 2 b' grad d grad d^T term, D11).  The barrier derivatives
+        // of each contact are evaluated once per tile into LDS, then every entry adds the
+        // tile's terms in contact order.
+        __shared__ double s_d[kBlock], s_db[kBlock], s_d2b[kBlock], s_vt[kBlock][3];
         const int64_t k0 = lower_bound_body(vg, nvg, b), k1 = lower_bound_body(vg, nvg, b + 1);
         const int ia = R / 3, ra = R % 3, ja = C / 3, ca = C % 3;
-        for (int64_t k = k0; k < k1; ++k) {
-            const int64_t vid = sc.voff[b] + vg[2 * k + 1];
-            const double y = x[3 * vid + 1];
-            const d3 vt = ld3(sc.x0 + 3 * vid);
-            const double d = y - P.ground;
-            const double d2b = barrier_d2(d * d, P.d2hat, P.kappa);
-            const double db = barrier_d1(d * d, P.d2hat, P.kappa);
-            const double th_i = ia == 0 ? 1.0 : comp(vt, ia - 1);
-            if (isg) {
-                const double nr = ra == 1 ? 1.0 : 0.0;
-                acc += (((d * 2.0) * nr) * th_i) * db;
-            } else {
-                const double th_j = ja == 0 ? 1.0 : comp(vt, ja - 1);
-                double scl = th_i * th_j * d2b * 4.0 * d * d;
-                if (P.psd == 2) scl += th_i * th_j * 2.0 * db;  // exact mode: the b' term
-                const double nn = (ra == 1 ? 1.0 : 0.0) * (ca == 1 ? 1.0 : 0.0);
-                acc += nn * scl;
+        for (int64_t base = k0; base < k1; base += kBlock) {
+            const int nt = (int)(k1 - base < kBlock ? k1 - base : kBlock);
+            if (t < nt) {
+                const int64_t vid = sc.voff[b] + vg[2 * (base + t) + 1];
+                const double d = x[3 * vid + 1] - P.ground;
+                s_d[t] = d;
+                s_d2b[t] = barrier_d2(d * d, P.d2hat, P.kappa);
+                s_db[t] = barrier_d1(d * d, P.d2hat, P.kappa);
+                s_vt[t][0] = sc.x0[3 * vid];
+                s_vt[t][1] = sc.x0[3 * vid + 1];
+                s_vt[t][2] = sc.x0[3 * vid + 2];
             }
+            __syncthreads();
+            if (act)
+                for (int q = 0; q < nt; ++q) {
+                    const double d = s_d[q], d2b = s_d2b[q], db = s_db[q];
+                    const double th_i = ia == 0 ? 1.0 : s_vt[q][ia - 1];
+                    if (isg) {
+                        const double nr = ra == 1 ? 1.0 : 0.0;
+                        acc += (((d * 2.0) * nr) * th_i) * db;
+                    } else {
+                        const double th_j = ja == 0 ? 1.0 : s_vt[q][ja - 1];
+                        double scl = th_i * th_j * d2b * 4.0 * d * d;
+                        if (P.psd == 2) scl += th_i * th_j * 2.0 * db;  // exact mode: the b' term
+                        const double nn = (ra == 1 ? 1.0 : 0.0) * (ca == 1 ? 1.0 : 0.0);
+                        acc += nn * scl;
+                    }
+                }
+            __syncthreads();
         }
     }
+    if (!act) return;
     if (isg) {
         g[3 * (4 * (int64_t)b + R / 3) + R % 3] += acc;
     } else {

@@ -398,7 +398,8 @@ const int kCullSlots[2][8] = {
 }  // namespace
 
 int prim_cull_begin(hipStream_t s, const AbdScene* sc, int mode, const int32_t* ij, int64_t nij, const float* lo1,
-                    const float* hi1, const float* lo2, const float* hi2, int set, PrimCull& C) {
+                    const float* hi1, const float* lo2, const float* hi2, int set, PrimCull& C,
+                    int64_t* total_copy) {
     const int* slot = kCullSlots[set];
     C.set = set;
     C.pt = mode == ABD_CULL_PT;
@@ -450,13 +451,13 @@ int prim_cull_begin(hipStream_t s, const AbdScene* sc, int mode, const int32_t* 
     hipLaunchKernelGGL(unit_chunks_kernel, dim3(grid_for(C.nunits)), dim3(kBlock), 0, s, C.nunits, C.tune.cull_chunk,
                        C.fcnt, C.nchunk);
     ABD_CHECK_LAUNCH();
-    if ((rc = scan_exclusive_i64(s, C.nchunk, C.choff, C.nunits, C.choff + C.nunits))) return rc;
+    if ((rc = scan_exclusive_i64(s, C.nchunk, C.choff, C.nunits, C.choff + C.nunits, total_copy))) return rc;
     C.total_chunks = C.choff + C.nunits;
     C.total_pairs = nullptr;
     return ABD_OK;
 }
 
-int prim_cull_count(hipStream_t s, PrimCull& C, int64_t T) {
+int prim_cull_count(hipStream_t s, PrimCull& C, int64_t T, int64_t* total_copy) {
     const int* slot = kCullSlots[C.set];
     C.T = T;
     if (T == 0) return ABD_OK;
@@ -470,7 +471,7 @@ int prim_cull_count(hipStream_t s, PrimCull& C, int64_t T) {
                        (const int64_t*)C.total_chunks, C.ccnt, (const int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s);
-    if ((rc = scan_exclusive_i64(s, C.ccnt, C.ccoff, T, C.ccoff + T))) return rc;
+    if ((rc = scan_exclusive_i64(s, C.ccnt, C.ccoff, T, C.ccoff + T, total_copy))) return rc;
     C.total_pairs = C.ccoff + T;
     return ABD_OK;
 }
@@ -487,7 +488,8 @@ int prim_cull_fill(hipStream_t s, PrimCull& C, int32_t* out, int64_t cap, int64_
 }
 
 int vg_cull_begin(hipStream_t s, const AbdScene* sc, const float* blo, const float* plo, const float* phi, double dhat,
-                  double ground, VgCull& C) {
+                  double ground, VgCull& C,
+                  int64_t* total_copy) {
     C.sc = sc;
     C.blo = blo;
     C.plo = plo;
@@ -506,7 +508,7 @@ int vg_cull_begin(hipStream_t s, const AbdScene* sc, const float* blo, const flo
                        plo, phi, C.yth, C.fg, C.bcnt, (const int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s);
-    if ((rc = scan_exclusive_i64(s, C.bcnt, C.boff, C.nblk, C.boff + C.nblk))) return rc;
+    if ((rc = scan_exclusive_i64(s, C.bcnt, C.boff, C.nblk, C.boff + C.nblk, total_copy))) return rc;
     C.total = C.boff + C.nblk;
     return ABD_OK;
 }

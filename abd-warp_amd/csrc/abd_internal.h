@@ -87,12 +87,16 @@ struct VgCull {
     int64_t *bcnt = nullptr, *boff = nullptr;
     const int64_t* total = nullptr;  // device, after begin
 };
+// total_copy (device, nullable): a second place for the stage's total, so that a caller
+// can gather the totals of several cullings into one buffer and read them with one copy.
 int prim_cull_begin(hipStream_t s, const AbdScene* sc, int mode, const int32_t* ij, int64_t nij, const float* lo1,
-                    const float* hi1, const float* lo2, const float* hi2, int set, PrimCull& C);
-int prim_cull_count(hipStream_t s, PrimCull& C, int64_t T);
+                    const float* hi1, const float* lo2, const float* hi2, int set, PrimCull& C,
+                    int64_t* total_copy = nullptr);
+int prim_cull_count(hipStream_t s, PrimCull& C, int64_t T, int64_t* total_copy = nullptr);
 int prim_cull_fill(hipStream_t s, PrimCull& C, int32_t* out, int64_t cap, int64_t n);
 int vg_cull_begin(hipStream_t s, const AbdScene* sc, const float* blo, const float* plo, const float* phi, double dhat,
-                  double ground, VgCull& C);
+                  double ground, VgCull& C,
+                  int64_t* total_copy = nullptr);
 int vg_cull_fill(hipStream_t s, VgCull& C, int32_t* out, int64_t cap, int64_t n);
 
 // Swept boxes dilated at both ends (kind ABD_KIND_*, or -1 for body boxes): the union of
@@ -131,7 +135,8 @@ int arena_release_current();
 
 // Exclusive scan of n int64 counts (device) in place into `out` (may alias),
 // writing the total to *total (device).  Fixed-order, single workgroup passes.
-int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total);
+int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total,
+                       int64_t* total2 = nullptr);
 // Deterministic sum of n doubles: *out (device) = sum, fixed order.
 // Copies one int64 from device to host (synchronizes `s`).
 int read_i64(hipStream_t s, const int64_t* dev, int64_t* host);

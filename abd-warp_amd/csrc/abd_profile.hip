@@ -25,6 +25,7 @@ int g_scale_used = 0;
 int g_bytes_used = 0;
 std::mutex g_mu;
 bool g_on = false;
+uint32_t g_mask = 0xffffffffu;  // kernel ids whose launches are timed
 std::vector<Rec> g_recs;
 // events are created once and recycled: creating two events per launch would cost
 // more host time than many of the kernels being timed
@@ -88,7 +89,7 @@ int resolve_locked() {
 bool prof_enabled() { return g_on; }
 
 int prof_start(int kernel, hipStream_t s, double bytes) {
-    if (!g_on) return -1;
+    if (!g_on || !((g_mask >> kernel) & 1u)) return -1;
     std::lock_guard<std::mutex> lock(g_mu);
     if (g_pool.size() < 2 && g_recs.size() - g_done > 1024) resolve_ready_locked();
     Rec r{kernel, take_event(), take_event(), bytes, -1, -1, false};
@@ -121,8 +122,7 @@ void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale, const double
         if (hipMemcpyAsync(&g_dbytes[slot], dev_bytes, sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess)
             g_recs[handle].bytes_slot = slot;
     }
-    hipEventRecord(g_recs[handle].b, s);
-    g_recs[handle].stopped = true;
+    if (hipEventRecord(g_recs[handle].b, s) == hipSuccess) g_recs[handle].stopped = true;
 }
 
 }  // namespace abd
@@ -151,6 +151,12 @@ int abd_profile_begin(void) {
     }
     g_on = true;
     return rc;
+}
+
+int abd_profile_select(uint32_t mask) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_mask = mask;
+    return ABD_OK;
 }
 
 int abd_profile_end(void) {

@@ -62,6 +62,7 @@ struct Sim {
     DevBuf<double> blocks;
     DevBuf<double> scal;  // [0] E contact, [1] E inertia, [2] toi, [3] dot, [4] resid
     DevBuf<int32_t> iters;
+    DevBuf<int64_t> dcount;     // culling totals gathered for one read-back
     double* host = nullptr;     // pinned scalars
     int64_t* hcount = nullptr;  // pinned list counts
     int64_t stats[10] = {};
@@ -98,44 +99,43 @@ int get_scalars(hipStream_t s, Sim& S, int first, int count) {
 }
 
 // The EE, PT and VG candidate lists of one set of boxes (culling.py:32-76), culled side
-// by side so that their counts come back in two host synchronizations instead of five.
+// by side so that their counts come back in two host synchronizations instead of five;
+// the stages copy their totals into S.dcount, one read-back each.
 int cull_lists(hipStream_t s, Sim& S, const BoxBuf& body, const BoxBuf& pts, const BoxBuf& tris,
                const BoxBuf& edges) {
     PrimCull ce, cp;
     VgCull cv;
     const bool pairs = S.nij > 0;
+    int64_t* dc = S.dcount.p;
     int rc;
     if (pairs) {
         if ((rc = prim_cull_begin(s, &S.sc, ABD_CULL_EE, S.ij.p, S.nij, edges.lo.p, edges.hi.p, edges.lo.p,
-                                  edges.hi.p, 0, ce)))
+                                  edges.hi.p, 0, ce, dc + 0)))
             return rc;
         if ((rc = prim_cull_begin(s, &S.sc, ABD_CULL_PT, S.ij.p, S.nij, pts.lo.p, pts.hi.p, tris.lo.p, tris.hi.p, 1,
-                                  cp)))
+                                  cp, dc + 1)))
             return rc;
     }
-    if ((rc = vg_cull_begin(s, &S.sc, body.lo.p, pts.lo.p, pts.hi.p, S.P.dhat, S.P.ground, cv))) return rc;
+    if ((rc = vg_cull_begin(s, &S.sc, body.lo.p, pts.lo.p, pts.hi.p, S.P.dhat, S.P.ground, cv, dc + 2))) return rc;
     int64_t* hv = S.hcount;
-    hv[0] = hv[1] = hv[2] = 0;
-    if (pairs) {
-        ABD_CHECK(hipMemcpyAsync(hv + 0, ce.total_chunks, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        ABD_CHECK(hipMemcpyAsync(hv + 1, cp.total_chunks, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (pairs || cv.total) {
+        ABD_CHECK(hipMemcpyAsync(hv, dc, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipStreamSynchronize(s));
     }
-    if (cv.total) ABD_CHECK(hipMemcpyAsync(hv + 2, cv.total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    ABD_CHECK(hipStreamSynchronize(s));
-    const int64_t T_ee = hv[0], T_pt = hv[1];
-    S.nvg = hv[2];
+    const int64_t T_ee = pairs ? hv[0] : 0, T_pt = pairs ? hv[1] : 0;
+    S.nvg = cv.total ? hv[2] : 0;
     if ((rc = S.vgl.reserve(2 * (size_t)S.nvg + 2))) return rc;
     if ((rc = vg_cull_fill(s, cv, S.vgl.p, (int64_t)(S.vgl.cap / 2), S.nvg))) return rc;
     S.nee = S.npt = 0;
     if (!pairs) return ABD_OK;
-    if ((rc = prim_cull_count(s, ce, T_ee))) return rc;
-    if ((rc = prim_cull_count(s, cp, T_pt))) return rc;
-    hv[3] = hv[4] = 0;
-    if (T_ee) ABD_CHECK(hipMemcpyAsync(hv + 3, ce.total_pairs, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    if (T_pt) ABD_CHECK(hipMemcpyAsync(hv + 4, cp.total_pairs, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    if (T_ee || T_pt) ABD_CHECK(hipStreamSynchronize(s));
-    S.nee = hv[3];
-    S.npt = hv[4];
+    if ((rc = prim_cull_count(s, ce, T_ee, dc + 3))) return rc;
+    if ((rc = prim_cull_count(s, cp, T_pt, dc + 4))) return rc;
+    if (T_ee || T_pt) {
+        ABD_CHECK(hipMemcpyAsync(hv + 3, dc + 3, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipStreamSynchronize(s));
+    }
+    S.nee = T_ee ? hv[3] : 0;
+    S.npt = T_pt ? hv[4] : 0;
     if ((rc = S.eel.reserve(5 * (size_t)S.nee + 5))) return rc;
     if ((rc = S.ptl.reserve(5 * (size_t)S.npt + 5))) return rc;
     if ((rc = prim_cull_fill(s, ce, S.eel.p, (int64_t)(S.eel.cap / 5), S.nee))) return rc;
@@ -573,6 +573,7 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
         if (!rc) rc = S->cand_E.reserve(2 * (size_t)nsc * kLsSpec);
         if (!rc) rc = S->cand_alpha.reserve((size_t)nsc * kLsSpec);
         if (!rc) rc = S->plan.reserve(reduce_plan_elems(nsc));
+        if (!rc) rc = S->dcount.reserve(8);
         if (!rc && hipHostMalloc((void**)&S->hcand, sizeof(double) * 3 * nsc * kLsSpec, hipHostMallocDefault) !=
                        hipSuccess)
             rc = ABD_ERR_ARG;
@@ -672,14 +673,16 @@ int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen) {
             if ((rc = abd_set_tuning(&t))) break;
             float ms_min = 1e30f;
             for (int rep = 0; rep < 3 && !rc; ++rep) {
-                hipEventRecord(e0, s);
-                rc = collision_set(s, S, false);
-                hipEventRecord(e1, s);
-                if (!rc && hipEventSynchronize(e1) == hipSuccess) {
-                    float ms = 0.f;
-                    hipEventElapsedTime(&ms, e0, e1);
-                    ms_min = ms < ms_min ? ms : ms_min;
+                hipError_t he = hipEventRecord(e0, s);
+                if (he != hipSuccess) {
+                    rc = -static_cast<int>(he);
+                    break;
                 }
+                rc = collision_set(s, S, false);
+                if (!rc && (he = hipEventRecord(e1, s)) != hipSuccess) rc = -static_cast<int>(he);
+                float ms = 0.f;
+                if (!rc && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+                    ms_min = ms < ms_min ? ms : ms_min;
             }
             if (rc) break;
             if (ms_min < best_ms) {
@@ -687,8 +690,8 @@ int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen) {
                 best = t;
             }
         }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     abd_set_tuning(rc ? &saved : &best);
     if (rc) return rc;
     if (chosen) *chosen = best;
@@ -726,6 +729,7 @@ int abd_sim_destroy(void* sim) {
     S->cand_E.release();
     S->cand_alpha.release();
     S->plan.release();
+    S->dcount.release();
     if (S->hcand) hipHostFree(S->hcand);
     for (void* h : {(void*)S->host, (void*)S->hcount, (void*)S->hsv, (void*)S->halpha, (void*)S->hccd, (void*)S->hmask,
                     (void*)S->hiters})

@@ -312,6 +312,9 @@ int abd_sim_destroy(void* sim);
  * 14 block-Jacobi diagonal-block inverses. */
 int abd_profile_begin(void);
 int abd_profile_end(void);
+/* Kernel ids timed while profiling (bit k = id k; all by default): timing only the
+ * kernels of interest keeps the event overhead out of a timed region. */
+int abd_profile_select(uint32_t mask);
 /* Synchronizes outstanding events; totals since abd_profile_begin(). */
 int abd_profile_query(int32_t kernel, double* total_ms, int64_t* launches, double* bytes);
 
