@@ -18,8 +18,6 @@
 namespace abd {
 
 constexpr int kSolveThreads = 1024;
-// Systems with 6 n doubles up to this size keep every Krylov vector in LDS.
-constexpr size_t kLdsVecBytes = 96 * 1024;
 // LDS for a system's staged block structure (row offsets and (column, slot) entries).
 constexpr size_t kLdsStructBytes = 24 * 1024;
 
@@ -53,18 +51,39 @@ __device__ __forceinline__ void load_block_row(const double* __restrict__ blocks
 }
 
 // A system's block structure as its SpMV reads it: from the global CSR (columns outside
-// the system filtered on the fly) or, with ST, from an LDS copy made once per solve by
-// stage_rows (local row offsets lrp, entries ent = (local column or -1, slot)), which
-// takes two dependent global loads off every block the SpMV visits.
+// the system filtered on the fly) or from an LDS copy made once per solve by stage_rows
+// (local row offsets lrp, entries ent = (local column or -1, slot)), which takes two
+// dependent global loads off every block the SpMV visits.
 struct Rows {
     int row0, nbs;
     const int* lrp;
     const int2* ent;
+    double* part;  // LDS partial products of the entry-parallel SpMV (12 per block)
 };
 
-template <bool ST>
+// SpMV modes: rows over the global structure, rows over the staged structure, and
+// entry-parallel over the staged structure (systems that fit the LDS budget, entries_fit).
+// The first two sum each row over (block, column triple) in one sequence; the third sums
+// each block's row first and the block results in column order.  The mode of a system
+// depends on its own size only, so a scene gives the same bits alone or in a batch.
+enum { kSpmvRows = 0, kSpmvRowsStaged = 1, kSpmvEntries = 2 };
+
+// lrp capacity of the staged structure (bodies + 1)
+constexpr int kLrpCap = 2048;
+constexpr size_t kLrpBytes = sizeof(int) * (size_t)kLrpCap;
+constexpr int kEntCap = (int)((kLdsStructBytes - kLrpBytes) / sizeof(int2));
+
+// Does a system of n rows and nnz blocks take the entry-parallel SpMV?  Its six vectors
+// and its partial products must fit the LDS budget beside the structure, counting the
+// vectors as if in LDS wherever they live, so that the answer depends on the system
+// alone and on the device, never on the other systems of a launch.
+__device__ __forceinline__ bool entries_fit(int n, int nnz, int budget) {
+    return budget >= 0 && 8 * (6 * (int64_t)n + 12 * (int64_t)nnz) + (int64_t)kLdsStructBytes <= budget;
+}
+
+template <int MODE>
 __device__ __forceinline__ void row_entry(const Bsr& M, const Rows& R, int e, int& lc, int64_t& slot) {
-    if (ST) {
+    if (MODE != kSpmvRows) {
         const int2 v = R.ent[e];
         lc = v.x;
         slot = v.y;
@@ -78,17 +97,17 @@ __device__ __forceinline__ void row_entry(const Bsr& M, const Rows& R, int e, in
 // (H in)[row (lb, ii, k)] of the system (local body lb), summed over the row's blocks in
 // column order.  Two blocks are loaded per step so that their memory latencies overlap;
 // the sum order is that of a one-block loop.
-template <bool ST>
+template <int MODE>
 __device__ __forceinline__ double row_product(const Bsr& M, const Rows& R, int lb, int ii, int k,
                                               const double* __restrict__ in) {
     double acc = 0.0;
-    int e = ST ? R.lrp[lb] : M.row_ptr[R.row0 + lb];
-    const int e1 = ST ? R.lrp[lb + 1] : M.row_ptr[R.row0 + lb + 1];
+    int e = MODE != kSpmvRows ? R.lrp[lb] : M.row_ptr[R.row0 + lb];
+    const int e1 = MODE != kSpmvRows ? R.lrp[lb + 1] : M.row_ptr[R.row0 + lb + 1];
     for (; e + 1 < e1; e += 2) {
         int lc0, lc1;
         int64_t s0, s1;
-        row_entry<ST>(M, R, e, lc0, s0);
-        row_entry<ST>(M, R, e + 1, lc1, s1);
+        row_entry<MODE>(M, R, e, lc0, s0);
+        row_entry<MODE>(M, R, e + 1, lc1, s1);
         double B0[12], B1[12];
         load_block_row(M.blocks, s0, ii, k, B0);
         load_block_row(M.blocks, s1, ii, k, B1);
@@ -98,7 +117,7 @@ __device__ __forceinline__ double row_product(const Bsr& M, const Rows& R, int l
     if (e < e1) {
         int lc;
         int64_t sl;
-        row_entry<ST>(M, R, e, lc, sl);
+        row_entry<MODE>(M, R, e, lc, sl);
         if (lc >= 0) {
             double B[12];
             load_block_row(M.blocks, sl, ii, k, B);
@@ -108,12 +127,49 @@ __device__ __forceinline__ double row_product(const Bsr& M, const Rows& R, int l
     return acc;
 }
 
-// Copies the structure of the system's rows into LDS (lrp: nbs + 1 ints, ent: cap_ent
-// entries) when it fits; returns false (nothing staged) otherwise.  Uniform over the
-// workgroup; ends with a barrier when it stages.
-__device__ __forceinline__ bool stage_rows(const Bsr& M, Rows& R, int* lrp, int2* ent, int cap_ent) {
+// Entry-parallel products: work item (block e, block row ii) multiplies rows 3 ii .. 3 ii + 2
+// of block e by its column's 12 entries of `in` (each row's four sub-block sums in
+// order) into R.part[12 e + 3 ii + r]; every load of an item is independent of the others.
+__device__ __forceinline__ void entry_products(const Bsr& M, const Rows& R, const double* __restrict__ in) {
+    const int nnz = R.lrp[R.nbs];
+    for (int wi = threadIdx.x; wi < 4 * nnz; wi += blockDim.x) {
+        const int e = wi >> 2, ii = wi & 3;
+        const int2 en = R.ent[e];
+        if (en.x < 0) continue;  // outside the system: skipped by the row sums too
+        const double* B = M.blocks + 144 * (int64_t)en.y + 9 * ii;
+        const double* xin = in + 12 * en.x;
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const double* Bj = B + 36 * jj;  // sub-block (ii, jj), row-major 3x3
+            const double x0 = xin[3 * jj], x1 = xin[3 * jj + 1], x2 = xin[3 * jj + 2];
+            y0 = y0 + (Bj[0] * x0 + Bj[1] * x1 + Bj[2] * x2);
+            y1 = y1 + (Bj[3] * x0 + Bj[4] * x1 + Bj[5] * x2);
+            y2 = y2 + (Bj[6] * x0 + Bj[7] * x1 + Bj[8] * x2);
+        }
+        double* pe = R.part + 12 * e + 3 * ii;
+        pe[0] = y0;
+        pe[1] = y1;
+        pe[2] = y2;
+    }
+    __syncthreads();
+}
+
+// row idx of the entry-parallel SpMV: its blocks' partial products in column order
+__device__ __forceinline__ double entry_row_sum(const Rows& R, int idx) {
+    const int lb = idx / 12, lr = idx % 12;
+    double acc = 0.0;
+    for (int e = R.lrp[lb]; e < R.lrp[lb + 1]; ++e)
+        if (R.ent[e].x >= 0) acc += R.part[12 * e + lr];
+    return acc;
+}
+
+// Copies the structure of the system's rows into LDS (lrp: kLrpCap ints, ent: kEntCap
+// entries) when the device holds it (budget >= 0) and it fits; returns the SpMV mode of
+// the system.  Uniform over the workgroup; ends with a barrier when it stages.
+__device__ __forceinline__ int stage_rows(const Bsr& M, Rows& R, int* lrp, int2* ent, int budget) {
     const int e0 = M.row_ptr[R.row0], nnz = M.row_ptr[R.row0 + R.nbs] - e0;
-    if (cap_ent < 0 || nnz > cap_ent) return false;
+    if (budget < 0 || nnz > kEntCap || R.nbs + 1 > kLrpCap) return kSpmvRows;
     for (int r = threadIdx.x; r <= R.nbs; r += blockDim.x) lrp[r] = M.row_ptr[R.row0 + r] - e0;
     for (int e = threadIdx.x; e < nnz; e += blockDim.x) {
         const int lc = M.cols[e0 + e] - R.row0;
@@ -122,31 +178,34 @@ __device__ __forceinline__ bool stage_rows(const Bsr& M, Rows& R, int* lrp, int2
     __syncthreads();
     R.lrp = lrp;
     R.ent = ent;
-    return true;
+    return entries_fit(12 * R.nbs, nnz, budget) ? kSpmvEntries : kSpmvRowsStaged;
 }
 
 // out = H in over the system's rows (in / out indexed from its first row); row idx of
 // local body idx / 12, sub-row (ii, k) = ((idx % 12) / 3, idx % 3)
-template <bool ST>
+template <int MODE>
 __device__ __forceinline__ void spmv(const Bsr& M, const Rows& R, const double* __restrict__ in,
                                      double* __restrict__ out, int n) {
+    if (MODE == kSpmvEntries) entry_products(M, R, in);
     for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
         const int lr = idx % 12;
-        out[idx] = row_product<ST>(M, R, idx / 12, lr / 3, lr % 3, in);
+        out[idx] = MODE == kSpmvEntries ? entry_row_sum(R, idx) : row_product<MODE>(M, R, idx / 12, lr / 3, lr % 3, in);
     }
     __syncthreads();
 }
 
 // out = H in over this thread's rows (the rows spmv() gives it), returning the
-// thread's partial sum of w[idx] * out[idx] in row order; no barrier: the caller
-// reduces the partials (which also orders the writes before any cross-thread read).
-template <bool ST>
+// thread's partial sum of w[idx] * out[idx] in row order; no barrier at the end: the
+// caller reduces the partials (which also orders the writes before any cross-thread read).
+template <int MODE>
 __device__ __forceinline__ double spmv_rows_dot(const Bsr& M, const Rows& R, const double* __restrict__ in,
                                                 double* __restrict__ out, int n, const double* __restrict__ w) {
+    if (MODE == kSpmvEntries) entry_products(M, R, in);
     double part = 0.0;
     for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
         const int lr = idx % 12;
-        const double acc = row_product<ST>(M, R, idx / 12, lr / 3, lr % 3, in);
+        const double acc =
+            MODE == kSpmvEntries ? entry_row_sum(R, idx) : row_product<MODE>(M, R, idx / 12, lr / 3, lr % 3, in);
         out[idx] = acc;
         part += w[idx] * acc;
     }
@@ -160,23 +219,27 @@ __device__ __forceinline__ double vdot(const double* __restrict__ a, const doubl
     return block_sum_fast(acc, sh);
 }
 
-// Dynamic LDS of a solve workgroup: the six Krylov vectors (when LDS; else in the
-// scratch arena), then the staged structure (lrp rounded up to 8 bytes, then ent).
+// Dynamic LDS of a solve workgroup: the staged structure (lrp, then ent), the six Krylov
+// vectors of the system's n rows when LDS (else in the scratch arena), then the partial
+// products of the entry-parallel SpMV.
 template <bool LDS>
 struct SolveLds {
     double* V;
+    double* part;
     int* lrp;
     int2* ent;
-    __device__ SolveLds(double* ws, int cap_rows) {
+    __device__ SolveLds(double* ws, int n) {
         extern __shared__ double lds_dyn[];
-        V = LDS ? lds_dyn : ws;
-        char* st = (char*)(lds_dyn + (LDS ? 6 * cap_rows : 0));
-        lrp = (int*)st;
-        ent = (int2*)(st + 8 * ((cap_rows / 12 + 2) / 2));
+        char* base = (char*)lds_dyn;
+        lrp = (int*)base;
+        ent = (int2*)(base + kLrpBytes);
+        double* after = (double*)(base + kLdsStructBytes);
+        V = LDS ? after : ws;
+        part = LDS ? after + 6 * n : after;
     }
 };
 
-template <bool ST>
+template <int MODE>
 __device__ __forceinline__ void bicgstab_body(const Bsr& M, const Rows& R, int n, const double* __restrict__ b,
                                               double* __restrict__ xg, double tol, int maxiter, int check_every,
                                               double* V, int32_t* __restrict__ iters, double* __restrict__ resid,
@@ -189,7 +252,7 @@ __device__ __forceinline__ void bicgstab_body(const Bsr& M, const Rows& R, int n
     double* t = V + 5 * n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = xg[i];
     __syncthreads();
-    spmv<ST>(M, R, x, t, n);
+    spmv<MODE>(M, R, x, t, n);
     for (int i = threadIdx.x; i < n; i += blockDim.x) r[i] = b[i] - t[i];
     __syncthreads();
     double rr = vdot(r, r, n, sh);
@@ -208,19 +271,18 @@ __device__ __forceinline__ void bicgstab_body(const Bsr& M, const Rows& R, int n
             // fused passes: each dot is accumulated where its operand is produced, in
             // the row order of vdot(), so the iterates match the unfused sequence bit for bit
             for (int c = 0; c < check_every; ++c) {
-                const double r0v = block_sum_fast(spmv_rows_dot<ST>(M, R, p, v, n, r0), sh);
+                const double r0v = block_sum_fast(spmv_rows_dot<MODE>(M, R, p, v, n, r0), sh);
                 const double alpha = r0v != 0.0 ? rho / r0v : 0.0;
                 for (int i = threadIdx.x; i < n; i += blockDim.x) {
                     x[i] += alpha * p[i];
                     r[i] -= alpha * v[i];
                 }
                 __syncthreads();
+                // t = H r, then t.r and t.t over this thread's rows in row order
+                spmv<MODE>(M, R, r, t, n);
                 double tr = 0.0, tt = 0.0;
                 for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-                    // t = H r for this thread's rows (as spmv_rows_dot), then t.r and t.t
-                    const int lr = idx % 12;
-                    const double acc = row_product<ST>(M, R, idx / 12, lr / 3, lr % 3, r);
-                    t[idx] = acc;
+                    const double acc = t[idx];
                     tr += acc * r[idx];
                     tt += acc * acc;
                 }
@@ -252,14 +314,13 @@ __device__ __forceinline__ void bicgstab_body(const Bsr& M, const Rows& R, int n
     }
 }
 
-// cap_ent: capacity of the staged structure (entries); a system with more blocks reads
-// its structure from the global CSR.
+// budget: bytes of dynamic LDS (SolveLds; -1: the device holds no staged structure).
 template <bool LDS>
 __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const int32_t* __restrict__ sboff,
                                                                   const uint8_t* __restrict__ active,
                                                                   const double* __restrict__ b,
                                                                   double* __restrict__ xg, double tol, int maxiter,
-                                                                  int check_every, int cap_rows, int cap_ent,
+                                                                  int check_every, int cap_rows, int budget,
                                                                   double* __restrict__ ws,
                                                                   int32_t* __restrict__ iters,
                                                                   double* __restrict__ resid) {
@@ -278,13 +339,16 @@ __global__ __launch_bounds__(kSolveThreads) void bicgstab_kernel(Bsr M, const in
         if (threadIdx.x == 0 && iters) *iters = -1;
         return;
     }
-    const SolveLds<LDS> S(ws + (LDS ? 0 : 6 * 12 * (int64_t)row0), cap_rows);
+    const SolveLds<LDS> S(ws + (LDS ? 0 : 6 * 12 * (int64_t)row0), n);
     if (maxiter <= 0) maxiter = n;  // the system size of this workgroup's system
-    Rows R{row0, n / 12, nullptr, nullptr};
-    if (stage_rows(M, R, S.lrp, S.ent, cap_ent))
-        bicgstab_body<true>(M, R, n, b, xg, tol, maxiter, check_every, S.V, iters, resid, sh, sh2);
+    Rows R{row0, n / 12, nullptr, nullptr, S.part};
+    const int mode = stage_rows(M, R, S.lrp, S.ent, budget);
+    if (mode == kSpmvEntries)
+        bicgstab_body<kSpmvEntries>(M, R, n, b, xg, tol, maxiter, check_every, S.V, iters, resid, sh, sh2);
+    else if (mode == kSpmvRowsStaged)
+        bicgstab_body<kSpmvRowsStaged>(M, R, n, b, xg, tol, maxiter, check_every, S.V, iters, resid, sh, sh2);
     else
-        bicgstab_body<false>(M, R, n, b, xg, tol, maxiter, check_every, S.V, iters, resid, sh, sh2);
+        bicgstab_body<kSpmvRows>(M, R, n, b, xg, tol, maxiter, check_every, S.V, iters, resid, sh, sh2);
 }
 
 __device__ __forceinline__ void jacobi(const double* __restrict__ dg, const double* __restrict__ r,
@@ -363,7 +427,7 @@ __device__ __forceinline__ double block_precond(const double* __restrict__ dinv,
     return acc;
 }
 
-template <bool BJ, bool ST>
+template <bool BJ, int MODE>
 __device__ __forceinline__ void cg_body(const Bsr& M, const Rows& R, int n, const double* __restrict__ b,
                                         double* __restrict__ xg, double tol, int maxiter, int check_every,
                                         double* V, const double* __restrict__ dinv, int32_t* __restrict__ iters,
@@ -382,7 +446,7 @@ __device__ __forceinline__ void cg_body(const Bsr& M, const Rows& R, int n, cons
         x[i] = xg[i];
     }
     __syncthreads();
-    spmv<ST>(M, R, x, Ap, n);
+    spmv<MODE>(M, R, x, Ap, n);
     for (int i = threadIdx.x; i < n; i += blockDim.x) r[i] = b[i] - Ap[i];
     __syncthreads();
     if (BJ) {
@@ -402,7 +466,7 @@ __device__ __forceinline__ void cg_body(const Bsr& M, const Rows& R, int n, cons
         // fused passes (dots accumulated where their operands are produced, in vdot()'s
         // row order: the iterates match the unfused sequence bit for bit)
         for (int i = 0; i < maxiter; ++i) {
-            const double pAp = block_sum_fast(spmv_rows_dot<ST>(M, R, p, Ap, n, p), sh);
+            const double pAp = block_sum_fast(spmv_rows_dot<MODE>(M, R, p, Ap, n, p), sh);
             // unguarded in the reference: p = 0 once the residual vanishes exactly between checks
             const double alpha = pAp != 0.0 ? rz_old / pAp : 0.0;
             double rz_p = 0.0;
@@ -450,13 +514,13 @@ __device__ __forceinline__ void cg_body(const Bsr& M, const Rows& R, int n, cons
 }
 
 // BJ: the per-body 12x12 block-Jacobi preconditioner (beyond the reference) in place of
-// the diagonal one; dinv: [nb][144] workspace of the inverses.  cap_ent as bicgstab_kernel.
+// the diagonal one; dinv: [nb][144] workspace of the inverses.  budget as bicgstab_kernel.
 template <bool LDS, bool BJ>
 __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t* __restrict__ sboff,
                                                            const uint8_t* __restrict__ active,
                                                            const double* __restrict__ b, double* __restrict__ xg,
                                                            double tol, int maxiter, int check_every, int cap_rows,
-                                                           int cap_ent, double* __restrict__ ws,
+                                                           int budget, double* __restrict__ ws,
                                                            double* __restrict__ dinv, int32_t* __restrict__ iters,
                                                            double* __restrict__ resid) {
     __shared__ double sh[17];
@@ -473,13 +537,16 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
         if (threadIdx.x == 0 && iters) *iters = -1;
         return;
     }
-    const SolveLds<LDS> S(ws + (LDS ? 0 : 6 * 12 * (int64_t)row0), cap_rows);
+    const SolveLds<LDS> S(ws + (LDS ? 0 : 6 * 12 * (int64_t)row0), n);
     if (maxiter <= 0) maxiter = n;  // the system size of this workgroup's system
-    Rows R{row0, n / 12, nullptr, nullptr};
-    if (stage_rows(M, R, S.lrp, S.ent, cap_ent))
-        cg_body<BJ, true>(M, R, n, b, xg, tol, maxiter, check_every, S.V, dinv, iters, resid, sh);
+    Rows R{row0, n / 12, nullptr, nullptr, S.part};
+    const int mode = stage_rows(M, R, S.lrp, S.ent, budget);
+    if (mode == kSpmvEntries)
+        cg_body<BJ, kSpmvEntries>(M, R, n, b, xg, tol, maxiter, check_every, S.V, dinv, iters, resid, sh);
+    else if (mode == kSpmvRowsStaged)
+        cg_body<BJ, kSpmvRowsStaged>(M, R, n, b, xg, tol, maxiter, check_every, S.V, dinv, iters, resid, sh);
     else
-        cg_body<BJ, false>(M, R, n, b, xg, tol, maxiter, check_every, S.V, dinv, iters, resid, sh);
+        cg_body<BJ, kSpmvRows>(M, R, n, b, xg, tol, maxiter, check_every, S.V, dinv, iters, resid, sh);
 }
 
 // Algorithmic bytes of one solve launch, summed over its systems: per iteration
@@ -513,7 +580,7 @@ __global__ void solve_bytes_kernel(const int32_t* __restrict__ row_ptr, int nb, 
 // stopping test (every check_every iterations) sets a device flag that ends the
 // remaining launches of the batch; the host reads it once per batch.
 constexpr int kMwThreads = 256;
-constexpr int kLargeBodies = 170;  // 6 vectors of 12 nb doubles no longer fit kLdsVecBytes
+constexpr int kLargeBodies = 170;  // from here one workgroup's SpMV is the bottleneck
 
 struct MwState {
     double* x;
@@ -701,25 +768,19 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
     }
     int ph = prof_start(kProfSolve, s, 1.0);
     const size_t vec_bytes = sizeof(double) * 6 * (size_t)max_rows;
-    const bool lds = vec_bytes <= kLdsVecBytes;
-    // dynamic LDS: the vectors (lds), then the structure region within the device's limit
+    // dynamic LDS: the whole budget of the device (beside the kernels' static LDS) when it
+    // holds the structure region, the vectors in it when they fit beside the structure
     int dev = 0, max_lds = 0;
     ABD_CHECK(hipGetDevice(&dev));
     ABD_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    const size_t vec_lds = lds ? vec_bytes : 0;
-    const size_t room = (size_t)max_lds > vec_lds + 1024 ? (size_t)max_lds - vec_lds - 1024 : 0;
-    size_t st_bytes = std::min(kLdsStructBytes, room);
-    const size_t lrp_bytes = 8 * (size_t)((max_rows / 12 + 2) / 2);
-    int cap_ent = -1;
-    if (st_bytes >= lrp_bytes)
-        cap_ent = (int)((st_bytes - lrp_bytes) / 8);
-    else
-        st_bytes = 0;
-    const size_t dyn = vec_lds + st_bytes;
+    const size_t usable = (size_t)max_lds > 1024 ? (size_t)max_lds - 1024 : 0;
+    const bool staged = usable >= kLdsStructBytes + 8 * 1024;
+    const int budget = staged ? (int)usable : -1;
+    const bool lds = staged && kLdsStructBytes + vec_bytes <= usable;
+    const size_t dyn = staged ? usable : 0;
     static bool attr_set = false;
     if (!attr_set) {
-        const int cap = (int)std::max(kLdsVecBytes, std::min((size_t)max_lds > 1024 ? (size_t)max_lds - 1024 : 0,
-                                                             kLdsVecBytes + kLdsStructBytes));
+        const int cap = (int)usable;
         const void* fns[] = {(const void*)bicgstab_kernel<true>,     (const void*)bicgstab_kernel<false>,
                              (const void*)cg_kernel<true, false>,    (const void*)cg_kernel<true, true>,
                              (const void*)cg_kernel<false, false>,   (const void*)cg_kernel<false, true>};
@@ -729,23 +790,23 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
     if (lds) {
         if (method == ABD_SOLVER_BICGSTAB)
             hipLaunchKernelGGL(bicgstab_kernel<true>, dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x, tol,
-                               maxiter, check_every, max_rows, cap_ent, ws, iters, resid);
+                               maxiter, check_every, max_rows, budget, ws, iters, resid);
         else if (bj)
             hipLaunchKernelGGL((cg_kernel<true, true>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, cap_ent, ws, dinv, iters, resid);
+                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid);
         else
             hipLaunchKernelGGL((cg_kernel<true, false>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, cap_ent, ws, dinv, iters, resid);
+                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid);
     } else {
         if (method == ABD_SOLVER_BICGSTAB)
             hipLaunchKernelGGL(bicgstab_kernel<false>, dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, cap_ent, ws, iters, resid);
+                               tol, maxiter, check_every, max_rows, budget, ws, iters, resid);
         else if (bj)
             hipLaunchKernelGGL((cg_kernel<false, true>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, cap_ent, ws, dinv, iters, resid);
+                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid);
         else
             hipLaunchKernelGGL((cg_kernel<false, false>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, cap_ent, ws, dinv, iters, resid);
+                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid);
     }
     ABD_CHECK_LAUNCH();
     if (ph >= 0 && iters) {
