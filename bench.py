@@ -26,8 +26,9 @@ the oracle in the same mode on the sampled scenes.
 Frames: W warmup frames, then (unless --no-profile) untimed attribution frames in
 which every instrumented kernel is timed with HIP events (the per-kernel table,
 the gh-stage rate and the choice of the dominant kernel), then the K timed
-frames, in which only the dominant kernel carries events (its roofline), so
-that per-launch event overhead stays out of `value`.
+frames, in which only the dominant kernel (its roofline) and the four gh
+kernels (the barrier-Hessian pair rate) carry events, so that per-launch event
+overhead stays out of `value`.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload batched|medium|small|stress]
 """
@@ -123,7 +124,7 @@ def cpu_baseline(spec, cp, warmup: int, steps: int, budget_s: float, sample=None
     from oracle import oracle as O
     from abd_hip import scenes
     specs = [spec] if sample is None else [scenes.scene_of(spec, s) for s in sample]
-    iters, t_cpu, frames, parts = 0, 0.0, 0, []
+    iters, t_cpu, frames, parts, frames_done = 0, 0.0, 0, [], []
     share = budget_s / len(specs)
     for k, sp in enumerate(specs):
         ref = O.OracleSim(sp, cp)
@@ -143,11 +144,58 @@ def cpu_baseline(spec, cp, warmup: int, steps: int, budget_s: float, sample=None
         iters += n
         t_cpu += t_here
         frames += f_here
+        frames_done.append(f_here)
         parts.append(f"{'scene ' + str(sample[k]) + ': ' if sample is not None else ''}frames {warmup}.."
                      f"{warmup + f_here - 1}, {n} it")
     return dict(value=iters / t_cpu if t_cpu > 0 else None, unit="Newton iters/s", cores=1, kind="port",
                 sample=f"oracle (C FP64 restatement, -O2, 1 thread), same mode: " + "; ".join(parts) +
-                       f"; {iters} Newton iterations in {t_cpu:.2f} s")
+                       f"; {iters} Newton iterations in {t_cpu:.2f} s"), frames_done
+
+
+def _cpu_worker(sp, cp, warmup, frames, barrier, q, k):
+    # one sampled scene on the oracle in its own process; the timed frames start together
+    from oracle import oracle as O
+    ref = O.OracleSim(sp, cp)
+    for _ in range(warmup):
+        ref.step()
+    it0 = int(ref.stats[0])
+    barrier.wait()
+    t = time.perf_counter()
+    for _ in range(frames):
+        ref.step()
+    q.put((k, int(ref.stats[0]) - it0, time.perf_counter() - t))
+
+
+def cpu_baseline_parallel(spec, cp, warmup: int, frames, sample):
+    """The 1-thread baseline's work (the same scenes and frames) spread over one
+    process per sampled scene, started together after their warmup frames:
+    Newton iterations per second of wall time on that many host cores."""
+    import multiprocessing as mp
+    from abd_hip import scenes
+    ctx = mp.get_context("spawn")  # fresh interpreters: nothing of this process's GPU state
+    n = len(sample)
+    barrier, q = ctx.Barrier(n), ctx.Queue()
+    procs = [ctx.Process(target=_cpu_worker, args=(scenes.scene_of(spec, s), cp, warmup, frames[k], barrier, q, k))
+             for k, s in enumerate(sample)]
+    import queue
+    for p in procs:
+        p.start()
+    res = []
+    try:
+        for _ in procs:
+            res.append(q.get(timeout=600))
+    except queue.Empty:
+        for p in procs:
+            p.terminate()
+        log("[cpu] parallel baseline: a worker did not finish")
+        return None
+    for p in procs:
+        p.join()
+    iters = sum(r[1] for r in res)
+    wall = max(r[2] for r in res)
+    return dict(value=iters / wall if wall > 0 else None, unit="Newton iters/s", cores=n, kind="port",
+                sample=f"oracle (C FP64 restatement, -O2), one process per scene {sample} on {n} cores, the frames of "
+                       f"the 1-thread run: {iters} Newton iterations in {wall:.2f} s of wall time")
 
 
 def main():
@@ -160,7 +208,8 @@ def main():
     ap.add_argument("--solver", type=int, default=2, help="0 BiCGSTAB, 1 Jacobi CG, 2 block-Jacobi CG")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--plain-sample", type=int, default=8, help="batched: scenes timed on the plain path")
+    ap.add_argument("--plain-sample", type=int, default=8,
+                    help="batched: scenes timed on the plain path (0: none)")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-kernel HIP events (no attribution frames, roofline or kernel breakdown)")
     ap.add_argument("--attrib-frames", type=int, default=2,
@@ -196,11 +245,10 @@ def main():
     # attribution frames (untimed): every instrumented kernel timed with HIP events, for the
     # per-kernel table, the gh-stage rate and the choice of the dominant kernel; the timed
     # frames then carry events on that kernel only, so the event overhead stays out of `value`
-    aprof, ad, dom = None, None, None
+    aprof, dom = None, None
     n_attrib = 0 if a.no_profile else max(a.attrib_frames, 1)
     if n_attrib:
         torch.cuda.synchronize()
-        sa = dict(sim.stats)
         _lib.profile_select(None)
         _lib.profile_begin()
         for f in range(n_attrib):
@@ -210,7 +258,6 @@ def main():
         torch.cuda.synchronize()
         _lib.profile_end()
         aprof = _lib.profile_query()
-        ad = {k: sim.stats[k] - sa[k] for k in sim.stats}
         # dominant kernel with an algorithmic byte count
         ranked = sorted(((v[0], k) for k, v in aprof.items() if v[1] > 0 and v[2] > 0), reverse=True)
         dom = ranked[0][1] if ranked else None
@@ -218,7 +265,8 @@ def main():
     torch.cuda.synchronize()
     s0 = dict(sim.stats)
     if dom:
-        _lib.profile_select([dom])
+        # the dominant kernel (roofline) and the four gh kernels (barrier-Hessian pairs/s)
+        _lib.profile_select(sorted({dom, *GH_KERNELS}))
         _lib.profile_begin()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -243,9 +291,9 @@ def main():
     d = {k: sim.stats[k] - s0[k] for k in sim.stats}
     wall = t1 - t0
     iters = d["newton_iters"]
-    # gh rate over the attribution frames (both drivers launch the same four gh kernels)
-    gh_s = sum(aprof[k][0] for k in GH_KERNELS if k in aprof) * 1e-3 if aprof else 0.0
-    cands = ad["gh_pt"] + ad["gh_ee"] if ad else 0
+    # gh rate over the timed frames (both drivers launch the same four gh kernels)
+    gh_s = sum(prof[k][0] for k in GH_KERNELS if k in prof) * 1e-3 if prof else 0.0
+    cands = d["gh_pt"] + d["gh_ee"]
     roof = None
     if dom:
         ms, n, byts = prof[dom]
@@ -256,15 +304,15 @@ def main():
     batched = spec.n_scenes > 1
     plain = None
     sample = None
-    if batched:
+    if batched and a.plain_sample > 0:
         ns = spec.n_scenes
-        k_n = max(a.plain_sample, 1)
+        k_n = a.plain_sample
         sample = [0] if k_n == 1 else sorted(set(int(round(k * (ns - 1) / (k_n - 1))) for k in range(k_n)))
         p_it, p_wall = plain_per_scene(spec, cp, sample, first, a.steps)
         plain = dict(value=p_it / p_wall if p_wall > 0 else None, unit="Newton iters/s",
                      sample=f"plain native driver, one scene at a time, scenes {sample}, frames "
                             f"{first}..{first + a.steps - 1}: {p_it} Newton iterations in {p_wall:.3f} s")
-    cpu = None
+    cpu, cpu_par = None, None
     if not a.no_cpu:
         if a.workload == "stress":
             # the oracle's brute-force culling cannot run the full stress scene in a bounded time:
@@ -272,10 +320,14 @@ def main():
             from abd_hip import scenes
             sub = scenes.subset_scene(spec, [i + 10 * j + 100 * k for k in range(3) for j in range(3)
                                              for i in range(3)])
-            cpu = cpu_baseline(sub, cp, first, a.steps, a.cpu_budget)
+            cpu, _ = cpu_baseline(sub, cp, first, a.steps, a.cpu_budget)
             cpu["sample"] = "27 bodies (the 3x3x3 corner of the lattice), " + cpu["sample"]
         else:
-            cpu = cpu_baseline(spec, cp, first, a.steps, a.cpu_budget, sample)
+            cpu_sample = sample if sample is not None or not batched else [0]
+            cpu, frames_done = cpu_baseline(spec, cp, first, a.steps, a.cpu_budget, cpu_sample)
+            if batched and len(cpu_sample) > 1:
+                # the same work on one core per sampled scene (scenes are independent)
+                cpu_par = cpu_baseline_parallel(spec, cp, first, frames_done, cpu_sample)
     out = {
         "metric": "Newton iters/s + barrier-Hessian pairs/s, batched vs plain, 1 MI355X",
         "value": iters / wall,
@@ -302,8 +354,9 @@ def main():
         "batched_over_plain": (iters / wall) / plain["value"] if plain and plain["value"] else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_par,
         # per-frame kernel times of the attribution frames (all instrumented kernels timed there;
-        # the timed frames time the dominant kernel only)
+        # the timed frames time the dominant kernel and the gh kernels only)
         "kernels_ms_per_frame": {k: round(v[0] / n_attrib, 3) for k, v in aprof.items() if v[1]} if aprof else None,
         "attribution_frames": f"{a.warmup}..{first - 1}" if n_attrib else None,
     }
