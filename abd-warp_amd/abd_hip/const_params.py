@@ -48,7 +48,8 @@ class ContactParams:
     vg: bool = True
     solver: int = SOLVER_BICGSTAB
     # 0: the reference's contact Hessian (indefinite); 1: contact-space PSD projection;
-    # 2: exact Hessian of the barrier energy (1 and 2 go beyond the reference)
+    # 2: exact Hessian of the barrier energy; 3: full IPC (closest feature pairs, edge-edge
+    # mollifier, per-contact 12x12 PSD projection) — 1, 2 and 3 go beyond the reference
     psd: int = 0
 
     @property

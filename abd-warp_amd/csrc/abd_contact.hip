@@ -147,11 +147,43 @@ __device__ bool contact_terms(int kind, const d3 v[4], const d3 r[4], double d2h
     return true;
 }
 
+// The dense 24x24 block J^T Hx J and gradient gscale J^T gx of one contact (out[0..575]
+// row-major, bodies [I | J]; out[576..599]): vertex t of body half h(t) maps through
+// [1, r_t^T] (x) I3; PT has one vertex on body I, EE two.
+constexpr int kDenseStride = 600;
+constexpr int kDenseGOff = 576;
+__device__ void dense_body_map(int kind, const d3 r[4], const double gx[12], double gscale, const double Hx[144],
+                               double* __restrict__ out) {
+    const int nI = kind == 0 ? 1 : 2;
+    for (int i = 0; i < 24; ++i) {
+        const int hi = i / 12, ai = (i % 12) / 3, ki = i % 3;
+        double gacc = 0.0;
+        for (int t = 0; t < 4; ++t) {
+            if ((t < nI ? 0 : 1) != hi) continue;
+            const double Ji = ai == 0 ? 1.0 : comp(r[t], ai - 1);
+            gacc += Ji * gx[3 * t + ki];
+        }
+        out[kDenseGOff + i] = gscale * gacc;
+        for (int j = 0; j < 24; ++j) {
+            const int hj = j / 12, aj = (j % 12) / 3, kj = j % 3;
+            double acc = 0.0;
+            for (int t = 0; t < 4; ++t) {
+                if ((t < nI ? 0 : 1) != hi) continue;
+                const double Ji = ai == 0 ? 1.0 : comp(r[t], ai - 1);
+                for (int q = 0; q < 4; ++q) {
+                    if ((q < nI ? 0 : 1) != hj) continue;
+                    const double Jj = aj == 0 ? 1.0 : comp(r[q], aj - 1);
+                    acc += Ji * Jj * Hx[12 * (3 * t + ki) + 3 * q + kj];
+                }
+            }
+            out[24 * i + j] = acc;
+        }
+    }
+}
+
 // Exact mode (psd = 2, beyond the reference): the dense 24x24 block
 // J^T (b' d2s/dx2 + b'' ds/dx ds/dx^T) J and the gradient b' J^T ds/dx of one
 // contact, written as out[0..575] (row-major, bodies [I | J]) and out[576..599].
-constexpr int kDenseStride = 600;
-constexpr int kDenseGOff = 576;
 __device__ bool contact_terms_exact(int kind, const d3 v[4], const d3 r[4], double d2hat, double kappa,
                                     double* __restrict__ out) {
     CMap c = kind == 0 ? cmap_pt(v[0], v[1], v[2], v[3]) : cmap_ee(v[0], v[1], v[2], v[3]);
@@ -186,32 +218,53 @@ __device__ bool contact_terms_exact(int kind, const d3 v[4], const d3 r[4], doub
                         for (int f = 0; f < 3; ++f) acc += L[4 * e + a] * L[4 * f + b] * He[9 * (3 * e + k) + 3 * f + m];
                     Hx[12 * (3 * a + k) + 3 * b + m] = acc * b1 + (gx[3 * a + k] * gx[3 * b + m]) * b2;
                 }
-    // J: vertex t of body half h(t) maps through [1, r_t^T] (x) I3
-    const int nI = kind == 0 ? 1 : 2;
-    for (int i = 0; i < 24; ++i) {
-        const int hi = i / 12, ai = (i % 12) / 3, ki = i % 3;
-        double gacc = 0.0;
-        for (int t = 0; t < 4; ++t) {
-            if ((t < nI ? 0 : 1) != hi) continue;
-            const double Ji = ai == 0 ? 1.0 : comp(r[t], ai - 1);
-            gacc += Ji * gx[3 * t + ki];
-        }
-        out[kDenseGOff + i] = b1 * gacc;
-        for (int j = 0; j < 24; ++j) {
-            const int hj = j / 12, aj = (j % 12) / 3, kj = j % 3;
-            double acc = 0.0;
-            for (int t = 0; t < 4; ++t) {
-                if ((t < nI ? 0 : 1) != hi) continue;
-                const double Ji = ai == 0 ? 1.0 : comp(r[t], ai - 1);
-                for (int q = 0; q < 4; ++q) {
-                    if ((q < nI ? 0 : 1) != hj) continue;
-                    const double Jj = aj == 0 ? 1.0 : comp(r[q], aj - 1);
-                    acc += Ji * Jj * Hx[12 * (3 * t + ki) + 3 * q + kj];
-                }
-            }
-            out[24 * i + j] = acc;
-        }
+    dense_body_map(kind, r, gx, b1, Hx, out);
+    return true;
+}
+
+// Full-IPC mode (psd = 3, beyond the reference): the barrier of the squared distance to the
+// closest feature pair (no validity gate), times the edge-edge mollifier for EE; the 12x12
+// Hessian in the points' coordinates projected onto the PSD cone, then mapped to the two
+// bodies like contact_terms_exact (same storage).
+__device__ bool contact_terms_full(int kind, const d3 v[4], const d3 r[4], double d2hat, double kappa,
+                                   double* __restrict__ out) {
+    const Feat f = kind == 0 ? pt_feature(v) : ee_feature(v);
+    double gs[12], Hx[144];
+    const double s = feature_dist2(kind, f, v, gs, Hx);
+    if (!(s < d2hat)) return false;
+    const double b0 = barrier(s, d2hat, kappa), b1 = barrier_d1(s, d2hat, kappa), b2 = barrier_d2(s, d2hat, kappa);
+    // mollifier m(c), c = |u x v|^2, u = x1 - x0, v = x3 - x2 (EE below eps only)
+    double m = 1.0, m1 = 0.0, m2 = 0.0, gc6[6], Hc6[36];
+    const double eps = kind == 1 ? ee_mollifier_eps(r) : 0.0;
+    const double c = kind == 1 ? cross2_derivs(v[1] - v[0], v[3] - v[2], gc6, Hc6) : 0.0;
+    const bool moll = kind == 1 && c < eps;
+    if (moll) {
+        m = mollifier(c, eps);
+        m1 = (2.0 - 2.0 * c / eps) / eps;
+        m2 = -2.0 / (eps * eps);
     }
+    // point coefficients of u (points 0, 1) and v (points 2, 3)
+    const double Lc[2][4] = {{-1.0, 1.0, 0.0, 0.0}, {0.0, 0.0, -1.0, 1.0}};
+    double gx[12], gc[12];
+    for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 3; ++k) {
+            gc[3 * a + k] = moll ? Lc[0][a] * gc6[k] + Lc[1][a] * gc6[3 + k] : 0.0;
+            gx[3 * a + k] = m * b1 * gs[3 * a + k] + b0 * m1 * gc[3 * a + k];
+        }
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) {
+            double h = m * (b1 * Hx[12 * i + j] + b2 * (gs[i] * gs[j]));
+            if (moll) {
+                const int a = i / 3, k = i % 3, bq = j / 3, mm = j % 3;
+                double hc = 0.0;
+                for (int e = 0; e < 2; ++e)
+                    for (int f2 = 0; f2 < 2; ++f2) hc += Lc[e][a] * Lc[f2][bq] * Hc6[6 * (3 * e + k) + 3 * f2 + mm];
+                h += b0 * (m2 * (gc[i] * gc[j]) + m1 * hc) + b1 * m1 * (gs[i] * gc[j] + gc[i] * gs[j]);
+            }
+            Hx[12 * i + j] = h;
+        }
+    psd_project12(Hx);
+    dense_body_map(kind, r, gx, 1.0, Hx, out);
     return true;
 }
 
@@ -241,7 +294,10 @@ __global__ __launch_bounds__(kBlock) void classify_kernel(AbdScene sc, AbdParams
             fetch_pt_idx(S, t5, vi);
         load4(x, vi, v);
         bool act;
-        if (is_ee) {
+        if (P.psd == 3) {  // full IPC: the closest feature pair, no validity gate
+            const Feat f = is_ee ? ee_feature(v) : pt_feature(v);
+            act = (is_ee ? P.ee : P.pt) && feature_dist2_value(is_ee ? 1 : 0, f, v) < P.d2hat;
+        } else if (is_ee) {
             CMap c = cmap_ee(v[0], v[1], v[2], v[3]);
             act = P.ee && dot(c.e2p, c.e2p) < P.d2hat && valid_ee(c.beta, c.gamma);
         } else {
@@ -294,7 +350,9 @@ __global__ __launch_bounds__(kBlock) void contact_kernel(AbdScene sc, AbdParams 
         d3 v[4], r[4];
         load4(x, vi, v);
         load4(sc.x0, vi, r);
-        if (P.psd == 2)
+        if (P.psd == 3)
+            contact_terms_full(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, cbuf + e * kDenseStride);
+        else if (P.psd == 2)
             contact_terms_exact(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, cbuf + e * kDenseStride);
         else
             contact_terms(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, P.psd, cbuf + e * kContactStride);
@@ -461,6 +519,10 @@ __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdP
                         const double th_j = ja == 0 ? 1.0 : s_vt[q][ja - 1];
                         double scl = th_i * th_j * d2b * 4.0 * d * d;
                         if (P.psd == 2) scl += th_i * th_j * 2.0 * db;  // exact mode: the b' term
+                        if (P.psd == 3) {  // full IPC: d2b/dy2 = 4 d^2 b'' + 2 b' clamped at 0
+                            const double cy = d2b * 4.0 * d * d + 2.0 * db;
+                            scl = th_i * th_j * (cy > 0.0 ? cy : 0.0);
+                        }
                         const double nn = (ra == 1 ? 1.0 : 0.0) * (ca == 1 ? 1.0 : 0.0);
                         acc += nn * scl;
                     }
@@ -495,9 +557,10 @@ __global__ __launch_bounds__(kBlock) void pair_gh_kernel(int kind, AbdParams P, 
             v[k] = ld3(x4 + 12 * i + 3 * k);
             r[k] = ld3(r4 + 12 * i + 3 * k);
         }
-        if (P.psd == 2) {
+        if (P.psd >= 2) {
             double* buf = scratch + i * kDenseStride;
-            bool act = contact_terms_exact(kind, v, r, P.d2hat, P.kappa, buf);
+            bool act = P.psd == 3 ? contact_terms_full(kind, v, r, P.d2hat, P.kappa, buf)
+                                  : contact_terms_exact(kind, v, r, P.d2hat, P.kappa, buf);
             active[i] = act ? 1 : 0;
             for (int a = 0; a < 24; ++a) {
                 g24[24 * i + a] = act ? buf[kDenseGOff + a] : 0.0;
@@ -657,7 +720,7 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
         double* cbuf;
         int64_t* ckey;
         uint8_t* swp;
-        const bool dense = P->psd == 2;
+        const bool dense = P->psd >= 2;
         const int stride = dense ? kDenseStride : kContactStride;
         if ((rc = arena_get(kSlotContact, (sizeof(double) * stride + sizeof(int64_t) + 1) * (size_t)nact,
                             (void**)&cbuf)))
@@ -696,7 +759,7 @@ int abd_pair_gh(abd_stream_t stream, int32_t kind, const AbdParams* P, const dou
     if (!P || n < 0 || (n && (!x4 || !r4 || !active || !g24 || !H24)) || (kind != 0 && kind != 1)) return ABD_ERR_ARG;
     if (n == 0) return ABD_OK;
     double* scratch;
-    const int stride = P->psd == 2 ? kDenseStride : kContactStride;
+    const int stride = P->psd >= 2 ? kDenseStride : kContactStride;
     int rc = arena_get(kSlotContact, sizeof(double) * stride * (size_t)n, (void**)&scratch);
     if (rc) return rc;
     hipLaunchKernelGGL(pair_gh_kernel, dim3(grid_for(n)), dim3(kBlock), 0, to_stream(stream), kind, *P, x4, r4, n,

@@ -477,3 +477,314 @@ __device__ inline void plane_dist2_derivs(d3 u, d3 v, d3 w, double g[9], double 
     skew_add(H, 3, 0, c, -2.0 * a2);
 }
 }  // namespace abd
+
+namespace abd {
+// ------------------------------------------------------------------ full IPC
+// psd = 3 (beyond the reference, which has no distance types): the squared distance of
+// a PT / EE candidate to its closest feature pair (Li et al. 2020, Incremental Potential
+// Contact), the edge-edge mollifier, and their exact derivatives in the 12 coordinates
+// of the contact's four points x0..x3 (PT: point, triangle t0 t1 t2; EE: a0 a1, b0 b1).
+
+// Feature pairs: kFeatPlane (PT: point-plane; EE: line-line), kFeatPE (point x[i] against
+// edge x[j] x[k]), kFeatPP (x[i] against x[j]).
+enum { kFeatPlane = 0, kFeatPE = 1, kFeatPP = 2 };
+struct Feat {
+    int kind, i, j, k;
+};
+
+// Closest feature of point x0 to triangle x1 x2 x3 by the Voronoi regions of the
+// triangle (Ericson, Real-Time Collision Detection, 2005, sec. 5.1.5).
+__device__ __forceinline__ Feat pt_feature(const d3 x[4]) {
+    const d3 a = x[1], b = x[2], c = x[3], p = x[0];
+    const d3 ab = b - a, ac = c - a, ap = p - a;
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) return Feat{kFeatPP, 0, 1, 0};
+    const d3 bp = p - b;
+    const double d3_ = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3_ >= 0.0 && d4 <= d3_) return Feat{kFeatPP, 0, 2, 0};
+    const double vc = d1 * d4 - d3_ * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3_ <= 0.0) return Feat{kFeatPE, 0, 1, 2};
+    const d3 cp = p - c;
+    const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) return Feat{kFeatPP, 0, 3, 0};
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return Feat{kFeatPE, 0, 1, 3};
+    const double va = d3_ * d6 - d5 * d4;
+    if (va <= 0.0 && d4 - d3_ >= 0.0 && d5 - d6 >= 0.0) return Feat{kFeatPE, 0, 2, 3};
+    return Feat{kFeatPlane, 0, 0, 0};
+}
+
+// Closest feature pair of segments x0 x1 and x2 x3: the line parameters of the closest
+// points, clamped to the segments in turn (Sunday, "Distance between 3D lines and
+// segments", geomalgorithms.com); (near-)parallel segments start from s = 0.
+__device__ __forceinline__ Feat ee_feature(const d3 x[4]) {
+    const d3 u = x[1] - x[0], v = x[3] - x[2], w = x[0] - x[2];
+    const double a = dot(u, u), b = dot(u, v), c = dot(v, v), d = dot(u, w), e = dot(v, w);
+    const double D = a * c - b * b;
+    double sN, sD = D, tN, tD = D;
+    int s_end = -1, t_end = -1;  // -1: interior, 0 / 1: that end
+    if (D <= 1e-12 * a * c) {
+        sN = 0.0;
+        sD = 1.0;
+        tN = e;
+        tD = c;
+        s_end = 0;
+    } else {
+        sN = b * e - c * d;
+        tN = a * e - b * d;
+        if (sN <= 0.0) {
+            sN = 0.0;
+            tN = e;
+            tD = c;
+            s_end = 0;
+        } else if (sN >= sD) {
+            sN = sD;
+            tN = e + b;
+            tD = c;
+            s_end = 1;
+        }
+    }
+    if (tN <= 0.0) {
+        t_end = 0;
+        if (-d <= 0.0)
+            s_end = 0;
+        else if (-d >= a)
+            s_end = 1;
+        else
+            s_end = -1;
+    } else if (tN >= tD) {
+        t_end = 1;
+        if (-d + b <= 0.0)
+            s_end = 0;
+        else if (-d + b >= a)
+            s_end = 1;
+        else
+            s_end = -1;
+    }
+    if (s_end < 0 && t_end < 0) return Feat{kFeatPlane, 0, 0, 0};
+    if (s_end >= 0 && t_end >= 0) return Feat{kFeatPP, s_end, 2 + t_end, 0};
+    if (s_end >= 0) return Feat{kFeatPE, s_end, 2, 3};
+    return Feat{kFeatPE, 2 + t_end, 0, 1};
+}
+
+// C = |a x b|^2 and its derivatives in (a, b): g (6) = (2 b x c, 2 c x a) with c = a x b
+// (the cross-product form keeps full relative accuracy when a and b are nearly parallel),
+// H (36, row-major) from C = |a|^2 |b|^2 - (a.b)^2.
+__device__ inline double cross2_derivs(d3 a, d3 b, double g[6], double H[36]) {
+    const double aa = dot(a, a), bb = dot(b, b), ab = dot(a, b);
+    const d3 c = cross(a, b), ga = cross(b, c), gb = cross(c, a);
+    for (int k = 0; k < 3; ++k) {
+        g[k] = 2.0 * comp(ga, k);
+        g[3 + k] = 2.0 * comp(gb, k);
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) {
+            const double I = r == k ? 1.0 : 0.0;
+            H[6 * r + k] = 2.0 * bb * I - 2.0 * comp(b, r) * comp(b, k);
+            H[6 * (3 + r) + 3 + k] = 2.0 * aa * I - 2.0 * comp(a, r) * comp(a, k);
+            H[6 * r + 3 + k] = 4.0 * comp(a, r) * comp(b, k) - 2.0 * comp(b, r) * comp(a, k) - 2.0 * ab * I;
+            H[6 * (3 + r) + k] = 4.0 * comp(b, r) * comp(a, k) - 2.0 * comp(a, r) * comp(b, k) - 2.0 * ab * I;
+        }
+    return dot(c, c);
+}
+
+// The squared distance s of the feature pair f and ds, d2s in the 12 point coordinates.
+__device__ inline double feature_dist2(int ckind, const Feat& f, const d3 x[4], double g[12], double H[144]) {
+    for (int i = 0; i < 12; ++i) g[i] = 0.0;
+    for (int i = 0; i < 144; ++i) H[i] = 0.0;
+    if (f.kind == kFeatPP) {
+        const d3 dlt = x[f.i] - x[f.j];
+        for (int k = 0; k < 3; ++k) {
+            g[3 * f.i + k] = 2.0 * comp(dlt, k);
+            g[3 * f.j + k] = -2.0 * comp(dlt, k);
+            H[12 * (3 * f.i + k) + 3 * f.i + k] = 2.0;
+            H[12 * (3 * f.j + k) + 3 * f.j + k] = 2.0;
+            H[12 * (3 * f.i + k) + 3 * f.j + k] = -2.0;
+            H[12 * (3 * f.j + k) + 3 * f.i + k] = -2.0;
+        }
+        return dot(dlt, dlt);
+    }
+    if (f.kind == kFeatPE) {
+        // s = C / N, C = |a x b|^2 with a = e0 - p, b = e1 - p; N = |e1 - e0|^2
+        const d3 p = x[f.i], e0 = x[f.j], e1 = x[f.k];
+        const d3 a = e0 - p, b = e1 - p, e = e1 - e0;
+        double gC[6], HC[36];
+        const double C = cross2_derivs(a, b, gC, HC);
+        const double N = dot(e, e);
+        double gN[6];
+        for (int k = 0; k < 3; ++k) {
+            gN[k] = -2.0 * comp(e, k);
+            gN[3 + k] = 2.0 * comp(e, k);
+        }
+        double gs[6], Hs[36];
+        for (int i = 0; i < 6; ++i) gs[i] = gC[i] / N - C * gN[i] / (N * N);
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) {
+                const double d2N = ((i < 3) == (j < 3) ? 2.0 : -2.0) * (i % 3 == j % 3 ? 1.0 : 0.0);
+                Hs[6 * i + j] = HC[6 * i + j] / N - (gC[i] * gN[j] + gN[i] * gC[j]) / (N * N) - C * d2N / (N * N) +
+                                2.0 * C * gN[i] * gN[j] / (N * N * N);
+            }
+        // (a, b) = (e0 - p, e1 - p): point coefficients per (a | b) row
+        const int pts[3] = {f.i, f.j, f.k};
+        const double La[3] = {-1.0, 1.0, 0.0}, Lb[3] = {-1.0, 0.0, 1.0};
+        for (int t = 0; t < 3; ++t)
+            for (int k = 0; k < 3; ++k) g[3 * pts[t] + k] += La[t] * gs[k] + Lb[t] * gs[3 + k];
+        for (int t = 0; t < 3; ++t)
+            for (int q = 0; q < 3; ++q) {
+                const double cf[2][2] = {{La[t] * La[q], La[t] * Lb[q]}, {Lb[t] * La[q], Lb[t] * Lb[q]}};
+                for (int k = 0; k < 3; ++k)
+                    for (int m = 0; m < 3; ++m)
+                        H[12 * (3 * pts[t] + k) + 3 * pts[q] + m] +=
+                            cf[0][0] * Hs[6 * k + m] + cf[0][1] * Hs[6 * k + 3 + m] + cf[1][0] * Hs[6 * (3 + k) + m] +
+                            cf[1][1] * Hs[6 * (3 + k) + 3 + m];
+            }
+        return C / N;
+    }
+    // plane / line: s = (w.(u x v))^2 / |u x v|^2 over edge vectors with point coefficients L
+    double L[3][4];
+    d3 u, v, w;
+    if (ckind == 0) {  // u = t1 - t0, v = t2 - t0, w = p - t0
+        const double Lpt[3][4] = {{0, -1, 1, 0}, {0, -1, 0, 1}, {1, -1, 0, 0}};
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) L[r][c] = Lpt[r][c];
+        u = x[2] - x[1];
+        v = x[3] - x[1];
+        w = x[0] - x[1];
+    } else {  // u = a1 - a0, v = b1 - b0, w = b0 - a0
+        const double Lee[3][4] = {{-1, 1, 0, 0}, {0, 0, -1, 1}, {-1, 0, 1, 0}};
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) L[r][c] = Lee[r][c];
+        u = x[1] - x[0];
+        v = x[3] - x[2];
+        w = x[2] - x[0];
+    }
+    double ge[9], He[81];
+    plane_dist2_derivs(u, v, w, ge, He);
+    for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 3; ++k) {
+            double acc = 0.0;
+            for (int e = 0; e < 3; ++e) acc += L[e][a] * ge[3 * e + k];
+            g[3 * a + k] = acc;
+        }
+    for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 3; ++k)
+            for (int b = 0; b < 4; ++b)
+                for (int m = 0; m < 3; ++m) {
+                    double acc = 0.0;
+                    for (int e = 0; e < 3; ++e)
+                        for (int f2 = 0; f2 < 3; ++f2) acc += L[e][a] * L[f2][b] * He[9 * (3 * e + k) + 3 * f2 + m];
+                    H[12 * (3 * a + k) + 3 * b + m] = acc;
+                }
+    const d3 cuv = cross(u, v);
+    const double T = dot(w, cuv);
+    return T * T / dot(cuv, cuv);
+}
+
+// Squared distance of the feature pair alone (no derivatives).
+__device__ __forceinline__ double feature_dist2_value(int ckind, const Feat& f, const d3 x[4]) {
+    if (f.kind == kFeatPP) {
+        const d3 dlt = x[f.i] - x[f.j];
+        return dot(dlt, dlt);
+    }
+    if (f.kind == kFeatPE) {
+        const d3 c = cross(x[f.j] - x[f.i], x[f.k] - x[f.i]), e = x[f.k] - x[f.j];
+        return dot(c, c) / dot(e, e);
+    }
+    const d3 u = ckind == 0 ? x[2] - x[1] : x[1] - x[0];
+    const d3 v = ckind == 0 ? x[3] - x[1] : x[3] - x[2];
+    const d3 w = ckind == 0 ? x[0] - x[1] : x[2] - x[0];
+    const d3 cuv = cross(u, v);
+    const double T = dot(w, cuv);
+    return T * T / dot(cuv, cuv);
+}
+
+// Edge-edge mollifier (Li et al. 2020): eps_x = 1e-3 |a1 - a0|^2 |b1 - b0|^2 at rest,
+// m(c) = (2 - c / eps_x) c / eps_x for c = |(a1 - a0) x (b1 - b0)|^2 < eps_x, else 1.
+__device__ __forceinline__ double ee_mollifier_eps(const d3 r[4]) {
+    const d3 ua = r[1] - r[0], ub = r[3] - r[2];
+    return 1e-3 * dot(ua, ua) * dot(ub, ub);
+}
+__device__ __forceinline__ double ee_cross2(const d3 x[4]) {
+    const d3 c = cross(x[1] - x[0], x[3] - x[2]);
+    return dot(c, c);
+}
+__device__ __forceinline__ double mollifier(double c, double eps) {
+    if (c >= eps) return 1.0;
+    const double t = c / eps;
+    return (2.0 - t) * t;
+}
+
+// Full-IPC barrier energy of one candidate (kind 0 PT, 1 EE) at points x, rest points r.
+__device__ __forceinline__ double ipc_full_energy(int kind, const d3 x[4], const d3 r[4], double d2hat,
+                                                  double kappa) {
+    const Feat f = kind == 0 ? pt_feature(x) : ee_feature(x);
+    const double s = feature_dist2_value(kind, f, x);
+    if (!(s < d2hat)) return 0.0;
+    const double b = barrier(s, d2hat, kappa);
+    return kind == 0 ? b : mollifier(ee_cross2(x), ee_mollifier_eps(r)) * b;
+}
+
+// Full-IPC energy of a PT (ckind 0) or EE (1) candidate tuple, points from X, rest points
+// from the scene's rest positions.
+template <class Src>
+__device__ __forceinline__ double cand_energy_full(int ckind, const SceneView& S, const Src& X, const int32_t* t5,
+                                                   double d2hat, double kappa) {
+    int64_t vi[4];
+    if (ckind == 0)
+        fetch_pt_idx(S, t5, vi);
+    else
+        fetch_ee_idx(S, t5, vi);
+    d3 v[4], r[4];
+    X.load4(t5[0], t5[1], vi, ckind == 0 ? 1 : 2, v);
+    for (int k = 0; k < 4; ++k) r[k] = ld3(S.x0 + 3 * vi[k]);
+    return ipc_full_energy(ckind, v, r, d2hat, kappa);
+}
+
+// In-place projection of a symmetric 12x12 matrix onto the PSD cone (negative eigenvalues
+// set to zero) by cyclic Jacobi rotations (Golub & Van Loan, Matrix Computations, 4th ed.,
+// sec. 8.5.2-8.5.3), to a relative off-diagonal norm of 1e-15 or 30 sweeps.
+__device__ inline void psd_project12(double A[144]) {
+    double V[144];
+    for (int i = 0; i < 144; ++i) V[i] = (i / 12 == i % 12) ? 1.0 : 0.0;
+    double fro = 0.0;
+    for (int i = 0; i < 144; ++i) fro += A[i] * A[i];
+    if (fro == 0.0) return;
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < 12; ++p)
+            for (int q = p + 1; q < 12; ++q) off += A[12 * p + q] * A[12 * p + q];
+        if (off <= 1e-30 * fro) break;
+        for (int p = 0; p < 11; ++p)
+            for (int q = p + 1; q < 12; ++q) {
+                const double apq = A[12 * p + q];
+                if (apq == 0.0) continue;
+                const double tau = (A[12 * q + q] - A[12 * p + p]) / (2.0 * apq);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < 12; ++k) {  // A <- A J (columns p, q)
+                    const double akp = A[12 * k + p], akq = A[12 * k + q];
+                    A[12 * k + p] = c * akp - s * akq;
+                    A[12 * k + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 12; ++k) {  // A <- J^T A (rows p, q)
+                    const double apk = A[12 * p + k], aqk = A[12 * q + k];
+                    A[12 * p + k] = c * apk - s * aqk;
+                    A[12 * q + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 12; ++k) {  // V <- V J
+                    const double vkp = V[12 * k + p], vkq = V[12 * k + q];
+                    V[12 * k + p] = c * vkp - s * vkq;
+                    V[12 * k + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    double lam[12];
+    for (int i = 0; i < 12; ++i) lam[i] = A[13 * i] > 0.0 ? A[13 * i] : 0.0;
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < 12; ++k) acc += V[12 * i + k] * lam[k] * V[12 * j + k];
+            A[12 * i + j] = acc;
+        }
+}
+}  // namespace abd

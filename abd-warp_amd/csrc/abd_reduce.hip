@@ -168,6 +168,8 @@ constexpr int kTrialLdsBodies = 128;
 template <class Src>
 __device__ __forceinline__ double energy_term(int kind, const AbdScene& sc, const SceneView& V, const AbdParams& P,
                                               const Src& X, const int32_t* t) {
+    if (P.psd == 3 && kind != kListVG)  // full IPC (beyond the reference)
+        return cand_energy_full(kind == kListPT ? 0 : 1, V, X, t, P.d2hat, P.kappa);
     if (kind == kListEE) return cand_energy_ee(V, X, t, P.d2hat, P.kappa);
     if (kind == kListPT) return cand_energy_pt(V, X, t, P.d2hat, P.kappa);
     return cand_energy_vg(sc.voff, X, t, P.ground, P.d2hat, P.kappa);

@@ -79,8 +79,13 @@ int abd_set_tuning(const AbdTuning* tuning);
  * projected in contact space (eigenvalues b' L_k and b' L_4 + b'' |q_4|^2 clamped at 0),
  * which keeps the Newton matrix positive semi-definite; 2 = the exact Hessian of the
  * barrier energy (b' d2s/dx2 + b'' ds/dx ds/dx^T with the curvature of the plane / line
- * projection, and the b' term of the vertex-ground block), which the adjoint needs.
- * 1 and 2 have no reference counterpart. */
+ * projection, and the b' term of the vertex-ground block), which the adjoint needs;
+ * 3 = full IPC (Li et al. 2020): each PT / EE candidate's barrier of the squared
+ * distance to its closest feature pair (point-plane / line-line, point-edge,
+ * point-point; no validity gate), EE times the mollifier of nearly parallel edges,
+ * the 12x12 Hessian in the points' coordinates projected onto the PSD cone, the
+ * vertex-ground block's 4 d^2 b'' + 2 b' clamped at 0.
+ * 1, 2 and 3 have no reference counterpart. */
 typedef struct AbdParams {
     double dt, dhat, d2hat, kappa, stiffness, gravity, ground, mass, I0, c1, alpha_min, tol;
     double solver_tol;

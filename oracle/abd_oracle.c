@@ -406,8 +406,32 @@ static void fetch_ee(const orc_scene* S, const double* x, const int32_t* t5, v3 
 }
 
 /* ----------------------------------------------------------------- energy */
+/* full-IPC mode (psd = 3, no reference counterpart): abd_oracle_full.c */
+double orc_full_energy(int kind, const double* x, const double* r, double d2hat, double kappa);
+int orc_full_local(int kind, const double* x, const double* r, double d2hat, double kappa, int project, double* E,
+                   double* g12, double* H12, int* feat_out);
+static double full_energy_list(const orc_scene* S, const orc_params* P, const double* xk, const int32_t* t5s, long n,
+                               int kind) {
+    double E = 0.0;
+    for (long i = 0; i < n; ++i) {
+        v3 v[4], r[4];
+        double x12[12], r12[12];
+        if (kind == 0) {
+            fetch_pt(S, xk, t5s + 5 * i, v);
+            fetch_pt(S, S->x0, t5s + 5 * i, r);
+        } else {
+            fetch_ee(S, xk, t5s + 5 * i, v);
+            fetch_ee(S, S->x0, t5s + 5 * i, r);
+        }
+        for (int t = 0; t < 4; ++t) { st3(x12 + 3 * t, v[t]); st3(r12 + 3 * t, r[t]); }
+        E += orc_full_energy(kind, x12, r12, P->d2hat, P->kappa);
+    }
+    return E;
+}
+
 /* ipcsk/energy.py:37-48 ipc_energy_pt (|e2p|^2, valid only) */
 double orc_energy_pt(const orc_scene* S, const orc_params* P, const double* xk, const int32_t* pt, long n) {
+    if (P->psd == 3) return full_energy_list(S, P, xk, pt, n, 0);
     double E = 0.0;
     for (long i = 0; i < n; ++i) {
         v3 v[4]; fetch_pt(S, xk, pt + 5 * i, v);
@@ -420,6 +444,7 @@ double orc_energy_pt(const orc_scene* S, const orc_params* P, const double* xk, 
 }
 /* ipcsk/energy.py:10-23 ipc_energy_ee (l^2, valid only) */
 double orc_energy_ee(const orc_scene* S, const orc_params* P, const double* xk, const int32_t* ee, long n) {
+    if (P->psd == 3) return full_energy_list(S, P, xk, ee, n, 1);
     double E = 0.0;
     for (long i = 0; i < n; ++i) {
         v3 v[4]; fetch_ee(S, xk, ee + 5 * i, v);
@@ -641,6 +666,22 @@ static void put_pair(int nb, long nij, const int32_t* t5, const double g24[24], 
 /* One contact's reference assembly.  Returns 1 when active.  kind 0 PT, 1 EE. */
 int orc_pair_gh(int kind, const orc_params* P, const v3 v[4], const v3 r[4], double g24[24], double Hi[144],
                 double Hj[144], double Hij[144], double* s_out) {
+    if (P->psd == 3) {
+        /* full IPC: closest feature pair, mollified EE, projected 12x12 (abd_oracle_full.c) */
+        double x12[12], r12[12], E, g12[12], H12[144];
+        for (int t = 0; t < 4; ++t) { st3(x12 + 3 * t, v[t]); st3(r12 + 3 * t, r[t]); }
+        if (s_out) *s_out = 0.0;
+        if (!orc_full_local(kind, x12, r12, P->d2hat, P->kappa, 1, &E, g12, H12, NULL)) return 0;
+        double JI[8], JJ[12];
+        int npI = kind == 0 ? 1 : 2;
+        for (int i = 0; i < npI; ++i) { JI[4 * i] = 1.0; JI[4 * i + 1] = r[i].x; JI[4 * i + 2] = r[i].y; JI[4 * i + 3] = r[i].z; }
+        for (int i = 0; i < 4 - npI; ++i) {
+            v3 q = r[npI + i];
+            JJ[4 * i] = 1.0; JJ[4 * i + 1] = q.x; JJ[4 * i + 2] = q.y; JJ[4 * i + 3] = q.z;
+        }
+        map_pair(g12, H12, npI, JI, JJ, 1.0, g24, Hi, Hj, Hij);
+        return 1;
+    }
     v3 e0p, e1p, e2p;
     int valid;
     double s, Q[45], lam[5], D[12];
@@ -745,6 +786,10 @@ void orc_contact_gh(const orc_scene* S, const orc_params* P, const double* x, co
                     double th_j = jj == 0 ? 1.0 : comp(vt, jj - 1);
                     double sc = th_i * th_j * d2b * 4.0 * d * d;
                     if (P->psd == 2) sc += th_i * th_j * 2.0 * db;  /* exact: the b' term of D11 */
+                    if (P->psd == 3) {  /* full IPC: 4 d^2 b'' + 2 b' clamped at 0 */
+                        double cy = d2b * 4.0 * d * d + 2.0 * db;
+                        sc = th_i * th_j * (cy > 0.0 ? cy : 0.0);
+                    }
                     double* B = blocks + 9 * (16 * (long)b + ii + jj * 4);
                     for (int rr = 0; rr < 3; ++rr)
                         for (int cc = 0; cc < 3; ++cc) B[3 * rr + cc] += (comp(nd, rr) * comp(nd, cc)) * sc;

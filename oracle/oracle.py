@@ -92,6 +92,9 @@ def lib():
         L.orc_toi.argtypes = [p, p, p, p, p, l, p, l, p, l]
         L.orc_pair_gh_flat.restype = i
         L.orc_pair_gh_flat.argtypes = [i, p, p, p, p, p, p, p, p]
+        L.orc_full_local.argtypes = [i, p, p, d, d, i, p, p, p, p]
+        L.orc_full_energy.argtypes = [i, p, p, d, d]
+        L.orc_full_energy.restype = d
         L.orc_pair_geometry.argtypes = [i, p, p, p, p, p, p, p]
         L.orc_update_q.argtypes = [i, p, p, p, d, p, p]
         L.orc_update_q0qdot.argtypes = [i, d, p, p, p, p, p, p]
@@ -258,6 +261,23 @@ def pair_gh(kind: int, prm: OrcParams, x4, r4):
     act = lib().orc_pair_gh_flat(kind, C.byref(prm), _ptr(x4), _ptr(r4), _ptr(g24), _ptr(Hi), _ptr(Hj), _ptr(Hij),
                                  _ptr(s))
     return bool(act), float(s[0]), g24, Hi, Hj, Hij
+
+
+def full_local(kind: int, x4, r4, d2hat: float, kappa: float, project: bool = True):
+    """Full-IPC mode, one candidate in its 12 point coordinates: (active, E, g12, H12, feature)
+    with feature = (type, i, j, k): type 0 point-plane / line-line, 1 point x[i] - edge x[j] x[k],
+    2 point x[i] - point x[j]."""
+    x4 = _c(x4, np.float64)
+    r4 = _c(r4, np.float64)
+    E = np.zeros(1)
+    g, H, f = np.zeros(12), np.zeros((12, 12)), np.zeros(4, np.int32)
+    act = lib().orc_full_local(kind, _ptr(x4), _ptr(r4), d2hat, kappa, 1 if project else 0, _ptr(E), _ptr(g),
+                               _ptr(H), _ptr(f))
+    return bool(act), float(E[0]), g, H, tuple(int(v) for v in f)
+
+
+def full_energy(kind: int, x4, r4, d2hat: float, kappa: float) -> float:
+    return float(lib().orc_full_energy(kind, _ptr(_c(x4, np.float64)), _ptr(_c(r4, np.float64)), d2hat, kappa))
 
 
 def pair_geometry(kind: int, x4):

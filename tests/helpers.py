@@ -109,3 +109,55 @@ def oracle_lists(sc, x, dhat, ground, traj_from=None):
     pt = O.cull_prims(True, ij, sc.voff, plo, phi, sc.foff, tlo, thi)
     vg = O.cull_vg(sc, blo, plo, phi, dhat, ground)
     return dict(blo=blo, bhi=bhi, tlo=tlo, thi=thi, elo=elo, ehi=ehi, plo=plo, phi=phi, ij=ij, ee=ee, pt=pt, vg=vg)
+
+
+def _unit(v):
+    return v / np.linalg.norm(v)
+
+
+def full_ipc_config(rng, kind: int, feature: str, dist: float):
+    """A PT (kind 0) or EE (kind 1) contact whose closest feature pair is `feature`
+    ('face' / 'line', 'edge' / 'pe', 'vertex' / 'pp', 'parallel'), at distance `dist`."""
+    while True:
+        t = rng.uniform(-1, 1, size=(3, 3))
+        if np.linalg.norm(np.cross(t[1] - t[0], t[2] - t[0])) > 0.5:
+            break
+    if kind == 0:
+        n = _unit(np.cross(t[1] - t[0], t[2] - t[0]))
+        if feature == "face":
+            return near_contact_pt(rng, dist)
+        if feature == "edge":
+            k = rng.integers(3)
+            a, b, c = t[k], t[(k + 1) % 3], t[(k + 2) % 3]
+            e = _unit(b - a)
+            out = _unit(np.cross(e, n))
+            if np.dot(out, c - a) > 0:
+                out = -out
+            ang = rng.uniform(-1.0, 1.0)
+            d = _unit(np.cos(ang) * out + np.sin(ang) * n)
+            p = a + rng.uniform(0.2, 0.8) * (b - a) + dist * d
+            return np.stack([p, t[0], t[1], t[2]])
+        k = rng.integers(3)  # vertex: beyond the vertex, away from the triangle
+        a = t[k]
+        away = _unit(2 * a - t[(k + 1) % 3] - t[(k + 2) % 3])
+        d = _unit(away + rng.uniform(-0.3, 0.3) * n)
+        return np.stack([a + dist * d, t[0], t[1], t[2]])
+    if feature == "line":
+        return near_contact_ee(rng, dist)
+    b0 = rng.uniform(-1, 1, 3)
+    db = _unit(rng.uniform(-1, 1, 3))
+    b1 = b0 + rng.uniform(0.8, 1.5) * db
+    perp = _unit(np.cross(db, rng.uniform(-1, 1, 3)))
+    if feature == "pe":  # endpoint a0 near the interior of edge b, edge a leaving it
+        a0 = b0 + rng.uniform(0.3, 0.7) * (b1 - b0) + dist * perp
+        da = _unit(perp + rng.uniform(-0.4, 0.4, 3))
+        return np.stack([a0, a0 + rng.uniform(0.6, 1.2) * da, b0, b1])
+    if feature == "pp":  # endpoints a0 and b0 close, both edges leaving
+        a0 = b0 + dist * _unit(-db + rng.uniform(-0.3, 0.3, 3))
+        da = _unit(-db + rng.uniform(-0.5, 0.5, 3))
+        return np.stack([a0, a0 + rng.uniform(0.6, 1.2) * da, b0, b1])
+    # nearly parallel, overlapping, closest points inside both edges
+    da = _unit(db + 0.01 * perp + 0.01 * _unit(np.cross(db, perp)))
+    off = _unit(np.cross(da, db)) if np.linalg.norm(np.cross(da, db)) > 0 else perp
+    a0 = b0 + 0.25 * (b1 - b0) + dist * off - 0.1 * da
+    return np.stack([a0, a0 + 0.7 * np.linalg.norm(b1 - b0) * da, b0, b1])

@@ -163,7 +163,7 @@ def test_trajectory_boxes_bit_exact(lib):
 
 
 # --------------------------------------------------------------- energy and gh
-@pytest.mark.parametrize("psd", [0, 1, 2])
+@pytest.mark.parametrize("psd", [0, 1, 2, 3])
 def test_energy_and_gh_match_oracle(lib, psd):
     from abd_hip.ipc import IPCContactEnergy
     spec = contact_scene()
