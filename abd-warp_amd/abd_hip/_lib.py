@@ -76,6 +76,8 @@ def load():
         "abd_cull_bodies": [p, S, p, p, p, p, i64, p],
         "abd_cull_prims": [p, S, i32, p, i64, p, p, p, p, p, i64, p],
         "abd_cull_vg": [p, S, p, p, p, d, d, p, i64, p],
+        "abd_cull_keep": [p, S, p, p, i64, p, i64, p, i64, p, i64, p, p, p, p, p, p, p, p, d, d, p, p, p, p, p,
+                          C.POINTER(C.c_int64)],
         "abd_set_triplets": [p, i32, p, i64, p, p],
         "abd_bsr_structure": [p, i32, p, p, i64, p, p, p],
         "abd_contact_energy": [p, S, P, p, p, i64, p, i64, p, i64, p],
@@ -104,6 +106,7 @@ def load():
         "abd_sim_destroy": [p],
         "abd_profile_begin": [],
         "abd_profile_end": [],
+        "abd_profile_select": [C.c_uint32],
         "abd_profile_query": [i32, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)],
     }
     for name, args in sigs.items():

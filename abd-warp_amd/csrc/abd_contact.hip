@@ -329,6 +329,14 @@ __global__ __launch_bounds__(kBlock) void compact_flags_kernel(const uint8_t* __
     if (!WRITE && threadIdx.x == 0) bcnt[blockIdx.x] = run;
 }
 
+// sort key of candidate i of [PT | EE]: (type << 32) | body pair
+__device__ __forceinline__ int64_t active_key(const int32_t* __restrict__ pt, int64_t npt,
+                                              const int32_t* __restrict__ ee, int64_t i) {
+    const bool is_ee = i >= npt;
+    const int32_t* t5 = is_ee ? ee + 5 * (i - npt) : pt + 5 * i;
+    return ((int64_t)(is_ee ? 1 : 0) << 32) | (int64_t)t5[2];
+}
+
 // stage C: one thread per active contact.
 __global__ __launch_bounds__(kBlock) void contact_kernel(AbdScene sc, AbdParams P, const double* __restrict__ x,
                                                           const int32_t* __restrict__ pt, int64_t npt,
@@ -336,7 +344,8 @@ __global__ __launch_bounds__(kBlock) void contact_kernel(AbdScene sc, AbdParams 
                                                           const int32_t* __restrict__ act, int64_t nact,
                                                           double* __restrict__ cbuf, int64_t* __restrict__ ckey,
                                                           uint8_t* __restrict__ swap, uint8_t* __restrict__ hl_pt,
-                                                          uint8_t* __restrict__ hl_ee) {
+                                                          uint8_t* __restrict__ hl_ee, uint8_t* __restrict__ pflag,
+                                                          int64_t nij, int64_t* __restrict__ pseg) {
     const SceneView S = view_of(sc);
     for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nact; e += (int64_t)gridDim.x * kBlock) {
         const int64_t i = act[e];
@@ -356,7 +365,16 @@ __global__ __launch_bounds__(kBlock) void contact_kernel(AbdScene sc, AbdParams 
             contact_terms_exact(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, cbuf + e * kDenseStride);
         else
             contact_terms(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, P.psd, cbuf + e * kContactStride);
-        ckey[e] = ((int64_t)(is_ee ? 1 : 0) << 32) | (int64_t)t5[2];
+        const int64_t key = ((int64_t)(is_ee ? 1 : 0) << 32) | (int64_t)t5[2];
+        ckey[e] = key;
+        pflag[t5[2]] = 1;  // the body pair has an active contact (every writer stores 1)
+        // the run of this key in the sorted active list: [begin, end) of (type, pair) from
+        // the neighbours' keys (recomputed here from the lists, not read from ckey)
+        const int64_t kprev = e > 0 ? active_key(pt, npt, ee, act[e - 1]) : -1;
+        const int64_t knext = e + 1 < nact ? active_key(pt, npt, ee, act[e + 1]) : -1;
+        int64_t* ps = pseg + 2 * ((is_ee ? nij : 0) + t5[2]);
+        if (kprev != key) ps[0] = e;
+        if (knext != key) ps[1] = e + 1;
         swap[e] = t5[0] > t5[1] ? 1 : 0;  // body I is ij1 (PT second order)
         if (is_ee)
             hl_ee[t5[1]] = 1;
@@ -365,84 +383,78 @@ __global__ __launch_bounds__(kBlock) void contact_kernel(AbdScene sc, AbdParams 
     }
 }
 
-__device__ __forceinline__ int64_t lower_bound_key(const int64_t* __restrict__ a, int64_t n, int64_t key) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (a[mid] < key)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 constexpr int kPairThreads = 576;  // one thread per entry of the 24x24 body-pair block
 constexpr int kPairBatch = 8;
 constexpr int kPairBufStride = 2 * 144 + 24;  // diag(ij0), diag(ij1), g(ij0), g(ij1)
+constexpr int64_t kPairGrid = 2048;           // workgroups of stage D (grid-stride over pairs)
 
-// stage D: one workgroup per body pair ii.  DENSE: contacts stored by contact_terms_exact.
+// stage D: per body pair ii (grid-stride).  DENSE: contacts stored by contact_terms_exact.
 template <bool DENSE>
 __global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64_t nij,
                                                                     const double* __restrict__ cbuf,
                                                                     const int64_t* __restrict__ ckey,
                                                                     const uint8_t* __restrict__ swap, int64_t nact,
+                                                                    const uint8_t* __restrict__ pflag,
+                                                                    const int64_t* __restrict__ pseg,
                                                                     double* __restrict__ pbuf,
                                                                     double* __restrict__ blocks) {
     constexpr int kStride = DENSE ? kDenseStride : kContactStride;
     constexpr int kBatch = DENSE ? 4 : kPairBatch;
     __shared__ double sc[kBatch][kStride];
     __shared__ uint8_t ssw[kBatch];
-    const int64_t ii = blockIdx.x;
     const int t = threadIdx.x;
-    int64_t seg[2][2];
-    for (int typ = 0; typ < 2; ++typ) {
-        int64_t key = ((int64_t)typ << 32) | ii;
-        seg[typ][0] = lower_bound_key(ckey, nact, key);
-        seg[typ][1] = lower_bound_key(ckey, nact, key + 1);
-    }
     const int R = t / 24, C = t % 24;
-    double h = 0.0, gacc = 0.0;
-    bool any = false;
-    for (int typ = 0; typ < 2; ++typ) {
-        for (int64_t e0 = seg[typ][0]; e0 < seg[typ][1]; e0 += kBatch) {
-            int nbat = (int)((seg[typ][1] - e0) < kBatch ? (seg[typ][1] - e0) : kBatch);
-            for (int q = t; q < nbat * kStride; q += kPairThreads)
-                sc[q / kStride][q % kStride] = cbuf[(e0 + q / kStride) * kStride + q % kStride];
-            if (t < nbat) ssw[t] = swap[e0 + t];
-            __syncthreads();
-            for (int b = 0; b < nbat; ++b) {
-                // body order [ij0 | ij1]: swap halves when body I is ij1
-                const int sw = ssw[b] ? 12 : 0;
-                const int r = (R + sw) % 24, cc = (C + sw) % 24;
-                if (DENSE) {
-                    h += sc[b][24 * r + cc];
-                    if (t < 24) gacc += sc[b][kDenseGOff + (t + sw) % 24];
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 5; ++k) {
-                        const double* V = &sc[b][kVOff + 24 * k];
-                        h += sc[b][kCoefOff + k] * (V[r] * V[cc]);
-                    }
-                    if (t < 24) gacc += sc[b][kB1Off] * sc[b][kVOff + 24 * 4 + (t + sw) % 24];
-                }
-            }
-            any = true;
-            __syncthreads();
+    // body pairs in a grid-stride loop: most pairs have no active contact and are skipped
+    // for the price of one flag read (stage E skips them too)
+    for (int64_t ii = blockIdx.x; ii < nij; ii += gridDim.x) {
+        if (!pflag[ii]) continue;  // uniform over the workgroup
+        int64_t seg[2][2];
+        for (int typ = 0; typ < 2; ++typ) {
+            seg[typ][0] = pseg[2 * (typ * nij + ii)];
+            seg[typ][1] = pseg[2 * (typ * nij + ii) + 1];
         }
-    }
-    double* pb = pbuf + ii * kPairBufStride;
-    // diagonal parts to the pair buffer (stage E adds them per body)
-    if (R < 12 && C < 12) pb[R * 12 + C] = h;
-    if (R >= 12 && C >= 12) pb[144 + (R - 12) * 12 + (C - 12)] = h;
-    if (t < 24) pb[288 + t] = gacc;
-    // off-diagonal: upper slot nb + ii holds rows ij0 x cols ij1, lower slot its transpose
-    if (any && R < 12 && C >= 12) {
-        const int c2 = C - 12;
-        const int bi = R / 3, rr = R % 3, bj = c2 / 3, cc = c2 % 3;
-        const int64_t up = nb + ii, lo = nb + nij + ii;
-        blocks[9 * (16 * up + bi + 4 * bj) + 3 * rr + cc] += h;
-        blocks[9 * (16 * lo + bj + 4 * bi) + 3 * cc + rr] += h;
+        double h = 0.0, gacc = 0.0;
+        bool any = false;
+        for (int typ = 0; typ < 2; ++typ) {
+            for (int64_t e0 = seg[typ][0]; e0 < seg[typ][1]; e0 += kBatch) {
+                int nbat = (int)((seg[typ][1] - e0) < kBatch ? (seg[typ][1] - e0) : kBatch);
+                for (int q = t; q < nbat * kStride; q += kPairThreads)
+                    sc[q / kStride][q % kStride] = cbuf[(e0 + q / kStride) * kStride + q % kStride];
+                if (t < nbat) ssw[t] = swap[e0 + t];
+                __syncthreads();
+                for (int b = 0; b < nbat; ++b) {
+                    // body order [ij0 | ij1]: swap halves when body I is ij1
+                    const int sw = ssw[b] ? 12 : 0;
+                    const int r = (R + sw) % 24, cc = (C + sw) % 24;
+                    if (DENSE) {
+                        h += sc[b][24 * r + cc];
+                        if (t < 24) gacc += sc[b][kDenseGOff + (t + sw) % 24];
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 5; ++k) {
+                            const double* V = &sc[b][kVOff + 24 * k];
+                            h += sc[b][kCoefOff + k] * (V[r] * V[cc]);
+                        }
+                        if (t < 24) gacc += sc[b][kB1Off] * sc[b][kVOff + 24 * 4 + (t + sw) % 24];
+                    }
+                }
+                any = true;
+                __syncthreads();
+            }
+        }
+        double* pb = pbuf + ii * kPairBufStride;
+        // diagonal parts to the pair buffer (stage E adds them per body)
+        if (R < 12 && C < 12) pb[R * 12 + C] = h;
+        if (R >= 12 && C >= 12) pb[144 + (R - 12) * 12 + (C - 12)] = h;
+        if (t < 24) pb[288 + t] = gacc;
+        // off-diagonal: upper slot nb + ii holds rows ij0 x cols ij1, lower slot its transpose
+        if (any && R < 12 && C >= 12) {
+            const int c2 = C - 12;
+            const int bi = R / 3, rr = R % 3, bj = c2 / 3, cc = c2 % 3;
+            const int64_t up = nb + ii, lo = nb + nij + ii;
+            blocks[9 * (16 * up + bi + 4 * bj) + 3 * rr + cc] += h;
+            blocks[9 * (16 * lo + bj + 4 * bi) + 3 * cc + rr] += h;
+        }
     }
 }
 
@@ -463,6 +475,7 @@ __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdP
                                                                 const double* __restrict__ x, int64_t nij,
                                                                 const int32_t* __restrict__ row_ptr,
                                                                 const int32_t* __restrict__ slots,
+                                                                const uint8_t* __restrict__ pflag,
                                                                 const double* __restrict__ pbuf,
                                                                 const int32_t* __restrict__ vg, int64_t nvg,
                                                                 double* __restrict__ g, double* __restrict__ blocks,
@@ -483,6 +496,7 @@ __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdP
             const int s = slots[e];
             if (s < nb) continue;
             const int64_t ii = s < nb + nij ? s - nb : s - nb - nij;
+            if (!pflag[ii]) continue;  // a pair without active contacts adds nothing
             const int side = s < nb + nij ? 0 : 1;  // upper slot in row b: b is ij0
             const double* pb = pbuf + ii * kPairBufStride;
             acc += isg ? pb[288 + 12 * side + R] : pb[144 * side + R * 12 + C];
@@ -545,6 +559,10 @@ __global__ void zero_u8_kernel(uint8_t* a, int64_t n) {
         a[i] = 0;
 }
 __global__ void zero_int_kernel(int* a) { *a = 0; }
+__global__ void zero_i64_kernel(int64_t* a, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = 0;
+}
 
 // Per-contact entry (tests, microbenchmarks): dense 24x24 and gradient per contact.
 __global__ __launch_bounds__(kBlock) void pair_gh_kernel(int kind, AbdParams P, const double* __restrict__ x4,
@@ -664,15 +682,17 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
     if (!P->vg) nvg = 0;
     const int64_t n = npt + nee;
     int rc;
-    // misc: err flag + highlight bytes
+    // misc: err flag, highlight bytes [2 nb], active-pair flags [nij]
     uint8_t* misc;
-    if ((rc = arena_get(kSlotMisc, 64 + 2 * (size_t)nb, (void**)&misc))) return rc;
+    if ((rc = arena_get(kSlotMisc, 64 + 2 * (size_t)nb + (size_t)nij, (void**)&misc))) return rc;
     int* err = reinterpret_cast<int*>(misc);
     uint8_t* hl_pt = misc + 64;
     uint8_t* hl_ee = hl_pt + nb;
+    uint8_t* pflag = hl_ee + nb;
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, err);
     ABD_CHECK_LAUNCH();
-    hipLaunchKernelGGL(zero_u8_kernel, dim3(grid_for(2 * nb)), dim3(kBlock), 0, s, hl_pt, (int64_t)(2 * nb));
+    hipLaunchKernelGGL(zero_u8_kernel, dim3(grid_for(2 * nb + nij)), dim3(kBlock), 0, s, hl_pt,
+                       (int64_t)(2 * nb + nij));
     ABD_CHECK_LAUNCH();
     int64_t nact = 0;
     int32_t* act = nullptr;
@@ -727,27 +747,33 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
             return rc;
         ckey = reinterpret_cast<int64_t*>(cbuf + stride * nact);
         swp = reinterpret_cast<uint8_t*>(ckey + nact);
+        int64_t* pseg;  // [2][nij][2]: empty runs unless written by stage C
+        if ((rc = arena_get(kSlotPairSeg, sizeof(int64_t) * 4 * (size_t)nij, (void**)&pseg))) return rc;
+        hipLaunchKernelGGL(zero_i64_kernel, dim3(grid_for(4 * nij)), dim3(kBlock), 0, s, pseg, 4 * nij);
+        ABD_CHECK_LAUNCH();
         int ph = prof_start(kProfContact, s, (4.0 + 20 + 16 + 192 + 8.0 * stride + 9) * nact);
         hipLaunchKernelGGL(contact_kernel, dim3(grid_for(nact, kBlock, 65535)), dim3(kBlock), 0, s, *sc, *P, x, pt,
-                           npt, ee, (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee);
+                           npt, ee, (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s);
         ph = prof_start(kProfPairReduce, s, (8.0 * stride + 9) * nact + 8.0 * kPairBufStride * nij);
         if (dense)
-            hipLaunchKernelGGL(pair_reduce_kernel<true>, dim3((unsigned)nij), dim3(kPairThreads), 0, s, nb, nij,
-                               (const double*)cbuf, (const int64_t*)ckey, (const uint8_t*)swp, nact, pbuf, blocks);
+            hipLaunchKernelGGL(pair_reduce_kernel<true>, dim3(grid_for(nij, 1, kPairGrid)), dim3(kPairThreads), 0, s,
+                               nb, nij,
+                               (const double*)cbuf, (const int64_t*)ckey, (const uint8_t*)swp, nact,
+                               (const uint8_t*)pflag, (const int64_t*)pseg, pbuf, blocks);
         else
-            hipLaunchKernelGGL(pair_reduce_kernel<false>, dim3((unsigned)nij), dim3(kPairThreads), 0, s, nb, nij,
-                               (const double*)cbuf, (const int64_t*)ckey, (const uint8_t*)swp, nact, pbuf, blocks);
+            hipLaunchKernelGGL(pair_reduce_kernel<false>, dim3(grid_for(nij, 1, kPairGrid)), dim3(kPairThreads), 0, s,
+                               nb, nij,
+                               (const double*)cbuf, (const int64_t*)ckey, (const uint8_t*)swp, nact,
+                               (const uint8_t*)pflag, (const int64_t*)pseg, pbuf, blocks);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s);
-    } else if (nij > 0) {
-        ABD_CHECK(hipMemsetAsync(pbuf, 0, sizeof(double) * kPairBufStride * (size_t)nij, s));
     }
     int phf = prof_start(kProfBodyFinalize, s, 2.0 * 8 * 156 * nb + 8.0 * kPairBufStride * nij + 40.0 * nvg);
     hipLaunchKernelGGL(body_finalize_kernel, dim3(nb), dim3(kBlock), 0, s, *sc, *P, x, nij, row_ptr, slots,
-                       (const double*)pbuf, vg, nvg, g, blocks, (const uint8_t*)hl_pt, (const uint8_t*)hl_ee,
-                       highlight);
+                       (const uint8_t*)pflag, (const double*)pbuf, vg, nvg, g, blocks, (const uint8_t*)hl_pt,
+                       (const uint8_t*)hl_ee, highlight);
     ABD_CHECK_LAUNCH();
     prof_stop(phf, s);
     if (n_active) *n_active = nact;

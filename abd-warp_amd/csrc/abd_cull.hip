@@ -651,3 +651,245 @@ int abd_bsr_structure(abd_stream_t stream, int32_t nb, const int32_t* ij, const 
 }
 
 }  // extern "C"
+
+namespace abd {
+
+// ------------------------------------------------- collision set from a superset
+// The pairs and candidates of a candidate set made from larger boxes (the swept set of
+// the last step: boxes over x - alpha dq, alpha in [0, 1], dilated as the collision set's)
+// that pass the culling predicates on the current boxes.  When the current boxes lie
+// inside those the superset was made from, this is the culling's own output: the same
+// predicates on the same boxes, and the superset's sorted order restricted to the kept
+// entries (pair indices renumbered through `remap`).
+namespace {
+
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void keep_pairs_kernel(int64_t n, const int32_t* __restrict__ in,
+                                                             const float* __restrict__ lo,
+                                                             const float* __restrict__ hi,
+                                                             int64_t* __restrict__ bcnt,
+                                                             const int64_t* __restrict__ boff,
+                                                             int32_t* __restrict__ out, int32_t* __restrict__ remap) {
+    __shared__ int shw[kBlock / kWave];
+    const int64_t base = (int64_t)blockIdx.x * kBlock * kItems;
+    int64_t run = WRITE ? boff[blockIdx.x] : 0;
+    for (int it = 0; it < kItems; ++it) {
+        const int64_t k = base + (int64_t)it * kBlock + threadIdx.x;
+        bool pred = false;
+        int i = 0, j = 0;
+        if (k < n) {
+            i = in[2 * k];
+            j = in[2 * k + 1];
+            pred = overlap(lo + 3 * i, hi + 3 * i, lo + 3 * j, hi + 3 * j);
+        }
+        int tot;
+        const int r = block_rank(pred, shw, &tot);
+        if (WRITE && k < n) {
+            if (pred) {
+                out[2 * (run + r)] = i;
+                out[2 * (run + r) + 1] = j;
+            }
+            if (remap) remap[k] = pred ? (int32_t)(run + r) : -1;
+        }
+        run += tot;
+    }
+    if (!WRITE && threadIdx.x == 0) bcnt[blockIdx.x] = run;
+}
+
+// PT / EE tuples (I, J, ii, a, b): kept when both bodies' boxes and the two primitives'
+// boxes overlap (box 1 of primitive off1[I] + a, box 2 of off2[J] + b); ii renumbered.
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void keep_prims_kernel(int64_t n, const int32_t* __restrict__ in,
+                                                             const float* __restrict__ blo,
+                                                             const float* __restrict__ bhi,
+                                                             const int32_t* __restrict__ off1,
+                                                             const float* __restrict__ lo1,
+                                                             const float* __restrict__ hi1,
+                                                             const int32_t* __restrict__ off2,
+                                                             const float* __restrict__ lo2,
+                                                             const float* __restrict__ hi2,
+                                                             const int32_t* __restrict__ remap,
+                                                             int64_t* __restrict__ bcnt,
+                                                             const int64_t* __restrict__ boff,
+                                                             int32_t* __restrict__ out) {
+    __shared__ int shw[kBlock / kWave];
+    const int64_t base = (int64_t)blockIdx.x * kBlock * kItems;
+    int64_t run = WRITE ? boff[blockIdx.x] : 0;
+    for (int it = 0; it < kItems; ++it) {
+        const int64_t k = base + (int64_t)it * kBlock + threadIdx.x;
+        bool pred = false;
+        const int32_t* t5 = in + 5 * k;
+        if (k < n) {
+            const int I = t5[0], J = t5[1];
+            const int64_t a = off1[I] + (int64_t)t5[3], b = off2[J] + (int64_t)t5[4];
+            pred = overlap(blo + 3 * I, bhi + 3 * I, blo + 3 * J, bhi + 3 * J) &&
+                   overlap(lo1 + 3 * a, hi1 + 3 * a, lo2 + 3 * b, hi2 + 3 * b);
+        }
+        int tot;
+        const int r = block_rank(pred, shw, &tot);
+        if (WRITE && pred) {
+            int32_t* o = out + 5 * (run + r);
+            o[0] = t5[0];
+            o[1] = t5[1];
+            o[2] = remap[t5[2]];
+            o[3] = t5[3];
+            o[4] = t5[4];
+        }
+        run += tot;
+    }
+    if (!WRITE && threadIdx.x == 0) bcnt[blockIdx.x] = run;
+}
+
+// VG tuples (b, v): the predicate of vg_kernel on the current boxes.
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void keep_vg_kernel(int64_t n, const int32_t* __restrict__ in,
+                                                          const int32_t* __restrict__ voff,
+                                                          const float* __restrict__ blo,
+                                                          const float* __restrict__ plo,
+                                                          const float* __restrict__ phi, double ythresh,
+                                                          float fground, int64_t* __restrict__ bcnt,
+                                                          const int64_t* __restrict__ boff,
+                                                          int32_t* __restrict__ out) {
+    __shared__ int shw[kBlock / kWave];
+    const float qlo[3] = {-1e3f, -1e3f, -1e3f};
+    const float qhi[3] = {1e3f, fground, 1e3f};
+    const int64_t base = (int64_t)blockIdx.x * kBlock * kItems;
+    int64_t run = WRITE ? boff[blockIdx.x] : 0;
+    for (int it = 0; it < kItems; ++it) {
+        const int64_t k = base + (int64_t)it * kBlock + threadIdx.x;
+        bool pred = false;
+        if (k < n) {
+            const int b = in[2 * k];
+            const int64_t v = voff[b] + (int64_t)in[2 * k + 1];
+            pred = (double)blo[3 * b + 1] < ythresh && overlap(plo + 3 * v, phi + 3 * v, qlo, qhi);
+        }
+        int tot;
+        const int r = block_rank(pred, shw, &tot);
+        if (WRITE && pred) {
+            out[2 * (run + r)] = in[2 * k];
+            out[2 * (run + r) + 1] = in[2 * k + 1];
+        }
+        run += tot;
+    }
+    if (!WRITE && threadIdx.x == 0) bcnt[blockIdx.x] = run;
+}
+
+}  // namespace
+
+int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, const KeepBoxes& bx, double dhat,
+                       double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount) {
+    const int64_t n[5] = {in.nij, in.nij, in.npt, in.nee, in.nvg};
+    int64_t nblk[5], *bcnt[5], *boff[5];
+    int rc;
+    int64_t total = 0;
+    for (int k = 0; k < 5; ++k) {
+        nblk[k] = ceil_div(n[k], (int64_t)kBlock * kItems);
+        total += nblk[k] + 1;
+    }
+    int64_t* buf;
+    if ((rc = arena_get(kSlotKeep, sizeof(int64_t) * 2 * (size_t)total, (void**)&buf))) return rc;
+    int32_t* remap;
+    if ((rc = arena_get(kSlotKeepMap, sizeof(int32_t) * ((size_t)in.nij + 1), (void**)&remap))) return rc;
+    for (int k = 0; k < 5; ++k) {
+        bcnt[k] = buf;
+        boff[k] = buf + nblk[k] + 1;
+        buf += 2 * (nblk[k] + 1);
+    }
+    const double yth = 0.5 * dhat + ground;
+    const float fg = (float)ground;
+    // counts, then scans (totals copied to dcount[0..4]: nij, nij (ji), npt, nee, nvg)
+    if (n[0] > 0) {
+        hipLaunchKernelGGL(keep_pairs_kernel<false>, dim3((unsigned)nblk[0]), dim3(kBlock), 0, s, n[0], in.ij, bx.blo,
+                           bx.bhi, bcnt[0], (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
+        ABD_CHECK_LAUNCH();
+        if (in.ji) {
+            hipLaunchKernelGGL(keep_pairs_kernel<false>, dim3((unsigned)nblk[1]), dim3(kBlock), 0, s, n[1], in.ji,
+                               bx.blo, bx.bhi, bcnt[1], (const int64_t*)nullptr, (int32_t*)nullptr,
+                               (int32_t*)nullptr);
+            ABD_CHECK_LAUNCH();
+        }
+    }
+    if (n[2] > 0) {
+        hipLaunchKernelGGL(keep_prims_kernel<false>, dim3((unsigned)nblk[2]), dim3(kBlock), 0, s, n[2], in.pt, bx.blo,
+                           bx.bhi, sc->voff, bx.plo, bx.phi, sc->foff, bx.tlo, bx.thi, (const int32_t*)nullptr,
+                           bcnt[2], (const int64_t*)nullptr, (int32_t*)nullptr);
+        ABD_CHECK_LAUNCH();
+    }
+    if (n[3] > 0) {
+        hipLaunchKernelGGL(keep_prims_kernel<false>, dim3((unsigned)nblk[3]), dim3(kBlock), 0, s, n[3], in.ee, bx.blo,
+                           bx.bhi, sc->eoff, bx.elo, bx.ehi, sc->eoff, bx.elo, bx.ehi, (const int32_t*)nullptr,
+                           bcnt[3], (const int64_t*)nullptr, (int32_t*)nullptr);
+        ABD_CHECK_LAUNCH();
+    }
+    if (n[4] > 0) {
+        hipLaunchKernelGGL(keep_vg_kernel<false>, dim3((unsigned)nblk[4]), dim3(kBlock), 0, s, n[4], in.vg, sc->voff,
+                           bx.blo, bx.plo, bx.phi, yth, fg, bcnt[4], (const int64_t*)nullptr, (int32_t*)nullptr);
+        ABD_CHECK_LAUNCH();
+    }
+    for (int k = 0; k < 5; ++k) {
+        if (n[k] == 0 || (k == 1 && !in.ji)) continue;
+        if ((rc = scan_exclusive_i64(s, bcnt[k], boff[k], nblk[k], boff[k] + nblk[k], dcount + k))) return rc;
+    }
+    // writes (pairs first: the tuples take the new pair indices)
+    if (n[0] > 0) {
+        hipLaunchKernelGGL(keep_pairs_kernel<true>, dim3((unsigned)nblk[0]), dim3(kBlock), 0, s, n[0], in.ij, bx.blo,
+                           bx.bhi, bcnt[0], (const int64_t*)boff[0], out.ij, remap);
+        ABD_CHECK_LAUNCH();
+        if (in.ji) {
+            hipLaunchKernelGGL(keep_pairs_kernel<true>, dim3((unsigned)nblk[1]), dim3(kBlock), 0, s, n[1], in.ji,
+                               bx.blo, bx.bhi, bcnt[1], (const int64_t*)boff[1], out.ji, (int32_t*)nullptr);
+            ABD_CHECK_LAUNCH();
+        }
+    }
+    if (n[2] > 0) {
+        hipLaunchKernelGGL(keep_prims_kernel<true>, dim3((unsigned)nblk[2]), dim3(kBlock), 0, s, n[2], in.pt, bx.blo,
+                           bx.bhi, sc->voff, bx.plo, bx.phi, sc->foff, bx.tlo, bx.thi, (const int32_t*)remap,
+                           bcnt[2], (const int64_t*)boff[2], out.pt);
+        ABD_CHECK_LAUNCH();
+    }
+    if (n[3] > 0) {
+        hipLaunchKernelGGL(keep_prims_kernel<true>, dim3((unsigned)nblk[3]), dim3(kBlock), 0, s, n[3], in.ee, bx.blo,
+                           bx.bhi, sc->eoff, bx.elo, bx.ehi, sc->eoff, bx.elo, bx.ehi, (const int32_t*)remap,
+                           bcnt[3], (const int64_t*)boff[3], out.ee);
+        ABD_CHECK_LAUNCH();
+    }
+    if (n[4] > 0) {
+        hipLaunchKernelGGL(keep_vg_kernel<true>, dim3((unsigned)nblk[4]), dim3(kBlock), 0, s, n[4], in.vg, sc->voff,
+                           bx.blo, bx.plo, bx.phi, yth, fg, bcnt[4], (const int64_t*)boff[4], out.vg);
+        ABD_CHECK_LAUNCH();
+    }
+    ABD_CHECK(hipMemcpyAsync(hcount, dcount, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    ABD_CHECK(hipStreamSynchronize(s));
+    for (int k = 0; k < 5; ++k)
+        if (n[k] == 0 || (k == 1 && !in.ji)) hcount[k] = 0;
+    return ABD_OK;
+}
+
+}  // namespace abd
+
+using namespace abd;
+
+extern "C" int abd_cull_keep(abd_stream_t stream, const AbdScene* sc, const int32_t* ij, const int32_t* ji,
+                             int64_t nij, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
+                             const int32_t* vg, int64_t nvg, const float* blo, const float* bhi, const float* plo,
+                             const float* phi, const float* elo, const float* ehi, const float* tlo,
+                             const float* thi, double dhat, double ground, int32_t* ij_out, int32_t* ji_out,
+                             int32_t* pt_out, int32_t* ee_out, int32_t* vg_out, int64_t* counts) {
+    if (!sc || !counts || nij < 0 || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
+    if ((nij && (!ij || !ij_out || (ji && !ji_out))) || (npt && (!pt || !pt_out)) || (nee && (!ee || !ee_out)) ||
+        (nvg && (!vg || !vg_out)))
+        return ABD_ERR_ARG;
+    if (!blo || !bhi || !plo || !phi || !elo || !ehi || !tlo || !thi) return ABD_ERR_ARG;
+    hipStream_t s = to_stream(stream);
+    int64_t *dcount, *hcount;
+    int rc = arena_get(kSlotKeepCount, 8 * sizeof(int64_t), (void**)&dcount);
+    if (!rc) rc = arena_host(8 * sizeof(int64_t), (void**)&hcount);
+    if (rc) return rc;
+    const KeepInput in{ij, ji, pt, ee, vg, nij, npt, nee, nvg};
+    const KeepBoxes bx{blo, bhi, plo, phi, elo, ehi, tlo, thi};
+    const KeepOutput out{ij_out, ji_out, pt_out, ee_out, vg_out};
+    if ((rc = keep_from_superset(s, sc, in, bx, dhat, ground, out, dcount, hcount))) return rc;
+    for (int k = 0; k < 5; ++k) counts[k] = hcount[k];
+    return ABD_OK;
+}
+

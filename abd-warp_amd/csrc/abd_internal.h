@@ -22,7 +22,7 @@ inline hipStream_t to_stream(abd_stream_t s) { return reinterpret_cast<hipStream
 // Per-device scratch arena: `slots` independent growable device buffers plus a
 // small pinned host buffer for count read-back.  Growing synchronizes the
 // device first, so a buffer still in use by queued work is never freed early.
-constexpr int kArenaSlots = 29;
+constexpr int kArenaSlots = 33;
 enum ArenaSlot {
     kSlotScanA = 0,
     kSlotScanB,
@@ -54,6 +54,10 @@ enum ArenaSlot {
     kSlotChunkB,
     kSlotVgA,
     kSlotVgB,
+    kSlotPairSeg,  // per body pair and contact type: the active contacts' [begin, end)
+    kSlotKeep,     // keep_from_superset: block counts and offsets
+    kSlotKeepMap,  // keep_from_superset: old -> new pair index
+    kSlotKeepCount,
 };
 // Launch shapes that change no result (abd_set_tuning); process-wide.
 AbdTuning tuning();
@@ -126,6 +130,23 @@ int update_q_dual(hipStream_t s, int nb, const double* A, const double* p, const
                   double* A_out, double* p_out, double* A_out2, double* p_out2);
 int update_q_scenes_dual(hipStream_t s, const AbdScene* sc, const double* A, const double* p, const double* dq,
                          const double* alpha, double* A_out, double* p_out, double* A_out2, double* p_out2);
+// Collision set from a superset of it (keep_from_superset, abd_cull.hip): the superset's
+// body pairs (ij, and ji when given), PT / EE / VG tuples, the current body / point /
+// edge / triangle boxes; outputs sized like the inputs (no aliasing).  The five totals
+// (nij, nij of ji, npt, nee, nvg) go to dcount (device) and, after one synchronization,
+// to hcount (host, pinned).
+struct KeepInput {
+    const int32_t *ij, *ji, *pt, *ee, *vg;
+    int64_t nij, npt, nee, nvg;
+};
+struct KeepBoxes {
+    const float *blo, *bhi, *plo, *phi, *elo, *ehi, *tlo, *thi;
+};
+struct KeepOutput {
+    int32_t *ij, *ji, *pt, *ee, *vg;
+};
+int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, const KeepBoxes& bx, double dhat,
+                       double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount);
 // abd_cull_bodies without the (j, i) order when ji == nullptr.
 int cull_bodies(hipStream_t s, const AbdScene* sc, const float* lo, const float* hi, int32_t* ij, int32_t* ji,
                 int64_t cap, int64_t* count);

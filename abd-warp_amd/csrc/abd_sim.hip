@@ -49,6 +49,28 @@ struct BoxBuf {
     }
 };
 
+// Body pairs (ij sorted by (i, j), ji the same pairs sorted by (j, i)) and the PT / EE / VG
+// candidate tuples of one set of boxes.
+struct CandSet {
+    DevBuf<int32_t> ij, ji, ptl, eel, vgl;
+    int64_t nij = 0, npt = 0, nee = 0, nvg = 0;
+    int reserve_like(const CandSet& o) {
+        int rc = ij.reserve(2 * (size_t)o.nij + 2);
+        if (!rc) rc = ji.reserve(2 * (size_t)o.nij + 2);
+        if (!rc) rc = ptl.reserve(5 * (size_t)o.npt + 5);
+        if (!rc) rc = eel.reserve(5 * (size_t)o.nee + 5);
+        if (!rc) rc = vgl.reserve(2 * (size_t)o.nvg + 2);
+        return rc;
+    }
+    void release() {
+        ij.release();
+        ji.release();
+        ptl.release();
+        eel.release();
+        vgl.release();
+    }
+};
+
 struct Sim {
     AbdScene sc;
     AbdParams P;
@@ -56,8 +78,10 @@ struct Sim {
     int nb, nv, ne, nf;
     BoxBuf body, tri, edge, pt;         // boxes of the last collision_set
     BoxBuf tbody, ttri, tedge, tpt;     // trajectory boxes
-    DevBuf<int32_t> ij, ji, ptl, eel, vgl;
-    int64_t nij = 0, npt = 0, nee = 0, nvg = 0;
+    // the collision set at xk (gh, BSR structure) and the swept set of the last CCD step
+    // (CCD, line search; the next iteration's collision set is filtered from it)
+    CandSet col, sw;
+    bool sw_valid = false;  // sw covers the current state (set by the CCD, cleared per frame)
     DevBuf<int32_t> row_ptr, cols, slots;
     DevBuf<double> blocks;
     DevBuf<double> scal;  // [0] E contact, [1] E inertia, [2] toi, [3] dot, [4] resid
@@ -101,18 +125,18 @@ int get_scalars(hipStream_t s, Sim& S, int first, int count) {
 // The EE, PT and VG candidate lists of one set of boxes (culling.py:32-76), culled side
 // by side so that their counts come back in two host synchronizations instead of five;
 // the stages copy their totals into S.dcount, one read-back each.
-int cull_lists(hipStream_t s, Sim& S, const BoxBuf& body, const BoxBuf& pts, const BoxBuf& tris,
+int cull_lists(hipStream_t s, Sim& S, CandSet& C, const BoxBuf& body, const BoxBuf& pts, const BoxBuf& tris,
                const BoxBuf& edges) {
     PrimCull ce, cp;
     VgCull cv;
-    const bool pairs = S.nij > 0;
+    const bool pairs = C.nij > 0;
     int64_t* dc = S.dcount.p;
     int rc;
     if (pairs) {
-        if ((rc = prim_cull_begin(s, &S.sc, ABD_CULL_EE, S.ij.p, S.nij, edges.lo.p, edges.hi.p, edges.lo.p,
+        if ((rc = prim_cull_begin(s, &S.sc, ABD_CULL_EE, C.ij.p, C.nij, edges.lo.p, edges.hi.p, edges.lo.p,
                                   edges.hi.p, 0, ce, dc + 0)))
             return rc;
-        if ((rc = prim_cull_begin(s, &S.sc, ABD_CULL_PT, S.ij.p, S.nij, pts.lo.p, pts.hi.p, tris.lo.p, tris.hi.p, 1,
+        if ((rc = prim_cull_begin(s, &S.sc, ABD_CULL_PT, C.ij.p, C.nij, pts.lo.p, pts.hi.p, tris.lo.p, tris.hi.p, 1,
                                   cp, dc + 1)))
             return rc;
     }
@@ -123,10 +147,10 @@ int cull_lists(hipStream_t s, Sim& S, const BoxBuf& body, const BoxBuf& pts, con
         ABD_CHECK(hipStreamSynchronize(s));
     }
     const int64_t T_ee = pairs ? hv[0] : 0, T_pt = pairs ? hv[1] : 0;
-    S.nvg = cv.total ? hv[2] : 0;
-    if ((rc = S.vgl.reserve(2 * (size_t)S.nvg + 2))) return rc;
-    if ((rc = vg_cull_fill(s, cv, S.vgl.p, (int64_t)(S.vgl.cap / 2), S.nvg))) return rc;
-    S.nee = S.npt = 0;
+    C.nvg = cv.total ? hv[2] : 0;
+    if ((rc = C.vgl.reserve(2 * (size_t)C.nvg + 2))) return rc;
+    if ((rc = vg_cull_fill(s, cv, C.vgl.p, (int64_t)(C.vgl.cap / 2), C.nvg))) return rc;
+    C.nee = C.npt = 0;
     if (!pairs) return ABD_OK;
     if ((rc = prim_cull_count(s, ce, T_ee, dc + 3))) return rc;
     if ((rc = prim_cull_count(s, cp, T_pt, dc + 4))) return rc;
@@ -134,34 +158,39 @@ int cull_lists(hipStream_t s, Sim& S, const BoxBuf& body, const BoxBuf& pts, con
         ABD_CHECK(hipMemcpyAsync(hv + 3, dc + 3, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         ABD_CHECK(hipStreamSynchronize(s));
     }
-    S.nee = T_ee ? hv[3] : 0;
-    S.npt = T_pt ? hv[4] : 0;
-    if ((rc = S.eel.reserve(5 * (size_t)S.nee + 5))) return rc;
-    if ((rc = S.ptl.reserve(5 * (size_t)S.npt + 5))) return rc;
-    if ((rc = prim_cull_fill(s, ce, S.eel.p, (int64_t)(S.eel.cap / 5), S.nee))) return rc;
-    return prim_cull_fill(s, cp, S.ptl.p, (int64_t)(S.ptl.cap / 5), S.npt);
+    C.nee = T_ee ? hv[3] : 0;
+    C.npt = T_pt ? hv[4] : 0;
+    if ((rc = C.eel.reserve(5 * (size_t)C.nee + 5))) return rc;
+    if ((rc = C.ptl.reserve(5 * (size_t)C.npt + 5))) return rc;
+    if ((rc = prim_cull_fill(s, ce, C.eel.p, (int64_t)(C.eel.cap / 5), C.nee))) return rc;
+    return prim_cull_fill(s, cp, C.ptl.p, (int64_t)(C.ptl.cap / 5), C.npt);
 }
 
-int body_pairs(hipStream_t s, Sim& S, const BoxBuf& body, bool structure) {
+// Body pairs of C (with the (j, i) order when with_ji), then the BSR structure of the
+// collision set when `structure`.
+int body_pairs(hipStream_t s, Sim& S, CandSet& C, const BoxBuf& body, bool with_ji) {
     for (;;) {
         int64_t cnt = 0;
-        // the (j, i) order only feeds the BSR structure
-        int rc = cull_bodies(s, &S.sc, body.lo.p, body.hi.p, S.ij.p, structure ? S.ji.p : nullptr,
-                             (int64_t)(S.ij.cap / 2), &cnt);
+        int rc = cull_bodies(s, &S.sc, body.lo.p, body.hi.p, C.ij.p, with_ji ? C.ji.p : nullptr,
+                             (int64_t)(C.ij.cap / 2), &cnt);
         if (rc == ABD_ERR_OVERFLOW) {
-            if ((rc = S.ij.reserve(2 * (size_t)cnt))) return rc;
-            if ((rc = S.ji.reserve(2 * (size_t)cnt))) return rc;
+            if ((rc = C.ij.reserve(2 * (size_t)cnt))) return rc;
+            if ((rc = C.ji.reserve(2 * (size_t)cnt))) return rc;
             continue;
         }
         if (rc) return rc;
-        S.nij = cnt;
+        C.nij = cnt;
         break;
     }
-    if (!structure) return ABD_OK;
+    return ABD_OK;
+}
+
+// abd.py:240-243: the slot layout of the collision set's body pairs
+int structure_of(hipStream_t s, Sim& S) {
     int rc;
-    if ((rc = S.cols.reserve(S.nb + 2 * (size_t)S.nij))) return rc;
-    if ((rc = S.slots.reserve(S.nb + 2 * (size_t)S.nij))) return rc;
-    return abd_bsr_structure(s, S.nb, S.ij.p, S.ji.p, S.nij, S.row_ptr.p, S.cols.p, S.slots.p);
+    if ((rc = S.cols.reserve(S.nb + 2 * (size_t)S.col.nij))) return rc;
+    if ((rc = S.slots.reserve(S.nb + 2 * (size_t)S.col.nij))) return rc;
+    return abd_bsr_structure(s, S.nb, S.col.ij.p, S.col.ji.p, S.col.nij, S.row_ptr.p, S.cols.p, S.slots.p);
 }
 
 // abd.py:187-207 (on xk)
@@ -169,11 +198,38 @@ int collision_set(hipStream_t s, Sim& S, bool structure) {
     const double dh = S.P.dhat * 0.5;
     int rc;
     if ((rc = abd_body_boxes(s, &S.sc, S.st.xk, dh, S.body.lo.p, S.body.hi.p))) return rc;
-    if ((rc = body_pairs(s, S, S.body, structure))) return rc;
+    if ((rc = body_pairs(s, S, S.col, S.body, structure))) return rc;
+    if (structure && (rc = structure_of(s, S))) return rc;
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_TRI, S.st.xk, 0.0, S.tri.lo.p, S.tri.hi.p))) return rc;
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_EDGE, S.st.xk, dh, S.edge.lo.p, S.edge.hi.p))) return rc;
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_POINT, S.st.xk, S.P.dhat, S.pt.lo.p, S.pt.hi.p))) return rc;
-    return cull_lists(s, S, S.body, S.pt, S.tri, S.edge);
+    return cull_lists(s, S, S.col, S.body, S.pt, S.tri, S.edge);
+}
+
+// The collision set at xk (with its structure) filtered from the swept set of the last
+// CCD step, which holds every pair and candidate of it: xk = x - alpha dq lies on that
+// step and the swept boxes contain the boxes at every point of it.  The result is the
+// culling's own (keep_from_superset); used for the Newton iterations after a frame's
+// first.
+int collision_from_swept(hipStream_t s, Sim& S) {
+    const double dh = S.P.dhat * 0.5;
+    int rc;
+    if ((rc = abd_body_boxes(s, &S.sc, S.st.xk, dh, S.body.lo.p, S.body.hi.p))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_TRI, S.st.xk, 0.0, S.tri.lo.p, S.tri.hi.p))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_EDGE, S.st.xk, dh, S.edge.lo.p, S.edge.hi.p))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_POINT, S.st.xk, S.P.dhat, S.pt.lo.p, S.pt.hi.p))) return rc;
+    if ((rc = S.col.reserve_like(S.sw))) return rc;
+    const CandSet& W = S.sw;
+    const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg};
+    const KeepBoxes bx{S.body.lo.p, S.body.hi.p, S.pt.lo.p, S.pt.hi.p, S.edge.lo.p, S.edge.hi.p, S.tri.lo.p,
+                       S.tri.hi.p};
+    const KeepOutput out{S.col.ij.p, S.col.ji.p, S.col.ptl.p, S.col.eel.p, S.col.vgl.p};
+    if ((rc = keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, out, S.dcount.p, S.hcount))) return rc;
+    S.col.nij = S.hcount[0];
+    S.col.npt = S.hcount[2];
+    S.col.nee = S.hcount[3];
+    S.col.nvg = S.hcount[4];
+    return structure_of(s, S);
 }
 
 // abd.py:170-185, with both ends of every box dilated as in collision_set: one candidate
@@ -189,11 +245,12 @@ int swept_set(hipStream_t s, Sim& S) {
     const AbdSimState& st = S.st;
     int rc;
     if ((rc = swept_boxes(s, &S.sc, -1, st.x, st.xk, dh, S.tbody.lo.p, S.tbody.hi.p))) return rc;
-    if ((rc = body_pairs(s, S, S.tbody, false))) return rc;
+    // with the (j, i) order: the next iteration's collision set is filtered from this set
+    if ((rc = body_pairs(s, S, S.sw, S.tbody, true))) return rc;
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, st.x, st.xk, 0.0, S.ttri.lo.p, S.ttri.hi.p))) return rc;
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, st.x, st.xk, dh, S.tedge.lo.p, S.tedge.hi.p))) return rc;
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, st.x, st.xk, S.P.dhat, S.tpt.lo.p, S.tpt.hi.p))) return rc;
-    return cull_lists(s, S, S.tbody, S.tpt, S.ttri, S.tedge);
+    return cull_lists(s, S, S.sw, S.tbody, S.tpt, S.ttri, S.tedge);
 }
 
 // abd.py:133-141
@@ -203,7 +260,7 @@ int compute_energy(hipStream_t s, Sim& S, double alpha, double* E) {
     if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, alpha, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
     if ((rc = collision_set(s, S, false))) return rc;
-    if ((rc = abd_contact_energy(s, &S.sc, &S.P, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.scal.p)))
+    if ((rc = abd_contact_energy(s, &S.sc, &S.P, st.xk, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg, S.scal.p)))
         return rc;
     if ((rc = abd_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot, S.scal.p + 1)))
         return rc;
@@ -224,7 +281,7 @@ int line_energy_multi(hipStream_t s, Sim& S, const double* alphas, int K, double
     for (int k = 0; k < K; ++k) ha[k] = alphas[k];
     ABD_CHECK(hipMemcpyAsync(S.cand_alpha.p, ha, sizeof(double) * K, hipMemcpyHostToDevice, s));
     if ((rc = line_trials(s, &S.sc, &S.P, 1, K, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
-                          S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p, S.cand_E.p)))
+                          S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p)))
         return rc;
     ABD_CHECK(hipMemcpyAsync(he, S.cand_E.p, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, s));
     ABD_CHECK(hipStreamSynchronize(s));
@@ -248,12 +305,15 @@ int ccd(hipStream_t s, Sim& S, double* alpha_cap) {
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
     if ((rc = swept_set(s, S))) return rc;
-    S.stats[8] += S.npt;
-    S.stats[9] += S.nee;
+    S.sw_valid = true;
+    S.stats[8] += S.sw.npt;
+    S.stats[9] += S.sw.nee;
     // one reduction plan of the swept set serves the TOI and every line-search trial
-    if ((rc = reduce_plan(s, &S.sc, &S.P, 1, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p))) return rc;
-    if ((rc = scene_toi(s, &S.sc, &S.P, 1, st.x, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p,
-                        S.scal.p + 2)))
+    if ((rc = reduce_plan(s, &S.sc, &S.P, 1, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg,
+                          S.plan.p)))
+        return rc;
+    if ((rc = scene_toi(s, &S.sc, &S.P, 1, st.x, st.xk, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p,
+                        S.sw.nvg, S.plan.p, S.scal.p + 2)))
         return rc;
     if ((rc = get_scalars(s, S, 2, 1))) return rc;
     double t = S.host[2];
@@ -274,25 +334,26 @@ int step(hipStream_t s, Sim& S) {
     ABD_CHECK(hipMemcpyAsync(st.Ak, st.A0, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
     ABD_CHECK(hipMemcpyAsync(st.pk, st.p0, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
     double E0;
+    S.sw_valid = false;  // a frame starts from the state the caller holds
     if ((rc = compute_energy(s, S, 0.0, &E0))) return rc;
     int it = 0;
     for (;;) {
         if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
         if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-        if ((rc = collision_set(s, S, true))) return rc;
-        const size_t nslot = nb + 2 * (size_t)S.nij;
+        if ((rc = S.sw_valid ? collision_from_swept(s, S) : collision_set(s, S, true))) return rc;
+        const size_t nslot = nb + 2 * (size_t)S.col.nij;
         if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
         ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
         ABD_CHECK(hipMemsetAsync(st.g, 0, sizeof(double) * 12 * nb, s));
         if ((rc = abd_inertia_gradient(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g))) return rc;
         if ((rc = abd_inertia_hessian(s, &S.sc, &P, st.A, S.blocks.p))) return rc;
         int64_t nact = 0;
-        if ((rc = abd_contact_gh(s, &S.sc, &P, st.x, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.nij,
+        if ((rc = abd_contact_gh(s, &S.sc, &P, st.x, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg, S.col.nij,
                                  S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, &nact)))
             return rc;
-        S.stats[1] += S.npt;
-        S.stats[2] += S.nee;
-        S.stats[3] += S.nvg;
+        S.stats[1] += S.col.npt;
+        S.stats[2] += S.col.nee;
+        S.stats[3] += S.col.nvg;
         S.stats[4] += nact;
         if ((rc = abd_bsr_solve(s, P.solver, nb, S.row_ptr.p, S.cols.p, S.slots.p, S.blocks.p, st.g, st.dq,
                                 P.solver_tol, P.solver_maxiter, 10, S.iters.p, S.scal.p + 4)))
@@ -362,7 +423,7 @@ int compute_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
     if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
     if ((rc = collision_set(s, S, false))) return rc;
-    if ((rc = abd_scene_contact_energy(s, &S.sc, &S.P, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg,
+    if ((rc = abd_scene_contact_energy(s, &S.sc, &S.P, st.xk, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg,
                                        S.sv.p + kSvEc * ns)))
         return rc;
     if ((rc = abd_scene_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot,
@@ -381,8 +442,8 @@ int line_energy_scenes_multi(hipStream_t s, Sim& S, const double* dalpha, int K)
     int rc;
     const AbdSimState& st = S.st;
     const int ns = S.ns;
-    if ((rc = line_trials(s, &S.sc, &S.P, ns, K, dalpha, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot, S.ptl.p,
-                          S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p, S.cand_E.p)))
+    if ((rc = line_trials(s, &S.sc, &S.P, ns, K, dalpha, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot, S.sw.ptl.p,
+                          S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p)))
         return rc;
     ABD_CHECK(hipMemcpyAsync(S.hcand + (size_t)kLsSpec * ns, S.cand_E.p, sizeof(double) * 2 * ns * K,
                              hipMemcpyDeviceToHost, s));
@@ -408,26 +469,27 @@ int step_scenes(hipStream_t s, Sim& S) {
     it.assign(ns, 0);
     for (int k = 0; k < ns; ++k) S.halpha[k] = 0.0;
     if ((rc = upload(s, dalpha, S.halpha, sizeof(double) * ns))) return rc;
+    S.sw_valid = false;  // a frame starts from the state the caller holds
     if ((rc = compute_energy_scenes(s, S, dalpha))) return rc;
     for (int k = 0; k < ns; ++k) E0[k] = S.hsv[kSvEc * ns + k] + S.hsv[kSvEi * ns + k];
     int n_active = ns;
     while (n_active > 0) {
         if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
         if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-        if ((rc = collision_set(s, S, true))) return rc;
-        const size_t nslot = nb + 2 * (size_t)S.nij;
+        if ((rc = S.sw_valid ? collision_from_swept(s, S) : collision_set(s, S, true))) return rc;
+        const size_t nslot = nb + 2 * (size_t)S.col.nij;
         if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
         ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
         ABD_CHECK(hipMemsetAsync(st.g, 0, sizeof(double) * 12 * nb, s));
         if ((rc = abd_inertia_gradient(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g))) return rc;
         if ((rc = abd_inertia_hessian(s, &S.sc, &P, st.A, S.blocks.p))) return rc;
         int64_t nact = 0;
-        if ((rc = abd_contact_gh(s, &S.sc, &P, st.x, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.nij,
+        if ((rc = abd_contact_gh(s, &S.sc, &P, st.x, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg, S.col.nij,
                                  S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, &nact)))
             return rc;
-        S.stats[1] += S.npt;
-        S.stats[2] += S.nee;
-        S.stats[3] += S.nvg;
+        S.stats[1] += S.col.npt;
+        S.stats[2] += S.col.nee;
+        S.stats[3] += S.col.nvg;
         S.stats[4] += nact;
         for (int k = 0; k < ns; ++k) {
             S.hmask[k] = newton[k];
@@ -444,12 +506,14 @@ int step_scenes(hipStream_t s, Sim& S) {
         if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
         if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
         if ((rc = swept_set(s, S))) return rc;
-        S.stats[8] += S.npt;
-        S.stats[9] += S.nee;
-        if ((rc = reduce_plan(s, &S.sc, &P, ns, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p)))
+        S.sw_valid = true;
+        S.stats[8] += S.sw.npt;
+        S.stats[9] += S.sw.nee;
+        if ((rc = reduce_plan(s, &S.sc, &P, ns, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg,
+                              S.plan.p)))
             return rc;
-        if ((rc = scene_toi(s, &S.sc, &P, ns, st.x, st.xk, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.plan.p,
-                            S.sv.p + kSvToi * ns)))
+        if ((rc = scene_toi(s, &S.sc, &P, ns, st.x, st.xk, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p,
+                            S.sw.nvg, S.plan.p, S.sv.p + kSvToi * ns)))
             return rc;
         if ((rc = abd_scene_dot(s, &S.sc, st.dq, st.g, S.sv.p + kSvQtg * ns))) return rc;
         ABD_CHECK(hipMemcpyAsync(S.hsv + kSvToi * ns, S.sv.p + kSvToi * ns, sizeof(double) * 2 * ns,
@@ -554,11 +618,13 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
     if (!rc) rc = S->tedge.reserve(S->ne + 1);
     if (!rc) rc = S->pt.reserve(S->nv);
     if (!rc) rc = S->tpt.reserve(S->nv);
-    if (!rc) rc = S->ij.reserve(2 * 256);
-    if (!rc) rc = S->ji.reserve(2 * 256);
-    if (!rc) rc = S->ptl.reserve(5 * 4096);
-    if (!rc) rc = S->eel.reserve(5 * 4096);
-    if (!rc) rc = S->vgl.reserve(2 * 1024);
+    for (CandSet* C : {&S->col, &S->sw}) {
+        if (!rc) rc = C->ij.reserve(2 * 256);
+        if (!rc) rc = C->ji.reserve(2 * 256);
+        if (!rc) rc = C->ptl.reserve(5 * 4096);
+        if (!rc) rc = C->eel.reserve(5 * 4096);
+        if (!rc) rc = C->vgl.reserve(2 * 1024);
+    }
     if (!rc) rc = S->row_ptr.reserve(nb + 1);
     if (!rc) rc = S->cols.reserve(nb + 512);
     if (!rc) rc = S->slots.reserve(nb + 512);
@@ -633,13 +699,13 @@ int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double
     if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.xk))) return rc;
     if ((rc = collision_set(s, S, true))) return rc;
-    const size_t nslot = S.nb + 2 * (size_t)S.nij;
+    const size_t nslot = S.nb + 2 * (size_t)S.col.nij;
     if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
     ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
     ABD_CHECK(hipMemsetAsync(st.g, 0, sizeof(double) * 12 * S.nb, s));
     if ((rc = abd_inertia_hessian(s, &S.sc, &S.P, st.A, S.blocks.p))) return rc;
     int64_t nact = 0;
-    if ((rc = abd_contact_gh(s, &S.sc, &S.P, st.x, S.ptl.p, S.npt, S.eel.p, S.nee, S.vgl.p, S.nvg, S.nij,
+    if ((rc = abd_contact_gh(s, &S.sc, &S.P, st.x, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg, S.col.nij,
                              S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, &nact)))
         return rc;
     ABD_CHECK(hipMemsetAsync(y, 0, sizeof(double) * 12 * S.nb, s));
@@ -713,11 +779,8 @@ int abd_sim_destroy(void* sim) {
         b->lo.release();
         b->hi.release();
     }
-    S->ij.release();
-    S->ji.release();
-    S->ptl.release();
-    S->eel.release();
-    S->vgl.release();
+    S->col.release();
+    S->sw.release();
     S->row_ptr.release();
     S->cols.release();
     S->slots.release();

@@ -175,6 +175,20 @@ int abd_cull_prims(abd_stream_t stream, const AbdScene* scene, int32_t mode, con
  * Sorted by (b, v).  *count (host).  Synchronizes. */
 int abd_cull_vg(abd_stream_t stream, const AbdScene* scene, const float* blo, const float* plo, const float* phi,
                 double dhat, double ground, int32_t* out, int64_t cap, int64_t* count);
+/* The collision set from a superset of it (no reference counterpart: the reference culls
+ * from scratch at every state, abd.py:187-207): the body pairs (ij, and ji when not NULL),
+ * PT / EE / VG tuples of a candidate set made from larger boxes (e.g. boxes swept over a
+ * step), kept when they pass the culling predicates on the current boxes (body boxes
+ * blo/bhi, point / edge / triangle boxes); tuples take their pairs' new indices.  When the
+ * current boxes lie inside those the superset was made from, the outputs equal
+ * abd_cull_bodies / abd_cull_prims / abd_cull_vg on the current boxes.  Outputs sized like
+ * the inputs, not aliasing them; counts (host) [5] = nij, nij, npt, nee, nvg.
+ * Synchronizes. */
+int abd_cull_keep(abd_stream_t stream, const AbdScene* scene, const int32_t* ij, const int32_t* ji, int64_t nij,
+                  const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
+                  const float* blo, const float* bhi, const float* plo, const float* phi, const float* elo,
+                  const float* ehi, const float* tlo, const float* thi, double dhat, double ground, int32_t* ij_out,
+                  int32_t* ji_out, int32_t* pt_out, int32_t* ee_out, int32_t* vg_out, int64_t* counts);
 
 /* ---- sparse structure ---- */
 /* orthogonal_energy.py:138-165 _set_triplets (over all Nb + 2 Nij slots): block row/col of

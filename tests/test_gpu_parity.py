@@ -462,3 +462,38 @@ def test_pair_gh_psd_mode_matches_oracle(lib, kind, psd):
             Href = np.block([[Hi, Hij], [Hij.T, Hj]])
             assert rel_err(H24[i], Href) <= 1e-9
             assert rel_err(g24[i], g) <= 1e-9
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_cull_keep_equals_culling(lib, which):
+    """abd_cull_keep on the lists of larger boxes (5 dhat) reproduces the culling of the
+    current boxes exactly: pairs in both orders and every tuple, in order."""
+    import ctypes as C
+    from abd_hip import _lib
+    spec = _scenes()[which]
+    cp = params(ground=spec.ground)
+    m = _gpu_scene(spec)
+    st = m.states()
+    _lib.call("abd_project_vertices", _lib.stream_handle(), m.c, _lib.ptr(st.A), _lib.ptr(st.p), _lib.ptr(m.xk))
+    G = _stage_lists(m, cp)
+    W = _stage_lists(m, params(ground=spec.ground, dhat=5 * cp.dhat))
+    nij, npt, nee, nvg = len(W["pairs"].ij), len(W["pt"]), len(W["ee"]), len(W["vg"])
+    assert nij >= len(G["pairs"].ij) and npt + nee >= len(G["pt"]) + len(G["ee"])
+    out = dict(ij=torch.zeros((nij + 1, 2), dtype=torch.int32, device=DEV),
+               ji=torch.zeros((nij + 1, 2), dtype=torch.int32, device=DEV),
+               pt=torch.zeros((npt + 1, 5), dtype=torch.int32, device=DEV),
+               ee=torch.zeros((nee + 1, 5), dtype=torch.int32, device=DEV),
+               vg=torch.zeros((nvg + 1, 2), dtype=torch.int32, device=DEV))
+    counts = (C.c_int64 * 5)()
+    P = _lib.ptr
+    _lib.call("abd_cull_keep", _lib.stream_handle(), m.c, P(W["pairs"].ij), P(W["pairs"].ji), nij, P(W["pt"]), npt,
+              P(W["ee"]), nee, P(W["vg"]), nvg, P(G["bodies"].lowers), P(G["bodies"].uppers), P(G["pts"].lowers),
+              P(G["pts"].uppers), P(G["edg"].lowers), P(G["edg"].uppers), P(G["tri"].lowers), P(G["tri"].uppers),
+              C.c_double(cp.dhat), C.c_double(cp.ground), P(out["ij"]), P(out["ji"]), P(out["pt"]), P(out["ee"]),
+              P(out["vg"]), counts)
+    n = list(counts)
+    np.testing.assert_array_equal(N(out["ij"])[:n[0]], N(G["pairs"].ij))
+    np.testing.assert_array_equal(N(out["ji"])[:n[1]], N(G["pairs"].ji))
+    np.testing.assert_array_equal(N(out["pt"])[:n[2]], N(G["pt"]).reshape(-1, 5))
+    np.testing.assert_array_equal(N(out["ee"])[:n[3]], N(G["ee"]).reshape(-1, 5))
+    np.testing.assert_array_equal(N(out["vg"])[:n[4]], N(G["vg"]).reshape(-1, 2))
