@@ -94,10 +94,9 @@ int arena_release_current() {
 constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 8;
 
-__global__ __launch_bounds__(kScanThreads) void scan_i64_kernel(const int64_t* __restrict__ in,
-                                                                 int64_t* __restrict__ out, int64_t n,
-                                                                 int64_t* __restrict__ total,
-                                                                 int64_t* __restrict__ total2) {
+__device__ __forceinline__ void scan_block_exclusive(const int64_t* __restrict__ in, int64_t* __restrict__ out,
+                                                     int64_t n, int64_t* __restrict__ total,
+                                                     int64_t* __restrict__ total2) {
     __shared__ int64_t sh[kScanThreads];
     int64_t carry = 0;
     const int t = threadIdx.x;
@@ -127,6 +126,32 @@ __global__ __launch_bounds__(kScanThreads) void scan_i64_kernel(const int64_t* _
     }
     if (t == 0 && total) *total = carry;
     if (t == 0 && total2) *total2 = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_i64_kernel(const int64_t* __restrict__ in,
+                                                                 int64_t* __restrict__ out, int64_t n,
+                                                                 int64_t* __restrict__ total,
+                                                                 int64_t* __restrict__ total2) {
+    scan_block_exclusive(in, out, n, total, total2);
+}
+
+struct ScanSegs {
+    ScanSeg s[kMaxScanSegs];
+};
+// one workgroup per segment
+__global__ __launch_bounds__(kScanThreads) void scan_i64_multi_kernel(ScanSegs S) {
+    const ScanSeg& g = S.s[blockIdx.x];
+    scan_block_exclusive(g.in, g.out, g.n, g.total, g.total2);
+}
+
+int scan_exclusive_i64_multi(hipStream_t s, const ScanSeg* segs, int nseg) {
+    if (nseg < 0 || nseg > kMaxScanSegs) return ABD_ERR_ARG;
+    if (nseg == 0) return ABD_OK;
+    ScanSegs S;
+    for (int k = 0; k < nseg; ++k) S.s[k] = segs[k];
+    hipLaunchKernelGGL(scan_i64_multi_kernel, dim3(nseg), dim3(kScanThreads), 0, s, S);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
 }
 
 int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int64_t* total2) {

@@ -663,205 +663,150 @@ namespace abd {
 // entries (pair indices renumbered through `remap`).
 namespace {
 
-template <bool WRITE>
-__global__ __launch_bounds__(kBlock) void keep_pairs_kernel(int64_t n, const int32_t* __restrict__ in,
-                                                             const float* __restrict__ lo,
-                                                             const float* __restrict__ hi,
-                                                             int64_t* __restrict__ bcnt,
-                                                             const int64_t* __restrict__ boff,
-                                                             int32_t* __restrict__ out, int32_t* __restrict__ remap) {
-    __shared__ int shw[kBlock / kWave];
-    const int64_t base = (int64_t)blockIdx.x * kBlock * kItems;
-    int64_t run = WRITE ? boff[blockIdx.x] : 0;
-    for (int it = 0; it < kItems; ++it) {
-        const int64_t k = base + (int64_t)it * kBlock + threadIdx.x;
-        bool pred = false;
-        int i = 0, j = 0;
-        if (k < n) {
-            i = in[2 * k];
-            j = in[2 * k + 1];
-            pred = overlap(lo + 3 * i, hi + 3 * i, lo + 3 * j, hi + 3 * j);
-        }
-        int tot;
-        const int r = block_rank(pred, shw, &tot);
-        if (WRITE && k < n) {
-            if (pred) {
-                out[2 * (run + r)] = i;
-                out[2 * (run + r) + 1] = j;
-            }
-            if (remap) remap[k] = pred ? (int32_t)(run + r) : -1;
-        }
-        run += tot;
-    }
-    if (!WRITE && threadIdx.x == 0) bcnt[blockIdx.x] = run;
-}
+// The five lists of a keep pass: 0 ij, 1 ji, 2 PT, 3 EE, 4 VG.
+constexpr int kKeepLists = 5;
+struct KeepArgs {
+    KeepInput in;
+    KeepBoxes bx;
+    KeepOutput out;
+    const int32_t *voff, *foff, *eoff;
+    int32_t* remap;
+    double yth;
+    float fg;
+    int64_t n[kKeepLists];
+    int64_t blk0[kKeepLists + 1];  // first workgroup of each list in the launch
+    int64_t* bcnt[kKeepLists];
+    int64_t* boff[kKeepLists];
+};
 
-// PT / EE tuples (I, J, ii, a, b): kept when both bodies' boxes and the two primitives'
-// boxes overlap (box 1 of primitive off1[I] + a, box 2 of off2[J] + b); ii renumbered.
-template <bool WRITE>
-__global__ __launch_bounds__(kBlock) void keep_prims_kernel(int64_t n, const int32_t* __restrict__ in,
-                                                             const float* __restrict__ blo,
-                                                             const float* __restrict__ bhi,
-                                                             const int32_t* __restrict__ off1,
-                                                             const float* __restrict__ lo1,
-                                                             const float* __restrict__ hi1,
-                                                             const int32_t* __restrict__ off2,
-                                                             const float* __restrict__ lo2,
-                                                             const float* __restrict__ hi2,
-                                                             const int32_t* __restrict__ remap,
-                                                             int64_t* __restrict__ bcnt,
-                                                             const int64_t* __restrict__ boff,
-                                                             int32_t* __restrict__ out) {
-    __shared__ int shw[kBlock / kWave];
-    const int64_t base = (int64_t)blockIdx.x * kBlock * kItems;
-    int64_t run = WRITE ? boff[blockIdx.x] : 0;
-    for (int it = 0; it < kItems; ++it) {
-        const int64_t k = base + (int64_t)it * kBlock + threadIdx.x;
-        bool pred = false;
-        const int32_t* t5 = in + 5 * k;
-        if (k < n) {
-            const int I = t5[0], J = t5[1];
-            const int64_t a = off1[I] + (int64_t)t5[3], b = off2[J] + (int64_t)t5[4];
-            pred = overlap(blo + 3 * I, bhi + 3 * I, blo + 3 * J, bhi + 3 * J) &&
-                   overlap(lo1 + 3 * a, hi1 + 3 * a, lo2 + 3 * b, hi2 + 3 * b);
-        }
-        int tot;
-        const int r = block_rank(pred, shw, &tot);
-        if (WRITE && pred) {
-            int32_t* o = out + 5 * (run + r);
-            o[0] = t5[0];
-            o[1] = t5[1];
-            o[2] = remap[t5[2]];
-            o[3] = t5[3];
-            o[4] = t5[4];
-        }
-        run += tot;
+__device__ __forceinline__ bool keep_pred(const KeepArgs& A, int k, int64_t i) {
+    const KeepBoxes& B = A.bx;
+    if (k <= 1) {
+        const int32_t* pr = (k == 0 ? A.in.ij : A.in.ji) + 2 * i;
+        const int a = pr[0], b = pr[1];
+        return overlap(B.blo + 3 * a, B.bhi + 3 * a, B.blo + 3 * b, B.bhi + 3 * b);
     }
-    if (!WRITE && threadIdx.x == 0) bcnt[blockIdx.x] = run;
-}
-
-// VG tuples (b, v): the predicate of vg_kernel on the current boxes.
-template <bool WRITE>
-__global__ __launch_bounds__(kBlock) void keep_vg_kernel(int64_t n, const int32_t* __restrict__ in,
-                                                          const int32_t* __restrict__ voff,
-                                                          const float* __restrict__ blo,
-                                                          const float* __restrict__ plo,
-                                                          const float* __restrict__ phi, double ythresh,
-                                                          float fground, int64_t* __restrict__ bcnt,
-                                                          const int64_t* __restrict__ boff,
-                                                          int32_t* __restrict__ out) {
-    __shared__ int shw[kBlock / kWave];
+    if (k <= 3) {  // (I, J, ii, a, b): both bodies' boxes and the two primitives' boxes
+        const int32_t* t5 = (k == 2 ? A.in.pt : A.in.ee) + 5 * i;
+        const int I = t5[0], J = t5[1];
+        if (!overlap(B.blo + 3 * I, B.bhi + 3 * I, B.blo + 3 * J, B.bhi + 3 * J)) return false;
+        const int64_t a = (k == 2 ? A.voff : A.eoff)[I] + (int64_t)t5[3];
+        const int64_t b = (k == 2 ? A.foff : A.eoff)[J] + (int64_t)t5[4];
+        const float *lo1 = k == 2 ? B.plo : B.elo, *hi1 = k == 2 ? B.phi : B.ehi;
+        const float *lo2 = k == 2 ? B.tlo : B.elo, *hi2 = k == 2 ? B.thi : B.ehi;
+        return overlap(lo1 + 3 * a, hi1 + 3 * a, lo2 + 3 * b, hi2 + 3 * b);
+    }
+    // VG (b, v): the predicate of vg_kernel
     const float qlo[3] = {-1e3f, -1e3f, -1e3f};
-    const float qhi[3] = {1e3f, fground, 1e3f};
-    const int64_t base = (int64_t)blockIdx.x * kBlock * kItems;
-    int64_t run = WRITE ? boff[blockIdx.x] : 0;
+    const float qhi[3] = {1e3f, A.fg, 1e3f};
+    const int b = A.in.vg[2 * i];
+    const int64_t v = A.voff[b] + (int64_t)A.in.vg[2 * i + 1];
+    return (double)B.blo[3 * b + 1] < A.yth && overlap(B.plo + 3 * v, B.phi + 3 * v, qlo, qhi);
+}
+
+__device__ __forceinline__ void keep_write(const KeepArgs& A, int k, int64_t i, int64_t o) {
+    if (k <= 1) {
+        const int32_t* pr = (k == 0 ? A.in.ij : A.in.ji) + 2 * i;
+        int32_t* q = (k == 0 ? A.out.ij : A.out.ji) + 2 * o;
+        q[0] = pr[0];
+        q[1] = pr[1];
+    } else if (k <= 3) {
+        const int32_t* t5 = (k == 2 ? A.in.pt : A.in.ee) + 5 * i;
+        int32_t* q = (k == 2 ? A.out.pt : A.out.ee) + 5 * o;
+        q[0] = t5[0];
+        q[1] = t5[1];
+        q[2] = A.remap[t5[2]];  // the pair's index in the kept ij
+        q[3] = t5[3];
+        q[4] = t5[4];
+    } else {
+        A.out.vg[2 * o] = A.in.vg[2 * i];
+        A.out.vg[2 * o + 1] = A.in.vg[2 * i + 1];
+    }
+}
+
+// One pass over lists [k0, k1): workgroup -> (list, block of kBlock * kItems entries).
+// Count (WRITE false): kept entries per block.  Write: entries at their ordered ranks;
+// list 0 also records every pair's new index (-1 when dropped) in remap.
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void keep_kernel(KeepArgs A, int k0, int k1) {
+    __shared__ int shw[kBlock / kWave];
+    const int64_t g = (int64_t)blockIdx.x + A.blk0[k0];
+    int k = k0;
+    while (k + 1 < k1 && g >= A.blk0[k + 1]) ++k;
+    const int64_t blk = g - A.blk0[k];
+    const int64_t base = blk * kBlock * kItems;
+    int64_t run = WRITE ? A.boff[k][blk] : 0;
     for (int it = 0; it < kItems; ++it) {
-        const int64_t k = base + (int64_t)it * kBlock + threadIdx.x;
-        bool pred = false;
-        if (k < n) {
-            const int b = in[2 * k];
-            const int64_t v = voff[b] + (int64_t)in[2 * k + 1];
-            pred = (double)blo[3 * b + 1] < ythresh && overlap(plo + 3 * v, phi + 3 * v, qlo, qhi);
-        }
+        const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
+        const bool pred = i < A.n[k] && keep_pred(A, k, i);
         int tot;
         const int r = block_rank(pred, shw, &tot);
-        if (WRITE && pred) {
-            out[2 * (run + r)] = in[2 * k];
-            out[2 * (run + r) + 1] = in[2 * k + 1];
+        if (WRITE) {
+            if (pred) keep_write(A, k, i, run + r);
+            if (k == 0 && i < A.n[0]) A.remap[i] = pred ? (int32_t)(run + r) : -1;
         }
         run += tot;
     }
-    if (!WRITE && threadIdx.x == 0) bcnt[blockIdx.x] = run;
+    if (!WRITE && threadIdx.x == 0) A.bcnt[k][blk] = run;
 }
 
 }  // namespace
 
 int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, const KeepBoxes& bx, double dhat,
                        double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount) {
-    const int64_t n[5] = {in.nij, in.nij, in.npt, in.nee, in.nvg};
-    int64_t nblk[5], *bcnt[5], *boff[5];
-    int rc;
-    int64_t total = 0;
-    for (int k = 0; k < 5; ++k) {
+    KeepArgs A;
+    A.in = in;
+    A.bx = bx;
+    A.out = out;
+    A.voff = sc->voff;
+    A.foff = sc->foff;
+    A.eoff = sc->eoff;
+    A.yth = 0.5 * dhat + ground;
+    A.fg = (float)ground;
+    const int64_t n[kKeepLists] = {in.nij, in.ji ? in.nij : 0, in.npt, in.nee, in.nvg};
+    int64_t nblk[kKeepLists], total = 0;
+    A.blk0[0] = 0;
+    for (int k = 0; k < kKeepLists; ++k) {
+        A.n[k] = n[k];
         nblk[k] = ceil_div(n[k], (int64_t)kBlock * kItems);
-        total += nblk[k] + 1;
+        A.blk0[k + 1] = A.blk0[k] + nblk[k];
+        total += 2 * (nblk[k] + 1);
     }
+    if (A.blk0[kKeepLists] > INT32_MAX) return ABD_ERR_ARG;
     int64_t* buf;
-    if ((rc = arena_get(kSlotKeep, sizeof(int64_t) * 2 * (size_t)total, (void**)&buf))) return rc;
-    int32_t* remap;
-    if ((rc = arena_get(kSlotKeepMap, sizeof(int32_t) * ((size_t)in.nij + 1), (void**)&remap))) return rc;
-    for (int k = 0; k < 5; ++k) {
-        bcnt[k] = buf;
-        boff[k] = buf + nblk[k] + 1;
+    int rc;
+    if ((rc = arena_get(kSlotKeep, sizeof(int64_t) * (size_t)total, (void**)&buf))) return rc;
+    if ((rc = arena_get(kSlotKeepMap, sizeof(int32_t) * ((size_t)in.nij + 1), (void**)&A.remap))) return rc;
+    for (int k = 0; k < kKeepLists; ++k) {
+        A.bcnt[k] = buf;
+        A.boff[k] = buf + nblk[k] + 1;
         buf += 2 * (nblk[k] + 1);
     }
-    const double yth = 0.5 * dhat + ground;
-    const float fg = (float)ground;
-    // counts, then scans (totals copied to dcount[0..4]: nij, nij (ji), npt, nee, nvg)
-    if (n[0] > 0) {
-        hipLaunchKernelGGL(keep_pairs_kernel<false>, dim3((unsigned)nblk[0]), dim3(kBlock), 0, s, n[0], in.ij, bx.blo,
-                           bx.bhi, bcnt[0], (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
-        ABD_CHECK_LAUNCH();
-        if (in.ji) {
-            hipLaunchKernelGGL(keep_pairs_kernel<false>, dim3((unsigned)nblk[1]), dim3(kBlock), 0, s, n[1], in.ji,
-                               bx.blo, bx.bhi, bcnt[1], (const int64_t*)nullptr, (int32_t*)nullptr,
-                               (int32_t*)nullptr);
-            ABD_CHECK_LAUNCH();
-        }
-    }
-    if (n[2] > 0) {
-        hipLaunchKernelGGL(keep_prims_kernel<false>, dim3((unsigned)nblk[2]), dim3(kBlock), 0, s, n[2], in.pt, bx.blo,
-                           bx.bhi, sc->voff, bx.plo, bx.phi, sc->foff, bx.tlo, bx.thi, (const int32_t*)nullptr,
-                           bcnt[2], (const int64_t*)nullptr, (int32_t*)nullptr);
+    // counts of all five lists in one launch, their scans (totals to dcount), then the
+    // pairs (ij, ji: the new pair indices) and the tuples
+    const int64_t nall = A.blk0[kKeepLists];
+    if (nall > 0) {
+        hipLaunchKernelGGL(keep_kernel<false>, dim3((unsigned)nall), dim3(kBlock), 0, s, A, 0, kKeepLists);
         ABD_CHECK_LAUNCH();
     }
-    if (n[3] > 0) {
-        hipLaunchKernelGGL(keep_prims_kernel<false>, dim3((unsigned)nblk[3]), dim3(kBlock), 0, s, n[3], in.ee, bx.blo,
-                           bx.bhi, sc->eoff, bx.elo, bx.ehi, sc->eoff, bx.elo, bx.ehi, (const int32_t*)nullptr,
-                           bcnt[3], (const int64_t*)nullptr, (int32_t*)nullptr);
+    ScanSeg sg[kKeepLists];
+    int nseg = 0;
+    for (int k = 0; k < kKeepLists; ++k)
+        if (n[k] > 0) sg[nseg++] = ScanSeg{A.bcnt[k], A.boff[k], nblk[k], A.boff[k] + nblk[k], dcount + k};
+    if ((rc = scan_exclusive_i64_multi(s, sg, nseg))) return rc;
+    const int64_t npairs = A.blk0[2], ntup = nall - npairs;
+    if (npairs > 0) {
+        hipLaunchKernelGGL(keep_kernel<true>, dim3((unsigned)npairs), dim3(kBlock), 0, s, A, 0, 2);
         ABD_CHECK_LAUNCH();
     }
-    if (n[4] > 0) {
-        hipLaunchKernelGGL(keep_vg_kernel<false>, dim3((unsigned)nblk[4]), dim3(kBlock), 0, s, n[4], in.vg, sc->voff,
-                           bx.blo, bx.plo, bx.phi, yth, fg, bcnt[4], (const int64_t*)nullptr, (int32_t*)nullptr);
+    if (ntup > 0) {
+        hipLaunchKernelGGL(keep_kernel<true>, dim3((unsigned)ntup), dim3(kBlock), 0, s, A, 2, kKeepLists);
         ABD_CHECK_LAUNCH();
     }
-    for (int k = 0; k < 5; ++k) {
-        if (n[k] == 0 || (k == 1 && !in.ji)) continue;
-        if ((rc = scan_exclusive_i64(s, bcnt[k], boff[k], nblk[k], boff[k] + nblk[k], dcount + k))) return rc;
-    }
-    // writes (pairs first: the tuples take the new pair indices)
-    if (n[0] > 0) {
-        hipLaunchKernelGGL(keep_pairs_kernel<true>, dim3((unsigned)nblk[0]), dim3(kBlock), 0, s, n[0], in.ij, bx.blo,
-                           bx.bhi, bcnt[0], (const int64_t*)boff[0], out.ij, remap);
-        ABD_CHECK_LAUNCH();
-        if (in.ji) {
-            hipLaunchKernelGGL(keep_pairs_kernel<true>, dim3((unsigned)nblk[1]), dim3(kBlock), 0, s, n[1], in.ji,
-                               bx.blo, bx.bhi, bcnt[1], (const int64_t*)boff[1], out.ji, (int32_t*)nullptr);
-            ABD_CHECK_LAUNCH();
-        }
-    }
-    if (n[2] > 0) {
-        hipLaunchKernelGGL(keep_prims_kernel<true>, dim3((unsigned)nblk[2]), dim3(kBlock), 0, s, n[2], in.pt, bx.blo,
-                           bx.bhi, sc->voff, bx.plo, bx.phi, sc->foff, bx.tlo, bx.thi, (const int32_t*)remap,
-                           bcnt[2], (const int64_t*)boff[2], out.pt);
-        ABD_CHECK_LAUNCH();
-    }
-    if (n[3] > 0) {
-        hipLaunchKernelGGL(keep_prims_kernel<true>, dim3((unsigned)nblk[3]), dim3(kBlock), 0, s, n[3], in.ee, bx.blo,
-                           bx.bhi, sc->eoff, bx.elo, bx.ehi, sc->eoff, bx.elo, bx.ehi, (const int32_t*)remap,
-                           bcnt[3], (const int64_t*)boff[3], out.ee);
-        ABD_CHECK_LAUNCH();
-    }
-    if (n[4] > 0) {
-        hipLaunchKernelGGL(keep_vg_kernel<true>, dim3((unsigned)nblk[4]), dim3(kBlock), 0, s, n[4], in.vg, sc->voff,
-                           bx.blo, bx.plo, bx.phi, yth, fg, bcnt[4], (const int64_t*)boff[4], out.vg);
-        ABD_CHECK_LAUNCH();
-    }
-    ABD_CHECK(hipMemcpyAsync(hcount, dcount, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    ABD_CHECK(hipMemcpyAsync(hcount, dcount, kKeepLists * sizeof(int64_t), hipMemcpyDeviceToHost, s));
     ABD_CHECK(hipStreamSynchronize(s));
-    for (int k = 0; k < 5; ++k)
-        if (n[k] == 0 || (k == 1 && !in.ji)) hcount[k] = 0;
+    for (int k = 0; k < kKeepLists; ++k)
+        if (n[k] == 0) hcount[k] = 0;
     return ABD_OK;
 }
 

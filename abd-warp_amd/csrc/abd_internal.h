@@ -158,6 +158,16 @@ int arena_release_current();
 // writing the total to *total (device).  Fixed-order, single workgroup passes.
 int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total,
                        int64_t* total2 = nullptr);
+// Several independent exclusive scans in one launch (one workgroup each, same order as
+// scan_exclusive_i64); total2 nullable.
+struct ScanSeg {
+    const int64_t* in;
+    int64_t* out;
+    int64_t n;
+    int64_t *total, *total2;
+};
+constexpr int kMaxScanSegs = 8;
+int scan_exclusive_i64_multi(hipStream_t s, const ScanSeg* segs, int nseg);
 // Deterministic sum of n doubles: *out (device) = sum, fixed order.
 // Copies one int64 from device to host (synchronizes `s`).
 int read_i64(hipStream_t s, const int64_t* dev, int64_t* host);
