@@ -108,8 +108,8 @@ int vg_cull_fill(hipStream_t s, VgCull& C, int32_t* out, int64_t cap, int64_t n)
 int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,
                 float* lo, float* hi);
 // K line-search trials q - alpha dq at once (alpha [K][S] on the device, S = 1: the
-// whole system as one scene): per-scene contact energies to E[2 S k + s] and inertia
-// energies to E[2 S k + S + s].
+// whole system as one scene; a negative alpha skips that scene's trial and gives 0):
+// per-scene contact energies to E[2 S k + s] and inertia energies to E[2 S k + S + s].
 int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
                 const double* A, const double* p, const double* dq, const double* A0, const double* p0,
                 const double* Adot, const double* pdot, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
@@ -120,10 +120,11 @@ int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, in
 size_t reduce_plan_elems(int S);
 int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const int32_t* pt, int64_t npt,
                 const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t* plan);
-// Per-scene TOI (abd_scene_toi; S = 1: the whole system as one scene) with an optional plan.
+// Per-scene TOI (abd_scene_toi; S = 1: the whole system as one scene) with an optional plan;
+// active (nullable, [S]): scenes whose TOI is wanted, the others report 1 unevaluated.
 int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
               const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
-              const int64_t* plan, double* toi);
+              const int64_t* plan, double* toi, const uint8_t* active);
 // abd_update_q / abd_update_q_scenes writing the result to a second state as well (when
 // A_out2, p_out2 are given).
 int update_q_dual(hipStream_t s, int nb, const double* A, const double* p, const double* dq, double alpha,

@@ -183,9 +183,12 @@ __device__ __forceinline__ double toi_term(int kind, const AbdScene& sc, const S
     return cand_toi(kind == kListPT ? 0 : 1, V, x0, x1, t);
 }
 
-// The line-search trials of chunk_kernel<2>: step lengths alpha [trial][S] and the state.
+// The line-search trials of chunk_kernel<2>: step lengths alpha [trial][S] (a negative
+// length skips that scene's trial: its sums are 0) and the state; `active` (OP 1,
+// nullable): scenes whose TOI is wanted (the others report 1 without evaluating).
 struct TrialArgs {
     const double *alpha, *A, *p, *dq;
+    const uint8_t* active;
 };
 
 // One chunk per workgroup.  OP 0 sums energies at positions x0; OP 1 takes the minimum
@@ -202,6 +205,11 @@ __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdPara
     const int64_t c = blockIdx.x;
     if (c >= choff[S]) return;
     const int s = scene_of_chunk(choff, S, c);
+    // skipped scenes (uniform over the workgroup): the neutral element of the reduction
+    if ((OP == 1 && T.active && !T.active[s]) || (OP == 2 && T.alpha[(int64_t)S * blockIdx.y + s] < 0.0)) {
+        if (threadIdx.x == 0) part[c] = OP == 1 ? 1.0 : 0.0;
+        return;
+    }
     int64_t k = c - choff[s];
     int l = 0;
     int64_t start = 0, end = 0;
@@ -343,7 +351,8 @@ int get_plan(hipStream_t s, const AbdScene& sc, int S, const Lists& L, const int
 // Plan + chunks + per-scene results for OP over the lists; S = 1 treats the lists as one scene.
 template <int OP>
 int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, const double* x0, const double* x1,
-               const Lists& L, const int64_t* plan_in, double* out, int prof_id, double prof_bytes) {
+               const Lists& L, const int64_t* plan_in, double* out, int prof_id, double prof_bytes,
+               const uint8_t* active = nullptr) {
     const int64_t bound = chunk_bound(L, S);
     if (bound > INT32_MAX) return ABD_ERR_ARG;
     const int64_t *seg, *choff;
@@ -352,8 +361,8 @@ int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
     if (!rc) rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound, (void**)&part);
     if (rc) return rc;
     int ph = prof_start(prof_id, s, prof_bytes);
-    hipLaunchKernelGGL(chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1, TrialArgs{},
-                       L, S, seg, choff, part, (int64_t)0);
+    hipLaunchKernelGGL(chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1,
+                       TrialArgs{nullptr, nullptr, nullptr, nullptr, active}, L, S, seg, choff, part, (int64_t)0);
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s);
     hipLaunchKernelGGL(scene_final_kernel<OP>, dim3(S), dim3(kRedThreads), 0, s, choff, part, out, (int64_t)0,
@@ -406,6 +415,10 @@ __global__ __launch_bounds__(kRedThreads) void inertia_trials_kernel(int S, cons
     __shared__ double sh[17];
     const int s = blockIdx.x, k = blockIdx.y;
     const double al = alpha[(int64_t)S * k + s];
+    if (al < 0.0) {  // skipped trial (uniform over the workgroup)
+        if (threadIdx.x == 0) E[(int64_t)2 * S * k + s] = 0.0;
+        return;
+    }
     const int b0 = S > 1 ? sboff[s] : 0, b1 = S > 1 ? sboff[s + 1] : nb;
     double acc = 0.0;
     for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
@@ -434,8 +447,8 @@ int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, in
     if ((rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound * K, (void**)&part))) return rc;
     int ph = prof_start(kProfEnergy, s, K * (128.0 * npt + 132.0 * nee + 32.0 * nvg));
     hipLaunchKernelGGL(chunk_kernel<2>, dim3((unsigned)bound, K), dim3(kRedThreads), 0, s, *sc, *P,
-                       (const double*)nullptr, (const double*)nullptr, TrialArgs{alpha, A, p, dq}, L, S, seg, choff,
-                       part, bound);
+                       (const double*)nullptr, (const double*)nullptr, TrialArgs{alpha, A, p, dq, nullptr}, L, S, seg,
+                       choff, part, bound);
     ABD_CHECK_LAUNCH();
     prof_stop(ph, s);
     hipLaunchKernelGGL(scene_final_kernel<0>, dim3(S, K), dim3(kRedThreads), 0, s, choff, (const double*)part, E,
@@ -465,12 +478,12 @@ int energy_impl(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, int
 
 int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
               const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
-              const int64_t* plan, double* toi) {
+              const int64_t* plan, double* toi, const uint8_t* active) {
     if (!P || !x_t0 || !x_t1 || !toi || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
     const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);  // abd.py:288-299
     return seg_reduce<1>(s, *sc, *P, S, x_t0, x_t1, L, plan, toi, kProfToi,
-                         224.0 * L.l[1].n + 228.0 * L.l[0].n + 56.0 * L.l[2].n);
+                         224.0 * L.l[1].n + 228.0 * L.l[0].n + 56.0 * L.l[2].n, active);
 }
 
 }  // namespace abd
@@ -497,7 +510,7 @@ int abd_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const d
             const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
             double* toi) {
     if (!sc) return ABD_ERR_ARG;
-    return scene_toi(to_stream(stream), sc, P, 1, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, nullptr, toi);
+    return scene_toi(to_stream(stream), sc, P, 1, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, nullptr, toi, nullptr);
 }
 
 int abd_scene_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x_t0,
@@ -505,7 +518,7 @@ int abd_scene_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, c
                   const int32_t* vg, int64_t nvg, double* toi) {
     if (!scenes_ok(sc)) return ABD_ERR_ARG;
     return scene_toi(to_stream(stream), sc, P, sc->n_scenes, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, nullptr,
-                     toi);
+                     toi, nullptr);
 }
 
 int abd_dot(abd_stream_t stream, const double* a, const double* b, int64_t n, double* out) {

@@ -34,8 +34,8 @@ struct DevBuf {
         cap = want;
         return ABD_OK;
     }
-    void release() {
-        if (p) hipFree(p);
+    void release() {  // teardown: a failure leaves nothing to recover
+        if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -193,31 +193,32 @@ int structure_of(hipStream_t s, Sim& S) {
     return abd_bsr_structure(s, S.nb, S.col.ij.p, S.col.ji.p, S.col.nij, S.row_ptr.p, S.cols.p, S.slots.p);
 }
 
-// abd.py:187-207 (on xk)
-int collision_set(hipStream_t s, Sim& S, bool structure) {
+// abd.py:187-207 on the vertices xv (xk of the reference's compute_energy, or x = xk at the
+// start of a Newton iteration)
+int collision_set(hipStream_t s, Sim& S, bool structure, const double* xv) {
     const double dh = S.P.dhat * 0.5;
     int rc;
-    if ((rc = abd_body_boxes(s, &S.sc, S.st.xk, dh, S.body.lo.p, S.body.hi.p))) return rc;
+    if ((rc = abd_body_boxes(s, &S.sc, xv, dh, S.body.lo.p, S.body.hi.p))) return rc;
     if ((rc = body_pairs(s, S, S.col, S.body, structure))) return rc;
     if (structure && (rc = structure_of(s, S))) return rc;
-    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_TRI, S.st.xk, 0.0, S.tri.lo.p, S.tri.hi.p))) return rc;
-    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_EDGE, S.st.xk, dh, S.edge.lo.p, S.edge.hi.p))) return rc;
-    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_POINT, S.st.xk, S.P.dhat, S.pt.lo.p, S.pt.hi.p))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_TRI, xv, 0.0, S.tri.lo.p, S.tri.hi.p))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_EDGE, xv, dh, S.edge.lo.p, S.edge.hi.p))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_POINT, xv, S.P.dhat, S.pt.lo.p, S.pt.hi.p))) return rc;
     return cull_lists(s, S, S.col, S.body, S.pt, S.tri, S.edge);
 }
 
-// The collision set at xk (with its structure) filtered from the swept set of the last
-// CCD step, which holds every pair and candidate of it: xk = x - alpha dq lies on that
-// step and the swept boxes contain the boxes at every point of it.  The result is the
-// culling's own (keep_from_superset); used for the Newton iterations after a frame's
-// first.
-int collision_from_swept(hipStream_t s, Sim& S) {
+// The collision set at the vertices xv = x(q) of the accepted step (with its structure)
+// filtered from the swept set of the last CCD step, which holds every pair and candidate
+// of it: q = q_prev - alpha dq lies on that step and the swept boxes contain the boxes at
+// every point of it.  The result is the culling's own (keep_from_superset); used for the
+// Newton iterations after a frame's first.
+int collision_from_swept(hipStream_t s, Sim& S, const double* xv) {
     const double dh = S.P.dhat * 0.5;
     int rc;
-    if ((rc = abd_body_boxes(s, &S.sc, S.st.xk, dh, S.body.lo.p, S.body.hi.p))) return rc;
-    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_TRI, S.st.xk, 0.0, S.tri.lo.p, S.tri.hi.p))) return rc;
-    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_EDGE, S.st.xk, dh, S.edge.lo.p, S.edge.hi.p))) return rc;
-    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_POINT, S.st.xk, S.P.dhat, S.pt.lo.p, S.pt.hi.p))) return rc;
+    if ((rc = abd_body_boxes(s, &S.sc, xv, dh, S.body.lo.p, S.body.hi.p))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_TRI, xv, 0.0, S.tri.lo.p, S.tri.hi.p))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_EDGE, xv, dh, S.edge.lo.p, S.edge.hi.p))) return rc;
+    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_POINT, xv, S.P.dhat, S.pt.lo.p, S.pt.hi.p))) return rc;
     if ((rc = S.col.reserve_like(S.sw))) return rc;
     const CandSet& W = S.sw;
     const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg};
@@ -259,7 +260,7 @@ int compute_energy(hipStream_t s, Sim& S, double alpha, double* E) {
     const AbdSimState& st = S.st;
     if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, alpha, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-    if ((rc = collision_set(s, S, false))) return rc;
+    if ((rc = collision_set(s, S, false, st.xk))) return rc;
     if ((rc = abd_contact_energy(s, &S.sc, &S.P, st.xk, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg, S.scal.p)))
         return rc;
     if ((rc = abd_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot, S.scal.p + 1)))
@@ -301,9 +302,9 @@ double dot_host(hipStream_t s, Sim& S, const double* a, const double* b, int* rc
 int ccd(hipStream_t s, Sim& S, double* alpha_cap) {
     int rc;
     const AbdSimState& st = S.st;
+    // x = x(A, p) from the start of the iteration; xk: the end of the full step
     if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, 1.0, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-    if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
     if ((rc = swept_set(s, S))) return rc;
     S.sw_valid = true;
     S.stats[8] += S.sw.npt;
@@ -313,7 +314,7 @@ int ccd(hipStream_t s, Sim& S, double* alpha_cap) {
                           S.plan.p)))
         return rc;
     if ((rc = scene_toi(s, &S.sc, &S.P, 1, st.x, st.xk, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p,
-                        S.sw.nvg, S.plan.p, S.scal.p + 2)))
+                        S.sw.nvg, S.plan.p, S.scal.p + 2, nullptr)))
         return rc;
     if ((rc = get_scalars(s, S, 2, 1))) return rc;
     double t = S.host[2];
@@ -338,9 +339,9 @@ int step(hipStream_t s, Sim& S) {
     if ((rc = compute_energy(s, S, 0.0, &E0))) return rc;
     int it = 0;
     for (;;) {
+        // (Ak, pk) = (A, p) here: the collision set is culled on x
         if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
-        if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-        if ((rc = S.sw_valid ? collision_from_swept(s, S) : collision_set(s, S, true))) return rc;
+        if ((rc = S.sw_valid ? collision_from_swept(s, S, st.x) : collision_set(s, S, true, st.x))) return rc;
         const size_t nslot = nb + 2 * (size_t)S.col.nij;
         if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
         ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
@@ -422,7 +423,7 @@ int compute_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
     const int ns = S.ns;
     if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-    if ((rc = collision_set(s, S, false))) return rc;
+    if ((rc = collision_set(s, S, false, st.xk))) return rc;
     if ((rc = abd_scene_contact_energy(s, &S.sc, &S.P, st.xk, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg,
                                        S.sv.p + kSvEc * ns)))
         return rc;
@@ -474,9 +475,9 @@ int step_scenes(hipStream_t s, Sim& S) {
     for (int k = 0; k < ns; ++k) E0[k] = S.hsv[kSvEc * ns + k] + S.hsv[kSvEi * ns + k];
     int n_active = ns;
     while (n_active > 0) {
+        // (Ak, pk) = (A, p) here: the collision set is culled on x
         if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
-        if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-        if ((rc = S.sw_valid ? collision_from_swept(s, S) : collision_set(s, S, true))) return rc;
+        if ((rc = S.sw_valid ? collision_from_swept(s, S, st.x) : collision_set(s, S, true, st.x))) return rc;
         const size_t nslot = nb + 2 * (size_t)S.col.nij;
         if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
         ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
@@ -504,7 +505,6 @@ int step_scenes(hipStream_t s, Sim& S) {
         if ((rc = upload(s, dccd, S.hccd, sizeof(double) * ns))) return rc;
         if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dccd, st.Ak, st.pk))) return rc;
         if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-        if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
         if ((rc = swept_set(s, S))) return rc;
         S.sw_valid = true;
         S.stats[8] += S.sw.npt;
@@ -512,8 +512,9 @@ int step_scenes(hipStream_t s, Sim& S) {
         if ((rc = reduce_plan(s, &S.sc, &P, ns, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg,
                               S.plan.p)))
             return rc;
+        // the TOI of the scenes still iterating (sactive: the solver's mask of this iteration)
         if ((rc = scene_toi(s, &S.sc, &P, ns, st.x, st.xk, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p,
-                            S.sw.nvg, S.plan.p, S.sv.p + kSvToi * ns)))
+                            S.sw.nvg, S.plan.p, S.sv.p + kSvToi * ns, S.sactive.p)))
             return rc;
         if ((rc = abd_scene_dot(s, &S.sc, st.dq, st.g, S.sv.p + kSvQtg * ns))) return rc;
         ABD_CHECK(hipMemcpyAsync(S.hsv + kSvToi * ns, S.sv.p + kSvToi * ns, sizeof(double) * 2 * ns,
@@ -530,10 +531,10 @@ int step_scenes(hipStream_t s, Sim& S) {
         // trials (alpha, alpha/2, ...) per pass for the scenes still searching
         for (int n_ls = n_active, pass = 0; n_ls > 0; ++pass) {
             const int K = pass == 0 ? 1 : kLsSpec;
-            double* ha = S.hcand;  // [K][ns] trial step lengths (0: the scene does not move)
+            double* ha = S.hcand;  // [K][ns] trial step lengths (-1: the scene is not searching)
             for (int k = 0; k < K; ++k)
                 for (int q = 0; q < ns; ++q)
-                    ha[(size_t)k * ns + q] = !ls[q] ? 0.0 : (k == 0 ? S.halpha[q] : ha[(size_t)(k - 1) * ns + q] * 0.5);
+                    ha[(size_t)k * ns + q] = !ls[q] ? -1.0 : (k == 0 ? S.halpha[q] : ha[(size_t)(k - 1) * ns + q] * 0.5);
             if ((rc = upload(s, S.cand_alpha.p, ha, sizeof(double) * ns * K))) return rc;
             if ((rc = line_energy_scenes_multi(s, S, S.cand_alpha.p, K))) return rc;
             const double* he = S.hcand + (size_t)kLsSpec * ns;
@@ -697,8 +698,7 @@ int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double
     int rc;
     // the Newton matrix of abd.py:230-243 at q = (A, p), contacts from x(q)
     if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
-    if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.xk))) return rc;
-    if ((rc = collision_set(s, S, true))) return rc;
+    if ((rc = collision_set(s, S, true, st.x))) return rc;
     const size_t nslot = S.nb + 2 * (size_t)S.col.nij;
     if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
     ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
@@ -744,7 +744,7 @@ int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen) {
                     rc = -static_cast<int>(he);
                     break;
                 }
-                rc = collision_set(s, S, false);
+                rc = collision_set(s, S, false, S.st.xk);
                 if (!rc && (he = hipEventRecord(e1, s)) != hipSuccess) rc = -static_cast<int>(he);
                 float ms = 0.f;
                 if (!rc && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
@@ -774,7 +774,7 @@ int abd_sim_scene_iters(void* sim, int32_t* iters) {
 int abd_sim_destroy(void* sim) {
     if (!sim) return ABD_OK;
     Sim* S = static_cast<Sim*>(sim);
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();  // queued work may still use the buffers released below
     for (BoxBuf* b : {&S->body, &S->tri, &S->edge, &S->pt, &S->tbody, &S->ttri, &S->tedge, &S->tpt}) {
         b->lo.release();
         b->hi.release();
@@ -793,10 +793,10 @@ int abd_sim_destroy(void* sim) {
     S->cand_alpha.release();
     S->plan.release();
     S->dcount.release();
-    if (S->hcand) hipHostFree(S->hcand);
+    if (S->hcand) (void)hipHostFree(S->hcand);
     for (void* h : {(void*)S->host, (void*)S->hcount, (void*)S->hsv, (void*)S->halpha, (void*)S->hccd, (void*)S->hmask,
                     (void*)S->hiters})
-        if (h) hipHostFree(h);
+        if (h) (void)hipHostFree(h);
     delete S;
     return ABD_OK;
 }
