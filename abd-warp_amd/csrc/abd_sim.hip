@@ -111,7 +111,20 @@ struct Sim {
 // alpha/2, ..., alpha/2^(kLsSpec-1) of the backtracking loop (abd.py:146-160), whose
 // accept test depends only on each trial's own energy, so taking the first accepted
 // trial of a pass gives exactly the sequential loop's step.
-constexpr int kLsSpec = 4;
+constexpr int kLsSpec = 12;
+
+// Trials a pass needs from step length a on: a, a/2, ... up to the first one below
+// alpha_min, which the loop accepts whatever its energy (at most kLsSpec; a pass that
+// ends above alpha_min is followed by another).  With alpha <= 1 and alpha_min = 1e-3
+// one pass after the first always decides.
+int trials_to_floor(double a, double alpha_min) {
+    int k = 1;
+    while (k < kLsSpec && !(a < alpha_min)) {
+        a *= 0.5;
+        ++k;
+    }
+    return k;
+}
 
 // per-scene value rows of Sim::sv / Sim::hsv
 enum { kSvEc = 0, kSvEi, kSvToi, kSvQtg, kSvGtg, kSvAlpha, kSvAlphaCcd, kSv };
@@ -366,8 +379,8 @@ int step(hipStream_t s, Sim& S) {
         if (rc) return rc;
         double alpha = alpha_cap, E1 = 0.0;
         for (int pass = 0;; ++pass) {
-            // the first trial alone (usually accepted), then kLsSpec trials per round trip
-            const int K = pass == 0 ? 1 : kLsSpec;
+            // the first trial alone (usually accepted), then the rest down to alpha_min
+            const int K = pass == 0 ? 1 : trials_to_floor(alpha, P.alpha_min);
             double trial[kLsSpec], Ek[kLsSpec];
             for (int k = 0; k < K; ++k) trial[k] = k == 0 ? alpha : trial[k - 1] * 0.5;
             if ((rc = line_energy_multi(s, S, trial, K, Ek))) return rc;
@@ -527,14 +540,19 @@ int step_scenes(hipStream_t s, Sim& S) {
             if (t < 1.0) t *= 0.9;
             S.halpha[k] = newton[k] ? t : 0.0;
         }
-        // line search, each scene on its own (abd.py:146-160): the first trial, then kLsSpec
-        // trials (alpha, alpha/2, ...) per pass for the scenes still searching
+        // line search, each scene on its own (abd.py:146-160): the first trial, then the
+        // trials (alpha, alpha/2, ...) down to alpha_min for the scenes still searching
         for (int n_ls = n_active, pass = 0; n_ls > 0; ++pass) {
-            const int K = pass == 0 ? 1 : kLsSpec;
-            double* ha = S.hcand;  // [K][ns] trial step lengths (-1: the scene is not searching)
-            for (int k = 0; k < K; ++k)
-                for (int q = 0; q < ns; ++q)
-                    ha[(size_t)k * ns + q] = !ls[q] ? -1.0 : (k == 0 ? S.halpha[q] : ha[(size_t)(k - 1) * ns + q] * 0.5);
+            int K = 1;
+            for (int q = 0; pass > 0 && q < ns; ++q)
+                if (ls[q]) K = std::max(K, trials_to_floor(S.halpha[q], P.alpha_min));
+            double* ha = S.hcand;  // [K][ns] trial step lengths (-1: not evaluated)
+            for (int q = 0; q < ns; ++q) {
+                const int need = !ls[q] ? 0 : pass == 0 ? 1 : trials_to_floor(S.halpha[q], P.alpha_min);
+                for (int k = 0; k < K; ++k)
+                    ha[(size_t)k * ns + q] =
+                        k >= need ? -1.0 : (k == 0 ? S.halpha[q] : ha[(size_t)(k - 1) * ns + q] * 0.5);
+            }
             if ((rc = upload(s, S.cand_alpha.p, ha, sizeof(double) * ns * K))) return rc;
             if ((rc = line_energy_scenes_multi(s, S, S.cand_alpha.p, K))) return rc;
             const double* he = S.hcand + (size_t)kLsSpec * ns;
@@ -542,8 +560,11 @@ int step_scenes(hipStream_t s, Sim& S) {
                 if (!ls[q]) continue;
                 const double qTg = S.hsv[kSvQtg * ns + q];
                 int acc = -1;
+                double last = 0.0;
                 for (int k = 0; k < K && acc < 0; ++k) {
                     const double a = ha[(size_t)k * ns + q];
+                    if (a < 0.0) break;  // past this scene's trials
+                    last = a;
                     const double e1 = he[(size_t)2 * ns * k + q] + he[(size_t)2 * ns * k + ns + q];
                     const bool wolfe = e1 < E0[q] + P.c1 * a * qTg;
                     if (wolfe || a < P.alpha_min) {
@@ -556,7 +577,7 @@ int step_scenes(hipStream_t s, Sim& S) {
                     ls[q] = 0;
                     --n_ls;
                 } else {
-                    S.halpha[q] = ha[(size_t)(K - 1) * ns + q] * 0.5;
+                    S.halpha[q] = last * 0.5;
                 }
             }
         }
