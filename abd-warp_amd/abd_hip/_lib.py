@@ -19,7 +19,7 @@ ABD_ERR_UNSORTED = -1001
 ABD_ERR_NODEVICE = -1002
 KIND_POINT, KIND_EDGE, KIND_TRI = 0, 1, 2
 CULL_EE, CULL_PT = 0, 1
-SOLVER_BICGSTAB, SOLVER_CG_JACOBI, SOLVER_CG_BLOCK = 0, 1, 2
+SOLVER_BICGSTAB, SOLVER_CG_JACOBI, SOLVER_CG_BLOCK, SOLVER_LDLT = 0, 1, 2, 3
 
 
 class AbdScene(C.Structure):

@@ -29,7 +29,7 @@ from .culling import BodyPairs, BvhBuilder, cull, cull_vg, ij_list as _ij_list
 from .ipc import IPCContactEnergy
 from .orthogonal_energy import InertialEnergy
 from .scenes import SceneSpec
-from .sparse import SolveInfo, bicgstab, bsr_cg, bsr_from_pairs
+from .sparse import SolveInfo, bicgstab, bsr_cg, bsr_from_pairs, bsr_ldlt
 
 
 class AffineBodySimulator:
@@ -285,7 +285,9 @@ class AffineBodySimulator:
             self.stats["gh_vg"] += int(vg_list.shape[0])
             self.stats["gh_active"] += nact
             hess = bsr_from_pairs(self.n_bodies, pairs, blocks)
-            if p.solver in (1, 2):
+            if p.solver == 3:
+                bsr_ldlt(hess, self.dq, self.g, p.solver_maxiter, p.solver_tol, info=self.solve_info)
+            elif p.solver in (1, 2):
                 bsr_cg(hess, self.dq, self.g, p.solver_maxiter, p.solver_tol, info=self.solve_info,
                        block_precond=p.solver == 2)
             else:

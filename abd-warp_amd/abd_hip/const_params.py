@@ -25,6 +25,7 @@ class AbdParamsC(ctypes.Structure):
 SOLVER_BICGSTAB = 0
 SOLVER_CG_JACOBI = 1
 SOLVER_CG_BLOCK = 2    # CG with the 12x12 block-Jacobi preconditioner (beyond the reference)
+SOLVER_LDLT = 3        # direct block LDL^T, Jacobi CG above 64 bodies (beyond the reference)
 
 
 @dataclasses.dataclass

@@ -18,6 +18,11 @@
 namespace abd {
 
 constexpr int kSolveThreads = 1024;
+// The direct block LDL^T (ldlt_kernel) takes systems of up to kLdltMaxBodies bodies; it
+// marks a system it declines with kLdltFailed in the iteration count, and the Jacobi CG
+// that follows it solves exactly those.
+constexpr int kLdltMaxBodies = 64;
+constexpr int kLdltFailed = -2;
 // LDS for a system's staged block structure (row offsets and (column, slot) entries).
 constexpr size_t kLdsStructBytes = 24 * 1024;
 
@@ -522,11 +527,14 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
                                                            double tol, int maxiter, int check_every, int cap_rows,
                                                            int budget, double* __restrict__ ws,
                                                            double* __restrict__ dinv, int32_t* __restrict__ iters,
-                                                           double* __restrict__ resid) {
+                                                           double* __restrict__ resid,
+                                                           const int32_t* __restrict__ pending) {
     __shared__ double sh[17];
-    // one system per workgroup: the bodies of scene blockIdx.x (all bodies without scenes)
+    // one system per workgroup: the bodies of scene blockIdx.x (all bodies without scenes);
+    // pending (nullable): only the systems the direct solve declined
     const int sidx = blockIdx.x;
     if (active && !active[sidx]) return;
+    if (pending && pending[sidx] != kLdltFailed) return;
     const int row0 = sboff ? sboff[sidx] : 0;
     const int n = 12 * ((sboff ? sboff[sidx + 1] : M.nb) - row0);
     b += 12 * (int64_t)row0;
@@ -547,6 +555,345 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
         cg_body<BJ, kSpmvRowsStaged>(M, R, n, b, xg, tol, maxiter, check_every, S.V, dinv, iters, resid, sh);
     else
         cg_body<BJ, kSpmvRows>(M, R, n, b, xg, tol, maxiter, check_every, S.V, dinv, iters, resid, sh);
+}
+
+// ------------------------------------------------------------------ direct solve
+// Beyond the reference: the system solved directly by a block LDL^T factorization of its
+// 12x12 body blocks (no exchanges between bodies), fill-in kept; one workgroup per system
+// of at most kLdltMaxBodies bodies.  Structure: one 64-bit upper row mask per body, its
+// fill (eliminating the bodies in their own order), the transposed masks, and the levels
+// of that order's elimination tree (parent of k: the first body above k coupled to it;
+// level: 0 for a leaf, else one more than the highest child's) -- bodies of one level are
+// never coupled to each other and are eliminated together, level after level, in chunks
+// of at most kLdltCols augmented columns.  The factor's upper blocks live in the system's
+// workspace region (row i's blocks from pos[i], in column order), the chunk's W blocks
+// after them.  Elimination of body k: Gauss-Jordan elimination without row exchanges on
+// [P | A'_k,j1 .. A'_k,jd | I], P = A'_kk, one thread per column (12 doubles in
+// registers; the chunk's pivot columns broadcast through LDS, one barrier per pivot
+// column), leaving W_j = P^-1 A'_kj and P^-1 (which replaces P in the factor).  Each
+// Schur target A'_ij (and each forward-solve target y_i) is updated by its chunk's bodies
+// in ascending order, A'_ij -= A'_ki^T W_j, y_i -= W_i^T y_k (12-term sums), by the
+// thread of the lowest of them, so a system gives the same bits whatever else a launch
+// holds; z_k = P^-1 y_k; W then replaces row k.  Backward solve, levels from the top:
+// x_k = z_k - sum_i W_i x_i, each W_i x_i a 12-term sum, added in ascending i.  This is
+// the arithmetic, in the same order, of eliminating body by body in (level, body) order.
+// A pivot that is not positive (the system is not positive definite as factored) or a
+// system above the size limit leaves x untouched and status kLdltFailed; else status 1
+// and resid = |b - H x|^2.
+constexpr int kLdltColsPerThread = 2;
+constexpr int kLdltCols = kLdltColsPerThread * kSolveThreads;
+constexpr int kLdltWBlocks = kLdltCols / 12;  // bound on a chunk's W blocks
+__host__ __device__ inline int64_t ldlt_region(int m) { return 144 * ((int64_t)m * (m + 1) / 2 + kLdltWBlocks); }
+
+// the largest index r in [0, n) with v[r] <= key (v ascending, v[0] <= key)
+__device__ __forceinline__ int last_le(const int* v, int n, int key) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (v[mid] <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ unsigned long long bits_above(unsigned long long row, int k) {
+    return row & ~((2ull << k) - 1ull);
+}
+__device__ __forceinline__ unsigned long long bits_below(int j) { return (1ull << j) - 1ull; }
+
+__global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_t* __restrict__ sboff,
+                                                             const uint8_t* __restrict__ active,
+                                                             const double* __restrict__ b, double* __restrict__ xg,
+                                                             int64_t region, double* __restrict__ ws,
+                                                             int32_t* __restrict__ status, double* __restrict__ resid) {
+    constexpr int MB = kLdltMaxBodies;
+    __shared__ unsigned long long adj[MB];   // upper row pattern with fill (bits >= row)
+    __shared__ unsigned long long colm[MB];  // colm[i]: the rows k < i whose pattern holds i
+    __shared__ unsigned long long levm[MB];  // the bodies of each level
+    __shared__ int lvl[MB];                  // the level of each body
+    __shared__ unsigned char mlist[MB][MB];  // mlist[k]: the columns above k in row k, ascending
+    __shared__ int lrp[MB + 1], pos[MB + 1];
+    __shared__ int cnode[MB], cstart[MB + 1], sstart[MB + 1], woff[MB];  // the chunk
+    __shared__ int nlev, fail, nchunk;
+    __shared__ unsigned long long chunkmask;
+    __shared__ double colc[MB][2][12];
+    __shared__ double y[12 * MB];  // b, then the forward solve, then x
+    __shared__ double z[12 * MB];
+    __shared__ double sh[17];
+    const int sidx = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+    if (active && !active[sidx]) return;
+    const int row0 = sboff ? sboff[sidx] : 0;
+    const int m = (sboff ? sboff[sidx + 1] : M.nb) - row0;
+    if (m > MB) {
+        if (t == 0) status[sidx] = kLdltFailed;
+        return;
+    }
+    const int n = 12 * m;
+    b += 12 * (int64_t)row0;
+    xg += 12 * (int64_t)row0;
+    double* U = ws + region * sidx;
+    double* W = U + 144 * ((int64_t)m * (m + 1) / 2);
+    const int e0 = M.row_ptr[row0];
+    if (t <= m) lrp[t] = M.row_ptr[row0 + t] - e0;
+    if (t < MB) {
+        adj[t] = t < m ? 1ull << t : 0ull;
+        levm[t] = 0ull;
+    }
+    if (t == 0) {
+        nlev = 0;
+        fail = 0;
+    }
+    for (int i = t; i < n; i += nt) y[i] = b[i];
+    __syncthreads();
+    const int nnz = lrp[m];
+    for (int e = t; e < nnz; e += nt) {
+        const int j = M.cols[e0 + e] - row0;
+        const int i = last_le(lrp, m, e);
+        if (j > i && j < m) atomicOr(&adj[i], 1ull << j);
+    }
+    __syncthreads();
+    if (t < 64) {  // wave 0, lane = row: fill, transposed pattern and tree levels, row by row
+        unsigned long long mine = t < m ? adj[t] : 0ull, cm = 0ull;
+        int h = 0;
+        for (int k = 0; k < m; ++k) {
+            const unsigned long long ak = __shfl(mine, k);  // row k is final once k is reached
+            const int hk = __shfl(h, k);                     // so is its level
+            const unsigned long long above = bits_above(ak, k);
+            if ((above >> t) & 1ull) {
+                mine |= ak & (~0ull << t);
+                cm |= 1ull << k;
+            }
+            if (above && t == __builtin_ctzll(above) && h < hk + 1) h = hk + 1;
+        }
+        if (t < m) {
+            adj[t] = mine;
+            colm[t] = cm;
+            lvl[t] = h;
+            atomicOr(&levm[h], 1ull << t);
+            atomicMax(&nlev, h + 1);
+        }
+    }
+    __syncthreads();
+    if (t < m) {
+        int idx = 0;
+        for (unsigned long long bt = bits_above(adj[t], t); bt; bt &= bt - 1ull) mlist[t][idx++] = __builtin_ctzll(bt);
+    }
+    if (t == 0) {
+        int acc = 0;
+        for (int i = 0; i < m; ++i) {
+            pos[i] = acc;
+            acc += __popcll(adj[i]);
+        }
+        pos[m] = acc;
+    }
+    __syncthreads();
+    for (int64_t e = t; e < 144 * (int64_t)pos[m]; e += nt) U[e] = 0.0;
+    __syncthreads();
+    // the matrix's upper blocks (row-major 12x12) into the factor
+    for (int64_t w = t; w < 144 * (int64_t)nnz; w += nt) {
+        const int e = (int)(w / 144), q = (int)(w % 144);
+        const int j = M.cols[e0 + e] - row0;
+        const int i = last_le(lrp, m, e);
+        if (j < i || j >= m) continue;
+        const int R = q / 12, Cq = q % 12;
+        const int64_t blk = pos[i] + __popcll(adj[i] & bits_below(j));
+        U[144 * blk + q] = M.blocks[144 * (int64_t)M.slots[e0 + e] + 9 * (R / 3 + 4 * (Cq / 3)) + 3 * (R % 3) + Cq % 3];
+    }
+    const int levels = nlev;
+    unsigned long long rest = 0ull;  // thread 0: the bodies of the level not yet eliminated
+    for (int L = 0; L < levels; ++L) {
+        if (t == 0) rest = levm[L];
+        for (;;) {
+            __syncthreads();  // the previous chunk is written back, its tables are free
+            if (t == 0) {
+                // the chunk: the lowest bodies of the level not yet eliminated whose
+                // augmented columns fit kLdltCols
+                unsigned long long cmask = 0ull;
+                int nc = 0, cols = 0, sitems = 0, wb = 0;
+                for (unsigned long long bt = rest; bt; bt &= bt - 1ull) {
+                    const int k = __builtin_ctzll(bt);
+                    const int d = __popcll(bits_above(adj[k], k));
+                    if (cols + 12 * (d + 2) > kLdltCols) break;
+                    cnode[nc] = k;
+                    cstart[nc] = cols;
+                    sstart[nc] = sitems;
+                    woff[k] = wb;
+                    cols += 12 * (d + 2);
+                    sitems += 144 * (d * (d + 1) / 2);
+                    wb += d;
+                    cmask |= 1ull << k;
+                    ++nc;
+                }
+                cstart[nc] = cols;
+                sstart[nc] = sitems;
+                nchunk = nc;
+                chunkmask = cmask;
+                rest &= ~cmask;
+            }
+            __syncthreads();
+            const int nc = nchunk;
+            if (nc == 0) break;  // uniform: the level is done
+            const unsigned long long cmask = chunkmask;
+            const int C = cstart[nc];
+            double a[kLdltColsPerThread][12];
+            int cln[kLdltColsPerThread], cg[kLdltColsPerThread], cc_[kLdltColsPerThread], cd[kLdltColsPerThread];
+#pragma unroll
+            for (int s = 0; s < kLdltColsPerThread; ++s) {
+                const int col = t + nt * s;
+                cln[s] = -1;
+                cg[s] = cc_[s] = cd[s] = 0;
+                if (col < C) {
+                    const int ln = last_le(cstart, nc, col);
+                    const int k = cnode[ln], within = col - cstart[ln];
+                    const int g = within / 12, c = within % 12, d = (cstart[ln + 1] - cstart[ln]) / 12 - 2;
+                    cln[s] = ln;
+                    cg[s] = g;
+                    cc_[s] = c;
+                    cd[s] = d;
+                    if (g <= d) {
+                        const double* src = U + 144 * (int64_t)(pos[k] + g) + c;
+#pragma unroll
+                        for (int r = 0; r < 12; ++r) a[s][r] = src[12 * r];
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 12; ++r) a[s][r] = r == c ? 1.0 : 0.0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < 12; ++cc) {
+#pragma unroll
+                for (int s = 0; s < kLdltColsPerThread; ++s)
+                    if (cln[s] >= 0 && cg[s] == 0 && cc_[s] == cc) {
+#pragma unroll
+                        for (int r = 0; r < 12; ++r) colc[cln[s]][cc & 1][r] = a[s][r];
+                        if (!(a[s][cc] > 0.0) || !isfinite(a[s][cc])) fail = 1;
+                    }
+                __syncthreads();
+#pragma unroll
+                for (int s = 0; s < kLdltColsPerThread; ++s)
+                    if (cln[s] >= 0) {
+                        const double* f = colc[cln[s]][cc & 1];
+                        const double inv = 1.0 / f[cc];
+                        const double sc = a[s][cc] * inv;
+#pragma unroll
+                        for (int r = 0; r < 12; ++r)
+                            if (r != cc) a[s][r] -= f[r] * sc;
+                        a[s][cc] = sc;
+                    }
+            }
+            if (fail) {  // uniform: written before the last pivot column's barrier
+                if (t == 0) status[sidx] = kLdltFailed;
+                return;
+            }
+            // W (g = 1..d) to the chunk's area, P^-1 (g = d + 1) over P
+#pragma unroll
+            for (int s = 0; s < kLdltColsPerThread; ++s)
+                if (cln[s] >= 0 && cg[s] >= 1) {
+                    const int k = cnode[cln[s]];
+                    double* dst = cg[s] <= cd[s] ? W + 144 * (int64_t)(woff[k] + cg[s] - 1) + cc_[s]
+                                                 : U + 144 * (int64_t)pos[k] + cc_[s];
+#pragma unroll
+                    for (int r = 0; r < 12; ++r) dst[12 * r] = a[s][r];
+                }
+            __syncthreads();
+            // Schur complement: target (i, j), i <= j, updated by the thread of its lowest
+            // contributor in the chunk, the contributors in ascending order
+            for (int w = t; w < sstart[nc]; w += nt) {
+                const int ln = last_le(sstart, nc, w);
+                const int k = cnode[ln], within = w - sstart[ln];
+                const int pidx = within / 144, q = within % 144;
+                int v = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
+                while (v * (v + 1) / 2 > pidx) --v;
+                while ((v + 1) * (v + 2) / 2 <= pidx) ++v;
+                const int u = pidx - v * (v + 1) / 2;
+                const int i = mlist[k][u], j = mlist[k][v];
+                const unsigned long long contrib = cmask & colm[i] & colm[j];
+                if (__builtin_ctzll(contrib) != k) continue;
+                double* tgt = U + 144 * (int64_t)(pos[i] + __popcll(adj[i] & bits_below(j))) + q;
+                double val = *tgt;
+                for (unsigned long long bt = contrib; bt; bt &= bt - 1ull) {
+                    const int k2 = __builtin_ctzll(bt);
+                    const unsigned long long ab = bits_above(adj[k2], k2);
+                    const double* Aki = U + 144 * (int64_t)(pos[k2] + 1 + __popcll(ab & bits_below(i))) + q / 12;
+                    const double* Wj = W + 144 * (int64_t)(woff[k2] + __popcll(ab & bits_below(j))) + q % 12;
+                    double acc = 0.0;
+#pragma unroll
+                    for (int r = 0; r < 12; ++r) acc += Aki[12 * r] * Wj[12 * r];
+                    val -= acc;
+                }
+                *tgt = val;
+            }
+            // forward solve: y_i -= W_i^T y_k over the chunk's bodies k coupled to i, ascending
+            for (int w = t; w < n; w += nt) {
+                const int i = w / 12, a2 = w % 12;
+                unsigned long long bt = cmask & colm[i];
+                if (!bt) continue;
+                double val = y[w];
+                for (; bt; bt &= bt - 1ull) {
+                    const int k2 = __builtin_ctzll(bt);
+                    const unsigned long long ab = bits_above(adj[k2], k2);
+                    const double* Wi = W + 144 * (int64_t)(woff[k2] + __popcll(ab & bits_below(i))) + a2;
+                    double acc = 0.0;
+#pragma unroll
+                    for (int r = 0; r < 12; ++r) acc += Wi[12 * r] * y[12 * k2 + r];
+                    val -= acc;
+                }
+                y[w] = val;
+            }
+            // z_k = P^-1 y_k for the chunk's bodies
+            for (int w = t; w < 12 * nc; w += nt) {
+                const int k = cnode[w / 12], a2 = w % 12;
+                const double* Di = U + 144 * (int64_t)pos[k] + 12 * a2;
+                double acc = 0.0;
+#pragma unroll
+                for (int r = 0; r < 12; ++r) acc += Di[r] * y[12 * k + r];
+                z[12 * k + a2] = acc;
+            }
+            __syncthreads();
+            // rows of the factor: W
+#pragma unroll
+            for (int s = 0; s < kLdltColsPerThread; ++s)
+                if (cln[s] >= 0 && cg[s] >= 1 && cg[s] <= cd[s]) {
+                    double* dst = U + 144 * (int64_t)(pos[cnode[cln[s]]] + cg[s]) + cc_[s];
+#pragma unroll
+                    for (int r = 0; r < 12; ++r) dst[12 * r] = a[s][r];
+                }
+        }
+    }
+    // backward solve, levels from the top
+    for (int L = levels - 1; L >= 0; --L) {
+        __syncthreads();
+        for (int w = t; w < n; w += nt) {
+            const int k = w / 12, a2 = w % 12;
+            if (lvl[k] != L) continue;
+            double acc = 0.0;
+            int u = 0;
+            for (unsigned long long bt = bits_above(adj[k], k); bt; bt &= bt - 1ull, ++u) {
+                const int i = __builtin_ctzll(bt);
+                const double* Wu = U + 144 * (int64_t)(pos[k] + 1 + u) + 12 * a2;
+                double p = 0.0;
+#pragma unroll
+                for (int c = 0; c < 12; ++c) p += Wu[c] * y[12 * i + c];
+                acc += p;
+            }
+            y[w] = z[w] - acc;
+        }
+    }
+    __syncthreads();
+    Rows R{row0, m, nullptr, nullptr, nullptr};
+    double rr = 0.0;
+    for (int i = t; i < n; i += nt) {
+        xg[i] = y[i];
+        const int lr = i % 12;
+        const double r = b[i] - row_product<kSpmvRows>(M, R, i / 12, lr / 3, lr % 3, y);
+        rr += r * r;
+    }
+    rr = block_sum_fast(rr, sh);
+    if (t == 0) {
+        status[sidx] = 1;
+        if (resid) resid[sidx] = rr;
+    }
 }
 
 // Algorithmic bytes of one solve launch, summed over its systems: per iteration
@@ -748,7 +1095,7 @@ int solve_threads(int) { return kSolveThreads; }
 
 int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const int32_t* sboff, const uint8_t* active,
                  int max_rows, const double* b, double* x, double tol, int32_t maxiter, int32_t check_every,
-                 int32_t* iters, double* resid, int threads) {
+                 int32_t* iters, double* resid, int threads, const int32_t* pending = nullptr) {
     const int n_all = 12 * M.nb;
     double* ws;
     int rc = arena_get(kSlotSolver, sizeof(double) * 6 * (size_t)n_all, (void**)&ws);
@@ -793,20 +1140,20 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
                                maxiter, check_every, max_rows, budget, ws, iters, resid);
         else if (bj)
             hipLaunchKernelGGL((cg_kernel<true, true>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid);
+                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid, pending);
         else
             hipLaunchKernelGGL((cg_kernel<true, false>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid);
+                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid, pending);
     } else {
         if (method == ABD_SOLVER_BICGSTAB)
             hipLaunchKernelGGL(bicgstab_kernel<false>, dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
                                tol, maxiter, check_every, max_rows, budget, ws, iters, resid);
         else if (bj)
             hipLaunchKernelGGL((cg_kernel<false, true>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid);
+                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid, pending);
         else
             hipLaunchKernelGGL((cg_kernel<false, false>), dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x,
-                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid);
+                               tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid, pending);
     }
     ABD_CHECK_LAUNCH();
     if (ph >= 0 && iters) {
@@ -821,6 +1168,26 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
         prof_stop(ph, s);
     }
     return ABD_OK;
+}
+// The direct solve (ldlt_kernel) of every system of at most kLdltMaxBodies bodies, then
+// the Jacobi CG of the systems it declined (status kLdltFailed in iters).
+int launch_ldlt(hipStream_t s, const Bsr& M, int nsys, const int32_t* sboff, const uint8_t* active, int max_bodies,
+                const double* b, double* x, double tol, int32_t maxiter, int32_t check_every, int32_t* iters,
+                double* resid) {
+    int rc;
+    int32_t* status = iters;
+    if (!status && (rc = arena_get(kSlotLdltStatus, sizeof(int32_t) * (size_t)nsys, (void**)&status))) return rc;
+    const int cap = max_bodies < kLdltMaxBodies ? max_bodies : kLdltMaxBodies;
+    const int64_t region = ldlt_region(cap);
+    double* ws;
+    if ((rc = arena_get(kSlotLdlt, sizeof(double) * (size_t)region * (size_t)nsys, (void**)&ws))) return rc;
+    const int ph = prof_start(kProfSolve, s, 1.0);
+    hipLaunchKernelGGL(ldlt_kernel, dim3(nsys), dim3(kSolveThreads), 0, s, M, sboff, active, b, x, region, ws, status,
+                       resid);
+    ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
+    return launch_solve(s, ABD_SOLVER_CG_JACOBI, M, nsys, sboff, active, 12 * max_bodies, b, x, tol, maxiter,
+                        check_every, status, resid, solve_threads(max_bodies), status);
 }
 }  // namespace
 
@@ -877,9 +1244,16 @@ int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t
                   const int32_t* slots, const double* blocks, const double* b, double* x, double tol, int32_t maxiter,
                   int32_t check_every, int32_t* iters, double* resid) {
     if (nb <= 0 || !row_ptr || !cols || !slots || !blocks || !b || !x || check_every <= 0) return ABD_ERR_ARG;
-    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI && method != ABD_SOLVER_CG_BLOCK)
+    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI && method != ABD_SOLVER_CG_BLOCK &&
+        method != ABD_SOLVER_LDLT)
         return ABD_ERR_ARG;
     Bsr M{nb, row_ptr, cols, slots, blocks};
+    if (method == ABD_SOLVER_LDLT) {
+        if (nb <= kLdltMaxBodies)
+            return launch_ldlt(to_stream(stream), M, 1, nullptr, nullptr, nb, b, x, tol, maxiter, check_every, iters,
+                               resid);
+        method = ABD_SOLVER_CG_JACOBI;  // above the direct solve's size: its fallback
+    }
     if (method == ABD_SOLVER_CG_JACOBI && nb > kLargeBodies)
         return solve_cg_multi(to_stream(stream), M, b, x, tol, maxiter, check_every, iters, resid);
     return launch_solve(to_stream(stream), method, M, 1, nullptr, nullptr, 12 * nb, b, x, tol, maxiter, check_every,
@@ -893,9 +1267,13 @@ int abd_bsr_solve_scenes(abd_stream_t stream, int32_t method, const AbdScene* sc
     if (!sc || sc->n_scenes <= 0 || !sc->scene_boff || max_scene_bodies <= 0 || !row_ptr || !cols || !slots ||
         !blocks || !b || !x || check_every <= 0)
         return ABD_ERR_ARG;
-    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI && method != ABD_SOLVER_CG_BLOCK)
+    if (method != ABD_SOLVER_BICGSTAB && method != ABD_SOLVER_CG_JACOBI && method != ABD_SOLVER_CG_BLOCK &&
+        method != ABD_SOLVER_LDLT)
         return ABD_ERR_ARG;
     Bsr M{sc->n_bodies, row_ptr, cols, slots, blocks};
+    if (method == ABD_SOLVER_LDLT)
+        return launch_ldlt(to_stream(stream), M, sc->n_scenes, sc->scene_boff, active, max_scene_bodies, b, x, tol,
+                           maxiter, check_every, iters, resid);
     return launch_solve(to_stream(stream), method, M, sc->n_scenes, sc->scene_boff, active, 12 * max_scene_bodies, b,
                         x, tol, maxiter, check_every, iters, resid, solve_threads(max_scene_bodies));
 }

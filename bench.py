@@ -205,7 +205,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="batched")
     ap.add_argument("--psd", type=int, default=1)
-    ap.add_argument("--solver", type=int, default=2, help="0 BiCGSTAB, 1 Jacobi CG, 2 block-Jacobi CG")
+    ap.add_argument("--solver", type=int, default=3,
+                    help="0 BiCGSTAB, 1 Jacobi CG, 2 block-Jacobi CG, 3 direct block LDL^T")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=8,
@@ -343,7 +344,7 @@ def main():
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": desc, "mode": "batched" if batched else "plain", "psd_projection": bool(a.psd),
                    "driver": "python" if a.python_driver else "native",
-                   "solver": {0: "bicgstab", 1: "jacobi-cg", 2: "block-jacobi-cg"}[a.solver],
+                   "solver": {0: "bicgstab", 1: "jacobi-cg", 2: "block-jacobi-cg", 3: "block-ldlt"}[a.solver],
                    "frames_timed": f"{first}..{first + a.steps - 1}", "tiles": tiles},
         "gh_pairs_per_s": cands / gh_s if gh_s > 0 else None,
         "gh_candidates": cands,

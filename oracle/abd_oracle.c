@@ -1108,6 +1108,153 @@ long orc_bsr_cg_block(long nslot, const int32_t* rows, const int32_t* cols, cons
     return end_iter;
 }
 
+/* Beyond the reference: the Newton system solved directly, by a block LDL^T factorization
+ * of the 12x12 body blocks (no exchanges between bodies), fill-in kept.  The structural
+ * pattern is the slots' (every triplet's block), so it matches the BSR matrix's; the fill
+ * is that of eliminating the bodies in their own order.  The bodies are then eliminated
+ * level by level of that order's elimination tree (parent of k: the first body above k
+ * coupled to it; level: 0 for a leaf, else one more than the highest child's), in
+ * ascending order within a level -- an order in which every body comes after all bodies
+ * that update it.  Step k: Gauss-Jordan elimination without row exchanges on
+ * [P | A'_k,j1 .. A'_k,jd | I], P = A'_kk (column by column: s = a[c][t] / p, a[r][t] -=
+ * a[r][c] s for r != c, a[c][t] = s), leaving W_j = P^-1 A'_kj and P^-1; then
+ * A'_ij -= A'_ki^T W_j for i <= j among j1..jd (each entry a 12-term sum in row order);
+ * the forward solve's step z_k = P^-1 y_k, y_i -= W_i^T y_k (12-term sums); row k becomes
+ * W.  The backward solve, levels from the top: x_k = z_k - sum_i W_i x_i with each W_i x_i
+ * a 12-term sum, added in ascending i.  Systems above ORC_LDLT_MAX_BODIES bodies, or a
+ * pivot that is not positive, return -2 with x untouched (the caller falls back to an
+ * iterative solve). */
+#define ORC_LDLT_MAX_BODIES 64
+long orc_bsr_ldlt(long nslot, const int32_t* rows, const int32_t* cols, const double* blocks, const double* b,
+                  double* x, long n) {
+    const long m = n / 12, nblk = 16 * nslot;
+    if (m > ORC_LDLT_MAX_BODIES) return -2;
+    if (m == 0) return 1;
+    double* U = calloc((size_t)m * m * 144, 8); /* dense block storage, upper part used */
+    unsigned char* nz = calloc((size_t)m * m, 1);
+    for (long i = 0; i < m; ++i) nz[i * m + i] = 1;
+    for (long t = 0; t < nblk; ++t) {
+        int I = rows[t] / 4, J = cols[t] / 4, ii = rows[t] % 4, jj = cols[t] % 4;
+        if (I > J) { int q = I; I = J; J = q; }
+        nz[I * m + J] = 1;
+        if (rows[t] / 4 > cols[t] / 4) continue; /* values from the upper slots */
+        for (int u = 0; u < 3; ++u)
+            for (int v = 0; v < 3; ++v) U[((size_t)I * m + J) * 144 + 12 * (3 * ii + u) + 3 * jj + v] += blocks[9 * t + 3 * u + v];
+    }
+    /* fill: eliminating k couples every pair of its later neighbours */
+    for (long k = 0; k < m; ++k)
+        for (long i = k + 1; i < m; ++i)
+            if (nz[k * m + i])
+                for (long j = i; j < m; ++j)
+                    if (nz[k * m + j]) nz[i * m + j] = 1;
+    /* elimination tree levels and the elimination order (level, body) */
+    int* lev = calloc((size_t)m, sizeof(int));
+    int maxlev = 0;
+    for (long k = 0; k < m; ++k) {
+        for (long j = k + 1; j < m; ++j)
+            if (nz[k * m + j]) {
+                if (lev[j] < lev[k] + 1) lev[j] = lev[k] + 1;
+                break;
+            }
+        if (lev[k] > maxlev) maxlev = lev[k];
+    }
+    long* order = malloc(sizeof(long) * (size_t)m);
+    long no = 0;
+    for (int L = 0; L <= maxlev; ++L)
+        for (long k = 0; k < m; ++k)
+            if (lev[k] == L) order[no++] = k;
+    double* y = malloc(8 * (size_t)n);
+    double* z = malloc(8 * (size_t)n);
+    memcpy(y, b, 8 * (size_t)n);
+    double* W = malloc(8 * 144 * (size_t)m);
+    long* mem = malloc(sizeof(long) * (size_t)m);
+    double* aug = malloc(8 * 12 * (size_t)(12 * (m + 2)));
+    int ok = 1;
+    for (long o = 0; o < m && ok; ++o) {
+        const long k = order[o];
+        long d = 0;
+        for (long j = k + 1; j < m; ++j)
+            if (nz[k * m + j]) mem[d++] = j;
+        const long nc = 12 * (d + 2);
+        /* aug[r * nc + t]: P | row-k blocks | I */
+        for (int r = 0; r < 12; ++r)
+            for (long t = 0; t < nc; ++t) {
+                const long g = t / 12, c = t % 12;
+                double v;
+                if (g == 0) v = U[((size_t)k * m + k) * 144 + 12 * r + c];
+                else if (g <= d) v = U[((size_t)k * m + mem[g - 1]) * 144 + 12 * r + c];
+                else v = r == c ? 1.0 : 0.0;
+                aug[r * nc + t] = v;
+            }
+        for (int c = 0; c < 12 && ok; ++c) {
+            double f[12];
+            for (int r = 0; r < 12; ++r) f[r] = aug[r * nc + c];
+            const double piv = f[c];
+            if (!(piv > 0.0) || !isfinite(piv)) { ok = 0; break; }
+            const double inv = 1.0 / piv;
+            for (long t = 0; t < nc; ++t) {
+                const double sc = aug[c * nc + t] * inv;
+                for (int r = 0; r < 12; ++r)
+                    if (r != c) aug[r * nc + t] -= f[r] * sc;
+                aug[c * nc + t] = sc;
+            }
+        }
+        if (!ok) break;
+        for (long g = 0; g < d; ++g)
+            for (int r = 0; r < 12; ++r)
+                for (int c = 0; c < 12; ++c) W[144 * g + 12 * r + c] = aug[r * nc + 12 * (g + 1) + c];
+        double Dinv[144];
+        for (int r = 0; r < 12; ++r)
+            for (int c = 0; c < 12; ++c) Dinv[12 * r + c] = aug[r * nc + 12 * (d + 1) + c];
+        /* Schur complement of the later rows */
+        for (long u = 0; u < d; ++u)
+            for (long v = u; v < d; ++v) {
+                const double* Aki = U + ((size_t)k * m + mem[u]) * 144;
+                double* Uij = U + ((size_t)mem[u] * m + mem[v]) * 144;
+                for (int a = 0; a < 12; ++a)
+                    for (int c = 0; c < 12; ++c) {
+                        double acc = 0.0;
+                        for (int r = 0; r < 12; ++r) acc += Aki[12 * r + a] * W[144 * v + 12 * r + c];
+                        Uij[12 * a + c] -= acc;
+                    }
+            }
+        /* forward solve, step k */
+        for (int a = 0; a < 12; ++a) {
+            double acc = 0.0;
+            for (int r = 0; r < 12; ++r) acc += Dinv[12 * a + r] * y[12 * k + r];
+            z[12 * k + a] = acc;
+        }
+        for (long u = 0; u < d; ++u)
+            for (int a = 0; a < 12; ++a) {
+                double acc = 0.0;
+                for (int r = 0; r < 12; ++r) acc += W[144 * u + 12 * r + a] * y[12 * k + r];
+                y[12 * mem[u] + a] -= acc;
+            }
+        /* row k of the factor: W */
+        for (long u = 0; u < d; ++u) memcpy(U + ((size_t)k * m + mem[u]) * 144, W + 144 * u, 8 * 144);
+    }
+    if (ok) {
+        double* xs = y; /* y_k is consumed once z_k exists */
+        for (long o = m - 1; o >= 0; --o) {
+            const long k = order[o];
+            double acc[12] = {0};
+            for (long i = k + 1; i < m; ++i) {
+                if (!nz[k * m + i]) continue;
+                const double* Wi = U + ((size_t)k * m + i) * 144;
+                for (int a = 0; a < 12; ++a) {
+                    double p = 0.0;
+                    for (int c = 0; c < 12; ++c) p += Wi[12 * a + c] * xs[12 * i + c];
+                    acc[a] += p;
+                }
+            }
+            for (int a = 0; a < 12; ++a) xs[12 * k + a] = z[12 * k + a] - acc[a];
+        }
+        memcpy(x, xs, 8 * (size_t)n);
+    }
+    free(U); free(nz); free(y); free(z); free(W); free(mem); free(aug); free(lev); free(order);
+    return ok ? 1 : -2;
+}
+
 /* ------------------------------------------------------------------- CCD */
 #define X_ERROR 1e-10
 /* Iteration bound the reference does not have; never reached on the test

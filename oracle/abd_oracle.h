@@ -12,7 +12,7 @@
 typedef struct {
     double dt, dhat, d2hat, kappa, stiffness, gravity, ground, mass, I0, c1, alpha_min, tol;
     double solver_tol;
-    int32_t max_iter, solver_maxiter, pt, ee, vg, solver;  /* solver: 0 BiCGSTAB, 1 Jacobi CG */
+    int32_t max_iter, solver_maxiter, pt, ee, vg, solver;  /* solver: 0 BiCGSTAB, 1 Jacobi CG, 2 block-Jacobi CG, 3 block LDL^T */
     int32_t psd;  /* 0 reference contact Hessian; 1 contact-space PSD projection; 2 exact Hessian;
                      3 full IPC (closest feature pairs, EE mollifier, projected 12x12) —
                      1, 2 and 3 are beyond the reference */

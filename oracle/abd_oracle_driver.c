@@ -49,6 +49,8 @@ long orc_bsr_cg(long nslot, const int32_t* rows, const int32_t* cols, const doub
                 double* x, long n, double tol, long maxiter, int check_every, double* res);
 long orc_bsr_cg_block(long nslot, const int32_t* rows, const int32_t* cols, const double* blocks, const double* b,
                       double* x, long n, double tol, long maxiter, int check_every, double* res);
+long orc_bsr_ldlt(long nslot, const int32_t* rows, const int32_t* cols, const double* blocks, const double* b,
+                  double* x, long n);
 double orc_toi(const orc_scene* S, const orc_params* P, const double* x, const double* xk, const int32_t* pt, long npt,
                const int32_t* ee, long nee, const int32_t* vg, long nvg);
 void orc_update_q(int nb, const double* A, const double* p, const double* dq, double alpha, double* Aout, double* pout);
@@ -212,7 +214,12 @@ int orc_step(const orc_scene* S, const orc_params* P, const int32_t* body_scene,
         orc_set_triplets(s.nb, nij, s.ij.d, rows, cols);
         double res;
         long sit;
-        if (P->solver == 2)
+        if (P->solver == 3) {
+            /* direct block LDL^T; the Jacobi CG where it does not apply */
+            sit = orc_bsr_ldlt(nslot, rows, cols, blocks, g, dq, n12);
+            if (sit < 0)
+                sit = orc_bsr_cg(nslot, rows, cols, blocks, g, dq, n12, P->solver_tol, P->solver_maxiter, 10, &res);
+        } else if (P->solver == 2)
             sit = orc_bsr_cg_block(nslot, rows, cols, blocks, g, dq, n12, P->solver_tol, P->solver_maxiter, 10, &res);
         else if (P->solver == 1)
             sit = orc_bsr_cg(nslot, rows, cols, blocks, g, dq, n12, P->solver_tol, P->solver_maxiter, 10, &res);

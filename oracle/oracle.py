@@ -82,6 +82,8 @@ def lib():
         L.orc_bsr_cg.argtypes = [l, p, p, p, p, p, l, d, l, i, p]
         L.orc_bsr_cg_block.restype = l
         L.orc_bsr_cg_block.argtypes = [l, p, p, p, p, p, l, d, l, i, p]
+        L.orc_bsr_ldlt.restype = l
+        L.orc_bsr_ldlt.argtypes = [l, p, p, p, p, p, l]
         L.orc_cubic_roots.restype = i
         L.orc_cubic_roots.argtypes = [p, d, d, p]
         L.orc_coplanar_poly.restype = i
@@ -360,6 +362,17 @@ def bsr_cg_block(nb: int, ij, blocks, b, x, tol, maxiter=0, check_every=10):
     it = lib().orc_bsr_cg_block(blocks.shape[0] // 16, _ptr(rows), _ptr(cols), _ptr(blocks), _ptr(b), _ptr(x),
                                 b.size, float(tol), int(maxiter), int(check_every), _ptr(res))
     return x, int(it), float(res[0])
+
+
+def bsr_ldlt(nb: int, ij, blocks, b, x):
+    """Direct block LDL^T solve (beyond the reference); returns (x, status): status 1, or -2
+    with x unchanged when a pivot is not positive or the system has more than 64 bodies."""
+    rows, cols = set_triplets(nb, ij)
+    blocks = _c(blocks, np.float64)
+    b = _c(b, np.float64)
+    x = np.array(x, np.float64, copy=True)
+    st = lib().orc_bsr_ldlt(blocks.shape[0] // 16, _ptr(rows), _ptr(cols), _ptr(blocks), _ptr(b), _ptr(x), b.size)
+    return x, int(st)
 
 
 def cubic_roots(coef, x0=0.0, x1=1.0):

@@ -280,6 +280,86 @@ def test_solvers_against_numpy(which):
     assert np.linalg.norm(r) <= 1e-3 * np.linalg.norm(b)
 
 
+def _chain_system(rng, nb, stiff):
+    """Bodies in a chain (i, i+1), each link a stiff PSD coupling: ill-conditioned like a
+    compressed stack.  Returns (ij, blocks, dense)."""
+    ij = np.array([(i, i + 1) for i in range(nb - 1)], np.int32)
+    nij = len(ij)
+    blocks = np.zeros(((nb + 2 * nij) * 16, 3, 3))
+    dense = np.zeros((12 * nb, 12 * nb))
+
+    def put(slot, M):
+        for ii in range(4):
+            for jj in range(4):
+                blocks[16 * slot + ii + 4 * jj] += M[3 * ii:3 * ii + 3, 3 * jj:3 * jj + 3]
+    Dg = [np.eye(12) * 10.0 for _ in range(nb)]
+    for k, (i, j) in enumerate(ij):
+        v = rng.normal(size=(12, 3))
+        K = stiff * (v @ v.T)  # the spring of link k between the two bodies' dofs
+        Dg[i] += K
+        Dg[j] += K
+        put(nb + k, -K)
+        put(nb + nij + k, -K.T)
+        dense[12 * i:12 * i + 12, 12 * j:12 * j + 12] -= K
+        dense[12 * j:12 * j + 12, 12 * i:12 * i + 12] -= K.T
+    for b in range(nb):
+        put(b, Dg[b])
+        dense[12 * b:12 * b + 12, 12 * b:12 * b + 12] += Dg[b]
+    return ij, blocks, dense
+
+
+@pytest.mark.parametrize("nb,npair", [(1, 0), (2, 1), (12, 20), (40, 120), (64, 200)])
+def test_ldlt_against_numpy(nb, npair):
+    """The direct block LDL^T (beyond the reference) against numpy's dense solve, on random
+    couplings (fill-in included) of up to the 64 bodies it takes."""
+    rng = np.random.default_rng(nb)
+    ij, blocks, dense = _system(rng, nb, npair) if nb > 1 else _system(rng, 2, 1)
+    if nb == 1:
+        ij, nb = np.zeros((0, 2), np.int32), 1
+        blocks = blocks[:16].copy()
+        dense = dense[:12, :12].copy()
+    b = rng.normal(size=(4 * nb, 3))
+    x, st = O.bsr_ldlt(nb, ij, blocks, b, np.zeros_like(b))
+    assert st == 1
+    xs = np.linalg.solve(dense, b.reshape(-1)).reshape(-1, 3)
+    assert rel_err(x, xs) <= 1e-10
+
+
+def test_ldlt_on_a_stiff_chain():
+    """A chain of stiff links: the block-Jacobi CG needs about a hundred iterations, the
+    factorization is exact to rounding (a chain has no fill-in)."""
+    rng = np.random.default_rng(3)
+    nb = 48
+    ij, blocks, dense = _chain_system(rng, nb, 1e6)
+    b = rng.normal(size=(4 * nb, 3))
+    x, st = O.bsr_ldlt(nb, ij, blocks, b, np.zeros_like(b))
+    assert st == 1
+    xs = np.linalg.solve(dense, b.reshape(-1))
+    r = b.reshape(-1) - dense @ x.reshape(-1)
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(dense, 2) * np.linalg.norm(xs)
+    assert rel_err(x.reshape(-1), xs) <= 1e-6
+    _, it, _ = O.bsr_cg_block(nb, ij, blocks, b, np.zeros_like(b), 1e-10)
+    assert it >= 90
+
+
+def test_ldlt_declines_what_it_cannot_factor():
+    """A non-positive pivot (indefinite diagonal block) and a system above 64 bodies give
+    status -2 with x untouched."""
+    rng = np.random.default_rng(4)
+    nb = 6
+    ij, blocks, dense = _system(rng, nb, 8)
+    blocks[16 * 2:16 * 3] *= -1.0  # body 2's diagonal block negative definite
+    b = rng.normal(size=(4 * nb, 3))
+    x0 = rng.normal(size=b.shape)
+    x, st = O.bsr_ldlt(nb, ij, blocks, b, x0)
+    assert st == -2 and np.array_equal(x, x0)
+    nb = 65
+    ij, blocks, dense = _chain_system(rng, nb, 1.0)
+    b = rng.normal(size=(4 * nb, 3))
+    x, st = O.bsr_ldlt(nb, ij, blocks, b, np.zeros_like(b))
+    assert st == -2 and not x.any()
+
+
 def test_block_jacobi_cg_is_preconditioned_cg():
     """The block-Jacobi CG (beyond the reference) equals a numpy restatement of PCG with
     M = blockdiag(12x12 diagonal blocks), iterate for iterate, and takes fewer iterations
