@@ -388,67 +388,111 @@ constexpr int kPairBatch = 8;
 constexpr int kPairBufStride = 2 * 144 + 24;  // diag(ij0), diag(ij1), g(ij0), g(ij1)
 constexpr int64_t kPairGrid = 2048;           // workgroups of stage D (grid-stride over pairs)
 
-// stage D: per body pair ii (grid-stride).  DENSE: contacts stored by contact_terms_exact.
+constexpr int kPairChunk = 64;  // contacts of one stage-D work item
+constexpr int kPartStride = 600;  // one chunk's partial sums: the 24x24 block, then g (24)
+
+// stage D1: the work items of each body pair: its PT run and its EE run cut into chunks of
+// kPairChunk contacts (runs from stage C; empty runs are [0, 0))
+__global__ __launch_bounds__(kBlock) void pair_chunks_kernel(int64_t nij, const int64_t* __restrict__ pseg,
+                                                             int64_t* __restrict__ cnt) {
+    for (int64_t ii = (int64_t)blockIdx.x * kBlock + threadIdx.x; ii < nij; ii += (int64_t)gridDim.x * kBlock) {
+        const int64_t npt = pseg[2 * ii + 1] - pseg[2 * ii], nee = pseg[2 * (nij + ii) + 1] - pseg[2 * (nij + ii)];
+        cnt[ii] = (npt + kPairChunk - 1) / kPairChunk + (nee + kPairChunk - 1) / kPairChunk;
+    }
+}
+
+// stage D2: one work item per workgroup (grid-stride over the *total items): the sums, in
+// contact order, of one chunk's contacts into part[item] (24x24 block in the pair's body
+// order [ij0 | ij1], then g).  DENSE: contacts stored by contact_terms_exact / _full.
 template <bool DENSE>
-__global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64_t nij,
-                                                                    const double* __restrict__ cbuf,
-                                                                    const int64_t* __restrict__ ckey,
-                                                                    const uint8_t* __restrict__ swap, int64_t nact,
-                                                                    const uint8_t* __restrict__ pflag,
-                                                                    const int64_t* __restrict__ pseg,
-                                                                    double* __restrict__ pbuf,
-                                                                    double* __restrict__ blocks) {
+__global__ __launch_bounds__(kPairThreads) void pair_partial_kernel(int64_t nij, const double* __restrict__ cbuf,
+                                                                     const uint8_t* __restrict__ swap,
+                                                                     const int64_t* __restrict__ pseg,
+                                                                     const int64_t* __restrict__ coff,
+                                                                     const int64_t* __restrict__ total,
+                                                                     double* __restrict__ part) {
     constexpr int kStride = DENSE ? kDenseStride : kContactStride;
     constexpr int kBatch = DENSE ? 4 : kPairBatch;
     __shared__ double sc[kBatch][kStride];
     __shared__ uint8_t ssw[kBatch];
     const int t = threadIdx.x;
     const int R = t / 24, C = t % 24;
-    // body pairs in a grid-stride loop: most pairs have no active contact and are skipped
-    // for the price of one flag read (stage E skips them too)
+    const int64_t nitem = *total;
+    for (int64_t gc = blockIdx.x; gc < nitem; gc += gridDim.x) {
+        // the pair: coff[ii] <= gc < coff[ii + 1] (pairs without items share their offset)
+        int64_t lo = 0, hi = nij - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (coff[mid] <= gc) lo = mid;
+            else hi = mid - 1;
+        }
+        const int64_t ii = lo;
+        int64_t lc = gc - coff[ii];
+        const int64_t* sp = pseg + 2 * ii;
+        const int64_t npc = (sp[1] - sp[0] + kPairChunk - 1) / kPairChunk;
+        if (lc >= npc) {  // an EE chunk
+            lc -= npc;
+            sp = pseg + 2 * (nij + ii);
+        }
+        const int64_t c0 = sp[0] + lc * kPairChunk;
+        const int64_t c1 = sp[1] < c0 + kPairChunk ? sp[1] : c0 + kPairChunk;
+        double h = 0.0, gacc = 0.0;
+        for (int64_t e0 = c0; e0 < c1; e0 += kBatch) {
+            const int nbat = (int)((c1 - e0) < kBatch ? (c1 - e0) : kBatch);
+            for (int q = t; q < nbat * kStride; q += kPairThreads)
+                sc[q / kStride][q % kStride] = cbuf[(e0 + q / kStride) * kStride + q % kStride];
+            if (t < nbat) ssw[t] = swap[e0 + t];
+            __syncthreads();
+            for (int bq = 0; bq < nbat; ++bq) {
+                // body order [ij0 | ij1]: swap halves when body I is ij1
+                const int sw = ssw[bq] ? 12 : 0;
+                const int r = (R + sw) % 24, cc = (C + sw) % 24;
+                if (DENSE) {
+                    h += sc[bq][24 * r + cc];
+                    if (t < 24) gacc += sc[bq][kDenseGOff + (t + sw) % 24];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) {
+                        const double* V = &sc[bq][kVOff + 24 * k];
+                        h += sc[bq][kCoefOff + k] * (V[r] * V[cc]);
+                    }
+                    if (t < 24) gacc += sc[bq][kB1Off] * sc[bq][kVOff + 24 * 4 + (t + sw) % 24];
+                }
+            }
+            __syncthreads();
+        }
+        double* pp = part + gc * kPartStride;
+        pp[t] = h;
+        if (t < 24) pp[576 + t] = gacc;
+    }
+}
+
+// stage D3: per body pair with active contacts (grid-stride): its chunks' partial sums
+// added in order (PT chunks, then EE), diagonal parts to the pair buffer (stage E adds
+// them per body), the off-diagonal block to the matrix.
+__global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64_t nij,
+                                                                    const uint8_t* __restrict__ pflag,
+                                                                    const int64_t* __restrict__ coff,
+                                                                    const int64_t* __restrict__ cnt,
+                                                                    const double* __restrict__ part,
+                                                                    double* __restrict__ pbuf,
+                                                                    double* __restrict__ blocks) {
+    const int t = threadIdx.x;
+    const int R = t / 24, C = t % 24;
     for (int64_t ii = blockIdx.x; ii < nij; ii += gridDim.x) {
         if (!pflag[ii]) continue;  // uniform over the workgroup
-        int64_t seg[2][2];
-        for (int typ = 0; typ < 2; ++typ) {
-            seg[typ][0] = pseg[2 * (typ * nij + ii)];
-            seg[typ][1] = pseg[2 * (typ * nij + ii) + 1];
-        }
+        const int64_t c0 = coff[ii], nc = cnt[ii];
         double h = 0.0, gacc = 0.0;
-        bool any = false;
-        for (int typ = 0; typ < 2; ++typ) {
-            for (int64_t e0 = seg[typ][0]; e0 < seg[typ][1]; e0 += kBatch) {
-                int nbat = (int)((seg[typ][1] - e0) < kBatch ? (seg[typ][1] - e0) : kBatch);
-                for (int q = t; q < nbat * kStride; q += kPairThreads)
-                    sc[q / kStride][q % kStride] = cbuf[(e0 + q / kStride) * kStride + q % kStride];
-                if (t < nbat) ssw[t] = swap[e0 + t];
-                __syncthreads();
-                for (int b = 0; b < nbat; ++b) {
-                    // body order [ij0 | ij1]: swap halves when body I is ij1
-                    const int sw = ssw[b] ? 12 : 0;
-                    const int r = (R + sw) % 24, cc = (C + sw) % 24;
-                    if (DENSE) {
-                        h += sc[b][24 * r + cc];
-                        if (t < 24) gacc += sc[b][kDenseGOff + (t + sw) % 24];
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < 5; ++k) {
-                            const double* V = &sc[b][kVOff + 24 * k];
-                            h += sc[b][kCoefOff + k] * (V[r] * V[cc]);
-                        }
-                        if (t < 24) gacc += sc[b][kB1Off] * sc[b][kVOff + 24 * 4 + (t + sw) % 24];
-                    }
-                }
-                any = true;
-                __syncthreads();
-            }
+        for (int64_t c = c0; c < c0 + nc; ++c) {
+            h += part[c * kPartStride + t];
+            if (t < 24) gacc += part[c * kPartStride + 576 + t];
         }
         double* pb = pbuf + ii * kPairBufStride;
-        // diagonal parts to the pair buffer (stage E adds them per body)
         if (R < 12 && C < 12) pb[R * 12 + C] = h;
         if (R >= 12 && C >= 12) pb[144 + (R - 12) * 12 + (C - 12)] = h;
         if (t < 24) pb[288 + t] = gacc;
         // off-diagonal: upper slot nb + ii holds rows ij0 x cols ij1, lower slot its transpose
-        if (any && R < 12 && C >= 12) {
+        if (R < 12 && C >= 12) {
             const int c2 = C - 12;
             const int bi = R / 3, rr = R % 3, bj = c2 / 3, cc = c2 % 3;
             const int64_t up = nb + ii, lo = nb + nij + ii;
@@ -756,17 +800,34 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
                            npt, ee, (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s);
+        // stage D: chunks of each pair's runs (count, scan), their partial sums, the per-pair sums
+        int64_t* pcnt;
+        if ((rc = arena_get(kSlotPairChunk, sizeof(int64_t) * (2 * (size_t)nij + 2), (void**)&pcnt))) return rc;
+        int64_t* pcoff = pcnt + nij;
+        int64_t* ptotal = pcoff + nij;
+        // items: at most one per kPairChunk contacts plus one per run (a run per pair and type)
+        const int64_t runs = 2 * nij < nact ? 2 * nij : nact;
+        const int64_t cap_items = nact / kPairChunk + runs + 1;
+        double* part;
+        if ((rc = arena_get(kSlotPairPart, sizeof(double) * kPartStride * (size_t)cap_items, (void**)&part)))
+            return rc;
         ph = prof_start(kProfPairReduce, s, (8.0 * stride + 9) * nact + 8.0 * kPairBufStride * nij);
+        hipLaunchKernelGGL(pair_chunks_kernel, dim3(grid_for(nij)), dim3(kBlock), 0, s, nij, (const int64_t*)pseg,
+                           pcnt);
+        ABD_CHECK_LAUNCH();
+        if ((rc = scan_exclusive_i64(s, pcnt, pcoff, nij, ptotal))) return rc;
         if (dense)
-            hipLaunchKernelGGL(pair_reduce_kernel<true>, dim3(grid_for(nij, 1, kPairGrid)), dim3(kPairThreads), 0, s,
-                               nb, nij,
-                               (const double*)cbuf, (const int64_t*)ckey, (const uint8_t*)swp, nact,
-                               (const uint8_t*)pflag, (const int64_t*)pseg, pbuf, blocks);
+            hipLaunchKernelGGL(pair_partial_kernel<true>, dim3(kPairGrid), dim3(kPairThreads), 0, s, nij,
+                               (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg,
+                               (const int64_t*)pcoff, (const int64_t*)ptotal, part);
         else
-            hipLaunchKernelGGL(pair_reduce_kernel<false>, dim3(grid_for(nij, 1, kPairGrid)), dim3(kPairThreads), 0, s,
-                               nb, nij,
-                               (const double*)cbuf, (const int64_t*)ckey, (const uint8_t*)swp, nact,
-                               (const uint8_t*)pflag, (const int64_t*)pseg, pbuf, blocks);
+            hipLaunchKernelGGL(pair_partial_kernel<false>, dim3(kPairGrid), dim3(kPairThreads), 0, s, nij,
+                               (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg,
+                               (const int64_t*)pcoff, (const int64_t*)ptotal, part);
+        ABD_CHECK_LAUNCH();
+        hipLaunchKernelGGL(pair_reduce_kernel, dim3(grid_for(nij, 1, kPairGrid)), dim3(kPairThreads), 0, s, nb, nij,
+                           (const uint8_t*)pflag, (const int64_t*)pcoff, (const int64_t*)pcnt, (const double*)part,
+                           pbuf, blocks);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s);
     }

@@ -22,7 +22,7 @@ inline hipStream_t to_stream(abd_stream_t s) { return reinterpret_cast<hipStream
 // Per-device scratch arena: `slots` independent growable device buffers plus a
 // small pinned host buffer for count read-back.  Growing synchronizes the
 // device first, so a buffer still in use by queued work is never freed early.
-constexpr int kArenaSlots = 35;
+constexpr int kArenaSlots = 37;
 enum ArenaSlot {
     kSlotScanA = 0,
     kSlotScanB,
@@ -60,6 +60,8 @@ enum ArenaSlot {
     kSlotKeepCount,
     kSlotLdlt,        // direct solve: the factors of the systems
     kSlotLdltStatus,  // direct solve: per-system status when the caller takes no iteration counts
+    kSlotPairChunk,   // contact stage D: per-pair work-item counts and offsets
+    kSlotPairPart,    // contact stage D: the work items' partial sums
 };
 // Launch shapes that change no result (abd_set_tuning); process-wide.
 AbdTuning tuning();
