@@ -288,6 +288,46 @@ int update_q_scenes_dual(hipStream_t s, const AbdScene* sc, const double* A, con
     return ABD_OK;
 }
 
+__global__ __launch_bounds__(kBlock) void vertex_ranges_kernel(int64_t n3, const double* __restrict__ x0,
+                                                                const double* __restrict__ x1, double skin,
+                                                                double* __restrict__ lo, double* __restrict__ hi) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock) {
+        const double a = x0[i], b = x1[i];
+        lo[i] = (a < b ? a : b) - skin;
+        hi[i] = (a > b ? a : b) + skin;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void within_ranges_kernel(int64_t n3, const double* __restrict__ x0,
+                                                                const double* __restrict__ x1,
+                                                                const double* __restrict__ lo,
+                                                                const double* __restrict__ hi,
+                                                                int64_t* __restrict__ flag) {
+    bool out = false;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock) {
+        const double a = x0[i], b = x1[i], l = lo[i], h = hi[i];
+        out |= !(a >= l && a <= h && b >= l && b <= h);  // NaN counts as outside
+    }
+    if (__any(out) && lane_id() == 0) *flag = 1;  // every writer stores 1
+}
+
+int vertex_ranges(hipStream_t s, int64_t n3, const double* x0, const double* x1, double skin, double* lo,
+                  double* hi) {
+    if (n3 <= 0) return ABD_OK;
+    hipLaunchKernelGGL(vertex_ranges_kernel, dim3(grid_for(n3)), dim3(kBlock), 0, s, n3, x0, x1, skin, lo, hi);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+int within_ranges(hipStream_t s, int64_t n3, const double* x0, const double* x1, const double* lo, const double* hi,
+                  int64_t* flag) {
+    if (n3 <= 0) return ABD_OK;
+    hipLaunchKernelGGL(within_ranges_kernel, dim3(grid_for(n3, kBlock, 4096)), dim3(kBlock), 0, s, n3, x0, x1, lo,
+                       hi, flag);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
 int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,
                 float* lo, float* hi) {
     if (kind == ABD_KIND_POINT) {

@@ -111,6 +111,15 @@ int vg_cull_fill(hipStream_t s, VgCull& C, int32_t* out, int64_t cap, int64_t n)
 // the dilated boxes at x0 and at x1.
 int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,
                 float* lo, float* hi);
+// Vertex ranges of a candidate superset (n3 = 3 * vertices): lo = min(x0, x1) - skin,
+// hi = max(x0, x1) + skin per coordinate.  Boxes of any positions inside these ranges
+// lie inside the swept boxes of (lo, hi) (every box formula is monotone in the
+// positions), so their culling output is a subset of that superset's.
+int vertex_ranges(hipStream_t s, int64_t n3, const double* x0, const double* x1, double skin, double* lo,
+                  double* hi);
+// *flag (device, 0 before) becomes 1 when some coordinate of x0 or x1 leaves [lo, hi].
+int within_ranges(hipStream_t s, int64_t n3, const double* x0, const double* x1, const double* lo, const double* hi,
+                  int64_t* flag);
 // K line-search trials q - alpha dq at once (alpha [K][S] on the device, S = 1: the
 // whole system as one scene; a negative alpha skips that scene's trial and gives 0):
 // per-scene contact energies to E[2 S k + s] and inertia energies to E[2 S k + S + s].

@@ -82,6 +82,13 @@ struct Sim {
     // (CCD, line search; the next iteration's collision set is filtered from it)
     CandSet col, sw;
     bool sw_valid = false;  // sw covers the current state (set by the CCD, cleared per frame)
+    // candidate superset: the culling of the swept boxes of vertex ranges (a CCD step's two
+    // ends widened by a skin); every later step whose two ends stay inside the ranges
+    // takes its swept set from it by filtering instead of culling
+    CandSet sup;
+    BoxBuf sbody, stri, sedge, spt;  // the superset's boxes
+    DevBuf<double> vlo, vhi;         // its vertex ranges
+    bool sup_valid = false;
     DevBuf<int32_t> row_ptr, cols, slots;
     DevBuf<double> blocks;
     DevBuf<double> scal;  // [0] E contact, [1] E inertia, [2] toi, [3] dot, [4] resid
@@ -246,6 +253,9 @@ int collision_from_swept(hipStream_t s, Sim& S, const double* xv) {
     return structure_of(s, S);
 }
 
+// Skin of the candidate superset's vertex ranges, in units of dhat.
+constexpr double kSkinOverDhat = 2.0;
+
 // abd.py:170-185, with both ends of every box dilated as in collision_set: one candidate
 // set per Newton iteration that holds both the CCD candidates of the step (a pair that
 // collides at some t has overlapping boxes at t, inside these) and every candidate of a
@@ -254,17 +264,60 @@ int collision_from_swept(hipStream_t s, Sim& S, const double* xv) {
 // verified root), and the line search's energies need no re-culling: a pair missing from
 // the collision set at x - alpha dq has boxes apart, hence distance >= dhat and a zero
 // barrier, so only the order of the nonzero terms' partial sums changes.
-int swept_set(hipStream_t s, Sim& S) {
+// The set is the culling output of the step's swept boxes.  With `superset` (one scene),
+// it is taken by filtering the candidate superset (keep_from_superset: the same
+// predicates) while both ends of the step lie inside the superset's vertex ranges, and
+// the superset is culled anew (from the ranges of this step's ends, widened by the skin)
+// when they do not.  Without (independent scenes: some scene nearly always leaves its
+// ranges), the swept boxes are culled directly.
+int swept_set(hipStream_t s, Sim& S, bool superset) {
     const double dh = S.P.dhat * 0.5;
     const AbdSimState& st = S.st;
+    const int64_t n3 = 3 * (int64_t)S.nv;
     int rc;
     if ((rc = swept_boxes(s, &S.sc, -1, st.x, st.xk, dh, S.tbody.lo.p, S.tbody.hi.p))) return rc;
-    // with the (j, i) order: the next iteration's collision set is filtered from this set
-    if ((rc = body_pairs(s, S, S.sw, S.tbody, true))) return rc;
+    if (!superset) {
+        S.sup_valid = false;
+        // with the (j, i) order: the next iteration's collision set is filtered from this set
+        if ((rc = body_pairs(s, S, S.sw, S.tbody, true))) return rc;
+    }
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, st.x, st.xk, 0.0, S.ttri.lo.p, S.ttri.hi.p))) return rc;
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, st.x, st.xk, dh, S.tedge.lo.p, S.tedge.hi.p))) return rc;
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, st.x, st.xk, S.P.dhat, S.tpt.lo.p, S.tpt.hi.p))) return rc;
-    return cull_lists(s, S, S.sw, S.tbody, S.tpt, S.ttri, S.tedge);
+    if (!superset) return cull_lists(s, S, S.sw, S.tbody, S.tpt, S.ttri, S.tedge);
+    bool reuse = false;
+    if (S.sup_valid) {
+        ABD_CHECK(hipMemsetAsync(S.dcount.p + 7, 0, sizeof(int64_t), s));
+        if ((rc = within_ranges(s, n3, st.x, st.xk, S.vlo.p, S.vhi.p, S.dcount.p + 7))) return rc;
+        ABD_CHECK(hipMemcpyAsync(S.hcount + 7, S.dcount.p + 7, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipStreamSynchronize(s));
+        reuse = S.hcount[7] == 0;
+    }
+    if (!reuse) {
+        S.sup_valid = false;
+        if ((rc = vertex_ranges(s, n3, st.x, st.xk, kSkinOverDhat * S.P.dhat, S.vlo.p, S.vhi.p))) return rc;
+        if ((rc = swept_boxes(s, &S.sc, -1, S.vlo.p, S.vhi.p, dh, S.sbody.lo.p, S.sbody.hi.p))) return rc;
+        // with the (j, i) order: the swept set (and the collision sets after it) keep it
+        if ((rc = body_pairs(s, S, S.sup, S.sbody, true))) return rc;
+        if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, S.vlo.p, S.vhi.p, 0.0, S.stri.lo.p, S.stri.hi.p))) return rc;
+        if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, S.vlo.p, S.vhi.p, dh, S.sedge.lo.p, S.sedge.hi.p))) return rc;
+        if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, S.vlo.p, S.vhi.p, S.P.dhat, S.spt.lo.p, S.spt.hi.p)))
+            return rc;
+        if ((rc = cull_lists(s, S, S.sup, S.sbody, S.spt, S.stri, S.sedge))) return rc;
+        S.sup_valid = true;
+    }
+    if ((rc = S.sw.reserve_like(S.sup))) return rc;
+    const CandSet& W = S.sup;
+    const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg};
+    const KeepBoxes bx{S.tbody.lo.p, S.tbody.hi.p, S.tpt.lo.p, S.tpt.hi.p, S.tedge.lo.p, S.tedge.hi.p, S.ttri.lo.p,
+                       S.ttri.hi.p};
+    const KeepOutput out{S.sw.ij.p, S.sw.ji.p, S.sw.ptl.p, S.sw.eel.p, S.sw.vgl.p};
+    if ((rc = keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, out, S.dcount.p, S.hcount))) return rc;
+    S.sw.nij = S.hcount[0];
+    S.sw.npt = S.hcount[2];
+    S.sw.nee = S.hcount[3];
+    S.sw.nvg = S.hcount[4];
+    return ABD_OK;
 }
 
 // abd.py:133-141
@@ -318,7 +371,7 @@ int ccd(hipStream_t s, Sim& S, double* alpha_cap) {
     // x = x(A, p) from the start of the iteration; xk: the end of the full step
     if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, 1.0, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-    if ((rc = swept_set(s, S))) return rc;
+    if ((rc = swept_set(s, S, true))) return rc;
     S.sw_valid = true;
     S.stats[8] += S.sw.npt;
     S.stats[9] += S.sw.nee;
@@ -518,7 +571,7 @@ int step_scenes(hipStream_t s, Sim& S) {
         if ((rc = upload(s, dccd, S.hccd, sizeof(double) * ns))) return rc;
         if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dccd, st.Ak, st.pk))) return rc;
         if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-        if ((rc = swept_set(s, S))) return rc;
+        if ((rc = swept_set(s, S, false))) return rc;
         S.sw_valid = true;
         S.stats[8] += S.sw.npt;
         S.stats[9] += S.sw.nee;
@@ -640,7 +693,13 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
     if (!rc) rc = S->tedge.reserve(S->ne + 1);
     if (!rc) rc = S->pt.reserve(S->nv);
     if (!rc) rc = S->tpt.reserve(S->nv);
-    for (CandSet* C : {&S->col, &S->sw}) {
+    if (!rc) rc = S->sbody.reserve(nb);
+    if (!rc) rc = S->stri.reserve(S->nf + 1);
+    if (!rc) rc = S->sedge.reserve(S->ne + 1);
+    if (!rc) rc = S->spt.reserve(S->nv);
+    if (!rc) rc = S->vlo.reserve(3 * (size_t)S->nv + 3);
+    if (!rc) rc = S->vhi.reserve(3 * (size_t)S->nv + 3);
+    for (CandSet* C : {&S->col, &S->sw, &S->sup}) {
         if (!rc) rc = C->ij.reserve(2 * 256);
         if (!rc) rc = C->ji.reserve(2 * 256);
         if (!rc) rc = C->ptl.reserve(5 * 4096);
@@ -796,12 +855,16 @@ int abd_sim_destroy(void* sim) {
     if (!sim) return ABD_OK;
     Sim* S = static_cast<Sim*>(sim);
     (void)hipDeviceSynchronize();  // queued work may still use the buffers released below
-    for (BoxBuf* b : {&S->body, &S->tri, &S->edge, &S->pt, &S->tbody, &S->ttri, &S->tedge, &S->tpt}) {
+    for (BoxBuf* b : {&S->body, &S->tri, &S->edge, &S->pt, &S->tbody, &S->ttri, &S->tedge, &S->tpt, &S->sbody,
+                      &S->stri, &S->sedge, &S->spt}) {
         b->lo.release();
         b->hi.release();
     }
     S->col.release();
     S->sw.release();
+    S->sup.release();
+    S->vlo.release();
+    S->vhi.release();
     S->row_ptr.release();
     S->cols.release();
     S->slots.release();
