@@ -30,7 +30,8 @@ __device__ __forceinline__ bool overlap(const float* __restrict__ alo, const flo
 }
 
 // ---------------------------------------------------------------- bodies
-constexpr int kItems = 8;  // pairs per thread in the body-pair compaction
+constexpr int kItems = 8;      // entries per thread in the vertex-ground and keep compactions
+constexpr int kPairItems = 1;  // pairs per thread in the body-pair compaction: enough workgroups to fill the device
 
 // k -> (i, j), i < j.  Order 0: row-major (i, j); order 1: sorted by (j, i).
 __device__ __forceinline__ void pair_of(int64_t k, int nb, int order, int& i, int& j) {
@@ -94,9 +95,9 @@ __global__ __launch_bounds__(kBlock) void body_pairs_kernel(int nb, int64_t npai
                                                              const int64_t* __restrict__ boff, int32_t* out,
                                                              int64_t cap) {
     __shared__ int shw[kBlock / kWave];
-    int64_t base = (int64_t)blockIdx.x * kBlock * kItems;
+    int64_t base = (int64_t)blockIdx.x * kBlock * kPairItems;
     int64_t run = WRITE ? boff[blockIdx.x] : 0;
-    for (int it = 0; it < kItems; ++it) {
+    for (int it = 0; it < kPairItems; ++it) {
         int64_t k = base + (int64_t)it * kBlock + threadIdx.x;
         bool pred = false;
         int i = 0, j = 0;
@@ -535,7 +536,7 @@ int cull_bodies(hipStream_t s, const AbdScene* sc, const float* lo, const float*
     }
     *count = 0;
     if (npairs <= 0) return ABD_OK;
-    const int64_t nblk = ceil_div(npairs, (int64_t)kBlock * kItems);
+    const int64_t nblk = ceil_div(npairs, (int64_t)kBlock * kPairItems);
     if (nblk > INT32_MAX) return ABD_ERR_ARG;
     int64_t *bcnt, *boff;
     int rc = arena_get(kSlotScanA, sizeof(int64_t) * (nblk + 1), (void**)&bcnt);
