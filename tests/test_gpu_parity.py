@@ -126,6 +126,37 @@ def test_boxes_and_culling_bit_exact(lib, which):
     assert len(R["ij"]) > 0
 
 
+def test_cull_overflow_reports_the_required_count(lib):
+    """Defect D2 (lists capped at 256 with no bound check, fenwick.py:127-136,165-166): a
+    list that does not fit its capacity returns ABD_ERR_OVERFLOW with the count it needs and
+    writes nothing past the capacity; a second call at that capacity gives the full list."""
+    import ctypes
+    from abd_hip import _lib
+    spec = _scenes()[1]
+    cp = params(ground=spec.ground)
+    m = _gpu_scene(spec)
+    st = m.states()
+    _lib.call("abd_project_vertices", _lib.stream_handle(), m.c, _lib.ptr(st.A), _lib.ptr(st.p), _lib.ptr(m.xk))
+    G = _stage_lists(m, cp)
+    full = N(G["pt"])
+    assert len(full) > 8
+    ijt = G["pairs"].ij
+    out = torch.full((len(full) + 4, 5), -7, dtype=torch.int32, device=DEV)
+    cnt = ctypes.c_int64(0)
+    cap = len(full) // 2
+    rc = _lib.call("abd_cull_prims", _lib.stream_handle(), m.c, _lib.CULL_PT, _lib.ptr(ijt), int(ijt.shape[0]),
+                   _lib.ptr(G["pts"].lowers), _lib.ptr(G["pts"].uppers), _lib.ptr(G["tri"].lowers),
+                   _lib.ptr(G["tri"].uppers), _lib.ptr(out), cap, ctypes.byref(cnt),
+                   ok=(_lib.ABD_OK, _lib.ABD_ERR_OVERFLOW))
+    assert rc == _lib.ABD_ERR_OVERFLOW and cnt.value == len(full)
+    assert (N(out)[cap:] == -7).all()  # nothing written past the capacity
+    rc = _lib.call("abd_cull_prims", _lib.stream_handle(), m.c, _lib.CULL_PT, _lib.ptr(ijt), int(ijt.shape[0]),
+                   _lib.ptr(G["pts"].lowers), _lib.ptr(G["pts"].uppers), _lib.ptr(G["tri"].lowers),
+                   _lib.ptr(G["tri"].uppers), _lib.ptr(out), cnt.value, ctypes.byref(cnt))
+    assert rc == _lib.ABD_OK
+    np.testing.assert_array_equal(N(out)[:len(full)], full)
+
+
 def test_trajectory_boxes_bit_exact(lib):
     from abd_hip import _lib
     from abd_hip.culling import BvhBuilder

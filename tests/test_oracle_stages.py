@@ -360,6 +360,25 @@ def test_ldlt_declines_what_it_cannot_factor():
     assert st == -2 and not x.any()
 
 
+def test_cg_guard_when_the_residual_vanishes_between_checks():
+    """Defect D16 (bsr_utils.py:70 divides rz_old / pAp unguarded): with a diagonal matrix the
+    Jacobi preconditioner is its exact inverse, the residual is exactly 0 after the first
+    iteration, and the reference's next step would be 0/0.  The guard skips the step: the
+    result is the exact solution, finite, stopping at the first check."""
+    rng = np.random.default_rng(12)
+    nb = 3
+    ij = np.array([(0, 1)], np.int32)
+    blocks = np.zeros(((nb + 2) * 16, 3, 3))
+    d = np.exp2(rng.integers(0, 4, size=(nb, 12))).astype(float)  # powers of two: exact quotients
+    for b in range(nb):
+        for ii in range(4):
+            blocks[16 * b + ii + 4 * ii] = np.diag(d[b, 3 * ii:3 * ii + 3])
+    rhs = np.exp2(rng.integers(-3, 3, size=(4 * nb, 3))).astype(float)
+    x, it, res = O.bsr_cg(nb, ij, blocks, rhs, np.zeros_like(rhs), 1e-12)
+    assert np.isfinite(x).all() and res == 0.0 and it == 9
+    np.testing.assert_array_equal(x.reshape(nb, 12), rhs.reshape(nb, 12) / d)
+
+
 def test_block_jacobi_cg_is_preconditioned_cg():
     """The block-Jacobi CG (beyond the reference) equals a numpy restatement of PCG with
     M = blockdiag(12x12 diagonal blocks), iterate for iterate, and takes fewer iterations
