@@ -811,25 +811,25 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
         double* part;
         if ((rc = arena_get(kSlotPairPart, sizeof(double) * kPartStride * (size_t)cap_items, (void**)&part)))
             return rc;
-        ph = prof_start(kProfPairReduce, s, (8.0 * stride + 9) * nact + 8.0 * kPairBufStride * nij);
         hipLaunchKernelGGL(pair_chunks_kernel, dim3(grid_for(nij)), dim3(kBlock), 0, s, nij, (const int64_t*)pseg,
                            pcnt);
         ABD_CHECK_LAUNCH();
         if ((rc = scan_exclusive_i64(s, pcnt, pcoff, nij, ptotal))) return rc;
+        // timed: the partial sums (each stored contact read once, its body-order flag once)
+        ph = prof_start_ext(kProfPairReduce, (8.0 * stride + 1) * nact);
         if (dense)
-            hipLaunchKernelGGL(pair_partial_kernel<true>, dim3(kPairGrid), dim3(kPairThreads), 0, s, nij,
-                               (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg,
-                               (const int64_t*)pcoff, (const int64_t*)ptotal, part);
+            prof_launch(ph, pair_partial_kernel<true>, dim3(kPairGrid), dim3(kPairThreads), 0, s, nij,
+                        (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg, (const int64_t*)pcoff,
+                        (const int64_t*)ptotal, part);
         else
-            hipLaunchKernelGGL(pair_partial_kernel<false>, dim3(kPairGrid), dim3(kPairThreads), 0, s, nij,
-                               (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg,
-                               (const int64_t*)pcoff, (const int64_t*)ptotal, part);
+            prof_launch(ph, pair_partial_kernel<false>, dim3(kPairGrid), dim3(kPairThreads), 0, s, nij,
+                        (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg, (const int64_t*)pcoff,
+                        (const int64_t*)ptotal, part);
         ABD_CHECK_LAUNCH();
         hipLaunchKernelGGL(pair_reduce_kernel, dim3(grid_for(nij, 1, kPairGrid)), dim3(kPairThreads), 0, s, nb, nij,
                            (const uint8_t*)pflag, (const int64_t*)pcoff, (const int64_t*)pcnt, (const double*)part,
                            pbuf, blocks);
         ABD_CHECK_LAUNCH();
-        prof_stop(ph, s);
     }
     int phf = prof_start(kProfBodyFinalize, s, 2.0 * 8 * 156 * nb + 8.0 * kPairBufStride * nij + 40.0 * nvg);
     hipLaunchKernelGGL(body_finalize_kernel, dim3(nb), dim3(kBlock), 0, s, *sc, *P, x, nij, row_ptr, slots,

@@ -3,6 +3,7 @@
 // abd_profile_begin() was called; then every instrumented launch records a
 // start/stop event pair that abd_profile_query() resolves.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -30,10 +31,30 @@ enum ProfKernel {
 bool prof_enabled();
 // Records the start of an instrumented launch; returns a handle (or -1 when off).
 int prof_start(int kernel, hipStream_t s, double bytes);
+// Ends the timed region of `handle` (the first call on a handle records its end event).
 // `dev_scale` (nullable): device int32 whose value multiplies `bytes` at query time
 // (iteration counts); `dev_bytes` (nullable): device double, likewise (byte counts
-// computed on the device, e.g. from per-system iteration counts).  Both are copied
-// on the launch stream, so recording them costs no host synchronization.
+// computed on the device, e.g. from per-system iteration counts).  Both are copied on
+// the launch stream after the end event, so recording them costs no host
+// synchronization and no time inside the region; a second call after the kernels that
+// compute such a value attaches it without moving the end.
 void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale = nullptr, const double* dev_bytes = nullptr);
+
+// A record whose events the kernel dispatch itself takes (prof_launch): it times the kernel
+// alone, without the gaps around it (e.g. a launch behind a host synchronization).
+int prof_start_ext(int kernel, double bytes);
+void prof_ext_events(int handle, hipEvent_t* a, hipEvent_t* b);
+void prof_ext_done(int handle);
+
+// Launches `kernel`; with a live record (prof_start_ext) the dispatch records its start and
+// stop events.  prof_stop may follow to attach device values.
+template <typename F, typename... Args>
+inline void prof_launch(int handle, F kernel, const dim3& grid, const dim3& block, uint32_t shm, hipStream_t s,
+                        Args... args) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (handle >= 0) prof_ext_events(handle, &a, &b);
+    hipExtLaunchKernelGGL(kernel, grid, block, shm, s, a, b, 0, args...);
+    if (handle >= 0) prof_ext_done(handle);
+}
 
 }  // namespace abd

@@ -13,6 +13,7 @@ namespace {
 struct Rec {
     int kernel;
     hipEvent_t a, b;
+    hipEvent_t c;    // after the copies of scale / byte values (nullptr: none)
     double bytes;
     int scale_slot;  // index into the pinned scale buffer, -1 when unscaled
     int bytes_slot;  // index into the pinned byte buffer, -1 when unscaled
@@ -55,13 +56,14 @@ void account(const Rec& r, float ms) {
     g_n[r.kernel] += 1;
     g_pool.push_back(r.a);
     g_pool.push_back(r.b);
+    if (r.c) g_pool.push_back(r.c);
 }
 
 // Resolves the records whose end event has completed, in order, without waiting.
 void resolve_ready_locked() {
     while (g_done < g_recs.size()) {
         Rec& r = g_recs[g_done];
-        if (!r.stopped || hipEventQuery(r.b) != hipSuccess) break;
+        if (!r.stopped || hipEventQuery(r.c ? r.c : r.b) != hipSuccess) break;
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) break;
         account(r, ms);
@@ -73,7 +75,7 @@ int resolve_locked() {
     for (; g_done < g_recs.size(); ++g_done) {
         Rec& r = g_recs[g_done];
         if (!r.stopped) continue;
-        ABD_CHECK(hipEventSynchronize(r.b));
+        ABD_CHECK(hipEventSynchronize(r.c ? r.c : r.b));
         float ms = 0.f;
         ABD_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
         account(r, ms);
@@ -92,7 +94,7 @@ int prof_start(int kernel, hipStream_t s, double bytes) {
     if (!g_on || !((g_mask >> kernel) & 1u)) return -1;
     std::lock_guard<std::mutex> lock(g_mu);
     if (g_pool.size() < 2 && g_recs.size() - g_done > 1024) resolve_ready_locked();
-    Rec r{kernel, take_event(), take_event(), bytes, -1, -1, false};
+    Rec r{kernel, take_event(), take_event(), nullptr, bytes, -1, -1, false};
     if (!r.a || !r.b) {
         if (r.a) g_pool.push_back(r.a);
         if (r.b) g_pool.push_back(r.b);
@@ -107,22 +109,61 @@ int prof_start(int kernel, hipStream_t s, double bytes) {
     return (int)g_recs.size() - 1;
 }
 
+int prof_start_ext(int kernel, double bytes) {
+    if (!g_on || !((g_mask >> kernel) & 1u)) return -1;
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (g_pool.size() < 2 && g_recs.size() - g_done > 1024) resolve_ready_locked();
+    Rec r{kernel, take_event(), take_event(), nullptr, bytes, -1, -1, false};
+    if (!r.a || !r.b) {
+        if (r.a) g_pool.push_back(r.a);
+        if (r.b) g_pool.push_back(r.b);
+        return -1;
+    }
+    g_recs.push_back(r);
+    return (int)g_recs.size() - 1;
+}
+
+void prof_ext_events(int handle, hipEvent_t* a, hipEvent_t* b) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    *a = g_recs[handle].a;
+    *b = g_recs[handle].b;
+}
+
+void prof_ext_done(int handle) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_recs[handle].stopped = true;
+}
+
 void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale, const double* dev_bytes) {
     if (handle < 0) return;
     std::lock_guard<std::mutex> lock(g_mu);
     if (handle >= (int)g_recs.size()) return;
+    Rec& r = g_recs[handle];
+    // the timed region ends here, before any copy (a second call only adds values)
+    if (!r.stopped && hipEventRecord(r.b, s) == hipSuccess) r.stopped = true;
+    bool copied = false;
     if (dev_scale && g_scale && g_scale_used < kScaleSlots) {
         // the value as of this launch, copied on the launch stream
         const int slot = g_scale_used++;
-        if (hipMemcpyAsync(&g_scale[slot], dev_scale, sizeof(int32_t), hipMemcpyDeviceToHost, s) == hipSuccess)
-            g_recs[handle].scale_slot = slot;
+        if (hipMemcpyAsync(&g_scale[slot], dev_scale, sizeof(int32_t), hipMemcpyDeviceToHost, s) == hipSuccess) {
+            r.scale_slot = slot;
+            copied = true;
+        }
     }
     if (dev_bytes && g_dbytes && g_bytes_used < kScaleSlots) {
         const int slot = g_bytes_used++;
-        if (hipMemcpyAsync(&g_dbytes[slot], dev_bytes, sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess)
-            g_recs[handle].bytes_slot = slot;
+        if (hipMemcpyAsync(&g_dbytes[slot], dev_bytes, sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess) {
+            r.bytes_slot = slot;
+            copied = true;
+        }
     }
-    if (hipEventRecord(g_recs[handle].b, s) == hipSuccess) g_recs[handle].stopped = true;
+    if (copied && r.stopped && !r.c) {
+        r.c = take_event();
+        if (r.c && hipEventRecord(r.c, s) != hipSuccess) {
+            g_pool.push_back(r.c);
+            r.c = nullptr;
+        }
+    }
 }
 
 }  // namespace abd

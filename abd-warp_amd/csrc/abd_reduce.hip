@@ -360,11 +360,10 @@ int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
     int rc = get_plan(s, sc, S, L, plan_in, &seg, &choff);
     if (!rc) rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound, (void**)&part);
     if (rc) return rc;
-    int ph = prof_start(prof_id, s, prof_bytes);
-    hipLaunchKernelGGL(chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1,
-                       TrialArgs{nullptr, nullptr, nullptr, nullptr, active}, L, S, seg, choff, part, (int64_t)0);
+    int ph = prof_start_ext(prof_id, prof_bytes);
+    prof_launch(ph, chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1,
+                TrialArgs{nullptr, nullptr, nullptr, nullptr, active}, L, S, seg, choff, part, (int64_t)0);
     ABD_CHECK_LAUNCH();
-    prof_stop(ph, s);
     hipLaunchKernelGGL(scene_final_kernel<OP>, dim3(S), dim3(kRedThreads), 0, s, choff, part, out, (int64_t)0,
                        (int64_t)0);
     ABD_CHECK_LAUNCH();
@@ -445,12 +444,11 @@ int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, in
     double* part;
     if ((rc = get_plan(s, *sc, S, L, plan_in, &seg, &choff))) return rc;
     if ((rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound * K, (void**)&part))) return rc;
-    int ph = prof_start(kProfEnergy, s, K * (128.0 * npt + 132.0 * nee + 32.0 * nvg));
-    hipLaunchKernelGGL(chunk_kernel<2>, dim3((unsigned)bound, K), dim3(kRedThreads), 0, s, *sc, *P,
-                       (const double*)nullptr, (const double*)nullptr, TrialArgs{alpha, A, p, dq, nullptr}, L, S, seg,
-                       choff, part, bound);
+    int ph = prof_start_ext(kProfEnergy, K * (128.0 * npt + 132.0 * nee + 32.0 * nvg));
+    prof_launch(ph, chunk_kernel<2>, dim3((unsigned)bound, K), dim3(kRedThreads), 0, s, *sc, *P,
+                (const double*)nullptr, (const double*)nullptr, TrialArgs{alpha, A, p, dq, nullptr}, L, S, seg, choff,
+                part, bound);
     ABD_CHECK_LAUNCH();
-    prof_stop(ph, s);
     hipLaunchKernelGGL(scene_final_kernel<0>, dim3(S, K), dim3(kRedThreads), 0, s, choff, (const double*)part, E,
                        bound, (int64_t)2 * S);
     ABD_CHECK_LAUNCH();

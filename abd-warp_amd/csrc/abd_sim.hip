@@ -133,6 +133,24 @@ int trials_to_floor(double a, double alpha_min) {
     return k;
 }
 
+// The line search's first step length of a scene from its TOI (abd.py:300-302: clamp to
+// [0, 1], x 0.9 below 1), or 0 when the scene is not iterating; the host applies the same
+// expression to the same TOI.
+__host__ __device__ inline double first_alpha(double t, bool iterating) {
+    t = t < 1.0 ? t : 1.0;  // std::min(1.0, t)
+    t = 0.0 < t ? t : 0.0;  // std::max(0.0, t)
+    if (t < 1.0) t *= 0.9;
+    return iterating ? t : 0.0;
+}
+
+// first trial of the batched line search: alpha[s] = first_alpha(toi[s]) for the scenes
+// iterating, -1 (no trial) for the others
+__global__ void first_trial_kernel(int ns, const double* __restrict__ toi, const uint8_t* __restrict__ active,
+                                   double* __restrict__ alpha) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < ns) alpha[q] = active[q] ? first_alpha(toi[q], true) : -1.0;
+}
+
 // per-scene value rows of Sim::sv / Sim::hsv
 enum { kSvEc = 0, kSvEi, kSvToi, kSvQtg, kSvGtg, kSvAlpha, kSvAlphaCcd, kSv };
 
@@ -383,10 +401,7 @@ int ccd(hipStream_t s, Sim& S, double* alpha_cap) {
                         S.sw.nvg, S.plan.p, S.scal.p + 2, nullptr)))
         return rc;
     if ((rc = get_scalars(s, S, 2, 1))) return rc;
-    double t = S.host[2];
-    t = std::max(0.0, std::min(1.0, t));
-    if (t < 1.0) t *= 0.9;
-    *alpha_cap = t;
+    *alpha_cap = first_alpha(S.host[2], true);
     return ABD_OK;
 }
 
@@ -505,17 +520,21 @@ int compute_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
 
 // line_energy_multi per scene, for K trial step lengths each (dalpha [K][ns] on the device):
 // energies to hcand[kLsSpec * ns + (2 k) ns + s] (contact) and [... + (2 k + 1) ns + s] (inertia)
-int line_energy_scenes_multi(hipStream_t s, Sim& S, const double* dalpha, int K) {
-    int rc;
+int line_energy_scenes_launch(hipStream_t s, Sim& S, const double* dalpha, int K) {
     const AbdSimState& st = S.st;
     const int ns = S.ns;
-    if ((rc = line_trials(s, &S.sc, &S.P, ns, K, dalpha, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot, S.sw.ptl.p,
-                          S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p)))
-        return rc;
+    int rc = line_trials(s, &S.sc, &S.P, ns, K, dalpha, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
+                         S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p);
+    if (rc) return rc;
     ABD_CHECK(hipMemcpyAsync(S.hcand + (size_t)kLsSpec * ns, S.cand_E.p, sizeof(double) * 2 * ns * K,
                              hipMemcpyDeviceToHost, s));
-    ABD_CHECK(hipStreamSynchronize(s));
     S.stats[5] += K;
+    return ABD_OK;
+}
+int line_energy_scenes_multi(hipStream_t s, Sim& S, const double* dalpha, int K) {
+    int rc = line_energy_scenes_launch(s, S, dalpha, K);
+    if (rc) return rc;
+    ABD_CHECK(hipStreamSynchronize(s));
     return ABD_OK;
 }
 
@@ -583,15 +602,18 @@ int step_scenes(hipStream_t s, Sim& S) {
                             S.sw.nvg, S.plan.p, S.sv.p + kSvToi * ns, S.sactive.p)))
             return rc;
         if ((rc = abd_scene_dot(s, &S.sc, st.dq, st.g, S.sv.p + kSvQtg * ns))) return rc;
+        // the line search's first trial straight behind the TOI (step lengths from it on the
+        // device), read back with the TOI and q.g
+        hipLaunchKernelGGL(first_trial_kernel, dim3((unsigned)ceil_div(ns, kBlock)), dim3(kBlock), 0, s, ns,
+                           (const double*)(S.sv.p + kSvToi * ns), (const uint8_t*)S.sactive.p, S.cand_alpha.p);
+        ABD_CHECK_LAUNCH();
+        if ((rc = line_energy_scenes_launch(s, S, S.cand_alpha.p, 1))) return rc;
         ABD_CHECK(hipMemcpyAsync(S.hsv + kSvToi * ns, S.sv.p + kSvToi * ns, sizeof(double) * 2 * ns,
                                  hipMemcpyDeviceToHost, s));
         ABD_CHECK(hipStreamSynchronize(s));
         for (int k = 0; k < ns; ++k) {
             ls[k] = newton[k];
-            double t = S.hsv[kSvToi * ns + k];
-            t = std::max(0.0, std::min(1.0, t));
-            if (t < 1.0) t *= 0.9;
-            S.halpha[k] = newton[k] ? t : 0.0;
+            S.halpha[k] = first_alpha(S.hsv[kSvToi * ns + k], newton[k] != 0);
         }
         // line search, each scene on its own (abd.py:146-160): the first trial, then the
         // trials (alpha, alpha/2, ...) down to alpha_min for the scenes still searching
@@ -606,8 +628,11 @@ int step_scenes(hipStream_t s, Sim& S) {
                     ha[(size_t)k * ns + q] =
                         k >= need ? -1.0 : (k == 0 ? S.halpha[q] : ha[(size_t)(k - 1) * ns + q] * 0.5);
             }
-            if ((rc = upload(s, S.cand_alpha.p, ha, sizeof(double) * ns * K))) return rc;
-            if ((rc = line_energy_scenes_multi(s, S, S.cand_alpha.p, K))) return rc;
+            // pass 0's energies came back with the TOI (the same step lengths, from the device)
+            if (pass > 0) {
+                if ((rc = upload(s, S.cand_alpha.p, ha, sizeof(double) * ns * K))) return rc;
+                if ((rc = line_energy_scenes_multi(s, S, S.cand_alpha.p, K))) return rc;
+            }
             const double* he = S.hcand + (size_t)kLsSpec * ns;
             for (int q = 0; q < ns; ++q) {
                 if (!ls[q]) continue;

@@ -900,11 +900,12 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
 // each SpMV streams the system's blocks (144 doubles), column and slot ids, and
 // row pointers; with vectors outside LDS, ~10 vector passes of 12 n doubles too.
 __global__ void solve_bytes_kernel(const int32_t* __restrict__ row_ptr, int nb, const int32_t* __restrict__ sboff,
-                                   int nsys, const int32_t* __restrict__ iters, int spmv_per_it, int lds,
-                                   double precond_per_body, double* __restrict__ out) {
+                                   const uint8_t* __restrict__ active, int nsys, const int32_t* __restrict__ iters,
+                                   int spmv_per_it, int lds, double precond_per_body, double* __restrict__ out) {
     __shared__ double sh[17];
     double acc = 0.0;
     for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
+        if (active && !active[s]) continue;  // not solved in this launch
         const int r0 = sboff ? sboff[s] : 0, r1 = sboff ? sboff[s + 1] : nb;
         const double nnzb = (double)(row_ptr[r1] - row_ptr[r0]);
         const double nbs = (double)(r1 - r0);
@@ -1113,7 +1114,9 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
             prof_stop(ph_inv, s);
         }
     }
-    int ph = prof_start(kProfSolve, s, 1.0);
+    // the fallback of a direct solve is timed with it (its launch returns at once unless a
+    // system was declined)
+    int ph = pending ? -1 : prof_start(kProfSolve, s, 1.0);
     const size_t vec_bytes = sizeof(double) * 6 * (size_t)max_rows;
     // dynamic LDS: the whole budget of the device (beside the kernels' static LDS) when it
     // holds the structure region, the vectors in it when they fit beside the structure
@@ -1156,11 +1159,12 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
                                tol, maxiter, check_every, max_rows, budget, ws, dinv, iters, resid, pending);
     }
     ABD_CHECK_LAUNCH();
+    prof_stop(ph, s);
     if (ph >= 0 && iters) {
         // the byte count needs the iteration counts: computed behind the solve, on its stream
         double* dbytes;
         if ((rc = arena_get(kSlotSolveBytes, sizeof(double), (void**)&dbytes))) return rc;
-        hipLaunchKernelGGL(solve_bytes_kernel, dim3(1), dim3(256), 0, s, M.row_ptr, M.nb, sboff, nsys, iters,
+        hipLaunchKernelGGL(solve_bytes_kernel, dim3(1), dim3(256), 0, s, M.row_ptr, M.nb, sboff, active, nsys, iters,
                            method == ABD_SOLVER_BICGSTAB ? 2 : 1, lds ? 1 : 0, bj ? 1152.0 : 0.0, dbytes);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s, nullptr, dbytes);
@@ -1181,13 +1185,23 @@ int launch_ldlt(hipStream_t s, const Bsr& M, int nsys, const int32_t* sboff, con
     const int64_t region = ldlt_region(cap);
     double* ws;
     if ((rc = arena_get(kSlotLdlt, sizeof(double) * (size_t)region * (size_t)nsys, (void**)&ws))) return rc;
-    const int ph = prof_start(kProfSolve, s, 1.0);
-    hipLaunchKernelGGL(ldlt_kernel, dim3(nsys), dim3(kSolveThreads), 0, s, M, sboff, active, b, x, region, ws, status,
-                       resid);
+    const int ph = prof_start_ext(kProfSolve, 1.0);
+    prof_launch(ph, ldlt_kernel, dim3(nsys), dim3(kSolveThreads), 0, s, M, sboff, active, b, x, region, ws, status,
+                resid);
     ABD_CHECK_LAUNCH();
-    prof_stop(ph, s);
-    return launch_solve(s, ABD_SOLVER_CG_JACOBI, M, nsys, sboff, active, 12 * max_bodies, b, x, tol, maxiter,
-                        check_every, status, resid, solve_threads(max_bodies), status);
+    if (ph >= 0) {
+        // algorithmic bytes of the systems solved directly: each block, b and x once
+        double* dbytes;
+        if ((rc = arena_get(kSlotSolveBytes, sizeof(double), (void**)&dbytes))) return rc;
+        hipLaunchKernelGGL(solve_bytes_kernel, dim3(1), dim3(256), 0, s, M.row_ptr, M.nb, sboff, active, nsys,
+                           (const int32_t*)status, 1, 1, 192.0, dbytes);
+        ABD_CHECK_LAUNCH();
+        prof_stop(ph, s, nullptr, dbytes);
+    }
+    if ((rc = launch_solve(s, ABD_SOLVER_CG_JACOBI, M, nsys, sboff, active, 12 * max_bodies, b, x, tol, maxiter,
+                           check_every, status, resid, solve_threads(max_bodies), status)))
+        return rc;
+    return ABD_OK;
 }
 }  // namespace
 

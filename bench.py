@@ -18,17 +18,18 @@ Solver mode: with the reference's contact Hessian (indefinite: b' < 0 times a
 clamped-PSD matrix) the reference's BiCGSTAB diverges on colliding scenes and
 the swept CCD boxes then cover the scene.  The benchmark therefore runs the
 documented stable mode: contact-space PSD projection of the barrier Hessian
-(psd=1) with the reference's CG (bsr_utils.py:126-235, same iteration and
-stopping rule) preconditioned by each body's 12x12 diagonal block instead of
-its scalar diagonal (--solver 1 restores the diagonal).  The CPU baseline runs
-the oracle in the same mode on the sampled scenes.
+(psd=1), with each Newton system solved directly by a block LDL^T factorization
+(solver 3; systems above 64 bodies fall back to the reference's Jacobi CG).
+--solver 2 runs the reference's CG (bsr_utils.py:126-235, same iteration and
+stopping rule) preconditioned by each body's 12x12 diagonal block, --solver 1
+with its scalar diagonal.  The CPU baseline runs the oracle in the same mode on
+the sampled scenes.
 
 Frames: W warmup frames, then (unless --no-profile) untimed attribution frames in
 which every instrumented kernel is timed with HIP events (the per-kernel table,
 the gh-stage rate and the choice of the dominant kernel), then the K timed
-frames, in which only the dominant kernel (its roofline) and the four gh
-kernels (the barrier-Hessian pair rate) carry events, so that per-launch event
-overhead stays out of `value`.
+frames, in which only the dominant kernel carries events (its roofline), so that
+per-launch event overhead stays out of `value`.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload batched|medium|small|stress]
 """
@@ -74,11 +75,14 @@ def make_workload(name: str):
 GH_KERNELS = ("gh_classify", "gh_contact", "gh_pair_reduce", "gh_body_finalize")
 # measured HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy of known
 # size) of the profiled kernels, kept under profiles/ as "<kernel>_<workload>_pmc.md"
-PMC_FILES = {"solve": "cg_kernel", "boxes": "prim_boxes"}
+PMC_FILES = {"ccd_toi": "chunk_kernel_toi", "solve": "ldlt_kernel", "gh_pair_reduce": "pair_partial_kernel",
+             "energy": "chunk_kernel_energy"}
 
 
-def measured_traffic(kernel: str, workload: str):
+def measured_traffic(kernel: str, workload: str, solver: int = 3):
     name = PMC_FILES.get(kernel)
+    if kernel == "solve" and solver != 3:
+        name = None  # the Krylov kernels' traffic is not profiled for this configuration
     if not name:
         return None
     path = os.path.join(ROOT, "profiles", f"{name}_{workload}_pmc.md")
@@ -244,12 +248,14 @@ def main():
         log(f"[gpu] autotuned culling tiles: {tiles}")
     frame = a.warmup
     # attribution frames (untimed): every instrumented kernel timed with HIP events, for the
-    # per-kernel table, the gh-stage rate and the choice of the dominant kernel; the timed
-    # frames then carry events on that kernel only, so the event overhead stays out of `value`
+    # per-kernel table, the gh-stage rate and the candidates for the dominant kernel; the timed
+    # frames then carry events on those only, so the event overhead stays out of `value`
     aprof, dom = None, None
+    gh_s, gh_cands = 0.0, 0
     n_attrib = 0 if a.no_profile else max(a.attrib_frames, 1)
     if n_attrib:
         torch.cuda.synchronize()
+        sa = dict(sim.stats)
         _lib.profile_select(None)
         _lib.profile_begin()
         for f in range(n_attrib):
@@ -259,15 +265,19 @@ def main():
         torch.cuda.synchronize()
         _lib.profile_end()
         aprof = _lib.profile_query()
-        # dominant kernel with an algorithmic byte count
+        # gh rate over these frames (both drivers launch the same four gh kernels)
+        gh_s = sum(aprof[k][0] for k in GH_KERNELS if k in aprof) * 1e-3
+        gh_cands = sim.stats["gh_pt"] + sim.stats["gh_ee"] - sa["gh_pt"] - sa["gh_ee"]
+        # the dominant kernel: the largest with an algorithmic byte count
         ranked = sorted(((v[0], k) for k, v in aprof.items() if v[1] > 0 and v[2] > 0), reverse=True)
-        dom = ranked[0][1] if ranked else None
+        cands_dom = [k for _, k in ranked[:1]]
+        dom = cands_dom[0] if cands_dom else None
     first = frame
     torch.cuda.synchronize()
     s0 = dict(sim.stats)
     if dom:
-        # the dominant kernel (roofline) and the four gh kernels (barrier-Hessian pairs/s)
-        _lib.profile_select(sorted({dom, *GH_KERNELS}))
+        # the dominant kernel (its roofline): the only launches with events in the timed frames
+        _lib.profile_select(sorted(cands_dom))
         _lib.profile_begin()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -292,15 +302,18 @@ def main():
     d = {k: sim.stats[k] - s0[k] for k in sim.stats}
     wall = t1 - t0
     iters = d["newton_iters"]
-    # gh rate over the timed frames (both drivers launch the same four gh kernels)
-    gh_s = sum(prof[k][0] for k in GH_KERNELS if k in prof) * 1e-3 if prof else 0.0
     cands = d["gh_pt"] + d["gh_ee"]
     roof = None
+    timed_kernels = None
     if dom:
+        timed_kernels = {k: dict(launches=prof[k][1], avg_us=round(1e3 * prof[k][0] / prof[k][1], 3),
+                                 ms_per_frame=round(prof[k][0] / a.steps, 3))
+                         for k in cands_dom if k in prof and prof[k][1] > 0}
+        dom = max(timed_kernels, key=lambda k: timed_kernels[k]["ms_per_frame"])
         ms, n, byts = prof[dom]
         achieved = byts / (ms * 1e-3) / 1e9
         roof = dict(kernel=dom, bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=measured_traffic(dom, a.workload), launches=n,
+                    frac=achieved / HBM_PEAK_GBS, traffic=measured_traffic(dom, a.workload, a.solver), launches=n,
                     avg_us=round(1e3 * ms / n, 3), bytes_per_launch=byts / n)
     batched = spec.n_scenes > 1
     plain = None
@@ -346,7 +359,9 @@ def main():
                    "driver": "python" if a.python_driver else "native",
                    "solver": {0: "bicgstab", 1: "jacobi-cg", 2: "block-jacobi-cg", 3: "block-ldlt"}[a.solver],
                    "frames_timed": f"{first}..{first + a.steps - 1}", "tiles": tiles},
-        "gh_pairs_per_s": cands / gh_s if gh_s > 0 else None,
+        # PT + EE candidates through the four gh kernels per second of their event time
+        # (attribution frames)
+        "gh_pairs_per_s": gh_cands / gh_s if gh_s > 0 else None,
         "gh_candidates": cands,
         "gh_active": d["gh_active"],
         "newton_iters": iters,
@@ -357,8 +372,10 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_par,
         # per-frame kernel times of the attribution frames (all instrumented kernels timed there;
-        # the timed frames time the dominant kernel and the gh kernels only)
+        # the timed frames time the dominant kernel only)
         "kernels_ms_per_frame": {k: round(v[0] / n_attrib, 3) for k, v in aprof.items() if v[1]} if aprof else None,
+        # the dominant kernel over the timed frames (HIP events)
+        "timed_kernels": timed_kernels,
         "attribution_frames": f"{a.warmup}..{first - 1}" if n_attrib else None,
     }
     print(json.dumps(out), flush=True)
