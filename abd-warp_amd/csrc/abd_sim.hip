@@ -89,6 +89,7 @@ struct Sim {
     BoxBuf sbody, stri, sedge, spt;  // the superset's boxes
     DevBuf<double> vlo, vhi;         // its vertex ranges
     bool sup_valid = false;
+    bool col_fresh = false;  // col (with its structure) is the collision set at the current x
     DevBuf<int32_t> row_ptr, cols, slots;
     DevBuf<double> blocks;
     DevBuf<double> scal;  // [0] E contact, [1] E inertia, [2] toi, [3] dot, [4] resid
@@ -250,15 +251,32 @@ int collision_set(hipStream_t s, Sim& S, bool structure, const double* xv) {
 // of it: q = q_prev - alpha dq lies on that step and the swept boxes contain the boxes at
 // every point of it.  The result is the culling's own (keep_from_superset); used for the
 // Newton iterations after a frame's first.
-int collision_from_swept(hipStream_t s, Sim& S, const double* xv) {
+int collision_from(hipStream_t s, Sim& S, const CandSet& W, const double* xv);
+int collision_from_swept(hipStream_t s, Sim& S, const double* xv) { return collision_from(s, S, S.sw, xv); }
+
+// A frame's first collision set: filtered from the candidate superset when the state lies
+// inside its vertex ranges (boxes at any positions inside them lie inside its boxes, so the
+// culling output is a subset of it), else culled.
+int collision_at_frame_start(hipStream_t s, Sim& S, const double* xv) {
+    if (S.sup_valid) {
+        int rc;
+        ABD_CHECK(hipMemsetAsync(S.dcount.p + 7, 0, sizeof(int64_t), s));
+        if ((rc = within_ranges(s, 3 * (int64_t)S.nv, xv, xv, S.vlo.p, S.vhi.p, S.dcount.p + 7))) return rc;
+        ABD_CHECK(hipMemcpyAsync(S.hcount + 7, S.dcount.p + 7, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipStreamSynchronize(s));
+        if (S.hcount[7] == 0) return collision_from(s, S, S.sup, xv);
+    }
+    return collision_set(s, S, true, xv);
+}
+
+int collision_from(hipStream_t s, Sim& S, const CandSet& W, const double* xv) {
     const double dh = S.P.dhat * 0.5;
     int rc;
     if ((rc = abd_body_boxes(s, &S.sc, xv, dh, S.body.lo.p, S.body.hi.p))) return rc;
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_TRI, xv, 0.0, S.tri.lo.p, S.tri.hi.p))) return rc;
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_EDGE, xv, dh, S.edge.lo.p, S.edge.hi.p))) return rc;
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_POINT, xv, S.P.dhat, S.pt.lo.p, S.pt.hi.p))) return rc;
-    if ((rc = S.col.reserve_like(S.sw))) return rc;
-    const CandSet& W = S.sw;
+    if ((rc = S.col.reserve_like(W))) return rc;
     const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg};
     const KeepBoxes bx{S.body.lo.p, S.body.hi.p, S.pt.lo.p, S.pt.hi.p, S.edge.lo.p, S.edge.hi.p, S.tri.lo.p,
                        S.tri.hi.p};
@@ -271,8 +289,11 @@ int collision_from_swept(hipStream_t s, Sim& S, const double* xv) {
     return structure_of(s, S);
 }
 
-// Skin of the candidate superset's vertex ranges, in units of dhat.
-constexpr double kSkinOverDhat = 2.0;
+// Skin of the candidate superset's vertex ranges, in units of dhat: wide enough that the
+// later Newton steps of a frame (and of the next frames while the scenes rest) stay inside,
+// so that a batch of scenes, which reuses the superset only when all of them do, skips
+// most cullings; 30 measured best among 2, 10, 30 and 100 on the 256-scene workload.
+constexpr double kSkinOverDhat = 30.0;
 
 // abd.py:170-185, with both ends of every box dilated as in collision_set: one candidate
 // set per Newton iteration that holds both the CCD candidates of the step (a pair that
@@ -282,27 +303,20 @@ constexpr double kSkinOverDhat = 2.0;
 // verified root), and the line search's energies need no re-culling: a pair missing from
 // the collision set at x - alpha dq has boxes apart, hence distance >= dhat and a zero
 // barrier, so only the order of the nonzero terms' partial sums changes.
-// The set is the culling output of the step's swept boxes.  With `superset` (one scene),
-// it is taken by filtering the candidate superset (keep_from_superset: the same
-// predicates) while both ends of the step lie inside the superset's vertex ranges, and
-// the superset is culled anew (from the ranges of this step's ends, widened by the skin)
-// when they do not.  Without (independent scenes: some scene nearly always leaves its
-// ranges), the swept boxes are culled directly.
-int swept_set(hipStream_t s, Sim& S, bool superset) {
+// The set is the culling output of the step's swept boxes, taken by filtering the
+// candidate superset (keep_from_superset: the same predicates) while both ends of the
+// step lie inside the superset's vertex ranges (every vertex of every scene), and the
+// superset is culled anew (from the ranges of this step's ends, widened by the skin) when
+// they do not.
+int swept_set(hipStream_t s, Sim& S) {
     const double dh = S.P.dhat * 0.5;
     const AbdSimState& st = S.st;
     const int64_t n3 = 3 * (int64_t)S.nv;
     int rc;
     if ((rc = swept_boxes(s, &S.sc, -1, st.x, st.xk, dh, S.tbody.lo.p, S.tbody.hi.p))) return rc;
-    if (!superset) {
-        S.sup_valid = false;
-        // with the (j, i) order: the next iteration's collision set is filtered from this set
-        if ((rc = body_pairs(s, S, S.sw, S.tbody, true))) return rc;
-    }
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, st.x, st.xk, 0.0, S.ttri.lo.p, S.ttri.hi.p))) return rc;
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, st.x, st.xk, dh, S.tedge.lo.p, S.tedge.hi.p))) return rc;
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, st.x, st.xk, S.P.dhat, S.tpt.lo.p, S.tpt.hi.p))) return rc;
-    if (!superset) return cull_lists(s, S, S.sw, S.tbody, S.tpt, S.ttri, S.tedge);
     bool reuse = false;
     if (S.sup_valid) {
         ABD_CHECK(hipMemsetAsync(S.dcount.p + 7, 0, sizeof(int64_t), s));
@@ -344,7 +358,10 @@ int compute_energy(hipStream_t s, Sim& S, double alpha, double* E) {
     const AbdSimState& st = S.st;
     if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, alpha, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-    if ((rc = collision_set(s, S, false, st.xk))) return rc;
+    // called at a frame's start (alpha = 0: xk = x): the first Newton iteration's collision
+    // set too, with its structure
+    if ((rc = collision_at_frame_start(s, S, st.xk))) return rc;
+    S.col_fresh = alpha == 0.0;
     if ((rc = abd_contact_energy(s, &S.sc, &S.P, st.xk, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg, S.scal.p)))
         return rc;
     if ((rc = abd_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot, S.scal.p + 1)))
@@ -389,7 +406,7 @@ int ccd(hipStream_t s, Sim& S, double* alpha_cap) {
     // x = x(A, p) from the start of the iteration; xk: the end of the full step
     if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, 1.0, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-    if ((rc = swept_set(s, S, true))) return rc;
+    if ((rc = swept_set(s, S))) return rc;
     S.sw_valid = true;
     S.stats[8] += S.sw.npt;
     S.stats[9] += S.sw.nee;
@@ -422,7 +439,10 @@ int step(hipStream_t s, Sim& S) {
     for (;;) {
         // (Ak, pk) = (A, p) here: the collision set is culled on x
         if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
-        if ((rc = S.sw_valid ? collision_from_swept(s, S, st.x) : collision_set(s, S, true, st.x))) return rc;
+        if (S.col_fresh)
+            S.col_fresh = false;  // the frame-start energy's set: the same positions
+        else if ((rc = S.sw_valid ? collision_from_swept(s, S, st.x) : collision_at_frame_start(s, S, st.x)))
+            return rc;
         const size_t nslot = nb + 2 * (size_t)S.col.nij;
         if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
         ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
@@ -504,7 +524,10 @@ int compute_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
     const int ns = S.ns;
     if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dalpha, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-    if ((rc = collision_set(s, S, false, st.xk))) return rc;
+    // at a frame's start (every alpha 0: zero-length steps are exact copies, xk = x): the
+    // first Newton iteration's collision set too, with its structure
+    if ((rc = collision_at_frame_start(s, S, st.xk))) return rc;
+    S.col_fresh = true;
     if ((rc = abd_scene_contact_energy(s, &S.sc, &S.P, st.xk, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg,
                                        S.sv.p + kSvEc * ns)))
         return rc;
@@ -562,7 +585,10 @@ int step_scenes(hipStream_t s, Sim& S) {
     while (n_active > 0) {
         // (Ak, pk) = (A, p) here: the collision set is culled on x
         if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
-        if ((rc = S.sw_valid ? collision_from_swept(s, S, st.x) : collision_set(s, S, true, st.x))) return rc;
+        if (S.col_fresh)
+            S.col_fresh = false;  // the frame-start energy's set: the same positions
+        else if ((rc = S.sw_valid ? collision_from_swept(s, S, st.x) : collision_at_frame_start(s, S, st.x)))
+            return rc;
         const size_t nslot = nb + 2 * (size_t)S.col.nij;
         if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
         ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
@@ -590,7 +616,7 @@ int step_scenes(hipStream_t s, Sim& S) {
         if ((rc = upload(s, dccd, S.hccd, sizeof(double) * ns))) return rc;
         if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dccd, st.Ak, st.pk))) return rc;
         if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-        if ((rc = swept_set(s, S, false))) return rc;
+        if ((rc = swept_set(s, S))) return rc;
         S.sw_valid = true;
         S.stats[8] += S.sw.npt;
         S.stats[9] += S.sw.nee;
