@@ -678,6 +678,8 @@ struct KeepArgs {
     int64_t blk0[kKeepLists + 1];  // first workgroup of each list in the launch
     int64_t* bcnt[kKeepLists];
     int64_t* boff[kKeepLists];
+    uint8_t* flag;                 // each entry's predicate (count pass), read by the write pass
+    int64_t flag0[kKeepLists];     // first flag of each list
 };
 
 __device__ __forceinline__ bool keep_pred(const KeepArgs& A, int k, int64_t i) {
@@ -739,7 +741,15 @@ __global__ __launch_bounds__(kBlock) void keep_kernel(KeepArgs A, int k0, int k1
     int64_t run = WRITE ? A.boff[k][blk] : 0;
     for (int it = 0; it < kItems; ++it) {
         const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
-        const bool pred = i < A.n[k] && keep_pred(A, k, i);
+        bool pred = false;
+        if (i < A.n[k]) {
+            if (WRITE) {
+                pred = A.flag[A.flag0[k] + i] != 0;
+            } else {
+                pred = keep_pred(A, k, i);
+                A.flag[A.flag0[k] + i] = pred ? 1 : 0;
+            }
+        }
         int tot;
         const int r = block_rank(pred, shw, &tot);
         if (WRITE) {
@@ -765,9 +775,11 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
     A.yth = 0.5 * dhat + ground;
     A.fg = (float)ground;
     const int64_t n[kKeepLists] = {in.nij, in.ji ? in.nij : 0, in.npt, in.nee, in.nvg};
-    int64_t nblk[kKeepLists], total = 0;
+    int64_t nblk[kKeepLists], total = 0, nflag = 0;
     A.blk0[0] = 0;
     for (int k = 0; k < kKeepLists; ++k) {
+        A.flag0[k] = nflag;
+        nflag += n[k];
         A.n[k] = n[k];
         nblk[k] = ceil_div(n[k], (int64_t)kBlock * kItems);
         A.blk0[k + 1] = A.blk0[k] + nblk[k];
@@ -786,6 +798,7 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
     // counts of all five lists in one launch, their scans (totals to dcount), then the
     // pairs (ij, ji: the new pair indices) and the tuples
     const int64_t nall = A.blk0[kKeepLists];
+    if ((rc = arena_get(kSlotKeepFlags, (size_t)nflag + 1, (void**)&A.flag))) return rc;
     if (nall > 0) {
         hipLaunchKernelGGL(keep_kernel<false>, dim3((unsigned)nall), dim3(kBlock), 0, s, A, 0, kKeepLists);
         ABD_CHECK_LAUNCH();

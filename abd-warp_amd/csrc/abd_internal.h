@@ -22,7 +22,7 @@ inline hipStream_t to_stream(abd_stream_t s) { return reinterpret_cast<hipStream
 // Per-device scratch arena: `slots` independent growable device buffers plus a
 // small pinned host buffer for count read-back.  Growing synchronizes the
 // device first, so a buffer still in use by queued work is never freed early.
-constexpr int kArenaSlots = 37;
+constexpr int kArenaSlots = 38;
 enum ArenaSlot {
     kSlotScanA = 0,
     kSlotScanB,
@@ -62,6 +62,7 @@ enum ArenaSlot {
     kSlotLdltStatus,  // direct solve: per-system status when the caller takes no iteration counts
     kSlotPairChunk,   // contact stage D: per-pair work-item counts and offsets
     kSlotPairPart,    // contact stage D: the work items' partial sums
+    kSlotKeepFlags,   // keep_from_superset: each entry's predicate
 };
 // Launch shapes that change no result (abd_set_tuning); process-wide.
 AbdTuning tuning();
@@ -108,9 +109,10 @@ int vg_cull_begin(hipStream_t s, const AbdScene* sc, const float* blo, const flo
 int vg_cull_fill(hipStream_t s, VgCull& C, int32_t* out, int64_t cap, int64_t n);
 
 // Swept boxes dilated at both ends (kind ABD_KIND_*, or -1 for body boxes): the union of
-// the dilated boxes at x0 and at x1.
+// the dilated boxes at x0 and at x1.  plo / phi (both or neither): the boxes at x0 with the
+// same dilation, already computed (read instead of recomputed; the same values).
 int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,
-                float* lo, float* hi);
+                float* lo, float* hi, const float* plo = nullptr, const float* phi = nullptr);
 // Vertex ranges of a candidate superset (n3 = 3 * vertices): lo = min(x0, x1) - skin,
 // hi = max(x0, x1) + skin per coordinate.  Boxes of any positions inside these ranges
 // lie inside the swept boxes of (lo, hi) (every box formula is monotone in the

@@ -313,10 +313,18 @@ int swept_set(hipStream_t s, Sim& S) {
     const AbdSimState& st = S.st;
     const int64_t n3 = 3 * (int64_t)S.nv;
     int rc;
-    if ((rc = swept_boxes(s, &S.sc, -1, st.x, st.xk, dh, S.tbody.lo.p, S.tbody.hi.p))) return rc;
-    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, st.x, st.xk, 0.0, S.ttri.lo.p, S.ttri.hi.p))) return rc;
-    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, st.x, st.xk, dh, S.tedge.lo.p, S.tedge.hi.p))) return rc;
-    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, st.x, st.xk, S.P.dhat, S.tpt.lo.p, S.tpt.hi.p))) return rc;
+    // the boxes at x: this iteration's collision set's (same dilations)
+    if ((rc = swept_boxes(s, &S.sc, -1, st.x, st.xk, dh, S.tbody.lo.p, S.tbody.hi.p, S.body.lo.p, S.body.hi.p)))
+        return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, st.x, st.xk, 0.0, S.ttri.lo.p, S.ttri.hi.p, S.tri.lo.p,
+                          S.tri.hi.p)))
+        return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, st.x, st.xk, dh, S.tedge.lo.p, S.tedge.hi.p, S.edge.lo.p,
+                          S.edge.hi.p)))
+        return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, st.x, st.xk, S.P.dhat, S.tpt.lo.p, S.tpt.hi.p, S.pt.lo.p,
+                          S.pt.hi.p)))
+        return rc;
     bool reuse = false;
     if (S.sup_valid) {
         ABD_CHECK(hipMemsetAsync(S.dcount.p + 7, 0, sizeof(int64_t), s));
