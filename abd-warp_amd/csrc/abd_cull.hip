@@ -764,7 +764,7 @@ __global__ __launch_bounds__(kBlock) void keep_kernel(KeepArgs A, int k0, int k1
 }  // namespace
 
 int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, const KeepBoxes& bx, double dhat,
-                       double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount) {
+                       double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount, bool sync) {
     KeepArgs A;
     A.in = in;
     A.bx = bx;
@@ -817,10 +817,11 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
         hipLaunchKernelGGL(keep_kernel<true>, dim3((unsigned)ntup), dim3(kBlock), 0, s, A, 2, kKeepLists);
         ABD_CHECK_LAUNCH();
     }
-    ABD_CHECK(hipMemcpyAsync(hcount, dcount, kKeepLists * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    ABD_CHECK(hipStreamSynchronize(s));
+    // an empty list's total is 0 (its scan segment is skipped)
     for (int k = 0; k < kKeepLists; ++k)
-        if (n[k] == 0) hcount[k] = 0;
+        if (n[k] == 0) ABD_CHECK(hipMemsetAsync(dcount + k, 0, sizeof(int64_t), s));
+    ABD_CHECK(hipMemcpyAsync(hcount, dcount, kKeepLists * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (sync) ABD_CHECK(hipStreamSynchronize(s));
     return ABD_OK;
 }
 

@@ -161,8 +161,9 @@ struct KeepBoxes {
 struct KeepOutput {
     int32_t *ij, *ji, *pt, *ee, *vg;
 };
+// sync false: the totals reach hcount with the stream's next synchronization (the caller's).
 int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, const KeepBoxes& bx, double dhat,
-                       double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount);
+                       double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount, bool sync = true);
 // abd_cull_bodies without the (j, i) order when ji == nullptr.
 int cull_bodies(hipStream_t s, const AbdScene* sc, const float* lo, const float* hi, int32_t* ij, int32_t* ji,
                 int64_t cap, int64_t* count);

@@ -90,6 +90,7 @@ struct Sim {
     DevBuf<double> vlo, vhi;         // its vertex ranges
     bool sup_valid = false;
     bool col_fresh = false;  // col (with its structure) is the collision set at the current x
+    bool sup_reused = false;  // the last swept set came from the superset
     DevBuf<int32_t> row_ptr, cols, slots;
     DevBuf<double> blocks;
     DevBuf<double> scal;  // [0] E contact, [1] E inertia, [2] toi, [3] dot, [4] resid
@@ -325,13 +326,34 @@ int swept_set(hipStream_t s, Sim& S) {
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, st.x, st.xk, S.P.dhat, S.tpt.lo.p, S.tpt.hi.p, S.pt.lo.p,
                           S.pt.hi.p)))
         return rc;
-    bool reuse = false;
+    const KeepBoxes bx{S.tbody.lo.p, S.tbody.hi.p, S.tpt.lo.p, S.tpt.hi.p, S.tedge.lo.p, S.tedge.hi.p, S.ttri.lo.p,
+                       S.ttri.hi.p};
+    // the superset filtered into sw (buffers taken after reserve_like, which may move them)
+    auto keep = [&](bool sync) {
+        const CandSet& W = S.sup;
+        const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg};
+        const KeepOutput o{S.sw.ij.p, S.sw.ji.p, S.sw.ptl.p, S.sw.eel.p, S.sw.vgl.p};
+        return keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, o, S.dcount.p, S.hcount, sync);
+    };
+    bool reuse = false, kept = false;
     if (S.sup_valid) {
+        // the range test; when the last step reused the superset, the filter from it runs
+        // behind the test, sharing its read-back (and is dropped if the step left the ranges)
         ABD_CHECK(hipMemsetAsync(S.dcount.p + 7, 0, sizeof(int64_t), s));
         if ((rc = within_ranges(s, n3, st.x, st.xk, S.vlo.p, S.vhi.p, S.dcount.p + 7))) return rc;
         ABD_CHECK(hipMemcpyAsync(S.hcount + 7, S.dcount.p + 7, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        if (S.sup_reused) {
+            if ((rc = S.sw.reserve_like(S.sup))) return rc;
+            if ((rc = keep(false))) return rc;
+            kept = true;
+        }
         ABD_CHECK(hipStreamSynchronize(s));
         reuse = S.hcount[7] == 0;
+    }
+    S.sup_reused = reuse;
+    if (reuse && !kept) {
+        if ((rc = S.sw.reserve_like(S.sup))) return rc;
+        if ((rc = keep(true))) return rc;
     }
     if (!reuse) {
         S.sup_valid = false;
@@ -345,14 +367,9 @@ int swept_set(hipStream_t s, Sim& S) {
             return rc;
         if ((rc = cull_lists(s, S, S.sup, S.sbody, S.spt, S.stri, S.sedge))) return rc;
         S.sup_valid = true;
+        if ((rc = S.sw.reserve_like(S.sup))) return rc;
+        if ((rc = keep(true))) return rc;
     }
-    if ((rc = S.sw.reserve_like(S.sup))) return rc;
-    const CandSet& W = S.sup;
-    const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg};
-    const KeepBoxes bx{S.tbody.lo.p, S.tbody.hi.p, S.tpt.lo.p, S.tpt.hi.p, S.tedge.lo.p, S.tedge.hi.p, S.ttri.lo.p,
-                       S.ttri.hi.p};
-    const KeepOutput out{S.sw.ij.p, S.sw.ji.p, S.sw.ptl.p, S.sw.eel.p, S.sw.vgl.p};
-    if ((rc = keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, out, S.dcount.p, S.hcount))) return rc;
     S.sw.nij = S.hcount[0];
     S.sw.npt = S.hcount[2];
     S.sw.nee = S.hcount[3];
