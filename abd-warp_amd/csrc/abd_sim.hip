@@ -314,18 +314,15 @@ int swept_set(hipStream_t s, Sim& S) {
     const AbdSimState& st = S.st;
     const int64_t n3 = 3 * (int64_t)S.nv;
     int rc;
-    // the boxes at x: this iteration's collision set's (same dilations)
+    // body boxes: the half at x is this iteration's collision set's (same dilation; reading
+    // it beats a second pass over every vertex).  Primitive boxes: recomputed at both ends
+    // (their vertices are shared and cached, so gathering them again costs less than
+    // streaming the stored boxes)
     if ((rc = swept_boxes(s, &S.sc, -1, st.x, st.xk, dh, S.tbody.lo.p, S.tbody.hi.p, S.body.lo.p, S.body.hi.p)))
         return rc;
-    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, st.x, st.xk, 0.0, S.ttri.lo.p, S.ttri.hi.p, S.tri.lo.p,
-                          S.tri.hi.p)))
-        return rc;
-    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, st.x, st.xk, dh, S.tedge.lo.p, S.tedge.hi.p, S.edge.lo.p,
-                          S.edge.hi.p)))
-        return rc;
-    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, st.x, st.xk, S.P.dhat, S.tpt.lo.p, S.tpt.hi.p, S.pt.lo.p,
-                          S.pt.hi.p)))
-        return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, st.x, st.xk, 0.0, S.ttri.lo.p, S.ttri.hi.p))) return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, st.x, st.xk, dh, S.tedge.lo.p, S.tedge.hi.p))) return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, st.x, st.xk, S.P.dhat, S.tpt.lo.p, S.tpt.hi.p))) return rc;
     const KeepBoxes bx{S.tbody.lo.p, S.tbody.hi.p, S.tpt.lo.p, S.tpt.hi.p, S.tedge.lo.p, S.tedge.hi.p, S.ttri.lo.p,
                        S.ttri.hi.p};
     // the superset filtered into sw (buffers taken after reserve_like, which may move them)
