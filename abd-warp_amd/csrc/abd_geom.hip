@@ -190,23 +190,14 @@ __global__ __launch_bounds__(kBlock) void body_boxes_kernel(int nb, const int32_
 // dilated box at x0 and at x1 (same R3 arithmetic as the boxes above).  Every pair whose
 // dilated boxes overlap at some x0 + t (x1 - x0), t in [0, 1], overlaps here, so the
 // lists culled from them hold every candidate of any collision set along the step.
-// plo / phi (nullable): the boxes at x0 already computed with the same dilation (the
-// collision set's), read instead of recomputed — the same values.
 __global__ __launch_bounds__(kBlock) void swept_point_boxes_kernel(const double* __restrict__ x0,
                                                                     const double* __restrict__ x1, int64_t n,
-                                                                    float dil, const float* __restrict__ plo,
-                                                                    const float* __restrict__ phi,
-                                                                    float* __restrict__ lo, float* __restrict__ hi) {
+                                                                    float dil, float* __restrict__ lo,
+                                                                    float* __restrict__ hi) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < 3 * n; i += (int64_t)gridDim.x * kBlock) {
-        const float c1 = (float)x1[i];
-        if (plo) {
-            lo[i] = fminf(plo[i], c1 - dil);
-            hi[i] = fmaxf(phi[i], c1 + dil);
-        } else {
-            const float c0 = (float)x0[i];
-            lo[i] = fminf(c0 - dil, c1 - dil);
-            hi[i] = fmaxf(c0 + dil, c1 + dil);
-        }
+        const float c0 = (float)x0[i], c1 = (float)x1[i];
+        lo[i] = fminf(c0 - dil, c1 - dil);
+        hi[i] = fmaxf(c0 + dil, c1 + dil);
     }
 }
 template <int NV>
@@ -215,9 +206,8 @@ __global__ __launch_bounds__(kBlock) void swept_prim_boxes_kernel(const int32_t*
                                                                    const int32_t* __restrict__ topo,
                                                                    const double* __restrict__ x0,
                                                                    const double* __restrict__ x1, int64_t n,
-                                                                   double dil, const float* __restrict__ plo,
-                                                                   const float* __restrict__ phi,
-                                                                   float* __restrict__ lo, float* __restrict__ hi) {
+                                                                   double dil, float* __restrict__ lo,
+                                                                   float* __restrict__ hi) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         int vb = voff[prim_body[i]];
         int64_t vi[NV];
@@ -225,33 +215,22 @@ __global__ __launch_bounds__(kBlock) void swept_prim_boxes_kernel(const int32_t*
         for (int j = 0; j < NV; ++j) vi[j] = vb + topo[NV * i + j];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            double mn1 = x1[3 * vi[0] + k], mx1 = mn1;
+            double mn0 = x0[3 * vi[0] + k], mx0 = mn0, mn1 = x1[3 * vi[0] + k], mx1 = mn1;
 #pragma unroll
             for (int j = 1; j < NV; ++j) {
-                const double c1 = x1[3 * vi[j] + k];
+                const double c0 = x0[3 * vi[j] + k], c1 = x1[3 * vi[j] + k];
+                mn0 = c0 < mn0 ? c0 : mn0;
+                mx0 = c0 > mx0 ? c0 : mx0;
                 mn1 = c1 < mn1 ? c1 : mn1;
                 mx1 = c1 > mx1 ? c1 : mx1;
             }
-            float l0, h0;
-            if (plo) {
-                l0 = plo[3 * i + k];
-                h0 = phi[3 * i + k];
-            } else {
-                double mn0 = x0[3 * vi[0] + k], mx0 = mn0;
-#pragma unroll
-                for (int j = 1; j < NV; ++j) {
-                    const double c0 = x0[3 * vi[j] + k];
-                    mn0 = c0 < mn0 ? c0 : mn0;
-                    mx0 = c0 > mx0 ? c0 : mx0;
-                }
-                l0 = (float)(mn0 - dil);
-                h0 = (float)(mx0 + dil);
-            }
-            lo[3 * i + k] = fminf(l0, (float)(mn1 - dil));
-            hi[3 * i + k] = fmaxf(h0, (float)(mx1 + dil));
+            lo[3 * i + k] = fminf((float)(mn0 - dil), (float)(mn1 - dil));
+            hi[3 * i + k] = fmaxf((float)(mx0 + dil), (float)(mx1 + dil));
         }
     }
 }
+// plo / phi (nullable): the body boxes at x0 already computed with the same dilation (the
+// collision set's), read instead of recomputed — the same values.
 __global__ __launch_bounds__(kBlock) void swept_body_boxes_kernel(int nb, const int32_t* __restrict__ voff,
                                                                    const double* __restrict__ x0,
                                                                    const double* __restrict__ x1, double dil,
@@ -357,19 +336,19 @@ int within_ranges(hipStream_t s, int64_t n3, const double* x0, const double* x1,
 
 int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,
                 float* lo, float* hi, const float* plo, const float* phi) {
-    if (!plo != !phi) return ABD_ERR_ARG;
+    if (!plo != !phi || (plo && kind != -1)) return ABD_ERR_ARG;
     if (kind == ABD_KIND_POINT) {
         if (!sc->n_verts) return ABD_OK;
         hipLaunchKernelGGL(swept_point_boxes_kernel, dim3(grid_for(3 * (int64_t)sc->n_verts)), dim3(kBlock), 0, s, x0,
-                           x1, (int64_t)sc->n_verts, (float)dil, plo, phi, lo, hi);
+                           x1, (int64_t)sc->n_verts, (float)dil, lo, hi);
     } else if (kind == ABD_KIND_EDGE) {
         if (!sc->n_edges) return ABD_OK;
         hipLaunchKernelGGL(swept_prim_boxes_kernel<2>, dim3(grid_for(sc->n_edges)), dim3(kBlock), 0, s,
-                           sc->edge_body, sc->voff, sc->edges, x0, x1, (int64_t)sc->n_edges, dil, plo, phi, lo, hi);
+                           sc->edge_body, sc->voff, sc->edges, x0, x1, (int64_t)sc->n_edges, dil, lo, hi);
     } else if (kind == ABD_KIND_TRI) {
         if (!sc->n_tris) return ABD_OK;
         hipLaunchKernelGGL(swept_prim_boxes_kernel<3>, dim3(grid_for(sc->n_tris)), dim3(kBlock), 0, s, sc->tri_body,
-                           sc->voff, sc->tris, x0, x1, (int64_t)sc->n_tris, dil, plo, phi, lo, hi);
+                           sc->voff, sc->tris, x0, x1, (int64_t)sc->n_tris, dil, lo, hi);
     } else if (kind == -1) {
         if (!sc->n_bodies) return ABD_OK;
         hipLaunchKernelGGL(swept_body_boxes_kernel, dim3((unsigned)ceil_div(sc->n_bodies, kBlock / kWave)),

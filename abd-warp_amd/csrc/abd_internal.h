@@ -109,8 +109,9 @@ int vg_cull_begin(hipStream_t s, const AbdScene* sc, const float* blo, const flo
 int vg_cull_fill(hipStream_t s, VgCull& C, int32_t* out, int64_t cap, int64_t n);
 
 // Swept boxes dilated at both ends (kind ABD_KIND_*, or -1 for body boxes): the union of
-// the dilated boxes at x0 and at x1.  plo / phi (both or neither): the boxes at x0 with the
-// same dilation, already computed (read instead of recomputed; the same values).
+// the dilated boxes at x0 and at x1.  plo / phi (body boxes only; both or neither): the
+// body boxes at x0 with the same dilation, already computed (read instead of recomputed;
+// the same values).
 int swept_boxes(hipStream_t s, const AbdScene* sc, int kind, const double* x0, const double* x1, double dil,
                 float* lo, float* hi, const float* plo = nullptr, const float* phi = nullptr);
 // Vertex ranges of a candidate superset (n3 = 3 * vertices): lo = min(x0, x1) - skin,
