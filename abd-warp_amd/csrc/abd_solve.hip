@@ -613,6 +613,8 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
     __shared__ int lvl[MB];                  // the level of each body
     __shared__ unsigned char mlist[MB][MB];  // mlist[k]: the columns above k in row k, ascending
     __shared__ int lrp[MB + 1], pos[MB + 1];
+    __shared__ int eslot[MB * MB];         // the system's CSR entries: slot (at most m^2 entries)
+    __shared__ unsigned char ecol[MB * MB];  // and local column (0xff: outside the system)
     __shared__ int cnode[MB], cstart[MB + 1], sstart[MB + 1], woff[MB];  // the chunk
     __shared__ int nlev, fail, nchunk;
     __shared__ unsigned long long chunkmask;
@@ -646,9 +648,15 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
     for (int i = t; i < n; i += nt) y[i] = b[i];
     __syncthreads();
     const int nnz = lrp[m];
+    if (nnz > MB * MB) {  // more entries than m^2 (blocks coupling other systems): declined
+        if (t == 0) status[sidx] = kLdltFailed;
+        return;
+    }
     for (int e = t; e < nnz; e += nt) {
         const int j = M.cols[e0 + e] - row0;
         const int i = last_le(lrp, m, e);
+        eslot[e] = M.slots[e0 + e];
+        ecol[e] = (unsigned)j < (unsigned)m ? (unsigned char)j : 0xff;
         if (j > i && j < m) atomicOr(&adj[i], 1ull << j);
     }
     __syncthreads();
@@ -656,8 +664,12 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
         unsigned long long mine = t < m ? adj[t] : 0ull, cm = 0ull;
         int h = 0;
         for (int k = 0; k < m; ++k) {
-            const unsigned long long ak = __shfl(mine, k);  // row k is final once k is reached
-            const int hk = __shfl(h, k);                     // so is its level
+            // row k is final once k is reached, and so is its level (lane k's values,
+            // read by the uniform lane index)
+            const unsigned lo32 = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)mine, k);
+            const unsigned hi32 = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(mine >> 32), k);
+            const unsigned long long ak = ((unsigned long long)hi32 << 32) | lo32;
+            const int hk = __builtin_amdgcn_readlane(h, k);
             const unsigned long long above = bits_above(ak, k);
             if ((above >> t) & 1ull) {
                 mine |= ak & (~0ull << t);
@@ -692,12 +704,12 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
     // the matrix's upper blocks (row-major 12x12) into the factor
     for (int64_t w = t; w < 144 * (int64_t)nnz; w += nt) {
         const int e = (int)(w / 144), q = (int)(w % 144);
-        const int j = M.cols[e0 + e] - row0;
+        const int j = ecol[e];
         const int i = last_le(lrp, m, e);
-        if (j < i || j >= m) continue;
+        if (j < i || j >= m) continue;  // lower, or outside the system (0xff)
         const int R = q / 12, Cq = q % 12;
         const int64_t blk = pos[i] + __popcll(adj[i] & bits_below(j));
-        U[144 * blk + q] = M.blocks[144 * (int64_t)M.slots[e0 + e] + 9 * (R / 3 + 4 * (Cq / 3)) + 3 * (R % 3) + Cq % 3];
+        U[144 * blk + q] = M.blocks[144 * (int64_t)eslot[e] + 9 * (R / 3 + 4 * (Cq / 3)) + 3 * (R % 3) + Cq % 3];
     }
     const int levels = nlev;
     unsigned long long rest = 0ull;  // thread 0: the bodies of the level not yet eliminated
