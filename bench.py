@@ -361,7 +361,7 @@ def main():
                    "frames_timed": f"{first}..{first + a.steps - 1}", "tiles": tiles},
         # PT + EE candidates through the four gh kernels per second of their event time
         # (attribution frames)
-        "gh_pairs_per_s": gh_cands / gh_s if gh_s > 0 else None,
+        "gh_pairs_per_s": gh_cands / gh_s if gh_s > 0 and gh_cands > 0 else None,
         "gh_candidates": cands,
         "gh_active": d["gh_active"],
         "newton_iters": iters,
@@ -376,6 +376,9 @@ def main():
         "kernels_ms_per_frame": {k: round(v[0] / n_attrib, 3) for k, v in aprof.items() if v[1]} if aprof else None,
         # the dominant kernel over the timed frames (HIP events)
         "timed_kernels": timed_kernels,
+        # algorithmic bytes per launch of every instrumented kernel (attribution frames)
+        "kernels_bytes_per_launch": {k: round(v[2] / v[1]) for k, v in aprof.items() if v[1] and v[2]}
+        if aprof else None,
         "attribution_frames": f"{a.warmup}..{first - 1}" if n_attrib else None,
     }
     print(json.dumps(out), flush=True)
