@@ -145,12 +145,12 @@ __host__ __device__ inline double first_alpha(double t, bool iterating) {
     return iterating ? t : 0.0;
 }
 
-// first trial of the batched line search: alpha[s] = first_alpha(toi[s]) for the scenes
-// iterating, -1 (no trial) for the others
+// first trial of the line search: alpha[s] = first_alpha(toi[s]) for the scenes iterating
+// (every scene when active is null), -1 (no trial) for the others
 __global__ void first_trial_kernel(int ns, const double* __restrict__ toi, const uint8_t* __restrict__ active,
                                    double* __restrict__ alpha) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < ns) alpha[q] = active[q] ? first_alpha(toi[q], true) : -1.0;
+    if (q < ns) alpha[q] = !active || active[q] ? first_alpha(toi[q], true) : -1.0;
 }
 
 // per-scene value rows of Sim::sv / Sim::hsv
@@ -414,15 +414,8 @@ int line_energy_multi(hipStream_t s, Sim& S, const double* alphas, int K, double
     return ABD_OK;
 }
 
-double dot_host(hipStream_t s, Sim& S, const double* a, const double* b, int* rc) {
-    *rc = abd_dot(s, a, b, 12 * (int64_t)S.nb, S.scal.p + 3);
-    if (*rc) return 0.0;
-    *rc = get_scalars(s, S, 3, 1);
-    return S.host[3];
-}
-
-// abd.py:278-303
-int ccd(hipStream_t s, Sim& S, double* alpha_cap) {
+// abd.py:278-303, launched only: the TOI of the full step lands in scal[2]
+int ccd(hipStream_t s, Sim& S) {
     int rc;
     const AbdSimState& st = S.st;
     // x = x(A, p) from the start of the iteration; xk: the end of the full step
@@ -439,8 +432,6 @@ int ccd(hipStream_t s, Sim& S, double* alpha_cap) {
     if ((rc = scene_toi(s, &S.sc, &S.P, 1, st.x, st.xk, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p,
                         S.sw.nvg, S.plan.p, S.scal.p + 2, nullptr)))
         return rc;
-    if ((rc = get_scalars(s, S, 2, 1))) return rc;
-    *alpha_cap = first_alpha(S.host[2], true);
     return ABD_OK;
 }
 
@@ -482,18 +473,39 @@ int step(hipStream_t s, Sim& S) {
         if ((rc = abd_bsr_solve(s, P.solver, nb, S.row_ptr.p, S.cols.p, S.slots.p, S.blocks.p, st.g, st.dq,
                                 P.solver_tol, P.solver_maxiter, 10, S.iters.p, S.scal.p + 4)))
             return rc;
-        double alpha_cap;
-        if ((rc = ccd(s, S, &alpha_cap))) return rc;
+        if ((rc = ccd(s, S))) return rc;
+        // q.g (abd.py:148) and g.g (abd.py:265): g and dq are final here
+        if ((rc = abd_dot(s, st.dq, st.g, 12 * (int64_t)nb, S.scal.p + 3))) return rc;
+        if ((rc = abd_dot(s, st.g, st.g, 12 * (int64_t)nb, S.scal.p + 5))) return rc;
+        // the line search's first trial straight behind the TOI (its step length from the TOI
+        // on the device); its energies, the TOI, both dots and the solver's iteration count
+        // come back in one read-back
+        hipLaunchKernelGGL(first_trial_kernel, dim3(1), dim3(kBlock), 0, s, 1, (const double*)(S.scal.p + 2),
+                           (const uint8_t*)nullptr, S.cand_alpha.p);
+        ABD_CHECK_LAUNCH();
+        if ((rc = line_trials(s, &S.sc, &P, 1, 1, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
+                              S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p,
+                              S.cand_E.p)))
+            return rc;
+        ABD_CHECK(hipMemcpyAsync(S.hcand + kLsSpec, S.cand_E.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipMemcpyAsync(S.host + 6, S.iters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        if ((rc = get_scalars(s, S, 2, 4))) return rc;
+        S.stats[5] += 1;
+        const double qTg = S.host[3], gTg = S.host[5];
+        int32_t sit;
+        memcpy(&sit, S.host + 6, sizeof(int32_t));
+        S.stats[6] += sit;
         // abd.py:146-160 line search (Armijo test as written)
-        const double qTg = dot_host(s, S, st.dq, st.g, &rc);
-        if (rc) return rc;
-        double alpha = alpha_cap, E1 = 0.0;
+        double alpha = first_alpha(S.host[2], true), E1 = 0.0;
         for (int pass = 0;; ++pass) {
             // the first trial alone (usually accepted), then the rest down to alpha_min
             const int K = pass == 0 ? 1 : trials_to_floor(alpha, P.alpha_min);
             double trial[kLsSpec], Ek[kLsSpec];
             for (int k = 0; k < K; ++k) trial[k] = k == 0 ? alpha : trial[k - 1] * 0.5;
-            if ((rc = line_energy_multi(s, S, trial, K, Ek))) return rc;
+            if (pass == 0)
+                Ek[0] = S.hcand[kLsSpec] + S.hcand[kLsSpec + 1];
+            else if ((rc = line_energy_multi(s, S, trial, K, Ek)))
+                return rc;
             int acc = -1;
             for (int k = 0; k < K && acc < 0; ++k) {
                 const bool wolfe = Ek[k] < E0 + P.c1 * trial[k] * qTg;
@@ -510,14 +522,6 @@ int step(hipStream_t s, Sim& S) {
         // q and (Ak, pk) = q - alpha dq, the accepted trial
         if ((rc = update_q_dual(s, nb, st.A, st.p, st.dq, alpha, st.A, st.p, st.Ak, st.pk))) return rc;
         ++it;
-        // g.g (abd.py:265) and the solver's iteration count in one read-back
-        if ((rc = abd_dot(s, st.g, st.g, 12 * (int64_t)nb, S.scal.p + 3))) return rc;
-        ABD_CHECK(hipMemcpyAsync(S.host + 6, S.iters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        if ((rc = get_scalars(s, S, 3, 1))) return rc;
-        const double gTg = S.host[3];
-        int32_t sit;
-        memcpy(&sit, S.host + 6, sizeof(int32_t));
-        S.stats[6] += sit;
         if (gTg < P.tol || it >= P.max_iter) break;
     }
     if ((rc = abd_update_q0qdot(s, nb, P.dt, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot))) return rc;
@@ -650,14 +654,18 @@ int step_scenes(hipStream_t s, Sim& S) {
                             S.sw.nvg, S.plan.p, S.sv.p + kSvToi * ns, S.sactive.p)))
             return rc;
         if ((rc = abd_scene_dot(s, &S.sc, st.dq, st.g, S.sv.p + kSvQtg * ns))) return rc;
+        // g.g of the stopping test (abd.py:265): g is final here
+        if ((rc = abd_scene_dot(s, &S.sc, st.g, st.g, S.sv.p + kSvGtg * ns))) return rc;
         // the line search's first trial straight behind the TOI (step lengths from it on the
-        // device), read back with the TOI and q.g
+        // device), read back with the TOI, q.g, g.g and the solver's iteration counts
         hipLaunchKernelGGL(first_trial_kernel, dim3((unsigned)ceil_div(ns, kBlock)), dim3(kBlock), 0, s, ns,
                            (const double*)(S.sv.p + kSvToi * ns), (const uint8_t*)S.sactive.p, S.cand_alpha.p);
         ABD_CHECK_LAUNCH();
         if ((rc = line_energy_scenes_launch(s, S, S.cand_alpha.p, 1))) return rc;
-        ABD_CHECK(hipMemcpyAsync(S.hsv + kSvToi * ns, S.sv.p + kSvToi * ns, sizeof(double) * 2 * ns,
+        static_assert(kSvQtg == kSvToi + 1 && kSvGtg == kSvToi + 2, "rows read back together");
+        ABD_CHECK(hipMemcpyAsync(S.hsv + kSvToi * ns, S.sv.p + kSvToi * ns, sizeof(double) * 3 * ns,
                                  hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipMemcpyAsync(S.hiters, S.iters.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
         ABD_CHECK(hipStreamSynchronize(s));
         for (int k = 0; k < ns; ++k) {
             ls[k] = newton[k];
@@ -713,11 +721,8 @@ int step_scenes(hipStream_t s, Sim& S) {
         if ((rc = upload(s, dalpha, S.halpha, sizeof(double) * ns))) return rc;
         // q and (Ak, pk) = the accepted trial of every scene
         if ((rc = update_q_scenes_dual(s, &S.sc, st.A, st.p, st.dq, dalpha, st.A, st.p, st.Ak, st.pk))) return rc;
-        if ((rc = abd_scene_dot(s, &S.sc, st.g, st.g, S.sv.p + kSvGtg * ns))) return rc;
-        ABD_CHECK(hipMemcpyAsync(S.hiters, S.iters.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
-        ABD_CHECK(hipMemcpyAsync(S.hsv + kSvGtg * ns, S.sv.p + kSvGtg * ns, sizeof(double) * ns,
-                                 hipMemcpyDeviceToHost, s));
-        ABD_CHECK(hipStreamSynchronize(s));
+        // the stopping test on the values read back with the TOI (no round trip here: the next
+        // iteration's first host write to a buffer uploaded above follows a synchronization)
         for (int k = 0; k < ns; ++k) {
             if (!newton[k]) continue;
             E0[k] = E1[k];
@@ -732,6 +737,8 @@ int step_scenes(hipStream_t s, Sim& S) {
     }
     if ((rc = abd_update_q0qdot(s, nb, P.dt, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.A0, st.p0, st.x_view))) return rc;
+    // the last accepted steps' upload reads halpha, which the next frame's start rewrites
+    ABD_CHECK(hipStreamSynchronize(s));
     S.stats[7] += 1;
     return ABD_OK;
 }
