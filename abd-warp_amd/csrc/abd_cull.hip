@@ -680,7 +680,44 @@ struct KeepArgs {
     int64_t* boff[kKeepLists];
     uint8_t* flag;                 // each entry's predicate (count pass), read by the write pass
     int64_t flag0[kKeepLists];     // first flag of each list
+    // primitive boxes from the vertices when x is set (KeepAtX)
+    const double* x;
+    const int32_t *tris, *edges;
+    float pdil;
+    double edil, tdil;
 };
+
+// point_boxes_kernel's box of vertex v
+__device__ __forceinline__ void point_box_at(const double* __restrict__ x, int64_t v, float dil, float* lo,
+                                             float* hi) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float c = (float)x[3 * v + k];
+        lo[k] = c - dil;
+        hi[k] = c + dil;
+    }
+}
+
+// prim_boxes_kernel's box of primitive i (NV vertices from topo, numbered from vb)
+template <int NV>
+__device__ __forceinline__ void prim_box_at(const double* __restrict__ x, const int32_t* __restrict__ topo,
+                                            int64_t i, int vb, double dil, float* lo, float* hi) {
+    int64_t vi[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) vi[j] = vb + topo[NV * i + j];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double mn = x[3 * vi[0] + k], mx = mn;
+#pragma unroll
+        for (int j = 1; j < NV; ++j) {
+            const double c = x[3 * vi[j] + k];
+            mn = c < mn ? c : mn;
+            mx = c > mx ? c : mx;
+        }
+        lo[k] = (float)(mn - dil);
+        hi[k] = (float)(mx + dil);
+    }
+}
 
 __device__ __forceinline__ bool keep_pred(const KeepArgs& A, int k, int64_t i) {
     const KeepBoxes& B = A.bx;
@@ -695,6 +732,18 @@ __device__ __forceinline__ bool keep_pred(const KeepArgs& A, int k, int64_t i) {
         if (!overlap(B.blo + 3 * I, B.bhi + 3 * I, B.blo + 3 * J, B.bhi + 3 * J)) return false;
         const int64_t a = (k == 2 ? A.voff : A.eoff)[I] + (int64_t)t5[3];
         const int64_t b = (k == 2 ? A.foff : A.eoff)[J] + (int64_t)t5[4];
+        if (A.x) {
+            // a: a point of I (PT) or an edge of I; b: a triangle or an edge of J
+            float l1[3], h1[3], l2[3], h2[3];
+            if (k == 2) {
+                point_box_at(A.x, a, A.pdil, l1, h1);
+                prim_box_at<3>(A.x, A.tris, b, A.voff[J], A.tdil, l2, h2);
+            } else {
+                prim_box_at<2>(A.x, A.edges, a, A.voff[I], A.edil, l1, h1);
+                prim_box_at<2>(A.x, A.edges, b, A.voff[J], A.edil, l2, h2);
+            }
+            return overlap(l1, h1, l2, h2);
+        }
         const float *lo1 = k == 2 ? B.plo : B.elo, *hi1 = k == 2 ? B.phi : B.ehi;
         const float *lo2 = k == 2 ? B.tlo : B.elo, *hi2 = k == 2 ? B.thi : B.ehi;
         return overlap(lo1 + 3 * a, hi1 + 3 * a, lo2 + 3 * b, hi2 + 3 * b);
@@ -704,7 +753,13 @@ __device__ __forceinline__ bool keep_pred(const KeepArgs& A, int k, int64_t i) {
     const float qhi[3] = {1e3f, A.fg, 1e3f};
     const int b = A.in.vg[2 * i];
     const int64_t v = A.voff[b] + (int64_t)A.in.vg[2 * i + 1];
-    return (double)B.blo[3 * b + 1] < A.yth && overlap(B.plo + 3 * v, B.phi + 3 * v, qlo, qhi);
+    if (!((double)B.blo[3 * b + 1] < A.yth)) return false;
+    if (A.x) {
+        float l[3], h[3];
+        point_box_at(A.x, v, A.pdil, l, h);
+        return overlap(l, h, qlo, qhi);
+    }
+    return overlap(B.plo + 3 * v, B.phi + 3 * v, qlo, qhi);
 }
 
 __device__ __forceinline__ void keep_write(const KeepArgs& A, int k, int64_t i, int64_t o) {
@@ -738,33 +793,60 @@ __global__ __launch_bounds__(kBlock) void keep_kernel(KeepArgs A, int k0, int k1
     while (k + 1 < k1 && g >= A.blk0[k + 1]) ++k;
     const int64_t blk = g - A.blk0[k];
     const int64_t base = blk * kBlock * kItems;
-    int64_t run = WRITE ? A.boff[k][blk] : 0;
-    for (int it = 0; it < kItems; ++it) {
-        const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
-        bool pred = false;
-        if (i < A.n[k]) {
-            if (WRITE) {
-                pred = A.flag[A.flag0[k] + i] != 0;
-            } else {
+    if constexpr (!WRITE) {
+        // the count: kept entries of the block (each entry's predicate stored for the write
+        // pass), summed item by item (a single sum at the end measured no faster: the
+        // predicates' box gathers bound the pass)
+        int64_t run = 0;
+        for (int it = 0; it < kItems; ++it) {
+            const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
+            bool pred = false;
+            if (i < A.n[k]) {
                 pred = keep_pred(A, k, i);
                 A.flag[A.flag0[k] + i] = pred ? 1 : 0;
             }
+            int tot;
+            block_rank(pred, shw, &tot);
+            run += tot;
         }
-        int tot;
-        const int r = block_rank(pred, shw, &tot);
-        if (WRITE) {
-            if (pred) keep_write(A, k, i, run + r);
-            if (k == 0 && i < A.n[0]) A.remap[i] = pred ? (int32_t)(run + r) : -1;
+        if (threadIdx.x == 0) A.bcnt[k][blk] = run;
+    } else {
+        // every item's per-wave counts in one exchange; an entry's rank = the kept entries of
+        // the earlier items, of the earlier waves in its item and of the lower lanes
+        constexpr int kW = kBlock / kWave;
+        __shared__ int cnt[kItems][kW];
+        const int lane = lane_id(), w = wave_id();
+        const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+        unsigned long long m[kItems];
+        for (int it = 0; it < kItems; ++it) {
+            const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
+            m[it] = __ballot(i < A.n[k] && A.flag[A.flag0[k] + i] != 0 ? 1 : 0);
+            if (lane == 0) cnt[it][w] = __popcll(m[it]);
         }
-        run += tot;
+        __syncthreads();
+        int64_t run = A.boff[k][blk];
+        for (int it = 0; it < kItems; ++it) {
+            const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
+            int off = 0, tot = 0;
+            for (int v = 0; v < kW; ++v) {
+                const int c = cnt[it][v];
+                off += v < w ? c : 0;
+                tot += c;
+            }
+            const bool pred = (m[it] >> lane) & 1ull;
+            const int64_t r = run + off + __popcll(m[it] & below);
+            if (pred) keep_write(A, k, i, r);
+            if (k == 0 && i < A.n[0]) A.remap[i] = pred ? (int32_t)r : -1;
+            run += tot;
+        }
     }
-    if (!WRITE && threadIdx.x == 0) A.bcnt[k][blk] = run;
 }
 
 }  // namespace
 
 int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, const KeepBoxes& bx, double dhat,
-                       double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount, bool sync) {
+                       double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount, bool sync,
+                       const KeepAtX* at) {
     KeepArgs A;
     A.in = in;
     A.bx = bx;
@@ -772,6 +854,12 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
     A.voff = sc->voff;
     A.foff = sc->foff;
     A.eoff = sc->eoff;
+    A.x = at ? at->x : nullptr;
+    A.tris = sc->tris;
+    A.edges = sc->edges;
+    A.pdil = at ? (float)at->pdil : 0.0f;
+    A.edil = at ? at->edil : 0.0;
+    A.tdil = at ? at->tdil : 0.0;
     A.yth = 0.5 * dhat + ground;
     A.fg = (float)ground;
     const int64_t n[kKeepLists] = {in.nij, in.ji ? in.nij : 0, in.npt, in.nee, in.nvg};

@@ -162,9 +162,18 @@ struct KeepBoxes {
 struct KeepOutput {
     int32_t *ij, *ji, *pt, *ee, *vg;
 };
+// The primitive boxes of the predicates computed per entry from the vertices x instead of
+// read from KeepBoxes (whose point, edge and triangle boxes are then unused): the box
+// kernels' own expressions, points (float)x -+ (float)pdil, edges and triangles the FP64
+// min / max of their vertices -+ edil / tdil, cast to FP32.
+struct KeepAtX {
+    const double* x;
+    double pdil, edil, tdil;
+};
 // sync false: the totals reach hcount with the stream's next synchronization (the caller's).
 int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, const KeepBoxes& bx, double dhat,
-                       double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount, bool sync = true);
+                       double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount, bool sync = true,
+                       const KeepAtX* at = nullptr);
 // abd_cull_bodies without the (j, i) order when ji == nullptr.
 int cull_bodies(hipStream_t s, const AbdScene* sc, const float* lo, const float* hi, int32_t* ij, int32_t* ji,
                 int64_t cap, int64_t* count);

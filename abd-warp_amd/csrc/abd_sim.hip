@@ -273,16 +273,17 @@ int collision_at_frame_start(hipStream_t s, Sim& S, const double* xv) {
 int collision_from(hipStream_t s, Sim& S, const CandSet& W, const double* xv) {
     const double dh = S.P.dhat * 0.5;
     int rc;
+    // the body boxes at x (also the start half of the next swept body boxes); the primitive
+    // boxes of collision_set's dilations are evaluated per entry of W by the filter: W holds
+    // far fewer entries than the scene has primitives
     if ((rc = abd_body_boxes(s, &S.sc, xv, dh, S.body.lo.p, S.body.hi.p))) return rc;
-    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_TRI, xv, 0.0, S.tri.lo.p, S.tri.hi.p))) return rc;
-    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_EDGE, xv, dh, S.edge.lo.p, S.edge.hi.p))) return rc;
-    if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_POINT, xv, S.P.dhat, S.pt.lo.p, S.pt.hi.p))) return rc;
     if ((rc = S.col.reserve_like(W))) return rc;
     const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg};
-    const KeepBoxes bx{S.body.lo.p, S.body.hi.p, S.pt.lo.p, S.pt.hi.p, S.edge.lo.p, S.edge.hi.p, S.tri.lo.p,
-                       S.tri.hi.p};
+    const KeepBoxes bx{S.body.lo.p, S.body.hi.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     const KeepOutput out{S.col.ij.p, S.col.ji.p, S.col.ptl.p, S.col.eel.p, S.col.vgl.p};
-    if ((rc = keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, out, S.dcount.p, S.hcount))) return rc;
+    const KeepAtX at{xv, S.P.dhat, dh, 0.0};
+    if ((rc = keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, out, S.dcount.p, S.hcount, true, &at)))
+        return rc;
     S.col.nij = S.hcount[0];
     S.col.npt = S.hcount[2];
     S.col.nee = S.hcount[3];

@@ -37,6 +37,8 @@ def test_small_two_cubes_100_frames(native):
     # A came to rest on B: gap of about dhat, no interpenetration
     assert 0.0 < sim.states.p[1, 1].item() - sim.states.p[0, 1].item() - 1.0 < 2e-3
     assert sim.stats["newton_iters"] == ref.stats[0]
+    # every Newton iteration's collision set is the culling's (PT, EE, VG sizes summed)
+    assert [sim.stats[k] for k in ("gh_pt", "gh_ee", "gh_vg")] == [int(v) for v in ref.stats[1:4]]
 
 
 @pytest.mark.parametrize("native", [False, True])
@@ -48,6 +50,9 @@ def test_medium_subset_frames(native):
     cp = ContactParams(ground=spec.ground, solver_tol=1e-12)
     sim, ref, worst = _run(spec, cp, 8, native)
     assert sim.stats["gh_pt"] + sim.stats["gh_ee"] > 0
+    # the same Newton iterations over the same collision sets (PT, EE, VG sizes summed)
+    assert sim.stats["newton_iters"] == ref.stats[0]
+    assert [sim.stats[k] for k in ("gh_pt", "gh_ee", "gh_vg")] == [int(v) for v in ref.stats[1:4]]
 
 
 def test_small_frames_match_golden_fixture():
