@@ -681,27 +681,36 @@ struct KeepArgs {
     uint8_t* flag;                 // each entry's predicate (count pass), read by the write pass
     int64_t flag0[kKeepLists];     // first flag of each list
     // primitive boxes from the vertices when x is set (KeepAtX)
-    const double* x;
+    const double *x, *x1;
     const int32_t *tris, *edges;
     float pdil;
     double edil, tdil;
 };
 
-// point_boxes_kernel's box of vertex v
-__device__ __forceinline__ void point_box_at(const double* __restrict__ x, int64_t v, float dil, float* lo,
-                                             float* hi) {
+// point_boxes_kernel's box of vertex v at x, or with x1 set swept_point_boxes_kernel's
+// (the union with the box at x1)
+__device__ __forceinline__ void point_box_at(const double* __restrict__ x, const double* __restrict__ x1,
+                                             int64_t v, float dil, float* lo, float* hi) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float c = (float)x[3 * v + k];
-        lo[k] = c - dil;
-        hi[k] = c + dil;
+        if (x1) {
+            const float c1 = (float)x1[3 * v + k];
+            lo[k] = fminf(c - dil, c1 - dil);
+            hi[k] = fmaxf(c + dil, c1 + dil);
+        } else {
+            lo[k] = c - dil;
+            hi[k] = c + dil;
+        }
     }
 }
 
-// prim_boxes_kernel's box of primitive i (NV vertices from topo, numbered from vb)
+// prim_boxes_kernel's box of primitive i (NV vertices from topo, numbered from vb) at x, or
+// with x1 set swept_prim_boxes_kernel's
 template <int NV>
-__device__ __forceinline__ void prim_box_at(const double* __restrict__ x, const int32_t* __restrict__ topo,
-                                            int64_t i, int vb, double dil, float* lo, float* hi) {
+__device__ __forceinline__ void prim_box_at(const double* __restrict__ x, const double* __restrict__ x1,
+                                            const int32_t* __restrict__ topo, int64_t i, int vb, double dil,
+                                            float* lo, float* hi) {
     int64_t vi[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) vi[j] = vb + topo[NV * i + j];
@@ -714,8 +723,20 @@ __device__ __forceinline__ void prim_box_at(const double* __restrict__ x, const 
             mn = c < mn ? c : mn;
             mx = c > mx ? c : mx;
         }
-        lo[k] = (float)(mn - dil);
-        hi[k] = (float)(mx + dil);
+        float l = (float)(mn - dil), h = (float)(mx + dil);
+        if (x1) {
+            double mn1 = x1[3 * vi[0] + k], mx1 = mn1;
+#pragma unroll
+            for (int j = 1; j < NV; ++j) {
+                const double c = x1[3 * vi[j] + k];
+                mn1 = c < mn1 ? c : mn1;
+                mx1 = c > mx1 ? c : mx1;
+            }
+            l = fminf(l, (float)(mn1 - dil));
+            h = fmaxf(h, (float)(mx1 + dil));
+        }
+        lo[k] = l;
+        hi[k] = h;
     }
 }
 
@@ -736,11 +757,11 @@ __device__ __forceinline__ bool keep_pred(const KeepArgs& A, int k, int64_t i) {
             // a: a point of I (PT) or an edge of I; b: a triangle or an edge of J
             float l1[3], h1[3], l2[3], h2[3];
             if (k == 2) {
-                point_box_at(A.x, a, A.pdil, l1, h1);
-                prim_box_at<3>(A.x, A.tris, b, A.voff[J], A.tdil, l2, h2);
+                point_box_at(A.x, A.x1, a, A.pdil, l1, h1);
+                prim_box_at<3>(A.x, A.x1, A.tris, b, A.voff[J], A.tdil, l2, h2);
             } else {
-                prim_box_at<2>(A.x, A.edges, a, A.voff[I], A.edil, l1, h1);
-                prim_box_at<2>(A.x, A.edges, b, A.voff[J], A.edil, l2, h2);
+                prim_box_at<2>(A.x, A.x1, A.edges, a, A.voff[I], A.edil, l1, h1);
+                prim_box_at<2>(A.x, A.x1, A.edges, b, A.voff[J], A.edil, l2, h2);
             }
             return overlap(l1, h1, l2, h2);
         }
@@ -756,7 +777,7 @@ __device__ __forceinline__ bool keep_pred(const KeepArgs& A, int k, int64_t i) {
     if (!((double)B.blo[3 * b + 1] < A.yth)) return false;
     if (A.x) {
         float l[3], h[3];
-        point_box_at(A.x, v, A.pdil, l, h);
+        point_box_at(A.x, A.x1, v, A.pdil, l, h);
         return overlap(l, h, qlo, qhi);
     }
     return overlap(B.plo + 3 * v, B.phi + 3 * v, qlo, qhi);
@@ -855,6 +876,7 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
     A.foff = sc->foff;
     A.eoff = sc->eoff;
     A.x = at ? at->x : nullptr;
+    A.x1 = at ? at->x1 : nullptr;
     A.tris = sc->tris;
     A.edges = sc->edges;
     A.pdil = at ? (float)at->pdil : 0.0f;

@@ -165,9 +165,10 @@ struct KeepOutput {
 // The primitive boxes of the predicates computed per entry from the vertices x instead of
 // read from KeepBoxes (whose point, edge and triangle boxes are then unused): the box
 // kernels' own expressions, points (float)x -+ (float)pdil, edges and triangles the FP64
-// min / max of their vertices -+ edil / tdil, cast to FP32.
+// min / max of their vertices -+ edil / tdil, cast to FP32; with x1 set, the swept boxes'
+// (the union of the boxes at x and at x1).
 struct KeepAtX {
-    const double* x;
+    const double *x, *x1;
     double pdil, edil, tdil;
 };
 // sync false: the totals reach hcount with the stream's next synchronization (the caller's).

@@ -77,7 +77,7 @@ struct Sim {
     AbdSimState st;
     int nb, nv, ne, nf;
     BoxBuf body, tri, edge, pt;         // boxes of the last collision_set
-    BoxBuf tbody, ttri, tedge, tpt;     // trajectory boxes
+    BoxBuf tbody;                       // swept body boxes (the primitives' are evaluated per entry)
     // the collision set at xk (gh, BSR structure) and the swept set of the last CCD step
     // (CCD, line search; the next iteration's collision set is filtered from it)
     CandSet col, sw;
@@ -281,7 +281,7 @@ int collision_from(hipStream_t s, Sim& S, const CandSet& W, const double* xv) {
     const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg};
     const KeepBoxes bx{S.body.lo.p, S.body.hi.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     const KeepOutput out{S.col.ij.p, S.col.ji.p, S.col.ptl.p, S.col.eel.p, S.col.vgl.p};
-    const KeepAtX at{xv, S.P.dhat, dh, 0.0};
+    const KeepAtX at{xv, nullptr, S.P.dhat, dh, 0.0};
     if ((rc = keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, out, S.dcount.p, S.hcount, true, &at)))
         return rc;
     S.col.nij = S.hcount[0];
@@ -316,22 +316,18 @@ int swept_set(hipStream_t s, Sim& S) {
     const int64_t n3 = 3 * (int64_t)S.nv;
     int rc;
     // body boxes: the half at x is this iteration's collision set's (same dilation; reading
-    // it beats a second pass over every vertex).  Primitive boxes: recomputed at both ends
-    // (their vertices are shared and cached, so gathering them again costs less than
-    // streaming the stored boxes)
+    // it beats a second pass over every vertex).  Primitive swept boxes: evaluated by the
+    // filter per entry of the superset from both ends (far fewer entries than primitives)
     if ((rc = swept_boxes(s, &S.sc, -1, st.x, st.xk, dh, S.tbody.lo.p, S.tbody.hi.p, S.body.lo.p, S.body.hi.p)))
         return rc;
-    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, st.x, st.xk, 0.0, S.ttri.lo.p, S.ttri.hi.p))) return rc;
-    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, st.x, st.xk, dh, S.tedge.lo.p, S.tedge.hi.p))) return rc;
-    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, st.x, st.xk, S.P.dhat, S.tpt.lo.p, S.tpt.hi.p))) return rc;
-    const KeepBoxes bx{S.tbody.lo.p, S.tbody.hi.p, S.tpt.lo.p, S.tpt.hi.p, S.tedge.lo.p, S.tedge.hi.p, S.ttri.lo.p,
-                       S.ttri.hi.p};
+    const KeepBoxes bx{S.tbody.lo.p, S.tbody.hi.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const KeepAtX at{st.x, st.xk, S.P.dhat, dh, 0.0};
     // the superset filtered into sw (buffers taken after reserve_like, which may move them)
     auto keep = [&](bool sync) {
         const CandSet& W = S.sup;
         const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg};
         const KeepOutput o{S.sw.ij.p, S.sw.ji.p, S.sw.ptl.p, S.sw.eel.p, S.sw.vgl.p};
-        return keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, o, S.dcount.p, S.hcount, sync);
+        return keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, o, S.dcount.p, S.hcount, sync, &at);
     };
     bool reuse = false, kept = false;
     if (S.sup_valid) {
@@ -769,11 +765,8 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
     for (BoxBuf* b : b4)
         if (!rc) rc = b->reserve(nb);
     if (!rc) rc = S->tri.reserve(S->nf + 1);
-    if (!rc) rc = S->ttri.reserve(S->nf + 1);
     if (!rc) rc = S->edge.reserve(S->ne + 1);
-    if (!rc) rc = S->tedge.reserve(S->ne + 1);
     if (!rc) rc = S->pt.reserve(S->nv);
-    if (!rc) rc = S->tpt.reserve(S->nv);
     if (!rc) rc = S->sbody.reserve(nb);
     if (!rc) rc = S->stri.reserve(S->nf + 1);
     if (!rc) rc = S->sedge.reserve(S->ne + 1);
@@ -936,7 +929,7 @@ int abd_sim_destroy(void* sim) {
     if (!sim) return ABD_OK;
     Sim* S = static_cast<Sim*>(sim);
     (void)hipDeviceSynchronize();  // queued work may still use the buffers released below
-    for (BoxBuf* b : {&S->body, &S->tri, &S->edge, &S->pt, &S->tbody, &S->ttri, &S->tedge, &S->tpt, &S->sbody,
+    for (BoxBuf* b : {&S->body, &S->tri, &S->edge, &S->pt, &S->tbody, &S->sbody,
                       &S->stri, &S->sedge, &S->spt}) {
         b->lo.release();
         b->hi.release();
