@@ -30,7 +30,7 @@ __device__ __forceinline__ bool overlap(const float* __restrict__ alo, const flo
 }
 
 // ---------------------------------------------------------------- bodies
-constexpr int kItems = 8;      // entries per thread in the vertex-ground and keep compactions
+constexpr int kItems = 8;      // entries per thread in the vertex-ground compaction
 constexpr int kPairItems = 1;  // pairs per thread in the body-pair compaction: enough workgroups to fill the device
 
 // k -> (i, j), i < j.  Order 0: row-major (i, j); order 1: sorted by (j, i).
@@ -675,7 +675,7 @@ struct KeepArgs {
     double yth;
     float fg;
     int64_t n[kKeepLists];
-    int64_t blk0[kKeepLists + 1];  // first workgroup of each list in the launch
+    int64_t blk0[kKeepLists + 1];  // first workgroup (block of kBlock entries) of each list
     int64_t* bcnt[kKeepLists];
     int64_t* boff[kKeepLists];
     uint8_t* flag;                 // each entry's predicate (count pass), read by the write pass
@@ -803,64 +803,46 @@ __device__ __forceinline__ void keep_write(const KeepArgs& A, int k, int64_t i, 
     }
 }
 
-// One pass over lists [k0, k1): workgroup -> (list, block of kBlock * kItems entries).
-// Count (WRITE false): kept entries per block.  Write: entries at their ordered ranks;
-// list 0 also records every pair's new index (-1 when dropped) in remap.
-template <bool WRITE>
-__global__ __launch_bounds__(kBlock) void keep_kernel(KeepArgs A, int k0, int k1) {
+// The count: one entry per thread (its predicate's gathers need many waves in flight to
+// hide their latency), the kept entries of each workgroup's kBlock entries to bcnt; each
+// entry's predicate is stored for the write pass.
+__global__ __launch_bounds__(kBlock) void keep_count_kernel(KeepArgs A) {
+    __shared__ int shw[kBlock / kWave];
+    const int64_t g = blockIdx.x;
+    int k = 0;
+    while (k + 1 < kKeepLists && g >= A.blk0[k + 1]) ++k;
+    const int64_t blk = g - A.blk0[k];
+    const int64_t i = blk * kBlock + threadIdx.x;
+    bool pred = false;
+    if (i < A.n[k]) {
+        pred = keep_pred(A, k, i);
+        A.flag[A.flag0[k] + i] = pred ? 1 : 0;
+    }
+    const int c = __popcll(__ballot(pred ? 1 : 0));
+    if (lane_id() == 0) shw[wave_id()] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += shw[w];
+        A.bcnt[k][blk] = t;
+    }
+}
+
+// The write pass over lists [k0, k1), one entry per thread: the kept entries at their
+// ordered ranks (the block's scanned offset + the rank in the block); list 0 also records
+// every pair's new index (-1 when dropped) in remap.
+__global__ __launch_bounds__(kBlock) void keep_write_kernel(KeepArgs A, int k0, int k1) {
     __shared__ int shw[kBlock / kWave];
     const int64_t g = (int64_t)blockIdx.x + A.blk0[k0];
     int k = k0;
     while (k + 1 < k1 && g >= A.blk0[k + 1]) ++k;
     const int64_t blk = g - A.blk0[k];
-    const int64_t base = blk * kBlock * kItems;
-    if constexpr (!WRITE) {
-        // the count: kept entries of the block (each entry's predicate stored for the write
-        // pass), summed item by item (a single sum at the end measured no faster: the
-        // predicates' box gathers bound the pass)
-        int64_t run = 0;
-        for (int it = 0; it < kItems; ++it) {
-            const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
-            bool pred = false;
-            if (i < A.n[k]) {
-                pred = keep_pred(A, k, i);
-                A.flag[A.flag0[k] + i] = pred ? 1 : 0;
-            }
-            int tot;
-            block_rank(pred, shw, &tot);
-            run += tot;
-        }
-        if (threadIdx.x == 0) A.bcnt[k][blk] = run;
-    } else {
-        // every item's per-wave counts in one exchange; an entry's rank = the kept entries of
-        // the earlier items, of the earlier waves in its item and of the lower lanes
-        constexpr int kW = kBlock / kWave;
-        __shared__ int cnt[kItems][kW];
-        const int lane = lane_id(), w = wave_id();
-        const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
-        unsigned long long m[kItems];
-        for (int it = 0; it < kItems; ++it) {
-            const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
-            m[it] = __ballot(i < A.n[k] && A.flag[A.flag0[k] + i] != 0 ? 1 : 0);
-            if (lane == 0) cnt[it][w] = __popcll(m[it]);
-        }
-        __syncthreads();
-        int64_t run = A.boff[k][blk];
-        for (int it = 0; it < kItems; ++it) {
-            const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
-            int off = 0, tot = 0;
-            for (int v = 0; v < kW; ++v) {
-                const int c = cnt[it][v];
-                off += v < w ? c : 0;
-                tot += c;
-            }
-            const bool pred = (m[it] >> lane) & 1ull;
-            const int64_t r = run + off + __popcll(m[it] & below);
-            if (pred) keep_write(A, k, i, r);
-            if (k == 0 && i < A.n[0]) A.remap[i] = pred ? (int32_t)r : -1;
-            run += tot;
-        }
-    }
+    const int64_t i = blk * kBlock + threadIdx.x;
+    const bool pred = i < A.n[k] && A.flag[A.flag0[k] + i] != 0;
+    int tot;
+    const int64_t r = A.boff[k][blk] + block_rank(pred, shw, &tot);
+    if (pred) keep_write(A, k, i, r);
+    if (k == 0 && i < A.n[0]) A.remap[i] = pred ? (int32_t)r : -1;
 }
 
 }  // namespace
@@ -891,7 +873,7 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
         A.flag0[k] = nflag;
         nflag += n[k];
         A.n[k] = n[k];
-        nblk[k] = ceil_div(n[k], (int64_t)kBlock * kItems);
+        nblk[k] = ceil_div(n[k], (int64_t)kBlock);
         A.blk0[k + 1] = A.blk0[k] + nblk[k];
         total += 2 * (nblk[k] + 1);
     }
@@ -910,7 +892,7 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
     const int64_t nall = A.blk0[kKeepLists];
     if ((rc = arena_get(kSlotKeepFlags, (size_t)nflag + 1, (void**)&A.flag))) return rc;
     if (nall > 0) {
-        hipLaunchKernelGGL(keep_kernel<false>, dim3((unsigned)nall), dim3(kBlock), 0, s, A, 0, kKeepLists);
+        hipLaunchKernelGGL(keep_count_kernel, dim3((unsigned)nall), dim3(kBlock), 0, s, A);
         ABD_CHECK_LAUNCH();
     }
     ScanSeg sg[kKeepLists];
@@ -920,11 +902,11 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
     if ((rc = scan_exclusive_i64_multi(s, sg, nseg))) return rc;
     const int64_t npairs = A.blk0[2], ntup = nall - npairs;
     if (npairs > 0) {
-        hipLaunchKernelGGL(keep_kernel<true>, dim3((unsigned)npairs), dim3(kBlock), 0, s, A, 0, 2);
+        hipLaunchKernelGGL(keep_write_kernel, dim3((unsigned)npairs), dim3(kBlock), 0, s, A, 0, 2);
         ABD_CHECK_LAUNCH();
     }
     if (ntup > 0) {
-        hipLaunchKernelGGL(keep_kernel<true>, dim3((unsigned)ntup), dim3(kBlock), 0, s, A, 2, kKeepLists);
+        hipLaunchKernelGGL(keep_write_kernel, dim3((unsigned)ntup), dim3(kBlock), 0, s, A, 2, kKeepLists);
         ABD_CHECK_LAUNCH();
     }
     // an empty list's total is 0 (its scan segment is skipped)
