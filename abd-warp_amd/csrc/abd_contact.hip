@@ -401,6 +401,14 @@ __global__ __launch_bounds__(kBlock) void pair_chunks_kernel(int64_t nij, const 
     }
 }
 
+// stage D's item -> pair map: pair ii owns the items [coff[ii], coff[ii] + cnt[ii])
+__global__ __launch_bounds__(kBlock) void pair_items_kernel(int64_t nij, const int64_t* __restrict__ cnt,
+                                                            const int64_t* __restrict__ coff,
+                                                            int32_t* __restrict__ ipair) {
+    for (int64_t ii = (int64_t)blockIdx.x * kBlock + threadIdx.x; ii < nij; ii += (int64_t)gridDim.x * kBlock)
+        for (int64_t c = 0; c < cnt[ii]; ++c) ipair[coff[ii] + c] = (int32_t)ii;
+}
+
 // stage D2: one work item per workgroup (grid-stride over the *total items): the sums, in
 // contact order, of one chunk's contacts into part[item] (24x24 block in the pair's body
 // order [ij0 | ij1], then g).  DENSE: contacts stored by contact_terms_exact / _full.
@@ -409,6 +417,7 @@ __global__ __launch_bounds__(kPairThreads) void pair_partial_kernel(int64_t nij,
                                                                      const uint8_t* __restrict__ swap,
                                                                      const int64_t* __restrict__ pseg,
                                                                      const int64_t* __restrict__ coff,
+                                                                     const int32_t* __restrict__ ipair,
                                                                      const int64_t* __restrict__ total,
                                                                      double* __restrict__ part) {
     constexpr int kStride = DENSE ? kDenseStride : kContactStride;
@@ -419,14 +428,8 @@ __global__ __launch_bounds__(kPairThreads) void pair_partial_kernel(int64_t nij,
     const int R = t / 24, C = t % 24;
     const int64_t nitem = *total;
     for (int64_t gc = blockIdx.x; gc < nitem; gc += gridDim.x) {
-        // the pair: coff[ii] <= gc < coff[ii + 1] (pairs without items share their offset)
-        int64_t lo = 0, hi = nij - 1;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi + 1) >> 1;
-            if (coff[mid] <= gc) lo = mid;
-            else hi = mid - 1;
-        }
-        const int64_t ii = lo;
+        // the pair: coff[ii] <= gc < coff[ii + 1]
+        const int64_t ii = ipair[gc];
         int64_t lc = gc - coff[ii];
         const int64_t* sp = pseg + 2 * ii;
         const int64_t npc = (sp[1] - sp[0] + kPairChunk - 1) / kPairChunk;
@@ -801,13 +804,16 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s);
         // stage D: chunks of each pair's runs (count, scan), their partial sums, the per-pair sums
-        int64_t* pcnt;
-        if ((rc = arena_get(kSlotPairChunk, sizeof(int64_t) * (2 * (size_t)nij + 2), (void**)&pcnt))) return rc;
-        int64_t* pcoff = pcnt + nij;
-        int64_t* ptotal = pcoff + nij;
         // items: at most one per kPairChunk contacts plus one per run (a run per pair and type)
         const int64_t runs = 2 * nij < nact ? 2 * nij : nact;
         const int64_t cap_items = nact / kPairChunk + runs + 1;
+        int64_t* pcnt;
+        if ((rc = arena_get(kSlotPairChunk, sizeof(int64_t) * (2 * (size_t)nij + 2) + sizeof(int32_t) * cap_items,
+                            (void**)&pcnt)))
+            return rc;
+        int64_t* pcoff = pcnt + nij;
+        int64_t* ptotal = pcoff + nij;
+        int32_t* ipair = reinterpret_cast<int32_t*>(ptotal + 2);
         double* part;
         if ((rc = arena_get(kSlotPairPart, sizeof(double) * kPartStride * (size_t)cap_items, (void**)&part)))
             return rc;
@@ -815,16 +821,19 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
                            pcnt);
         ABD_CHECK_LAUNCH();
         if ((rc = scan_exclusive_i64(s, pcnt, pcoff, nij, ptotal))) return rc;
+        hipLaunchKernelGGL(pair_items_kernel, dim3(grid_for(nij)), dim3(kBlock), 0, s, nij, (const int64_t*)pcnt,
+                           (const int64_t*)pcoff, ipair);
+        ABD_CHECK_LAUNCH();
         // timed: the partial sums (each stored contact read once, its body-order flag once)
         ph = prof_start_ext(kProfPairReduce, (8.0 * stride + 1) * nact);
         if (dense)
             prof_launch(ph, pair_partial_kernel<true>, dim3(kPairGrid), dim3(kPairThreads), 0, s, nij,
                         (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg, (const int64_t*)pcoff,
-                        (const int64_t*)ptotal, part);
+                        (const int32_t*)ipair, (const int64_t*)ptotal, part);
         else
             prof_launch(ph, pair_partial_kernel<false>, dim3(kPairGrid), dim3(kPairThreads), 0, s, nij,
                         (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg, (const int64_t*)pcoff,
-                        (const int64_t*)ptotal, part);
+                        (const int32_t*)ipair, (const int64_t*)ptotal, part);
         ABD_CHECK_LAUNCH();
         hipLaunchKernelGGL(pair_reduce_kernel, dim3(grid_for(nij, 1, kPairGrid)), dim3(kPairThreads), 0, s, nb, nij,
                            (const uint8_t*)pflag, (const int64_t*)pcoff, (const int64_t*)pcnt, (const double*)part,
