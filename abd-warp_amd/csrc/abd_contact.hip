@@ -337,7 +337,10 @@ __device__ __forceinline__ int64_t active_key(const int32_t* __restrict__ pt, in
     return ((int64_t)(is_ee ? 1 : 0) << 32) | (int64_t)t5[2];
 }
 
-// stage C: one thread per active contact.
+// stage C: one thread per active contact.  MODE: 1 the contact-space terms (psd 0 and 1),
+// 2 the exact Hessian, 3 full IPC; one kernel per mode, so that the contact-space path does
+// not carry the dense modes' register and stack footprint.
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void contact_kernel(AbdScene sc, AbdParams P, const double* __restrict__ x,
                                                           const int32_t* __restrict__ pt, int64_t npt,
                                                           const int32_t* __restrict__ ee,
@@ -359,9 +362,9 @@ __global__ __launch_bounds__(kBlock) void contact_kernel(AbdScene sc, AbdParams 
         d3 v[4], r[4];
         load4(x, vi, v);
         load4(sc.x0, vi, r);
-        if (P.psd == 3)
+        if (MODE == 3)
             contact_terms_full(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, cbuf + e * kDenseStride);
-        else if (P.psd == 2)
+        else if (MODE == 2)
             contact_terms_exact(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, cbuf + e * kDenseStride);
         else
             contact_terms(is_ee ? 1 : 0, v, r, P.d2hat, P.kappa, P.psd, cbuf + e * kContactStride);
@@ -799,8 +802,16 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
         hipLaunchKernelGGL(zero_i64_kernel, dim3(grid_for(4 * nij)), dim3(kBlock), 0, s, pseg, 4 * nij);
         ABD_CHECK_LAUNCH();
         int ph = prof_start(kProfContact, s, (4.0 + 20 + 16 + 192 + 8.0 * stride + 9) * nact);
-        hipLaunchKernelGGL(contact_kernel, dim3(grid_for(nact, kBlock, 65535)), dim3(kBlock), 0, s, *sc, *P, x, pt,
-                           npt, ee, (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
+        const dim3 cgrid(grid_for(nact, kBlock, 65535));
+        if (P->psd == 3)
+            hipLaunchKernelGGL(contact_kernel<3>, cgrid, dim3(kBlock), 0, s, *sc, *P, x, pt, npt, ee,
+                               (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
+        else if (P->psd == 2)
+            hipLaunchKernelGGL(contact_kernel<2>, cgrid, dim3(kBlock), 0, s, *sc, *P, x, pt, npt, ee,
+                               (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
+        else
+            hipLaunchKernelGGL(contact_kernel<1>, cgrid, dim3(kBlock), 0, s, *sc, *P, x, pt, npt, ee,
+                               (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s);
         // stage D: chunks of each pair's runs (count, scan), their partial sums, the per-pair sums
