@@ -161,3 +161,29 @@ def full_ipc_config(rng, kind: int, feature: str, dist: float):
     off = _unit(np.cross(da, db)) if np.linalg.norm(np.cross(da, db)) > 0 else perp
     a0 = b0 + 0.25 * (b1 - b0) + dist * off - 0.1 * da
     return np.stack([a0, a0 + 0.7 * np.linalg.norm(b1 - b0) * da, b0, b1])
+
+
+# ------------------------------------------------------------------ driver pins
+# Scenes whose frame has a closed-form outcome (abd.py:212-303): a free-falling
+# cube, a cube falling onto the ground, a cube falling onto another.  Each case
+# runs one frame of any driver (oracle or GPU) and is checked against
+# arithmetic written out here, not against another implementation.
+
+def one_cube(p, pdot, ground=-100.0):
+    from abd_hip import scenes
+    V, F = scenes.cube_mesh()
+    return scenes.build_scene([(V, F)], np.eye(3)[None], np.asarray([p], np.float64), np.zeros((1, 3, 3)),
+                              np.asarray([pdot], np.float64), ground=ground, name="one-cube")
+
+
+def two_cubes(pA, pdotA, pB, ground=-100.0):
+    from abd_hip import scenes
+    V, F = scenes.cube_mesh()
+    return scenes.build_scene([(V, F), (V, F)], np.stack([np.eye(3)] * 2), np.asarray([pB, pA], np.float64),
+                              np.zeros((2, 3, 3)), np.asarray([[0.0, 0.0, 0.0], pdotA], np.float64), ground=ground,
+                              name="two-cubes")
+
+
+def inertial_target(spec, cp):
+    """p~ = p0 + dt pdot + dt^2 (0, gravity, 0) (orthogonal_energy.py:25-52)."""
+    return spec.p + cp.dt * spec.pdot + cp.dt * cp.dt * np.array([0.0, cp.gravity, 0.0])
