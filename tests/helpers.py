@@ -187,3 +187,79 @@ def two_cubes(pA, pdotA, pB, ground=-100.0):
 def inertial_target(spec, cp):
     """p~ = p0 + dt pdot + dt^2 (0, gravity, 0) (orthogonal_energy.py:25-52)."""
     return spec.p + cp.dt * spec.pdot + cp.dt * cp.dt * np.array([0.0, cp.gravity, 0.0])
+
+
+def _planted(spec, cp, scale):
+    """Warm start dq = scale * (p0 - p~) on the first body's translation (zero on A)."""
+    dq = np.zeros((4 * spec.n_bodies, 3))
+    dq[0] = scale * (spec.p[0] - inertial_target(spec, cp)[0])
+    return dq
+
+
+def free_energy(spec, cp, p):
+    """Inertial energy of a contact-free translated cube at A = I (orthogonal_energy.py:25-52)."""
+    d = p - inertial_target(spec, cp)[0]
+    return 0.5 * cp.mass * float((d * d).sum())
+
+
+def driver_pin_cases():
+    """{name: (spec, params, planted warm start or None, expected p after one frame,
+    expected Newton iterations, tolerance on p)}.  The line-search cases plant the
+    search direction through the warm start: with solver_tol so large that the Jacobi
+    CG's first residual test passes (bsr_utils.py:195-205), the solve returns it."""
+    from abd_hip.const_params import ContactParams
+    cases = {}
+    hi = [0.0, 5.0, 0.0]
+    v = [0.3, -0.7, 0.2]
+    # free fall: the inertial energy is quadratic in p with its minimum at p~ and A = I is a
+    # stationary point of the orthogonality energy; iteration 1 lands on p~ (g.g = m^2 |p0 - p~|^2
+    # > tol), iteration 2 sees g = 0 and stops
+    for solver in (0, 1, 2, 3):
+        spec = one_cube(hi, v)
+        cp = ContactParams(ground=spec.ground, solver=solver)
+        cases[f"free_fall_solver{solver}"] = (spec, cp, None, inertial_target(spec, cp), 2, 1e-12)
+    # the stop test reads g before the update: a tolerance above the first g.g still takes
+    # that iteration's full step (at least one iteration per frame)
+    spec = one_cube(hi, v)
+    cp = ContactParams(ground=spec.ground, solver=3, tol=1e30)
+    cases["stop_after_first_update"] = (spec, cp, None, inertial_target(spec, cp), 1, 1e-12)
+    spec = one_cube(hi, v)
+    cp = ContactParams(ground=spec.ground, solver=3, max_iter=1)
+    cases["max_iter_1"] = (spec, cp, None, inertial_target(spec, cp), 1, 1e-12)
+    # D8: E1 < E0 + c1 alpha q.g with the step -alpha dq and q.g > 0 accepts an increase below
+    # c1 alpha q.g at alpha = 1 (E(1) = (1 + 1e-4)^2 E0 here)
+    spec = one_cube(hi, v)
+    cp = ContactParams(ground=spec.ground, solver=1, solver_tol=1e30, max_iter=1)
+    dq = _planted(spec, cp, 2.0 + 1e-4)
+    cases["armijo_as_written_accepts"] = (spec, cp, dq, spec.p - dq[:1], 1, 1e-12)
+    # E(1) = 1.0201 E0, above the bound: one halving
+    dq = _planted(spec, cp, 2.0 + 1e-2)
+    cases["armijo_halves"] = (spec, cp, dq, spec.p - 0.5 * dq[:1], 1, 1e-12)
+    # an ascent direction is never accepted: the halving stops at the first step length
+    # below alpha_min (2^-10, evaluated before the test); the planted direction survives
+    # every solve, so each of max_iter = 4 iterations takes that step
+    cp4 = ContactParams(ground=spec.ground, solver=1, solver_tol=1e30, max_iter=4)
+    dq = _planted(spec, cp4, -1.0)
+    cases["alpha_min_and_max_iter"] = (spec, cp4, dq, spec.p - 4 * 2.0 ** -10 * dq[:1], 4, 1e-12)
+    # CCD cap against the ground: bottom face 0.05 above it at 10 m/s down; the vertex-ground
+    # TOI is d0 / (d0 - d1) (ccd.py:189-195), the step 0.9 of it (abd.py:300-302), leaving a
+    # gap of 0.1 d0 > dhat (no barrier), which the line search accepts
+    spec = one_cube([0.0, 0.55, 0.0], [0.0, -10.0, 0.0], ground=0.0)
+    cp = ContactParams(ground=0.0, solver=3, max_iter=1)
+    pt = inertial_target(spec, cp)
+    d0 = 0.05
+    d1 = d0 + (pt[0, 1] - spec.p[0, 1])
+    a = 0.9 * d0 / (d0 - d1)
+    cases["ccd_cap_ground"] = (spec, cp, None, spec.p + a * (pt - spec.p), 1, 1e-12)
+    # CCD cap between two cubes: A shifted in x and z so that vertices of each face lie
+    # strictly inside the other's face, falling at 5 m/s onto B; the PT / EE TOIs are the time
+    # the relative vertical motion closes the gap (cubic roots bracketed to 1e-10,
+    # cubic_roots.py:4)
+    gap = 0.04
+    spec = two_cubes([0.1, 1.0 + gap, 0.13], [0.0, -5.0, 0.0], [0.0, 0.0, 0.0])
+    cp = ContactParams(ground=spec.ground, solver=3, max_iter=1)
+    pt = inertial_target(spec, cp)
+    rel = (pt[1, 1] - spec.p[1, 1]) - (pt[0, 1] - spec.p[0, 1])
+    a = 0.9 * gap / (gap - (gap + rel))
+    cases["ccd_cap_two_cubes"] = (spec, cp, None, spec.p + a * (pt - spec.p), 1, 1e-9)
+    return cases

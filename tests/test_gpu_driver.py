@@ -93,3 +93,28 @@ def test_medium_subset_block_jacobi_cg_frames():
     cp = ContactParams(ground=spec.ground, psd=1, solver=2, solver_tol=1e-10, solver_maxiter=20000)
     sim, ref, worst = _run(spec, cp, 10, native=True)
     assert sim.stats["gh_active"] > 0
+
+
+def _pin_cases():
+    from helpers import driver_pin_cases
+    return driver_pin_cases()
+
+
+@pytest.mark.parametrize("native", [False, True])
+@pytest.mark.parametrize("name", sorted(_pin_cases()))
+def test_closed_form_frame(name, native):
+    """The driver's control logic against closed forms (no oracle in the loop): free fall,
+    the pre-update stop test, max_iter, the as-written Armijo test, halving, alpha_min, and
+    the CCD caps against the ground and between two cubes (cases in tests/helpers.py)."""
+    from abd_hip.abd import AffineBodySimulator
+    spec, cp, dq, p_want, it_want, tol = _pin_cases()[name]
+    print(f"pin case {name} native={native}", flush=True)
+    sim = AffineBodySimulator(spec, cp, "cuda", native=native)
+    if dq is not None:
+        sim.dq.copy_(torch.as_tensor(dq, device="cuda"))
+    it = sim.step(0)
+    torch.cuda.synchronize()
+    assert it == it_want
+    np.testing.assert_allclose(sim.states.p.cpu().numpy(), p_want, rtol=0, atol=tol)
+    if name.startswith("free_fall"):
+        np.testing.assert_array_equal(sim.states.A.cpu().numpy(), np.eye(3)[None])

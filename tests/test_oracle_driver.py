@@ -17,141 +17,40 @@ import pytest
 
 from abd_hip import scenes
 from abd_hip.const_params import ContactParams
-from helpers import inertial_target, one_cube, two_cubes
+from helpers import driver_pin_cases, free_energy, inertial_target
 from oracle import oracle as O
 
-HI = [0.0, 5.0, 0.0]
+CASES = driver_pin_cases()
 
 
-def _frame(spec, cp, dq=None):
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_closed_form_frame(name):
+    spec, cp, dq, p_want, it_want, tol = CASES[name]
     sim = O.OracleSim(spec, cp)
     if dq is not None:
         sim.dq[...] = dq
     it = sim.step()
-    return sim, it
+    assert it == it_want
+    np.testing.assert_allclose(sim.p, p_want, rtol=0, atol=tol)
+    if name.startswith("free_fall"):
+        np.testing.assert_array_equal(sim.A, np.eye(3)[None])
+        np.testing.assert_allclose(sim.pdot, (p_want - spec.p) / cp.dt, rtol=0, atol=1e-10)
+        np.testing.assert_array_equal(sim.p0, sim.p)
+    if dq is not None:
+        np.testing.assert_array_equal(sim.dq, dq)     # the solve kept the planted direction
 
 
-@pytest.mark.parametrize("solver", [0, 1, 2, 3])
-def test_free_fall_is_one_newton_step(solver):
-    """Contact-free cube: the inertial energy is quadratic in p with its minimum at
-    p~, A = I is a stationary point of the orthogonality energy, so the first
-    Newton step lands on p~ and the second iteration sees g = 0 and stops."""
-    spec = one_cube(HI, [0.3, -0.7, 0.2])
-    cp = ContactParams(ground=spec.ground, solver=solver)
-    sim, it = _frame(spec, cp)
-    pt = inertial_target(spec, cp)
-    assert it == 2          # iteration 1 has g.g = m^2 |p0 - p~|^2 > tol; iteration 2 has g.g ~ 0
-    np.testing.assert_allclose(sim.p, pt, rtol=0, atol=1e-12)
-    np.testing.assert_array_equal(sim.A, np.eye(3)[None])
-    np.testing.assert_allclose(sim.pdot, (pt - spec.p) / cp.dt, rtol=0, atol=1e-10)
-    np.testing.assert_array_equal(sim.p0, sim.p)
-
-
-def test_stop_test_reads_the_gradient_before_the_update():
-    """With a tolerance above g.g of the first iteration the frame still takes
-    that iteration's full step, then stops (at least one iteration per frame,
-    test after the update of q)."""
-    spec = one_cube(HI, [0.3, -0.7, 0.2])
-    cp = ContactParams(ground=spec.ground, solver=3, tol=1e30)
-    sim, it = _frame(spec, cp)
-    assert it == 1
-    np.testing.assert_allclose(sim.p, inertial_target(spec, cp), rtol=0, atol=1e-12)
-
-
-def test_max_iter_cuts_the_loop():
-    spec = one_cube(HI, [0.3, -0.7, 0.2])
-    cp = ContactParams(ground=spec.ground, solver=3, max_iter=1)
-    sim, it = _frame(spec, cp)
-    assert it == 1
-    np.testing.assert_allclose(sim.p, inertial_target(spec, cp), rtol=0, atol=1e-12)
-
-
-def _planted(spec, cp, scale):
-    """Warm start dq = scale * (p0 - p~) on the cube's translation (zero on A)."""
-    dq = np.zeros((4, 3))
-    dq[0] = scale * (spec.p[0] - inertial_target(spec, cp)[0])
-    return dq
-
-
-def _energy(spec, cp, p):
-    # inertial energy of a contact-free translated cube at A = I (orthogonal_energy.py:25-52)
-    d = p - inertial_target(spec, cp)
-    return 0.5 * cp.mass * float((d * d).sum())
-
-
-def test_line_search_accepts_a_small_increase_as_written():
-    """D8: the test is E1 < E0 + c1 alpha q.g with the step -alpha dq and q.g > 0,
-    so an energy increase below c1 alpha q.g is accepted at alpha = 1 (a test
-    E1 < E0 - c1 alpha q.g, or E1 < E0, would halve)."""
-    spec = one_cube(HI, [0.3, -0.7, 0.2])
-    cp = ContactParams(ground=spec.ground, solver=1, solver_tol=1e30, max_iter=1)
-    dq = _planted(spec, cp, 2.0 + 1e-4)
-    sim, it = _frame(spec, cp, dq)
-    p1 = spec.p[0] - dq[0]
-    E0, E1 = _energy(spec, cp, spec.p[0]), _energy(spec, cp, p1)
+def test_armijo_case_is_the_as_written_branch():
+    """The accepted step of the D8 case raises the energy (E0 < E1 < E0 + c1 q.g), and the
+    driver reports that energy as the frame's last."""
+    spec, cp, dq, p_want, _, _ = CASES["armijo_as_written_accepts"]
+    E0, E1 = free_energy(spec, cp, spec.p[0]), free_energy(spec, cp, p_want[0])
     g = cp.mass * (spec.p[0] - inertial_target(spec, cp)[0])
-    qTg = float(dq[0] @ g)
-    assert E0 < E1 < E0 + cp.c1 * qTg           # the case is the D8 branch
-    assert it == 1
-    np.testing.assert_allclose(sim.p[0], p1, rtol=0, atol=1e-12)
+    assert E0 < E1 < E0 + cp.c1 * float(dq[0] @ g)
+    sim = O.OracleSim(spec, cp)
+    sim.dq[...] = dq
+    sim.step()
     assert sim.E == pytest.approx(E1, rel=1e-12)
-
-
-def test_line_search_halves_past_the_armijo_bound():
-    spec = one_cube(HI, [0.3, -0.7, 0.2])
-    cp = ContactParams(ground=spec.ground, solver=1, solver_tol=1e30, max_iter=1)
-    dq = _planted(spec, cp, 2.0 + 1e-2)         # E(1) = 1.0201 E0: above E0 + c1 q.g
-    sim, _ = _frame(spec, cp, dq)
-    np.testing.assert_allclose(sim.p[0], spec.p[0] - 0.5 * dq[0], rtol=0, atol=1e-12)
-
-
-def test_line_search_stops_below_alpha_min_and_max_iter_bounds_the_frame():
-    """An ascent direction is never accepted: the halving runs until alpha <
-    alpha_min (1e-3), so the step taken is 2^-10 (the first step length below
-    alpha_min, evaluated before the test); with the planted direction kept by
-    every solve, each of the max_iter iterations takes that step."""
-    spec = one_cube(HI, [0.3, -0.7, 0.2])
-    cp = ContactParams(ground=spec.ground, solver=1, solver_tol=1e30, max_iter=4)
-    dq = _planted(spec, cp, -1.0)
-    sim, it = _frame(spec, cp, dq)
-    assert it == 4
-    np.testing.assert_allclose(sim.p[0], spec.p[0] - 4 * 2.0 ** -10 * dq[0], rtol=0, atol=1e-12)
-    np.testing.assert_allclose(sim.dq, dq, rtol=0, atol=0)
-
-
-def test_ground_ccd_caps_the_first_step():
-    """A cube 0.05 above the ground at 10 m/s downward: the full Newton step
-    would pass through the ground; the vertex-ground TOI of the bottom face is
-    t = d0 / (d0 - d1) (ccd.py:189-195) and the step is capped at 0.9 t, which
-    leaves a gap of 0.1 d0 (> dhat: no barrier), so the line search accepts it."""
-    ground = 0.0
-    spec = one_cube([0.0, ground + 0.5 + 0.05, 0.0], [0.0, -10.0, 0.0], ground=ground)
-    cp = ContactParams(ground=ground, solver=3, max_iter=1)
-    sim, it = _frame(spec, cp)
-    pt = inertial_target(spec, cp)[0]
-    d0 = 0.05
-    d1 = d0 + (pt[1] - spec.p[0, 1])
-    alpha = 0.9 * d0 / (d0 - d1)
-    assert it == 1
-    np.testing.assert_allclose(sim.p[0], spec.p[0] + alpha * (pt - spec.p[0]), rtol=0, atol=1e-12)
-
-
-def test_two_cube_ccd_caps_the_first_step():
-    """Cube A (shifted in x and z so that vertices of each face lie strictly
-    inside the other's face) falls onto cube B at 5 m/s: the point-triangle and
-    edge-edge TOIs of the touching faces are the time the relative vertical
-    motion closes the gap d0, the cubic roots of ccd.py:134-186 (bracketed to
-    1e-10, cubic_roots.py:4)."""
-    gap = 0.04
-    pA = [0.1, 1.0 + gap, 0.13]
-    spec = two_cubes(pA, [0.0, -5.0, 0.0], [0.0, 0.0, 0.0])
-    cp = ContactParams(ground=spec.ground, solver=3, max_iter=1)
-    sim, it = _frame(spec, cp)
-    pt = inertial_target(spec, cp)
-    rel = (pt[1, 1] - spec.p[1, 1]) - (pt[0, 1] - spec.p[0, 1])
-    alpha = 0.9 * gap / (gap - (gap + rel))
-    assert it == 1
-    np.testing.assert_allclose(sim.p, spec.p + alpha * (pt - spec.p), rtol=0, atol=1e-9)
 
 
 def test_two_cubes_settle():
