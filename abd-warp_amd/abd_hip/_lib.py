@@ -17,6 +17,8 @@ ABD_ERR_OVERFLOW = 1
 ABD_ERR_ARG = -1000
 ABD_ERR_UNSORTED = -1001
 ABD_ERR_NODEVICE = -1002
+ABD_ERR_CAPACITY = -1003
+TOI_NOALLOC_MAX_CHUNKS = 65536
 KIND_POINT, KIND_EDGE, KIND_TRI = 0, 1, 2
 CULL_EE, CULL_PT = 0, 1
 SOLVER_BICGSTAB, SOLVER_CG_JACOBI, SOLVER_CG_BLOCK, SOLVER_LDLT = 0, 1, 2, 3
@@ -88,6 +90,7 @@ def load():
         "abd_inertia_hessian": [p, S, P, p, p],
         "abd_bsr_solve": [p, i32, i32, p, p, p, p, p, p, d, i32, i32, p, p],
         "abd_toi": [p, S, P, p, p, p, i64, p, i64, p, i64, p],
+        "abd_toi_noalloc": [p, S, P, p, p, p, i64, p, i64, p, i64, p],
         "abd_cubic_roots": [p, p, i64, d, d, p, p],
         "abd_collision_time": [p, i32, p, p, i64, p],
         "abd_scene_contact_energy": [p, S, P, p, p, i64, p, i64, p, i64, p],

@@ -190,6 +190,7 @@ const char* abd_error_string(int code) {
         case ABD_ERR_ARG: return "invalid argument";
         case ABD_ERR_UNSORTED: return "candidate list not grouped by body pair / body";
         case ABD_ERR_NODEVICE: return "no HIP device";
+        case ABD_ERR_CAPACITY: return "input exceeds the fixed scratch of an allocation-free entry point";
         default: break;
     }
     if (code < 0) return hipGetErrorString(static_cast<hipError_t>(-code));

@@ -11,6 +11,8 @@
 // path to the last bit.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "abd_block.h"
 #include "abd_device.h"
 #include "abd_inertia.h"
@@ -328,17 +330,25 @@ int64_t chunk_bound(const Lists& L, int S) {
     return bound;
 }
 
+// Caller-provided scratch of a reduction (the allocation-free entry point's): room for
+// the plan and for `part_cap` chunk partials.  Null members: the arena's.
+struct ReduceScratch {
+    int64_t* plan = nullptr;
+    double* part = nullptr;
+    int64_t part_cap = 0;
+};
+
 // The per-scene segments and chunk offsets of the lists: plan_in when given (made by
-// reduce_plan for these lists), else computed into the arena.
+// reduce_plan for these lists), else computed into plan_buf (nullable: the arena).
 int get_plan(hipStream_t s, const AbdScene& sc, int S, const Lists& L, const int64_t* plan_in,
-             const int64_t** seg, const int64_t** choff) {
+             const int64_t** seg, const int64_t** choff, int64_t* plan_buf = nullptr) {
     if (plan_in) {
         *seg = plan_in;
         *choff = plan_in + 2 * kLists * (size_t)S;
         return ABD_OK;
     }
-    int64_t* plan;
-    int rc = arena_get(kSlotPlan, sizeof(int64_t) * reduce_plan_elems(S), (void**)&plan);
+    int64_t* plan = plan_buf;
+    int rc = plan ? ABD_OK : arena_get(kSlotPlan, sizeof(int64_t) * reduce_plan_elems(S), (void**)&plan);
     if (rc) return rc;
     hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kPlanThreads), 0, s, S, sc.body_scene, L, plan,
                        plan + 2 * kLists * (size_t)S);
@@ -349,18 +359,20 @@ int get_plan(hipStream_t s, const AbdScene& sc, int S, const Lists& L, const int
 }
 
 // Plan + chunks + per-scene results for OP over the lists; S = 1 treats the lists as one scene.
+// scr (nullable): scratch of the caller instead of the arena (then no launch is timed).
 template <int OP>
 int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, const double* x0, const double* x1,
                const Lists& L, const int64_t* plan_in, double* out, int prof_id, double prof_bytes,
-               const uint8_t* active = nullptr) {
+               const uint8_t* active = nullptr, const ReduceScratch* scr = nullptr) {
     const int64_t bound = chunk_bound(L, S);
     if (bound > INT32_MAX) return ABD_ERR_ARG;
+    if (scr && bound > scr->part_cap) return ABD_ERR_CAPACITY;
     const int64_t *seg, *choff;
-    double* part;
-    int rc = get_plan(s, sc, S, L, plan_in, &seg, &choff);
-    if (!rc) rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound, (void**)&part);
+    double* part = scr ? scr->part : nullptr;
+    int rc = get_plan(s, sc, S, L, plan_in, &seg, &choff, scr ? scr->plan : nullptr);
+    if (!rc && !part) rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound, (void**)&part);
     if (rc) return rc;
-    int ph = prof_start_ext(prof_id, prof_bytes);
+    int ph = scr ? -1 : prof_start_ext(prof_id, prof_bytes);
     prof_launch(ph, chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1,
                 TrialArgs{nullptr, nullptr, nullptr, nullptr, active}, L, S, seg, choff, part, (int64_t)0);
     ABD_CHECK_LAUNCH();
@@ -484,6 +496,39 @@ int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, cons
                          224.0 * L.l[1].n + 228.0 * L.l[0].n + 56.0 * L.l[2].n, active);
 }
 
+// ------------------------------------------------- allocation-free TOI scratch
+// The fixed scratch of abd_toi_noalloc: the chunk plan (kNoAllocPlanElems int64 at the
+// front) and ABD_TOI_NOALLOC_MAX_CHUNKS chunk partials.  In host code `scratch` names a
+// host-side stand-in, not device memory: its device address is looked up once per device
+// (noalloc_scratch) and only that address reaches kernels.
+constexpr int64_t kNoAllocPlanElems = 64;
+static __device__ __align__(256) double scratch[kNoAllocPlanElems + ABD_TOI_NOALLOC_MAX_CHUNKS];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocDevices] = {};
+
+// The device address of `scratch` on the current device (looked up on its first use there).
+int noalloc_scratch(ReduceScratch* out) {
+    int dev = -1;
+    ABD_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) return ABD_ERR_NODEVICE;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        ABD_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc_ptr[dev] = p;
+    }
+    double* base = static_cast<double*>(g_noalloc_ptr[dev]);
+    out->plan = reinterpret_cast<int64_t*>(base);
+    out->part = base + kNoAllocPlanElems;
+    out->part_cap = ABD_TOI_NOALLOC_MAX_CHUNKS;
+    return ABD_OK;
+}
+static_assert(2 * kLists + 2 <= kNoAllocPlanElems, "one scene's plan fits the front of the scratch");
+}  // namespace
+
 }  // namespace abd
 
 using namespace abd;
@@ -509,6 +554,19 @@ int abd_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const d
             double* toi) {
     if (!sc) return ABD_ERR_ARG;
     return scene_toi(to_stream(stream), sc, P, 1, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, nullptr, toi, nullptr);
+}
+
+int abd_toi_noalloc(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x_t0,
+                    const double* x_t1, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
+                    const int32_t* vg, int64_t nvg, double* toi) {
+    if (!sc || !P || !x_t0 || !x_t1 || !toi || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
+    if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
+    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);
+    if (chunk_bound(L, 1) > ABD_TOI_NOALLOC_MAX_CHUNKS) return ABD_ERR_CAPACITY;  // before any device call
+    ReduceScratch scr;
+    int rc = noalloc_scratch(&scr);
+    if (rc) return rc;
+    return seg_reduce<1>(to_stream(stream), *sc, *P, 1, x_t0, x_t1, L, nullptr, toi, kProfToi, 0.0, nullptr, &scr);
 }
 
 int abd_scene_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x_t0,

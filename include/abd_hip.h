@@ -53,6 +53,7 @@ typedef void* abd_stream_t; /* hipStream_t */
 #define ABD_ERR_ARG (-1000)      /* bad argument (null pointer, negative size, ...) */
 #define ABD_ERR_UNSORTED (-1001) /* candidate list not grouped by body pair / body */
 #define ABD_ERR_NODEVICE (-1002) /* no HIP device */
+#define ABD_ERR_CAPACITY (-1003) /* input larger than the fixed scratch of an allocation-free entry point */
 
 #define ABD_KIND_POINT 0
 #define ABD_KIND_EDGE 1
@@ -257,6 +258,19 @@ int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t
 int abd_toi(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* x_t0,
             const double* x_t1, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg,
             int64_t nvg, double* toi);
+/* abd_toi without allocation (no reference counterpart; for callers that cannot call
+ * hipMalloc: code under HIP graph capture, serving loops, harnesses that forbid it).
+ * Same arguments, kernels and result as abd_toi, bit for bit; its scratch (the chunk plan
+ * and one partial minimum per 2048-candidate chunk of each list) is a fixed array in
+ * device memory, ABD_TOI_NOALLOC_MAX_CHUNKS chunks (512 KiB, held by every device the
+ * library runs on until the process ends): sum over the PT, EE and VG lists of
+ * ceil(n / 2048), plus 3, must not exceed it, else ABD_ERR_CAPACITY and nothing is
+ * launched.  The first call on a device looks the array up (no allocation).  All calls
+ * on one device share the array: calls that overlap in time on one device race. */
+#define ABD_TOI_NOALLOC_MAX_CHUNKS 65536
+int abd_toi_noalloc(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* x_t0,
+                    const double* x_t1, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
+                    const int32_t* vg, int64_t nvg, double* toi);
 /* cubic_roots.py:95-193 for n polynomials coef [n][4] (constant first) on [x0, x1]:
  * nroots [n], roots [n][3]. */
 int abd_cubic_roots(abd_stream_t stream, const double* coef, int64_t n, double x0, double x1, int32_t* nroots,
