@@ -317,26 +317,79 @@ def diff_targets(spec: SceneSpec, seed: int = 1) -> np.ndarray:
     return spec.p + rng.uniform(-0.3, 0.3, size=spec.p.shape)
 
 
-def config_stress(seed: int = 0, n_bodies: int = 1024) -> SceneSpec:
-    """Stress: 1024 bodies (equal thirds cube / icosphere-3 / icosphere-4
-    ellipsoid with radii U[0.3,0.8]^3 and a radial perturbation) on a jittered
-    10 x 10 x 10.24 lattice with gap 0.02."""
-    rng = np.random.default_rng(seed)
+def _packed_lattice(rng: np.random.Generator, meshes, nx: int, nz: int, gap: float = 0.02,
+                    jitter: float = 0.005) -> np.ndarray:
+    """Positions packing `meshes` (axis-aligned, identity orientation) into a lattice of
+    rows of nx bodies along x, nz rows per layer along z and as many layers along y as
+    needed: along x each body's box starts `gap` after its neighbour's; rows and layers
+    are spaced by their largest box plus `gap`; the lowest layer's lowest vertex sits
+    `gap` above y = 0.  Each position is jittered by U[-jitter, jitter]^3 (jitter <
+    gap / 2 keeps every box apart)."""
+    lo = np.array([V.min(0) for V, _ in meshes])
+    hi = np.array([V.max(0) for V, _ in meshes])
+    n = len(meshes)
+    per_layer = nx * nz
+    p = np.zeros((n, 3))
+    y = None
+    for k0 in range(0, n, per_layer):
+        layer = np.arange(k0, min(k0 + per_layer, n))
+        y = gap - lo[layer, 1].min() if y is None else y + gap - lo[layer, 1].min()
+        z = None
+        for r0 in range(0, len(layer), nx):
+            row = layer[r0:r0 + nx]
+            z = -lo[row, 2].min() if z is None else z + gap - lo[row, 2].min()
+            x = 0.0
+            for t, b in enumerate(row):
+                x = -lo[b, 0] if t == 0 else x + gap - lo[b, 0]
+                p[b] = (x, y, z)
+                x += hi[b, 0]
+            z += hi[row, 2].max()
+        y += hi[layer, 1].max()
+    return p + rng.uniform(-jitter, jitter, size=p.shape)
+
+
+def _stress_meshes(rng: np.random.Generator, n_bodies: int):
     ico3 = icosphere_mesh(3, 0.5)
     cube = cube_mesh()
-    meshes = []
-    for b in range(n_bodies):
-        k = b % 3
-        meshes.append(cube if k == 0 else ico3 if k == 1 else ellipsoid_mesh(rng))
-    spacing = 1.0 + 0.6 + 0.02
-    ps = []
-    for b in range(n_bodies):
-        i, j, k = b % 10, (b // 10) % 10, b // 100
-        jit = rng.uniform(-0.005, 0.005, size=3)
-        ps.append([spacing * i + jit[0], 0.9 + spacing * k + jit[1], spacing * j + jit[2]])
+    return [cube if b % 3 == 0 else ico3 if b % 3 == 1 else ellipsoid_mesh(rng) for b in range(n_bodies)]
+
+
+def config_stress(seed: int = 0, n_bodies: int = 1024, gap: float = 0.02, jitter: float = 0.005) -> SceneSpec:
+    """Stress: 1024 bodies in equal thirds cube / icosphere-3 / icosphere-4 ellipsoid
+    (radii U[0.3, 0.8]^3, radial perturbation +-5%), packed in a 10 x 10 x 10.24
+    jittered lattice with `gap` between neighbouring boxes (_packed_lattice), at
+    rest above a floor at y = 0; they collapse under gravity into a pile.  With a gap
+    below dhat (config_stress_compressed) the bodies start in contact with their
+    neighbours and the floor."""
+    rng = np.random.default_rng(seed)
+    meshes = _stress_meshes(rng, n_bodies)
+    p = _packed_lattice(rng, meshes, 10, 10, gap, jitter)
     A = np.stack([np.eye(3)] * n_bodies)
-    return build_scene(meshes, A, np.array(ps), np.zeros((n_bodies, 3, 3)), np.zeros((n_bodies, 3)), ground=0.0,
+    return build_scene(meshes, A, p, np.zeros((n_bodies, 3, 3)), np.zeros((n_bodies, 3)), ground=0.0,
                        name="stress")
+
+
+def config_stress_compressed(seed: int = 0, n_bodies: int = 1024) -> SceneSpec:
+    """config_stress packed with a gap of 2 dhat (jitter 2e-4): the lowest layer
+    lands on the floor within two frames and each layer on the one below right
+    after, so the pile is in contact within the first frames (a gap inside dhat
+    would start every contact deep in the barrier)."""
+    return config_stress(seed, n_bodies, gap=2 * DHAT, jitter=2e-4)
+
+
+def config_stress_batched(n_scenes: int = 8, n_bodies: int = 128, seed0: int = 2000, gap: float = 0.02,
+                          jitter: float = 0.005) -> SceneSpec:
+    """Batched stress: n_scenes independent scenes (scene s seeded seed0 + s), each
+    n_bodies of the stress mix packed in an 8 x 8 x 2 lattice with config_stress's
+    layout and gap, offset 40 apart in x."""
+    specs = []
+    for s in range(n_scenes):
+        rng = np.random.default_rng(seed0 + s)
+        meshes = _stress_meshes(rng, n_bodies)
+        p = _packed_lattice(rng, meshes, 8, 8, gap, jitter) + np.array([40.0 * s, 0.0, 0.0])
+        specs.append(build_scene(meshes, np.stack([np.eye(3)] * n_bodies), p, np.zeros((n_bodies, 3, 3)),
+                                 np.zeros((n_bodies, 3)), ground=0.0, name=f"stress-{s}"))
+    return concat_scenes(specs, name="stress-batched")
 
 
 def subset_scene(spec: SceneSpec, bodies: Sequence[int]) -> SceneSpec:

@@ -62,5 +62,5 @@ def bsr_cg(A: BsrMatrix, x, b, max_iters=0, tol=1e-4, check_every=10, info=None,
 def bsr_ldlt(A: BsrMatrix, x, b, max_iters=0, tol=1e-4, check_every=10, info=None):
     """Direct block LDL^T of the 12x12 body blocks (beyond the reference) for systems of at
     most 64 bodies; larger systems, or one that is not positive definite as factored, take
-    bsr_cg (Jacobi) with max_iters / tol / check_every.  info.iters = 1 for a direct solve."""
+    bsr_cg (12x12 block-Jacobi) with max_iters / tol / check_every.  info.iters = 1 for a direct solve."""
     return _solve(_lib.SOLVER_LDLT, A, b, x, tol, max_iters, check_every, info)

@@ -19,8 +19,8 @@ namespace abd {
 
 constexpr int kSolveThreads = 1024;
 // The direct block LDL^T (ldlt_kernel) takes systems of up to kLdltMaxBodies bodies; it
-// marks a system it declines with kLdltFailed in the iteration count, and the Jacobi CG
-// that follows it solves exactly those.
+// marks a system it declines with kLdltFailed in the iteration count, and the block-Jacobi
+// CG that follows it solves exactly those.
 constexpr int kLdltMaxBodies = 64;
 constexpr int kLdltFailed = -2;
 // LDS for a system's staged block structure (row offsets and (column, slot) entries).
@@ -372,12 +372,31 @@ __device__ __forceinline__ void jacobi(const double* __restrict__ dg, const doub
 // matrix in LDS, the 288 entries of each elimination step updated in parallel, each with
 // the arithmetic of the sequential elimination (row c scaled by 1 / pivot, then
 // a[r][j] -= a[r][c] a[c][j] for every other row with a[r][c] != 0).
+// pending (nullable): per-system status of a direct solve; then only the bodies of the
+// systems it declined (kLdltFailed) are inverted, the system of body b found in sboff [nsys + 1].
 constexpr int kInvThreads = 64;
-__global__ __launch_bounds__(kInvThreads) void invert_blocks_kernel(Bsr M, double* __restrict__ dinv) {
+__global__ __launch_bounds__(kInvThreads) void invert_blocks_kernel(Bsr M, double* __restrict__ dinv,
+                                                                    const int32_t* __restrict__ pending,
+                                                                    const int32_t* __restrict__ sboff, int nsys) {
     __shared__ double a[12][24];
     __shared__ double srow[24];
     __shared__ double f[12];
     const int b = blockIdx.x, t = threadIdx.x;
+    if (pending) {
+        int sys = 0;
+        if (sboff) {
+            int lo = 0, hi = nsys - 1;  // the last system whose first body is <= b
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sboff[mid] <= b)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            sys = lo;
+        }
+        if (pending[sys] != kLdltFailed) return;  // uniform over the workgroup
+    }
     for (int e = t; e < 288; e += kInvThreads) {
         const int i = e / 24, j = e % 24;
         a[i][j] = j < 12 ? M.blocks[9 * (16 * (int64_t)b + i / 3 + 4 * (j / 3)) + 3 * (i % 3) + j % 3]
@@ -931,15 +950,18 @@ __global__ void solve_bytes_kernel(const int32_t* __restrict__ row_ptr, int nb, 
 }
 
 // ---------------------------------------------------------------------------
-// Jacobi CG of bsr_utils.py:126-235 for one system too large for one workgroup's
-// LDS (more than kLargeBodies bodies): the rows are split over workgroups of
-// kMwThreads (one row per thread), and each iteration is three launches —
+// Jacobi CG of bsr_utils.py:126-235 (or its 12x12 block-Jacobi form) for one system
+// too large for one workgroup's LDS (more than kLargeBodies bodies): the rows are split
+// over workgroups of kMwThreads (kMwBjThreads, whole bodies, for the block form; one row
+// per thread), and each iteration is three launches —
 // [beta, p update], [SpMV + partial p.Ap], [alpha, x / r / z update + partial r.z].
 // Every workgroup reduces the per-workgroup partials itself, in workgroup order,
 // so all of them hold the same scalars and the sequence is deterministic.  The
 // stopping test (every check_every iterations) sets a device flag that ends the
 // remaining launches of the batch; the host reads it once per batch.
 constexpr int kMwThreads = 256;
+constexpr int kMwBjThreads = 192;  // 16 bodies of 12 rows
+static_assert(kMwBjThreads % 12 == 0 && kMwBjThreads <= kMwThreads, "whole bodies per block-Jacobi workgroup");
 constexpr int kLargeBodies = 170;  // from here one workgroup's SpMV is the bottleneck
 
 struct MwState {
@@ -983,13 +1005,28 @@ __device__ __forceinline__ double mw_jacobi(const double* __restrict__ dg, const
     return (d0 * d0 + d1 * d1 + d2 * d2 == 0.0) ? r : r / fabs(dg[i]);
 }
 
+// Block-Jacobi preconditioner of row i: z_i = (D_b^-1 r_b)_i with the 12 residuals of body b
+// in rs (LDS, the workgroup's rows; a workgroup of kMwBjThreads rows holds whole bodies).
+__device__ __forceinline__ double mw_block_precond(const double* __restrict__ dinv, const double* rs, int i) {
+    const int lr = i % 12;
+    const double* Di = dinv + 144 * (int64_t)(i / 12) + 12 * lr;
+    const double* rb = rs + (threadIdx.x - lr);
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) acc += Di[q] * rb[q];
+    return acc;
+}
+
 // setup: diagonal, x = x0, r = b - H x, z, p = z, partial r.z
+template <bool BJ>
 __global__ __launch_bounds__(kMwThreads) void mw_init_kernel(Bsr M, const double* __restrict__ b,
-                                                             const double* __restrict__ xg, MwState W) {
+                                                             const double* __restrict__ xg, MwState W,
+                                                             const double* __restrict__ dinv) {
     __shared__ double sh[17];
+    __shared__ double rs[BJ ? kMwThreads : 1];
     const int n = 12 * M.nb;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double part = 0.0;
+    double part = 0.0, r = 0.0, z = 0.0;
     if (i < n) {
         // bsr_get_diag: the three diagonal entries of this row's 3x3 diagonal sub-block (a
         // 3-row group may straddle two workgroups, so each thread reads all three itself)
@@ -998,9 +1035,16 @@ __global__ __launch_bounds__(kMwThreads) void mw_init_kernel(Bsr M, const double
         const double d[3] = {D[0], D[4], D[8]};
         W.dg[i] = d[a];
         W.x[i] = xg[i];
-        const double r = b[i] - mw_row(M, xg, i);
+        r = b[i] - mw_row(M, xg, i);
         W.r[i] = r;
-        const double z = (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] == 0.0) ? r : r / fabs(d[a]);
+        if (!BJ) z = (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] == 0.0) ? r : r / fabs(d[a]);
+    }
+    if (BJ) {
+        rs[threadIdx.x] = r;
+        __syncthreads();
+        if (i < n) z = mw_block_precond(dinv, rs, i);
+    }
+    if (i < n) {
         W.z[i] = z;
         W.p[i] = z;
         part = r * z;
@@ -1037,21 +1081,31 @@ __global__ __launch_bounds__(kMwThreads) void mw_spmv_kernel(Bsr M, MwState W) {
 }
 
 // part 2: alpha, x / r / z update, partial r.z
-__global__ __launch_bounds__(kMwThreads) void mw_update_kernel(int n, int iter, MwState W) {
+template <bool BJ>
+__global__ __launch_bounds__(kMwThreads) void mw_update_kernel(int n, int iter, MwState W,
+                                                               const double* __restrict__ dinv) {
     __shared__ double sh[17];
-    if (*W.done) return;
+    __shared__ double rs[BJ ? kMwThreads : 1];
+    if (*W.done) return;  // uniform over the grid
     const int nblk = gridDim.x;
     const double pAp = mw_sum(W.part, nblk, sh);
     const double rz_old = W.rz[iter & 1];
     // unguarded in the reference: p = 0 once the residual vanishes exactly between checks
     const double alpha = pAp != 0.0 ? rz_old / pAp : 0.0;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double part = 0.0;
+    double part = 0.0, r = 0.0, z = 0.0;
     if (i < n) {
         W.x[i] += alpha * W.p[i];
-        const double r = W.r[i] - alpha * W.Ap[i];
+        r = W.r[i] - alpha * W.Ap[i];
         W.r[i] = r;
-        const double z = mw_jacobi(W.dg, r, i);
+        if (!BJ) z = mw_jacobi(W.dg, r, i);
+    }
+    if (BJ) {
+        rs[threadIdx.x] = r;
+        __syncthreads();
+        if (i < n) z = mw_block_precond(dinv, rs, i);
+    }
+    if (i < n) {
         W.z[i] = z;
         part = r * z;
     }
@@ -1120,8 +1174,9 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
         // every body's diagonal-block inverse, one thread per body (inactive scenes included: cheap)
         if (M.nb > 0) {
             // algorithmic bytes: each diagonal block read, its inverse written
-            const int ph_inv = prof_start(kProfPrecond, s, 2304.0 * M.nb);
-            hipLaunchKernelGGL(invert_blocks_kernel, dim3(M.nb), dim3(kInvThreads), 0, s, M, dinv);
+            const int ph_inv = pending ? -1 : prof_start(kProfPrecond, s, 2304.0 * M.nb);
+            hipLaunchKernelGGL(invert_blocks_kernel, dim3(M.nb), dim3(kInvThreads), 0, s, M, dinv, pending, sboff,
+                               nsys);
             ABD_CHECK_LAUNCH();
             prof_stop(ph_inv, s);
         }
@@ -1186,7 +1241,7 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
     return ABD_OK;
 }
 // The direct solve (ldlt_kernel) of every system of at most kLdltMaxBodies bodies, then
-// the Jacobi CG of the systems it declined (status kLdltFailed in iters).
+// the block-Jacobi CG of the systems it declined (status kLdltFailed in iters).
 int launch_ldlt(hipStream_t s, const Bsr& M, int nsys, const int32_t* sboff, const uint8_t* active, int max_bodies,
                 const double* b, double* x, double tol, int32_t maxiter, int32_t check_every, int32_t* iters,
                 double* resid) {
@@ -1210,18 +1265,21 @@ int launch_ldlt(hipStream_t s, const Bsr& M, int nsys, const int32_t* sboff, con
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s, nullptr, dbytes);
     }
-    if ((rc = launch_solve(s, ABD_SOLVER_CG_JACOBI, M, nsys, sboff, active, 12 * max_bodies, b, x, tol, maxiter,
+    if ((rc = launch_solve(s, ABD_SOLVER_CG_BLOCK, M, nsys, sboff, active, 12 * max_bodies, b, x, tol, maxiter,
                            check_every, status, resid, solve_threads(max_bodies), status)))
         return rc;
     return ABD_OK;
 }
 }  // namespace
 
-// Jacobi CG over several workgroups (systems of more than kLargeBodies bodies).
+// Jacobi (bj: 12x12 block-Jacobi) CG over several workgroups (systems of more than
+// kLargeBodies bodies).  The block-Jacobi form runs kMwBjThreads rows per workgroup so that
+// every body's 12 rows share one.
 int solve_cg_multi(hipStream_t s, const Bsr& M, const double* b, double* x, double tol, int32_t maxiter,
-                   int32_t check_every, int32_t* iters, double* resid) {
+                   int32_t check_every, int32_t* iters, double* resid, bool bj) {
     const int n = 12 * M.nb;
-    const int nblk = (n + kMwThreads - 1) / kMwThreads;
+    const int threads = bj ? kMwBjThreads : kMwThreads;
+    const int nblk = (n + threads - 1) / threads;
     if (maxiter <= 0) maxiter = n;
     double* ws;
     const size_t nd = 6 * (size_t)n + 2 * (size_t)nblk + 3;
@@ -1241,18 +1299,37 @@ int solve_cg_multi(hipStream_t s, const Bsr& M, const double* b, double* x, doub
     W.it = W.done + 1;
     int32_t* hdone;
     if ((rc = arena_host(sizeof(int32_t), (void**)&hdone))) return rc;
+    double* dinv = nullptr;
+    if (bj) {
+        if ((rc = arena_get(kSlotPrecond, sizeof(double) * 144 * (size_t)M.nb, (void**)&dinv))) return rc;
+        const int ph_inv = prof_start(kProfPrecond, s, 2304.0 * M.nb);
+        hipLaunchKernelGGL(invert_blocks_kernel, dim3(M.nb), dim3(kInvThreads), 0, s, M, dinv,
+                           (const int32_t*)nullptr, (const int32_t*)nullptr, 1);
+        ABD_CHECK_LAUNCH();
+        prof_stop(ph_inv, s);
+    }
     const double tol_sq = tol * tol * (double)n;
     int ph = prof_start(kProfSolve, s, 0.0);
-    hipLaunchKernelGGL(mw_init_kernel, dim3(nblk), dim3(kMwThreads), 0, s, M, b, (const double*)x, W);
+    if (bj)
+        hipLaunchKernelGGL(mw_init_kernel<true>, dim3(nblk), dim3(threads), 0, s, M, b, (const double*)x, W,
+                           (const double*)dinv);
+    else
+        hipLaunchKernelGGL(mw_init_kernel<false>, dim3(nblk), dim3(threads), 0, s, M, b, (const double*)x, W,
+                           (const double*)dinv);
     ABD_CHECK_LAUNCH();
     hipLaunchKernelGGL(mw_start_kernel, dim3(1), dim3(kMwThreads), 0, s, nblk, tol_sq, maxiter, W);
     ABD_CHECK_LAUNCH();
     for (int i0 = 0; i0 < maxiter; i0 += check_every) {
         const int i1 = i0 + check_every < maxiter ? i0 + check_every : maxiter;
         for (int i = i0; i < i1; ++i) {
-            hipLaunchKernelGGL(mw_spmv_kernel, dim3(nblk), dim3(kMwThreads), 0, s, M, W);
-            hipLaunchKernelGGL(mw_update_kernel, dim3(nblk), dim3(kMwThreads), 0, s, n, i, W);
-            hipLaunchKernelGGL(mw_direction_kernel, dim3(nblk), dim3(kMwThreads), 0, s, n, i, (int)check_every,
+            hipLaunchKernelGGL(mw_spmv_kernel, dim3(nblk), dim3(threads), 0, s, M, W);
+            if (bj)
+                hipLaunchKernelGGL(mw_update_kernel<true>, dim3(nblk), dim3(threads), 0, s, n, i, W,
+                                   (const double*)dinv);
+            else
+                hipLaunchKernelGGL(mw_update_kernel<false>, dim3(nblk), dim3(threads), 0, s, n, i, W,
+                                   (const double*)dinv);
+            hipLaunchKernelGGL(mw_direction_kernel, dim3(nblk), dim3(threads), 0, s, n, i, (int)check_every,
                                tol_sq, W);
         }
         ABD_CHECK_LAUNCH();
@@ -1278,10 +1355,11 @@ int abd_bsr_solve(abd_stream_t stream, int32_t method, int32_t nb, const int32_t
         if (nb <= kLdltMaxBodies)
             return launch_ldlt(to_stream(stream), M, 1, nullptr, nullptr, nb, b, x, tol, maxiter, check_every, iters,
                                resid);
-        method = ABD_SOLVER_CG_JACOBI;  // above the direct solve's size: its fallback
+        method = ABD_SOLVER_CG_BLOCK;  // above the direct solve's size: its fallback
     }
-    if (method == ABD_SOLVER_CG_JACOBI && nb > kLargeBodies)
-        return solve_cg_multi(to_stream(stream), M, b, x, tol, maxiter, check_every, iters, resid);
+    if ((method == ABD_SOLVER_CG_JACOBI || method == ABD_SOLVER_CG_BLOCK) && nb > kLargeBodies)
+        return solve_cg_multi(to_stream(stream), M, b, x, tol, maxiter, check_every, iters, resid,
+                              method == ABD_SOLVER_CG_BLOCK);
     return launch_solve(to_stream(stream), method, M, 1, nullptr, nullptr, 12 * nb, b, x, tol, maxiter, check_every,
                         iters, resid, solve_threads(nb));
 }

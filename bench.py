@@ -31,7 +31,12 @@ the gh-stage rate and the choice of the dominant kernel), then the K timed
 frames, in which only the dominant kernel carries events (its roofline), so that
 per-launch event overhead stays out of `value`.
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload batched|medium|small|stress]
+"--workload stress" runs the 1024-body lattice (and "stress_batched" 8 scenes of 128
+bodies); their contact phase starts once the pile has formed, so run them with
+"--warmup 200" or so.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+                       [--workload batched|medium|small|stress|stress_batched]
 """
 from __future__ import annotations
 
@@ -61,9 +66,14 @@ def make_workload(name: str):
         return scenes.config_small(), "small: 2 unit cubes (8 V / 12 F / 18 E), B at origin, A at 1.05"
     if name == "stress":
         spec = scenes.config_stress()
-        return spec, (f"stress: 1024 bodies on a jittered 10x10x10.24 lattice, gap 0.02 (cube / icosphere-3 / "
-                      f"perturbed icosphere-4 ellipsoid in equal thirds; {spec.n_verts} V / {spec.n_tris} F), "
+        return spec, (f"stress: 1024 bodies packed in a jittered 10x10x10.24 lattice, gap 0.02 (cube / icosphere-3 "
+                      f"/ perturbed icosphere-4 ellipsoid in equal thirds; {spec.n_verts} V / {spec.n_tris} F), "
                       "collapsing onto a floor")
+    if name == "stress_batched":
+        spec = scenes.config_stress_batched()
+        return spec, (f"stress_batched: 8 independent scenes of 128 bodies of the stress mix, each packed in an "
+                      f"8x8x2 lattice with gap 0.02 ({spec.n_verts} V / {spec.n_tris} F in all), collapsing onto a "
+                      "floor")
     if name == "batched":
         spec = scenes.config_batched()
         return spec, (f"batched: 256 independent scenes, U{{2..64}} bodies each ({spec.n_bodies} bodies, "
@@ -120,11 +130,48 @@ def plain_per_scene(spec, cp, sample, warmup: int, steps: int):
     return iters, wall
 
 
-def cpu_baseline(spec, cp, warmup: int, steps: int, budget_s: float, sample=None):
+def reference_solver_line(spec, psd: int, first: int, steps: int):
+    """The batch with the reference's solver, the Jacobi-preconditioned CG of
+    bsr_utils.py:126-235 at its tolerance 1e-4 (solver 1), on a simulator of its own
+    over the same frames as the timed region: Newton iterations per second."""
+    import torch
+    from abd_hip.abd import AffineBodySimulator
+    from abd_hip.const_params import ContactParams
+    cp1 = ContactParams(ground=spec.ground, psd=psd, solver=1)
+    sim = AffineBodySimulator(spec, cp1, "cuda", native=True)
+    for f in range(first):
+        sim.step(f)
+    torch.cuda.synchronize()
+    n0 = sim.stats["newton_iters"]
+    t = time.perf_counter()
+    for f in range(first, first + steps):
+        sim.step(f)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    n = sim.stats["newton_iters"] - n0
+    log(f"[gpu] reference solver (Jacobi CG): {n} Newton iterations in {wall:.3f} s")
+    return dict(value=n / wall, unit="Newton iters/s", solver="jacobi-cg", ms_per_step=1e3 * wall / steps,
+                sample=f"batched native driver, solver 1 (Jacobi CG, tol 1e-4), psd {psd}, frames "
+                       f"{first}..{first + steps - 1}: {n} Newton iterations in {wall:.3f} s")
+
+
+def gpu_state(sim, bodies):
+    """Frame-start state (A0, p0, Adot, pdot, warm start dq) of some bodies of a GPU
+    simulator, on the host."""
+    import numpy as np
+    st = sim.states
+    out = {k: getattr(st, k).cpu().numpy()[bodies] for k in ("A0", "p0", "Adot", "pdot")}
+    out["dq"] = sim.dq.cpu().numpy().reshape(-1, 4, 3)[bodies].reshape(-1, 3)
+    return out
+
+
+def cpu_baseline(spec, cp, warmup: int, steps: int, budget_s: float, sample=None, start=None):
     """The oracle (C FP64 restatement, 1 thread) on the same frames as the timed
     GPU region, stopped after `budget_s` seconds of timed CPU work.  With `sample`
     (scene ids of a batch), the sampled scenes run one after another, each from
-    its own frame 0, sharing the budget."""
+    its own frame 0, sharing the budget.  With `start` (one state per sampled
+    scene, gpu_state), each starts from the GPU's state at the first timed frame
+    instead of simulating the warmup frames."""
     from oracle import oracle as O
     from abd_hip import scenes
     specs = [spec] if sample is None else [scenes.scene_of(spec, s) for s in sample]
@@ -132,7 +179,10 @@ def cpu_baseline(spec, cp, warmup: int, steps: int, budget_s: float, sample=None
     share = budget_s / len(specs)
     for k, sp in enumerate(specs):
         ref = O.OracleSim(sp, cp)
-        for _ in range(warmup):
+        if start is not None:
+            for name, v in start[k].items():
+                getattr(ref, name)[...] = v
+        for _ in range(warmup if start is None else 0):
             ref.step()
         it0 = int(ref.stats[0])
         t_here, f_here = 0.0, 0
@@ -156,8 +206,29 @@ def cpu_baseline(spec, cp, warmup: int, steps: int, budget_s: float, sample=None
                        f"; {iters} Newton iterations in {t_cpu:.2f} s"), frames_done
 
 
-def _cpu_worker(sp, cp, warmup, frames, barrier, q, k):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def host_cores() -> int:
+    """Cores this process may run on (the box's share, not the machine's count)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _cpu_worker(sp, cp, warmup, frames, budget, barrier, q, k):
     # one sampled scene on the oracle in its own process; the timed frames start together
+    # and stop at the budget (whole frames)
     from oracle import oracle as O
     ref = O.OracleSim(sp, cp)
     for _ in range(warmup):
@@ -165,21 +236,28 @@ def _cpu_worker(sp, cp, warmup, frames, barrier, q, k):
     it0 = int(ref.stats[0])
     barrier.wait()
     t = time.perf_counter()
-    for _ in range(frames):
+    f = 0
+    while f < frames and time.perf_counter() - t < budget:
         ref.step()
-    q.put((k, int(ref.stats[0]) - it0, time.perf_counter() - t))
+        f += 1
+    q.put((k, int(ref.stats[0]) - it0, time.perf_counter() - t, f))
 
 
-def cpu_baseline_parallel(spec, cp, warmup: int, frames, sample):
-    """The 1-thread baseline's work (the same scenes and frames) spread over one
-    process per sampled scene, started together after their warmup frames:
-    Newton iterations per second of wall time on that many host cores."""
+def cpu_baseline_parallel(spec, cp, warmup: int, frames: int, budget_s: float, n_cores: int):
+    """All host cores on independent scenes of the batch: one oracle process per core,
+    each on its own scene (n_cores scenes spread evenly over the batch), started
+    together after their warmup frames and run for the same frames as the timed GPU
+    region, stopped at `budget_s` of wall time: Newton iterations per second of wall
+    time, summed over the processes."""
     import multiprocessing as mp
     from abd_hip import scenes
     ctx = mp.get_context("spawn")  # fresh interpreters: nothing of this process's GPU state
+    ns = spec.n_scenes
+    sample = sorted(set(int(round(k * (ns - 1) / max(n_cores - 1, 1))) for k in range(n_cores)))
     n = len(sample)
     barrier, q = ctx.Barrier(n), ctx.Queue()
-    procs = [ctx.Process(target=_cpu_worker, args=(scenes.scene_of(spec, s), cp, warmup, frames[k], barrier, q, k))
+    procs = [ctx.Process(target=_cpu_worker, args=(scenes.scene_of(spec, s), cp, warmup, frames, budget_s, barrier,
+                                                   q, k))
              for k, s in enumerate(sample)]
     import queue
     for p in procs:
@@ -196,10 +274,13 @@ def cpu_baseline_parallel(spec, cp, warmup: int, frames, sample):
     for p in procs:
         p.join()
     iters = sum(r[1] for r in res)
-    wall = max(r[2] for r in res)
-    return dict(value=iters / wall if wall > 0 else None, unit="Newton iters/s", cores=n, kind="port",
-                sample=f"oracle (C FP64 restatement, -O2), one process per scene {sample} on {n} cores, the frames of "
-                       f"the 1-thread run: {iters} Newton iterations in {wall:.2f} s of wall time")
+    rate = sum(r[1] / r[2] for r in res if r[2] > 0)
+    fdone = sum(r[3] for r in res)
+    return dict(value=rate, unit="Newton iters/s", cores=n, kind="port", cpu_model=cpu_model(),
+                sample=f"oracle (C FP64 restatement, -O2), one process per core on {n} scenes spread over the batch "
+                       f"(scenes {sample[0]}..{sample[-1]}), frames {warmup}.. (up to {frames} each, "
+                       f"{budget_s:.0f} s budget): {fdone} frames, {iters} Newton iterations; the sum over "
+                       f"processes of iterations per second")
 
 
 def main():
@@ -222,6 +303,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with one 256 MiB device copy (reference kernel for HBM counter scaling)")
     ap.add_argument("--python-driver", action="store_true", help="run the Python mirror of the driver")
+    ap.add_argument("--no-ref-solver", dest="ref_solver", action="store_false",
+                    help="batched: skip the line with the reference's Jacobi CG beside the chosen solver")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the contact path of one scene is "
@@ -273,6 +356,13 @@ def main():
         cands_dom = [k for _, k in ranked[:1]]
         dom = cands_dom[0] if cands_dom else None
     first = frame
+    torch.cuda.synchronize()
+    start_state = None
+    if a.workload == "stress":
+        start_state = gpu_state(sim, [i + 10 * j + 100 * k for k in range(3) for j in range(3) for i in range(3)])
+    elif a.workload == "stress_batched":
+        import numpy as np
+        start_state = gpu_state(sim, np.nonzero(spec.body_scene == 0)[0])
     torch.cuda.synchronize()
     s0 = dict(sim.stats)
     if dom:
@@ -326,22 +416,32 @@ def main():
         plain = dict(value=p_it / p_wall if p_wall > 0 else None, unit="Newton iters/s",
                      sample=f"plain native driver, one scene at a time, scenes {sample}, frames "
                             f"{first}..{first + a.steps - 1}: {p_it} Newton iterations in {p_wall:.3f} s")
+    ref_solver = None
+    if a.workload == "batched" and a.solver != 1 and a.ref_solver:
+        ref_solver = reference_solver_line(spec, a.psd, first, a.steps)
     cpu, cpu_par = None, None
     if not a.no_cpu:
         if a.workload == "stress":
             # the oracle's brute-force culling cannot run the full stress scene in a bounded time:
-            # its sample is the 3 x 3 x 3 corner block of the lattice (floor layer included), same frames
+            # its sample is the 3 x 3 x 3 corner block of the lattice (floor layer included) from
+            # the GPU's state at the first timed frame, same frames
             from abd_hip import scenes
-            sub = scenes.subset_scene(spec, [i + 10 * j + 100 * k for k in range(3) for j in range(3)
-                                             for i in range(3)])
-            cpu, _ = cpu_baseline(sub, cp, first, a.steps, a.cpu_budget)
-            cpu["sample"] = "27 bodies (the 3x3x3 corner of the lattice), " + cpu["sample"]
+            corner = [i + 10 * j + 100 * k for k in range(3) for j in range(3) for i in range(3)]
+            sub = scenes.subset_scene(spec, corner)
+            cpu, _ = cpu_baseline(sub, cp, first, a.steps, a.cpu_budget, start=[start_state])
+            cpu["sample"] = ("27 bodies (the 3x3x3 corner of the lattice) from the GPU's state at frame "
+                             f"{first}, " + cpu["sample"])
+        elif a.workload == "stress_batched":
+            # one scene of 128 bodies from the GPU's state at the first timed frame
+            cpu, _ = cpu_baseline(spec, cp, first, a.steps, a.cpu_budget, [0], start=[start_state])
+            cpu["sample"] = f"scene 0 from the GPU's state at frame {first}, " + cpu["sample"]
         else:
             cpu_sample = sample if sample is not None or not batched else [0]
             cpu, frames_done = cpu_baseline(spec, cp, first, a.steps, a.cpu_budget, cpu_sample)
-            if batched and len(cpu_sample) > 1:
-                # the same work on one core per sampled scene (scenes are independent)
-                cpu_par = cpu_baseline_parallel(spec, cp, first, frames_done, cpu_sample)
+            if a.workload == "batched":
+                # every host core on its own scene of the batch (scenes are independent)
+                cpu_par = cpu_baseline_parallel(spec, cp, first, a.steps, a.cpu_budget, host_cores())
+        cpu["cpu_model"] = cpu_model()
     out = {
         "metric": "Newton iters/s + barrier-Hessian pairs/s, batched vs plain, 1 MI355X",
         "value": iters / wall,
@@ -371,6 +471,9 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_par,
+        "host_cores": host_cores(),
+        # the same batch with the reference's solver (bsr_utils.py:126-235 Jacobi CG), same frames
+        "reference_solver": ref_solver,
         # per-frame kernel times of the attribution frames (all instrumented kernels timed there;
         # the timed frames time the dominant kernel only)
         "kernels_ms_per_frame": {k: round(v[0] / n_attrib, 3) for k, v in aprof.items() if v[1]} if aprof else None,

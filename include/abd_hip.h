@@ -65,7 +65,7 @@ typedef void* abd_stream_t; /* hipStream_t */
 #define ABD_SOLVER_CG_BLOCK 2 /* CG, 12x12 block-Jacobi preconditioner (no reference counterpart) */
 /* Direct block LDL^T of the 12x12 body blocks (no reference counterpart) for systems of at
  * most 64 bodies; a larger system, or one whose factorization meets a pivot that is not
- * positive, is solved by the Jacobi CG instead.  iters: 1 for a direct solve (else the
+ * positive, is solved by the 12x12 block-Jacobi CG (ABD_SOLVER_CG_BLOCK) instead.  iters: 1 for a direct solve (else the
  * CG's count); resid: |b - H x|^2 for a direct solve. */
 #define ABD_SOLVER_LDLT 3
 

@@ -215,10 +215,11 @@ int orc_step(const orc_scene* S, const orc_params* P, const int32_t* body_scene,
         double res;
         long sit;
         if (P->solver == 3) {
-            /* direct block LDL^T; the Jacobi CG where it does not apply */
+            /* direct block LDL^T; the block-Jacobi CG where it does not apply */
             sit = orc_bsr_ldlt(nslot, rows, cols, blocks, g, dq, n12);
             if (sit < 0)
-                sit = orc_bsr_cg(nslot, rows, cols, blocks, g, dq, n12, P->solver_tol, P->solver_maxiter, 10, &res);
+                sit = orc_bsr_cg_block(nslot, rows, cols, blocks, g, dq, n12, P->solver_tol, P->solver_maxiter, 10,
+                                       &res);
         } else if (P->solver == 2)
             sit = orc_bsr_cg_block(nslot, rows, cols, blocks, g, dq, n12, P->solver_tol, P->solver_maxiter, 10, &res);
         else if (P->solver == 1)

@@ -1,6 +1,6 @@
 """The direct block LDL^T solve (ABD_SOLVER_LDLT, beyond the reference) through
 libabd_hip.so: against the oracle's restatement (same elimination order and sums) and
-numpy's dense solve; its Jacobi-CG fallback (systems above 64 bodies, a pivot that is not
+numpy's dense solve; its block-Jacobi-CG fallback (systems above 64 bodies, a pivot that is not
 positive); one workgroup per scene in a batch against each system solved alone; and the
 Newton driver in this mode against the oracle's driver in the same mode."""
 import ctypes
@@ -101,9 +101,9 @@ def test_ldlt_matches_oracle(lib, nb, npair):
 
 
 @pytest.mark.parametrize("case", ["indefinite", "large"])
-def test_ldlt_falls_back_to_jacobi_cg(lib, case):
-    """A system the factorization declines is solved by the Jacobi CG (bsr_cg) exactly as
-    method 1 would solve it."""
+def test_ldlt_falls_back_to_block_jacobi_cg(lib, case):
+    """A system the factorization declines is solved by the block-Jacobi CG exactly as
+    method 2 would solve it."""
     from abd_hip.sparse import BsrMatrix, SolveInfo, bsr_cg, bsr_ldlt
     rng = np.random.default_rng(7 if case == "indefinite" else 8)
     nb = 10 if case == "indefinite" else 70
@@ -118,7 +118,7 @@ def test_ldlt_falls_back_to_jacobi_cg(lib, case):
     x1, x2 = T(x0), T(x0)
     i1, i2 = SolveInfo(DEV), SolveInfo(DEV)
     bsr_ldlt(A, x1, T(b), 500, 1e-8, 10, i1)
-    bsr_cg(A, x2, T(b), 500, 1e-8, 10, i2)
+    bsr_cg(A, x2, T(b), 500, 1e-8, 10, i2, block_precond=True)
     assert np.array_equal(N(x1), N(x2))
     assert int(N(i1.iters)[0]) == int(N(i2.iters)[0]) > 1
 
@@ -176,7 +176,7 @@ def test_ldlt_batched_equals_each_system_alone(lib):
             assert rel_err(xb[r0:r1].reshape(-1, 3), ref) <= 1e-14
         else:
             assert st == -2 and itb[s] > 1
-            xc, itc, _ = O.bsr_cg(m, ijs, bs, b[4 * r0:4 * r1], x0[4 * r0:4 * r1], 1e-8, 500)
+            xc, itc, _ = O.bsr_cg_block(m, ijs, bs, b[4 * r0:4 * r1], x0[4 * r0:4 * r1], 1e-8, 500)
             assert rel_err(xb[r0:r1].reshape(-1, 3), xc) <= 1e-7
 
 
