@@ -18,7 +18,7 @@ ABD_ERR_ARG = -1000
 ABD_ERR_UNSORTED = -1001
 ABD_ERR_NODEVICE = -1002
 ABD_ERR_CAPACITY = -1003
-TOI_NOALLOC_MAX_CHUNKS = 65536
+TOI_NOALLOC_MAX_CANDIDATES = 1048576
 KIND_POINT, KIND_EDGE, KIND_TRI = 0, 1, 2
 CULL_EE, CULL_PT = 0, 1
 SOLVER_BICGSTAB, SOLVER_CG_JACOBI, SOLVER_CG_BLOCK, SOLVER_LDLT = 0, 1, 2, 3
@@ -152,7 +152,8 @@ def stream_handle(device=None) -> int | None:
 
 
 PROF_NAMES = ["gh_classify", "gh_contact", "gh_pair_reduce", "gh_body_finalize", "energy", "ccd_toi", "cull_filter",
-              "cull_count", "cull_fill", "cull_body_pairs", "cull_vg", "solve", "project", "boxes", "solve_precond"]
+              "cull_count", "cull_fill", "cull_body_pairs", "cull_vg", "solve", "project", "boxes", "solve_precond",
+              "ccd_toi_solve"]
 
 
 def profile_begin():

@@ -180,87 +180,70 @@ __device__ __forceinline__ void deflate(const double c[4], double root, double d
     d[1] = c[2] + root * d[2];
     d[0] = c[1] + root * d[1];
 }
-// cubic_roots.py:95-193 cubic_roots (one pass of the reference's while-True block)
-__device__ inline int cubic_roots(const double coef[4], double x0, double x1, double roots[3]) {
-    roots[0] = roots[1] = roots[2] = 0.0;
-    double y0 = evaluate(coef, x0), y1 = evaluate(coef, x1);
-    double a = coef[3] * 3.0, b_2 = coef[2], c = coef[1];
-    double deriv[4] = {c, 2.0 * b_2, a, 0.0};
-    double delta_4 = b_2 * b_2 - a * c;
-    double qr[2], dp[3];
+// cubic_roots.py:95-193 cubic_roots, split into its bracketing decisions (which interval
+// the Newton / bisection search runs on, and whether the deflated quadratic follows) and
+// the searches themselves: a candidate whose bracket is kNone has no root in [x0, x1]
+// without any search.
+struct CubicBracket {
+    int kind;  // kNone: no root; kOne: one root in [lo, hi]; kDeflate: it, then the
+               // deflated quadratic's roots in [qlo, x1]
+    double lo, hi, ylo, qlo;
+    static constexpr int kNone = 0, kOne = 1, kDeflate = 2;
+};
+__device__ inline CubicBracket cubic_bracket(const double coef[4], double x0, double x1) {
+    const double y0 = evaluate(coef, x0), y1 = evaluate(coef, x1);
+    const double a = coef[3] * 3.0, b_2 = coef[2], c = coef[1];
+    const double delta_4 = b_2 * b_2 - a * c;
     if (delta_4 > 0.0) {
-        double d_2 = sqrt(delta_4);
-        double q = -(b_2 + multi_sign(d_2, b_2));
-        double rv0 = q / a, rv1 = c / q;
-        double xa = wmin(rv0, rv1), xb = wmax(rv0, rv1);
-        bool outside = xa >= x1 || xb <= x0 || (xa <= x0 && xb >= x1);
+        const double d_2 = sqrt(delta_4);
+        const double q = -(b_2 + multi_sign(d_2, b_2));
+        const double rv0 = q / a, rv1 = c / q;
+        const double xa = wmin(rv0, rv1), xb = wmax(rv0, rv1);
+        const bool outside = xa >= x1 || xb <= x0 || (xa <= x0 && xb >= x1);
         if (diff_sign(y0, y1)) {
-            if (outside) {
-                roots[0] = find_closed(coef, deriv, x0, x1, y0);
-                return 1;
-            }
+            if (outside) return {CubicBracket::kOne, x0, x1, y0, 0.0};
         } else if (outside) {
-            return 0;
+            return {CubicBracket::kNone, 0.0, 0.0, 0.0, 0.0};
         }
         if (xa > x0) {
-            double ya = evaluate(coef, xa);
+            const double ya = evaluate(coef, xa);
             if (diff_sign(y0, ya)) {
-                roots[0] = find_closed(coef, deriv, x0, xa, y0);
-                if (diff_sign(ya, y1) || (xb < x1 && diff_sign(ya, evaluate(coef, xb)))) {
-                    deflate(coef, roots[0], dp);
-                    int r = quadratic_roots(dp, xa, x1, qr);
-                    roots[1] = qr[0];
-                    roots[2] = qr[1];
-                    return r + 1;
-                }
-                return 1;
+                const bool more = diff_sign(ya, y1) || (xb < x1 && diff_sign(ya, evaluate(coef, xb)));
+                return {more ? CubicBracket::kDeflate : CubicBracket::kOne, x0, xa, y0, xa};
             }
             if (xb < x1) {
-                double yb = evaluate(coef, xb);
-                if (diff_sign(ya, yb)) {
-                    roots[0] = find_closed(coef, deriv, xa, xb, ya);
-                    if (diff_sign(yb, y1)) {
-                        deflate(coef, roots[0], dp);
-                        int r = quadratic_roots(dp, xb, x1, qr);
-                        roots[1] = qr[0];
-                        roots[2] = qr[1];
-                        return r + 1;
-                    }
-                    return 1;
-                }
-                if (diff_sign(yb, y1)) {
-                    roots[0] = find_closed(coef, deriv, xb, x1, yb);
-                    return 1;
-                }
+                const double yb = evaluate(coef, xb);
+                if (diff_sign(ya, yb))
+                    return {diff_sign(yb, y1) ? CubicBracket::kDeflate : CubicBracket::kOne, xa, xb, ya, xb};
+                if (diff_sign(yb, y1)) return {CubicBracket::kOne, xb, x1, yb, 0.0};
             } else if (diff_sign(ya, y1)) {
-                roots[0] = find_closed(coef, deriv, xa, x1, ya);
-                return 1;
+                return {CubicBracket::kOne, xa, x1, ya, 0.0};
             }
         } else {
-            double yb = evaluate(coef, xb);
-            if (diff_sign(y0, yb)) {
-                roots[0] = find_closed(coef, deriv, x0, xb, y0);
-                if (diff_sign(yb, y1)) {
-                    deflate(coef, roots[0], dp);
-                    int r = quadratic_roots(dp, xb, x1, qr);
-                    roots[1] = qr[0];
-                    roots[2] = qr[1];
-                    return r + 1;
-                }
-                return 1;
-            }
-            if (diff_sign(yb, y1)) {
-                roots[0] = find_closed(coef, deriv, xb, x1, yb);
-                return 1;
-            }
+            const double yb = evaluate(coef, xb);
+            if (diff_sign(y0, yb))
+                return {diff_sign(yb, y1) ? CubicBracket::kDeflate : CubicBracket::kOne, x0, xb, y0, xb};
+            if (diff_sign(yb, y1)) return {CubicBracket::kOne, xb, x1, yb, 0.0};
         }
-        return 0;
+        return {CubicBracket::kNone, 0.0, 0.0, 0.0, 0.0};
     }
-    if (diff_sign(y0, y1)) {
-        roots[0] = find_closed(coef, deriv, x0, x1, y0);
-        return 1;
-    }
-    return 0;
+    if (diff_sign(y0, y1)) return {CubicBracket::kOne, x0, x1, y0, 0.0};
+    return {CubicBracket::kNone, 0.0, 0.0, 0.0, 0.0};
+}
+// the searches of a bracket (one pass of the reference's while-True block)
+__device__ inline int cubic_roots(const double coef[4], double x0, double x1, double roots[3]) {
+    roots[0] = roots[1] = roots[2] = 0.0;
+    const CubicBracket br = cubic_bracket(coef, x0, x1);
+    if (br.kind == CubicBracket::kNone) return 0;
+    const double deriv[4] = {coef[1], 2.0 * coef[2], coef[3] * 3.0, 0.0};
+    roots[0] = find_closed(coef, deriv, br.lo, br.hi, br.ylo);
+    if (br.kind == CubicBracket::kOne) return 1;
+    double qr[2], dp[3];
+    deflate(coef, roots[0], dp);
+    const int r = quadratic_roots(dp, br.qlo, x1, qr);
+    roots[1] = qr[0];
+    roots[2] = qr[1];
+    return r + 1;
 }
 
 // ccd.py:122-129 cubic_binomial: prod_k (b_k + t a_k), constant first, accumulated into acc
@@ -407,6 +390,29 @@ __device__ __forceinline__ double cand_toi(int kind, const SceneView& S, const d
         b[q] = ld3(x1 + 3 * vi[q]);
     }
     return collision_time(kind, a, b);
+}
+// The four vertices of a PT (kind 0) / EE (kind 1) candidate at both ends of the step.
+__device__ __forceinline__ void cand_verts(int kind, const SceneView& S, const double* __restrict__ x0,
+                                           const double* __restrict__ x1, const int32_t* t, d3 a[4], d3 b[4]) {
+    int64_t vi[4];
+    if (kind == 0)
+        fetch_pt_idx(S, t, vi);
+    else
+        fetch_ee_idx(S, t, vi);
+    for (int q = 0; q < 4; ++q) {
+        a[q] = ld3(x0 + 3 * vi[q]);
+        b[q] = ld3(x1 + 3 * vi[q]);
+    }
+}
+// Whether collision_time of the candidate can be below 1: its coplanarity cubic has a
+// bracketed root in [0, 1] (cubic_bracket); otherwise it has no root and its TOI is 1.
+__device__ __forceinline__ bool cand_toi_bracketed(int kind, const SceneView& S, const double* __restrict__ x0,
+                                                   const double* __restrict__ x1, const int32_t* t) {
+    d3 a[4], b[4];
+    cand_verts(kind, S, x0, x1, t, a, b);
+    double coef[4];
+    coplanar_poly(a, b, coef);
+    return cubic_bracket(coef, 0.0, 1.0).kind != CubicBracket::kNone;
 }
 __device__ __forceinline__ double cand_toi_vg(const int32_t* __restrict__ voff, const double* __restrict__ x0,
                                               const double* __restrict__ x1, const int32_t* t2, double ground) {

@@ -22,7 +22,7 @@ inline hipStream_t to_stream(abd_stream_t s) { return reinterpret_cast<hipStream
 // Per-device scratch arena: `slots` independent growable device buffers plus a
 // small pinned host buffer for count read-back.  Growing synchronizes the
 // device first, so a buffer still in use by queued work is never freed early.
-constexpr int kArenaSlots = 38;
+constexpr int kArenaSlots = 40;
 enum ArenaSlot {
     kSlotScanA = 0,
     kSlotScanB,
@@ -63,6 +63,8 @@ enum ArenaSlot {
     kSlotPairChunk,   // contact stage D: per-pair work-item counts and offsets
     kSlotPairPart,    // contact stage D: the work items' partial sums
     kSlotKeepFlags,   // keep_from_superset: each entry's predicate
+    kSlotToiQueue,    // CCD: the candidates whose cubic has a bracketed root
+    kSlotToiSlots,    // CCD: per-scene minima of the queued candidates and the queue count
 };
 // Launch shapes that change no result (abd_set_tuning); process-wide.
 AbdTuning tuning();

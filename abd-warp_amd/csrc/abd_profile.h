@@ -25,6 +25,7 @@ enum ProfKernel {
     kProfProject,       // vertex projection
     kProfBoxes,         // primitive boxes
     kProfPrecond,       // block-Jacobi: diagonal-block inverses
+    kProfToiSolve,      // CCD: root searches of the queued candidates
     kProfCount
 };
 

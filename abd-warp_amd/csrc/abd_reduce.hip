@@ -11,6 +11,7 @@
 // path to the last bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 
 #include "abd_block.h"
@@ -177,25 +178,14 @@ __device__ __forceinline__ double energy_term(int kind, const AbdScene& sc, cons
     return cand_energy_vg(sc.voff, X, t, P.ground, P.d2hat, P.kappa);
 }
 
-// ccd.py:9-40 TOI per candidate
-__device__ __forceinline__ double toi_term(int kind, const AbdScene& sc, const SceneView& V, const AbdParams& P,
-                                           const double* __restrict__ x0, const double* __restrict__ x1,
-                                           const int32_t* t) {
-    if (kind == kListVG) return cand_toi_vg(sc.voff, x0, x1, t, P.ground);
-    return cand_toi(kind == kListPT ? 0 : 1, V, x0, x1, t);
-}
-
 // The line-search trials of chunk_kernel<2>: step lengths alpha [trial][S] (a negative
-// length skips that scene's trial: its sums are 0) and the state; `active` (OP 1,
-// nullable): scenes whose TOI is wanted (the others report 1 without evaluating).
+// length skips that scene's trial: its sums are 0) and the state.
 struct TrialArgs {
     const double *alpha, *A, *p, *dq;
-    const uint8_t* active;
 };
 
-// One chunk per workgroup.  OP 0 sums energies at positions x0; OP 1 takes the minimum
-// TOI below 1 over x0 -> x1; OP 2 sums energies at the trial state blockIdx.y of T
-// (partials part + part_stride blockIdx.y).
+// One chunk per workgroup.  OP 0 sums energies at positions x0; OP 2 sums energies at the
+// trial state blockIdx.y of T (partials part + part_stride blockIdx.y).
 template <int OP>
 __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdParams P, const double* __restrict__ x0,
                                                             const double* __restrict__ x1, TrialArgs T, Lists L,
@@ -207,9 +197,9 @@ __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdPara
     const int64_t c = blockIdx.x;
     if (c >= choff[S]) return;
     const int s = scene_of_chunk(choff, S, c);
-    // skipped scenes (uniform over the workgroup): the neutral element of the reduction
-    if ((OP == 1 && T.active && !T.active[s]) || (OP == 2 && T.alpha[(int64_t)S * blockIdx.y + s] < 0.0)) {
-        if (threadIdx.x == 0) part[c] = OP == 1 ? 1.0 : 0.0;
+    // skipped trials (uniform over the workgroup): the neutral element of the sum
+    if (OP == 2 && T.alpha[(int64_t)S * blockIdx.y + s] < 0.0) {
+        if (threadIdx.x == 0) part[c] = 0.0;
         return;
     }
     int64_t k = c - choff[s];
@@ -228,7 +218,7 @@ __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdPara
     const SceneView V{sc.voff, sc.foff, sc.eoff, sc.tris, sc.edges, sc.x0};
     const int kind = l < kLists ? L.l[l].kind : 0;
     const int w = width_of(kind);
-    double acc = OP == 1 ? 1.0 : 0.0;
+    double acc = 0.0;
     if (l < kLists) {
         if (OP == 0) {
             const Verts X{x0};
@@ -255,14 +245,9 @@ __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdPara
                 for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x)
                     acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i);
             }
-        } else {
-            for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x) {
-                const double t = toi_term(kind, sc, V, P, x0, x1, L.l[l].p + w * i);
-                if (t < 1.0 && t < acc) acc = t;  // `if t < 1: atomic_min`
-            }
         }
     }
-    const double r = OP == 1 ? block_min(acc, sh) : block_sum_fast(acc, sh);
+    const double r = block_sum_fast(acc, sh);
     if (threadIdx.x == 0) part[c] = r;
 }
 
@@ -275,15 +260,138 @@ __global__ __launch_bounds__(kRedThreads) void scene_final_kernel(const int64_t*
     part += part_stride * blockIdx.y;
     out += out_stride * blockIdx.y;
     const int s = blockIdx.x;
-    double acc = OP == 1 ? 1.0 : 0.0;
-    for (int64_t i = choff[s] + threadIdx.x; i < choff[s + 1]; i += blockDim.x) {
-        const double v = part[i];
-        if (OP == 1)
-            acc = v < acc ? v : acc;
-        else
-            acc += v;
+    double acc = 0.0;
+    for (int64_t i = choff[s] + threadIdx.x; i < choff[s + 1]; i += blockDim.x) acc += part[i];
+    const double r = block_sum_fast(acc, sh);
+    if (threadIdx.x == 0) out[s] = r;
+}
+
+// ---------------------------------------------------------------- CCD TOI
+// The scene minimum of the TOIs below 1 (ccd.py:9-40, abd.py:284-299) in two passes.
+// toi_scan_kernel: one candidate per thread (kToiSplit workgroups of kRedThreads per chunk
+// of the plan): the vertex-ground TOI directly, and for a PT / EE candidate its
+// coplanarity cubic and bracket test; the few candidates with a bracketed root in [0, 1]
+// go to a queue (one atomic per wave).  toi_solve_kernel: the full collision_time of each
+// queued candidate (root searches and validity), its TOI below 1 min-ed into its scene's
+// slot (non-negative doubles order like their bit patterns, so an integer atomic min is
+// exact).  toi_final_kernel: per scene the minimum of its chunks' partials and its slot.
+// A minimum does not depend on the order, so the result is the one of evaluating every
+// candidate in turn.
+constexpr int kToiSplit = kChunkItems / kRedThreads;  // one candidate per thread
+constexpr int kToiSolveThreads = 64;
+constexpr int kToiSolveGrid = 2048;
+constexpr unsigned long long kBitsOne = 0x3FF0000000000000ull;  // 1.0
+
+__global__ void toi_init_kernel(int S, unsigned long long* __restrict__ slots, int32_t* __restrict__ qcount) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) slots[s] = kBitsOne;
+    if (s == 0) *qcount = 0;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double u = __shfl_xor(v, o);
+        v = u < v ? u : v;
     }
-    const double r = OP == 1 ? block_min(acc, sh) : block_sum_fast(acc, sh);
+    return v;
+}
+
+__global__ __launch_bounds__(kRedThreads, 4) void toi_scan_kernel(AbdScene sc, AbdParams P, const double* __restrict__ x0,
+                                                               const double* __restrict__ x1, Lists L, int S,
+                                                               const int64_t* __restrict__ seg,
+                                                               const int64_t* __restrict__ choff,
+                                                               const uint8_t* __restrict__ active,
+                                                               double* __restrict__ part, int32_t* __restrict__ queue,
+                                                               int32_t* __restrict__ qcount) {
+    __shared__ double sh[kRedThreads / kWave];
+    const int64_t c = blockIdx.x;
+    if (c >= choff[S]) return;  // uniform over the workgroup
+    const int s = scene_of_chunk(choff, S, c);
+    double acc = 1.0;
+    bool queued = false;
+    int32_t entry = 0;
+    if (!active || active[s]) {
+        int64_t k = c - choff[s];
+        int l = 0;
+        int64_t start = 0, end = 0;
+        for (; l < kLists; ++l) {
+            const int64_t a = seg[2 * (kLists * (int64_t)s + l)], b = seg[2 * (kLists * (int64_t)s + l) + 1];
+            const int64_t nch = n_chunks(b - a);
+            if (k < nch) {
+                start = a + k * kChunkItems;
+                end = start + kChunkItems < b ? start + kChunkItems : b;
+                break;
+            }
+            k -= nch;
+        }
+        const int64_t i = start + (int64_t)blockIdx.y * kRedThreads + threadIdx.x;
+        if (l < kLists && i < end) {
+            const int kind = L.l[l].kind;
+            const int32_t* t = L.l[l].p + (int64_t)width_of(kind) * i;
+            if (kind == kListVG) {
+                const double tv = cand_toi_vg(sc.voff, x0, x1, t, P.ground);
+                if (tv < 1.0) acc = tv;  // `if t < 1: atomic_min`
+            } else {
+                const SceneView V{sc.voff, sc.foff, sc.eoff, sc.tris, sc.edges, sc.x0};
+                queued = cand_toi_bracketed(kind == kListPT ? 0 : 1, V, x0, x1, t);
+                entry = (int32_t)((i << 1) | (kind == kListEE ? 1 : 0));
+            }
+        }
+    }
+    // wave-aggregated push of the queued candidates
+    const unsigned long long mask = __ballot(queued);
+    if (mask) {
+        const int lane = lane_id();
+        const int leader = __ffsll((long long)mask) - 1;
+        int32_t base = 0;
+        if (lane == leader) base = atomicAdd(qcount, (int32_t)__popcll(mask));
+        base = __shfl(base, leader);
+        if (queued) queue[base + __popcll(mask & ((1ull << lane) - 1ull))] = entry;
+    }
+    const double wm = wave_min(acc);
+    if (lane_id() == 0) sh[wave_id()] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sh[0];
+        for (int w = 1; w < kRedThreads / kWave; ++w) r = sh[w] < r ? sh[w] : r;
+        part[c * kToiSplit + blockIdx.y] = r;
+    }
+}
+
+__global__ __launch_bounds__(kToiSolveThreads) void toi_solve_kernel(AbdScene sc, Lists L, int S,
+                                                                     const double* __restrict__ x0,
+                                                                     const double* __restrict__ x1,
+                                                                     const int32_t* __restrict__ queue,
+                                                                     const int32_t* __restrict__ qcount,
+                                                                     unsigned long long* __restrict__ slots) {
+    const int32_t n = *qcount;
+    const SceneView V{sc.voff, sc.foff, sc.eoff, sc.tris, sc.edges, sc.x0};
+    for (int32_t q = blockIdx.x * kToiSolveThreads + threadIdx.x; q < n; q += gridDim.x * kToiSolveThreads) {
+        const int32_t e = queue[q];
+        const int ee = e & 1;
+        const ListRef& Lr = L.l[ee ? 0 : 1];  // the lists' order: EE, PT, VG
+        const int32_t* t = Lr.p + 5 * (int64_t)(e >> 1);
+        const double tt = cand_toi(ee, V, x0, x1, t);
+        if (tt < 1.0) {
+            const int s = S > 1 ? sc.body_scene[t[0]] : 0;
+            atomicMin(slots + s, (unsigned long long)__double_as_longlong(tt));
+        }
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void toi_final_kernel(const int64_t* __restrict__ choff,
+                                                                const double* __restrict__ part,
+                                                                const unsigned long long* __restrict__ slots,
+                                                                double* __restrict__ out) {
+    __shared__ double sh[kRedThreads];
+    const int s = blockIdx.x;
+    double acc = __longlong_as_double((long long)slots[s]);
+    for (int64_t i = choff[s] * kToiSplit + threadIdx.x; i < choff[s + 1] * kToiSplit; i += blockDim.x) {
+        const double v = part[i];
+        acc = v < acc ? v : acc;
+    }
+    const double r = block_min(acc, sh);
     if (threadIdx.x == 0) out[s] = r;
 }
 
@@ -330,12 +438,17 @@ int64_t chunk_bound(const Lists& L, int S) {
     return bound;
 }
 
-// Caller-provided scratch of a reduction (the allocation-free entry point's): room for
-// the plan and for `part_cap` chunk partials.  Null members: the arena's.
+// Caller-provided scratch of the TOI (the allocation-free entry point's): the plan, the
+// scene slots and queue count, room for part_cap chunk partials (kToiSplit per chunk) and
+// queue_cap queued candidates.  Null: the arena's.
 struct ReduceScratch {
     int64_t* plan = nullptr;
+    unsigned long long* slots = nullptr;
+    int32_t* qcount = nullptr;
     double* part = nullptr;
     int64_t part_cap = 0;
+    int32_t* queue = nullptr;
+    int64_t queue_cap = 0;
 };
 
 // The per-scene segments and chunk offsets of the lists: plan_in when given (made by
@@ -358,26 +471,67 @@ int get_plan(hipStream_t s, const AbdScene& sc, int S, const Lists& L, const int
     return ABD_OK;
 }
 
-// Plan + chunks + per-scene results for OP over the lists; S = 1 treats the lists as one scene.
-// scr (nullable): scratch of the caller instead of the arena (then no launch is timed).
+// Plan + chunks + per-scene sums (OP 0) over the lists; S = 1 treats the lists as one scene.
 template <int OP>
 int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, const double* x0, const double* x1,
-               const Lists& L, const int64_t* plan_in, double* out, int prof_id, double prof_bytes,
-               const uint8_t* active = nullptr, const ReduceScratch* scr = nullptr) {
+               const Lists& L, const int64_t* plan_in, double* out, int prof_id, double prof_bytes) {
     const int64_t bound = chunk_bound(L, S);
     if (bound > INT32_MAX) return ABD_ERR_ARG;
-    if (scr && bound > scr->part_cap) return ABD_ERR_CAPACITY;
     const int64_t *seg, *choff;
-    double* part = scr ? scr->part : nullptr;
-    int rc = get_plan(s, sc, S, L, plan_in, &seg, &choff, scr ? scr->plan : nullptr);
-    if (!rc && !part) rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound, (void**)&part);
+    double* part = nullptr;
+    int rc = get_plan(s, sc, S, L, plan_in, &seg, &choff);
+    if (!rc) rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound, (void**)&part);
     if (rc) return rc;
-    int ph = scr ? -1 : prof_start_ext(prof_id, prof_bytes);
+    int ph = prof_start_ext(prof_id, prof_bytes);
     prof_launch(ph, chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1,
-                TrialArgs{nullptr, nullptr, nullptr, nullptr, active}, L, S, seg, choff, part, (int64_t)0);
+                TrialArgs{nullptr, nullptr, nullptr, nullptr}, L, S, seg, choff, part, (int64_t)0);
     ABD_CHECK_LAUNCH();
     hipLaunchKernelGGL(scene_final_kernel<OP>, dim3(S), dim3(kRedThreads), 0, s, choff, part, out, (int64_t)0,
                        (int64_t)0);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
+// Per-scene TOI (toi_scan_kernel, toi_solve_kernel, toi_final_kernel) with scratch from
+// scr (nullable: the arena; then the scan and the solve are timed when profiling).
+int toi_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, const double* x0, const double* x1,
+               const Lists& L, const int64_t* plan_in, double* out, const uint8_t* active,
+               const ReduceScratch* scr) {
+    const int64_t bound = chunk_bound(L, S);
+    if (bound * kToiSplit > INT32_MAX) return ABD_ERR_ARG;
+    const int64_t nq = L.l[0].n + L.l[1].n;  // every PT and EE candidate may queue
+    if (nq > (int64_t)INT32_MAX / 2) return ABD_ERR_ARG;
+    if (scr && (bound * kToiSplit > scr->part_cap || nq > scr->queue_cap)) return ABD_ERR_CAPACITY;
+    ReduceScratch w;
+    if (scr) {
+        w = *scr;
+    } else {
+        int rc;
+        if ((rc = arena_get(kSlotReduce, sizeof(double) * (size_t)(bound * kToiSplit), (void**)&w.part))) return rc;
+        if ((rc = arena_get(kSlotToiQueue, sizeof(int32_t) * (size_t)(nq + 1), (void**)&w.queue))) return rc;
+        if ((rc = arena_get(kSlotToiSlots, sizeof(unsigned long long) * (size_t)(S + 1), (void**)&w.slots)))
+            return rc;
+        w.qcount = reinterpret_cast<int32_t*>(w.slots + S);
+    }
+    const int64_t *seg, *choff;
+    int rc = get_plan(s, sc, S, L, plan_in, &seg, &choff, w.plan);
+    if (rc) return rc;
+    hipLaunchKernelGGL(toi_init_kernel, dim3((unsigned)ceil_div(S, kRedThreads)), dim3(kRedThreads), 0, s, S,
+                       w.slots, w.qcount);
+    ABD_CHECK_LAUNCH();
+    // algorithmic bytes of the scan: tuple, indices and four vertices at both ends per PT / EE
+    // candidate (224 / 228 B), tuple, offset and a vertex at both ends per VG candidate (56 B)
+    const int ph = scr ? -1 : prof_start_ext(kProfToi, 224.0 * L.l[1].n + 228.0 * L.l[0].n + 56.0 * L.l[2].n);
+    prof_launch(ph, toi_scan_kernel, dim3((unsigned)bound, kToiSplit), dim3(kRedThreads), 0, s, sc, P, x0, x1, L, S,
+                seg, choff, active, w.part, w.queue, w.qcount);
+    ABD_CHECK_LAUNCH();
+    const unsigned grid = (unsigned)std::min<int64_t>(kToiSolveGrid, std::max<int64_t>(1, ceil_div(nq, 4 * kToiSolveThreads)));
+    const int ph2 = scr ? -1 : prof_start_ext(kProfToiSolve, 0.0);
+    prof_launch(ph2, toi_solve_kernel, dim3(grid), dim3(kToiSolveThreads), 0, s, sc, L, S, x0, x1,
+                (const int32_t*)w.queue, (const int32_t*)w.qcount, w.slots);
+    ABD_CHECK_LAUNCH();
+    hipLaunchKernelGGL(toi_final_kernel, dim3(S), dim3(kRedThreads), 0, s, choff, (const double*)w.part,
+                       (const unsigned long long*)w.slots, out);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }
@@ -458,7 +612,7 @@ int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, in
     if ((rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound * K, (void**)&part))) return rc;
     int ph = prof_start_ext(kProfEnergy, K * (128.0 * npt + 132.0 * nee + 32.0 * nvg));
     prof_launch(ph, chunk_kernel<2>, dim3((unsigned)bound, K), dim3(kRedThreads), 0, s, *sc, *P,
-                (const double*)nullptr, (const double*)nullptr, TrialArgs{alpha, A, p, dq, nullptr}, L, S, seg, choff,
+                (const double*)nullptr, (const double*)nullptr, TrialArgs{alpha, A, p, dq}, L, S, seg, choff,
                 part, bound);
     ABD_CHECK_LAUNCH();
     hipLaunchKernelGGL(scene_final_kernel<0>, dim3(S, K), dim3(kRedThreads), 0, s, choff, (const double*)part, E,
@@ -492,17 +646,22 @@ int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, cons
     if (!P || !x_t0 || !x_t1 || !toi || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
     const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);  // abd.py:288-299
-    return seg_reduce<1>(s, *sc, *P, S, x_t0, x_t1, L, plan, toi, kProfToi,
-                         224.0 * L.l[1].n + 228.0 * L.l[0].n + 56.0 * L.l[2].n, active);
+    return toi_reduce(s, *sc, *P, S, x_t0, x_t1, L, plan, toi, active, nullptr);
 }
 
 // ------------------------------------------------- allocation-free TOI scratch
-// The fixed scratch of abd_toi_noalloc: the chunk plan (kNoAllocPlanElems int64 at the
-// front) and ABD_TOI_NOALLOC_MAX_CHUNKS chunk partials.  In host code `scratch` names a
-// host-side stand-in, not device memory: its device address is looked up once per device
-// (noalloc_scratch) and only that address reaches kernels.
+// The fixed scratch of abd_toi_noalloc: the chunk plan (kNoAllocPlanElems int64), the scene
+// slot and queue count, kNoAllocPartElems chunk partials and a queue of
+// ABD_TOI_NOALLOC_MAX_CANDIDATES int32 (in doubles: 2 entries each).  In host code `scratch`
+// names a host-side stand-in, not device memory: its device address is looked up once per
+// device (noalloc_scratch) and only that address reaches kernels.
 constexpr int64_t kNoAllocPlanElems = 64;
-static __device__ __align__(256) double scratch[kNoAllocPlanElems + ABD_TOI_NOALLOC_MAX_CHUNKS];
+constexpr int64_t kNoAllocSlotElems = 64;
+// chunks of every list at most: 3 + (all candidates in one list) / 2048, kToiSplit partials each
+constexpr int64_t kNoAllocPartElems = kToiSplit * (8 + ABD_TOI_NOALLOC_MAX_CANDIDATES / kChunkItems);
+constexpr int64_t kNoAllocQueueElems = ABD_TOI_NOALLOC_MAX_CANDIDATES / 2;
+static __device__ __align__(256) double scratch[kNoAllocPlanElems + kNoAllocSlotElems + kNoAllocPartElems +
+                                                kNoAllocQueueElems];
 
 namespace {
 constexpr int kNoAllocDevices = 64;
@@ -522,8 +681,12 @@ int noalloc_scratch(ReduceScratch* out) {
     }
     double* base = static_cast<double*>(g_noalloc_ptr[dev]);
     out->plan = reinterpret_cast<int64_t*>(base);
-    out->part = base + kNoAllocPlanElems;
-    out->part_cap = ABD_TOI_NOALLOC_MAX_CHUNKS;
+    out->slots = reinterpret_cast<unsigned long long*>(base + kNoAllocPlanElems);
+    out->qcount = reinterpret_cast<int32_t*>(base + kNoAllocPlanElems + 1);
+    out->part = base + kNoAllocPlanElems + kNoAllocSlotElems;
+    out->part_cap = kNoAllocPartElems;
+    out->queue = reinterpret_cast<int32_t*>(base + kNoAllocPlanElems + kNoAllocSlotElems + kNoAllocPartElems);
+    out->queue_cap = ABD_TOI_NOALLOC_MAX_CANDIDATES;
     return ABD_OK;
 }
 static_assert(2 * kLists + 2 <= kNoAllocPlanElems, "one scene's plan fits the front of the scratch");
@@ -562,11 +725,12 @@ int abd_toi_noalloc(abd_stream_t stream, const AbdScene* sc, const AbdParams* P,
     if (!sc || !P || !x_t0 || !x_t1 || !toi || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
     const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);
-    if (chunk_bound(L, 1) > ABD_TOI_NOALLOC_MAX_CHUNKS) return ABD_ERR_CAPACITY;  // before any device call
+    // before any device call
+    if (L.l[0].n + L.l[1].n + L.l[2].n > ABD_TOI_NOALLOC_MAX_CANDIDATES) return ABD_ERR_CAPACITY;
     ReduceScratch scr;
     int rc = noalloc_scratch(&scr);
     if (rc) return rc;
-    return seg_reduce<1>(to_stream(stream), *sc, *P, 1, x_t0, x_t1, L, nullptr, toi, kProfToi, 0.0, nullptr, &scr);
+    return toi_reduce(to_stream(stream), *sc, *P, 1, x_t0, x_t1, L, nullptr, toi, nullptr, &scr);
 }
 
 int abd_scene_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x_t0,

@@ -260,14 +260,14 @@ int abd_toi(abd_stream_t stream, const AbdScene* scene, const AbdParams* params,
             int64_t nvg, double* toi);
 /* abd_toi without allocation (no reference counterpart; for callers that cannot call
  * hipMalloc: code under HIP graph capture, serving loops, harnesses that forbid it).
- * Same arguments, kernels and result as abd_toi, bit for bit; its scratch (the chunk plan
- * and one partial minimum per 2048-candidate chunk of each list) is a fixed array in
- * device memory, ABD_TOI_NOALLOC_MAX_CHUNKS chunks (512 KiB, held by every device the
- * library runs on until the process ends): sum over the PT, EE and VG lists of
- * ceil(n / 2048), plus 3, must not exceed it, else ABD_ERR_CAPACITY and nothing is
- * launched.  The first call on a device looks the array up (no allocation).  All calls
- * on one device share the array: calls that overlap in time on one device race. */
-#define ABD_TOI_NOALLOC_MAX_CHUNKS 65536
+ * Same arguments, kernels and result as abd_toi, bit for bit; its scratch (the chunk plan,
+ * the chunks' partial minima and the queue of candidates whose cubic has a bracketed root)
+ * is a fixed array in device memory of about 4.1 MiB, held by every device the library
+ * runs on until the process ends: npt + nee + nvg must not exceed
+ * ABD_TOI_NOALLOC_MAX_CANDIDATES, else ABD_ERR_CAPACITY and nothing is launched.  The
+ * first call on a device looks the array up (no allocation).  All calls on one device
+ * share the array: calls that overlap in time on one device race. */
+#define ABD_TOI_NOALLOC_MAX_CANDIDATES 1048576
 int abd_toi_noalloc(abd_stream_t stream, const AbdScene* scene, const AbdParams* params, const double* x_t0,
                     const double* x_t1, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
                     const int32_t* vg, int64_t nvg, double* toi);
@@ -347,7 +347,7 @@ int abd_sim_destroy(void* sim);
  * 0 gh classify, 1 gh contact terms, 2 gh pair reduce, 3 gh body finalize,
  * 4 energy, 5 CCD, 6 cull filter, 7 cull count, 8 cull fill, 9 body pairs,
  * 10 vertex-ground cull, 11 Krylov solve, 12 vertex projection, 13 boxes,
- * 14 block-Jacobi diagonal-block inverses. */
+ * 14 block-Jacobi diagonal-block inverses, 15 CCD root searches. */
 int abd_profile_begin(void);
 int abd_profile_end(void);
 /* Kernel ids timed while profiling (bit k = id k; all by default): timing only the

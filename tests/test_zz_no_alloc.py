@@ -61,9 +61,14 @@ def test_noalloc_toi_matches_abd_toi():
     _lib.call("abd_project_vertices", _lib.stream_handle(), ctypes.byref(m.c), _lib.ptr(st.A), _lib.ptr(p1),
               _lib.ptr(x1))
     pt, ee, vg = _swept_set(spec, cp, m, x1)
+    print(f"swept set: {pt.shape[0]} PT, {ee.shape[0]} EE, {vg.shape[0]} VG", flush=True)
+    # the first call as large as the fixed scratch takes (the lists' leading parts)
+    cap = _lib.TOI_NOALLOC_MAX_CANDIDATES
+    k_pt = min(int(pt.shape[0]), cap // 3)
+    k_ee = min(int(ee.shape[0]), cap - k_pt - int(vg.shape[0]))
+    pt, ee = pt[:k_pt].contiguous(), ee[:k_ee].contiguous()
     n = (int(pt.shape[0]), int(ee.shape[0]), int(vg.shape[0]))
-    print(f"swept set: {n[0]} PT, {n[1]} EE, {n[2]} VG", flush=True)
-    assert sum(-(-k // 2048) for k in n) > 8          # several chunks per list
+    assert sum(n) <= cap and sum(n) > cap // 2
     t_ref = _toi("abd_toi", m, pc, pt, ee, vg)
     t = _toi("abd_toi_noalloc", m, pc, pt, ee, vg)
     assert t_ref < 1.0
@@ -90,7 +95,7 @@ def test_noalloc_toi_rejects_lists_beyond_its_scratch():
     m = Meshes(spec, "cuda")
     pt = torch.zeros((1, 5), dtype=torch.int32, device="cuda")
     out = torch.ones(1, dtype=torch.float64, device="cuda")
-    too_many = 2048 * _lib.TOI_NOALLOC_MAX_CHUNKS   # the size check runs before any launch
+    too_many = _lib.TOI_NOALLOC_MAX_CANDIDATES + 1   # the size check runs before any launch
     rc = _lib.load().abd_toi_noalloc(_lib.stream_handle(), ctypes.byref(m.c), ctypes.byref(pc), _lib.ptr(m.x),
                                      _lib.ptr(m.x), _lib.ptr(pt), too_many, None, 0, None, 0, _lib.ptr(out))
     assert rc == _lib.ABD_ERR_CAPACITY
