@@ -269,12 +269,20 @@ __device__ bool contact_terms_full(int kind, const d3 v[4], const d3 r[4], doubl
 }
 
 // stage A: active flags over [PT | EE], plus ordering checks.
+// dn (nullable): the list sizes on the device, [nij, nij, npt, nee, nvg] (the arguments are
+// then upper bounds that only size the launch)
 __global__ __launch_bounds__(kBlock) void classify_kernel(AbdScene sc, AbdParams P, const double* __restrict__ x,
                                                            const int32_t* __restrict__ pt, int64_t npt,
                                                            const int32_t* __restrict__ ee, int64_t nee,
                                                            const int32_t* __restrict__ vg, int64_t nvg,
+                                                           const int64_t* __restrict__ dn,
                                                            uint8_t* __restrict__ flag, int* __restrict__ err) {
     const SceneView S = view_of(sc);
+    if (dn) {
+        npt = dn[2];
+        nee = dn[3];
+        nvg = dn[4];
+    }
     const int64_t n = npt + nee;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n + nvg; i += (int64_t)gridDim.x * kBlock) {
         if (i >= n) {
@@ -312,10 +320,12 @@ __global__ __launch_bounds__(kBlock) void classify_kernel(AbdScene sc, AbdParams
 // stage B: ordered compaction of the flags (count / write passes).
 template <bool WRITE>
 __global__ __launch_bounds__(kBlock) void compact_flags_kernel(const uint8_t* __restrict__ flag, int64_t n,
+                                                                const int64_t* __restrict__ dn,
                                                                 int64_t* __restrict__ bcnt,
                                                                 const int64_t* __restrict__ boff,
                                                                 int32_t* __restrict__ out) {
     __shared__ int shw[kBlock / kWave];
+    if (dn) n = dn[2] + dn[3];
     int64_t base = (int64_t)blockIdx.x * kBlock * kCItems;
     int64_t run = WRITE ? boff[blockIdx.x] : 0;
     for (int it = 0; it < kCItems; ++it) {
@@ -345,11 +355,15 @@ __global__ __launch_bounds__(kBlock) void contact_kernel(AbdScene sc, AbdParams 
                                                           const int32_t* __restrict__ pt, int64_t npt,
                                                           const int32_t* __restrict__ ee,
                                                           const int32_t* __restrict__ act, int64_t nact,
+                                                          const int64_t* __restrict__ dnact,
+                                                          const int64_t* __restrict__ dn,
                                                           double* __restrict__ cbuf, int64_t* __restrict__ ckey,
                                                           uint8_t* __restrict__ swap, uint8_t* __restrict__ hl_pt,
                                                           uint8_t* __restrict__ hl_ee, uint8_t* __restrict__ pflag,
                                                           int64_t nij, int64_t* __restrict__ pseg) {
     const SceneView S = view_of(sc);
+    if (dnact) nact = *dnact;
+    if (dn) npt = dn[2];
     for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nact; e += (int64_t)gridDim.x * kBlock) {
         const int64_t i = act[e];
         const bool is_ee = i >= npt;
@@ -475,8 +489,10 @@ __global__ __launch_bounds__(kPairThreads) void pair_partial_kernel(int64_t nij,
 
 // stage D3: per body pair with active contacts (grid-stride): its chunks' partial sums
 // added in order (PT chunks, then EE), diagonal parts to the pair buffer (stage E adds
-// them per body), the off-diagonal block to the matrix.
+// them per body), the off-diagonal block to the matrix.  nij: pairs of the buffers (an upper
+// bound when dn gives the count of the matrix's pairs, dn[0], which places its lower slots).
 __global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64_t nij,
+                                                                    const int64_t* __restrict__ dn,
                                                                     const uint8_t* __restrict__ pflag,
                                                                     const int64_t* __restrict__ coff,
                                                                     const int64_t* __restrict__ cnt,
@@ -485,6 +501,7 @@ __global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64
                                                                     double* __restrict__ blocks) {
     const int t = threadIdx.x;
     const int R = t / 24, C = t % 24;
+    const int64_t nij_m = dn ? dn[0] : nij;
     for (int64_t ii = blockIdx.x; ii < nij; ii += gridDim.x) {
         if (!pflag[ii]) continue;  // uniform over the workgroup
         const int64_t c0 = coff[ii], nc = cnt[ii];
@@ -501,7 +518,7 @@ __global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64
         if (R < 12 && C >= 12) {
             const int c2 = C - 12;
             const int bi = R / 3, rr = R % 3, bj = c2 / 3, cc = c2 % 3;
-            const int64_t up = nb + ii, lo = nb + nij + ii;
+            const int64_t up = nb + ii, lo = nb + nij_m + ii;
             blocks[9 * (16 * up + bi + 4 * bj) + 3 * rr + cc] += h;
             blocks[9 * (16 * lo + bj + 4 * bi) + 3 * cc + rr] += h;
         }
@@ -523,6 +540,7 @@ __device__ __forceinline__ int64_t lower_bound_body(const int32_t* __restrict__ 
 // stage E: one workgroup per body (144 Hessian entries + 12 gradient entries).
 __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdParams P,
                                                                 const double* __restrict__ x, int64_t nij,
+                                                                const int64_t* __restrict__ dn,
                                                                 const int32_t* __restrict__ row_ptr,
                                                                 const int32_t* __restrict__ slots,
                                                                 const uint8_t* __restrict__ pflag,
@@ -535,6 +553,10 @@ __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdP
     const int b = blockIdx.x;
     const int t = threadIdx.x;
     const int nb = sc.n_bodies;
+    if (dn) {
+        nij = dn[0];
+        nvg = dn[4];
+    }
     // threads 0..143: the entries of the diagonal block; 144..155: the gradient; the rest
     // only help to evaluate the vertex-ground terms
     const bool act = t < 156;
@@ -712,24 +734,21 @@ __global__ void inertia_hessian_kernel(int nb, AbdParams P, const double* __rest
     for (int k = 0; k < 9; ++k) B[k] += dh[k];
 }
 
-}  // namespace abd
-
-using namespace abd;
-
-extern "C" {
-
-int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x, const int32_t* pt,
-                   int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t nij,
-                   const int32_t* row_ptr, const int32_t* slots, double* g, double* blocks, uint8_t* highlight,
-                   int64_t* n_active) {
-    if (!sc || !P || !x || !g || !blocks || npt < 0 || nee < 0 || nvg < 0 || nij < 0) return ABD_ERR_ARG;
-    if ((npt && !pt) || (nee && !ee) || (nvg && !vg) || (nij && (!row_ptr || !slots))) return ABD_ERR_ARG;
-    if (nij == 0 && (npt || nee)) return ABD_ERR_ARG;
-    hipStream_t s = to_stream(stream);
+// The gh stage (stages A-E).  With L.dn the list sizes are on the device (L's counts bound
+// them and size the launches and buffers), nothing is read back and the lists are taken
+// as grouped (the drivers' own culling output); n_active_dev (nullable) then receives the
+// active count on the device.  Without it the call reads the active count and the
+// ordering flag back (n_active, host, nullable) and returns ABD_ERR_UNSORTED for lists
+// that are not grouped.
+int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const double* x, const GhLists& L,
+               const int32_t* row_ptr, const int32_t* slots, double* g, double* blocks, uint8_t* highlight,
+               int64_t* n_active, int64_t* n_active_dev) {
     const int nb = sc->n_bodies;
-    if (n_active) *n_active = 0;
     if (nb == 0) return ABD_OK;
-    if (!P->vg) nvg = 0;
+    const int32_t *pt = L.pt, *ee = L.ee, *vg = L.vg;
+    const int64_t npt = L.npt, nee = L.nee, nij = L.nij;
+    const int64_t nvg = P->vg ? L.nvg : 0;
+    const int64_t* dn = L.dn;
     const int64_t n = npt + nee;
     int rc;
     // misc: err flag, highlight bytes [2 nb], active-pair flags [nij]
@@ -750,11 +769,12 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
     if (n + nvg > 0) {
         uint8_t* flag;
         if ((rc = arena_get(kSlotFlags, (size_t)n + 1, (void**)&flag))) return rc;
-        int ph = prof_start(kProfClassify, s, 129.0 * npt + 133.0 * nee + 8.0 * nvg);
+        int ph = prof_start(kProfClassify, s, dn ? 1.0 : 129.0 * npt + 133.0 * nee + 8.0 * nvg);
         hipLaunchKernelGGL(classify_kernel, dim3(grid_for(n + nvg, kBlock, 8192)), dim3(kBlock), 0, s, *sc, *P, x,
-                           pt, npt, ee, nee, vg, nvg, flag, err);
+                           pt, npt, ee, nee, vg, nvg, dn, flag, err);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s);
+        if ((rc = prof_count_bytes(ph, s, dn, 129.0, 133.0, 8.0, 0.0))) return rc;
         if (n > 0) {
             const int64_t nblk = ceil_div(n, (int64_t)kBlock * kCItems);
             int64_t *bcnt, *boff;
@@ -762,26 +782,32 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
             if ((rc = arena_get(kSlotScanB, sizeof(int64_t) * (nblk + 1), (void**)&boff))) return rc;
             if ((rc = arena_get(kSlotList, sizeof(int32_t) * (size_t)n, (void**)&act))) return rc;
             hipLaunchKernelGGL(compact_flags_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s,
-                               (const uint8_t*)flag, n, bcnt, (const int64_t*)nullptr, (int32_t*)nullptr);
+                               (const uint8_t*)flag, n, dn, bcnt, (const int64_t*)nullptr, (int32_t*)nullptr);
             ABD_CHECK_LAUNCH();
-            if ((rc = scan_exclusive_i64(s, bcnt, boff, nblk, boff + nblk))) return rc;
+            if ((rc = scan_exclusive_i64(s, bcnt, boff, nblk, boff + nblk, n_active_dev))) return rc;
             hipLaunchKernelGGL(compact_flags_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s,
-                               (const uint8_t*)flag, n, bcnt, (const int64_t*)boff, act);
+                               (const uint8_t*)flag, n, dn, bcnt, (const int64_t*)boff, act);
             ABD_CHECK_LAUNCH();
             dev_nact = boff + nblk;
         }
     }
-    // the active count and the ordering flag in one read-back
-    int64_t* hv;
-    if ((rc = arena_host(2 * sizeof(int64_t), (void**)&hv))) return rc;
-    hv[0] = 0;
-    if (dev_nact) ABD_CHECK(hipMemcpyAsync(hv, dev_nact, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    ABD_CHECK(hipMemcpyAsync(hv + 1, err, sizeof(int), hipMemcpyDeviceToHost, s));
-    ABD_CHECK(hipStreamSynchronize(s));
-    nact = hv[0];
-    int herr = 0;
-    memcpy(&herr, hv + 1, sizeof(int));
-    if (herr) return ABD_ERR_UNSORTED;
+    if (!dev_nact && n_active_dev) ABD_CHECK(hipMemsetAsync(n_active_dev, 0, sizeof(int64_t), s));
+    if (dn) {
+        nact = dev_nact ? n : 0;  // an upper bound: the stages below read the count on the device
+    } else {
+        // the active count and the ordering flag in one read-back
+        int64_t* hv;
+        if ((rc = arena_host(2 * sizeof(int64_t), (void**)&hv))) return rc;
+        hv[0] = 0;
+        if (dev_nact) ABD_CHECK(hipMemcpyAsync(hv, dev_nact, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipMemcpyAsync(hv + 1, err, sizeof(int), hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipStreamSynchronize(s));
+        nact = hv[0];
+        int herr = 0;
+        memcpy(&herr, hv + 1, sizeof(int));
+        if (herr) return ABD_ERR_UNSORTED;
+        if (n_active) *n_active = nact;
+    }
     double* pbuf = nullptr;
     if (nij > 0) {
         if ((rc = arena_get(kSlotPair, sizeof(double) * kPairBufStride * (size_t)nij, (void**)&pbuf))) return rc;
@@ -801,19 +827,22 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
         if ((rc = arena_get(kSlotPairSeg, sizeof(int64_t) * 4 * (size_t)nij, (void**)&pseg))) return rc;
         hipLaunchKernelGGL(zero_i64_kernel, dim3(grid_for(4 * nij)), dim3(kBlock), 0, s, pseg, 4 * nij);
         ABD_CHECK_LAUNCH();
-        int ph = prof_start(kProfContact, s, (4.0 + 20 + 16 + 192 + 8.0 * stride + 9) * nact);
+        const double cbytes = 4.0 + 20 + 16 + 192 + 8.0 * stride + 9;
+        int ph = prof_start(kProfContact, s, dn ? 1.0 : cbytes * nact);
+        const int64_t* dnact = dn ? dev_nact : nullptr;
         const dim3 cgrid(grid_for(nact, kBlock, 65535));
         if (P->psd == 3)
             hipLaunchKernelGGL(contact_kernel<3>, cgrid, dim3(kBlock), 0, s, *sc, *P, x, pt, npt, ee,
-                               (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
+                               (const int32_t*)act, nact, dnact, dn, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
         else if (P->psd == 2)
             hipLaunchKernelGGL(contact_kernel<2>, cgrid, dim3(kBlock), 0, s, *sc, *P, x, pt, npt, ee,
-                               (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
+                               (const int32_t*)act, nact, dnact, dn, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
         else
             hipLaunchKernelGGL(contact_kernel<1>, cgrid, dim3(kBlock), 0, s, *sc, *P, x, pt, npt, ee,
-                               (const int32_t*)act, nact, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
+                               (const int32_t*)act, nact, dnact, dn, cbuf, ckey, swp, hl_pt, hl_ee, pflag, nij, pseg);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s);
+        if ((rc = prof_count_bytes(ph, s, dnact, cbytes, 0.0, 0.0, 0.0, true))) return rc;
         // stage D: chunks of each pair's runs (count, scan), their partial sums, the per-pair sums
         // items: at most one per kPairChunk contacts plus one per run (a run per pair and type)
         const int64_t runs = 2 * nij < nact ? 2 * nij : nact;
@@ -836,7 +865,7 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
                            (const int64_t*)pcoff, ipair);
         ABD_CHECK_LAUNCH();
         // timed: the partial sums (each stored contact read once, its body-order flag once)
-        ph = prof_start_ext(kProfPairReduce, (8.0 * stride + 1) * nact);
+        ph = prof_start_ext(kProfPairReduce, dn ? 1.0 : (8.0 * stride + 1) * nact);
         if (dense)
             prof_launch(ph, pair_partial_kernel<true>, dim3(kPairGrid), dim3(kPairThreads), 0, s, nij,
                         (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg, (const int64_t*)pcoff,
@@ -846,19 +875,39 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
                         (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg, (const int64_t*)pcoff,
                         (const int32_t*)ipair, (const int64_t*)ptotal, part);
         ABD_CHECK_LAUNCH();
+        if ((rc = prof_count_bytes(ph, s, dnact, 8.0 * stride + 1, 0.0, 0.0, 0.0, true))) return rc;
         hipLaunchKernelGGL(pair_reduce_kernel, dim3(grid_for(nij, 1, kPairGrid)), dim3(kPairThreads), 0, s, nb, nij,
-                           (const uint8_t*)pflag, (const int64_t*)pcoff, (const int64_t*)pcnt, (const double*)part,
-                           pbuf, blocks);
+                           dn, (const uint8_t*)pflag, (const int64_t*)pcoff, (const int64_t*)pcnt,
+                           (const double*)part, pbuf, blocks);
         ABD_CHECK_LAUNCH();
     }
-    int phf = prof_start(kProfBodyFinalize, s, 2.0 * 8 * 156 * nb + 8.0 * kPairBufStride * nij + 40.0 * nvg);
-    hipLaunchKernelGGL(body_finalize_kernel, dim3(nb), dim3(kBlock), 0, s, *sc, *P, x, nij, row_ptr, slots,
+    const double fbytes = 2.0 * 8 * 156 * nb;
+    int phf = prof_start(kProfBodyFinalize, s, dn ? 1.0 : fbytes + 8.0 * kPairBufStride * nij + 40.0 * nvg);
+    hipLaunchKernelGGL(body_finalize_kernel, dim3(nb), dim3(kBlock), 0, s, *sc, *P, x, nij, dn, row_ptr, slots,
                        (const uint8_t*)pflag, (const double*)pbuf, vg, nvg, g, blocks, (const uint8_t*)hl_pt,
                        (const uint8_t*)hl_ee, highlight);
     ABD_CHECK_LAUNCH();
     prof_stop(phf, s);
-    if (n_active) *n_active = nact;
+    if ((rc = prof_count_bytes(phf, s, dn, 0.0, 0.0, 40.0, 8.0 * kPairBufStride, false, fbytes))) return rc;
     return ABD_OK;
+}
+
+}  // namespace abd
+
+using namespace abd;
+
+extern "C" {
+
+int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x, const int32_t* pt,
+                   int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t nij,
+                   const int32_t* row_ptr, const int32_t* slots, double* g, double* blocks, uint8_t* highlight,
+                   int64_t* n_active) {
+    if (!sc || !P || !x || !g || !blocks || npt < 0 || nee < 0 || nvg < 0 || nij < 0) return ABD_ERR_ARG;
+    if ((npt && !pt) || (nee && !ee) || (nvg && !vg) || (nij && (!row_ptr || !slots))) return ABD_ERR_ARG;
+    if (nij == 0 && (npt || nee)) return ABD_ERR_ARG;
+    if (n_active) *n_active = 0;
+    GhLists L{pt, npt, ee, nee, vg, nvg, nij, nullptr};
+    return contact_gh(to_stream(stream), sc, P, x, L, row_ptr, slots, g, blocks, highlight, n_active, nullptr);
 }
 
 int abd_pair_gh(abd_stream_t stream, int32_t kind, const AbdParams* P, const double* x4, const double* r4, int64_t n,

@@ -350,20 +350,25 @@ __device__ __forceinline__ int64_t lower_bound2(const int32_t* __restrict__ a, i
     return lo;
 }
 
-__global__ void rowlen_kernel(int nb, int64_t nij, const int32_t* __restrict__ ij, const int32_t* __restrict__ ji,
-                              int64_t* __restrict__ len) {
+// dnij (nullable): the pair count on the device (nij is then unused)
+__global__ void rowlen_kernel(int nb, int64_t nij, const int64_t* __restrict__ dnij, const int32_t* __restrict__ ij,
+                              const int32_t* __restrict__ ji, int64_t* __restrict__ len) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
+    if (dnij) nij = *dnij;
     int64_t nu = lower_bound2(ij, 0, 0, nij, b + 1) - lower_bound2(ij, 0, 0, nij, b);
     int64_t nl = lower_bound2(ji, 1, 0, nij, b + 1) - lower_bound2(ji, 1, 0, nij, b);
     len[b] = nl + 1 + nu;
 }
 
-__global__ void rowfill_kernel(int nb, int64_t nij, const int32_t* __restrict__ ij, const int32_t* __restrict__ ji,
-                               const int64_t* __restrict__ off, int64_t total, int32_t* __restrict__ row_ptr,
-                               int32_t* __restrict__ cols, int32_t* __restrict__ slots) {
+__global__ void rowfill_kernel(int nb, int64_t nij, const int64_t* __restrict__ dnij, const int32_t* __restrict__ ij,
+                               const int32_t* __restrict__ ji, const int64_t* __restrict__ off,
+                               int32_t* __restrict__ row_ptr, int32_t* __restrict__ cols,
+                               int32_t* __restrict__ slots) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
+    if (dnij) nij = *dnij;
+    const int64_t total = (int64_t)nb + 2 * nij;
     int64_t o = off[b];
     row_ptr[b] = (int32_t)o;
     if (b == nb - 1) row_ptr[nb] = (int32_t)total;
@@ -637,23 +642,27 @@ int abd_set_triplets(abd_stream_t stream, int32_t nb, const int32_t* ij, int64_t
 int abd_bsr_structure(abd_stream_t stream, int32_t nb, const int32_t* ij, const int32_t* ji, int64_t nij,
                       int32_t* row_ptr, int32_t* cols, int32_t* slots) {
     if (nb <= 0 || nij < 0 || (nij > 0 && (!ij || !ji)) || !row_ptr || !cols || !slots) return ABD_ERR_ARG;
-    hipStream_t s = to_stream(stream);
-    int64_t *len, *off;
-    int rc;
-    if ((rc = arena_get(kSlotScanA, sizeof(int64_t) * (nb + 1), (void**)&len))) return rc;
-    if ((rc = arena_get(kSlotScanB, sizeof(int64_t) * (nb + 1), (void**)&off))) return rc;
-    hipLaunchKernelGGL(rowlen_kernel, dim3(grid_for(nb)), dim3(kBlock), 0, s, nb, nij, ij, ji, len);
-    ABD_CHECK_LAUNCH();
-    if ((rc = scan_exclusive_i64(s, len, off, nb, off + nb))) return rc;
-    hipLaunchKernelGGL(rowfill_kernel, dim3(grid_for(nb)), dim3(kBlock), 0, s, nb, nij, ij, ji, (const int64_t*)off,
-                       (int64_t)nb + 2 * nij, row_ptr, cols, slots);
-    ABD_CHECK_LAUNCH();
-    return ABD_OK;
+    return bsr_structure(to_stream(stream), nb, ij, ji, nij, nullptr, row_ptr, cols, slots);
 }
 
 }  // extern "C"
 
 namespace abd {
+
+int bsr_structure(hipStream_t s, int nb, const int32_t* ij, const int32_t* ji, int64_t nij, const int64_t* dnij,
+                  int32_t* row_ptr, int32_t* cols, int32_t* slots) {
+    int64_t *len, *off;
+    int rc;
+    if ((rc = arena_get(kSlotScanA, sizeof(int64_t) * (nb + 1), (void**)&len))) return rc;
+    if ((rc = arena_get(kSlotScanB, sizeof(int64_t) * (nb + 1), (void**)&off))) return rc;
+    hipLaunchKernelGGL(rowlen_kernel, dim3(grid_for(nb)), dim3(kBlock), 0, s, nb, nij, dnij, ij, ji, len);
+    ABD_CHECK_LAUNCH();
+    if ((rc = scan_exclusive_i64(s, len, off, nb, off + nb))) return rc;
+    hipLaunchKernelGGL(rowfill_kernel, dim3(grid_for(nb)), dim3(kBlock), 0, s, nb, nij, dnij, ij, ji,
+                       (const int64_t*)off, row_ptr, cols, slots);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
 
 // ------------------------------------------------- collision set from a superset
 // The pairs and candidates of a candidate set made from larger boxes (the swept set of
@@ -806,6 +815,12 @@ __device__ __forceinline__ void keep_write(const KeepArgs& A, int k, int64_t i, 
 // The count: one entry per thread (its predicate's gathers need many waves in flight to
 // hide their latency), the kept entries of each workgroup's kBlock entries to bcnt; each
 // entry's predicate is stored for the write pass.
+// entries of input list k: A.n[k], or with the input sizes on the device (A.in.dn, the
+// keep output layout) that count (A.n[k] then bounds it)
+__device__ __forceinline__ int64_t keep_n(const KeepArgs& A, int k) {
+    return A.in.dn ? (k == 1 && !A.in.ji ? 0 : A.in.dn[k]) : A.n[k];
+}
+
 __global__ __launch_bounds__(kBlock) void keep_count_kernel(KeepArgs A) {
     __shared__ int shw[kBlock / kWave];
     const int64_t g = blockIdx.x;
@@ -814,7 +829,7 @@ __global__ __launch_bounds__(kBlock) void keep_count_kernel(KeepArgs A) {
     const int64_t blk = g - A.blk0[k];
     const int64_t i = blk * kBlock + threadIdx.x;
     bool pred = false;
-    if (i < A.n[k]) {
+    if (i < keep_n(A, k)) {
         pred = keep_pred(A, k, i);
         A.flag[A.flag0[k] + i] = pred ? 1 : 0;
     }
@@ -838,11 +853,12 @@ __global__ __launch_bounds__(kBlock) void keep_write_kernel(KeepArgs A, int k0, 
     while (k + 1 < k1 && g >= A.blk0[k + 1]) ++k;
     const int64_t blk = g - A.blk0[k];
     const int64_t i = blk * kBlock + threadIdx.x;
-    const bool pred = i < A.n[k] && A.flag[A.flag0[k] + i] != 0;
+    const int64_t nk = keep_n(A, k);
+    const bool pred = i < nk && A.flag[A.flag0[k] + i] != 0;
     int tot;
     const int64_t r = A.boff[k][blk] + block_rank(pred, shw, &tot);
     if (pred) keep_write(A, k, i, r);
-    if (k == 0 && i < A.n[0]) A.remap[i] = pred ? (int32_t)r : -1;
+    if (k == 0 && i < nk) A.remap[i] = pred ? (int32_t)r : -1;
 }
 
 }  // namespace
@@ -912,8 +928,10 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
     // an empty list's total is 0 (its scan segment is skipped)
     for (int k = 0; k < kKeepLists; ++k)
         if (n[k] == 0) ABD_CHECK(hipMemsetAsync(dcount + k, 0, sizeof(int64_t), s));
-    ABD_CHECK(hipMemcpyAsync(hcount, dcount, kKeepLists * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    if (sync) ABD_CHECK(hipStreamSynchronize(s));
+    if (hcount) {
+        ABD_CHECK(hipMemcpyAsync(hcount, dcount, kKeepLists * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        if (sync) ABD_CHECK(hipStreamSynchronize(s));
+    }
     return ABD_OK;
 }
 

@@ -22,7 +22,7 @@ inline hipStream_t to_stream(abd_stream_t s) { return reinterpret_cast<hipStream
 // Per-device scratch arena: `slots` independent growable device buffers plus a
 // small pinned host buffer for count read-back.  Growing synchronizes the
 // device first, so a buffer still in use by queued work is never freed early.
-constexpr int kArenaSlots = 40;
+constexpr int kArenaSlots = 41;
 enum ArenaSlot {
     kSlotScanA = 0,
     kSlotScanB,
@@ -65,6 +65,7 @@ enum ArenaSlot {
     kSlotKeepFlags,   // keep_from_superset: each entry's predicate
     kSlotToiQueue,    // CCD: the candidates whose cubic has a bracketed root
     kSlotToiSlots,    // CCD: per-scene minima of the queued candidates and the queue count
+    kSlotProfBytes,   // profiling: algorithmic byte counts computed on the device
 };
 // Launch shapes that change no result (abd_set_tuning); process-wide.
 AbdTuning tuning();
@@ -128,21 +129,28 @@ int within_ranges(hipStream_t s, int64_t n3, const double* x0, const double* x1,
 // K line-search trials q - alpha dq at once (alpha [K][S] on the device, S = 1: the
 // whole system as one scene; a negative alpha skips that scene's trial and gives 0):
 // per-scene contact energies to E[2 S k + s] and inertia energies to E[2 S k + S + s].
+// dn (here and below, nullable): the lists' sizes on the device, [nij, nij, npt, nee, nvg],
+// the counts passed then bounding them.
 int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
                 const double* A, const double* p, const double* dq, const double* A0, const double* p0,
                 const double* Adot, const double* pdot, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
-                const int32_t* vg, int64_t nvg, const int64_t* plan, double* E);
+                const int32_t* vg, int64_t nvg, const int64_t* plan, double* E, const int64_t* dn = nullptr);
 // The per-scene chunk plan of the energy / TOI reductions over one set of candidate lists
 // (plan: reduce_plan_elems(S) int64 on the device), made once and passed as `plan` to every
 // reduction over those lists (nullptr: each makes its own).
 size_t reduce_plan_elems(int S);
 int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const int32_t* pt, int64_t npt,
-                const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t* plan);
+                const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t* plan,
+                const int64_t* dn = nullptr);
 // Per-scene TOI (abd_scene_toi; S = 1: the whole system as one scene) with an optional plan;
 // active (nullable, [S]): scenes whose TOI is wanted, the others report 1 unevaluated.
 int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
               const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
-              const int64_t* plan, double* toi, const uint8_t* active);
+              const int64_t* plan, double* toi, const uint8_t* active, const int64_t* dn = nullptr);
+// abd_contact_energy (S = 1) / abd_scene_contact_energy with the lists' sizes on the device.
+int contact_energy(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* xk, const int32_t* pt,
+                   int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, const int64_t* dn,
+                   double* E);
 // abd_update_q / abd_update_q_scenes writing the result to a second state as well (when
 // A_out2, p_out2 are given).
 int update_q_dual(hipStream_t s, int nb, const double* A, const double* p, const double* dq, double alpha,
@@ -154,9 +162,12 @@ int update_q_scenes_dual(hipStream_t s, const AbdScene* sc, const double* A, con
 // edge / triangle boxes; outputs sized like the inputs (no aliasing).  The five totals
 // (nij, nij of ji, npt, nee, nvg) go to dcount (device) and, after one synchronization,
 // to hcount (host, pinned).
+// dn (nullable): the input sizes on the device in the output layout [nij, nij, npt, nee,
+// nvg] (the counts here then bound them).
 struct KeepInput {
     const int32_t *ij, *ji, *pt, *ee, *vg;
     int64_t nij, npt, nee, nvg;
+    const int64_t* dn = nullptr;
 };
 struct KeepBoxes {
     const float *blo, *bhi, *plo, *phi, *elo, *ehi, *tlo, *thi;
@@ -173,13 +184,33 @@ struct KeepAtX {
     const double *x, *x1;
     double pdil, edil, tdil;
 };
-// sync false: the totals reach hcount with the stream's next synchronization (the caller's).
+// sync false: the totals reach hcount with the stream's next synchronization (the caller's);
+// hcount null: they stay on the device.
 int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, const KeepBoxes& bx, double dhat,
                        double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount, bool sync = true,
                        const KeepAtX* at = nullptr);
+// abd_bsr_structure; dnij (nullable): the pair count on the device (nij bounds it; cols and
+// slots hold nb + 2 nij entries).
+int bsr_structure(hipStream_t s, int nb, const int32_t* ij, const int32_t* ji, int64_t nij, const int64_t* dnij,
+                  int32_t* row_ptr, int32_t* cols, int32_t* slots);
 // abd_cull_bodies without the (j, i) order when ji == nullptr.
 int cull_bodies(hipStream_t s, const AbdScene* sc, const float* lo, const float* hi, int32_t* ij, int32_t* ji,
                 int64_t cap, int64_t* count);
+// The gh stage's candidate lists.  dn (nullable): their sizes on the device, [nij, nij,
+// npt, nee, nvg]; the counts here are then upper bounds.
+struct GhLists {
+    const int32_t* pt;
+    int64_t npt;
+    const int32_t* ee;
+    int64_t nee;
+    const int32_t* vg;
+    int64_t nvg;
+    int64_t nij;
+    const int64_t* dn;
+};
+int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const double* x, const GhLists& L,
+               const int32_t* row_ptr, const int32_t* slots, double* g, double* blocks, uint8_t* highlight,
+               int64_t* n_active, int64_t* n_active_dev);
 int arena_get(int slot, size_t bytes, void** ptr);
 int arena_host(size_t bytes, void** ptr);  // pinned host memory
 int arena_release_current();

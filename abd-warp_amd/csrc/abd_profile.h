@@ -41,6 +41,15 @@ int prof_start(int kernel, hipStream_t s, double bytes);
 // compute such a value attaches it without moving the end.
 void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale = nullptr, const double* dev_bytes = nullptr);
 
+// Algorithmic bytes of a launch whose list sizes are on the device (dn in the layout
+// [nij, nij, npt, nee, nvg]): base + c_pt dn[2] + c_ee dn[3] + c_vg dn[4] + c_nij dn[0], or
+// with `single` base + c_pt dn[0] (dn one count), computed on the stream and attached to
+// `handle` (after its end event: call it after prof_stop or on a prof_start_ext record),
+// whose own byte count (the multiplier of device values) must be 1.  Nothing when the
+// handle is -1 or dn is null.
+int prof_count_bytes(int handle, hipStream_t s, const int64_t* dn, double c_pt, double c_ee, double c_vg,
+                     double c_nij, bool single = false, double base = 0.0);
+
 // A record whose events the kernel dispatch itself takes (prof_launch): it times the kernel
 // alone, without the gaps around it (e.g. a launch behind a host synchronization).
 int prof_start_ext(int kernel, double bytes);

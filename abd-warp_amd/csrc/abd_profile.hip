@@ -88,6 +88,32 @@ int resolve_locked() {
 }
 }  // namespace
 
+__global__ void prof_bytes_kernel(const int64_t* __restrict__ dn, double c_pt, double c_ee, double c_vg,
+                                  double c_nij, int single, double base, double* __restrict__ out) {
+    out[0] = single ? base + c_pt * (double)dn[0]
+                    : base + c_pt * (double)dn[2] + c_ee * (double)dn[3] + c_vg * (double)dn[4] +
+                          c_nij * (double)dn[0];
+}
+
+namespace {
+constexpr int kByteSlots = 1 << 14;  // device byte counts in flight (reused in stream order)
+int g_byte_next = 0;
+}  // namespace
+
+int prof_count_bytes(int handle, hipStream_t s, const int64_t* dn, double c_pt, double c_ee, double c_vg,
+                     double c_nij, bool single, double base) {
+    if (handle < 0 || !dn) return ABD_OK;
+    double* slots;
+    int rc = arena_get(kSlotProfBytes, sizeof(double) * kByteSlots, (void**)&slots);
+    if (rc) return rc;
+    double* out = slots + (g_byte_next++ % kByteSlots);
+    hipLaunchKernelGGL(prof_bytes_kernel, dim3(1), dim3(1), 0, s, dn, c_pt, c_ee, c_vg, c_nij, single ? 1 : 0, base,
+                       out);
+    ABD_CHECK_LAUNCH();
+    prof_stop(handle, s, nullptr, out);
+    return ABD_OK;
+}
+
 bool prof_enabled() { return g_on; }
 
 int prof_start(int kernel, hipStream_t s, double bytes) {

@@ -33,17 +33,26 @@ struct ListRef {
     int64_t n;
     int kind;  // kListPT / kListEE / kListVG
 };
+// dn (nullable): the lists' sizes on the device, [nij, nij, npt, nee, nvg] (the keep
+// filter's layout); the n of each ListRef then bounds its size (a list with n = 0 is empty).
 struct Lists {
     ListRef l[kLists];
+    const int64_t* dn = nullptr;
 };
+
+__device__ __forceinline__ int64_t list_n(const Lists& L, int l) {
+    const ListRef& r = L.l[l];
+    if (!L.dn || r.n == 0) return r.n;
+    return L.dn[r.kind == kListPT ? 2 : r.kind == kListEE ? 3 : 4];
+}
 
 __device__ __forceinline__ int width_of(int kind) { return kind == kListVG ? 2 : 5; }
 
 // first k in [0, n) whose tuple's first body belongs to a scene >= s
-__device__ __forceinline__ int64_t scene_lower_bound(const ListRef& L, const int32_t* __restrict__ body_scene,
-                                                     int s) {
+__device__ __forceinline__ int64_t scene_lower_bound(const ListRef& L, int64_t n,
+                                                     const int32_t* __restrict__ body_scene, int s) {
     const int w = width_of(L.kind);
-    int64_t lo = 0, hi = L.n;
+    int64_t lo = 0, hi = n;
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;
         if (body_scene[L.p[w * mid]] < s)
@@ -67,11 +76,12 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(int S, const int32_t
     __shared__ int64_t tile_total;
     for (int idx = threadIdx.x; idx < kLists * (S + 1); idx += blockDim.x) {
         const int l = idx / (S + 1), s = idx % (S + 1);
+        const int64_t n = list_n(L, l);
         int64_t v;
         if (S == 1)
-            v = s == 0 ? 0 : L.l[l].n;
+            v = s == 0 ? 0 : n;
         else
-            v = L.l[l].n > 0 ? scene_lower_bound(L.l[l], body_scene, s) : 0;
+            v = n > 0 ? scene_lower_bound(L.l[l], n, body_scene, s) : 0;
         if (s < S) seg[2 * (kLists * (int64_t)s + l)] = v;
         if (s > 0) seg[2 * (kLists * (int64_t)(s - 1) + l) + 1] = v;
     }
@@ -472,9 +482,10 @@ int get_plan(hipStream_t s, const AbdScene& sc, int S, const Lists& L, const int
 }
 
 // Plan + chunks + per-scene sums (OP 0) over the lists; S = 1 treats the lists as one scene.
+// Algorithmic bytes: 128 per PT, 132 per EE and 32 per VG candidate.
 template <int OP>
 int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, const double* x0, const double* x1,
-               const Lists& L, const int64_t* plan_in, double* out, int prof_id, double prof_bytes) {
+               const Lists& L, const int64_t* plan_in, double* out, int prof_id) {
     const int64_t bound = chunk_bound(L, S);
     if (bound > INT32_MAX) return ABD_ERR_ARG;
     const int64_t *seg, *choff;
@@ -482,10 +493,11 @@ int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
     int rc = get_plan(s, sc, S, L, plan_in, &seg, &choff);
     if (!rc) rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound, (void**)&part);
     if (rc) return rc;
-    int ph = prof_start_ext(prof_id, prof_bytes);
+    int ph = prof_start_ext(prof_id, L.dn ? 1.0 : 128.0 * L.l[1].n + 132.0 * L.l[0].n + 32.0 * L.l[2].n);
     prof_launch(ph, chunk_kernel<OP>, dim3((unsigned)bound), dim3(kRedThreads), 0, s, sc, P, x0, x1,
                 TrialArgs{nullptr, nullptr, nullptr, nullptr}, L, S, seg, choff, part, (int64_t)0);
     ABD_CHECK_LAUNCH();
+    if ((rc = prof_count_bytes(ph, s, L.dn, 128.0, 132.0, 32.0, 0.0))) return rc;
     hipLaunchKernelGGL(scene_final_kernel<OP>, dim3(S), dim3(kRedThreads), 0, s, choff, part, out, (int64_t)0,
                        (int64_t)0);
     ABD_CHECK_LAUNCH();
@@ -521,10 +533,12 @@ int toi_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
     ABD_CHECK_LAUNCH();
     // algorithmic bytes of the scan: tuple, indices and four vertices at both ends per PT / EE
     // candidate (224 / 228 B), tuple, offset and a vertex at both ends per VG candidate (56 B)
-    const int ph = scr ? -1 : prof_start_ext(kProfToi, 224.0 * L.l[1].n + 228.0 * L.l[0].n + 56.0 * L.l[2].n);
+    const int ph = scr ? -1
+                       : prof_start_ext(kProfToi, L.dn ? 1.0 : 224.0 * L.l[1].n + 228.0 * L.l[0].n + 56.0 * L.l[2].n);
     prof_launch(ph, toi_scan_kernel, dim3((unsigned)bound, kToiSplit), dim3(kRedThreads), 0, s, sc, P, x0, x1, L, S,
                 seg, choff, active, w.part, w.queue, w.qcount);
     ABD_CHECK_LAUNCH();
+    if ((rc = prof_count_bytes(ph, s, L.dn, 224.0, 228.0, 56.0, 0.0))) return rc;
     const unsigned grid = (unsigned)std::min<int64_t>(kToiSolveGrid, std::max<int64_t>(1, ceil_div(nq, 4 * kToiSolveThreads)));
     const int ph2 = scr ? -1 : prof_start_ext(kProfToiSolve, 0.0);
     prof_launch(ph2, toi_solve_kernel, dim3(grid), dim3(kToiSolveThreads), 0, s, sc, L, S, x0, x1,
@@ -540,18 +554,18 @@ int toi_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
 // sum order of the energy; a minimum does not depend on it).  The switches of
 // const_params.py:42-44 drop a whole list.
 Lists contact_lists(const AbdParams& P, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
-                    const int32_t* vg, int64_t nvg) {
+                    const int32_t* vg, int64_t nvg, const int64_t* dn = nullptr) {
     if (!P.pt) npt = 0;
     if (!P.ee) nee = 0;
     if (!P.vg) nvg = 0;
-    return Lists{{{ee, nee, kListEE}, {pt, npt, kListPT}, {vg, nvg, kListVG}}};
+    return Lists{{{ee, nee, kListEE}, {pt, npt, kListPT}, {vg, nvg, kListVG}}, dn};
 }
 
 size_t reduce_plan_elems(int S) { return 2 * kLists * (size_t)S + (size_t)S + 1; }
 
 int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const int32_t* pt, int64_t npt,
-                const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t* plan) {
-    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);
+                const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, int64_t* plan, const int64_t* dn) {
+    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg, dn);
     hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kPlanThreads), 0, s, S, sc->body_scene, L, plan,
                        plan + 2 * kLists * (size_t)S);
     ABD_CHECK_LAUNCH();
@@ -598,9 +612,9 @@ __global__ __launch_bounds__(kRedThreads) void inertia_trials_kernel(int S, cons
 int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
                 const double* A, const double* p, const double* dq, const double* A0, const double* p0,
                 const double* Adot, const double* pdot, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
-                const int32_t* vg, int64_t nvg, const int64_t* plan_in, double* E) {
+                const int32_t* vg, int64_t nvg, const int64_t* plan_in, double* E, const int64_t* dn) {
     int rc;
-    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);
+    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg, dn);
     npt = L.l[1].n;
     nee = L.l[0].n;
     nvg = L.l[2].n;
@@ -610,11 +624,12 @@ int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, in
     double* part;
     if ((rc = get_plan(s, *sc, S, L, plan_in, &seg, &choff))) return rc;
     if ((rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound * K, (void**)&part))) return rc;
-    int ph = prof_start_ext(kProfEnergy, K * (128.0 * npt + 132.0 * nee + 32.0 * nvg));
+    int ph = prof_start_ext(kProfEnergy, dn ? (double)K : K * (128.0 * npt + 132.0 * nee + 32.0 * nvg));
     prof_launch(ph, chunk_kernel<2>, dim3((unsigned)bound, K), dim3(kRedThreads), 0, s, *sc, *P,
                 (const double*)nullptr, (const double*)nullptr, TrialArgs{alpha, A, p, dq}, L, S, seg, choff,
                 part, bound);
     ABD_CHECK_LAUNCH();
+    if ((rc = prof_count_bytes(ph, s, dn, 128.0 * K, 132.0 * K, 32.0 * K, 0.0))) return rc;
     hipLaunchKernelGGL(scene_final_kernel<0>, dim3(S, K), dim3(kRedThreads), 0, s, choff, (const double*)part, E,
                        bound, (int64_t)2 * S);
     ABD_CHECK_LAUNCH();
@@ -633,19 +648,24 @@ int energy_impl(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, int
                 double* E) {
     if (!P || !xk || !E || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
-    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);
-    return seg_reduce<0>(to_stream(stream), *sc, *P, S, xk, nullptr, L, nullptr, E, kProfEnergy,
-                         128.0 * L.l[1].n + 132.0 * L.l[0].n + 32.0 * L.l[2].n);
+    return contact_energy(to_stream(stream), sc, P, S, xk, pt, npt, ee, nee, vg, nvg, nullptr, E);
 }
 
 }  // namespace
 
+int contact_energy(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* xk, const int32_t* pt,
+                   int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, const int64_t* dn,
+                   double* E) {
+    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg, dn);
+    return seg_reduce<0>(s, *sc, *P, S, xk, nullptr, L, nullptr, E, kProfEnergy);
+}
+
 int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
               const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
-              const int64_t* plan, double* toi, const uint8_t* active) {
+              const int64_t* plan, double* toi, const uint8_t* active, const int64_t* dn) {
     if (!P || !x_t0 || !x_t1 || !toi || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
-    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg);  // abd.py:288-299
+    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg, dn);  // abd.py:288-299
     return toi_reduce(s, *sc, *P, S, x_t0, x_t1, L, plan, toi, active, nullptr);
 }
 
@@ -716,7 +736,8 @@ int abd_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const d
             const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
             double* toi) {
     if (!sc) return ABD_ERR_ARG;
-    return scene_toi(to_stream(stream), sc, P, 1, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, nullptr, toi, nullptr);
+    return scene_toi(to_stream(stream), sc, P, 1, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, nullptr, toi, nullptr,
+                     nullptr);
 }
 
 int abd_toi_noalloc(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, const double* x_t0,
@@ -738,7 +759,7 @@ int abd_scene_toi(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, c
                   const int32_t* vg, int64_t nvg, double* toi) {
     if (!scenes_ok(sc)) return ABD_ERR_ARG;
     return scene_toi(to_stream(stream), sc, P, sc->n_scenes, x_t0, x_t1, pt, npt, ee, nee, vg, nvg, nullptr,
-                     toi, nullptr);
+                     toi, nullptr, nullptr);
 }
 
 int abd_dot(abd_stream_t stream, const double* a, const double* b, int64_t n, double* out) {

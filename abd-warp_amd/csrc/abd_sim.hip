@@ -1,9 +1,11 @@
 // abd_sim.hip — native frame driver: AffineBodySimulator.step (abd.py:212-276)
-// run from C++ over the stage entry points of this library, so that a Newton
-// iteration costs a handful of host synchronizations (the sizes of candidate
-// lists and the scalars the line search and convergence test branch on)
-// instead of a Python round trip per stage.  The arithmetic is the same
-// sequence of kernels as the Python mirror, so both give identical states.
+// run from C++ over the stage entry points of this library.  The candidate lists a
+// Newton iteration filters from the candidate superset keep their sizes on the device
+// (every launch sized by the superset's, which bound them), so an iteration waits for
+// the device once: for the scalars the line search and the convergence test branch on
+// (time of impact, q.g, g.g, the first trial's energies), read back with the lists'
+// sizes.  The arithmetic is the same sequence of kernels as the Python mirror, so both
+// give identical states.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -50,10 +52,15 @@ struct BoxBuf {
 };
 
 // Body pairs (ij sorted by (i, j), ji the same pairs sorted by (j, i)) and the PT / EE / VG
-// candidate tuples of one set of boxes.
+// candidate tuples of one set of boxes.  The counts are exact when the set was culled
+// (exact: consumers take them from here); a set filtered on the device has its sizes at dn
+// ([nij, nij, npt, nee, nvg], device) and the counts here bound them.
 struct CandSet {
     DevBuf<int32_t> ij, ji, ptl, eel, vgl;
     int64_t nij = 0, npt = 0, nee = 0, nvg = 0;
+    bool exact = true;
+    const int64_t* dn = nullptr;
+    const int64_t* counts() const { return exact ? nullptr : dn; }
     int reserve_like(const CandSet& o) {
         int rc = ij.reserve(2 * (size_t)o.nij + 2);
         if (!rc) rc = ji.reserve(2 * (size_t)o.nij + 2);
@@ -90,12 +97,13 @@ struct Sim {
     DevBuf<double> vlo, vhi;         // its vertex ranges
     bool sup_valid = false;
     bool col_fresh = false;  // col (with its structure) is the collision set at the current x
-    bool sup_reused = false;  // the last swept set came from the superset
     DevBuf<int32_t> row_ptr, cols, slots;
     DevBuf<double> blocks;
     DevBuf<double> scal;  // [0] E contact, [1] E inertia, [2] toi, [3] dot, [4] resid
     DevBuf<int32_t> iters;
     DevBuf<int64_t> dcount;     // culling totals gathered for one read-back
+    DevBuf<int64_t> cnt;        // device sizes of the filtered sets (kCnt*), read back per iteration
+    int64_t* hcnt = nullptr;    // pinned copy of cnt
     double* host = nullptr;     // pinned scalars
     int64_t* hcount = nullptr;  // pinned list counts
     int64_t stats[10] = {};
@@ -155,6 +163,10 @@ __global__ void first_trial_kernel(int ns, const double* __restrict__ toi, const
 
 // per-scene value rows of Sim::sv / Sim::hsv
 enum { kSvEc = 0, kSvEi, kSvToi, kSvQtg, kSvGtg, kSvAlpha, kSvAlphaCcd, kSv };
+// Sim::cnt: the collision set's sizes, the swept set's, the active contacts of the gh
+// stage, the flag of a step leaving the superset's ranges, and a staging row for the
+// exact sizes of a culled collision set
+enum { kCntCol = 0, kCntSw = 8, kCntNact = 16, kCntRange = 17, kCntLen = 24 };
 
 int get_scalars(hipStream_t s, Sim& S, int first, int count) {
     ABD_CHECK(hipMemcpyAsync(S.host + first, S.scal.p + first, sizeof(double) * count, hipMemcpyDeviceToHost, s));
@@ -225,34 +237,73 @@ int body_pairs(hipStream_t s, Sim& S, CandSet& C, const BoxBuf& body, bool with_
     return ABD_OK;
 }
 
-// abd.py:240-243: the slot layout of the collision set's body pairs
+// abd.py:240-243: the slot layout of the collision set's body pairs (its pair count on the
+// device when the set was filtered)
 int structure_of(hipStream_t s, Sim& S) {
     int rc;
     if ((rc = S.cols.reserve(S.nb + 2 * (size_t)S.col.nij))) return rc;
     if ((rc = S.slots.reserve(S.nb + 2 * (size_t)S.col.nij))) return rc;
-    return abd_bsr_structure(s, S.nb, S.col.ij.p, S.col.ji.p, S.col.nij, S.row_ptr.p, S.cols.p, S.slots.p);
+    return bsr_structure(s, S.nb, S.col.ij.p, S.col.ji.p, S.col.nij, S.col.counts(), S.row_ptr.p, S.cols.p,
+                         S.slots.p);
 }
 
 // abd.py:187-207 on the vertices xv (xk of the reference's compute_energy, or x = xk at the
-// start of a Newton iteration)
+// start of a Newton iteration).  The culled counts are exact; with `structure` they are also
+// staged to the device row the filtered sets use, so that every consumer of the collision
+// set reads its sizes there.
 int collision_set(hipStream_t s, Sim& S, bool structure, const double* xv) {
     const double dh = S.P.dhat * 0.5;
     int rc;
     if ((rc = abd_body_boxes(s, &S.sc, xv, dh, S.body.lo.p, S.body.hi.p))) return rc;
     if ((rc = body_pairs(s, S, S.col, S.body, structure))) return rc;
+    S.col.exact = true;
     if (structure && (rc = structure_of(s, S))) return rc;
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_TRI, xv, 0.0, S.tri.lo.p, S.tri.hi.p))) return rc;
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_EDGE, xv, dh, S.edge.lo.p, S.edge.hi.p))) return rc;
     if ((rc = abd_prim_boxes(s, &S.sc, ABD_KIND_POINT, xv, S.P.dhat, S.pt.lo.p, S.pt.hi.p))) return rc;
-    return cull_lists(s, S, S.col, S.body, S.pt, S.tri, S.edge);
+    if ((rc = cull_lists(s, S, S.col, S.body, S.pt, S.tri, S.edge))) return rc;
+    if (structure) {
+        // the pinned staging row is rewritten only after the next read-back of the iteration
+        int64_t* h = S.hcnt + kCntLen;
+        h[0] = h[1] = S.col.nij;
+        h[2] = S.col.npt;
+        h[3] = S.col.nee;
+        h[4] = S.col.nvg;
+        ABD_CHECK(hipMemcpyAsync(S.cnt.p + kCntCol, h, 5 * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        S.col.dn = S.cnt.p + kCntCol;
+        S.col.exact = false;  // consumers read the staged sizes (equal to the counts here)
+    }
+    return ABD_OK;
 }
 
 // The collision set at the vertices xv = x(q) of the accepted step (with its structure)
-// filtered from the swept set of the last CCD step, which holds every pair and candidate
-// of it: q = q_prev - alpha dq lies on that step and the swept boxes contain the boxes at
-// every point of it.  The result is the culling's own (keep_from_superset); used for the
-// Newton iterations after a frame's first.
-int collision_from(hipStream_t s, Sim& S, const CandSet& W, const double* xv);
+// filtered from a set that holds every pair and candidate of it (the swept set of the last
+// CCD step: q = q_prev - alpha dq lies on that step and the swept boxes contain the boxes at
+// every point of it; or the candidate superset while xv lies inside its ranges).  The result
+// is the culling's own (keep_from_superset); its sizes stay on the device.
+int collision_from(hipStream_t s, Sim& S, const CandSet& W, const double* xv) {
+    const double dh = S.P.dhat * 0.5;
+    int rc;
+    // the body boxes at x (also the start half of the next swept body boxes); the primitive
+    // boxes of collision_set's dilations are evaluated per entry of W by the filter: W holds
+    // far fewer entries than the scene has primitives
+    if ((rc = abd_body_boxes(s, &S.sc, xv, dh, S.body.lo.p, S.body.hi.p))) return rc;
+    if ((rc = S.col.reserve_like(W))) return rc;
+    const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg, W.counts()};
+    const KeepBoxes bx{S.body.lo.p, S.body.hi.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const KeepOutput out{S.col.ij.p, S.col.ji.p, S.col.ptl.p, S.col.eel.p, S.col.vgl.p};
+    const KeepAtX at{xv, nullptr, S.P.dhat, dh, 0.0};
+    if ((rc = keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, out, S.cnt.p + kCntCol, nullptr, false,
+                                 &at)))
+        return rc;
+    S.col.nij = W.nij;
+    S.col.npt = W.npt;
+    S.col.nee = W.nee;
+    S.col.nvg = W.nvg;
+    S.col.dn = S.cnt.p + kCntCol;
+    S.col.exact = false;
+    return structure_of(s, S);
+}
 int collision_from_swept(hipStream_t s, Sim& S, const double* xv) { return collision_from(s, S, S.sw, xv); }
 
 // A frame's first collision set: filtered from the candidate superset when the state lies
@@ -270,27 +321,6 @@ int collision_at_frame_start(hipStream_t s, Sim& S, const double* xv) {
     return collision_set(s, S, true, xv);
 }
 
-int collision_from(hipStream_t s, Sim& S, const CandSet& W, const double* xv) {
-    const double dh = S.P.dhat * 0.5;
-    int rc;
-    // the body boxes at x (also the start half of the next swept body boxes); the primitive
-    // boxes of collision_set's dilations are evaluated per entry of W by the filter: W holds
-    // far fewer entries than the scene has primitives
-    if ((rc = abd_body_boxes(s, &S.sc, xv, dh, S.body.lo.p, S.body.hi.p))) return rc;
-    if ((rc = S.col.reserve_like(W))) return rc;
-    const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg};
-    const KeepBoxes bx{S.body.lo.p, S.body.hi.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    const KeepOutput out{S.col.ij.p, S.col.ji.p, S.col.ptl.p, S.col.eel.p, S.col.vgl.p};
-    const KeepAtX at{xv, nullptr, S.P.dhat, dh, 0.0};
-    if ((rc = keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, out, S.dcount.p, S.hcount, true, &at)))
-        return rc;
-    S.col.nij = S.hcount[0];
-    S.col.npt = S.hcount[2];
-    S.col.nee = S.hcount[3];
-    S.col.nvg = S.hcount[4];
-    return structure_of(s, S);
-}
-
 // Skin of the candidate superset's vertex ranges, in units of dhat: wide enough that the
 // later Newton steps of a frame (and of the next frames while the scenes rest) stay inside,
 // so that a batch of scenes, which reuses the superset only when all of them do, skips
@@ -306,82 +336,92 @@ constexpr double kSkinOverDhat = 30.0;
 // the collision set at x - alpha dq has boxes apart, hence distance >= dhat and a zero
 // barrier, so only the order of the nonzero terms' partial sums changes.
 // The set is the culling output of the step's swept boxes, taken by filtering the
-// candidate superset (keep_from_superset: the same predicates) while both ends of the
-// step lie inside the superset's vertex ranges (every vertex of every scene), and the
-// superset is culled anew (from the ranges of this step's ends, widened by the skin) when
-// they do not.
-int swept_set(hipStream_t s, Sim& S) {
+// candidate superset (keep_from_superset: the same predicates), valid while both ends of
+// the step lie inside the superset's vertex ranges (every vertex of every scene).  Launched
+// only: the range test's flag (cnt[kCntRange]) comes back with the iteration's scalars, and
+// a step that left the ranges is redone after swept_set_recull.  Without a superset the
+// superset is culled first (swept_set_recull's path).
+int swept_set_launch(hipStream_t s, Sim& S);
+
+// The superset culled anew around the current step (its ends widened by the skin; host
+// synchronizations for the culling counts), then the swept set filtered from it.
+int swept_set_recull(hipStream_t s, Sim& S) {
     const double dh = S.P.dhat * 0.5;
     const AbdSimState& st = S.st;
     const int64_t n3 = 3 * (int64_t)S.nv;
     int rc;
+    S.sup_valid = false;
+    if ((rc = vertex_ranges(s, n3, st.x, st.xk, kSkinOverDhat * S.P.dhat, S.vlo.p, S.vhi.p))) return rc;
+    if ((rc = swept_boxes(s, &S.sc, -1, S.vlo.p, S.vhi.p, dh, S.sbody.lo.p, S.sbody.hi.p))) return rc;
+    // with the (j, i) order: the swept set (and the collision sets after it) keep it
+    if ((rc = body_pairs(s, S, S.sup, S.sbody, true))) return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, S.vlo.p, S.vhi.p, 0.0, S.stri.lo.p, S.stri.hi.p))) return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, S.vlo.p, S.vhi.p, dh, S.sedge.lo.p, S.sedge.hi.p))) return rc;
+    if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, S.vlo.p, S.vhi.p, S.P.dhat, S.spt.lo.p, S.spt.hi.p))) return rc;
+    if ((rc = cull_lists(s, S, S.sup, S.sbody, S.spt, S.stri, S.sedge))) return rc;
+    S.sup.exact = true;
+    S.sup_valid = true;
+    return swept_set_launch(s, S);
+}
+
+int swept_set_launch(hipStream_t s, Sim& S) {
+    const double dh = S.P.dhat * 0.5;
+    const AbdSimState& st = S.st;
+    int rc;
+    if (!S.sup_valid) return swept_set_recull(s, S);
     // body boxes: the half at x is this iteration's collision set's (same dilation; reading
     // it beats a second pass over every vertex).  Primitive swept boxes: evaluated by the
     // filter per entry of the superset from both ends (far fewer entries than primitives)
     if ((rc = swept_boxes(s, &S.sc, -1, st.x, st.xk, dh, S.tbody.lo.p, S.tbody.hi.p, S.body.lo.p, S.body.hi.p)))
         return rc;
+    // the range test (the flag is 1 once some vertex of either end leaves the ranges)
+    ABD_CHECK(hipMemsetAsync(S.cnt.p + kCntRange, 0, sizeof(int64_t), s));
+    if ((rc = within_ranges(s, 3 * (int64_t)S.nv, st.x, st.xk, S.vlo.p, S.vhi.p, S.cnt.p + kCntRange))) return rc;
+    if ((rc = S.sw.reserve_like(S.sup))) return rc;
+    const CandSet& W = S.sup;
+    const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg, W.counts()};
     const KeepBoxes bx{S.tbody.lo.p, S.tbody.hi.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const KeepOutput o{S.sw.ij.p, S.sw.ji.p, S.sw.ptl.p, S.sw.eel.p, S.sw.vgl.p};
     const KeepAtX at{st.x, st.xk, S.P.dhat, dh, 0.0};
-    // the superset filtered into sw (buffers taken after reserve_like, which may move them)
-    auto keep = [&](bool sync) {
-        const CandSet& W = S.sup;
-        const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg};
-        const KeepOutput o{S.sw.ij.p, S.sw.ji.p, S.sw.ptl.p, S.sw.eel.p, S.sw.vgl.p};
-        return keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, o, S.dcount.p, S.hcount, sync, &at);
-    };
-    bool reuse = false, kept = false;
-    if (S.sup_valid) {
-        // the range test; when the last step reused the superset, the filter from it runs
-        // behind the test, sharing its read-back (and is dropped if the step left the ranges)
-        ABD_CHECK(hipMemsetAsync(S.dcount.p + 7, 0, sizeof(int64_t), s));
-        if ((rc = within_ranges(s, n3, st.x, st.xk, S.vlo.p, S.vhi.p, S.dcount.p + 7))) return rc;
-        ABD_CHECK(hipMemcpyAsync(S.hcount + 7, S.dcount.p + 7, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        if (S.sup_reused) {
-            if ((rc = S.sw.reserve_like(S.sup))) return rc;
-            if ((rc = keep(false))) return rc;
-            kept = true;
-        }
-        ABD_CHECK(hipStreamSynchronize(s));
-        reuse = S.hcount[7] == 0;
-    }
-    S.sup_reused = reuse;
-    if (reuse && !kept) {
-        if ((rc = S.sw.reserve_like(S.sup))) return rc;
-        if ((rc = keep(true))) return rc;
-    }
-    if (!reuse) {
-        S.sup_valid = false;
-        if ((rc = vertex_ranges(s, n3, st.x, st.xk, kSkinOverDhat * S.P.dhat, S.vlo.p, S.vhi.p))) return rc;
-        if ((rc = swept_boxes(s, &S.sc, -1, S.vlo.p, S.vhi.p, dh, S.sbody.lo.p, S.sbody.hi.p))) return rc;
-        // with the (j, i) order: the swept set (and the collision sets after it) keep it
-        if ((rc = body_pairs(s, S, S.sup, S.sbody, true))) return rc;
-        if ((rc = swept_boxes(s, &S.sc, ABD_KIND_TRI, S.vlo.p, S.vhi.p, 0.0, S.stri.lo.p, S.stri.hi.p))) return rc;
-        if ((rc = swept_boxes(s, &S.sc, ABD_KIND_EDGE, S.vlo.p, S.vhi.p, dh, S.sedge.lo.p, S.sedge.hi.p))) return rc;
-        if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, S.vlo.p, S.vhi.p, S.P.dhat, S.spt.lo.p, S.spt.hi.p)))
-            return rc;
-        if ((rc = cull_lists(s, S, S.sup, S.sbody, S.spt, S.stri, S.sedge))) return rc;
-        S.sup_valid = true;
-        if ((rc = S.sw.reserve_like(S.sup))) return rc;
-        if ((rc = keep(true))) return rc;
-    }
-    S.sw.nij = S.hcount[0];
-    S.sw.npt = S.hcount[2];
-    S.sw.nee = S.hcount[3];
-    S.sw.nvg = S.hcount[4];
+    if ((rc = keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, o, S.cnt.p + kCntSw, nullptr, false, &at)))
+        return rc;
+    S.sw.nij = W.nij;
+    S.sw.npt = W.npt;
+    S.sw.nee = W.nee;
+    S.sw.nvg = W.nvg;
+    S.sw.dn = S.cnt.p + kCntSw;
+    S.sw.exact = false;
+    S.sw_valid = true;
     return ABD_OK;
 }
 
-// abd.py:133-141
+// The swept set's read-back values (after the iteration's synchronization): true when the
+// step left the superset's ranges, so that the swept set must be recomputed.
+bool swept_left_ranges(const Sim& S) { return S.hcnt[kCntRange] != 0; }
+
+// The lists' sizes for the statistics (after the iteration's synchronization).
+void count_stats(Sim& S) {
+    const int64_t* h = S.hcnt;
+    S.stats[1] += h[kCntCol + 2];
+    S.stats[2] += h[kCntCol + 3];
+    S.stats[3] += h[kCntCol + 4];
+    S.stats[4] += h[kCntNact];
+    S.stats[8] += h[kCntSw + 2];
+    S.stats[9] += h[kCntSw + 3];
+}
+
+// abd.py:133-141 at a frame's start (alpha = 0: xk = x), which also gives the first Newton
+// iteration's collision set with its structure
 int compute_energy(hipStream_t s, Sim& S, double alpha, double* E) {
     int rc;
     const AbdSimState& st = S.st;
     if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, alpha, st.Ak, st.pk))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-    // called at a frame's start (alpha = 0: xk = x): the first Newton iteration's collision
-    // set too, with its structure
     if ((rc = collision_at_frame_start(s, S, st.xk))) return rc;
     S.col_fresh = alpha == 0.0;
-    if ((rc = abd_contact_energy(s, &S.sc, &S.P, st.xk, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg, S.scal.p)))
+    const CandSet& C = S.col;
+    if ((rc = contact_energy(s, &S.sc, &S.P, 1, st.xk, C.ptl.p, C.npt, C.eel.p, C.nee, C.vgl.p, C.nvg, C.counts(),
+                             S.scal.p)))
         return rc;
     if ((rc = abd_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot, S.scal.p + 1)))
         return rc;
@@ -393,7 +433,7 @@ int compute_energy(hipStream_t s, Sim& S, double alpha, double* E) {
 
 // abd.py:133-141 inside the line search, for K trial step lengths at once (energies to
 // E[K], one host round trip): the energy at q - alpha dq over the swept candidate set of
-// this iteration (swept_set), without re-culling
+// this iteration (swept_set_launch), without re-culling
 int line_energy_multi(hipStream_t s, Sim& S, const double* alphas, int K, double* E) {
     int rc;
     const AbdSimState& st = S.st;
@@ -402,7 +442,8 @@ int line_energy_multi(hipStream_t s, Sim& S, const double* alphas, int K, double
     for (int k = 0; k < K; ++k) ha[k] = alphas[k];
     ABD_CHECK(hipMemcpyAsync(S.cand_alpha.p, ha, sizeof(double) * K, hipMemcpyHostToDevice, s));
     if ((rc = line_trials(s, &S.sc, &S.P, 1, K, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
-                          S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p)))
+                          S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p,
+                          S.sw.counts())))
         return rc;
     ABD_CHECK(hipMemcpyAsync(he, S.cand_E.p, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, s));
     ABD_CHECK(hipStreamSynchronize(s));
@@ -411,25 +452,41 @@ int line_energy_multi(hipStream_t s, Sim& S, const double* alphas, int K, double
     return ABD_OK;
 }
 
-// abd.py:278-303, launched only: the TOI of the full step lands in scal[2]
-int ccd(hipStream_t s, Sim& S) {
+// abd.py:278-303 and the line search's first trial, launched only, then one read-back: the
+// step's swept set (ccd: xk = the end of the full step), its reduction plan, the TOI into
+// scal[2], q.g and g.g, the first trial's energies (its step length from the TOI, on the
+// device), the solver's iteration count and the sets' sizes.  A step that left the
+// superset's ranges is recomputed from a superset culled around it.
+int ccd_and_first_trial(hipStream_t s, Sim& S) {
     int rc;
     const AbdSimState& st = S.st;
-    // x = x(A, p) from the start of the iteration; xk: the end of the full step
-    if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, 1.0, st.Ak, st.pk))) return rc;
-    if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-    if ((rc = swept_set(s, S))) return rc;
-    S.sw_valid = true;
-    S.stats[8] += S.sw.npt;
-    S.stats[9] += S.sw.nee;
-    // one reduction plan of the swept set serves the TOI and every line-search trial
-    if ((rc = reduce_plan(s, &S.sc, &S.P, 1, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg,
-                          S.plan.p)))
-        return rc;
-    if ((rc = scene_toi(s, &S.sc, &S.P, 1, st.x, st.xk, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p,
-                        S.sw.nvg, S.plan.p, S.scal.p + 2, nullptr)))
-        return rc;
-    return ABD_OK;
+    const int nb = S.nb;
+    for (int attempt = 0;; ++attempt) {
+        if ((rc = attempt == 0 ? swept_set_launch(s, S) : swept_set_recull(s, S))) return rc;
+        const CandSet& W = S.sw;
+        // one reduction plan of the swept set serves the TOI and every line-search trial
+        if ((rc = reduce_plan(s, &S.sc, &S.P, 1, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg, S.plan.p,
+                              W.counts())))
+            return rc;
+        if ((rc = scene_toi(s, &S.sc, &S.P, 1, st.x, st.xk, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg,
+                            S.plan.p, S.scal.p + 2, nullptr, W.counts())))
+            return rc;
+        // q.g (abd.py:148) and g.g (abd.py:265): g and dq are final here
+        if ((rc = abd_dot(s, st.dq, st.g, 12 * (int64_t)nb, S.scal.p + 3))) return rc;
+        if ((rc = abd_dot(s, st.g, st.g, 12 * (int64_t)nb, S.scal.p + 5))) return rc;
+        hipLaunchKernelGGL(first_trial_kernel, dim3(1), dim3(kBlock), 0, s, 1, (const double*)(S.scal.p + 2),
+                           (const uint8_t*)nullptr, S.cand_alpha.p);
+        ABD_CHECK_LAUNCH();
+        if ((rc = line_trials(s, &S.sc, &S.P, 1, 1, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot,
+                              st.pdot, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg, S.plan.p, S.cand_E.p,
+                              W.counts())))
+            return rc;
+        ABD_CHECK(hipMemcpyAsync(S.hcand + kLsSpec, S.cand_E.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipMemcpyAsync(S.host + 6, S.iters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        ABD_CHECK(hipMemcpyAsync(S.hcnt, S.cnt.p, kCntLen * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        if ((rc = get_scalars(s, S, 2, 4))) return rc;
+        if (!swept_left_ranges(S)) return ABD_OK;
+    }
 }
 
 int step(hipStream_t s, Sim& S) {
@@ -459,34 +516,19 @@ int step(hipStream_t s, Sim& S) {
         ABD_CHECK(hipMemsetAsync(st.g, 0, sizeof(double) * 12 * nb, s));
         if ((rc = abd_inertia_gradient(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g))) return rc;
         if ((rc = abd_inertia_hessian(s, &S.sc, &P, st.A, S.blocks.p))) return rc;
-        int64_t nact = 0;
-        if ((rc = abd_contact_gh(s, &S.sc, &P, st.x, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg, S.col.nij,
-                                 S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, &nact)))
+        const CandSet& C = S.col;
+        const GhLists L{C.ptl.p, C.npt, C.eel.p, C.nee, C.vgl.p, C.nvg, C.nij, C.counts()};
+        if ((rc = contact_gh(s, &S.sc, &P, st.x, L, S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, nullptr,
+                             S.cnt.p + kCntNact)))
             return rc;
-        S.stats[1] += S.col.npt;
-        S.stats[2] += S.col.nee;
-        S.stats[3] += S.col.nvg;
-        S.stats[4] += nact;
         if ((rc = abd_bsr_solve(s, P.solver, nb, S.row_ptr.p, S.cols.p, S.slots.p, S.blocks.p, st.g, st.dq,
                                 P.solver_tol, P.solver_maxiter, 10, S.iters.p, S.scal.p + 4)))
             return rc;
-        if ((rc = ccd(s, S))) return rc;
-        // q.g (abd.py:148) and g.g (abd.py:265): g and dq are final here
-        if ((rc = abd_dot(s, st.dq, st.g, 12 * (int64_t)nb, S.scal.p + 3))) return rc;
-        if ((rc = abd_dot(s, st.g, st.g, 12 * (int64_t)nb, S.scal.p + 5))) return rc;
-        // the line search's first trial straight behind the TOI (its step length from the TOI
-        // on the device); its energies, the TOI, both dots and the solver's iteration count
-        // come back in one read-back
-        hipLaunchKernelGGL(first_trial_kernel, dim3(1), dim3(kBlock), 0, s, 1, (const double*)(S.scal.p + 2),
-                           (const uint8_t*)nullptr, S.cand_alpha.p);
-        ABD_CHECK_LAUNCH();
-        if ((rc = line_trials(s, &S.sc, &P, 1, 1, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
-                              S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p,
-                              S.cand_E.p)))
-            return rc;
-        ABD_CHECK(hipMemcpyAsync(S.hcand + kLsSpec, S.cand_E.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
-        ABD_CHECK(hipMemcpyAsync(S.host + 6, S.iters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        if ((rc = get_scalars(s, S, 2, 4))) return rc;
+        // CCD over the full step (xk = its end) and the first trial (abd.py:278-303)
+        if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, 1.0, st.Ak, st.pk))) return rc;
+        if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
+        if ((rc = ccd_and_first_trial(s, S))) return rc;
+        count_stats(S);
         S.stats[5] += 1;
         const double qTg = S.host[3], gTg = S.host[5];
         int32_t sit;
@@ -551,8 +593,9 @@ int compute_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
     // first Newton iteration's collision set too, with its structure
     if ((rc = collision_at_frame_start(s, S, st.xk))) return rc;
     S.col_fresh = true;
-    if ((rc = abd_scene_contact_energy(s, &S.sc, &S.P, st.xk, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg,
-                                       S.sv.p + kSvEc * ns)))
+    const CandSet& C = S.col;
+    if ((rc = contact_energy(s, &S.sc, &S.P, ns, st.xk, C.ptl.p, C.npt, C.eel.p, C.nee, C.vgl.p, C.nvg, C.counts(),
+                             S.sv.p + kSvEc * ns)))
         return rc;
     if ((rc = abd_scene_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot,
                                        S.sv.p + kSvEi * ns)))
@@ -570,7 +613,8 @@ int line_energy_scenes_launch(hipStream_t s, Sim& S, const double* dalpha, int K
     const AbdSimState& st = S.st;
     const int ns = S.ns;
     int rc = line_trials(s, &S.sc, &S.P, ns, K, dalpha, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
-                         S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p);
+                         S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p,
+                         S.sw.counts());
     if (rc) return rc;
     ABD_CHECK(hipMemcpyAsync(S.hcand + (size_t)kLsSpec * ns, S.cand_E.p, sizeof(double) * 2 * ns * K,
                              hipMemcpyDeviceToHost, s));
@@ -618,14 +662,11 @@ int step_scenes(hipStream_t s, Sim& S) {
         ABD_CHECK(hipMemsetAsync(st.g, 0, sizeof(double) * 12 * nb, s));
         if ((rc = abd_inertia_gradient(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g))) return rc;
         if ((rc = abd_inertia_hessian(s, &S.sc, &P, st.A, S.blocks.p))) return rc;
-        int64_t nact = 0;
-        if ((rc = abd_contact_gh(s, &S.sc, &P, st.x, S.col.ptl.p, S.col.npt, S.col.eel.p, S.col.nee, S.col.vgl.p, S.col.nvg, S.col.nij,
-                                 S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, &nact)))
+        const CandSet& C = S.col;
+        const GhLists L{C.ptl.p, C.npt, C.eel.p, C.nee, C.vgl.p, C.nvg, C.nij, C.counts()};
+        if ((rc = contact_gh(s, &S.sc, &P, st.x, L, S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, nullptr,
+                             S.cnt.p + kCntNact)))
             return rc;
-        S.stats[1] += S.col.npt;
-        S.stats[2] += S.col.nee;
-        S.stats[3] += S.col.nvg;
-        S.stats[4] += nact;
         for (int k = 0; k < ns; ++k) {
             S.hmask[k] = newton[k];
             S.hccd[k] = newton[k] ? 1.0 : 0.0;
@@ -639,31 +680,36 @@ int step_scenes(hipStream_t s, Sim& S) {
         if ((rc = upload(s, dccd, S.hccd, sizeof(double) * ns))) return rc;
         if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dccd, st.Ak, st.pk))) return rc;
         if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
-        if ((rc = swept_set(s, S))) return rc;
-        S.sw_valid = true;
-        S.stats[8] += S.sw.npt;
-        S.stats[9] += S.sw.nee;
-        if ((rc = reduce_plan(s, &S.sc, &P, ns, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg,
-                              S.plan.p)))
-            return rc;
-        // the TOI of the scenes still iterating (sactive: the solver's mask of this iteration)
-        if ((rc = scene_toi(s, &S.sc, &P, ns, st.x, st.xk, S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p,
-                            S.sw.nvg, S.plan.p, S.sv.p + kSvToi * ns, S.sactive.p)))
-            return rc;
-        if ((rc = abd_scene_dot(s, &S.sc, st.dq, st.g, S.sv.p + kSvQtg * ns))) return rc;
-        // g.g of the stopping test (abd.py:265): g is final here
-        if ((rc = abd_scene_dot(s, &S.sc, st.g, st.g, S.sv.p + kSvGtg * ns))) return rc;
-        // the line search's first trial straight behind the TOI (step lengths from it on the
-        // device), read back with the TOI, q.g, g.g and the solver's iteration counts
-        hipLaunchKernelGGL(first_trial_kernel, dim3((unsigned)ceil_div(ns, kBlock)), dim3(kBlock), 0, s, ns,
-                           (const double*)(S.sv.p + kSvToi * ns), (const uint8_t*)S.sactive.p, S.cand_alpha.p);
-        ABD_CHECK_LAUNCH();
-        if ((rc = line_energy_scenes_launch(s, S, S.cand_alpha.p, 1))) return rc;
-        static_assert(kSvQtg == kSvToi + 1 && kSvGtg == kSvToi + 2, "rows read back together");
-        ABD_CHECK(hipMemcpyAsync(S.hsv + kSvToi * ns, S.sv.p + kSvToi * ns, sizeof(double) * 3 * ns,
-                                 hipMemcpyDeviceToHost, s));
-        ABD_CHECK(hipMemcpyAsync(S.hiters, S.iters.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
-        ABD_CHECK(hipStreamSynchronize(s));
+        for (int attempt = 0;; ++attempt) {
+            if ((rc = attempt == 0 ? swept_set_launch(s, S) : swept_set_recull(s, S))) return rc;
+            const CandSet& W = S.sw;
+            if ((rc = reduce_plan(s, &S.sc, &P, ns, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg, S.plan.p,
+                                  W.counts())))
+                return rc;
+            // the TOI of the scenes still iterating (sactive: the solver's mask of this iteration)
+            if ((rc = scene_toi(s, &S.sc, &P, ns, st.x, st.xk, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg,
+                                S.plan.p, S.sv.p + kSvToi * ns, S.sactive.p, W.counts())))
+                return rc;
+            if ((rc = abd_scene_dot(s, &S.sc, st.dq, st.g, S.sv.p + kSvQtg * ns))) return rc;
+            // g.g of the stopping test (abd.py:265): g is final here
+            if ((rc = abd_scene_dot(s, &S.sc, st.g, st.g, S.sv.p + kSvGtg * ns))) return rc;
+            // the line search's first trial straight behind the TOI (step lengths from it on the
+            // device), read back with the TOI, q.g, g.g, the solver's iteration counts and the
+            // sets' sizes: the iteration's one wait for the device
+            hipLaunchKernelGGL(first_trial_kernel, dim3((unsigned)ceil_div(ns, kBlock)), dim3(kBlock), 0, s, ns,
+                               (const double*)(S.sv.p + kSvToi * ns), (const uint8_t*)S.sactive.p, S.cand_alpha.p);
+            ABD_CHECK_LAUNCH();
+            if ((rc = line_energy_scenes_launch(s, S, S.cand_alpha.p, 1))) return rc;
+            static_assert(kSvQtg == kSvToi + 1 && kSvGtg == kSvToi + 2, "rows read back together");
+            ABD_CHECK(hipMemcpyAsync(S.hsv + kSvToi * ns, S.sv.p + kSvToi * ns, sizeof(double) * 3 * ns,
+                                     hipMemcpyDeviceToHost, s));
+            ABD_CHECK(hipMemcpyAsync(S.hiters, S.iters.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
+            ABD_CHECK(hipMemcpyAsync(S.hcnt, S.cnt.p, kCntLen * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            ABD_CHECK(hipStreamSynchronize(s));
+            if (!swept_left_ranges(S)) break;
+            S.stats[5] -= 1;  // the trial is evaluated again on the recomputed swept set
+        }
+        count_stats(S);
         for (int k = 0; k < ns; ++k) {
             ls[k] = newton[k];
             S.halpha[k] = first_alpha(S.hsv[kSvToi * ns + k], newton[k] != 0);
@@ -795,6 +841,9 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
         if (!rc) rc = S->cand_alpha.reserve((size_t)nsc * kLsSpec);
         if (!rc) rc = S->plan.reserve(reduce_plan_elems(nsc));
         if (!rc) rc = S->dcount.reserve(8);
+        if (!rc) rc = S->cnt.reserve(kCntLen);
+        if (!rc && hipHostMalloc((void**)&S->hcnt, 2 * kCntLen * sizeof(int64_t), hipHostMallocDefault) != hipSuccess)
+            rc = ABD_ERR_ARG;
         if (!rc && hipHostMalloc((void**)&S->hcand, sizeof(double) * 3 * nsc * kLsSpec, hipHostMallocDefault) !=
                        hipSuccess)
             rc = ABD_ERR_ARG;
@@ -951,8 +1000,9 @@ int abd_sim_destroy(void* sim) {
     S->cand_alpha.release();
     S->plan.release();
     S->dcount.release();
+    S->cnt.release();
     if (S->hcand) (void)hipHostFree(S->hcand);
-    for (void* h : {(void*)S->host, (void*)S->hcount, (void*)S->hsv, (void*)S->halpha, (void*)S->hccd, (void*)S->hmask,
+    for (void* h : {(void*)S->host, (void*)S->hcount, (void*)S->hcnt, (void*)S->hsv, (void*)S->halpha, (void*)S->hccd, (void*)S->hmask,
                     (void*)S->hiters})
         if (h) (void)hipHostFree(h);
     delete S;
