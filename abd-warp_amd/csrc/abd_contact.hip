@@ -626,14 +626,13 @@ __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdP
     if (t == 0 && highlight) highlight[b] = (uint8_t)((hl_pt[b] ? 1 : 0) | (hl_ee[b] ? 2 : 0));
 }
 
-__global__ void zero_u8_kernel(uint8_t* a, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        a[i] = 0;
-}
-__global__ void zero_int_kernel(int* a) { *a = 0; }
-__global__ void zero_i64_kernel(int64_t* a, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        a[i] = 0;
+// the gh stage's flags and counters: the ordering flag, n8 bytes at a8, n64 int64 at a64
+__global__ void gh_zero_kernel(int* __restrict__ err, uint8_t* __restrict__ a8, int64_t n8,
+                               int64_t* __restrict__ a64, int64_t n64) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *err = 0;
+    if (i < n8) a8[i] = 0;
+    if (i < n64) a64[i] = 0;
 }
 
 // Per-contact entry (tests, microbenchmarks): dense 24x24 and gradient per contact.
@@ -758,11 +757,15 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
     uint8_t* hl_pt = misc + 64;
     uint8_t* hl_ee = hl_pt + nb;
     uint8_t* pflag = hl_ee + nb;
-    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, err);
-    ABD_CHECK_LAUNCH();
-    hipLaunchKernelGGL(zero_u8_kernel, dim3(grid_for(2 * nb + nij)), dim3(kBlock), 0, s, hl_pt,
-                       (int64_t)(2 * nb + nij));
-    ABD_CHECK_LAUNCH();
+    // the pair runs of stage C (empty unless written), zeroed with the flags
+    int64_t* pseg = nullptr;  // [2][nij][2]
+    if (nij > 0 && (rc = arena_get(kSlotPairSeg, sizeof(int64_t) * 4 * (size_t)nij, (void**)&pseg))) return rc;
+    {
+        const int64_t n8 = 2 * (int64_t)nb + nij, n64 = 4 * nij;
+        hipLaunchKernelGGL(gh_zero_kernel, dim3(grid_for(n8 > n64 ? n8 : n64, kBlock, INT32_MAX)), dim3(kBlock), 0,
+                           s, err, hl_pt, n8, pseg, n64);
+        ABD_CHECK_LAUNCH();
+    }
     int64_t nact = 0;
     int32_t* act = nullptr;
     const int64_t* dev_nact = nullptr;
@@ -823,10 +826,6 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
             return rc;
         ckey = reinterpret_cast<int64_t*>(cbuf + stride * nact);
         swp = reinterpret_cast<uint8_t*>(ckey + nact);
-        int64_t* pseg;  // [2][nij][2]: empty runs unless written by stage C
-        if ((rc = arena_get(kSlotPairSeg, sizeof(int64_t) * 4 * (size_t)nij, (void**)&pseg))) return rc;
-        hipLaunchKernelGGL(zero_i64_kernel, dim3(grid_for(4 * nij)), dim3(kBlock), 0, s, pseg, 4 * nij);
-        ABD_CHECK_LAUNCH();
         const double cbytes = 4.0 + 20 + 16 + 192 + 8.0 * stride + 9;
         int ph = prof_start(kProfContact, s, dn ? 1.0 : cbytes * nact);
         const int64_t* dnact = dn ? dev_nact : nullptr;

@@ -31,6 +31,59 @@ __global__ __launch_bounds__(kBlock) void project_kernel(const int32_t* __restri
     }
 }
 
+// The end of a step, x = x(q - alpha dq): per vertex the body's state with the arithmetic of
+// update_q_kernel (one alpha_host) or of update_q_scenes_kernel (alpha_dev per scene, or
+// alpha_host for the scenes in `active` and 0 for the others; a scene with alpha 0 keeps its
+// state bit for bit), placed as project_kernel does.  With lo / hi, the range test of
+// within_ranges on this end and on x_other: *flag = gen when some coordinate leaves them.
+__global__ __launch_bounds__(kBlock) void project_step_kernel(const int32_t* __restrict__ vert_body,
+                                                               const int32_t* __restrict__ body_scene,
+                                                               const double* __restrict__ x0,
+                                                               const double* __restrict__ A,
+                                                               const double* __restrict__ p,
+                                                               const double* __restrict__ dq,
+                                                               const double* __restrict__ alpha_dev,
+                                                               double alpha_host,
+                                                               const uint8_t* __restrict__ active,
+                                                               double* __restrict__ x,
+                                                               const double* __restrict__ x_other,
+                                                               const double* __restrict__ lo,
+                                                               const double* __restrict__ hi,
+                                                               int64_t* __restrict__ flag, int64_t gen, int64_t nv) {
+    bool out = false;
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kBlock) {
+        const int b = vert_body[v];
+        double pb[3], Ab[9];
+        if (alpha_dev || active) {
+            const int sc = body_scene ? body_scene[b] : 0;
+            const double al = alpha_dev ? alpha_dev[sc] : (active[sc] ? alpha_host : 0.0);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) pb[r] = al == 0.0 ? p[3 * b + r] : p[3 * b + r] - dq[3 * (4 * b) + r] * al;
+#pragma unroll
+            for (int r = 0; r < 9; ++r)
+                Ab[r] = al == 0.0 ? A[9 * b + r] : A[9 * b + r] - dq[3 * (4 * b + 1 + r / 3) + r % 3] * al;
+        } else {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) pb[r] = p[3 * b + r] - dq[3 * (4 * b) + r] * alpha_host;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) Ab[r] = A[9 * b + r] - dq[3 * (4 * b + 1 + r / 3) + r % 3] * alpha_host;
+        }
+        const double r0 = x0[3 * v], r1 = x0[3 * v + 1], r2 = x0[3 * v + 2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double r = pb[k] + r0 * Ab[k];
+            r = r + r1 * Ab[3 + k];
+            r = r + r2 * Ab[6 + k];
+            x[3 * v + k] = r;
+            if (lo) {
+                const double l = lo[3 * v + k], h = hi[3 * v + k], o = x_other[3 * v + k];
+                out |= !(r >= l && r <= h && o >= l && o <= h);  // NaN counts as outside
+            }
+        }
+    }
+    if (lo && __any(out) && lane_id() == 0) *flag = gen;  // every writer stores gen
+}
+
 // abd.py:337-353
 __global__ void update_q_kernel(int nb, const double* __restrict__ A, const double* __restrict__ p,
                                 const double* __restrict__ dq, double alpha, double* A_out, double* p_out,
@@ -272,6 +325,17 @@ __global__ void update_q_scenes_kernel(int nb, const int32_t* __restrict__ body_
     }
     *out = v;
     if (out2) *out2 = v;
+}
+
+int project_step(hipStream_t s, const AbdScene* sc, const double* A, const double* p, const double* dq,
+                 const double* alpha_dev, double alpha_host, const uint8_t* active, double* x_out,
+                 const double* x_other, const double* lo, const double* hi, int64_t* flag, int64_t gen) {
+    if (sc->n_verts == 0) return ABD_OK;
+    hipLaunchKernelGGL(project_step_kernel, dim3(grid_for(sc->n_verts, kBlock, 1 << 20)), dim3(kBlock), 0, s,
+                       sc->vert_body, sc->n_scenes > 1 ? sc->body_scene : nullptr, sc->x0, A, p, dq, alpha_dev,
+                       alpha_host, active, x_out, x_other, lo, hi, flag, gen, (int64_t)sc->n_verts);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
 }
 
 // q_out = q - alpha dq, also written to (A_out2, p_out2) when they are given

@@ -146,7 +146,27 @@ int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, co
 // active (nullable, [S]): scenes whose TOI is wanted, the others report 1 unevaluated.
 int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
               const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
-              const int64_t* plan, double* toi, const uint8_t* active, const int64_t* dn = nullptr);
+              const int64_t* plan, double* toi, const uint8_t* active, const int64_t* dn = nullptr,
+              double* first_alpha = nullptr);
+// The line search's first step length from a TOI (abd.py:300-302): clamped to [0, 1], x 0.9
+// below 1.  scene_toi's first_alpha (device, nullable) receives it per scene (-1 for the
+// scenes not active).
+__host__ __device__ inline double first_step_length(double t) {
+    t = t < 1.0 ? t : 1.0;  // std::min(1.0, t)
+    t = 0.0 < t ? t : 0.0;  // std::max(0.0, t)
+    if (t < 1.0) t *= 0.9;
+    return t;
+}
+// out1 = a . b and out2 = c . d per scene (S = 1: over all 12 nb rows), one launch
+int dots(hipStream_t s, const AbdScene* sc, int S, const double* a, const double* b, const double* c,
+         const double* d, double* out1, double* out2);
+// The end of a step: x_out = x(q - alpha dq), the arithmetic of update_q + project_vertices
+// with alpha_host (alpha_dev, active null), or of update_q_scenes + project_vertices with
+// alpha_dev [n_scenes] or alpha_host for the scenes with active[s] (0 for the others).  With
+// lo / hi, *flag (device) becomes gen when a coordinate of x_out or of x_other leaves them.
+int project_step(hipStream_t s, const AbdScene* sc, const double* A, const double* p, const double* dq,
+                 const double* alpha_dev, double alpha_host, const uint8_t* active, double* x_out,
+                 const double* x_other, const double* lo, const double* hi, int64_t* flag, int64_t gen);
 // abd_contact_energy (S = 1) / abd_scene_contact_energy with the lists' sizes on the device.
 int contact_energy(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* xk, const int32_t* pt,
                    int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg, const int64_t* dn,

@@ -390,10 +390,14 @@ __global__ __launch_bounds__(kToiSolveThreads) void toi_solve_kernel(AbdScene sc
     }
 }
 
+// alpha (nullable): the line search's first step length of each scene from its TOI
+// (first_step_length; -1 for a scene not in `active`)
 __global__ __launch_bounds__(kRedThreads) void toi_final_kernel(const int64_t* __restrict__ choff,
                                                                 const double* __restrict__ part,
                                                                 const unsigned long long* __restrict__ slots,
-                                                                double* __restrict__ out) {
+                                                                double* __restrict__ out,
+                                                                const uint8_t* __restrict__ active,
+                                                                double* __restrict__ alpha) {
     __shared__ double sh[kRedThreads];
     const int s = blockIdx.x;
     double acc = __longlong_as_double((long long)slots[s]);
@@ -402,7 +406,10 @@ __global__ __launch_bounds__(kRedThreads) void toi_final_kernel(const int64_t* _
         acc = v < acc ? v : acc;
     }
     const double r = block_min(acc, sh);
-    if (threadIdx.x == 0) out[s] = r;
+    if (threadIdx.x == 0) {
+        out[s] = r;
+        if (alpha) alpha[s] = !active || active[s] ? first_step_length(r) : -1.0;
+    }
 }
 
 // out[s] = sum over the rows [12 sboff[s], 12 sboff[s + 1]) of a * b (one workgroup per scene;
@@ -417,6 +424,29 @@ __global__ __launch_bounds__(kRedThreads) void seg_dot_kernel(const int32_t* __r
     for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) acc += a[i] * b[i];
     const double r = block_sum_fast(acc, sh);
     if (threadIdx.x == 0) out[s] = r;
+}
+
+// out1[s] = a . b and out2[s] = c . d over the rows of scene s, each bit-identical to its
+// seg_dot_kernel
+__global__ __launch_bounds__(kRedThreads) void seg_dot2_kernel(const int32_t* __restrict__ sboff, int64_t n,
+                                                               const double* __restrict__ a,
+                                                               const double* __restrict__ b,
+                                                               const double* __restrict__ c,
+                                                               const double* __restrict__ d,
+                                                               double* __restrict__ out1, double* __restrict__ out2) {
+    __shared__ double sh[34];
+    const int s = blockIdx.x;
+    const int64_t i0 = sboff ? 12 * (int64_t)sboff[s] : 0, i1 = sboff ? 12 * (int64_t)sboff[s + 1] : n;
+    double acc1 = 0.0, acc2 = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        acc1 += a[i] * b[i];
+        acc2 += c[i] * d[i];
+    }
+    block_sum2_fast(acc1, acc2, sh);
+    if (threadIdx.x == 0) {
+        out1[s] = acc1;
+        out2[s] = acc2;
+    }
 }
 
 // orthogonal_energy.py:25-51 summed over the bodies of each scene (one workgroup per scene)
@@ -508,7 +538,7 @@ int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
 // scr (nullable: the arena; then the scan and the solve are timed when profiling).
 int toi_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, const double* x0, const double* x1,
                const Lists& L, const int64_t* plan_in, double* out, const uint8_t* active,
-               const ReduceScratch* scr) {
+               const ReduceScratch* scr, double* first_alpha = nullptr) {
     const int64_t bound = chunk_bound(L, S);
     if (bound * kToiSplit > INT32_MAX) return ABD_ERR_ARG;
     const int64_t nq = L.l[0].n + L.l[1].n;  // every PT and EE candidate may queue
@@ -545,7 +575,7 @@ int toi_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
                 (const int32_t*)w.queue, (const int32_t*)w.qcount, w.slots);
     ABD_CHECK_LAUNCH();
     hipLaunchKernelGGL(toi_final_kernel, dim3(S), dim3(kRedThreads), 0, s, choff, (const double*)w.part,
-                       (const unsigned long long*)w.slots, out);
+                       (const unsigned long long*)w.slots, out, active, first_alpha);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }
@@ -662,11 +692,19 @@ int contact_energy(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S,
 
 int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
               const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
-              const int64_t* plan, double* toi, const uint8_t* active, const int64_t* dn) {
+              const int64_t* plan, double* toi, const uint8_t* active, const int64_t* dn, double* first_alpha) {
     if (!P || !x_t0 || !x_t1 || !toi || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
     const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg, dn);  // abd.py:288-299
-    return toi_reduce(s, *sc, *P, S, x_t0, x_t1, L, plan, toi, active, nullptr);
+    return toi_reduce(s, *sc, *P, S, x_t0, x_t1, L, plan, toi, active, nullptr, first_alpha);
+}
+
+int dots(hipStream_t s, const AbdScene* sc, int S, const double* a, const double* b, const double* c,
+         const double* d, double* out1, double* out2) {
+    hipLaunchKernelGGL(seg_dot2_kernel, dim3(S), dim3(kRedThreads), 0, s, S > 1 ? sc->scene_boff : nullptr,
+                       12 * (int64_t)sc->n_bodies, a, b, c, d, out1, out2);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
 }
 
 // ------------------------------------------------- allocation-free TOI scratch

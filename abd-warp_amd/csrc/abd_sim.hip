@@ -104,6 +104,7 @@ struct Sim {
     DevBuf<int64_t> dcount;     // culling totals gathered for one read-back
     DevBuf<int64_t> cnt;        // device sizes of the filtered sets (kCnt*), read back per iteration
     int64_t* hcnt = nullptr;    // pinned copy of cnt
+    int64_t gen = 0;            // the current step's number: the range test's flag value
     double* host = nullptr;     // pinned scalars
     int64_t* hcount = nullptr;  // pinned list counts
     int64_t stats[10] = {};
@@ -113,7 +114,6 @@ struct Sim {
     DevBuf<uint8_t> sactive;  // solver mask [ns]
     double* hsv = nullptr;    // pinned [kSv][ns]
     double* halpha = nullptr;   // pinned step lengths of the line search [ns]
-    double* hccd = nullptr;     // pinned step lengths of the CCD sweep [ns]
     uint8_t* hmask = nullptr;   // pinned solver mask [ns]
     int32_t* hiters = nullptr;  // pinned solver iterations [ns]
     std::vector<int32_t> scene_it;  // Newton iterations of each scene in the last frame
@@ -143,26 +143,8 @@ int trials_to_floor(double a, double alpha_min) {
     return k;
 }
 
-// The line search's first step length of a scene from its TOI (abd.py:300-302: clamp to
-// [0, 1], x 0.9 below 1), or 0 when the scene is not iterating; the host applies the same
-// expression to the same TOI.
-__host__ __device__ inline double first_alpha(double t, bool iterating) {
-    t = t < 1.0 ? t : 1.0;  // std::min(1.0, t)
-    t = 0.0 < t ? t : 0.0;  // std::max(0.0, t)
-    if (t < 1.0) t *= 0.9;
-    return iterating ? t : 0.0;
-}
-
-// first trial of the line search: alpha[s] = first_alpha(toi[s]) for the scenes iterating
-// (every scene when active is null), -1 (no trial) for the others
-__global__ void first_trial_kernel(int ns, const double* __restrict__ toi, const uint8_t* __restrict__ active,
-                                   double* __restrict__ alpha) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < ns) alpha[q] = !active || active[q] ? first_alpha(toi[q], true) : -1.0;
-}
-
 // per-scene value rows of Sim::sv / Sim::hsv
-enum { kSvEc = 0, kSvEi, kSvToi, kSvQtg, kSvGtg, kSvAlpha, kSvAlphaCcd, kSv };
+enum { kSvEc = 0, kSvEi, kSvToi, kSvQtg, kSvGtg, kSvAlpha, kSv };
 // Sim::cnt: the collision set's sizes, the swept set's, the active contacts of the gh
 // stage, the flag of a step leaving the superset's ranges, and a staging row for the
 // exact sizes of a culled collision set
@@ -340,7 +322,8 @@ constexpr double kSkinOverDhat = 30.0;
 // the step lie inside the superset's vertex ranges (every vertex of every scene).  Launched
 // only: the range test's flag (cnt[kCntRange]) comes back with the iteration's scalars, and
 // a step that left the ranges is redone after swept_set_recull.  Without a superset the
-// superset is culled first (swept_set_recull's path).
+// superset is culled first (swept_set_recull's path).  The range test itself runs with
+// the step's end (step_end).
 int swept_set_launch(hipStream_t s, Sim& S);
 
 // The superset culled anew around the current step (its ends widened by the skin; host
@@ -374,9 +357,6 @@ int swept_set_launch(hipStream_t s, Sim& S) {
     // filter per entry of the superset from both ends (far fewer entries than primitives)
     if ((rc = swept_boxes(s, &S.sc, -1, st.x, st.xk, dh, S.tbody.lo.p, S.tbody.hi.p, S.body.lo.p, S.body.hi.p)))
         return rc;
-    // the range test (the flag is 1 once some vertex of either end leaves the ranges)
-    ABD_CHECK(hipMemsetAsync(S.cnt.p + kCntRange, 0, sizeof(int64_t), s));
-    if ((rc = within_ranges(s, 3 * (int64_t)S.nv, st.x, st.xk, S.vlo.p, S.vhi.p, S.cnt.p + kCntRange))) return rc;
     if ((rc = S.sw.reserve_like(S.sup))) return rc;
     const CandSet& W = S.sup;
     const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg, W.counts()};
@@ -395,9 +375,19 @@ int swept_set_launch(hipStream_t s, Sim& S) {
     return ABD_OK;
 }
 
-// The swept set's read-back values (after the iteration's synchronization): true when the
-// step left the superset's ranges, so that the swept set must be recomputed.
-bool swept_left_ranges(const Sim& S) { return S.hcnt[kCntRange] != 0; }
+// The end of the step (xk = x(q - dq) for the scenes in `active`, or every scene), with the
+// range test of both ends against the superset's ranges when there is a superset.
+int step_end(hipStream_t s, Sim& S, const uint8_t* active) {
+    const AbdSimState& st = S.st;
+    ++S.gen;
+    return project_step(s, &S.sc, st.A, st.p, st.dq, nullptr, 1.0, active, st.xk, st.x,
+                        S.sup_valid ? S.vlo.p : nullptr, S.sup_valid ? S.vhi.p : nullptr, S.cnt.p + kCntRange,
+                        S.gen);
+}
+
+// After the iteration's synchronization: true when the step left the superset's ranges, so
+// that the swept set must be recomputed.
+bool swept_left_ranges(const Sim& S) { return S.hcnt[kCntRange] == S.gen; }
 
 // The lists' sizes for the statistics (after the iteration's synchronization).
 void count_stats(Sim& S) {
@@ -468,15 +458,12 @@ int ccd_and_first_trial(hipStream_t s, Sim& S) {
         if ((rc = reduce_plan(s, &S.sc, &S.P, 1, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg, S.plan.p,
                               W.counts())))
             return rc;
+        // the TOI, and from it on the device the first trial's step length
         if ((rc = scene_toi(s, &S.sc, &S.P, 1, st.x, st.xk, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg,
-                            S.plan.p, S.scal.p + 2, nullptr, W.counts())))
+                            S.plan.p, S.scal.p + 2, nullptr, W.counts(), S.cand_alpha.p)))
             return rc;
         // q.g (abd.py:148) and g.g (abd.py:265): g and dq are final here
-        if ((rc = abd_dot(s, st.dq, st.g, 12 * (int64_t)nb, S.scal.p + 3))) return rc;
-        if ((rc = abd_dot(s, st.g, st.g, 12 * (int64_t)nb, S.scal.p + 5))) return rc;
-        hipLaunchKernelGGL(first_trial_kernel, dim3(1), dim3(kBlock), 0, s, 1, (const double*)(S.scal.p + 2),
-                           (const uint8_t*)nullptr, S.cand_alpha.p);
-        ABD_CHECK_LAUNCH();
+        if ((rc = dots(s, &S.sc, 1, st.dq, st.g, st.g, st.g, S.scal.p + 3, S.scal.p + 5))) return rc;
         if ((rc = line_trials(s, &S.sc, &S.P, 1, 1, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot,
                               st.pdot, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg, S.plan.p, S.cand_E.p,
                               W.counts())))
@@ -485,7 +472,8 @@ int ccd_and_first_trial(hipStream_t s, Sim& S) {
         ABD_CHECK(hipMemcpyAsync(S.host + 6, S.iters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         ABD_CHECK(hipMemcpyAsync(S.hcnt, S.cnt.p, kCntLen * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         if ((rc = get_scalars(s, S, 2, 4))) return rc;
-        if (!swept_left_ranges(S)) return ABD_OK;
+        // a recomputed swept set lies inside the superset culled around this very step
+        if (attempt > 0 || !swept_left_ranges(S)) return ABD_OK;
     }
 }
 
@@ -513,7 +501,7 @@ int step(hipStream_t s, Sim& S) {
         const size_t nslot = nb + 2 * (size_t)S.col.nij;
         if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
         ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
-        ABD_CHECK(hipMemsetAsync(st.g, 0, sizeof(double) * 12 * nb, s));
+        // g: overwritten by the inertia gradient (abd.py:236-237 zeroes it first)
         if ((rc = abd_inertia_gradient(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g))) return rc;
         if ((rc = abd_inertia_hessian(s, &S.sc, &P, st.A, S.blocks.p))) return rc;
         const CandSet& C = S.col;
@@ -525,8 +513,7 @@ int step(hipStream_t s, Sim& S) {
                                 P.solver_tol, P.solver_maxiter, 10, S.iters.p, S.scal.p + 4)))
             return rc;
         // CCD over the full step (xk = its end) and the first trial (abd.py:278-303)
-        if ((rc = abd_update_q(s, S.nb, st.A, st.p, st.dq, 1.0, st.Ak, st.pk))) return rc;
-        if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
+        if ((rc = step_end(s, S, nullptr))) return rc;
         if ((rc = ccd_and_first_trial(s, S))) return rc;
         count_stats(S);
         S.stats[5] += 1;
@@ -535,7 +522,7 @@ int step(hipStream_t s, Sim& S) {
         memcpy(&sit, S.host + 6, sizeof(int32_t));
         S.stats[6] += sit;
         // abd.py:146-160 line search (Armijo test as written)
-        double alpha = first_alpha(S.host[2], true), E1 = 0.0;
+        double alpha = first_step_length(S.host[2]), E1 = 0.0;
         for (int pass = 0;; ++pass) {
             // the first trial alone (usually accepted), then the rest down to alpha_min
             const int K = pass == 0 ? 1 : trials_to_floor(alpha, P.alpha_min);
@@ -634,7 +621,6 @@ int step_scenes(hipStream_t s, Sim& S) {
     const int nb = S.nb, ns = S.ns;
     int rc;
     double* dalpha = S.sv.p + kSvAlpha * ns;
-    double* dccd = S.sv.p + kSvAlphaCcd * ns;
     ABD_CHECK(hipMemcpyAsync(st.A, st.A0, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
     ABD_CHECK(hipMemcpyAsync(st.p, st.p0, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
     ABD_CHECK(hipMemcpyAsync(st.Ak, st.A0, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
@@ -659,7 +645,7 @@ int step_scenes(hipStream_t s, Sim& S) {
         const size_t nslot = nb + 2 * (size_t)S.col.nij;
         if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
         ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
-        ABD_CHECK(hipMemsetAsync(st.g, 0, sizeof(double) * 12 * nb, s));
+        // g: overwritten by the inertia gradient (abd.py:236-237 zeroes it first)
         if ((rc = abd_inertia_gradient(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g))) return rc;
         if ((rc = abd_inertia_hessian(s, &S.sc, &P, st.A, S.blocks.p))) return rc;
         const CandSet& C = S.col;
@@ -667,19 +653,14 @@ int step_scenes(hipStream_t s, Sim& S) {
         if ((rc = contact_gh(s, &S.sc, &P, st.x, L, S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, nullptr,
                              S.cnt.p + kCntNact)))
             return rc;
-        for (int k = 0; k < ns; ++k) {
-            S.hmask[k] = newton[k];
-            S.hccd[k] = newton[k] ? 1.0 : 0.0;
-        }
+        for (int k = 0; k < ns; ++k) S.hmask[k] = newton[k];
         if ((rc = upload(s, S.sactive.p, S.hmask, ns))) return rc;
         if ((rc = abd_bsr_solve_scenes(s, P.solver, &S.sc, S.max_scene_bodies, S.row_ptr.p, S.cols.p, S.slots.p,
                                        S.blocks.p, st.g, st.dq, P.solver_tol, P.solver_maxiter, 10, S.sactive.p,
                                        S.iters.p, S.scal.p + 8)))
             return rc;
         // CCD over the full steps of the scenes still iterating (abd.py:278-303)
-        if ((rc = upload(s, dccd, S.hccd, sizeof(double) * ns))) return rc;
-        if ((rc = abd_update_q_scenes(s, &S.sc, st.A, st.p, st.dq, dccd, st.Ak, st.pk))) return rc;
-        if ((rc = abd_project_vertices(s, &S.sc, st.Ak, st.pk, st.xk))) return rc;
+        if ((rc = step_end(s, S, S.sactive.p))) return rc;
         for (int attempt = 0;; ++attempt) {
             if ((rc = attempt == 0 ? swept_set_launch(s, S) : swept_set_recull(s, S))) return rc;
             const CandSet& W = S.sw;
@@ -687,18 +668,16 @@ int step_scenes(hipStream_t s, Sim& S) {
                                   W.counts())))
                 return rc;
             // the TOI of the scenes still iterating (sactive: the solver's mask of this iteration)
+            // and from it the first trial's step lengths, on the device
             if ((rc = scene_toi(s, &S.sc, &P, ns, st.x, st.xk, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg,
-                                S.plan.p, S.sv.p + kSvToi * ns, S.sactive.p, W.counts())))
+                                S.plan.p, S.sv.p + kSvToi * ns, S.sactive.p, W.counts(), S.cand_alpha.p)))
                 return rc;
-            if ((rc = abd_scene_dot(s, &S.sc, st.dq, st.g, S.sv.p + kSvQtg * ns))) return rc;
-            // g.g of the stopping test (abd.py:265): g is final here
-            if ((rc = abd_scene_dot(s, &S.sc, st.g, st.g, S.sv.p + kSvGtg * ns))) return rc;
-            // the line search's first trial straight behind the TOI (step lengths from it on the
-            // device), read back with the TOI, q.g, g.g, the solver's iteration counts and the
-            // sets' sizes: the iteration's one wait for the device
-            hipLaunchKernelGGL(first_trial_kernel, dim3((unsigned)ceil_div(ns, kBlock)), dim3(kBlock), 0, s, ns,
-                               (const double*)(S.sv.p + kSvToi * ns), (const uint8_t*)S.sactive.p, S.cand_alpha.p);
-            ABD_CHECK_LAUNCH();
+            // q.g and g.g of the stopping test (abd.py:148,265): g and dq are final here
+            if ((rc = dots(s, &S.sc, ns, st.dq, st.g, st.g, st.g, S.sv.p + kSvQtg * ns, S.sv.p + kSvGtg * ns)))
+                return rc;
+            // the line search's first trial straight behind the TOI, read back with the TOI,
+            // q.g, g.g, the solver's iteration counts and the sets' sizes: the iteration's one
+            // wait for the device
             if ((rc = line_energy_scenes_launch(s, S, S.cand_alpha.p, 1))) return rc;
             static_assert(kSvQtg == kSvToi + 1 && kSvGtg == kSvToi + 2, "rows read back together");
             ABD_CHECK(hipMemcpyAsync(S.hsv + kSvToi * ns, S.sv.p + kSvToi * ns, sizeof(double) * 3 * ns,
@@ -706,13 +685,14 @@ int step_scenes(hipStream_t s, Sim& S) {
             ABD_CHECK(hipMemcpyAsync(S.hiters, S.iters.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
             ABD_CHECK(hipMemcpyAsync(S.hcnt, S.cnt.p, kCntLen * sizeof(int64_t), hipMemcpyDeviceToHost, s));
             ABD_CHECK(hipStreamSynchronize(s));
-            if (!swept_left_ranges(S)) break;
+            // a recomputed swept set lies inside the superset culled around this very step
+            if (attempt > 0 || !swept_left_ranges(S)) break;
             S.stats[5] -= 1;  // the trial is evaluated again on the recomputed swept set
         }
         count_stats(S);
         for (int k = 0; k < ns; ++k) {
             ls[k] = newton[k];
-            S.halpha[k] = first_alpha(S.hsv[kSvToi * ns + k], newton[k] != 0);
+            S.halpha[k] = newton[k] ? first_step_length(S.hsv[kSvToi * ns + k]) : 0.0;
         }
         // line search, each scene on its own (abd.py:146-160): the first trial, then the
         // trials (alpha, alpha/2, ...) down to alpha_min for the scenes still searching
@@ -842,6 +822,7 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
         if (!rc) rc = S->plan.reserve(reduce_plan_elems(nsc));
         if (!rc) rc = S->dcount.reserve(8);
         if (!rc) rc = S->cnt.reserve(kCntLen);
+        if (!rc && hipMemset(S->cnt.p, 0, sizeof(int64_t) * S->cnt.cap) != hipSuccess) rc = ABD_ERR_ARG;
         if (!rc && hipHostMalloc((void**)&S->hcnt, 2 * kCntLen * sizeof(int64_t), hipHostMallocDefault) != hipSuccess)
             rc = ABD_ERR_ARG;
         if (!rc && hipHostMalloc((void**)&S->hcand, sizeof(double) * 3 * nsc * kLsSpec, hipHostMallocDefault) !=
@@ -867,7 +848,6 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
         if (!rc) rc = S->scal.reserve(8 + (size_t)ns);
         if (!rc && (hipHostMalloc((void**)&S->hsv, sizeof(double) * kSv * ns, hipHostMallocDefault) != hipSuccess ||
                     hipHostMalloc((void**)&S->halpha, sizeof(double) * ns, hipHostMallocDefault) != hipSuccess ||
-                    hipHostMalloc((void**)&S->hccd, sizeof(double) * ns, hipHostMallocDefault) != hipSuccess ||
                     hipHostMalloc((void**)&S->hmask, ns, hipHostMallocDefault) != hipSuccess ||
                     hipHostMalloc((void**)&S->hiters, sizeof(int32_t) * ns, hipHostMallocDefault) != hipSuccess))
             rc = ABD_ERR_ARG;
@@ -1002,7 +982,7 @@ int abd_sim_destroy(void* sim) {
     S->dcount.release();
     S->cnt.release();
     if (S->hcand) (void)hipHostFree(S->hcand);
-    for (void* h : {(void*)S->host, (void*)S->hcount, (void*)S->hcnt, (void*)S->hsv, (void*)S->halpha, (void*)S->hccd, (void*)S->hmask,
+    for (void* h : {(void*)S->host, (void*)S->hcount, (void*)S->hcnt, (void*)S->hsv, (void*)S->halpha, (void*)S->hmask,
                     (void*)S->hiters})
         if (h) (void)hipHostFree(h);
     delete S;
