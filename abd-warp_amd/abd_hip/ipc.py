@@ -63,7 +63,7 @@ class IPCContactEnergy:
 
 
 def pair_gh(kind: int, params: ContactParams, x4: torch.Tensor, r4: torch.Tensor):
-    """Per-contact gh (ipcsk/pt_ipc.py:111-185, ipcsk/ee_ipc.py:194-239) for n contacts:
+    """Per-contact gh (ipcsk/pt_ipc.py:111-185, ipcsk/ee_ipc.py:94-143) for n contacts:
     x4/r4 [n,4,3] current/rest positions -> (active [n] bool, g24 [n,24], H24 [n,24,24])."""
     n = int(x4.shape[0])
     dev = x4.device

@@ -77,7 +77,7 @@ __device__ bool contact_terms(int kind, const d3 v[4], const d3 r[4], double d2h
     lam[2] = -l / (2.0 * e0pn) * (1.0 + f02);
     lam[3] = -l / (2.0 * e0pn) * (1.0 - f02);
     const double lt = l / e2pn;
-    // dcdx (psd/vf.py:44-72, psd/ee.py:311-339), row-major 3x4
+    // dcdx (psd/vf.py:44-72, psd/ee.py:135-163), row-major 3x4
     double D[12];
     if (kind == 0) {
         D[0] = 0.0; D[1] = 1.0; D[2] = -1.0; D[3] = 0.0;
@@ -97,7 +97,7 @@ __device__ bool contact_terms(int kind, const d3 v[4], const d3 r[4], double d2h
         if (k < 4) {
             double om = lam[k] / (lam[k] - lt);
             L = lam[k] * 2.0 * l;
-            // pt_ipc.py:266-269 (EE unclamped, ee_ipc.py:133-136); the psd mode projects the
+            // pt_ipc.py:266-269 (EE unclamped, ee_ipc.py:33-36); the psd mode projects the
             // whole contact-space Hessian below and starts from the unclamped eigenvalue
             if (kind == 0 && !psd) L = L > 0.0 ? L : 0.0;
             if (k < 2) {

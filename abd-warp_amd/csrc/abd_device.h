@@ -73,7 +73,7 @@ __device__ __forceinline__ void inv22_mul(double m00, double m01, double m10, do
 }
 
 // Contact-space coordinates c = (e0, e1perp, e2perp) with (alpha, beta, gamma):
-// psd/vf.py:4-41 (PT: x0 point, x1..x3 triangle) and psd/ee.py:6-31,295-308
+// psd/vf.py:4-41 (PT: x0 point, x1..x3 triangle) and psd/ee.py:6-31,119-133
 // (EE: edges x0x1, x2x3).  The two maps differ only in the edge vectors.
 struct CMap {
     d3 e0, e1p, e2p;
@@ -304,7 +304,7 @@ __device__ inline double collision_time(int kind, const d3 t0[4], const d3 t1[4]
 }
 
 // ------------------------------------------------------------- fetchers
-// affine_body.py:148-263 fetch_pt / fetch_ee over the flattened scene.
+// affine_body.py:106-234 fetch_pt / fetch_ee over the flattened scene.
 struct SceneView {
     const int32_t* voff;
     const int32_t* foff;

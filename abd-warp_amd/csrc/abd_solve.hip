@@ -598,7 +598,10 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
 // the arithmetic, in the same order, of eliminating body by body in (level, body) order.
 // A pivot that is not positive (the system is not positive definite as factored) or a
 // system above the size limit leaves x untouched and status kLdltFailed; else status 1
-// and resid = |b - H x|^2.
+// and resid = |b - H x|^2.  The factor and the W blocks live in the workgroup's dynamic
+// LDS (dyn_blocks 12x12 blocks) when they fit there -- the factor's upper blocks with fill
+// and at most min(kLdltWBlocks, off-diagonal blocks) W blocks -- else in the system's
+// workspace region; the arithmetic is the same in both places.
 constexpr int kLdltColsPerThread = 2;
 constexpr int kLdltCols = kLdltColsPerThread * kSolveThreads;
 constexpr int kLdltWBlocks = kLdltCols / 12;  // bound on a chunk's W blocks
@@ -624,8 +627,10 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
                                                              const uint8_t* __restrict__ active,
                                                              const double* __restrict__ b, double* __restrict__ xg,
                                                              int64_t region, double* __restrict__ ws,
-                                                             int32_t* __restrict__ status, double* __restrict__ resid) {
+                                                             int32_t* __restrict__ status, double* __restrict__ resid,
+                                                             int dyn_blocks) {
     constexpr int MB = kLdltMaxBodies;
+    extern __shared__ double ldlt_dyn[];
     __shared__ unsigned long long adj[MB];   // upper row pattern with fill (bits >= row)
     __shared__ unsigned long long colm[MB];  // colm[i]: the rows k < i whose pattern holds i
     __shared__ unsigned long long levm[MB];  // the bodies of each level
@@ -652,7 +657,7 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
     const int n = 12 * m;
     b += 12 * (int64_t)row0;
     xg += 12 * (int64_t)row0;
-    double* U = ws + region * sidx;
+    double* U = ws + region * sidx;  // in LDS instead when the factor fits there (below)
     double* W = U + 144 * ((int64_t)m * (m + 1) / 2);
     const int e0 = M.row_ptr[row0];
     if (t <= m) lrp[t] = M.row_ptr[row0 + t] - e0;
@@ -716,8 +721,16 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
             acc += __popcll(adj[i]);
         }
         pos[m] = acc;
+        const int offd = acc - m, wmax = offd < kLdltWBlocks ? offd : kLdltWBlocks;
+        fail = acc + wmax <= dyn_blocks ? 2 : 0;  // 2: the factor in LDS (reset below)
     }
     __syncthreads();
+    if (fail == 2) {  // uniform
+        U = ldlt_dyn;
+        W = ldlt_dyn + 144 * (int64_t)pos[m];
+    }
+    __syncthreads();
+    if (t == 0) fail = 0;
     for (int64_t e = t; e < 144 * (int64_t)pos[m]; e += nt) U[e] = 0.0;
     __syncthreads();
     // the matrix's upper blocks (row-major 12x12) into the factor
@@ -1252,9 +1265,23 @@ int launch_ldlt(hipStream_t s, const Bsr& M, int nsys, const int32_t* sboff, con
     const int64_t region = ldlt_region(cap);
     double* ws;
     if ((rc = arena_get(kSlotLdlt, sizeof(double) * (size_t)region * (size_t)nsys, (void**)&ws))) return rc;
+    // dynamic LDS: the device's budget beside the kernel's static LDS, in whole blocks
+    static int dyn_bytes = -1;
+    if (dyn_bytes < 0) {
+        int dev = 0, max_lds = 0;
+        ABD_CHECK(hipGetDevice(&dev));
+        ABD_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        hipFuncAttributes fa;
+        ABD_CHECK(hipFuncGetAttributes(&fa, (const void*)ldlt_kernel));
+        const int avail = max_lds - (int)fa.sharedSizeBytes - 256;
+        dyn_bytes = avail > 0 ? (avail / 1152) * 1152 : 0;
+        if (dyn_bytes > 0)
+            ABD_CHECK(hipFuncSetAttribute((const void*)ldlt_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          dyn_bytes));
+    }
     const int ph = prof_start_ext(kProfSolve, 1.0);
-    prof_launch(ph, ldlt_kernel, dim3(nsys), dim3(kSolveThreads), 0, s, M, sboff, active, b, x, region, ws, status,
-                resid);
+    prof_launch(ph, ldlt_kernel, dim3(nsys), dim3(kSolveThreads), (uint32_t)dyn_bytes, s, M, sboff, active, b, x,
+                region, ws, status, resid, dyn_bytes / 1152);
     ABD_CHECK_LAUNCH();
     if (ph >= 0) {
         // algorithmic bytes of the systems solved directly: each block, b and x once

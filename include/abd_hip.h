@@ -8,7 +8,7 @@
  *  - Every call is stream-ordered on `stream` (a hipStream_t; NULL = the
  *    default stream).  Calls that must return a size to the host (candidate
  *    counts, active counts) wait for `stream` up to the point where that size
- *    is known — as the reference's `compress()` does (simulator/fenwick.py:289);
+ *    is known — as the reference's `compress()` does (simulator/fenwick.py:152-170);
  *    the list itself is written by work still queued on `stream`.
  *  - Return value: ABD_OK (0); ABD_ERR_OVERFLOW (1) when an output list is too
  *    small (the required count is written to *count and nothing is written past

@@ -281,7 +281,7 @@ static void C_vf(v3 x0, v3 x1, v3 x2, v3 x3, v3* e0p, v3* e1p, v3* e2p) {
     *e0p = e0; *e1p = e1perp;
     *e2p = sub(sub(e2, scl(e0, be)), scl(e1, ga));
 }
-/* psd/ee.py:295-308 beta_gamma_ee and :6-31 C_ee (edge a = x0x1, edge b = x2x3) */
+/* psd/ee.py:119-133 beta_gamma_ee and :6-31 C_ee (edge a = x0x1, edge b = x2x3) */
 static void bg_ee(v3 x0, v3 x1, v3 x2, v3 x3, double* beta, double* gamma) {
     v3 e0 = sub(x1, x0), e1 = sub(x3, x2), e2 = sub(x2, x0);
     double e0e0 = dot(e0, e0), e0e1 = dot(e0, e1), e1e1 = dot(e1, e1);
@@ -321,7 +321,7 @@ static void dcdx_pt(v3 x0, v3 x1, v3 x2, v3 x3, double D[12]) {
     D[1 * 4 + 1] = -alpha; D[1 * 4 + 2] = alpha - 1.0; D[1 * 4 + 3] = 1.0;
     D[2 * 4 + 0] = 1.0; D[2 * 4 + 1] = -be; D[2 * 4 + 2] = be + ga - 1.0; D[2 * 4 + 3] = -ga;
 }
-/* psd/ee.py:311-339 dceedx_s */
+/* psd/ee.py:135-163 dceedx_s */
 static void dcdx_ee(v3 x0, v3 x1, v3 x2, v3 x3, double D[12]) {
     v3 e0 = sub(x1, x0), e1 = sub(x3, x2);
     double al = dot(e0, e1) / dot(e0, e0);
@@ -331,7 +331,7 @@ static void dcdx_ee(v3 x0, v3 x1, v3 x2, v3 x3, double D[12]) {
     D[8] = be - 1.0; D[9] = -be; D[10] = ga + 1.0; D[11] = -ga;
 }
 /* psd/hl.py:156-195 eig_Hl_tid + the packaging of ipcsk/pt_ipc.py:259-280 /
- * ipcsk/ee_ipc.py:126-147.  Q: 15 vec3 (5 eigenvectors of 9), lam: 5. */
+ * ipcsk/ee_ipc.py:26-47.  Q: 15 vec3 (5 eigenvectors of 9), lam: 5. */
 static void Q_lambda(v3 e0p, v3 e1p, v3 e2p, int clamp, double Q[45], double lam[5]) {
     double l = signed_distance(e0p, e1p, e2p);
     double e0pn = dot(e0p, e0p), e1pn = dot(e1p, e1p), e2pn = dot(e2p, e2p);
@@ -352,7 +352,7 @@ static void Q_lambda(v3 e0p, v3 e1p, v3 e2p, int clamp, double Q[45], double lam
     q[6] = e2p; q[7] = z; q[8] = scl(e0p, om2);
     q[9] = e2p; q[10] = z; q[11] = scl(e0p, om3);
     double L[4] = {lam0 * 2.0 * l, lam1 * 2.0 * l, lam2 * 2.0 * l, lam3 * 2.0 * l};
-    if (clamp)  /* pt_ipc.py:266-269; EE leaves them unclamped (ee_ipc.py:133-136, D9) */
+    if (clamp)  /* pt_ipc.py:266-269; EE leaves them unclamped (ee_ipc.py:33-36, D9) */
         for (int k = 0; k < 4; ++k) L[k] = L[k] > 0.0 ? L[k] : 0.0;
     /* psd/hl.py:37-39 gl = (0, 0, e2o * l / |e2o|^2); q4 = gl * 2 * l */
     v3 gl2 = dvs(scl(e2p, l), dot(e2p, e2p));
@@ -498,9 +498,9 @@ static void transpose(const double* A, int m, int n, double* T) {
 }
 
 /* Reference per-contact assembly, ipcsk/pt_ipc.py:111-185 (PT) and
- * ipcsk/ee_ipc.py:194-239 (EE), with helpers extract_g/JTg (:194-209 / :262-276),
- * extract_Q, QLQinv (:221-231 / :246-260), dcTHldc (:233-235), JTH12J
- * (:237-246 / :278-287).  npI: number of contact vertices on body I (PT 1, EE 2).
+ * ipcsk/ee_ipc.py:94-143 (EE), with helpers extract_g/JTg (:194-209 / :162-177),
+ * extract_Q, QLQinv (:221-231 / :146-161), dcTHldc (:233-235 / :154-157), JTH12J
+ * (:237-246 / :178-190).  npI: number of contact vertices on body I (PT 1, EE 2).
  * JI: npI x 4, JJ: (4-npI) x 4 rows [1, x0^T].  Outputs g24 (24),
  * Hi/Hj/Hij (12x12 row-major). */
 static void map_pair(const double g12[12], const double Hipc[144], int npI, const double* JI, const double* JJ,
@@ -594,7 +594,7 @@ static void pair_gh_dense(const double Q[45], const double lam[5], const double 
     map_pair(g12, Hipc, npI, JI, JJ, B1, g24, Hi, Hj, Hij);
 }
 
-/* JTg and JTH12J of pt_ipc.py:170-172,237-246 (ee_ipc.py:278-287): the 12-entry
+/* JTg and JTH12J of pt_ipc.py:170-172,237-246 (ee_ipc.py:170-190): the 12-entry
  * vertex-space gradient g12 and Hessian Hipc to the two bodies' affine coordinates. */
 static void map_pair(const double g12[12], const double Hipc[144], int npI, const double* JI, const double* JJ,
                      double B1, double g24[24], double Hi[144], double Hj[144], double Hij[144]) {
@@ -701,7 +701,7 @@ int orc_pair_gh(int kind, const orc_params* P, const v3 v[4], const v3 r[4], dou
     if (!(s < P->d2hat && valid)) return 0;
     double B1 = orc_barrier_d1(s, P->d2hat, P->kappa);
     double B2 = orc_barrier_d2(s, P->d2hat, P->kappa);
-    /* pt_ipc.py:249-255 extract_JpJt / ee_ipc.py:301-311 extract_JeiJej */
+    /* pt_ipc.py:249-255 extract_JpJt / ee_ipc.py:201-211 extract_JeiJej */
     double JI[8], JJ[12];
     int npI = kind == 0 ? 1 : 2;
     for (int i = 0; i < npI; ++i) { JI[4 * i] = 1.0; JI[4 * i + 1] = r[i].x; JI[4 * i + 2] = r[i].y; JI[4 * i + 3] = r[i].z; }

@@ -4,7 +4,7 @@ constructed inside/outside points (test_distance.py:17-81), the analytic
 eigensystem by numpy eigh of the reference's block formula for d2Psi
 (psd/hl.py:20-70), the gradient by finite differences of the barrier energy in
 affine coordinates, and the 12x12 assembly by a numpy restatement of the
-reference's dense host path (ipcsk/pt_ipc.py:194-246, ipcsk/ee_ipc.py:246-287)."""
+reference's dense host path (ipcsk/pt_ipc.py:194-246, ipcsk/ee_ipc.py:146-211)."""
 import numpy as np
 import pytest
 
@@ -164,7 +164,7 @@ def test_gradient_is_derivative_of_barrier_energy(kind):
 
 # ----------------------------------------------- dense reference assembly
 def _ref_dense(kind, G, r4, B1, B2):
-    """ipcsk/pt_ipc.py:194-246 / ipcsk/ee_ipc.py:246-287 in numpy: extract_Q,
+    """ipcsk/pt_ipc.py:194-246 / ipcsk/ee_ipc.py:146-211 in numpy: extract_Q,
     QLQinv, dcTHldc, extract_g, JTg, JTH12J with np.kron."""
     Qn = G["Q"].reshape(15, 3)
     Qm = Qn.reshape(5, 9).T
