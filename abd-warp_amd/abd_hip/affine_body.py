@@ -56,6 +56,7 @@ class WarpMesh:
     x_view: torch.Tensor
     triangles: torch.Tensor
     edges: torch.Tensor
+    meshes: Optional["Meshes"] = None  # the scene the views belong to
 
 
 class Meshes:
@@ -117,11 +118,13 @@ class Meshes:
         return self.n_bodies
 
     def __getitem__(self, b: int) -> WarpMesh:
+        if not 0 <= b < self.n_bodies:
+            raise IndexError(b)
         v0, v1 = int(self.spec.voff[b]), int(self.spec.voff[b + 1])
         f0, f1 = int(self.spec.foff[b]), int(self.spec.foff[b + 1])
         e0, e1 = int(self.spec.eoff[b]), int(self.spec.eoff[b + 1])
         return WarpMesh(b, self.x0[v0:v1], self.x[v0:v1], self.xk[v0:v1], self.x_view[v0:v1],
-                        self.triangles[f0:f1], self.edges[e0:e1])
+                        self.triangles[f0:f1], self.edges[e0:e1], self)
 
     @property
     def shape(self):

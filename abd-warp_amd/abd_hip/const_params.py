@@ -25,7 +25,7 @@ class AbdParamsC(ctypes.Structure):
 SOLVER_BICGSTAB = 0
 SOLVER_CG_JACOBI = 1
 SOLVER_CG_BLOCK = 2    # CG with the 12x12 block-Jacobi preconditioner (beyond the reference)
-SOLVER_LDLT = 3        # direct block LDL^T, Jacobi CG above 64 bodies (beyond the reference)
+SOLVER_LDLT = 3        # direct block LDL^T, block-Jacobi CG above 64 bodies (beyond the reference)
 
 
 @dataclasses.dataclass
@@ -58,6 +58,8 @@ class ContactParams:
         return self.dhat * self.dhat
 
     def to_c(self) -> AbdParamsC:
+        if int(self.psd) not in (0, 1, 2, 3):
+            raise ValueError(f"psd must be 0, 1, 2 or 3, not {self.psd}")
         return AbdParamsC(self.dt, self.dhat, self.d2hat, self.kappa, self.stiffness, self.gravity, self.ground,
                           self.mass, self.I0, self.c1, self.alpha_min, self.tol, self.solver_tol, int(self.max_iter),
                           int(self.solver_maxiter), int(bool(self.pt)), int(bool(self.ee)), int(bool(self.vg)),

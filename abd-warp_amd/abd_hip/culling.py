@@ -1,25 +1,36 @@
 """Candidate culling (culling.py / culling2.py in the reference).
 
-`BvhBuilder` keeps the reference's method names (culling2.py:83-210) but
-builds box sets over the whole flattened scene instead of one Warp BVH per body;
-each returned `BoxSet` exposes `.lowers` / `.uppers` (float32 [n,3]) like a
-`wp.Bvh`, and `.refit()` is a no-op because culling tests the boxes directly
-(abd_cull_prims: exact hull filter + exhaustive chunked tests).
+`BvhBuilder` keeps the reference's method names (culling2.py:83-210) and takes
+either form of their arguments:
+
+- scene-wide: V is a scene array ([V,3] meshes.x / meshes.xk) and the call
+  returns / updates a `BoxSet` over every body;
+- per body, as abd.py:84-93 and :174-200 call it: V and F are one body's views
+  (`meshes[b].xk`, `meshes[b].triangles`) and the call returns / updates that
+  body's `BodyBoxes`, a view of one slot of a scene-wide `BoxSet` (the body's
+  boxes are computed by the same kernel, so both forms give the same boxes).
+
+Both expose `.lowers` / `.uppers` (float32 [n,3]) like a `wp.Bvh`, and
+`.refit()` is a no-op because culling tests the boxes directly (abd_cull_prims:
+exact hull filter + exhaustive chunked tests).
 
 `cull(ij_list, bvh_list1, bvh_list2=None)` (culling.py:8) and
-`cull_vg(lowers, bvh_list, meshes)` (culling.py:54) return int32 tuple lists
+`cull_vg(lowers, bvh_list, meshes)` (culling.py:54) take a `BoxSet` or the
+per-body list of `BodyBoxes` the builder returned, and return int32 tuple lists
 [N,5] / [N,2] like the reference's compressed vec5i / vec2i arrays.
 """
 from __future__ import annotations
 
 import ctypes
 import dataclasses
-from typing import Optional
+from typing import Optional, Sequence, Union
 
+import numpy as np
 import torch
 
 from . import _lib
-from .affine_body import Meshes
+from .affine_body import Meshes, WarpMesh
+from .const_params import ContactParams
 
 
 @dataclasses.dataclass
@@ -32,6 +43,49 @@ class BoxSet:
 
     def refit(self):  # the wp.Bvh API; boxes are tested directly
         return self
+
+
+@dataclasses.dataclass
+class BodyBoxes:
+    """One body's boxes (the per-body wp.Bvh of culling2.py): views of the body's
+    slot of a scene-wide BoxSet."""
+    parent: BoxSet
+    body: int
+    lowers: torch.Tensor
+    uppers: torch.Tensor
+
+    def refit(self):
+        return self
+
+
+Boxes = Union[BoxSet, Sequence[BodyBoxes]]
+
+
+def _scene_boxes(b: Boxes) -> BoxSet:
+    """The scene-wide BoxSet behind `b`: b itself, or the parent of a complete per-body
+    list (every body once, in body order, all views of one BoxSet)."""
+    if isinstance(b, BoxSet):
+        return b
+    items = list(b)
+    if not items or not all(isinstance(x, BodyBoxes) for x in items):
+        raise TypeError("expected a BoxSet or the per-body list of BodyBoxes a BvhBuilder returned")
+    parent = items[0].parent
+    if [x.body for x in items] != list(range(parent.meshes.n_bodies)) or any(x.parent is not parent for x in items):
+        raise ValueError("a per-body box list must hold every body of one scene once, in body order")
+    return parent
+
+
+def _scene_meshes(bodies) -> Meshes:
+    """The Meshes behind `bodies`: a Meshes, or a sequence of its per-body WarpMesh views."""
+    if isinstance(bodies, Meshes):
+        return bodies
+    items = list(bodies)
+    if not items or not all(isinstance(x, WarpMesh) and x.meshes is not None for x in items):
+        raise TypeError("expected a Meshes or a sequence of its per-body WarpMesh views")
+    m = items[0].meshes
+    if [x.id for x in items] != list(range(m.n_bodies)) or any(x.meshes is not m for x in items):
+        raise ValueError("a per-body mesh list must hold every body of one scene once, in body order")
+    return m
 
 
 @dataclasses.dataclass
@@ -61,86 +115,169 @@ def _stream():
 
 
 class BvhBuilder:
-    """culling2.py:83-210.  The positions V passed in are scene-wide [V,3] tensors
-    (meshes.x / meshes.xk), the topology the scene's flattened triangles / edges."""
+    """culling2.py:83-210.  Positions V are a scene array ([V,3] meshes.x / meshes.xk;
+    topology the scene's flattened triangles / edges) or one body's view of it (then
+    F / E are that body's views and the boxes are BodyBoxes)."""
 
-    def __init__(self, meshes: Meshes):
+    def __init__(self, meshes: Optional[Meshes] = None):
         self.meshes = meshes
+        # per kind: the scene-wide box sets behind per-body builds, each with the bodies
+        # already handed out (the reference builds two lists per kind: current and swept)
+        self._parents = {}
+
+    def _scene(self) -> Meshes:
+        if self.meshes is None:
+            raise ValueError("BvhBuilder needs the scene: BvhBuilder(meshes), or build_body_bvh first")
+        return self.meshes
 
     def _new(self, kind: int, n: int) -> BoxSet:
-        dev = self.meshes.device
-        return BoxSet(self.meshes, kind, torch.zeros((n, 3), dtype=torch.float32, device=dev),
+        dev = self._scene().device
+        return BoxSet(self._scene(), kind, torch.zeros((n, 3), dtype=torch.float32, device=dev),
                       torch.zeros((n, 3), dtype=torch.float32, device=dev))
 
-    # points (culling2.py:90-109)
-    def build_point_bvh(self, V, dialation) -> BoxSet:
-        bvh = self._new(_lib.KIND_POINT, self.meshes.n_verts)
-        return self.update_point_bvh(V, dialation, bvh)
+    def _count(self, kind: int) -> int:
+        m = self._scene()
+        return m.n_verts if kind == _lib.KIND_POINT else m.n_edges if kind == _lib.KIND_EDGE else m.n_tris
 
-    def update_point_bvh(self, V, dialation, bvh: BoxSet) -> BoxSet:
-        _lib.call("abd_prim_boxes", _stream(), self.meshes.c, _lib.KIND_POINT, _lib.ptr(V), float(dialation),
-                  _lib.ptr(bvh.lowers), _lib.ptr(bvh.uppers))
+    def _range(self, kind: int, b: int):
+        sp = self._scene().spec
+        off = sp.voff if kind == _lib.KIND_POINT else sp.eoff if kind == _lib.KIND_EDGE else sp.foff
+        return int(off[b]), int(off[b + 1])
+
+    def _body_of(self, V: torch.Tensor):
+        """(body, scene array) of a per-body position view V, or (None, V) for a scene array."""
+        m = self._scene()
+        for base in (m.xk, m.x, m.x0, m.x_view):
+            if V.data_ptr() == base.data_ptr() and V.shape[0] == m.n_verts:
+                return None, base
+        for base in (m.xk, m.x, m.x0, m.x_view):
+            if V.untyped_storage().data_ptr() != base.untyped_storage().data_ptr():
+                continue
+            off = (V.data_ptr() - base.data_ptr()) // (3 * V.element_size())
+            b = int(np.searchsorted(m.spec.voff, off, side="right")) - 1
+            if 0 <= b < m.n_bodies and int(m.spec.voff[b]) == off and int(m.spec.voff[b + 1]) - off == V.shape[0]:
+                return b, base
+        raise ValueError("V must be a scene position array or one body's view of it")
+
+    def _new_body(self, kind: int, b: int) -> BodyBoxes:
+        lst = self._parents.setdefault(kind, [])
+        for parent, taken in lst:
+            if b not in taken:
+                taken.add(b)
+                break
+        else:
+            parent = self._new(kind, self._count(kind))
+            lst.append((parent, {b}))
+        f0, f1 = self._range(kind, b)
+        return BodyBoxes(parent, b, parent.lowers[f0:f1], parent.uppers[f0:f1])
+
+    def _sub_scene(self, kind: int, b: int):
+        """The scene struct restricted to body b's primitives of `kind`."""
+        m = self._scene()
+        c = _lib.AbdScene.from_buffer_copy(m.c)
+        f0, f1 = self._range(kind, b)
+        if kind == _lib.KIND_POINT:
+            c.n_verts = f1 - f0
+        elif kind == _lib.KIND_EDGE:
+            c.n_edges = f1 - f0
+            c.edges = m.edges.data_ptr() + 8 * f0
+            c.edge_body = m.edge_body.data_ptr() + 4 * f0
+        else:
+            c.n_tris = f1 - f0
+            c.tris = m.triangles.data_ptr() + 12 * f0
+            c.tri_body = m.tri_body.data_ptr() + 4 * f0
+        return c
+
+    def _boxes(self, kind: int, V, dialation, bvh):
+        if isinstance(bvh, BodyBoxes):
+            b, base = self._body_of(V)
+            if b != bvh.body:
+                raise ValueError(f"positions of body {b} given for the boxes of body {bvh.body}")
+            c = self._sub_scene(kind, b)
+            x = V if kind == _lib.KIND_POINT else base  # local indices + voff[b] address the scene array
+            _lib.call("abd_prim_boxes", _stream(), ctypes.byref(c), kind, _lib.ptr(x), float(dialation),
+                      _lib.ptr(bvh.lowers), _lib.ptr(bvh.uppers))
+        else:
+            _lib.call("abd_prim_boxes", _stream(), self._scene().c, kind, _lib.ptr(V), float(dialation),
+                      _lib.ptr(bvh.lowers), _lib.ptr(bvh.uppers))
         return bvh
 
-    def point_bvh_to_traj(self, V, bvh_point: BoxSet, bvh_traj: BoxSet) -> BoxSet:
-        bvh_traj.lowers.copy_(bvh_point.lowers)
-        bvh_traj.uppers.copy_(bvh_point.uppers)
-        _lib.call("abd_prim_boxes_traj", _stream(), self.meshes.c, _lib.KIND_POINT, _lib.ptr(V),
-                  _lib.ptr(bvh_traj.lowers), _lib.ptr(bvh_traj.uppers))
-        return bvh_traj
+    def _traj(self, kind: int, V, src, dst):
+        dst.lowers.copy_(src.lowers)
+        dst.uppers.copy_(src.uppers)
+        if isinstance(dst, BodyBoxes):
+            b, base = self._body_of(V)
+            if b != dst.body:
+                raise ValueError(f"positions of body {b} given for the boxes of body {dst.body}")
+            c = self._sub_scene(kind, b)
+            x = V if kind == _lib.KIND_POINT else base
+            _lib.call("abd_prim_boxes_traj", _stream(), ctypes.byref(c), kind, _lib.ptr(x), _lib.ptr(dst.lowers),
+                      _lib.ptr(dst.uppers))
+        else:
+            _lib.call("abd_prim_boxes_traj", _stream(), self._scene().c, kind, _lib.ptr(V), _lib.ptr(dst.lowers),
+                      _lib.ptr(dst.uppers))
+        return dst
+
+    def _build(self, kind: int, V, dialation):
+        b, _ = self._body_of(V)
+        bvh = self._new(kind, self._count(kind)) if b is None else self._new_body(kind, b)
+        return self._boxes(kind, V, dialation, bvh)
+
+    # points (culling2.py:90-109)
+    def build_point_bvh(self, V, dialation):
+        return self._build(_lib.KIND_POINT, V, dialation)
+
+    def update_point_bvh(self, V, dialation, bvh):
+        return self._boxes(_lib.KIND_POINT, V, dialation, bvh)
+
+    def point_bvh_to_traj(self, V, bvh_point, bvh_traj):
+        return self._traj(_lib.KIND_POINT, V, bvh_point, bvh_traj)
 
     # triangles (culling2.py:111-130; the reference spells it "bulid")
-    def bulid_triangle_bvh(self, V, F, dialation) -> BoxSet:
-        bvh = self._new(_lib.KIND_TRI, self.meshes.n_tris)
-        return self.update_triangle_bvh(V, F, dialation, bvh)
+    def bulid_triangle_bvh(self, V, F, dialation):
+        return self._build(_lib.KIND_TRI, V, dialation)
 
     build_triangle_bvh = bulid_triangle_bvh
 
-    def update_triangle_bvh(self, V, F, dialation, bvh: BoxSet) -> BoxSet:
-        _lib.call("abd_prim_boxes", _stream(), self.meshes.c, _lib.KIND_TRI, _lib.ptr(V), float(dialation),
-                  _lib.ptr(bvh.lowers), _lib.ptr(bvh.uppers))
-        return bvh
+    def update_triangle_bvh(self, V, F, dialation, bvh):
+        return self._boxes(_lib.KIND_TRI, V, dialation, bvh)
 
-    def triangle_bvh_to_traj(self, V, F, bvh_triangle: BoxSet, bvh_traj: BoxSet) -> BoxSet:
-        bvh_traj.lowers.copy_(bvh_triangle.lowers)
-        bvh_traj.uppers.copy_(bvh_triangle.uppers)
-        _lib.call("abd_prim_boxes_traj", _stream(), self.meshes.c, _lib.KIND_TRI, _lib.ptr(V),
-                  _lib.ptr(bvh_traj.lowers), _lib.ptr(bvh_traj.uppers))
-        return bvh_traj
+    def triangle_bvh_to_traj(self, V, F, bvh_triangle, bvh_traj):
+        return self._traj(_lib.KIND_TRI, V, bvh_triangle, bvh_traj)
 
     # edges (culling2.py:132-152)
-    def build_edge_bvh(self, V, E, dialation) -> BoxSet:
-        bvh = self._new(_lib.KIND_EDGE, self.meshes.n_edges)
-        return self.update_edge_bvh(V, E, dialation, bvh)
+    def build_edge_bvh(self, V, E, dialation):
+        return self._build(_lib.KIND_EDGE, V, dialation)
 
-    def update_edge_bvh(self, V, E, dialation, bvh: BoxSet) -> BoxSet:
-        _lib.call("abd_prim_boxes", _stream(), self.meshes.c, _lib.KIND_EDGE, _lib.ptr(V), float(dialation),
+    def update_edge_bvh(self, V, E, dialation, bvh):
+        return self._boxes(_lib.KIND_EDGE, V, dialation, bvh)
+
+    def edge_bvh_to_traj(self, V, E, bvh_edge, bvh_traj):
+        return self._traj(_lib.KIND_EDGE, V, bvh_edge, bvh_traj)
+
+    # bodies (culling2.py:154-210): reduced on the GPU instead of per-body numpy; `bodies`
+    # is the Meshes or the sequence of its per-body views (the reference's list of meshes)
+    def build_body_bvh(self, bodies, dialation, field: str = "x") -> BoxSet:
+        m = _scene_meshes(bodies)
+        if self.meshes is None:
+            self.meshes = m
+        bvh = BoxSet(m, -1, torch.zeros((m.n_bodies, 3), dtype=torch.float32, device=m.device),
+                     torch.zeros((m.n_bodies, 3), dtype=torch.float32, device=m.device))
+        _lib.call("abd_body_boxes", _stream(), m.c, _lib.ptr(getattr(m, field)), float(dialation),
                   _lib.ptr(bvh.lowers), _lib.ptr(bvh.uppers))
         return bvh
 
-    def edge_bvh_to_traj(self, V, E, bvh_edge: BoxSet, bvh_traj: BoxSet) -> BoxSet:
-        bvh_traj.lowers.copy_(bvh_edge.lowers)
-        bvh_traj.uppers.copy_(bvh_edge.uppers)
-        _lib.call("abd_prim_boxes_traj", _stream(), self.meshes.c, _lib.KIND_EDGE, _lib.ptr(V),
-                  _lib.ptr(bvh_traj.lowers), _lib.ptr(bvh_traj.uppers))
-        return bvh_traj
-
-    # bodies (culling2.py:154-210): reduced on the GPU instead of per-body numpy
-    def build_body_bvh(self, bodies: Meshes, dialation, field: str = "x") -> BoxSet:
-        bvh = self._new(-1, bodies.n_bodies)
-        _lib.call("abd_body_boxes", _stream(), bodies.c, _lib.ptr(getattr(bodies, field)), float(dialation),
+    def update_body_bvh(self, bodies, dialation, bvh: BoxSet) -> BoxSet:
+        m = _scene_meshes(bodies)
+        _lib.call("abd_body_boxes", _stream(), m.c, _lib.ptr(m.xk), float(dialation),
                   _lib.ptr(bvh.lowers), _lib.ptr(bvh.uppers))
         return bvh
 
-    def update_body_bvh(self, bodies: Meshes, dialation, bvh: BoxSet) -> BoxSet:
-        _lib.call("abd_body_boxes", _stream(), bodies.c, _lib.ptr(bodies.xk), float(dialation),
-                  _lib.ptr(bvh.lowers), _lib.ptr(bvh.uppers))
-        return bvh
-
-    def body_bvh_to_traj(self, bodies: Meshes, bvh_body: BoxSet, bvh_traj: BoxSet) -> BoxSet:
+    def body_bvh_to_traj(self, bodies, bvh_body: BoxSet, bvh_traj: BoxSet) -> BoxSet:
+        m = _scene_meshes(bodies)
         bvh_traj.lowers.copy_(bvh_body.lowers)
         bvh_traj.uppers.copy_(bvh_body.uppers)
-        _lib.call("abd_body_boxes_traj", _stream(), bodies.c, _lib.ptr(bodies.xk), _lib.ptr(bvh_traj.lowers),
+        _lib.call("abd_body_boxes_traj", _stream(), m.c, _lib.ptr(m.xk), _lib.ptr(bvh_traj.lowers),
                   _lib.ptr(bvh_traj.uppers))
         return bvh_traj
 
@@ -196,18 +333,27 @@ def _cull_prims(mode: int, ij: BodyPairs, s1: BoxSet, s2: BoxSet) -> torch.Tenso
     return out[:n]
 
 
-def cull(ij_list, _bvh_list1: BoxSet, _bvh_list2: Optional[BoxSet] = None) -> torch.Tensor:
+def cull(ij_list, _bvh_list1: Boxes, _bvh_list2: Optional[Boxes] = None) -> torch.Tensor:
     """culling.py:8 / :32-52 (cull_ semantics, defect D1 fixed).  With a second box
     set: PT candidates (points of set 1 vs triangles of set 2, both body orders);
     without: EE candidates."""
+    s1 = _scene_boxes(_bvh_list1)
     if _bvh_list2 is None:
-        return _cull_prims(_lib.CULL_EE, ij_list, _bvh_list1, _bvh_list1)
-    return _cull_prims(_lib.CULL_PT, ij_list, _bvh_list1, _bvh_list2)
+        return _cull_prims(_lib.CULL_EE, ij_list, s1, s1)
+    return _cull_prims(_lib.CULL_PT, ij_list, s1, _scene_boxes(_bvh_list2))
 
 
-def cull_vg(lowers: torch.Tensor, _bvh_list: BoxSet, bodies: Meshes, dhat: float, ground: float) -> torch.Tensor:
-    """culling.py:54-76: vertex-ground candidates (body, vertex)."""
-    m = bodies
+def cull_vg(lowers: torch.Tensor, _bvh_list: Boxes, bodies, dhat: Optional[float] = None,
+            ground: Optional[float] = None) -> torch.Tensor:
+    """culling.py:54-76: vertex-ground candidates (body, vertex).  dhat and ground
+    default to the reference's module constants (const_params.py: dhat 1e-3) and the
+    scene's floor height."""
+    m = _scene_meshes(bodies)
+    _bvh_list = _scene_boxes(_bvh_list)
+    if dhat is None:
+        dhat = ContactParams().dhat
+    if ground is None:
+        ground = float(m.spec.ground)
     cap = max(64, getattr(m, "_cap_vg", 64))
     s = _stream()
     while True:

@@ -10,15 +10,10 @@
 namespace abd {
 
 namespace {
-struct DeviceArena {
-    void* ptr[kArenaSlots] = {};
-    size_t size[kArenaSlots] = {};
-    void* host = nullptr;
-    size_t host_size = 0;
-};
 constexpr int kMaxDevices = 64;
-DeviceArena g_arena[kMaxDevices];
+Arena g_arena[kMaxDevices];
 std::mutex g_arena_mutex;
+thread_local Arena* t_arena = nullptr;  // a simulator's own arena during its calls
 
 int current_device(int* dev) {
     hipError_t e = hipGetDevice(dev);
@@ -28,12 +23,16 @@ int current_device(int* dev) {
 }
 }  // namespace
 
+ArenaScope::ArenaScope(Arena* a) : prev_(t_arena) { t_arena = a; }
+ArenaScope::~ArenaScope() { t_arena = prev_; }
+
 int arena_get(int slot, size_t bytes, void** ptr) {
     int dev;
     int rc = current_device(&dev);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lock(g_arena_mutex);
-    DeviceArena& A = g_arena[dev];
+    std::unique_lock<std::mutex> lock(g_arena_mutex, std::defer_lock);
+    if (!t_arena) lock.lock();
+    Arena& A = t_arena ? *t_arena : g_arena[dev];
     if (bytes == 0) bytes = 256;
     if (A.size[slot] < bytes) {
         size_t want = bytes + bytes / 2;
@@ -55,8 +54,9 @@ int arena_host(size_t bytes, void** ptr) {
     int dev;
     int rc = current_device(&dev);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lock(g_arena_mutex);
-    DeviceArena& A = g_arena[dev];
+    std::unique_lock<std::mutex> lock(g_arena_mutex, std::defer_lock);
+    if (!t_arena) lock.lock();
+    Arena& A = t_arena ? *t_arena : g_arena[dev];
     if (A.host_size < bytes) {
         if (A.host) {
             ABD_CHECK(hipDeviceSynchronize());
@@ -76,7 +76,10 @@ int arena_release_current() {
     int rc = current_device(&dev);
     if (rc) return rc;
     std::lock_guard<std::mutex> lock(g_arena_mutex);
-    DeviceArena& A = g_arena[dev];
+    return arena_free(g_arena[dev]);
+}
+
+int arena_free(Arena& A) {
     ABD_CHECK(hipDeviceSynchronize());
     for (int i = 0; i < kArenaSlots; ++i) {
         if (A.ptr[i]) ABD_CHECK(hipFree(A.ptr[i]));

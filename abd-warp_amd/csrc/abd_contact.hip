@@ -418,12 +418,17 @@ __global__ __launch_bounds__(kBlock) void pair_chunks_kernel(int64_t nij, const 
     }
 }
 
-// stage D's item -> pair map: pair ii owns the items [coff[ii], coff[ii] + cnt[ii])
+// stage D's item -> pair map: pair ii owns the items [coff[ii], coff[ii] + cnt[ii]); one
+// wave per pair, its lanes striding over the pair's items
 __global__ __launch_bounds__(kBlock) void pair_items_kernel(int64_t nij, const int64_t* __restrict__ cnt,
                                                             const int64_t* __restrict__ coff,
                                                             int32_t* __restrict__ ipair) {
-    for (int64_t ii = (int64_t)blockIdx.x * kBlock + threadIdx.x; ii < nij; ii += (int64_t)gridDim.x * kBlock)
-        for (int64_t c = 0; c < cnt[ii]; ++c) ipair[coff[ii] + c] = (int32_t)ii;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nw = (int64_t)gridDim.x * (kBlock / kWave);
+    for (int64_t ii = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; ii < nij; ii += nw) {
+        const int64_t n = cnt[ii], o = coff[ii];
+        for (int64_t c = lane; c < n; c += kWave) ipair[o + c] = (int32_t)ii;
+    }
 }
 
 // stage D2: one work item per workgroup (grid-stride over the *total items): the sums, in
@@ -860,7 +865,8 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
                            pcnt);
         ABD_CHECK_LAUNCH();
         if ((rc = scan_exclusive_i64(s, pcnt, pcoff, nij, ptotal))) return rc;
-        hipLaunchKernelGGL(pair_items_kernel, dim3(grid_for(nij)), dim3(kBlock), 0, s, nij, (const int64_t*)pcnt,
+        hipLaunchKernelGGL(pair_items_kernel, dim3(grid_for(nij, kBlock / kWave)), dim3(kBlock), 0, s, nij,
+                           (const int64_t*)pcnt,
                            (const int64_t*)pcoff, ipair);
         ABD_CHECK_LAUNCH();
         // timed: the partial sums (each stored contact read once, its body-order flag once)
@@ -902,6 +908,7 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
                    const int32_t* row_ptr, const int32_t* slots, double* g, double* blocks, uint8_t* highlight,
                    int64_t* n_active) {
     if (!sc || !P || !x || !g || !blocks || npt < 0 || nee < 0 || nvg < 0 || nij < 0) return ABD_ERR_ARG;
+    if (P->psd < 0 || P->psd > 3) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg) || (nij && (!row_ptr || !slots))) return ABD_ERR_ARG;
     if (nij == 0 && (npt || nee)) return ABD_ERR_ARG;
     if (n_active) *n_active = 0;
@@ -912,6 +919,7 @@ int abd_contact_gh(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, 
 int abd_pair_gh(abd_stream_t stream, int32_t kind, const AbdParams* P, const double* x4, const double* r4, int64_t n,
                 uint8_t* active, double* g24, double* H24) {
     if (!P || n < 0 || (n && (!x4 || !r4 || !active || !g24 || !H24)) || (kind != 0 && kind != 1)) return ABD_ERR_ARG;
+    if (P->psd < 0 || P->psd > 3) return ABD_ERR_ARG;
     if (n == 0) return ABD_OK;
     double* scratch;
     const int stride = P->psd >= 2 ? kDenseStride : kContactStride;

@@ -231,9 +231,30 @@ struct GhLists {
 int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const double* x, const GhLists& L,
                const int32_t* row_ptr, const int32_t* slots, double* g, double* blocks, uint8_t* highlight,
                int64_t* n_active, int64_t* n_active_dev);
+// A set of arena buffers.  The stateless entry points share one per device (process-wide);
+// each simulator owns one, made the calling thread's arena by an ArenaScope for the length
+// of a simulator call, so simulators stepped from different threads or streams never share
+// scratch.
+struct Arena {
+    void* ptr[kArenaSlots] = {};
+    size_t size[kArenaSlots] = {};
+    void* host = nullptr;
+    size_t host_size = 0;
+};
+class ArenaScope {
+  public:
+    explicit ArenaScope(Arena* a);
+    ~ArenaScope();
+    ArenaScope(const ArenaScope&) = delete;
+    ArenaScope& operator=(const ArenaScope&) = delete;
+
+  private:
+    Arena* prev_;
+};
 int arena_get(int slot, size_t bytes, void** ptr);
 int arena_host(size_t bytes, void** ptr);  // pinned host memory
 int arena_release_current();
+int arena_free(Arena& a);  // synchronizes the device, then releases every buffer of `a`
 
 // Exclusive scan of n int64 counts (device) in place into `out` (may alias),
 // writing the total to *total (device).  Fixed-order, single workgroup passes.

@@ -676,7 +676,7 @@ bool scenes_ok(const AbdScene* sc) { return sc && sc->n_scenes > 0 && sc->scene_
 int energy_impl(abd_stream_t stream, const AbdScene* sc, const AbdParams* P, int S, const double* xk,
                 const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
                 double* E) {
-    if (!P || !xk || !E || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
+    if (!P || !xk || !E || npt < 0 || nee < 0 || nvg < 0 || P->psd < 0 || P->psd > 3) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
     return contact_energy(to_stream(stream), sc, P, S, xk, pt, npt, ee, nee, vg, nvg, nullptr, E);
 }

@@ -79,6 +79,7 @@ struct CandSet {
 };
 
 struct Sim {
+    Arena arena;  // the stages' scratch during this simulator's calls
     AbdScene sc;
     AbdParams P;
     AbdSimState st;
@@ -776,6 +777,7 @@ extern "C" {
 
 int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimState* state, void** out) {
     if (!scene || !params || !state || !out || scene->n_bodies <= 0) return ABD_ERR_ARG;
+    if (params->psd < 0 || params->psd > 3) return ABD_ERR_ARG;
     Sim* S = new (std::nothrow) Sim();
     if (!S) return ABD_ERR_ARG;
     S->sc = *scene;
@@ -863,6 +865,7 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
 int abd_sim_step(void* sim, abd_stream_t stream, int64_t* stats) {
     if (!sim) return ABD_ERR_ARG;
     Sim& S = *static_cast<Sim*>(sim);
+    ArenaScope scope(&S.arena);
     int rc = S.ns > 1 ? step_scenes(to_stream(stream), S) : step(to_stream(stream), S);
     if (stats)
         for (int k = 0; k < 10; ++k) stats[k] = S.stats[k];
@@ -876,6 +879,7 @@ int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double
         return ABD_ERR_ARG;
     Sim& S = *static_cast<Sim*>(sim);
     if (S.ns > 1) return ABD_ERR_ARG;
+    ArenaScope scope(&S.arena);
     hipStream_t s = to_stream(stream);
     const AbdSimState& st = S.st;
     int rc;
@@ -904,6 +908,7 @@ int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double
 int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen) {
     if (!sim) return ABD_ERR_ARG;
     Sim& S = *static_cast<Sim*>(sim);
+    ArenaScope scope(&S.arena);
     hipStream_t s = to_stream(stream);
     const AbdTuning saved = tuning();
     int rc;
@@ -958,6 +963,7 @@ int abd_sim_destroy(void* sim) {
     if (!sim) return ABD_OK;
     Sim* S = static_cast<Sim*>(sim);
     (void)hipDeviceSynchronize();  // queued work may still use the buffers released below
+    (void)arena_free(S->arena);
     for (BoxBuf* b : {&S->body, &S->tri, &S->edge, &S->pt, &S->tbody, &S->sbody,
                       &S->stri, &S->sedge, &S->spt}) {
         b->lo.release();

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "abd_block.h"
 #include "abd_internal.h"
@@ -1209,6 +1210,8 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
     const bool lds = staged && kLdsStructBytes + vec_bytes <= usable;
     const size_t dyn = staged ? usable : 0;
     static bool attr_set = false;
+    static std::mutex attr_mu;  // the first calls of several threads
+    std::unique_lock<std::mutex> attr_lock(attr_mu);
     if (!attr_set) {
         const int cap = (int)usable;
         const void* fns[] = {(const void*)bicgstab_kernel<true>,     (const void*)bicgstab_kernel<false>,
@@ -1217,6 +1220,7 @@ int launch_solve(hipStream_t s, int32_t method, const Bsr& M, int nsys, const in
         for (const void* f : fns) ABD_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, cap));
         attr_set = true;
     }
+    attr_lock.unlock();
     if (lds) {
         if (method == ABD_SOLVER_BICGSTAB)
             hipLaunchKernelGGL(bicgstab_kernel<true>, dim3(nsys), dim3(threads), dyn, s, M, sboff, active, b, x, tol,
@@ -1267,6 +1271,8 @@ int launch_ldlt(hipStream_t s, const Bsr& M, int nsys, const int32_t* sboff, con
     if ((rc = arena_get(kSlotLdlt, sizeof(double) * (size_t)region * (size_t)nsys, (void**)&ws))) return rc;
     // dynamic LDS: the device's budget beside the kernel's static LDS, in whole blocks
     static int dyn_bytes = -1;
+    static std::mutex dyn_mu;  // the first calls of several threads
+    std::unique_lock<std::mutex> dyn_lock(dyn_mu);
     if (dyn_bytes < 0) {
         int dev = 0, max_lds = 0;
         ABD_CHECK(hipGetDevice(&dev));
@@ -1279,6 +1285,7 @@ int launch_ldlt(hipStream_t s, const Bsr& M, int nsys, const int32_t* sboff, con
             ABD_CHECK(hipFuncSetAttribute((const void*)ldlt_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           dyn_bytes));
     }
+    dyn_lock.unlock();
     const int ph = prof_start_ext(kProfSolve, 1.0);
     prof_launch(ph, ldlt_kernel, dim3(nsys), dim3(kSolveThreads), (uint32_t)dyn_bytes, s, M, sboff, active, b, x,
                 region, ws, status, resid, dyn_bytes / 1152);

@@ -91,7 +91,9 @@ int abd_set_tuning(const AbdTuning* tuning);
  * point-point; no validity gate), EE times the mollifier of nearly parallel edges,
  * the 12x12 Hessian in the points' coordinates projected onto the PSD cone, the
  * vertex-ground block's 4 d^2 b'' + 2 b' clamped at 0.
- * 1, 2 and 3 have no reference counterpart. */
+ * 1, 2 and 3 have no reference counterpart.  The entry points that read psd (contact
+ * gradient/Hessian, pair terms, contact energy, simulator creation) return ABD_ERR_ARG
+ * for any other value. */
 typedef struct AbdParams {
     double dt, dhat, d2hat, kappa, stiffness, gravity, ground, mass, I0, c1, alpha_min, tol;
     double solver_tol;

@@ -105,3 +105,12 @@ def test_tuning_round_trip_and_bounds():
     L.abd_get_tuning(ctypes.byref(t1))
     assert (t1.cull_chunk, t1.cull_grid) == (8192, 4096)
     assert L.abd_set_tuning(ctypes.byref(t0)) == 0
+
+
+def test_contact_params_reject_unknown_psd():
+    from abd_hip.const_params import ContactParams
+    for psd in (0, 1, 2, 3):
+        assert ContactParams(psd=psd).to_c().psd == psd
+    for psd in (-1, 4):
+        with pytest.raises(ValueError):
+            ContactParams(psd=psd).to_c()

@@ -118,3 +118,48 @@ def test_closed_form_frame(name, native):
     np.testing.assert_allclose(sim.states.p.cpu().numpy(), p_want, rtol=0, atol=tol)
     if name.startswith("free_fall"):
         np.testing.assert_array_equal(sim.states.A.cpu().numpy(), np.eye(3)[None])
+
+
+def test_two_simulators_stepped_from_two_threads():
+    """Each simulator owns its scratch: two native simulators stepped at the same time
+    from two threads, each on its own stream, follow the trajectories they follow alone."""
+    import threading
+    from abd_hip import scenes
+    from abd_hip.abd import AffineBodySimulator
+    from abd_hip.const_params import ContactParams
+
+    def specs():
+        a = scenes.config_medium(seed=0, n_side=2, subdiv=2)
+        a.p[:] = a.p * 0.93
+        return [a, scenes.config_batched(n_scenes=6, seed0=11)]
+
+    frames = 12
+
+    def run(sim, out, stream):
+        with torch.cuda.stream(stream):
+            for f in range(frames):
+                sim.step(f)
+            stream.synchronize()
+            out.append((sim.states.A.cpu().numpy(), sim.states.p.cpu().numpy()))
+
+    alone = []
+    for spec in specs():
+        cp = ContactParams(ground=spec.ground, solver=3, psd=1)
+        out = []
+        sim = AffineBodySimulator(spec, cp, "cuda", native=True)
+        torch.cuda.synchronize()
+        run(sim, out, torch.cuda.Stream())
+        alone.append(out[0])
+    sims = [AffineBodySimulator(spec, ContactParams(ground=spec.ground, solver=3, psd=1), "cuda", native=True)
+            for spec in specs()]
+    torch.cuda.synchronize()
+    outs = [[], []]
+    threads = [threading.Thread(target=run, args=(sims[k], outs[k], torch.cuda.Stream())) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for k in range(2):
+        assert outs[k], f"simulator {k} did not finish"
+        np.testing.assert_array_equal(outs[k][0][0], alone[k][0])
+        np.testing.assert_array_equal(outs[k][0][1], alone[k][1])

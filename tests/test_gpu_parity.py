@@ -90,6 +90,86 @@ def _stage_lists(m, cp):
     return dict(bodies=bodies, tri=tri, edg=edg, pts=pts, pairs=pairs, ee=ee, pt=pt, vg=vg)
 
 
+@pytest.mark.parametrize("which", [0, 1])
+def test_per_body_builder_loops_equal_scene_wide(lib, which):
+    """The reference's per-body BvhBuilder loops (abd.py:84-93, :171-184, :192-206), run
+    unchanged over Meshes' per-body views, give the scene-wide boxes and candidate lists."""
+    from abd_hip.culling import BvhBuilder, cull, cull_vg, ij_list
+    spec = _scenes()[which]
+    cp = params(ground=spec.ground)
+    m = _gpu_scene(spec)
+    st = m.states()
+    from abd_hip import _lib
+    _lib.call("abd_project_vertices", _lib.stream_handle(), m.c, _lib.ptr(st.A), _lib.ptr(st.p), _lib.ptr(m.xk))
+    m.x.copy_(m.xk)
+    m.xk.add_(torch.as_tensor([0.0, -3e-3, 1e-3], dtype=torch.float64, device=DEV))  # a swept step
+    dhat, dh = cp.dhat, 0.5 * cp.dhat
+    meshes = list(m)  # the reference's per-body mesh list
+    assert len(meshes) == spec.n_bodies
+    bb = BvhBuilder()
+    bvh_bodies = bb.build_body_bvh(meshes, dh)
+    bvh_body_traj = bb.build_body_bvh(meshes, dh)
+    tris, edges, points, tris_t, edges_t, points_t = [], [], [], [], [], []
+    for b in meshes:
+        tris.append(bb.bulid_triangle_bvh(b.x, b.triangles, 0.0))
+        edges.append(bb.build_edge_bvh(b.x, b.edges, dh))
+        points.append(bb.build_point_bvh(b.x, dhat))
+        tris_t.append(bb.bulid_triangle_bvh(b.x, b.triangles, 0.0))
+        edges_t.append(bb.build_edge_bvh(b.x, b.edges, dh))
+        points_t.append(bb.build_point_bvh(b.x, dhat))
+    # collision_set (abd.py:192-206)
+    bb.update_body_bvh(meshes, dh, bvh_bodies)
+    ij = ij_list(bvh_bodies, m)
+    for b, t, e, p in zip(meshes, tris, edges, points):
+        bb.update_triangle_bvh(b.xk, b.triangles, 0.0, t)
+        bb.update_edge_bvh(b.xk, b.edges, dh, e)
+        bb.update_point_bvh(b.xk, dhat, p)
+    ee, pt = cull(ij, edges), cull(ij, points, tris)
+    vg = cull_vg(bvh_bodies.lowers, points, meshes)
+    # the same flows scene-wide
+    sw = BvhBuilder(m)
+    S_bodies = sw.build_body_bvh(m, dh, field="xk")
+    S_tri = sw.bulid_triangle_bvh(m.xk, m.triangles, 0.0)
+    S_edg = sw.build_edge_bvh(m.xk, m.edges, dh)
+    S_pts = sw.build_point_bvh(m.xk, dhat)
+    S_ij = ij_list(S_bodies, m)
+    assert torch.equal(bvh_bodies.lowers, S_bodies.lowers) and torch.equal(bvh_bodies.uppers, S_bodies.uppers)
+    for lst, S in ((tris, S_tri), (edges, S_edg), (points, S_pts)):
+        parent = lst[0].parent
+        assert all(x.parent is parent for x in lst) and [x.body for x in lst] == list(range(spec.n_bodies))
+        assert torch.equal(parent.lowers, S.lowers) and torch.equal(parent.uppers, S.uppers)
+    assert torch.equal(S_ij.ij, ij.ij)
+    assert torch.equal(cull(S_ij, S_edg), ee) and torch.equal(cull(S_ij, S_pts, S_tri), pt)
+    assert torch.equal(cull_vg(S_bodies.lowers, S_pts, m, cp.dhat, cp.ground), vg)
+    # the trajectory set (abd.py:170-184): boxes of x, swept to xk
+    bb.body_bvh_to_traj(meshes, bb.build_body_bvh(meshes, dh), bvh_body_traj)
+    for i, b in enumerate(meshes):
+        bb.update_triangle_bvh(b.x, b.triangles, 0.0, tris[i])
+        bb.update_edge_bvh(b.x, b.edges, dh, edges[i])
+        bb.update_point_bvh(b.x, dhat, points[i])
+        bb.triangle_bvh_to_traj(b.xk, b.triangles, tris[i], tris_t[i])
+        bb.edge_bvh_to_traj(b.xk, b.edges, edges[i], edges_t[i])
+        bb.point_bvh_to_traj(b.xk, points[i], points_t[i])
+    ij_t = ij_list(bvh_body_traj, m)
+    ee_t, pt_t = cull(ij_t, edges_t), cull(ij_t, points_t, tris_t)
+
+    S_tri_x = sw.bulid_triangle_bvh(m.x, m.triangles, 0.0)
+    S_edg_x = sw.build_edge_bvh(m.x, m.edges, dh)
+    S_pts_x = sw.build_point_bvh(m.x, dhat)
+    S_tri_t = sw.triangle_bvh_to_traj(m.xk, m.triangles, S_tri_x, sw.bulid_triangle_bvh(m.x, m.triangles, 0.0))
+    S_edg_t = sw.edge_bvh_to_traj(m.xk, m.edges, S_edg_x, sw.build_edge_bvh(m.x, m.edges, dh))
+    S_pts_t = sw.point_bvh_to_traj(m.xk, S_pts_x, sw.build_point_bvh(m.x, dhat))
+    for lst, S in ((tris_t, S_tri_t), (edges_t, S_edg_t), (points_t, S_pts_t)):
+        parent = lst[0].parent
+        assert all(x.parent is parent for x in lst)
+        assert torch.equal(parent.lowers, S.lowers) and torch.equal(parent.uppers, S.uppers)
+    S_bt = sw.body_bvh_to_traj(m, sw.build_body_bvh(m, dh), sw.build_body_bvh(m, dh))
+    assert torch.equal(S_bt.lowers, bvh_body_traj.lowers) and torch.equal(S_bt.uppers, bvh_body_traj.uppers)
+    S_ij_t = ij_list(S_bt, m)
+    assert torch.equal(cull(S_ij_t, S_edg_t), ee_t) and torch.equal(cull(S_ij_t, S_pts_t, S_tri_t), pt_t)
+    assert pt.shape[0] + ee.shape[0] > 0 and pt_t.shape[0] + ee_t.shape[0] > 0
+
+
 def _scenes():
     from abd_hip import scenes
     med = scenes.config_medium(seed=3, n_side=2)  # 8 icosphere-3 bodies
@@ -250,6 +330,39 @@ def test_gh_rejects_unsorted_lists(lib):
     assert e.value.code == _lib.ABD_ERR_UNSORTED
 
 
+@pytest.mark.parametrize("psd", [-1, 4])
+def test_entry_points_reject_unknown_psd(lib, psd):
+    """psd outside 0..3 is an argument error of every entry point that reads it."""
+    from abd_hip import _lib
+    from abd_hip._lib import AbdError
+    from abd_hip.abd import AffineBodySimulator
+    from abd_hip.ipc import IPCContactEnergy, pair_gh
+    spec = contact_scene()
+    good = params(ground=spec.ground)
+    bad = good.to_c()
+    bad.psd = psd  # past ContactParams' own check: the library's
+    cp = params(ground=spec.ground)
+    cp.to_c = lambda: bad
+    x4 = torch.zeros((1, 4, 3), dtype=torch.float64, device=DEV)
+    with pytest.raises(AbdError) as e:
+        pair_gh(0, cp, x4, x4.clone())
+    assert e.value.code == _lib.ABD_ERR_ARG
+    m = _gpu_scene(spec)
+    _lib.call("abd_project_vertices", _lib.stream_handle(), m.c, _lib.ptr(m.states().A), _lib.ptr(m.states().p),
+              _lib.ptr(m.xk))
+    m.x.copy_(m.xk)
+    G = _stage_lists(m, good)
+    nb, nij = spec.n_bodies, G["pairs"].nij
+    g = torch.zeros((4 * nb, 3), dtype=torch.float64, device=DEV)
+    blocks = torch.zeros(((nb + 2 * nij) * 16, 3, 3), dtype=torch.float64, device=DEV)
+    with pytest.raises(AbdError) as e:
+        IPCContactEnergy(cp).gh([G["pt"], G["ee"], G["vg"], nij, m, g, blocks])
+    assert e.value.code == _lib.ABD_ERR_ARG
+    with pytest.raises(AbdError) as e:
+        AffineBodySimulator(spec, cp, "cuda", native=True)
+    assert e.value.code == _lib.ABD_ERR_ARG
+
+
 def test_inertia_matches_oracle(lib):
     from abd_hip.orthogonal_energy import InertialEnergy
     from abd_hip import scenes
@@ -354,6 +467,23 @@ def test_bsr_solve_matches_oracle(lib, method, nb):
 
 
 # -------------------------------------------------------------------------- CCD
+@pytest.mark.parametrize("nb,npair", [(5, 0), (12, 20), (300, 900)])
+def test_set_triplets_equal_oracle(lib, nb, npair):
+    """abd_set_triplets (orthogonal_energy.py:138-165 over all Nb + 2 Nij slots) equals the
+    oracle's triplets, which assemble the dense matrix (test_oracle_stages)."""
+    from abd_hip.orthogonal_energy import set_triplets
+    rng = np.random.default_rng(nb + npair)
+    pairs = sorted({tuple(sorted(rng.choice(nb, 2, replace=False))) for _ in range(npair)})
+    ij = np.array(pairs, np.int32).reshape(-1, 2)
+    nij = ij.shape[0]
+    rows = torch.full(((nb + 2 * nij) * 16,), -7, dtype=torch.int32, device=DEV)
+    cols = torch.full_like(rows, -7)
+    set_triplets(nb, nij, torch.from_numpy(ij).to(DEV), rows, cols)
+    r_ref, c_ref = O.set_triplets(nb, ij)
+    np.testing.assert_array_equal(rows.cpu().numpy(), r_ref)
+    np.testing.assert_array_equal(cols.cpu().numpy(), c_ref)
+
+
 def test_cubic_roots_match_oracle(lib):
     from abd_hip.ccd import cubic_roots
     rng = np.random.default_rng(7)
