@@ -50,6 +50,11 @@ void prof_stop(int handle, hipStream_t s, const int32_t* dev_scale = nullptr, co
 int prof_count_bytes(int handle, hipStream_t s, const int64_t* dn, double c_pt, double c_ee, double c_vg,
                      double c_nij, bool single = false, double base = 0.0);
 
+// A device double (a slot of a ring reused in stream order) for a byte count that a caller's
+// own kernel computes; attach it with prof_stop(handle, s, nullptr, *out) after that kernel.
+// *out is null when the handle is -1.
+int prof_byte_slot(int handle, double** out);
+
 // A record whose events the kernel dispatch itself takes (prof_launch): it times the kernel
 // alone, without the gaps around it (e.g. a launch behind a host synchronization).
 int prof_start_ext(int kernel, double bytes);

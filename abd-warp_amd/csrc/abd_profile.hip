@@ -100,13 +100,23 @@ constexpr int kByteSlots = 1 << 14;  // device byte counts in flight (reused in 
 int g_byte_next = 0;
 }  // namespace
 
-int prof_count_bytes(int handle, hipStream_t s, const int64_t* dn, double c_pt, double c_ee, double c_vg,
-                     double c_nij, bool single, double base) {
-    if (handle < 0 || !dn) return ABD_OK;
+int prof_byte_slot(int handle, double** out) {
+    *out = nullptr;
+    if (handle < 0) return ABD_OK;
     double* slots;
     int rc = arena_get(kSlotProfBytes, sizeof(double) * kByteSlots, (void**)&slots);
     if (rc) return rc;
-    double* out = slots + (g_byte_next++ % kByteSlots);
+    std::lock_guard<std::mutex> lock(g_mu);
+    *out = slots + (g_byte_next++ % kByteSlots);
+    return ABD_OK;
+}
+
+int prof_count_bytes(int handle, hipStream_t s, const int64_t* dn, double c_pt, double c_ee, double c_vg,
+                     double c_nij, bool single, double base) {
+    if (handle < 0 || !dn) return ABD_OK;
+    double* out;
+    int rc = prof_byte_slot(handle, &out);
+    if (rc) return rc;
     hipLaunchKernelGGL(prof_bytes_kernel, dim3(1), dim3(1), 0, s, dn, c_pt, c_ee, c_vg, c_nij, single ? 1 : 0, base,
                        out);
     ABD_CHECK_LAUNCH();

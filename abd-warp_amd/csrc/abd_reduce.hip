@@ -534,6 +534,39 @@ int seg_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
     return ABD_OK;
 }
 
+// Algorithmic bytes of a launch over the plan's scenes: c_ee, c_pt, c_vg per EE, PT and VG
+// candidate of every (trial k, scene s) it evaluates -- K trials with step lengths alpha
+// [K][S] (nullable; a negative step length skips the scene's trial) of the scenes with
+// active[s] set (nullable: all) -- from the plan's per-scene segments.
+__global__ __launch_bounds__(kRedThreads) void seg_bytes_kernel(int S, int K, const double* __restrict__ alpha,
+                                                                const uint8_t* __restrict__ active,
+                                                                const int64_t* __restrict__ seg, double c_ee,
+                                                                double c_pt, double c_vg, double* __restrict__ out) {
+    __shared__ double sh[kRedThreads];
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < (int64_t)S * K; i += blockDim.x) {
+        const int64_t sc = i % S;
+        if ((alpha && !(alpha[i] >= 0.0)) || (active && !active[sc])) continue;
+        const int64_t* q = seg + 2 * (int64_t)kLists * sc;
+        acc += c_ee * (double)(q[1] - q[0]) + c_pt * (double)(q[3] - q[2]) + c_vg * (double)(q[5] - q[4]);
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) *out = acc;
+}
+
+// Attaches to `handle` the bytes of seg_bytes_kernel (nothing when the handle is -1).
+int attach_seg_bytes(int handle, hipStream_t s, int S, int K, const double* alpha, const uint8_t* active,
+                     const int64_t* seg, double c_ee, double c_pt, double c_vg) {
+    double* out;
+    int rc = prof_byte_slot(handle, &out);
+    if (rc || !out) return rc;
+    hipLaunchKernelGGL(seg_bytes_kernel, dim3(1), dim3(kRedThreads), 0, s, S, K, alpha, active, seg, c_ee, c_pt,
+                       c_vg, out);
+    ABD_CHECK_LAUNCH();
+    prof_stop(handle, s, nullptr, out);
+    return ABD_OK;
+}
+
 // Per-scene TOI (toi_scan_kernel, toi_solve_kernel, toi_final_kernel) with scratch from
 // scr (nullable: the arena; then the scan and the solve are timed when profiling).
 int toi_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, const double* x0, const double* x1,
@@ -562,13 +595,13 @@ int toi_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
                        w.slots, w.qcount);
     ABD_CHECK_LAUNCH();
     // algorithmic bytes of the scan: tuple, indices and four vertices at both ends per PT / EE
-    // candidate (224 / 228 B), tuple, offset and a vertex at both ends per VG candidate (56 B)
-    const int ph = scr ? -1
-                       : prof_start_ext(kProfToi, L.dn ? 1.0 : 224.0 * L.l[1].n + 228.0 * L.l[0].n + 56.0 * L.l[2].n);
+    // candidate (224 / 228 B), tuple, offset and a vertex at both ends per VG candidate (56 B),
+    // over the scenes it evaluates (counted on the device)
+    const int ph = scr ? -1 : prof_start_ext(kProfToi, 1.0);
     prof_launch(ph, toi_scan_kernel, dim3((unsigned)bound, kToiSplit), dim3(kRedThreads), 0, s, sc, P, x0, x1, L, S,
                 seg, choff, active, w.part, w.queue, w.qcount);
     ABD_CHECK_LAUNCH();
-    if ((rc = prof_count_bytes(ph, s, L.dn, 224.0, 228.0, 56.0, 0.0))) return rc;
+    if ((rc = attach_seg_bytes(ph, s, S, 1, nullptr, active, seg, 228.0, 224.0, 56.0))) return rc;
     const unsigned grid = (unsigned)std::min<int64_t>(kToiSolveGrid, std::max<int64_t>(1, ceil_div(nq, 4 * kToiSolveThreads)));
     const int ph2 = scr ? -1 : prof_start_ext(kProfToiSolve, 0.0);
     prof_launch(ph2, toi_solve_kernel, dim3(grid), dim3(kToiSolveThreads), 0, s, sc, L, S, x0, x1,
@@ -654,12 +687,13 @@ int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, in
     double* part;
     if ((rc = get_plan(s, *sc, S, L, plan_in, &seg, &choff))) return rc;
     if ((rc = arena_get(kSlotReduce, sizeof(double) * (size_t)bound * K, (void**)&part))) return rc;
-    int ph = prof_start_ext(kProfEnergy, dn ? (double)K : K * (128.0 * npt + 132.0 * nee + 32.0 * nvg));
+    // bytes: those of the (trial, scene) pairs evaluated, counted on the device
+    int ph = prof_start_ext(kProfEnergy, 1.0);
     prof_launch(ph, chunk_kernel<2>, dim3((unsigned)bound, K), dim3(kRedThreads), 0, s, *sc, *P,
                 (const double*)nullptr, (const double*)nullptr, TrialArgs{alpha, A, p, dq}, L, S, seg, choff,
                 part, bound);
     ABD_CHECK_LAUNCH();
-    if ((rc = prof_count_bytes(ph, s, dn, 128.0 * K, 132.0 * K, 32.0 * K, 0.0))) return rc;
+    if ((rc = attach_seg_bytes(ph, s, S, K, alpha, nullptr, seg, 132.0, 128.0, 32.0))) return rc;
     hipLaunchKernelGGL(scene_final_kernel<0>, dim3(S, K), dim3(kRedThreads), 0, s, choff, (const double*)part, E,
                        bound, (int64_t)2 * S);
     ABD_CHECK_LAUNCH();

@@ -2,10 +2,11 @@
 // run from C++ over the stage entry points of this library.  The candidate lists a
 // Newton iteration filters from the candidate superset keep their sizes on the device
 // (every launch sized by the superset's, which bound them), so an iteration waits for
-// the device once: for the scalars the line search and the convergence test branch on
-// (time of impact, q.g, g.g, the first trial's energies), read back with the lists'
-// sizes.  The arithmetic is the same sequence of kernels as the Python mirror, so both
-// give identical states.
+// the device once: the plain driver for the scalars its line search and convergence
+// test branch on (time of impact, q.g, g.g, the first trial's energies), the batched
+// driver, whose line searches and convergence tests run on the device, for the lists'
+// sizes and the number of scenes still iterating.  The arithmetic is the same sequence
+// of kernels as the Python mirror, so both give identical states.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -111,12 +112,15 @@ struct Sim {
     int64_t stats[10] = {};
     // independent scenes (n_scenes > 1): per-scene device values and pinned host copies
     int ns = 1, max_scene_bodies = 0;
-    DevBuf<double> sv;        // [kSv][ns]
-    DevBuf<uint8_t> sactive;  // solver mask [ns]
-    double* hsv = nullptr;    // pinned [kSv][ns]
-    double* halpha = nullptr;   // pinned step lengths of the line search [ns]
-    uint8_t* hmask = nullptr;   // pinned solver mask [ns]
-    int32_t* hiters = nullptr;  // pinned solver iterations [ns]
+    DevBuf<double> sv;          // [kSv][ns]
+    // the batched driver's Newton / line-search state on the device (ls_decide_kernel): rows
+    // of the current iteration (lcur) and of the next, so that a step redone from a
+    // recomputed swept set repeats the same decisions
+    DevBuf<uint8_t> lsflag;     // [kLfRows][ns]
+    DevBuf<double> lsval;       // [kLvRows][ns]
+    DevBuf<int32_t> lsit;       // [2][ns] Newton iterations of each scene in this frame
+    int lcur = 0;
+    int32_t* hiters = nullptr;  // pinned [ns]: the iteration counts read back at a frame's end
     std::vector<int32_t> scene_it;  // Newton iterations of each scene in the last frame
     // line-search candidates evaluated together: states, vertices and per-candidate scalars
     DevBuf<double> cand_E;          // [kLsSpec][2 ns] trial energies
@@ -135,7 +139,7 @@ constexpr int kLsSpec = 12;
 // alpha_min, which the loop accepts whatever its energy (at most kLsSpec; a pass that
 // ends above alpha_min is followed by another).  With alpha <= 1 and alpha_min = 1e-3
 // one pass after the first always decides.
-int trials_to_floor(double a, double alpha_min) {
+__host__ __device__ inline int trials_to_floor(double a, double alpha_min) {
     int k = 1;
     while (k < kLsSpec && !(a < alpha_min)) {
         a *= 0.5;
@@ -144,12 +148,30 @@ int trials_to_floor(double a, double alpha_min) {
     return k;
 }
 
-// per-scene value rows of Sim::sv / Sim::hsv
+// per-scene value rows of Sim::sv
 enum { kSvEc = 0, kSvEi, kSvToi, kSvQtg, kSvGtg, kSvAlpha, kSv };
 // Sim::cnt: the collision set's sizes, the swept set's, the active contacts of the gh
-// stage, the flag of a step leaving the superset's ranges, and a staging row for the
-// exact sizes of a culled collision set
-enum { kCntCol = 0, kCntSw = 8, kCntNact = 16, kCntRange = 17, kCntLen = 24 };
+// stage, the flag of a step leaving the superset's ranges, the batched line search's
+// outputs (scenes iterating after this iteration, scenes still searching, trials of the
+// second pass and of the next, the solver's iterations summed over the iterating scenes);
+// the host's copy holds a staging row for the exact sizes of a culled collision set after it
+enum {
+    kCntCol = 0,
+    kCntSw = 8,
+    kCntNact = 16,
+    kCntRange = 17,
+    kCntNewton = 18,
+    kCntSearch = 19,
+    kCntK1 = 20,
+    kCntK2 = 21,
+    kCntSolver = 22,
+    kCntLen = 24
+};
+// rows of Sim::lsflag (iterating: the current and the next iteration's; searching) and of
+// Sim::lsval (E0: the current and the next; the accepted trial's energy; a searching
+// scene's next step length)
+enum { kLfNewton = 0, kLfSearch = 2, kLfRows };
+enum { kLvE0 = 0, kLvE1 = 2, kLvNext, kLvRows };
 
 int get_scalars(hipStream_t s, Sim& S, int first, int count) {
     ABD_CHECK(hipMemcpyAsync(S.host + first, S.scal.p + first, sizeof(double) * count, hipMemcpyDeviceToHost, s));
@@ -563,14 +585,161 @@ int step(hipStream_t s, Sim& S) {
 // Many independent scenes advanced together: one launch per stage covers every
 // scene, and each scene keeps its own Newton loop, step length and stopping
 // test, exactly the per-scene sequence of step() above.  A scene that has
-// converged is masked out of the solve and takes zero-length steps.
+// converged is masked out of the solve and takes zero-length steps.  The line
+// searches and the stopping tests run on the device (ls_decide_kernel), so an
+// iteration waits for the device once.
 
-int upload(hipStream_t s, void* dst, const void* src, size_t bytes) {
-    ABD_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
-    return ABD_OK;
+// The device rows of the batched Newton / line-search state for the current iteration.
+struct LsDev {
+    int ns;
+    const uint8_t* newton;  // scenes iterating (the solver's mask of this iteration)
+    uint8_t* newton_nxt;
+    uint8_t* search;        // scenes whose line search has not accepted a trial
+    const double* E0;       // energy at the iteration's start (abd.py:225)
+    double* E0_nxt;
+    double* E1;             // the accepted trial's energy
+    double* next;           // a searching scene's next step length
+    double* dalpha;         // the accepted step length (0: a scene not iterating)
+    const int32_t* it;      // Newton iterations of the frame so far
+    int32_t* it_nxt;
+    const double* qtg;      // q.g and g.g of the iteration (abd.py:148, 265)
+    const double* gtg;
+    const int32_t* iters;   // the solver's iterations
+    int64_t* cnt;           // Sim::cnt
+};
+
+LsDev ls_dev(Sim& S) {
+    const int ns = S.ns, c = S.lcur, n = 1 - S.lcur;
+    LsDev L;
+    L.ns = ns;
+    L.newton = S.lsflag.p + (size_t)(kLfNewton + c) * ns;
+    L.newton_nxt = S.lsflag.p + (size_t)(kLfNewton + n) * ns;
+    L.search = S.lsflag.p + (size_t)kLfSearch * ns;
+    L.E0 = S.lsval.p + (size_t)(kLvE0 + c) * ns;
+    L.E0_nxt = S.lsval.p + (size_t)(kLvE0 + n) * ns;
+    L.E1 = S.lsval.p + (size_t)kLvE1 * ns;
+    L.next = S.lsval.p + (size_t)kLvNext * ns;
+    L.dalpha = S.sv.p + (size_t)kSvAlpha * ns;
+    L.it = S.lsit.p + (size_t)c * ns;
+    L.it_nxt = S.lsit.p + (size_t)n * ns;
+    L.qtg = S.sv.p + (size_t)kSvQtg * ns;
+    L.gtg = S.sv.p + (size_t)kSvGtg * ns;
+    L.iters = S.iters.p;
+    L.cnt = S.cnt.p;
+    return L;
 }
 
-// per-scene energies at q = q_base - alpha_s dq (abd.py:133-141); hsv rows Ec, Ei
+constexpr int kLsThreads = 256;
+
+__device__ __forceinline__ int64_t ls_block_reduce(int64_t v, bool take_max, int64_t* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int w = kLsThreads / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] = take_max ? (sh[t] > sh[t + w] ? sh[t] : sh[t + w]) : sh[t] + sh[t + w];
+        __syncthreads();
+    }
+    const int64_t r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// One pass of every scene's line search (abd.py:146-160; the Armijo test as written, a
+// trial below alpha_min accepted whatever its energy), then the next pass's trials:
+// alpha [K][ns] holds this pass's step lengths (negative: not evaluated) and E [K][2 ns]
+// their contact and inertia energies (line_trials); a scene takes the first trial that
+// passes.  first: the pass of the first trial, over the iterating scenes (the others get
+// step length 0).  Then alpha [kLsSpec][ns] receives the next pass's trials of the scenes
+// still searching: a, a/2, ... from half the last rejected trial down to the first below
+// alpha_min (trials_to_floor), -1 elsewhere.  When no scene is searching any more, the
+// stopping test of every iterating scene (abd.py:265-268: g.g < tol or the iteration cap)
+// writes the next iteration's rows.  One workgroup; the host's step() makes the same
+// decisions with the same arithmetic.
+__global__ __launch_bounds__(kLsThreads) void ls_decide_kernel(LsDev L, double c1, double alpha_min, double tol,
+                                                                int max_iter, int first, int K,
+                                                                double* __restrict__ alpha,
+                                                                const double* __restrict__ E) {
+    __shared__ int64_t sh[kLsThreads];
+    const int ns = L.ns;
+    int64_t nsearch = 0, kmax = 0;
+    for (int s = threadIdx.x; s < ns; s += kLsThreads) {
+        bool searching = first ? L.newton[s] != 0 : L.search[s] != 0;
+        if (first && !searching) L.dalpha[s] = 0.0;
+        if (searching) {
+            bool acc = false;
+            double last = 0.0;
+            for (int k = 0; k < K && !acc; ++k) {
+                const double a = alpha[(int64_t)k * ns + s];
+                if (a < 0.0) break;  // past this scene's trials
+                last = a;
+                const double e1 = E[2 * (int64_t)ns * k + s] + E[2 * (int64_t)ns * k + ns + s];
+                const bool wolfe = e1 < L.E0[s] + c1 * a * L.qtg[s];
+                if (wolfe || a < alpha_min) {
+                    acc = true;
+                    L.dalpha[s] = a;
+                    L.E1[s] = e1;
+                }
+            }
+            if (acc)
+                searching = false;
+            else
+                L.next[s] = last * 0.5;
+        }
+        L.search[s] = searching ? 1 : 0;
+        const int need = searching ? trials_to_floor(L.next[s], alpha_min) : 0;
+        for (int k = 0; k < kLsSpec; ++k)
+            alpha[(int64_t)k * ns + s] =
+                k >= need ? -1.0 : (k == 0 ? L.next[s] : alpha[(int64_t)(k - 1) * ns + s] * 0.5);
+        nsearch += searching ? 1 : 0;
+        kmax = need > kmax ? need : kmax;
+    }
+    nsearch = ls_block_reduce(nsearch, false, sh);
+    kmax = ls_block_reduce(kmax, true, sh);
+    int64_t nact = 0, solver = 0;
+    if (nsearch == 0) {  // uniform: every line search has ended
+        for (int s = threadIdx.x; s < ns; s += kLsThreads) {
+            uint8_t nn = 0;
+            double e0 = L.E0[s];
+            int32_t itn = L.it[s];
+            if (L.newton[s]) {
+                e0 = L.E1[s];
+                itn += 1;
+                solver += L.iters[s] > 0 ? L.iters[s] : 0;
+                nn = (L.gtg[s] < tol || itn >= max_iter) ? 0 : 1;
+            }
+            L.E0_nxt[s] = e0;
+            L.it_nxt[s] = itn;
+            L.newton_nxt[s] = nn;
+            nact += nn;
+        }
+        nact = ls_block_reduce(nact, false, sh);
+        solver = ls_block_reduce(solver, false, sh);
+    }
+    if (threadIdx.x == 0) {
+        L.cnt[kCntSearch] = nsearch;
+        L.cnt[first ? kCntK1 : kCntK2] = kmax;
+        if (nsearch == 0) {
+            L.cnt[kCntNewton] = nact;
+            L.cnt[kCntSolver] = solver;
+        }
+    }
+}
+
+// A frame's start: E0 = the energy at q0 (abd.py:224-225), every scene iterating, no
+// iterations yet.
+__global__ __launch_bounds__(kLsThreads) void ls_frame_start_kernel(int ns, const double* __restrict__ Ec,
+                                                                     const double* __restrict__ Ei,
+                                                                     double* __restrict__ E0,
+                                                                     uint8_t* __restrict__ newton,
+                                                                     int32_t* __restrict__ it) {
+    for (int s = blockIdx.x * kLsThreads + threadIdx.x; s < ns; s += gridDim.x * kLsThreads) {
+        E0[s] = Ec[s] + Ei[s];
+        newton[s] = 1;
+        it[s] = 0;
+    }
+}
+
+// per-scene energies at q = q_base - alpha_s dq (abd.py:133-141): sv rows Ec, Ei
 int compute_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
     int rc;
     const AbdSimState& st = S.st;
@@ -588,30 +757,28 @@ int compute_energy_scenes(hipStream_t s, Sim& S, const double* dalpha) {
     if ((rc = abd_scene_inertia_energy(s, &S.sc, &S.P, st.Ak, st.pk, st.A0, st.p0, st.Adot, st.pdot,
                                        S.sv.p + kSvEi * ns)))
         return rc;
-    ABD_CHECK(hipMemcpyAsync(S.hsv + kSvEc * ns, S.sv.p + kSvEc * ns, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost,
-                             s));
-    ABD_CHECK(hipStreamSynchronize(s));
     S.stats[5] += 1;
     return ABD_OK;
 }
 
-// line_energy_multi per scene, for K trial step lengths each (dalpha [K][ns] on the device):
-// energies to hcand[kLsSpec * ns + (2 k) ns + s] (contact) and [... + (2 k + 1) ns + s] (inertia)
-int line_energy_scenes_launch(hipStream_t s, Sim& S, const double* dalpha, int K) {
+// One line-search pass over every scene: the energies at q - alpha dq of the trials alpha
+// [K][ns] (device) over the swept candidate set of this iteration, into cand_E [K][2 ns]
+int line_pass(hipStream_t s, Sim& S, int K) {
     const AbdSimState& st = S.st;
-    const int ns = S.ns;
-    int rc = line_trials(s, &S.sc, &S.P, ns, K, dalpha, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
-                         S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p,
-                         S.sw.counts());
-    if (rc) return rc;
-    ABD_CHECK(hipMemcpyAsync(S.hcand + (size_t)kLsSpec * ns, S.cand_E.p, sizeof(double) * 2 * ns * K,
-                             hipMemcpyDeviceToHost, s));
-    S.stats[5] += K;
+    return line_trials(s, &S.sc, &S.P, S.ns, K, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
+                       S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p,
+                       S.sw.counts());
+}
+
+int ls_decide(hipStream_t s, Sim& S, const LsDev& L, bool first, int K) {
+    hipLaunchKernelGGL(ls_decide_kernel, dim3(1), dim3(kLsThreads), 0, s, L, S.P.c1, S.P.alpha_min, S.P.tol,
+                       (int)S.P.max_iter, first ? 1 : 0, K, S.cand_alpha.p, (const double*)S.cand_E.p);
+    ABD_CHECK_LAUNCH();
     return ABD_OK;
 }
-int line_energy_scenes_multi(hipStream_t s, Sim& S, const double* dalpha, int K) {
-    int rc = line_energy_scenes_launch(s, S, dalpha, K);
-    if (rc) return rc;
+
+int read_counts(hipStream_t s, Sim& S) {
+    ABD_CHECK(hipMemcpyAsync(S.hcnt, S.cnt.p, kCntLen * sizeof(int64_t), hipMemcpyDeviceToHost, s));
     ABD_CHECK(hipStreamSynchronize(s));
     return ABD_OK;
 }
@@ -626,17 +793,20 @@ int step_scenes(hipStream_t s, Sim& S) {
     ABD_CHECK(hipMemcpyAsync(st.p, st.p0, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
     ABD_CHECK(hipMemcpyAsync(st.Ak, st.A0, 72 * (size_t)nb, hipMemcpyDeviceToDevice, s));
     ABD_CHECK(hipMemcpyAsync(st.pk, st.p0, 24 * (size_t)nb, hipMemcpyDeviceToDevice, s));
-    std::vector<double> E0(ns), E1(ns);
-    std::vector<uint8_t> newton(ns, 1), ls(ns);
-    std::vector<int32_t>& it = S.scene_it;
-    it.assign(ns, 0);
-    for (int k = 0; k < ns; ++k) S.halpha[k] = 0.0;
-    if ((rc = upload(s, dalpha, S.halpha, sizeof(double) * ns))) return rc;
+    ABD_CHECK(hipMemsetAsync(dalpha, 0, sizeof(double) * ns, s));
     S.sw_valid = false;  // a frame starts from the state the caller holds
+    S.lcur = 0;
     if ((rc = compute_energy_scenes(s, S, dalpha))) return rc;
-    for (int k = 0; k < ns; ++k) E0[k] = S.hsv[kSvEc * ns + k] + S.hsv[kSvEi * ns + k];
-    int n_active = ns;
+    {
+        const LsDev L = ls_dev(S);
+        hipLaunchKernelGGL(ls_frame_start_kernel, dim3((unsigned)ceil_div(ns, kLsThreads)), dim3(kLsThreads), 0, s,
+                           ns, (const double*)(S.sv.p + kSvEc * ns), (const double*)(S.sv.p + kSvEi * ns),
+                           const_cast<double*>(L.E0), const_cast<uint8_t*>(L.newton), const_cast<int32_t*>(L.it));
+        ABD_CHECK_LAUNCH();
+    }
+    int64_t n_active = ns;
     while (n_active > 0) {
+        const LsDev L = ls_dev(S);
         // (Ak, pk) = (A, p) here: the collision set is culled on x
         if ((rc = abd_project_vertices(s, &S.sc, st.A, st.p, st.x))) return rc;
         if (S.col_fresh)
@@ -650,119 +820,62 @@ int step_scenes(hipStream_t s, Sim& S) {
         if ((rc = abd_inertia_gradient(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g))) return rc;
         if ((rc = abd_inertia_hessian(s, &S.sc, &P, st.A, S.blocks.p))) return rc;
         const CandSet& C = S.col;
-        const GhLists L{C.ptl.p, C.npt, C.eel.p, C.nee, C.vgl.p, C.nvg, C.nij, C.counts()};
-        if ((rc = contact_gh(s, &S.sc, &P, st.x, L, S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, nullptr,
+        const GhLists GL{C.ptl.p, C.npt, C.eel.p, C.nee, C.vgl.p, C.nvg, C.nij, C.counts()};
+        if ((rc = contact_gh(s, &S.sc, &P, st.x, GL, S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, nullptr,
                              S.cnt.p + kCntNact)))
             return rc;
-        for (int k = 0; k < ns; ++k) S.hmask[k] = newton[k];
-        if ((rc = upload(s, S.sactive.p, S.hmask, ns))) return rc;
         if ((rc = abd_bsr_solve_scenes(s, P.solver, &S.sc, S.max_scene_bodies, S.row_ptr.p, S.cols.p, S.slots.p,
-                                       S.blocks.p, st.g, st.dq, P.solver_tol, P.solver_maxiter, 10, S.sactive.p,
+                                       S.blocks.p, st.g, st.dq, P.solver_tol, P.solver_maxiter, 10, L.newton,
                                        S.iters.p, S.scal.p + 8)))
             return rc;
         // CCD over the full steps of the scenes still iterating (abd.py:278-303)
-        if ((rc = step_end(s, S, S.sactive.p))) return rc;
+        if ((rc = step_end(s, S, L.newton))) return rc;
         for (int attempt = 0;; ++attempt) {
             if ((rc = attempt == 0 ? swept_set_launch(s, S) : swept_set_recull(s, S))) return rc;
             const CandSet& W = S.sw;
             if ((rc = reduce_plan(s, &S.sc, &P, ns, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg, S.plan.p,
                                   W.counts())))
                 return rc;
-            // the TOI of the scenes still iterating (sactive: the solver's mask of this iteration)
-            // and from it the first trial's step lengths, on the device
+            // the TOI of the scenes still iterating and from it the first trial's step lengths
             if ((rc = scene_toi(s, &S.sc, &P, ns, st.x, st.xk, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg,
-                                S.plan.p, S.sv.p + kSvToi * ns, S.sactive.p, W.counts(), S.cand_alpha.p)))
+                                S.plan.p, S.sv.p + kSvToi * ns, L.newton, W.counts(), S.cand_alpha.p)))
                 return rc;
-            // q.g and g.g of the stopping test (abd.py:148,265): g and dq are final here
+            // q.g and g.g of the Armijo and stopping tests (abd.py:148,265): g and dq are final here
             if ((rc = dots(s, &S.sc, ns, st.dq, st.g, st.g, st.g, S.sv.p + kSvQtg * ns, S.sv.p + kSvGtg * ns)))
                 return rc;
-            // the line search's first trial straight behind the TOI, read back with the TOI,
-            // q.g, g.g, the solver's iteration counts and the sets' sizes: the iteration's one
-            // wait for the device
-            if ((rc = line_energy_scenes_launch(s, S, S.cand_alpha.p, 1))) return rc;
-            static_assert(kSvQtg == kSvToi + 1 && kSvGtg == kSvToi + 2, "rows read back together");
-            ABD_CHECK(hipMemcpyAsync(S.hsv + kSvToi * ns, S.sv.p + kSvToi * ns, sizeof(double) * 3 * ns,
-                                     hipMemcpyDeviceToHost, s));
-            ABD_CHECK(hipMemcpyAsync(S.hiters, S.iters.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
-            ABD_CHECK(hipMemcpyAsync(S.hcnt, S.cnt.p, kCntLen * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-            ABD_CHECK(hipStreamSynchronize(s));
+            // the line search: the first trial, then every remaining halving of the scenes
+            // that rejected it, each decided on the device; then the iteration's one wait
+            if ((rc = line_pass(s, S, 1))) return rc;
+            if ((rc = ls_decide(s, S, L, true, 1))) return rc;
+            if ((rc = line_pass(s, S, kLsSpec))) return rc;
+            if ((rc = ls_decide(s, S, L, false, kLsSpec))) return rc;
+            if ((rc = read_counts(s, S))) return rc;
             // a recomputed swept set lies inside the superset culled around this very step
             if (attempt > 0 || !swept_left_ranges(S)) break;
-            S.stats[5] -= 1;  // the trial is evaluated again on the recomputed swept set
         }
         count_stats(S);
-        for (int k = 0; k < ns; ++k) {
-            ls[k] = newton[k];
-            S.halpha[k] = newton[k] ? first_step_length(S.hsv[kSvToi * ns + k]) : 0.0;
+        S.stats[5] += 1 + S.hcnt[kCntK1];
+        // further passes: only when alpha_min lies more than kLsSpec halvings below a first trial
+        while (S.hcnt[kCntSearch] > 0) {
+            const int64_t K = S.hcnt[kCntK2];
+            if ((rc = line_pass(s, S, kLsSpec))) return rc;
+            if ((rc = ls_decide(s, S, L, false, kLsSpec))) return rc;
+            if ((rc = read_counts(s, S))) return rc;
+            S.stats[5] += K;
         }
-        // line search, each scene on its own (abd.py:146-160): the first trial, then the
-        // trials (alpha, alpha/2, ...) down to alpha_min for the scenes still searching
-        for (int n_ls = n_active, pass = 0; n_ls > 0; ++pass) {
-            int K = 1;
-            for (int q = 0; pass > 0 && q < ns; ++q)
-                if (ls[q]) K = std::max(K, trials_to_floor(S.halpha[q], P.alpha_min));
-            double* ha = S.hcand;  // [K][ns] trial step lengths (-1: not evaluated)
-            for (int q = 0; q < ns; ++q) {
-                const int need = !ls[q] ? 0 : pass == 0 ? 1 : trials_to_floor(S.halpha[q], P.alpha_min);
-                for (int k = 0; k < K; ++k)
-                    ha[(size_t)k * ns + q] =
-                        k >= need ? -1.0 : (k == 0 ? S.halpha[q] : ha[(size_t)(k - 1) * ns + q] * 0.5);
-            }
-            // pass 0's energies came back with the TOI (the same step lengths, from the device)
-            if (pass > 0) {
-                if ((rc = upload(s, S.cand_alpha.p, ha, sizeof(double) * ns * K))) return rc;
-                if ((rc = line_energy_scenes_multi(s, S, S.cand_alpha.p, K))) return rc;
-            }
-            const double* he = S.hcand + (size_t)kLsSpec * ns;
-            for (int q = 0; q < ns; ++q) {
-                if (!ls[q]) continue;
-                const double qTg = S.hsv[kSvQtg * ns + q];
-                int acc = -1;
-                double last = 0.0;
-                for (int k = 0; k < K && acc < 0; ++k) {
-                    const double a = ha[(size_t)k * ns + q];
-                    if (a < 0.0) break;  // past this scene's trials
-                    last = a;
-                    const double e1 = he[(size_t)2 * ns * k + q] + he[(size_t)2 * ns * k + ns + q];
-                    const bool wolfe = e1 < E0[q] + P.c1 * a * qTg;
-                    if (wolfe || a < P.alpha_min) {
-                        acc = k;
-                        S.halpha[q] = a;
-                        E1[q] = e1;
-                    }
-                }
-                if (acc >= 0) {
-                    ls[q] = 0;
-                    --n_ls;
-                } else {
-                    S.halpha[q] = last * 0.5;
-                }
-            }
-        }
-        // halpha holds the accepted step of every iterating scene and 0 elsewhere
-        for (int q = 0; q < ns; ++q)
-            if (!newton[q]) S.halpha[q] = 0.0;
-        if ((rc = upload(s, dalpha, S.halpha, sizeof(double) * ns))) return rc;
-        // q and (Ak, pk) = the accepted trial of every scene
+        S.stats[0] += n_active;
+        S.stats[6] += S.hcnt[kCntSolver];
+        n_active = S.hcnt[kCntNewton];
+        S.lcur = 1 - S.lcur;
+        // q and (Ak, pk) = the accepted trial of every scene (step length 0 where a scene has stopped)
         if ((rc = update_q_scenes_dual(s, &S.sc, st.A, st.p, st.dq, dalpha, st.A, st.p, st.Ak, st.pk))) return rc;
-        // the stopping test on the values read back with the TOI (no round trip here: the next
-        // iteration's first host write to a buffer uploaded above follows a synchronization)
-        for (int k = 0; k < ns; ++k) {
-            if (!newton[k]) continue;
-            E0[k] = E1[k];
-            ++it[k];
-            S.stats[0] += 1;
-            S.stats[6] += S.hiters[k] > 0 ? S.hiters[k] : 0;
-            if (S.hsv[kSvGtg * ns + k] < P.tol || it[k] >= P.max_iter) {
-                newton[k] = 0;
-                --n_active;
-            }
-        }
     }
     if ((rc = abd_update_q0qdot(s, nb, P.dt, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot))) return rc;
     if ((rc = abd_project_vertices(s, &S.sc, st.A0, st.p0, st.x_view))) return rc;
-    // the last accepted steps' upload reads halpha, which the next frame's start rewrites
+    ABD_CHECK(hipMemcpyAsync(S.hiters, S.lsit.p + (size_t)S.lcur * ns, sizeof(int32_t) * ns, hipMemcpyDeviceToHost,
+                             s));
     ABD_CHECK(hipStreamSynchronize(s));
+    S.scene_it.assign(S.hiters, S.hiters + ns);
     S.stats[7] += 1;
     return ABD_OK;
 }
@@ -845,13 +958,12 @@ int abd_sim_create(const AbdScene* scene, const AbdParams* params, const AbdSimS
         if (!rc && (sb[0] != 0 || sb[ns] != scene->n_bodies)) rc = ABD_ERR_ARG;
         S->ns = ns;
         if (!rc) rc = S->sv.reserve((size_t)kSv * ns);
-        if (!rc) rc = S->sactive.reserve(ns);
+        if (!rc) rc = S->lsflag.reserve((size_t)kLfRows * ns);
+        if (!rc) rc = S->lsval.reserve((size_t)kLvRows * ns);
+        if (!rc) rc = S->lsit.reserve(2 * (size_t)ns);
         if (!rc) rc = S->iters.reserve(ns);
         if (!rc) rc = S->scal.reserve(8 + (size_t)ns);
-        if (!rc && (hipHostMalloc((void**)&S->hsv, sizeof(double) * kSv * ns, hipHostMallocDefault) != hipSuccess ||
-                    hipHostMalloc((void**)&S->halpha, sizeof(double) * ns, hipHostMallocDefault) != hipSuccess ||
-                    hipHostMalloc((void**)&S->hmask, ns, hipHostMallocDefault) != hipSuccess ||
-                    hipHostMalloc((void**)&S->hiters, sizeof(int32_t) * ns, hipHostMallocDefault) != hipSuccess))
+        if (!rc && hipHostMalloc((void**)&S->hiters, sizeof(int32_t) * ns, hipHostMallocDefault) != hipSuccess)
             rc = ABD_ERR_ARG;
     }
     if (rc) {
@@ -981,15 +1093,16 @@ int abd_sim_destroy(void* sim) {
     S->scal.release();
     S->iters.release();
     S->sv.release();
-    S->sactive.release();
+    S->lsflag.release();
+    S->lsval.release();
+    S->lsit.release();
     S->cand_E.release();
     S->cand_alpha.release();
     S->plan.release();
     S->dcount.release();
     S->cnt.release();
     if (S->hcand) (void)hipHostFree(S->hcand);
-    for (void* h : {(void*)S->host, (void*)S->hcount, (void*)S->hcnt, (void*)S->hsv, (void*)S->halpha, (void*)S->hmask,
-                    (void*)S->hiters})
+    for (void* h : {(void*)S->host, (void*)S->hcount, (void*)S->hcnt, (void*)S->hiters})
         if (h) (void)hipHostFree(h);
     delete S;
     return ABD_OK;

@@ -14,9 +14,11 @@
  *    small (the required count is written to *count and nothing is written past
  *    `cap`); negative on error (ABD_ERR_* below, or -hipError_t).
  *  - Scratch memory (partial sums, compaction counters, filtered primitive
- *    lists) comes from a per-device arena that the library grows on demand and
- *    keeps until abd_release().  Calls on one device must not overlap in time
- *    (they share the arena); the library is otherwise stateless.
+ *    lists) comes from an arena that the library grows on demand.  The
+ *    stateless entry points share one arena per device, kept until
+ *    abd_release(): those calls must not overlap in time on one device.  Each
+ *    simulator (abd_sim_*) owns an arena of its own, freed by abd_sim_destroy,
+ *    so simulators may be stepped concurrently from several host threads.
  *  - No C++ exception crosses this boundary.
  *
  * Data layouts (shared with the Python mirror and the CPU oracle)
