@@ -102,6 +102,8 @@ def load():
         "abd_sim_create": [S, P, C.c_void_p, C.POINTER(C.c_void_p)],
         "abd_sim_step": [p, p, C.POINTER(C.c_int64)],
         "abd_sim_scene_iters": [p, C.POINTER(C.c_int32)],
+        "abd_sim_set_graphs": [p, i32],
+        "abd_sim_graph_counts": [p, C.POINTER(C.c_int64)],
         "abd_sim_autotune": [p, p, C.POINTER(AbdTuning)],
         "abd_get_tuning": [C.POINTER(AbdTuning)],
         "abd_set_tuning": [C.POINTER(AbdTuning)],

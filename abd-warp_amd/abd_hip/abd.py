@@ -111,6 +111,21 @@ class AffineBodySimulator:
         _lib.call("abd_sim_autotune", self._sim, _lib.stream_handle(), ctypes.byref(t))
         return {"cull_chunk": t.cull_chunk, "cull_grid": t.cull_grid}
 
+    def set_graphs(self, on: bool):
+        """Graph replay of the batched native driver's Newton iterations (abd_sim_set_graphs;
+        on by default, results unchanged)."""
+        if self._sim is None:
+            raise RuntimeError("set_graphs() needs the native driver")
+        _lib.call("abd_sim_set_graphs", self._sim, 1 if on else 0)
+
+    def graph_counts(self) -> dict:
+        """Batched Newton iterations replayed from a graph, recorded and launched eagerly."""
+        if self._sim is None:
+            raise RuntimeError("graph_counts() needs the native driver")
+        out = (ctypes.c_int64 * 3)()
+        _lib.call("abd_sim_graph_counts", self._sim, out)
+        return {"replayed": out[0], "recorded": out[1], "eager": out[2]}
+
     def scene_iters(self) -> list:
         """Newton iterations of each scene in the last native frame."""
         import ctypes

@@ -2,6 +2,7 @@
 // generic deterministic scan / reduction kernels the stages share.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <mutex>
 
 #include "abd_block.h"
@@ -14,6 +15,8 @@ constexpr int kMaxDevices = 64;
 Arena g_arena[kMaxDevices];
 std::mutex g_arena_mutex;
 thread_local Arena* t_arena = nullptr;  // a simulator's own arena during its calls
+thread_local bool t_capturing = false;    // this thread is recording a HIP graph
+std::atomic<uint64_t> g_alloc_gen{1};
 
 int current_device(int* dev) {
     hipError_t e = hipGetDevice(dev);
@@ -26,6 +29,11 @@ int current_device(int* dev) {
 ArenaScope::ArenaScope(Arena* a) : prev_(t_arena) { t_arena = a; }
 ArenaScope::~ArenaScope() { t_arena = prev_; }
 
+uint64_t alloc_generation() { return g_alloc_gen.load(std::memory_order_relaxed); }
+void note_reallocation() { g_alloc_gen.fetch_add(1, std::memory_order_relaxed); }
+void set_capturing(bool on) { t_capturing = on; }
+bool capturing() { return t_capturing; }
+
 int arena_get(int slot, size_t bytes, void** ptr) {
     int dev;
     int rc = current_device(&dev);
@@ -35,6 +43,8 @@ int arena_get(int slot, size_t bytes, void** ptr) {
     Arena& A = t_arena ? *t_arena : g_arena[dev];
     if (bytes == 0) bytes = 256;
     if (A.size[slot] < bytes) {
+        if (t_capturing) return ABD_ERR_CAPACITY;  // no allocation inside a recorded graph
+        note_reallocation();
         size_t want = bytes + bytes / 2;
         want = (want + 255) & ~size_t(255);
         if (A.ptr[slot]) {
@@ -58,6 +68,8 @@ int arena_host(size_t bytes, void** ptr) {
     if (!t_arena) lock.lock();
     Arena& A = t_arena ? *t_arena : g_arena[dev];
     if (A.host_size < bytes) {
+        if (t_capturing) return ABD_ERR_CAPACITY;
+        note_reallocation();
         if (A.host) {
             ABD_CHECK(hipDeviceSynchronize());
             ABD_CHECK(hipHostFree(A.host));
@@ -89,6 +101,20 @@ int arena_free(Arena& A) {
     if (A.host) ABD_CHECK(hipHostFree(A.host));
     A.host = nullptr;
     A.host_size = 0;
+    return ABD_OK;
+}
+
+// ------------------------------------------------------------------ fill
+__global__ __launch_bounds__(kBlock) void zero_words_kernel(uint64_t* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = 0;
+}
+
+int zero_fill(hipStream_t s, void* p, size_t bytes) {
+    if (bytes % 8 || ((uintptr_t)p & 7)) return ABD_ERR_ARG;
+    const int64_t n = (int64_t)(bytes / 8);
+    if (n == 0) return ABD_OK;
+    hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(n, kBlock, 1 << 16)), dim3(kBlock), 0, s, (uint64_t*)p, n);
+    ABD_CHECK_LAUNCH();
     return ABD_OK;
 }
 

@@ -799,7 +799,7 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
             dev_nact = boff + nblk;
         }
     }
-    if (!dev_nact && n_active_dev) ABD_CHECK(hipMemsetAsync(n_active_dev, 0, sizeof(int64_t), s));
+    if (!dev_nact && n_active_dev && (rc = zero_fill(s, n_active_dev, sizeof(int64_t)))) return rc;
     if (dn) {
         nact = dev_nact ? n : 0;  // an upper bound: the stages below read the count on the device
     } else {

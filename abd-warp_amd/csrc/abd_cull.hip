@@ -927,7 +927,7 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
     }
     // an empty list's total is 0 (its scan segment is skipped)
     for (int k = 0; k < kKeepLists; ++k)
-        if (n[k] == 0) ABD_CHECK(hipMemsetAsync(dcount + k, 0, sizeof(int64_t), s));
+        if (n[k] == 0 && (rc = zero_fill(s, dcount + k, sizeof(int64_t)))) return rc;
     if (hcount) {
         ABD_CHECK(hipMemcpyAsync(hcount, dcount, kKeepLists * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         if (sync) ABD_CHECK(hipStreamSynchronize(s));

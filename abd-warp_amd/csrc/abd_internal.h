@@ -255,6 +255,17 @@ int arena_get(int slot, size_t bytes, void** ptr);
 int arena_host(size_t bytes, void** ptr);  // pinned host memory
 int arena_release_current();
 int arena_free(Arena& a);  // synchronizes the device, then releases every buffer of `a`
+// A counter that every (re)allocation of library scratch advances: device pointers baked into
+// a recorded HIP graph stay valid while it is unchanged.
+uint64_t alloc_generation();
+void note_reallocation();
+// While set on this thread (graph capture), scratch that would have to grow returns
+// ABD_ERR_CAPACITY instead of allocating.
+void set_capturing(bool on);
+bool capturing();
+// Zeroes `bytes` (a multiple of 8, 8-byte aligned) with a kernel: a fill that may be recorded in
+// a HIP graph (memset nodes are avoided there).
+int zero_fill(hipStream_t s, void* p, size_t bytes);
 
 // Exclusive scan of n int64 counts (device) in place into `out` (may alias),
 // writing the total to *total (device).  Fixed-order, single workgroup passes.

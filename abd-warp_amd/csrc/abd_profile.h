@@ -66,10 +66,14 @@ void prof_ext_done(int handle);
 template <typename F, typename... Args>
 inline void prof_launch(int handle, F kernel, const dim3& grid, const dim3& block, uint32_t shm, hipStream_t s,
                         Args... args) {
+    if (handle < 0) {  // an ordinary launch (also the form a recorded graph takes)
+        hipLaunchKernelGGL(kernel, grid, block, shm, s, args...);
+        return;
+    }
     hipEvent_t a = nullptr, b = nullptr;
-    if (handle >= 0) prof_ext_events(handle, &a, &b);
+    prof_ext_events(handle, &a, &b);
     hipExtLaunchKernelGGL(kernel, grid, block, shm, s, a, b, 0, args...);
-    if (handle >= 0) prof_ext_done(handle);
+    prof_ext_done(handle);
 }
 
 }  // namespace abd

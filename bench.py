@@ -28,8 +28,10 @@ the sampled scenes.
 Frames: W warmup frames, then (unless --no-profile) untimed attribution frames in
 which every instrumented kernel is timed with HIP events (the per-kernel table,
 the gh-stage rate and the choice of the dominant kernel), then the K timed
-frames, in which only the dominant kernel carries events (its roofline), so that
-per-launch event overhead stays out of `value`.
+frames with no kernel timing (`value`; the batched driver replays its recorded
+Newton iterations as HIP graphs there), then --roof-frames frames (default K) in
+which only the dominant kernel carries events (its roofline; with kernel timing on
+every kernel is launched on its own).
 
 "--workload stress" runs the 1024-body lattice (and "stress_batched" 8 scenes of 128
 bodies); their contact phase starts once the pile has formed, so run them with
@@ -300,12 +302,16 @@ def main():
                     help="no per-kernel HIP events (no attribution frames, roofline or kernel breakdown)")
     ap.add_argument("--attrib-frames", type=int, default=2,
                     help="untimed frames with every instrumented kernel timed, before the timed frames")
+    ap.add_argument("--roof-frames", type=int, default=None,
+                    help="frames after the timed ones with HIP events on the dominant kernel (default: --steps)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with one 256 MiB device copy (reference kernel for HBM counter scaling)")
     ap.add_argument("--python-driver", action="store_true", help="run the Python mirror of the driver")
     ap.add_argument("--no-ref-solver", dest="ref_solver", action="store_false",
                     help="batched: skip the line with the reference's Jacobi CG beside the chosen solver")
     a = ap.parse_args()
+    if a.roof_frames is None:
+        a.roof_frames = a.steps
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the contact path of one scene is "
               "not split across GPUs in this build", file=sys.stderr)
@@ -365,10 +371,9 @@ def main():
         start_state = gpu_state(sim, np.nonzero(spec.body_scene == 0)[0])
     torch.cuda.synchronize()
     s0 = dict(sim.stats)
-    if dom:
-        # the dominant kernel (its roofline): the only launches with events in the timed frames
-        _lib.profile_select(sorted(cands_dom))
-        _lib.profile_begin()
+    g0 = sim.graph_counts() if not a.python_driver else None
+    # the timed frames: no kernel timing, so the batched driver replays its recorded Newton
+    # iterations (HIP graphs) as it does in production
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for f in range(a.steps):
@@ -376,8 +381,22 @@ def main():
         log(f"[gpu] frame {first + f} newton {it} elapsed {time.perf_counter() - t0:.2f}s")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    d = {k: sim.stats[k] - s0[k] for k in sim.stats}
+    graphs = None
+    if g0 is not None:
+        g1 = sim.graph_counts()
+        graphs = {k: g1[k] - g0[k] for k in g1}
     prof = None
+    roof_first = first + a.steps
     if dom:
+        # roofline frames, after the timed ones: HIP events on the dominant kernel's launches
+        # only (with kernel timing on, every kernel is launched on its own, no graph replay)
+        _lib.profile_select(sorted(cands_dom))
+        _lib.profile_begin()
+        for f in range(a.roof_frames):
+            it = sim.step(roof_first + f)
+            log(f"[gpu] roofline frame {roof_first + f} newton {it}")
+        torch.cuda.synchronize()
         _lib.profile_end()
         prof = _lib.profile_query()
         _lib.profile_select(None)
@@ -389,7 +408,6 @@ def main():
         torch.cuda.synchronize()
         log("[gpu] calibration copy: 268435456 bytes read, 268435456 bytes written")
         del src, dst
-    d = {k: sim.stats[k] - s0[k] for k in sim.stats}
     wall = t1 - t0
     iters = d["newton_iters"]
     cands = d["gh_pt"] + d["gh_ee"]
@@ -397,7 +415,7 @@ def main():
     timed_kernels = None
     if dom:
         timed_kernels = {k: dict(launches=prof[k][1], avg_us=round(1e3 * prof[k][0] / prof[k][1], 3),
-                                 ms_per_frame=round(prof[k][0] / a.steps, 3))
+                                 ms_per_frame=round(prof[k][0] / a.roof_frames, 3))
                          for k in cands_dom if k in prof and prof[k][1] > 0}
         dom = max(timed_kernels, key=lambda k: timed_kernels[k]["ms_per_frame"])
         ms, n, byts = prof[dom]
@@ -458,7 +476,9 @@ def main():
         "config": {"workload": desc, "mode": "batched" if batched else "plain", "psd_projection": bool(a.psd),
                    "driver": "python" if a.python_driver else "native",
                    "solver": {0: "bicgstab", 1: "jacobi-cg", 2: "block-jacobi-cg", 3: "block-ldlt"}[a.solver],
-                   "frames_timed": f"{first}..{first + a.steps - 1}", "tiles": tiles},
+                   "frames_timed": f"{first}..{first + a.steps - 1}", "tiles": tiles,
+                   "frames_roofline": (f"{roof_first}..{roof_first + a.roof_frames - 1}" if dom else None),
+                   "graph_iterations": graphs},
         # PT + EE candidates through the four gh kernels per second of their event time
         # (attribution frames)
         "gh_pairs_per_s": gh_cands / gh_s if gh_s > 0 and gh_cands > 0 else None,

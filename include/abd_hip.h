@@ -342,6 +342,15 @@ int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen);
 /* Newton iterations of each scene in the last abd_sim_step (iters: host [n_scenes];
  * with one scene, stats[0] holds the same count summed over frames). */
 int abd_sim_scene_iters(void* sim, int32_t* iters);
+/* Graph replay of the batched driver's Newton iterations (no reference counterpart; on by
+ * default): an iteration whose launches are those of an earlier one (same candidate
+ * superset, scratch and culling tiles) replays a HIP graph recorded from it instead of
+ * launching each kernel; results are bit for bit those of the launches.  on = 0 launches
+ * every kernel.  Iterations run with kernel timing on (abd_profile_begin) launch eagerly. */
+int abd_sim_set_graphs(void* sim, int32_t on);
+/* counts (host [3]): batched Newton iterations replayed from a graph, recorded into one
+ * (and replayed), and launched kernel by kernel, since the simulator was created. */
+int abd_sim_graph_counts(void* sim, int64_t* counts);
 int abd_sim_destroy(void* sim);
 
 /* ---- measurement ---- */

@@ -162,3 +162,30 @@ def test_autotuned_tiles_change_no_result():
         L.abd_set_tuning(ctypes.byref(t0))
     for A, p, st in out[1:]:
         assert np.array_equal(A, out[0][0]) and np.array_equal(p, out[0][1]) and st == out[0][2]
+
+
+@pytest.mark.parametrize("solver", [3, 1])
+def test_graph_replay_equals_launches(solver):
+    """Newton iterations replayed from a recorded HIP graph give the states, iteration
+    counts and statistics of the same iterations launched kernel by kernel, frame after
+    frame (each frame's replays run between eager work and host waits), and the replay
+    path is taken."""
+    from abd_hip import scenes
+    from abd_hip.abd import AffineBodySimulator
+    from abd_hip.const_params import ContactParams
+    spec = scenes.config_batched(n_scenes=16, seed0=41, nb_range=(2, 24))
+    cp = ContactParams(ground=spec.ground, psd=1, solver=solver)
+    eager = AffineBodySimulator(spec, cp, "cuda", native=True)
+    eager.set_graphs(False)
+    graph = AffineBodySimulator(spec, cp, "cuda", native=True)
+    for f in range(14):
+        eager.step(f)
+        graph.step(f)
+        torch.cuda.synchronize()
+        assert eager.scene_iters() == graph.scene_iters(), f
+        assert np.array_equal(eager.states.A.cpu().numpy(), graph.states.A.cpu().numpy()), f
+        assert np.array_equal(eager.states.p.cpu().numpy(), graph.states.p.cpu().numpy()), f
+        assert dict(eager.stats) == dict(graph.stats), f
+    gc = graph.graph_counts()
+    assert gc["replayed"] >= 10 and gc["recorded"] >= 1, gc
+    assert eager.graph_counts()["replayed"] == 0
