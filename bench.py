@@ -19,11 +19,12 @@ clamped-PSD matrix) the reference's BiCGSTAB diverges on colliding scenes and
 the swept CCD boxes then cover the scene.  The benchmark therefore runs the
 documented stable mode: contact-space PSD projection of the barrier Hessian
 (psd=1), with each Newton system solved directly by a block LDL^T factorization
-(solver 3; systems above 64 bodies fall back to the reference's Jacobi CG).
+(solver 3; systems above 64 bodies fall back to the block-Jacobi CG).
 --solver 2 runs the reference's CG (bsr_utils.py:126-235, same iteration and
 stopping rule) preconditioned by each body's 12x12 diagonal block, --solver 1
 with its scalar diagonal.  The CPU baseline runs the oracle in the same mode on
-the sampled scenes.
+the sampled scenes (1 core) and, over every host core, on the whole batch's scenes
+dealt out in turn.
 
 Frames: W warmup frames, then (unless --no-profile) untimed attribution frames in
 which every instrumented kernel is timed with HIP events (the per-kernel table,
@@ -228,46 +229,53 @@ def host_cores() -> int:
         return os.cpu_count() or 1
 
 
-def _cpu_worker(sp, cp, warmup, frames, budget, barrier, q, k):
-    # one sampled scene on the oracle in its own process; the timed frames start together
-    # and stop at the budget (whole frames)
+def _cpu_worker(subs, cp, warmup, frames, budget, barrier, q, k):
+    # one host core: its share of the batch's scenes on the oracle, one scene after another,
+    # each advanced through the untimed warmup frames and then the timed frames; the timed
+    # frames' seconds are summed, and the worker stops after the scene that uses up the budget
     from oracle import oracle as O
-    ref = O.OracleSim(sp, cp)
-    for _ in range(warmup):
-        ref.step()
-    it0 = int(ref.stats[0])
     barrier.wait()
-    t = time.perf_counter()
-    f = 0
-    while f < frames and time.perf_counter() - t < budget:
-        ref.step()
-        f += 1
-    q.put((k, int(ref.stats[0]) - it0, time.perf_counter() - t, f))
+    iters, timed, fdone, nsc = 0, 0.0, 0, 0
+    for sp in subs:
+        ref = O.OracleSim(sp, cp)
+        for _ in range(warmup):
+            ref.step()
+        it0 = int(ref.stats[0])
+        t = time.perf_counter()
+        for _ in range(frames):
+            ref.step()
+        timed += time.perf_counter() - t
+        iters += int(ref.stats[0]) - it0
+        fdone += frames
+        nsc += 1
+        if timed >= budget:
+            break
+    q.put((k, iters, timed, fdone, nsc))
 
 
 def cpu_baseline_parallel(spec, cp, warmup: int, frames: int, budget_s: float, n_cores: int):
-    """All host cores on independent scenes of the batch: one oracle process per core,
-    each on its own scene (n_cores scenes spread evenly over the batch), started
-    together after their warmup frames and run for the same frames as the timed GPU
-    region, stopped at `budget_s` of wall time: Newton iterations per second of wall
-    time, summed over the processes."""
+    """All host cores on the batch: one oracle process per core, the batch's scenes dealt
+    out in turn (core k takes scenes k, k + n, ...), each scene run through the same frames
+    as the timed GPU region after its warmup frames; a core stops after the scene that
+    takes its timed seconds past `budget_s`.  Each core's rate (its Newton iterations per
+    second of its timed frames, over a mix of small and large scenes) is summed over the
+    cores: the throughput of all cores working through the batch."""
     import multiprocessing as mp
     from abd_hip import scenes
     ctx = mp.get_context("spawn")  # fresh interpreters: nothing of this process's GPU state
     ns = spec.n_scenes
-    sample = sorted(set(int(round(k * (ns - 1) / max(n_cores - 1, 1))) for k in range(n_cores)))
-    n = len(sample)
+    n = max(1, min(n_cores, ns))
     barrier, q = ctx.Barrier(n), ctx.Queue()
-    procs = [ctx.Process(target=_cpu_worker, args=(scenes.scene_of(spec, s), cp, warmup, frames, budget_s, barrier,
-                                                   q, k))
-             for k, s in enumerate(sample)]
+    procs = [ctx.Process(target=_cpu_worker, args=([scenes.scene_of(spec, s) for s in range(k, ns, n)], cp, warmup,
+                                                   frames, budget_s, barrier, q, k))
+             for k in range(n)]
     import queue
     for p in procs:
         p.start()
     res = []
     try:
         for _ in procs:
-            res.append(q.get(timeout=600))
+            res.append(q.get(timeout=900))
     except queue.Empty:
         for p in procs:
             p.terminate()
@@ -278,11 +286,12 @@ def cpu_baseline_parallel(spec, cp, warmup: int, frames: int, budget_s: float, n
     iters = sum(r[1] for r in res)
     rate = sum(r[1] / r[2] for r in res if r[2] > 0)
     fdone = sum(r[3] for r in res)
+    nsc = sum(r[4] for r in res)
     return dict(value=rate, unit="Newton iters/s", cores=n, kind="port", cpu_model=cpu_model(),
-                sample=f"oracle (C FP64 restatement, -O2), one process per core on {n} scenes spread over the batch "
-                       f"(scenes {sample[0]}..{sample[-1]}), frames {warmup}.. (up to {frames} each, "
-                       f"{budget_s:.0f} s budget): {fdone} frames, {iters} Newton iterations; the sum over "
-                       f"processes of iterations per second")
+                sample=f"oracle (C FP64 restatement, -O2), one process per core, the batch's scenes dealt out in "
+                       f"turn: {nsc} of {ns} scenes run, frames {warmup}..{warmup + frames - 1} each ({fdone} "
+                       f"frames, {iters} Newton iterations, about {budget_s:.0f} s of timed frames per core); the "
+                       f"sum over cores of iterations per second of timed frames")
 
 
 def main():
@@ -302,6 +311,8 @@ def main():
                     help="no per-kernel HIP events (no attribution frames, roofline or kernel breakdown)")
     ap.add_argument("--attrib-frames", type=int, default=2,
                     help="untimed frames with every instrumented kernel timed, before the timed frames")
+    ap.add_argument("--no-graphs", dest="graphs", action="store_false",
+                    help="batched native driver: launch every kernel (no HIP graph replay of Newton iterations)")
     ap.add_argument("--roof-frames", type=int, default=None,
                     help="frames after the timed ones with HIP events on the dominant kernel (default: --steps)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -327,6 +338,8 @@ def main():
     spec, desc = make_workload(a.workload)
     cp = ContactParams(ground=spec.ground, psd=a.psd, solver=a.solver)
     sim = AffineBodySimulator(spec, cp, "cuda", native=not a.python_driver)
+    if not a.python_driver and not a.graphs:
+        sim.set_graphs(False)
     for f in range(a.warmup):
         it = sim.step(f)
         log(f"[gpu] warmup frame {f} newton {it}")
@@ -359,7 +372,11 @@ def main():
         gh_cands = sim.stats["gh_pt"] + sim.stats["gh_ee"] - sa["gh_pt"] - sa["gh_ee"]
         # the dominant kernel: the largest with an algorithmic byte count
         ranked = sorted(((v[0], k) for k, v in aprof.items() if v[1] > 0 and v[2] > 0), reverse=True)
-        cands_dom = [k for _, k in ranked[:1]]
+        # the two largest: the first is the dominant kernel; its runner-up gets a roofline too
+        # (their order can change from one run to the next), and so does the solve
+        cands_dom = [k for _, k in ranked[:2]]
+        if "solve" in aprof and "solve" not in cands_dom and aprof["solve"][2] > 0:
+            cands_dom.append("solve")
         dom = cands_dom[0] if cands_dom else None
     first = frame
     torch.cuda.synchronize()
@@ -412,17 +429,22 @@ def main():
     iters = d["newton_iters"]
     cands = d["gh_pt"] + d["gh_ee"]
     roof = None
+    rooflines = None
     timed_kernels = None
     if dom:
         timed_kernels = {k: dict(launches=prof[k][1], avg_us=round(1e3 * prof[k][0] / prof[k][1], 3),
                                  ms_per_frame=round(prof[k][0] / a.roof_frames, 3))
                          for k in cands_dom if k in prof and prof[k][1] > 0}
         dom = max(timed_kernels, key=lambda k: timed_kernels[k]["ms_per_frame"])
-        ms, n, byts = prof[dom]
-        achieved = byts / (ms * 1e-3) / 1e9
-        roof = dict(kernel=dom, bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=measured_traffic(dom, a.workload, a.solver), launches=n,
-                    avg_us=round(1e3 * ms / n, 3), bytes_per_launch=byts / n)
+
+        def roofline_of(k):
+            ms, n, byts = prof[k]
+            achieved = byts / (ms * 1e-3) / 1e9
+            return dict(kernel=k, bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=achieved / HBM_PEAK_GBS, traffic=measured_traffic(k, a.workload, a.solver), launches=n,
+                        avg_us=round(1e3 * ms / n, 3), bytes_per_launch=byts / n)
+        roof = roofline_of(dom)
+        rooflines = {k: roofline_of(k) for k in timed_kernels}
     batched = spec.n_scenes > 1
     plain = None
     sample = None
@@ -489,6 +511,7 @@ def main():
         "plain_per_scene": plain,
         "batched_over_plain": (iters / wall) / plain["value"] if plain and plain["value"] else None,
         "roofline": roof,
+        "rooflines": rooflines,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_par,
         "host_cores": host_cores(),
