@@ -546,7 +546,7 @@ int step(hipStream_t s, Sim& S) {
                              S.cnt.p + kCntNact)))
             return rc;
         if ((rc = abd_bsr_solve(s, P.solver, nb, S.row_ptr.p, S.cols.p, S.slots.p, S.blocks.p, st.g, st.dq,
-                                P.solver_tol, P.solver_maxiter, 10, S.iters.p, S.scal.p + 4)))
+                                P.solver_tol, P.solver_maxiter, 10, S.iters.p, nullptr)))
             return rc;
         // CCD over the full step (xk = its end) and the first trial (abd.py:278-303)
         if ((rc = step_end(s, S, nullptr))) return rc;
@@ -878,7 +878,7 @@ int iteration_scenes(hipStream_t s, Sim& S) {
         return rc;
     if ((rc = abd_bsr_solve_scenes(s, P.solver, &S.sc, S.max_scene_bodies, S.row_ptr.p, S.cols.p, S.slots.p,
                                    S.blocks.p, st.g, st.dq, P.solver_tol, P.solver_maxiter, 10, L.newton, S.iters.p,
-                                   S.scal.p + 8)))
+                                   nullptr)))
         return rc;
     // CCD over the full steps of the scenes still iterating (abd.py:278-303)
     if ((rc = step_end_scenes(s, S, L.newton))) return rc;

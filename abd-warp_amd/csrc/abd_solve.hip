@@ -599,11 +599,11 @@ __global__ __launch_bounds__(kSolveThreads) void cg_kernel(Bsr M, const int32_t*
 // the arithmetic, in the same order, of eliminating body by body in (level, body) order.
 // A pivot that is not positive (the system is not positive definite as factored) or a
 // system above the size limit leaves x untouched and status kLdltFailed; else status 1
-// and resid = |b - H x|^2.  The factor and the W blocks live in the workgroup's dynamic
+// and resid (nullable) = |b - H x|^2.  The factor and the W blocks live in the workgroup's dynamic
 // LDS (dyn_blocks 12x12 blocks) when they fit there -- the factor's upper blocks with fill
 // and at most min(kLdltWBlocks, off-diagonal blocks) W blocks -- else in the system's
 // workspace region; the arithmetic is the same in both places.
-constexpr int kLdltColsPerThread = 2;
+constexpr int kLdltColsPerThread = 2;  // adjacent columns 2t, 2t + 1 of one body
 constexpr int kLdltCols = kLdltColsPerThread * kSolveThreads;
 constexpr int kLdltWBlocks = kLdltCols / 12;  // bound on a chunk's W blocks
 __host__ __device__ inline int64_t ldlt_region(int m) { return 144 * ((int64_t)m * (m + 1) / 2 + kLdltWBlocks); }
@@ -642,8 +642,9 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
     __shared__ unsigned char ecol[MB * MB];  // and local column (0xff: outside the system)
     __shared__ int cnode[MB], cstart[MB + 1], sstart[MB + 1], woff[MB];  // the chunk
     __shared__ int nlev, fail, nchunk;
-    __shared__ unsigned long long chunkmask;
+    __shared__ unsigned long long chunkmask, restm;  // the chunk; the level's bodies not yet eliminated
     __shared__ double colc[MB][2][12];
+    __shared__ double colinv[MB][2];  // 1 / the pivot, computed once by the pivot column's thread
     __shared__ double y[12 * MB];  // b, then the forward solve, then x
     __shared__ double z[12 * MB];
     __shared__ double sh[17];
@@ -688,7 +689,11 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
     if (t < 64) {  // wave 0, lane = row: fill, transposed pattern and tree levels, row by row
         unsigned long long mine = t < m ? adj[t] : 0ull, cm = 0ull;
         int h = 0;
-        for (int k = 0; k < m; ++k) {
+        // only rows with entries above the diagonal change anything (fill, colm, a parent's
+        // level); fill reaches only later rows, so the rows left are found again after each
+        unsigned long long todo = __ballot(t < m && bits_above(mine, t) != 0ull);
+        while (todo) {  // uniform
+            const int k = __builtin_ctzll(todo);
             // row k is final once k is reached, and so is its level (lane k's values,
             // read by the uniform lane index)
             const unsigned lo32 = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)mine, k);
@@ -701,6 +706,7 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
                 cm |= 1ull << k;
             }
             if (above && t == __builtin_ctzll(above) && h < hk + 1) h = hk + 1;
+            todo = __ballot(t < m && t > k && bits_above(mine, t) != 0ull);
         }
         if (t < m) {
             adj[t] = mine;
@@ -715,15 +721,20 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
         int idx = 0;
         for (unsigned long long bt = bits_above(adj[t], t); bt; bt &= bt - 1ull) mlist[t][idx++] = __builtin_ctzll(bt);
     }
-    if (t == 0) {
-        int acc = 0;
-        for (int i = 0; i < m; ++i) {
-            pos[i] = acc;
-            acc += __popcll(adj[i]);
+    if (t < kWave) {  // wave 0: pos = the exclusive prefix sum of the rows' block counts
+        const int cnt = t < m ? __popcll(adj[t]) : 0;
+        int incl = cnt;
+        for (int off = 1; off < kWave; off <<= 1) {
+            const int v = __shfl_up(incl, off, kWave);
+            if (t >= off) incl += v;
         }
-        pos[m] = acc;
-        const int offd = acc - m, wmax = offd < kLdltWBlocks ? offd : kLdltWBlocks;
-        fail = acc + wmax <= dyn_blocks ? 2 : 0;  // 2: the factor in LDS (reset below)
+        if (t < m) pos[t] = incl - cnt;
+        const int acc = __shfl(incl, kWave - 1, kWave);
+        if (t == 0) {
+            pos[m] = acc;
+            const int offd = acc - m, wmax = offd < kLdltWBlocks ? offd : kLdltWBlocks;
+            fail = acc + wmax <= dyn_blocks ? 2 : 0;  // 2: the factor in LDS (reset below)
+        }
     }
     __syncthreads();
     if (fail == 2) {  // uniform
@@ -734,46 +745,66 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
     if (t == 0) fail = 0;
     for (int64_t e = t; e < 144 * (int64_t)pos[m]; e += nt) U[e] = 0.0;
     __syncthreads();
-    // the matrix's upper blocks (row-major 12x12) into the factor
-    for (int64_t w = t; w < 144 * (int64_t)nnz; w += nt) {
-        const int e = (int)(w / 144), q = (int)(w % 144);
-        const int j = ecol[e];
-        const int i = last_le(lrp, m, e);
-        if (j < i || j >= m) continue;  // lower, or outside the system (0xff)
-        const int R = q / 12, Cq = q % 12;
-        const int64_t blk = pos[i] + __popcll(adj[i] & bits_below(j));
-        U[144 * blk + q] = M.blocks[144 * (int64_t)eslot[e] + 9 * (R / 3 + 4 * (Cq / 3)) + 3 * (R % 3) + Cq % 3];
+    // the matrix's upper blocks into the factor (row-major 12x12): one wave per block, its
+    // lanes reading the slot's 144 doubles in storage order (16 3x3 sub-blocks, sub-block
+    // (R/3, C/3) at 9 (R/3 + 4 C/3), row-major inside)
+    {
+        const int lane = t & (kWave - 1), wv = t / kWave, nwv = nt / kWave;
+        for (int e = wv; e < nnz; e += nwv) {  // uniform over the wave
+            const int j = ecol[e];
+            const int i = last_le(lrp, m, e);
+            if (j < i || j >= m) continue;  // lower, or outside the system (0xff)
+            const double* src = M.blocks + 144 * (int64_t)eslot[e];
+            double* dst = U + 144 * (int64_t)(pos[i] + __popcll(adj[i] & bits_below(j)));
+            for (int q = lane; q < 144; q += kWave) {
+                const int sb = q / 9, w9 = q % 9;
+                dst[12 * (3 * (sb % 4) + w9 / 3) + 3 * (sb / 4) + w9 % 3] = src[q];
+            }
+        }
     }
     const int levels = nlev;
-    unsigned long long rest = 0ull;  // thread 0: the bodies of the level not yet eliminated
     for (int L = 0; L < levels; ++L) {
-        if (t == 0) rest = levm[L];
+        if (t == 0) restm = levm[L];
         for (;;) {
             __syncthreads();  // the previous chunk is written back, its tables are free
-            if (t == 0) {
-                // the chunk: the lowest bodies of the level not yet eliminated whose
-                // augmented columns fit kLdltCols
-                unsigned long long cmask = 0ull;
-                int nc = 0, cols = 0, sitems = 0, wb = 0;
-                for (unsigned long long bt = rest; bt; bt &= bt - 1ull) {
-                    const int k = __builtin_ctzll(bt);
-                    const int d = __popcll(bits_above(adj[k], k));
-                    if (cols + 12 * (d + 2) > kLdltCols) break;
-                    cnode[nc] = k;
-                    cstart[nc] = cols;
-                    sstart[nc] = sitems;
-                    woff[k] = wb;
-                    cols += 12 * (d + 2);
-                    sitems += 144 * (d * (d + 1) / 2);
-                    wb += d;
-                    cmask |= 1ull << k;
-                    ++nc;
+            if (t < kWave) {
+                // wave 0, lane = body: the chunk is the lowest bodies of the level not yet
+                // eliminated whose augmented columns fit kLdltCols (prefix sums in body order;
+                // a single body always fits)
+                const unsigned long long rest = restm;
+                const bool member = t < m && ((rest >> t) & 1ull);
+                const int d = member ? __popcll(bits_above(adj[t], t)) : 0;
+                const int c0 = member ? 12 * (d + 2) : 0, s0 = member ? 144 * (d * (d + 1) / 2) : 0, w0 = d;
+                int ic = c0, is = s0, iw = w0;
+                for (int off = 1; off < kWave; off <<= 1) {
+                    const int vc = __shfl_up(ic, off, kWave), vs = __shfl_up(is, off, kWave),
+                              vw = __shfl_up(iw, off, kWave);
+                    if (t >= off) {
+                        ic += vc;
+                        is += vs;
+                        iw += vw;
+                    }
                 }
-                cstart[nc] = cols;
-                sstart[nc] = sitems;
-                nchunk = nc;
-                chunkmask = cmask;
-                rest &= ~cmask;
+                const bool take = member && ic <= kLdltCols;
+                const unsigned long long cmask = __ballot(take);
+                const int nc = __popcll(cmask);
+                if (take) {
+                    const int q = __popcll(cmask & bits_below(t));
+                    cnode[q] = t;
+                    cstart[q] = ic - c0;
+                    sstart[q] = is - s0;
+                    woff[t] = iw - w0;
+                    if (q == nc - 1) {
+                        cstart[nc] = ic;
+                        sstart[nc] = is;
+                    }
+                }
+                if (t == 0) {
+                    if (nc == 0) cstart[0] = sstart[0] = 0;
+                    nchunk = nc;
+                    chunkmask = cmask;
+                    restm = rest & ~cmask;
+                }
             }
             __syncthreads();
             const int nc = nchunk;
@@ -784,7 +815,7 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
             int cln[kLdltColsPerThread], cg[kLdltColsPerThread], cc_[kLdltColsPerThread], cd[kLdltColsPerThread];
 #pragma unroll
             for (int s = 0; s < kLdltColsPerThread; ++s) {
-                const int col = t + nt * s;
+                const int col = kLdltColsPerThread * t + s;
                 cln[s] = -1;
                 cg[s] = cc_[s] = cd[s] = 0;
                 if (col < C) {
@@ -812,20 +843,28 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
                     if (cln[s] >= 0 && cg[s] == 0 && cc_[s] == cc) {
 #pragma unroll
                         for (int r = 0; r < 12; ++r) colc[cln[s]][cc & 1][r] = a[s][r];
+                        colinv[cln[s]][cc & 1] = 1.0 / a[s][cc];
                         if (!(a[s][cc] > 0.0) || !isfinite(a[s][cc])) fail = 1;
                     }
                 __syncthreads();
+                // a thread's columns are adjacent and a body's column count is a multiple of
+                // 12, so both belong to one body: its pivot column is read from LDS once
+                if (cln[0] >= 0) {
+                    const double* f = colc[cln[0]][cc & 1];
+                    const double inv = colinv[cln[0]][cc & 1];
+                    double fr[12];
 #pragma unroll
-                for (int s = 0; s < kLdltColsPerThread; ++s)
-                    if (cln[s] >= 0) {
-                        const double* f = colc[cln[s]][cc & 1];
-                        const double inv = 1.0 / f[cc];
-                        const double sc = a[s][cc] * inv;
+                    for (int r = 0; r < 12; ++r) fr[r] = f[r];
 #pragma unroll
-                        for (int r = 0; r < 12; ++r)
-                            if (r != cc) a[s][r] -= f[r] * sc;
-                        a[s][cc] = sc;
-                    }
+                    for (int s = 0; s < kLdltColsPerThread; ++s)
+                        if (cln[s] >= 0) {
+                            const double sc = a[s][cc] * inv;
+#pragma unroll
+                            for (int r = 0; r < 12; ++r)
+                                if (r != cc) a[s][r] -= fr[r] * sc;
+                            a[s][cc] = sc;
+                        }
+                }
             }
             if (fail) {  // uniform: written before the last pivot column's barrier
                 if (t == 0) status[sidx] = kLdltFailed;
@@ -926,15 +965,17 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
         }
     }
     __syncthreads();
-    Rows R{row0, m, nullptr, nullptr, nullptr};
+    for (int i = t; i < n; i += nt) xg[i] = y[i];
     double rr = 0.0;
-    for (int i = t; i < n; i += nt) {
-        xg[i] = y[i];
-        const int lr = i % 12;
-        const double r = b[i] - row_product<kSpmvRows>(M, R, i / 12, lr / 3, lr % 3, y);
-        rr += r * r;
+    if (resid) {  // uniform: the residual's product with H only when asked for
+        Rows R{row0, m, nullptr, nullptr, nullptr};
+        for (int i = t; i < n; i += nt) {
+            const int lr = i % 12;
+            const double r = b[i] - row_product<kSpmvRows>(M, R, i / 12, lr / 3, lr % 3, y);
+            rr += r * r;
+        }
+        rr = block_sum_fast(rr, sh);
     }
-    rr = block_sum_fast(rr, sh);
     if (t == 0) {
         status[sidx] = 1;
         if (resid) resid[sidx] = rr;
