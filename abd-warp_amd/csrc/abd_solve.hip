@@ -638,13 +638,18 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
     __shared__ int lvl[MB];                  // the level of each body
     __shared__ unsigned char mlist[MB][MB];  // mlist[k]: the columns above k in row k, ascending
     __shared__ int lrp[MB + 1], pos[MB + 1];
-    __shared__ int eslot[MB * MB];         // the system's CSR entries: slot (at most m^2 entries)
-    __shared__ unsigned char ecol[MB * MB];  // and local column (0xff: outside the system)
+    // the system's CSR entries (slot, local column: 0xff outside the system; at most m^2), read
+    // only while the factor is filled, share their LDS with the elimination's pivot columns
+    // (two per step, double-buffered) and reciprocal pivots
+    constexpr int kEntryBytes = MB * MB * 5, kPivotBytes = MB * 2 * 2 * 13 * 8;
+    __shared__ __align__(16) unsigned char tables[kEntryBytes > kPivotBytes ? kEntryBytes : kPivotBytes];
+    int* eslot = reinterpret_cast<int*>(tables);
+    unsigned char* ecol = tables + 4 * MB * MB;
+    double(*colc)[2][2][12] = reinterpret_cast<double(*)[2][2][12]>(tables);
+    double(*colinv)[2][2] = reinterpret_cast<double(*)[2][2]>(tables + MB * 2 * 2 * 12 * 8);
     __shared__ int cnode[MB], cstart[MB + 1], sstart[MB + 1], woff[MB];  // the chunk
     __shared__ int nlev, fail, nchunk;
     __shared__ unsigned long long chunkmask, restm;  // the chunk; the level's bodies not yet eliminated
-    __shared__ double colc[MB][2][12];
-    __shared__ double colinv[MB][2];  // 1 / the pivot, computed once by the pivot column's thread
     __shared__ double y[12 * MB];  // b, then the forward solve, then x
     __shared__ double z[12 * MB];
     __shared__ double sh[17];
@@ -836,34 +841,50 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
                     }
                 }
             }
+            // two pivot columns per barrier.  A thread's columns are adjacent and a body's
+            // column count is a multiple of 12, so both belong to one body and the pivot pair
+            // (cp, cp + 1) of a body is one thread's: it publishes column cp as it stands and
+            // column cp + 1 as pivot cp leaves it (the update every column gets), with both
+            // reciprocal pivots; every column then takes pivot cp, then pivot cp + 1 -- the
+            // arithmetic of one pivot per step
 #pragma unroll
-            for (int cc = 0; cc < 12; ++cc) {
+            for (int cp = 0; cp < 12; cp += 2) {
+                const int bf = (cp >> 1) & 1;
+                if (cln[0] >= 0 && cg[0] == 0 && cc_[0] == cp) {
+                    const double inv0 = 1.0 / a[0][cp];
+                    if (!(a[0][cp] > 0.0) || !isfinite(a[0][cp])) fail = 1;
+                    const double sc1 = a[1][cp] * inv0;
+                    double piv1 = 0.0;
 #pragma unroll
-                for (int s = 0; s < kLdltColsPerThread; ++s)
-                    if (cln[s] >= 0 && cg[s] == 0 && cc_[s] == cc) {
-#pragma unroll
-                        for (int r = 0; r < 12; ++r) colc[cln[s]][cc & 1][r] = a[s][r];
-                        colinv[cln[s]][cc & 1] = 1.0 / a[s][cc];
-                        if (!(a[s][cc] > 0.0) || !isfinite(a[s][cc])) fail = 1;
+                    for (int r = 0; r < 12; ++r) {
+                        const double v = r == cp ? sc1 : a[1][r] - a[0][r] * sc1;
+                        colc[cln[0]][bf][0][r] = a[0][r];
+                        colc[cln[0]][bf][1][r] = v;
+                        if (r == cp + 1) piv1 = v;
                     }
+                    if (!(piv1 > 0.0) || !isfinite(piv1)) fail = 1;
+                    colinv[cln[0]][bf][0] = inv0;
+                    colinv[cln[0]][bf][1] = 1.0 / piv1;
+                }
                 __syncthreads();
-                // a thread's columns are adjacent and a body's column count is a multiple of
-                // 12, so both belong to one body: its pivot column is read from LDS once
                 if (cln[0] >= 0) {
-                    const double* f = colc[cln[0]][cc & 1];
-                    const double inv = colinv[cln[0]][cc & 1];
-                    double fr[12];
 #pragma unroll
-                    for (int r = 0; r < 12; ++r) fr[r] = f[r];
+                    for (int h = 0; h < 2; ++h) {  // pivot cp + h
+                        const int pc = cp + h;
+                        double f[12];
 #pragma unroll
-                    for (int s = 0; s < kLdltColsPerThread; ++s)
-                        if (cln[s] >= 0) {
-                            const double sc = a[s][cc] * inv;
+                        for (int r = 0; r < 12; ++r) f[r] = colc[cln[0]][bf][h][r];
+                        const double inv = colinv[cln[0]][bf][h];
 #pragma unroll
-                            for (int r = 0; r < 12; ++r)
-                                if (r != cc) a[s][r] -= fr[r] * sc;
-                            a[s][cc] = sc;
-                        }
+                        for (int s = 0; s < kLdltColsPerThread; ++s)
+                            if (cln[s] >= 0) {
+                                const double sc = a[s][pc] * inv;
+#pragma unroll
+                                for (int r = 0; r < 12; ++r)
+                                    if (r != pc) a[s][r] -= f[r] * sc;
+                                a[s][pc] = sc;
+                            }
+                    }
                 }
             }
             if (fail) {  // uniform: written before the last pivot column's barrier
