@@ -803,7 +803,9 @@ __device__ __forceinline__ void keep_write(const KeepArgs& A, int k, int64_t i, 
         int32_t* q = (k == 2 ? A.out.pt : A.out.ee) + 5 * o;
         q[0] = t5[0];
         q[1] = t5[1];
-        q[2] = A.remap[t5[2]];  // the pair's index in the kept ij
+        // the pair's index in the kept ij: its block's offset + its rank in the block (the
+        // tuple is kept only if its pair's body boxes overlap, so the pair is kept)
+        q[2] = (int32_t)(A.boff[0][t5[2] / kBlock] + A.remap[t5[2]]);
         q[3] = t5[3];
         q[4] = t5[4];
     } else {
@@ -814,7 +816,9 @@ __device__ __forceinline__ void keep_write(const KeepArgs& A, int k, int64_t i, 
 
 // The count: one entry per thread (its predicate's gathers need many waves in flight to
 // hide their latency), the kept entries of each workgroup's kBlock entries to bcnt; each
-// entry's predicate is stored for the write pass.
+// entry's predicate is stored for the write pass, and each kept pair's rank in its
+// workgroup's kept pairs (ij) to remap, so that the write pass places pairs and tuples in one
+// launch.
 // entries of input list k: A.n[k], or with the input sizes on the device (A.in.dn, the
 // keep output layout) that count (A.n[k] then bounds it)
 __device__ __forceinline__ int64_t keep_n(const KeepArgs& A, int k) {
@@ -829,9 +833,17 @@ __global__ __launch_bounds__(kBlock) void keep_count_kernel(KeepArgs A) {
     const int64_t blk = g - A.blk0[k];
     const int64_t i = blk * kBlock + threadIdx.x;
     bool pred = false;
-    if (i < keep_n(A, k)) {
+    const bool in_list = i < keep_n(A, k);
+    if (in_list) {
         pred = keep_pred(A, k, i);
         A.flag[A.flag0[k] + i] = pred ? 1 : 0;
+    }
+    if (k == 0) {  // uniform over the workgroup
+        int tot;
+        const int r = block_rank(pred, shw, &tot);
+        if (in_list) A.remap[i] = pred ? r : -1;
+        if (threadIdx.x == 0) A.bcnt[k][blk] = tot;
+        return;
     }
     const int c = __popcll(__ballot(pred ? 1 : 0));
     if (lane_id() == 0) shw[wave_id()] = c;
@@ -844,8 +856,7 @@ __global__ __launch_bounds__(kBlock) void keep_count_kernel(KeepArgs A) {
 }
 
 // The write pass over lists [k0, k1), one entry per thread: the kept entries at their
-// ordered ranks (the block's scanned offset + the rank in the block); list 0 also records
-// every pair's new index (-1 when dropped) in remap.
+// ordered ranks (the block's scanned offset + the rank in the block).
 __global__ __launch_bounds__(kBlock) void keep_write_kernel(KeepArgs A, int k0, int k1) {
     __shared__ int shw[kBlock / kWave];
     const int64_t g = (int64_t)blockIdx.x + A.blk0[k0];
@@ -858,7 +869,6 @@ __global__ __launch_bounds__(kBlock) void keep_write_kernel(KeepArgs A, int k0, 
     int tot;
     const int64_t r = A.boff[k][blk] + block_rank(pred, shw, &tot);
     if (pred) keep_write(A, k, i, r);
-    if (k == 0 && i < nk) A.remap[i] = pred ? (int32_t)r : -1;
 }
 
 }  // namespace
@@ -903,8 +913,8 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
         A.boff[k] = buf + nblk[k] + 1;
         buf += 2 * (nblk[k] + 1);
     }
-    // counts of all five lists in one launch, their scans (totals to dcount), then the
-    // pairs (ij, ji: the new pair indices) and the tuples
+    // counts of all five lists in one launch (with the kept pairs' ranks in their
+    // workgroups), their scans (totals to dcount), then every list written in one launch
     const int64_t nall = A.blk0[kKeepLists];
     if ((rc = arena_get(kSlotKeepFlags, (size_t)nflag + 1, (void**)&A.flag))) return rc;
     if (nall > 0) {
@@ -916,13 +926,8 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
     for (int k = 0; k < kKeepLists; ++k)
         if (n[k] > 0) sg[nseg++] = ScanSeg{A.bcnt[k], A.boff[k], nblk[k], A.boff[k] + nblk[k], dcount + k};
     if ((rc = scan_exclusive_i64_multi(s, sg, nseg))) return rc;
-    const int64_t npairs = A.blk0[2], ntup = nall - npairs;
-    if (npairs > 0) {
-        hipLaunchKernelGGL(keep_write_kernel, dim3((unsigned)npairs), dim3(kBlock), 0, s, A, 0, 2);
-        ABD_CHECK_LAUNCH();
-    }
-    if (ntup > 0) {
-        hipLaunchKernelGGL(keep_write_kernel, dim3((unsigned)ntup), dim3(kBlock), 0, s, A, 2, kKeepLists);
+    if (nall > 0) {
+        hipLaunchKernelGGL(keep_write_kernel, dim3((unsigned)nall), dim3(kBlock), 0, s, A, 0, kKeepLists);
         ABD_CHECK_LAUNCH();
     }
     // an empty list's total is 0 (its scan segment is skipped)

@@ -179,11 +179,11 @@ enum {
     kCntLen = 24
 };
 // rows of Sim::lsflag (iterating: the current and the next iteration's; searching) and of
-// Sim::lsval (E0: the current and the next; the accepted trial's energy; a searching
-// scene's next step length); the next iteration's rows are copied over the current ones at
-// the start of every iteration (ls_commit_kernel)
+// Sim::lsval (E0: the current and the next; the accepted trial's energy); the next
+// iteration's rows are copied over the current ones at the start of every iteration
+// (ls_commit_kernel)
 enum { kLfNewton = 0, kLfSearch = 2, kLfRows };
-enum { kLvE0 = 0, kLvE1 = 2, kLvNext, kLvRows };
+enum { kLvE0 = 0, kLvE1 = 2, kLvRows };
 
 int get_scalars(hipStream_t s, Sim& S, int first, int count) {
     ABD_CHECK(hipMemcpyAsync(S.host + first, S.scal.p + first, sizeof(double) * count, hipMemcpyDeviceToHost, s));
@@ -611,7 +611,6 @@ struct LsDev {
     const double* E0;       // energy at the iteration's start (abd.py:225)
     double* E0_nxt;
     double* E1;             // the accepted trial's energy
-    double* next;           // a searching scene's next step length
     double* dalpha;         // the accepted step length (0: a scene not iterating)
     const int32_t* it;      // Newton iterations of the frame so far
     int32_t* it_nxt;
@@ -631,7 +630,6 @@ LsDev ls_dev(Sim& S) {
     L.E0 = S.lsval.p + (size_t)(kLvE0 + c) * ns;
     L.E0_nxt = S.lsval.p + (size_t)(kLvE0 + n) * ns;
     L.E1 = S.lsval.p + (size_t)kLvE1 * ns;
-    L.next = S.lsval.p + (size_t)kLvNext * ns;
     L.dalpha = S.sv.p + (size_t)kSvAlpha * ns;
     L.it = S.lsit.p + (size_t)c * ns;
     L.it_nxt = S.lsit.p + (size_t)n * ns;
@@ -678,31 +676,36 @@ __global__ __launch_bounds__(kLsThreads) void ls_decide_kernel(LsDev L, double c
     for (int s = threadIdx.x; s < ns; s += kLsThreads) {
         bool searching = first ? L.newton[s] != 0 : L.search[s] != 0;
         if (first && !searching) L.dalpha[s] = 0.0;
+        double a0 = 0.0;  // the next pass's first trial
         if (searching) {
             bool acc = false;
             double last = 0.0;
+            const double e0 = L.E0[s], qtg = L.qtg[s];
             for (int k = 0; k < K && !acc; ++k) {
                 const double a = alpha[(int64_t)k * ns + s];
                 if (a < 0.0) break;  // past this scene's trials
                 last = a;
                 const double e1 = E[2 * (int64_t)ns * k + s] + E[2 * (int64_t)ns * k + ns + s];
-                const bool wolfe = e1 < L.E0[s] + c1 * a * L.qtg[s];
+                const bool wolfe = e1 < e0 + c1 * a * qtg;
                 if (wolfe || a < alpha_min) {
                     acc = true;
                     L.dalpha[s] = a;
                     L.E1[s] = e1;
                 }
             }
-            if (acc)
+            if (acc) {
                 searching = false;
-            else
-                L.next[s] = last * 0.5;
+            } else {
+                a0 = last * 0.5;
+            }
         }
         L.search[s] = searching ? 1 : 0;
-        const int need = searching ? trials_to_floor(L.next[s], alpha_min) : 0;
-        for (int k = 0; k < kLsSpec; ++k)
-            alpha[(int64_t)k * ns + s] =
-                k >= need ? -1.0 : (k == 0 ? L.next[s] : alpha[(int64_t)(k - 1) * ns + s] * 0.5);
+        const int need = searching ? trials_to_floor(a0, alpha_min) : 0;
+        double ak = a0;  // a0, a0 / 2, ... in a register (no re-read of what was just written)
+        for (int k = 0; k < kLsSpec; ++k) {
+            alpha[(int64_t)k * ns + s] = k >= need ? -1.0 : ak;
+            ak *= 0.5;
+        }
         nsearch += searching ? 1 : 0;
         kmax = need > kmax ? need : kmax;
     }
