@@ -714,15 +714,11 @@ __device__ __forceinline__ void point_box_at(const double* __restrict__ x, const
     }
 }
 
-// prim_boxes_kernel's box of primitive i (NV vertices from topo, numbered from vb) at x, or
-// with x1 set swept_prim_boxes_kernel's
+// prim_boxes_kernel's box of the primitive with scene vertices vi[0..NV) (its topology's
+// order) at x, or with x1 set swept_prim_boxes_kernel's
 template <int NV>
-__device__ __forceinline__ void prim_box_at(const double* __restrict__ x, const double* __restrict__ x1,
-                                            const int32_t* __restrict__ topo, int64_t i, int vb, double dil,
-                                            float* lo, float* hi) {
-    int64_t vi[NV];
-#pragma unroll
-    for (int j = 0; j < NV; ++j) vi[j] = vb + topo[NV * i + j];
+__device__ __forceinline__ void prim_box_of(const double* __restrict__ x, const double* __restrict__ x1,
+                                            const int64_t (&vi)[NV], double dil, float* lo, float* hi) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         double mn = x[3 * vi[0] + k], mx = mn;
@@ -748,6 +744,16 @@ __device__ __forceinline__ void prim_box_at(const double* __restrict__ x, const 
         hi[k] = h;
     }
 }
+// the same for primitive i (NV vertices from topo, numbered from vb)
+template <int NV>
+__device__ __forceinline__ void prim_box_at(const double* __restrict__ x, const double* __restrict__ x1,
+                                            const int32_t* __restrict__ topo, int64_t i, int vb, double dil,
+                                            float* lo, float* hi) {
+    int64_t vi[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) vi[j] = vb + topo[NV * i + j];
+    prim_box_of<NV>(x, x1, vi, dil, lo, hi);
+}
 
 __device__ __forceinline__ bool keep_pred(const KeepArgs& A, int k, int64_t i) {
     const KeepBoxes& B = A.bx;
@@ -759,6 +765,23 @@ __device__ __forceinline__ bool keep_pred(const KeepArgs& A, int k, int64_t i) {
     if (k <= 3) {  // (I, J, ii, a, b): both bodies' boxes and the two primitives' boxes
         const int32_t* t5 = (k == 2 ? A.in.pt : A.in.ee) + 5 * i;
         const int I = t5[0], J = t5[1];
+        const int32_t* vid = k == 2 ? A.in.vids.pt : A.in.vids.ee;
+        if (A.x && vid) {  // the candidate's vertex ids stored beside the list
+            vid += 4 * i;
+            const int64_t v0 = vid[0], v1 = vid[1], v2 = vid[2], v3 = vid[3];
+            if (!overlap(B.blo + 3 * I, B.bhi + 3 * I, B.blo + 3 * J, B.bhi + 3 * J)) return false;
+            float l1[3], h1[3], l2[3], h2[3];
+            if (k == 2) {
+                point_box_at(A.x, A.x1, v0, A.pdil, l1, h1);
+                const int64_t tv[3] = {v1, v2, v3};
+                prim_box_of<3>(A.x, A.x1, tv, A.tdil, l2, h2);
+            } else {
+                const int64_t ea[2] = {v0, v1}, eb[2] = {v2, v3};
+                prim_box_of<2>(A.x, A.x1, ea, A.edil, l1, h1);
+                prim_box_of<2>(A.x, A.x1, eb, A.edil, l2, h2);
+            }
+            return overlap(l1, h1, l2, h2);
+        }
         if (!overlap(B.blo + 3 * I, B.bhi + 3 * I, B.blo + 3 * J, B.bhi + 3 * J)) return false;
         const int64_t a = (k == 2 ? A.voff : A.eoff)[I] + (int64_t)t5[3];
         const int64_t b = (k == 2 ? A.foff : A.eoff)[J] + (int64_t)t5[4];
@@ -808,6 +831,12 @@ __device__ __forceinline__ void keep_write(const KeepArgs& A, int k, int64_t i, 
         q[2] = (int32_t)(A.boff[0][t5[2] / kBlock] + A.remap[t5[2]]);
         q[3] = t5[3];
         q[4] = t5[4];
+        const int32_t* vin = k == 2 ? A.in.vids.pt : A.in.vids.ee;
+        int32_t* vout = k == 2 ? A.out.ptv : A.out.eev;
+        if (vin && vout) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) vout[4 * o + j] = vin[4 * i + j];
+        }
     } else {
         A.out.vg[2 * o] = A.in.vg[2 * i];
         A.out.vg[2 * o + 1] = A.in.vg[2 * i + 1];
@@ -872,6 +901,26 @@ __global__ __launch_bounds__(kBlock) void keep_write_kernel(KeepArgs A, int k0, 
 }
 
 }  // namespace
+
+namespace {
+__global__ __launch_bounds__(kBlock) void cand_vid_kernel(SceneView V, int kind, const int32_t* __restrict__ tuples,
+                                                          int64_t n, int32_t* __restrict__ vid) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        int64_t v[4];
+        cand_idx(kind, V, tuples + 5 * i, nullptr, v);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) vid[4 * i + j] = (int32_t)v[j];
+    }
+}
+}  // namespace
+
+int cand_vertex_ids(hipStream_t s, const AbdScene* sc, int kind, const int32_t* tuples, int64_t n, int32_t* vid) {
+    if (n <= 0) return ABD_OK;
+    const SceneView V{sc->voff, sc->foff, sc->eoff, sc->tris, sc->edges, sc->x0};
+    hipLaunchKernelGGL(cand_vid_kernel, dim3(grid_for(n, kBlock, 1 << 16)), dim3(kBlock), 0, s, V, kind, tuples, n, vid);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
 
 int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, const KeepBoxes& bx, double dhat,
                        double ground, const KeepOutput& out, int64_t* dcount, int64_t* hcount, bool sync,

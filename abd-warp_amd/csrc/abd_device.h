@@ -332,6 +332,20 @@ __device__ __forceinline__ void fetch_ee_idx(const SceneView& S, const int32_t* 
     v[2] = vj + EJ[0];
     v[3] = vj + EJ[1];
 }
+// The four scene vertex ids of a PT (kind 0) / EE (kind 1) candidate: from `vid` (the ids
+// fetch_pt_idx / fetch_ee_idx give, stored beside the list; nullable) or fetched through
+// the topology.
+__device__ __forceinline__ void cand_idx(int kind, const SceneView& S, const int32_t* t5, const int32_t* vid,
+                                         int64_t v[4]) {
+    if (vid) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = vid[q];
+    } else if (kind == 0) {
+        fetch_pt_idx(S, t5, v);
+    } else {
+        fetch_ee_idx(S, t5, v);
+    }
+}
 
 }  // namespace abd
 
@@ -351,9 +365,9 @@ struct Verts {
 
 template <class Src>
 __device__ __forceinline__ double cand_energy_ee(const SceneView& S, const Src& X, const int32_t* t5, double d2hat,
-                                                 double kappa) {
+                                                 double kappa, const int32_t* vid = nullptr) {
     int64_t vi[4];
-    fetch_ee_idx(S, t5, vi);
+    cand_idx(1, S, t5, vid, vi);
     d3 v[4];
     X.load4(t5[0], t5[1], vi, 2, v);
     CMap c = cmap_ee(v[0], v[1], v[2], v[3]);
@@ -362,9 +376,9 @@ __device__ __forceinline__ double cand_energy_ee(const SceneView& S, const Src& 
 }
 template <class Src>
 __device__ __forceinline__ double cand_energy_pt(const SceneView& S, const Src& X, const int32_t* t5, double d2hat,
-                                                 double kappa) {
+                                                 double kappa, const int32_t* vid = nullptr) {
     int64_t vi[4];
-    fetch_pt_idx(S, t5, vi);
+    cand_idx(0, S, t5, vid, vi);
     d3 v[4];
     X.load4(t5[0], t5[1], vi, 1, v);
     CMap c = cmap_pt(v[0], v[1], v[2], v[3]);
@@ -378,12 +392,10 @@ __device__ __forceinline__ double cand_energy_vg(const int32_t* __restrict__ vof
     return barrier(d * d, d2hat, kappa);
 }
 __device__ __forceinline__ double cand_toi(int kind, const SceneView& S, const double* __restrict__ x0,
-                                           const double* __restrict__ x1, const int32_t* t) {
+                                           const double* __restrict__ x1, const int32_t* t,
+                                           const int32_t* vid = nullptr) {
     int64_t vi[4];
-    if (kind == 0)
-        fetch_pt_idx(S, t, vi);
-    else
-        fetch_ee_idx(S, t, vi);
+    cand_idx(kind, S, t, vid, vi);
     d3 a[4], b[4];
     for (int q = 0; q < 4; ++q) {
         a[q] = ld3(x0 + 3 * vi[q]);
@@ -393,12 +405,10 @@ __device__ __forceinline__ double cand_toi(int kind, const SceneView& S, const d
 }
 // The four vertices of a PT (kind 0) / EE (kind 1) candidate at both ends of the step.
 __device__ __forceinline__ void cand_verts(int kind, const SceneView& S, const double* __restrict__ x0,
-                                           const double* __restrict__ x1, const int32_t* t, d3 a[4], d3 b[4]) {
+                                           const double* __restrict__ x1, const int32_t* t, d3 a[4], d3 b[4],
+                                           const int32_t* vid = nullptr) {
     int64_t vi[4];
-    if (kind == 0)
-        fetch_pt_idx(S, t, vi);
-    else
-        fetch_ee_idx(S, t, vi);
+    cand_idx(kind, S, t, vid, vi);
     for (int q = 0; q < 4; ++q) {
         a[q] = ld3(x0 + 3 * vi[q]);
         b[q] = ld3(x1 + 3 * vi[q]);
@@ -407,9 +417,10 @@ __device__ __forceinline__ void cand_verts(int kind, const SceneView& S, const d
 // Whether collision_time of the candidate can be below 1: its coplanarity cubic has a
 // bracketed root in [0, 1] (cubic_bracket); otherwise it has no root and its TOI is 1.
 __device__ __forceinline__ bool cand_toi_bracketed(int kind, const SceneView& S, const double* __restrict__ x0,
-                                                   const double* __restrict__ x1, const int32_t* t) {
+                                                   const double* __restrict__ x1, const int32_t* t,
+                                                   const int32_t* vid = nullptr) {
     d3 a[4], b[4];
-    cand_verts(kind, S, x0, x1, t, a, b);
+    cand_verts(kind, S, x0, x1, t, a, b, vid);
     double coef[4];
     coplanar_poly(a, b, coef);
     return cubic_bracket(coef, 0.0, 1.0).kind != CubicBracket::kNone;

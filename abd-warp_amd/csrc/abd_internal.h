@@ -131,10 +131,18 @@ int within_ranges(hipStream_t s, int64_t n3, const double* x0, const double* x1,
 // per-scene contact energies to E[2 S k + s] and inertia energies to E[2 S k + S + s].
 // dn (here and below, nullable): the lists' sizes on the device, [nij, nij, npt, nee, nvg],
 // the counts passed then bounding them.
+// The scene vertex ids of each PT / EE candidate of a list (4 per tuple, in the order
+// fetch_pt_idx / fetch_ee_idx give them), stored beside it so that a kernel reads them
+// instead of gathering them through the topology; either may be null.
+struct VertIds {
+    const int32_t* pt = nullptr;
+    const int32_t* ee = nullptr;
+};
 int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
                 const double* A, const double* p, const double* dq, const double* A0, const double* p0,
                 const double* Adot, const double* pdot, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
-                const int32_t* vg, int64_t nvg, const int64_t* plan, double* E, const int64_t* dn = nullptr);
+                const int32_t* vg, int64_t nvg, const int64_t* plan, double* E, const int64_t* dn = nullptr,
+                VertIds vids = {});
 // The per-scene chunk plan of the energy / TOI reductions over one set of candidate lists
 // (plan: reduce_plan_elems(S) int64 on the device), made once and passed as `plan` to every
 // reduction over those lists (nullptr: each makes its own).
@@ -147,7 +155,7 @@ int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, co
 int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
               const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
               const int64_t* plan, double* toi, const uint8_t* active, const int64_t* dn = nullptr,
-              double* first_alpha = nullptr);
+              double* first_alpha = nullptr, VertIds vids = {});
 // The line search's first step length from a TOI (abd.py:300-302): clamped to [0, 1], x 0.9
 // below 1.  scene_toi's first_alpha (device, nullable) receives it per scene (-1 for the
 // scenes not active).
@@ -188,13 +196,17 @@ struct KeepInput {
     const int32_t *ij, *ji, *pt, *ee, *vg;
     int64_t nij, npt, nee, nvg;
     const int64_t* dn = nullptr;
+    VertIds vids = {};  // the PT / EE tuples' vertex ids (nullable)
 };
 struct KeepBoxes {
     const float *blo, *bhi, *plo, *phi, *elo, *ehi, *tlo, *thi;
 };
 struct KeepOutput {
     int32_t *ij, *ji, *pt, *ee, *vg;
+    int32_t *ptv = nullptr, *eev = nullptr;  // the kept tuples' vertex ids (when the input has them)
 };
+// The vertex ids of n PT (kind 0) or EE (kind 1) tuples into vid [n][4] (VertIds).
+int cand_vertex_ids(hipStream_t s, const AbdScene* sc, int kind, const int32_t* tuples, int64_t n, int32_t* vid);
 // The primitive boxes of the predicates computed per entry from the vertices x instead of
 // read from KeepBoxes (whose point, edge and triangle boxes are then unused): the box
 // kernels' own expressions, points (float)x -+ (float)pdil, edges and triangles the FP64

@@ -31,8 +31,10 @@ enum { kListPT = 0, kListEE = 1, kListVG = 2 };
 struct ListRef {
     const int32_t* p;
     int64_t n;
-    int kind;  // kListPT / kListEE / kListVG
+    int kind;                   // kListPT / kListEE / kListVG
+    const int32_t* v = nullptr;  // PT / EE: the candidates' scene vertex ids, 4 per tuple (nullable)
 };
+__device__ __forceinline__ const int32_t* vids_of(const ListRef& r, int64_t i) { return r.v ? r.v + 4 * i : nullptr; }
 // dn (nullable): the lists' sizes on the device, [nij, nij, npt, nee, nvg] (the keep
 // filter's layout); the n of each ListRef then bounds its size (a list with n = 0 is empty).
 struct Lists {
@@ -180,11 +182,11 @@ constexpr int kTrialLdsBodies = 128;
 // ipcsk/energy.py:10-48 per candidate
 template <class Src>
 __device__ __forceinline__ double energy_term(int kind, const AbdScene& sc, const SceneView& V, const AbdParams& P,
-                                              const Src& X, const int32_t* t) {
+                                              const Src& X, const int32_t* t, const int32_t* vid) {
     if (P.psd == 3 && kind != kListVG)  // full IPC (beyond the reference)
         return cand_energy_full(kind == kListPT ? 0 : 1, V, X, t, P.d2hat, P.kappa);
-    if (kind == kListEE) return cand_energy_ee(V, X, t, P.d2hat, P.kappa);
-    if (kind == kListPT) return cand_energy_pt(V, X, t, P.d2hat, P.kappa);
+    if (kind == kListEE) return cand_energy_ee(V, X, t, P.d2hat, P.kappa, vid);
+    if (kind == kListPT) return cand_energy_pt(V, X, t, P.d2hat, P.kappa, vid);
     return cand_energy_vg(sc.voff, X, t, P.ground, P.d2hat, P.kappa);
 }
 
@@ -233,7 +235,7 @@ __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdPara
         if (OP == 0) {
             const Verts X{x0};
             for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x)
-                acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i);
+                acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i, vids_of(L.l[l], i));
         } else if (OP == 2) {
             const double al = T.alpha[(int64_t)S * blockIdx.y + s];
             const int b0 = S > 1 ? sc.scene_boff[s] : 0, b1 = S > 1 ? sc.scene_boff[s + 1] : sc.n_bodies;
@@ -249,11 +251,11 @@ __global__ __launch_bounds__(kRedThreads) void chunk_kernel(AbdScene sc, AbdPara
                 __syncthreads();
                 const VertsTrialLds X{sc.x0, st, b0};
                 for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x)
-                    acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i);
+                    acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i, vids_of(L.l[l], i));
             } else {
                 const VertsTrial X{sc.x0, T.A, T.p, T.dq, al};
                 for (int64_t i = start + threadIdx.x; i < end; i += blockDim.x)
-                    acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i);
+                    acc += energy_term(kind, sc, V, P, X, L.l[l].p + w * i, vids_of(L.l[l], i));
             }
         }
     }
@@ -344,7 +346,7 @@ __global__ __launch_bounds__(kRedThreads, 4) void toi_scan_kernel(AbdScene sc, A
                 if (tv < 1.0) acc = tv;  // `if t < 1: atomic_min`
             } else {
                 const SceneView V{sc.voff, sc.foff, sc.eoff, sc.tris, sc.edges, sc.x0};
-                queued = cand_toi_bracketed(kind == kListPT ? 0 : 1, V, x0, x1, t);
+                queued = cand_toi_bracketed(kind == kListPT ? 0 : 1, V, x0, x1, t, vids_of(L.l[l], i));
                 entry = (int32_t)((i << 1) | (kind == kListEE ? 1 : 0));
             }
         }
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(kToiSolveThreads) void toi_solve_kernel(AbdScene sc
         const int ee = e & 1;
         const ListRef& Lr = L.l[ee ? 0 : 1];  // the lists' order: EE, PT, VG
         const int32_t* t = Lr.p + 5 * (int64_t)(e >> 1);
-        const double tt = cand_toi(ee, V, x0, x1, t);
+        const double tt = cand_toi(ee, V, x0, x1, t, vids_of(Lr, e >> 1));
         if (tt < 1.0) {
             const int s = S > 1 ? sc.body_scene[t[0]] : 0;
             atomicMin(slots + s, (unsigned long long)__double_as_longlong(tt));
@@ -617,11 +619,11 @@ int toi_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
 // sum order of the energy; a minimum does not depend on it).  The switches of
 // const_params.py:42-44 drop a whole list.
 Lists contact_lists(const AbdParams& P, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
-                    const int32_t* vg, int64_t nvg, const int64_t* dn = nullptr) {
+                    const int32_t* vg, int64_t nvg, const int64_t* dn = nullptr, VertIds vids = {}) {
     if (!P.pt) npt = 0;
     if (!P.ee) nee = 0;
     if (!P.vg) nvg = 0;
-    return Lists{{{ee, nee, kListEE}, {pt, npt, kListPT}, {vg, nvg, kListVG}}, dn};
+    return Lists{{{ee, nee, kListEE, vids.ee}, {pt, npt, kListPT, vids.pt}, {vg, nvg, kListVG, nullptr}}, dn};
 }
 
 size_t reduce_plan_elems(int S) { return 2 * kLists * (size_t)S + (size_t)S + 1; }
@@ -675,9 +677,9 @@ __global__ __launch_bounds__(kRedThreads) void inertia_trials_kernel(int S, cons
 int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
                 const double* A, const double* p, const double* dq, const double* A0, const double* p0,
                 const double* Adot, const double* pdot, const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee,
-                const int32_t* vg, int64_t nvg, const int64_t* plan_in, double* E, const int64_t* dn) {
+                const int32_t* vg, int64_t nvg, const int64_t* plan_in, double* E, const int64_t* dn, VertIds vids) {
     int rc;
-    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg, dn);
+    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg, dn, vids);
     npt = L.l[1].n;
     nee = L.l[0].n;
     nvg = L.l[2].n;
@@ -726,10 +728,11 @@ int contact_energy(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S,
 
 int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
               const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
-              const int64_t* plan, double* toi, const uint8_t* active, const int64_t* dn, double* first_alpha) {
+              const int64_t* plan, double* toi, const uint8_t* active, const int64_t* dn, double* first_alpha,
+              VertIds vids) {
     if (!P || !x_t0 || !x_t1 || !toi || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
-    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg, dn);  // abd.py:288-299
+    const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg, dn, vids);  // abd.py:288-299
     return toi_reduce(s, *sc, *P, S, x_t0, x_t1, L, plan, toi, active, nullptr, first_alpha);
 }
 

@@ -59,18 +59,24 @@ struct BoxBuf {
 // candidate tuples of one set of boxes.  The counts are exact when the set was culled
 // (exact: consumers take them from here); a set filtered on the device has its sizes at dn
 // ([nij, nij, npt, nee, nvg], device) and the counts here bound them.
+// With `vids`, the PT and EE tuples' scene vertex ids (4 each, VertIds) are kept beside them.
 struct CandSet {
     DevBuf<int32_t> ij, ji, ptl, eel, vgl;
+    DevBuf<int32_t> ptv, eev;
     int64_t nij = 0, npt = 0, nee = 0, nvg = 0;
     bool exact = true;
+    bool vids = false;
     const int64_t* dn = nullptr;
     const int64_t* counts() const { return exact ? nullptr : dn; }
-    int reserve_like(const CandSet& o) {
+    VertIds vertex_ids() const { return vids ? VertIds{ptv.p, eev.p} : VertIds{}; }
+    int reserve_like(const CandSet& o, bool with_vids = false) {
         int rc = ij.reserve(2 * (size_t)o.nij + 2);
         if (!rc) rc = ji.reserve(2 * (size_t)o.nij + 2);
         if (!rc) rc = ptl.reserve(5 * (size_t)o.npt + 5);
         if (!rc) rc = eel.reserve(5 * (size_t)o.nee + 5);
         if (!rc) rc = vgl.reserve(2 * (size_t)o.nvg + 2);
+        if (!rc && with_vids) rc = ptv.reserve(4 * (size_t)o.npt + 4);
+        if (!rc && with_vids) rc = eev.reserve(4 * (size_t)o.nee + 4);
         return rc;
     }
     void release() {
@@ -79,6 +85,8 @@ struct CandSet {
         ptl.release();
         eel.release();
         vgl.release();
+        ptv.release();
+        eev.release();
     }
 };
 
@@ -306,7 +314,8 @@ int collision_from(hipStream_t s, Sim& S, const CandSet& W, const double* xv) {
     // far fewer entries than the scene has primitives
     if ((rc = abd_body_boxes(s, &S.sc, xv, dh, S.body.lo.p, S.body.hi.p))) return rc;
     if ((rc = S.col.reserve_like(W))) return rc;
-    const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg, W.counts()};
+    const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg, W.counts(),
+                       W.vertex_ids()};
     const KeepBoxes bx{S.body.lo.p, S.body.hi.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     const KeepOutput out{S.col.ij.p, S.col.ji.p, S.col.ptl.p, S.col.eel.p, S.col.vgl.p};
     const KeepAtX at{xv, nullptr, S.P.dhat, dh, 0.0};
@@ -379,6 +388,13 @@ int swept_set_recull(hipStream_t s, Sim& S) {
     if ((rc = swept_boxes(s, &S.sc, ABD_KIND_POINT, S.vlo.p, S.vhi.p, S.P.dhat, S.spt.lo.p, S.spt.hi.p))) return rc;
     if ((rc = cull_lists(s, S, S.sup, S.sbody, S.spt, S.stri, S.sedge))) return rc;
     S.sup.exact = true;
+    // the tuples' vertex ids, read by the filters and, carried into the swept set, by the CCD
+    // and the line search (the superset serves many steps)
+    if ((rc = S.sup.ptv.reserve(4 * (size_t)S.sup.npt + 4))) return rc;
+    if ((rc = S.sup.eev.reserve(4 * (size_t)S.sup.nee + 4))) return rc;
+    if ((rc = cand_vertex_ids(s, &S.sc, 0, S.sup.ptl.p, S.sup.npt, S.sup.ptv.p))) return rc;
+    if ((rc = cand_vertex_ids(s, &S.sc, 1, S.sup.eel.p, S.sup.nee, S.sup.eev.p))) return rc;
+    S.sup.vids = true;
     S.sup_valid = true;
     return swept_set_launch(s, S);
 }
@@ -393,11 +409,16 @@ int swept_set_launch(hipStream_t s, Sim& S) {
     // filter per entry of the superset from both ends (far fewer entries than primitives)
     if ((rc = swept_boxes(s, &S.sc, -1, st.x, st.xk, dh, S.tbody.lo.p, S.tbody.hi.p, S.body.lo.p, S.body.hi.p)))
         return rc;
-    if ((rc = S.sw.reserve_like(S.sup))) return rc;
+    if ((rc = S.sw.reserve_like(S.sup, S.sup.vids))) return rc;
     const CandSet& W = S.sup;
-    const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg, W.counts()};
+    const KeepInput in{W.ij.p, W.ji.p, W.ptl.p, W.eel.p, W.vgl.p, W.nij, W.npt, W.nee, W.nvg, W.counts(),
+                       W.vertex_ids()};
     const KeepBoxes bx{S.tbody.lo.p, S.tbody.hi.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    const KeepOutput o{S.sw.ij.p, S.sw.ji.p, S.sw.ptl.p, S.sw.eel.p, S.sw.vgl.p};
+    KeepOutput o{S.sw.ij.p, S.sw.ji.p, S.sw.ptl.p, S.sw.eel.p, S.sw.vgl.p};
+    if (W.vids) {
+        o.ptv = S.sw.ptv.p;
+        o.eev = S.sw.eev.p;
+    }
     const KeepAtX at{st.x, st.xk, S.P.dhat, dh, 0.0};
     if ((rc = keep_from_superset(s, &S.sc, in, bx, S.P.dhat, S.P.ground, o, S.cnt.p + kCntSw, nullptr, false, &at)))
         return rc;
@@ -407,6 +428,7 @@ int swept_set_launch(hipStream_t s, Sim& S) {
     S.sw.nvg = W.nvg;
     S.sw.dn = S.cnt.p + kCntSw;
     S.sw.exact = false;
+    S.sw.vids = W.vids;
     S.sw_valid = true;
     return ABD_OK;
 }
@@ -469,7 +491,7 @@ int line_energy_multi(hipStream_t s, Sim& S, const double* alphas, int K, double
     ABD_CHECK(hipMemcpyAsync(S.cand_alpha.p, ha, sizeof(double) * K, hipMemcpyHostToDevice, s));
     if ((rc = line_trials(s, &S.sc, &S.P, 1, K, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
                           S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p,
-                          S.sw.counts())))
+                          S.sw.counts(), S.sw.vertex_ids())))
         return rc;
     ABD_CHECK(hipMemcpyAsync(he, S.cand_E.p, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, s));
     ABD_CHECK(hipStreamSynchronize(s));
@@ -496,13 +518,13 @@ int ccd_and_first_trial(hipStream_t s, Sim& S) {
             return rc;
         // the TOI, and from it on the device the first trial's step length
         if ((rc = scene_toi(s, &S.sc, &S.P, 1, st.x, st.xk, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg,
-                            S.plan.p, S.scal.p + 2, nullptr, W.counts(), S.cand_alpha.p)))
+                            S.plan.p, S.scal.p + 2, nullptr, W.counts(), S.cand_alpha.p, W.vertex_ids())))
             return rc;
         // q.g (abd.py:148) and g.g (abd.py:265): g and dq are final here
         if ((rc = dots(s, &S.sc, 1, st.dq, st.g, st.g, st.g, S.scal.p + 3, S.scal.p + 5))) return rc;
         if ((rc = line_trials(s, &S.sc, &S.P, 1, 1, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot,
                               st.pdot, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg, S.plan.p, S.cand_E.p,
-                              W.counts())))
+                              W.counts(), W.vertex_ids())))
             return rc;
         ABD_CHECK(hipMemcpyAsync(S.hcand + kLsSpec, S.cand_E.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
         ABD_CHECK(hipMemcpyAsync(S.host + 6, S.iters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -797,7 +819,7 @@ int line_pass(hipStream_t s, Sim& S, int K) {
     const AbdSimState& st = S.st;
     return line_trials(s, &S.sc, &S.P, S.ns, K, S.cand_alpha.p, st.A, st.p, st.dq, st.A0, st.p0, st.Adot, st.pdot,
                        S.sw.ptl.p, S.sw.npt, S.sw.eel.p, S.sw.nee, S.sw.vgl.p, S.sw.nvg, S.plan.p, S.cand_E.p,
-                       S.sw.counts());
+                       S.sw.counts(), S.sw.vertex_ids());
 }
 
 int ls_decide(hipStream_t s, Sim& S, const LsDev& L, bool first, int K) {
@@ -833,7 +855,7 @@ int ccd_and_search_scenes(hipStream_t s, Sim& S, const LsDev& L) {
         return rc;
     // the TOI of the scenes still iterating and from it the first trial's step lengths
     if ((rc = scene_toi(s, &S.sc, &P, ns, st.x, st.xk, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg, S.plan.p,
-                        S.sv.p + kSvToi * ns, L.newton, W.counts(), S.cand_alpha.p)))
+                        S.sv.p + kSvToi * ns, L.newton, W.counts(), S.cand_alpha.p, W.vertex_ids())))
         return rc;
     // q.g and g.g of the Armijo and stopping tests (abd.py:148,265): g and dq are final here
     if ((rc = dots(s, &S.sc, ns, st.dq, st.g, st.g, st.g, S.sv.p + kSvQtg * ns, S.sv.p + kSvGtg * ns))) return rc;
