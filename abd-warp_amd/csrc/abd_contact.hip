@@ -530,16 +530,45 @@ __global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64
     }
 }
 
-__device__ __forceinline__ int64_t lower_bound_body(const int32_t* __restrict__ vg, int64_t n, int b) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (vg[2 * mid] < b)
-            lo = mid + 1;
-        else
-            hi = mid;
+// The first vertex-ground candidate of body b and of body b + 1 in a list grouped by body
+// ascending, found by the whole workgroup at once: each half probes 128 evenly spaced
+// entries for its key and the count of entries below the key narrows the range 128-fold,
+// so a list of n entries takes about log_128(n) + 1 dependent loads instead of log_2(n).
+// Every thread runs the same trip count (the ranges are computed from block-wide counts).
+__device__ __forceinline__ void body_vg_range(const int32_t* __restrict__ vg, int64_t n, int b, int64_t& k0,
+                                              int64_t& k1) {
+    static_assert(kBlock == 256, "two halves of 128 probes");
+    __shared__ int s_cnt[2][kBlock / kWave];
+    const int t = threadIdx.x, h = t >> 7, j = t & 127, w = t / kWave;
+    const int key = b + h;
+    int64_t lo[2] = {0, 0}, len[2] = {n, n};
+    for (int it = 0; len[0] > 0 || len[1] > 0; ++it) {
+        const int64_t L = len[h];
+        bool below = false;
+        if (L > 128)
+            below = vg[2 * (lo[h] + (int64_t)j * L / 128)] < key;
+        else if (j < L)
+            below = vg[2 * (lo[h] + j)] < key;
+        const uint64_t m = __ballot(below);
+        if ((t & (kWave - 1)) == 0) s_cnt[it & 1][w] = __popcll(m);
+        __syncthreads();
+        for (int q = 0; q < 2; ++q) {
+            const int c = s_cnt[it & 1][2 * q] + s_cnt[it & 1][2 * q + 1];
+            const int64_t Lq = len[q];
+            if (Lq <= 128) {
+                lo[q] += c;
+                len[q] = 0;
+            } else {
+                // entries before probe c-1 (inclusive) are below the key, probe c is not
+                const int64_t a = c > 0 ? lo[q] + (int64_t)(c - 1) * Lq / 128 + 1 : lo[q];
+                const int64_t e = c < 128 ? lo[q] + (int64_t)c * Lq / 128 : lo[q] + Lq;
+                lo[q] = a;
+                len[q] = e - a;
+            }
+        }
     }
-    return lo;
+    k0 = lo[0];
+    k1 = lo[1];
 }
 
 // stage E: one workgroup per body (144 Hessian entries + 12 gradient entries).
@@ -584,7 +613,8 @@ __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdP
         // of each contact are evaluated once per tile into LDS, then every entry adds the
         // tile's terms in contact order.
         __shared__ double s_d[kBlock], s_db[kBlock], s_d2b[kBlock], s_vt[kBlock][3];
-        const int64_t k0 = lower_bound_body(vg, nvg, b), k1 = lower_bound_body(vg, nvg, b + 1);
+        int64_t k0, k1;
+        body_vg_range(vg, nvg, b, k0, k1);
         const int ia = R / 3, ra = R % 3, ja = C / 3, ca = C % 3;
         for (int64_t base = k0; base < k1; base += kBlock) {
             const int nt = (int)(k1 - base < kBlock ? k1 - base : kBlock);
