@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstdio>
+#include <cstdlib>
 #include <mutex>
 
 #include "abd_block.h"
@@ -25,6 +27,12 @@ int current_device(int* dev) {
     return ABD_OK;
 }
 }  // namespace
+
+int hip_failure(hipError_t e, const char* file, int line, const char* expr) {
+    static const bool report = std::getenv("ABD_DEBUG_ERRORS") != nullptr;
+    if (report) std::fprintf(stderr, "abd: %s failed at %s:%d: %s\n", expr, file, line, hipGetErrorString(e));
+    return -static_cast<int>(e);
+}
 
 ArenaScope::ArenaScope(Arena* a) : prev_(t_arena) { t_arena = a; }
 ArenaScope::~ArenaScope() { t_arena = prev_; }
@@ -53,7 +61,13 @@ int arena_get(int slot, size_t bytes, void** ptr) {
             A.ptr[slot] = nullptr;
             A.size[slot] = 0;
         }
-        ABD_CHECK(hipMalloc(&A.ptr[slot], want));
+        const hipError_t e = hipMalloc(&A.ptr[slot], want);
+        if (e != hipSuccess) {
+            A.ptr[slot] = nullptr;
+            if (std::getenv("ABD_DEBUG_ERRORS"))
+                std::fprintf(stderr, "abd: scratch slot %d: %zu bytes not allocated\n", slot, want);
+            return hip_failure(e, __FILE__, __LINE__, "hipMalloc");
+        }
         A.size[slot] = want;
     }
     *ptr = A.ptr[slot];
@@ -190,7 +204,7 @@ int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n
 }
 
 namespace {
-AbdTuning g_tuning{4096, 16384};
+AbdTuning g_tuning{4096, 16384, 0};
 }
 AbdTuning tuning() { return g_tuning; }
 
@@ -236,9 +250,10 @@ int abd_get_tuning(AbdTuning* t) {
 
 int abd_set_tuning(const AbdTuning* t) {
     if (!t || t->cull_chunk < 256 || t->cull_chunk > 65536 || (t->cull_chunk & (t->cull_chunk - 1)) ||
-        t->cull_grid < 64 || t->cull_grid > (1 << 20))
+        t->cull_grid < 64 || t->cull_grid > (1 << 20) || t->contact_bound_mib < 0)
         return ABD_ERR_ARG;
     g_tuning = *t;
+    note_reallocation();  // graphs recorded under the old shapes are not replayed
     return ABD_OK;
 }
 

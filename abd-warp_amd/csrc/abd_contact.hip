@@ -32,6 +32,7 @@
 namespace abd {
 
 constexpr int kCItems = 8;          // candidates per thread in the compaction passes
+constexpr int kGhBoundMib = 8192;  // stored contacts sized by a list bound, at most (default)
 constexpr int kContactStride = 128;  // doubles per stored contact: c[0..4], b', pad, then V[5][24]
 constexpr int kCoefOff = 0;
 constexpr int kB1Off = 5;
@@ -401,7 +402,6 @@ __global__ __launch_bounds__(kBlock) void contact_kernel(AbdScene sc, AbdParams 
 }
 
 constexpr int kPairThreads = 576;  // one thread per entry of the 24x24 body-pair block
-constexpr int kPairBatch = 8;
 constexpr int kPairBufStride = 2 * 144 + 24;  // diag(ij0), diag(ij1), g(ij0), g(ij1)
 constexpr int64_t kPairGrid = 2048;           // workgroups of stage D (grid-stride over pairs)
 
@@ -418,49 +418,50 @@ __global__ __launch_bounds__(kBlock) void pair_chunks_kernel(int64_t nij, const 
     }
 }
 
-// stage D's item -> pair map: pair ii owns the items [coff[ii], coff[ii] + cnt[ii]); one
-// wave per pair, its lanes striding over the pair's items
+// stage D's items: pair ii owns the items [coff[ii], coff[ii] + cnt[ii]), its PT run's chunks
+// then its EE run's; item i's descriptor is its first contact | its contact count << 56.
+// One wave per pair, its lanes striding over the pair's items.
+constexpr int kItemShift = 56;
 __global__ __launch_bounds__(kBlock) void pair_items_kernel(int64_t nij, const int64_t* __restrict__ cnt,
                                                             const int64_t* __restrict__ coff,
-                                                            int32_t* __restrict__ ipair) {
+                                                            const int64_t* __restrict__ pseg,
+                                                            int64_t* __restrict__ idesc) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nw = (int64_t)gridDim.x * (kBlock / kWave);
     for (int64_t ii = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; ii < nij; ii += nw) {
         const int64_t n = cnt[ii], o = coff[ii];
-        for (int64_t c = lane; c < n; c += kWave) ipair[o + c] = (int32_t)ii;
+        const int64_t p0 = pseg[2 * ii], p1 = pseg[2 * ii + 1];
+        const int64_t e0 = pseg[2 * (nij + ii)], e1 = pseg[2 * (nij + ii) + 1];
+        const int64_t npc = (p1 - p0 + kPairChunk - 1) / kPairChunk;
+        for (int64_t c = lane; c < n; c += kWave) {
+            const bool pt = c < npc;
+            const int64_t c0 = pt ? p0 + c * kPairChunk : e0 + (c - npc) * kPairChunk;
+            const int64_t end = pt ? p1 : e1;
+            const int64_t nc = end - c0 < kPairChunk ? end - c0 : kPairChunk;
+            idesc[o + c] = c0 | (nc << kItemShift);
+        }
     }
 }
 
 // stage D2: one work item per workgroup (grid-stride over the *total items): the sums, in
 // contact order, of one chunk's contacts into part[item] (24x24 block in the pair's body
-// order [ij0 | ij1], then g).  DENSE: contacts stored by contact_terms_exact / _full.
-template <bool DENSE>
-__global__ __launch_bounds__(kPairThreads) void pair_partial_kernel(int64_t nij, const double* __restrict__ cbuf,
-                                                                     const uint8_t* __restrict__ swap,
-                                                                     const int64_t* __restrict__ pseg,
-                                                                     const int64_t* __restrict__ coff,
-                                                                     const int32_t* __restrict__ ipair,
+// order [ij0 | ij1], then g), for contacts stored whole by contact_terms_exact / _full.
+__global__ __launch_bounds__(kPairThreads) void pair_partial_dense_kernel(const double* __restrict__ cbuf,
+                                                                           const uint8_t* __restrict__ swap,
+                                                                           const int64_t* __restrict__ idesc,
                                                                      const int64_t* __restrict__ total,
                                                                      double* __restrict__ part) {
-    constexpr int kStride = DENSE ? kDenseStride : kContactStride;
-    constexpr int kBatch = DENSE ? 4 : kPairBatch;
+    constexpr int kStride = kDenseStride;
+    constexpr int kBatch = 4;
     __shared__ double sc[kBatch][kStride];
     __shared__ uint8_t ssw[kBatch];
     const int t = threadIdx.x;
     const int R = t / 24, C = t % 24;
     const int64_t nitem = *total;
     for (int64_t gc = blockIdx.x; gc < nitem; gc += gridDim.x) {
-        // the pair: coff[ii] <= gc < coff[ii + 1]
-        const int64_t ii = ipair[gc];
-        int64_t lc = gc - coff[ii];
-        const int64_t* sp = pseg + 2 * ii;
-        const int64_t npc = (sp[1] - sp[0] + kPairChunk - 1) / kPairChunk;
-        if (lc >= npc) {  // an EE chunk
-            lc -= npc;
-            sp = pseg + 2 * (nij + ii);
-        }
-        const int64_t c0 = sp[0] + lc * kPairChunk;
-        const int64_t c1 = sp[1] < c0 + kPairChunk ? sp[1] : c0 + kPairChunk;
+        const int64_t d = idesc[gc];
+        const int64_t c0 = d & ((int64_t(1) << kItemShift) - 1);
+        const int64_t c1 = c0 + (d >> kItemShift);
         double h = 0.0, gacc = 0.0;
         for (int64_t e0 = c0; e0 < c1; e0 += kBatch) {
             const int nbat = (int)((c1 - e0) < kBatch ? (c1 - e0) : kBatch);
@@ -472,23 +473,95 @@ __global__ __launch_bounds__(kPairThreads) void pair_partial_kernel(int64_t nij,
                 // body order [ij0 | ij1]: swap halves when body I is ij1
                 const int sw = ssw[bq] ? 12 : 0;
                 const int r = (R + sw) % 24, cc = (C + sw) % 24;
-                if (DENSE) {
-                    h += sc[bq][24 * r + cc];
-                    if (t < 24) gacc += sc[bq][kDenseGOff + (t + sw) % 24];
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 5; ++k) {
-                        const double* V = &sc[bq][kVOff + 24 * k];
-                        h += sc[bq][kCoefOff + k] * (V[r] * V[cc]);
-                    }
-                    if (t < 24) gacc += sc[bq][kB1Off] * sc[bq][kVOff + 24 * 4 + (t + sw) % 24];
-                }
+                h += sc[bq][24 * r + cc];
+                if (t < 24) gacc += sc[bq][kDenseGOff + (t + sw) % 24];
             }
             __syncthreads();
         }
         double* pp = part + gc * kPartStride;
         pp[t] = h;
         if (t < 24) pp[576 + t] = gacc;
+    }
+}
+
+// stage D2 for rank-one contacts (the stored form of contact_terms): one work item per
+// workgroup, the chunk's contacts staged into LDS at once (without the pad), then one
+// thread per entry of the upper triangle of the 24x24 block summing c_k (V_k[r] V_k[c]) over
+// the chunk's contacts in order, and threads 0..23 the gradient; an entry of the lower
+// triangle is the same sum (V_k[r] V_k[c] == V_k[c] V_k[r]), written from its mirror.
+constexpr int kRankThreads = 320;                   // 300 upper-triangle entries, 5 waves
+constexpr int kStaged = 126;                        // doubles per staged contact: c[5], b', V[5][24]
+constexpr int kRankPieces = (kPairChunk * 63 + kRankThreads - 1) / kRankThreads;  // 16-byte loads a thread
+static_assert(kContactStride == 128 && kCoefOff == 0 && kB1Off == 5 && kVOff == 8, "staged layout");
+
+__global__ __launch_bounds__(kRankThreads) void pair_partial_rank_kernel(const double* __restrict__ cbuf,
+                                                                         const uint8_t* __restrict__ swap,
+                                                                         const int64_t* __restrict__ idesc,
+                                                                         const int64_t* __restrict__ total,
+                                                                         double* __restrict__ part) {
+    static_assert(kPairChunk <= kWave, "one swap bit a lane");
+    // the staged contacts, then a 16-byte slot the pieces past the chunk are stored to
+    __shared__ __align__(16) double st[kPairChunk * kStaged + 2];
+    const int t = threadIdx.x, lane = t % kWave;
+    // this thread's entry (r, c), r <= c, of the upper triangle (rows of 24 - r entries)
+    int r = 0, j = t;
+    while (r < 24 && j >= 24 - r) {
+        j -= 24 - r;
+        ++r;
+    }
+    const bool ent = r < 24;
+    const int c = ent ? r + j : 0;
+    if (!ent) r = 0;
+    // V indices in the stored order, without and with the halves swapped (body I is ij1)
+    const int tg = t % 24;
+    const int rA = r, rB = (r + 12) % 24, cA = c, cB = (c + 12) % 24, gA = tg, gB = (tg + 12) % 24;
+    const int64_t nitem = *total;
+    for (int64_t gc = blockIdx.x; gc < nitem; gc += gridDim.x) {
+        const int64_t d = idesc[gc];
+        const int64_t c0 = d & ((int64_t(1) << kItemShift) - 1);
+        const int nc = (int)(d >> kItemShift);
+        // 63 pieces of 16 bytes per contact: c[0..5] in pieces 0-2, V from piece 4 of the row;
+        // every load of the chunk is issued before the first store (one wait, not one a piece)
+        const double2* src = reinterpret_cast<const double2*>(cbuf + c0 * kContactStride);
+        double2* dst = reinterpret_cast<double2*>(st);
+        // (a thread past the chunk reloads its last piece into the spare slot, so neither the
+        // loads nor the stores are conditional and no load waits for an earlier store)
+        const int npiece = nc * 63;
+        double2 piece[kRankPieces];
+#pragma unroll
+        for (int u = 0; u < kRankPieces; ++u) {
+            const int q = min(t + u * kRankThreads, npiece - 1), e = q / 63, pc = q - 63 * e;
+            piece[u] = src[e * (kContactStride / 2) + pc + (pc >= 3 ? 1 : 0)];
+        }
+        // the chunk's swap flags, one bit a contact, in every wave
+        const uint64_t swm = __ballot(lane < nc && swap[c0 + lane] != 0);
+#pragma unroll
+        for (int u = 0; u < kRankPieces; ++u) {
+            const int q = t + u * kRankThreads, e = q / 63, pc = q - 63 * e;
+            dst[q < npiece ? e * (kStaged / 2) + pc : kPairChunk * kStaged / 2] = piece[u];
+        }
+        __syncthreads();
+        double h = 0.0, gacc = 0.0;
+        // every thread also runs the gradient chain of entry t % 24 (kept by threads 0..23),
+        // so the loop has no branch and the loads of unrolled contacts issue together
+#pragma unroll 4
+        for (int e = 0; e < nc; ++e) {
+            const double* S = st + e * kStaged;
+            // body order [ij0 | ij1]: halves swapped when body I is ij1
+            const bool sw = (swm >> e) & 1;
+            const double* Vr = S + 6 + (sw ? rB : rA);
+            const double* Vc = S + 6 + (sw ? cB : cA);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) h += S[k] * (Vr[24 * k] * Vc[24 * k]);
+            gacc += S[5] * S[6 + 96 + (sw ? gB : gA)];
+        }
+        double* pp = part + gc * kPartStride;
+        if (ent) {
+            pp[24 * r + c] = h;
+            pp[24 * c + r] = h;
+        }
+        if (t < 24) pp[576 + t] = gacc;
+        __syncthreads();  // the next item's staging overwrites the contacts
     }
 }
 
@@ -830,8 +903,23 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         }
     }
     if (!dev_nact && n_active_dev && (rc = zero_fill(s, n_active_dev, sizeof(int64_t)))) return rc;
+    const bool dense = P->psd >= 2;
+    const int stride = dense ? kDenseStride : kContactStride;
     if (dn) {
         nact = dev_nact ? n : 0;  // an upper bound: the stages below read the count on the device
+        // The bound is the capacity of the candidate lists, which a superset swept by a very
+        // long step can make many times the contacts: past the tuning's contact_bound_mib of
+        // stored contacts the active count is read back and sizes the buffers instead (never
+        // while a graph is recorded: that iteration then runs eagerly)
+        const int mib = tuning().contact_bound_mib;
+        if ((double)nact * (8.0 * stride + 9.0) > (mib > 0 ? mib : kGhBoundMib) * 1048576.0) {
+            if (capturing()) return ABD_ERR_CAPACITY;
+            int64_t* hv;
+            if ((rc = arena_host(sizeof(int64_t), (void**)&hv))) return rc;
+            ABD_CHECK(hipMemcpyAsync(hv, dev_nact, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            ABD_CHECK(hipStreamSynchronize(s));
+            nact = hv[0];
+        }
     } else {
         // the active count and the ordering flag in one read-back
         int64_t* hv;
@@ -854,8 +942,6 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         double* cbuf;
         int64_t* ckey;
         uint8_t* swp;
-        const bool dense = P->psd >= 2;
-        const int stride = dense ? kDenseStride : kContactStride;
         if ((rc = arena_get(kSlotContact, (sizeof(double) * stride + sizeof(int64_t) + 1) * (size_t)nact,
                             (void**)&cbuf)))
             return rc;
@@ -882,12 +968,12 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         const int64_t runs = 2 * nij < nact ? 2 * nij : nact;
         const int64_t cap_items = nact / kPairChunk + runs + 1;
         int64_t* pcnt;
-        if ((rc = arena_get(kSlotPairChunk, sizeof(int64_t) * (2 * (size_t)nij + 2) + sizeof(int32_t) * cap_items,
+        if ((rc = arena_get(kSlotPairChunk, sizeof(int64_t) * (2 * (size_t)nij + 2 + cap_items),
                             (void**)&pcnt)))
             return rc;
         int64_t* pcoff = pcnt + nij;
         int64_t* ptotal = pcoff + nij;
-        int32_t* ipair = reinterpret_cast<int32_t*>(ptotal + 2);
+        int64_t* idesc = ptotal + 2;
         double* part;
         if ((rc = arena_get(kSlotPairPart, sizeof(double) * kPartStride * (size_t)cap_items, (void**)&part)))
             return rc;
@@ -896,19 +982,18 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         ABD_CHECK_LAUNCH();
         if ((rc = scan_exclusive_i64(s, pcnt, pcoff, nij, ptotal))) return rc;
         hipLaunchKernelGGL(pair_items_kernel, dim3(grid_for(nij, kBlock / kWave)), dim3(kBlock), 0, s, nij,
-                           (const int64_t*)pcnt,
-                           (const int64_t*)pcoff, ipair);
+                           (const int64_t*)pcnt, (const int64_t*)pcoff, (const int64_t*)pseg, idesc);
         ABD_CHECK_LAUNCH();
         // timed: the partial sums (each stored contact read once, its body-order flag once)
         ph = prof_start_ext(kProfPairReduce, dn ? 1.0 : (8.0 * stride + 1) * nact);
         if (dense)
-            prof_launch(ph, pair_partial_kernel<true>, dim3(kPairGrid), dim3(kPairThreads), 0, s, nij,
-                        (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg, (const int64_t*)pcoff,
-                        (const int32_t*)ipair, (const int64_t*)ptotal, part);
+            prof_launch(ph, pair_partial_dense_kernel, dim3(kPairGrid), dim3(kPairThreads), 0, s,
+                        (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)idesc, (const int64_t*)ptotal,
+                        part);
         else
-            prof_launch(ph, pair_partial_kernel<false>, dim3(kPairGrid), dim3(kPairThreads), 0, s, nij,
-                        (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)pseg, (const int64_t*)pcoff,
-                        (const int32_t*)ipair, (const int64_t*)ptotal, part);
+            prof_launch(ph, pair_partial_rank_kernel, dim3(kPairGrid), dim3(kRankThreads), 0, s,
+                        (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)idesc, (const int64_t*)ptotal,
+                        part);
         ABD_CHECK_LAUNCH();
         if ((rc = prof_count_bytes(ph, s, dnact, 8.0 * stride + 1, 0.0, 0.0, 0.0, true))) return rc;
         hipLaunchKernelGGL(pair_reduce_kernel, dim3(grid_for(nij, 1, kPairGrid)), dim3(kPairThreads), 0, s, nb, nij,

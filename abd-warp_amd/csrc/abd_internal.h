@@ -10,10 +10,13 @@
 
 namespace abd {
 
-#define ABD_CHECK(expr)                                   \
-    do {                                                  \
-        hipError_t _e = (expr);                           \
-        if (_e != hipSuccess) return -static_cast<int>(_e); \
+// The error code of a failed HIP call (-hipError_t); with ABD_DEBUG_ERRORS set in the
+// environment it also prints the call's site and the runtime's message to stderr.
+int hip_failure(hipError_t e, const char* file, int line, const char* expr);
+#define ABD_CHECK(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess) return ::abd::hip_failure(_e, __FILE__, __LINE__, #expr); \
     } while (0)
 #define ABD_CHECK_LAUNCH() ABD_CHECK(hipGetLastError())
 

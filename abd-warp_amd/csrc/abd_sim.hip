@@ -12,6 +12,8 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -36,7 +38,13 @@ struct DevBuf {
             ABD_CHECK(hipFree(p));
             p = nullptr;
         }
-        ABD_CHECK(hipMalloc((void**)&p, want * sizeof(T)));
+        const hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            if (std::getenv("ABD_DEBUG_ERRORS"))
+                std::fprintf(stderr, "abd: simulator buffer of %zu bytes not allocated\n", want * sizeof(T));
+            return hip_failure(e, __FILE__, __LINE__, "hipMalloc");
+        }
         cap = want;
         return ABD_OK;
     }
@@ -1210,7 +1218,7 @@ int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen) {
     const int grids[] = {4096, 16384, 65536};
     for (int c : chunks)
         for (int g : grids) {
-            AbdTuning t{c, g};
+            AbdTuning t{c, g, saved.contact_bound_mib};
             if ((rc = abd_set_tuning(&t))) break;
             float ms_min = 1e30f;
             for (int rep = 0; rep < 3 && !rc; ++rep) {

@@ -71,12 +71,17 @@ typedef void* abd_stream_t; /* hipStream_t */
  * CG's count); resid: |b - H x|^2 for a direct solve. */
 #define ABD_SOLVER_LDLT 3
 
-/* Launch shapes that change no result (no reference counterpart): candidate lists are
- * exact and emitted in sorted order whatever the tiling.  cull_chunk: primitive-pair tests
- * per chunk of abd_cull_prims (power of two, 256..65536; default 4096); cull_grid: most
- * workgroups of its chunk passes (64..2^20; default 16384).  Process-wide. */
+/* Launch shapes and buffer sizing that change no result (no reference counterpart):
+ * candidate lists are exact and emitted in sorted order whatever the tiling.  cull_chunk:
+ * primitive-pair tests per chunk of abd_cull_prims (power of two, 256..65536; default
+ * 4096); cull_grid: most workgroups of its chunk passes (64..2^20; default 16384);
+ * contact_bound_mib: the simulator's per-iteration contact buffer is sized by its candidate
+ * lists' capacity while that needs at most this many MiB (0: 8192), else by the active
+ * count read back (one host wait; such an iteration is launched, not replayed from a graph).
+ * Process-wide; a change invalidates recorded graphs. */
 typedef struct AbdTuning {
     int32_t cull_chunk, cull_grid;
+    int32_t contact_bound_mib;
 } AbdTuning;
 int abd_get_tuning(AbdTuning* tuning);
 int abd_set_tuning(const AbdTuning* tuning);

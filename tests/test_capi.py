@@ -96,9 +96,10 @@ def test_tuning_round_trip_and_bounds():
     L = _lib.load()
     t0 = _lib.AbdTuning()
     assert L.abd_get_tuning(ctypes.byref(t0)) == 0
-    assert (t0.cull_chunk, t0.cull_grid) == (4096, 16384)
+    assert (t0.cull_chunk, t0.cull_grid, t0.contact_bound_mib) == (4096, 16384, 0)
     bad = _lib.AbdTuning(3000, 16384)   # not a power of two
     assert L.abd_set_tuning(ctypes.byref(bad)) != 0
+    assert L.abd_set_tuning(ctypes.byref(_lib.AbdTuning(4096, 16384, -1))) != 0
     good = _lib.AbdTuning(8192, 4096)
     assert L.abd_set_tuning(ctypes.byref(good)) == 0
     t1 = _lib.AbdTuning()

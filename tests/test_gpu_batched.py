@@ -164,6 +164,41 @@ def test_autotuned_tiles_change_no_result():
         assert np.array_equal(A, out[0][0]) and np.array_equal(p, out[0][1]) and st == out[0][2]
 
 
+def test_contact_bound_read_back_equals_bound_sizing():
+    """Sizing the contact buffer by the active count read back (taken when the candidate
+    lists' capacity would need more than contact_bound_mib) gives the states, iteration
+    counts and statistics of sizing it by that capacity; those iterations are launched, not
+    replayed from a graph."""
+    import ctypes
+    from abd_hip import _lib, scenes
+    from abd_hip.abd import AffineBodySimulator
+    from abd_hip.const_params import ContactParams
+    L = _lib.load()
+    t0 = _lib.AbdTuning()
+    assert L.abd_get_tuning(ctypes.byref(t0)) == 0
+    spec = scenes.config_batched(n_scenes=16, seed0=41, nb_range=(2, 24))
+    cp = ContactParams(ground=spec.ground, psd=1, solver=3)
+    sized = AffineBodySimulator(spec, cp, "cuda", native=True)
+    for f in range(10):
+        sized.step(f)
+    torch.cuda.synchronize()
+    try:
+        # 1 MiB: every iteration with more than about a thousand candidates reads back
+        assert L.abd_set_tuning(ctypes.byref(_lib.AbdTuning(t0.cull_chunk, t0.cull_grid, 1))) == 0
+        read = AffineBodySimulator(spec, cp, "cuda", native=True)
+        for f in range(10):
+            read.step(f)
+        torch.cuda.synchronize()
+    finally:
+        assert L.abd_set_tuning(ctypes.byref(t0)) == 0
+    assert sized.scene_iters() == read.scene_iters()
+    assert np.array_equal(sized.states.A.cpu().numpy(), read.states.A.cpu().numpy())
+    assert np.array_equal(sized.states.p.cpu().numpy(), read.states.p.cpu().numpy())
+    assert dict(sized.stats) == dict(read.stats)
+    assert read.graph_counts()["replayed"] < sized.graph_counts()["replayed"], (read.graph_counts(),
+                                                                               sized.graph_counts())
+
+
 @pytest.mark.parametrize("solver", [3, 1])
 def test_graph_replay_equals_launches(solver):
     """Newton iterations replayed from a recorded HIP graph give the states, iteration
