@@ -742,249 +742,260 @@ __global__ __launch_bounds__(kSolveThreads) void ldlt_kernel(Bsr M, const int32_
         }
     }
     __syncthreads();
-    if (fail == 2) {  // uniform
-        U = ldlt_dyn;
-        W = ldlt_dyn + 144 * (int64_t)pos[m];
-    }
+    const bool in_lds = fail == 2;  // uniform
     __syncthreads();
     if (t == 0) fail = 0;
-    for (int64_t e = t; e < 144 * (int64_t)pos[m]; e += nt) U[e] = 0.0;
-    __syncthreads();
-    // the matrix's upper blocks into the factor (row-major 12x12): one wave per block, its
-    // lanes reading the slot's 144 doubles in storage order (16 3x3 sub-blocks, sub-block
-    // (R/3, C/3) at 9 (R/3 + 4 C/3), row-major inside)
-    {
-        const int lane = t & (kWave - 1), wv = t / kWave, nwv = nt / kWave;
-        for (int e = wv; e < nnz; e += nwv) {  // uniform over the wave
-            const int j = ecol[e];
-            const int i = last_le(lrp, m, e);
-            if (j < i || j >= m) continue;  // lower, or outside the system (0xff)
-            const double* src = M.blocks + 144 * (int64_t)eslot[e];
-            double* dst = U + 144 * (int64_t)(pos[i] + __popcll(adj[i] & bits_below(j)));
-            for (int q = lane; q < 144; q += kWave) {
-                const int sb = q / 9, w9 = q % 9;
-                dst[12 * (3 * (sb % 4) + w9 / 3) + 3 * (sb / 4) + w9 % 3] = src[q];
+    // The factorization and both solves, instantiated for a factor in LDS (its accesses then
+    // LDS instructions) and for one in the global workspace; false: a pivot was not positive.
+    auto factor_solve = [&](auto U, auto W) __attribute__((always_inline)) -> bool {
+        for (int64_t e = t; e < 144 * (int64_t)pos[m]; e += nt) U[e] = 0.0;
+        __syncthreads();
+        // the matrix's upper blocks into the factor (row-major 12x12): one wave per block, its
+        // lanes reading the slot's 144 doubles in storage order (16 3x3 sub-blocks, sub-block
+        // (R/3, C/3) at 9 (R/3 + 4 C/3), row-major inside)
+        {
+            const int lane = t & (kWave - 1), wv = t / kWave, nwv = nt / kWave;
+            for (int e = wv; e < nnz; e += nwv) {  // uniform over the wave
+                const int j = ecol[e];
+                const int i = last_le(lrp, m, e);
+                if (j < i || j >= m) continue;  // lower, or outside the system (0xff)
+                const double* src = M.blocks + 144 * (int64_t)eslot[e];
+                auto dst = U + 144 * (int64_t)(pos[i] + __popcll(adj[i] & bits_below(j)));
+                for (int q = lane; q < 144; q += kWave) {
+                    const int sb = q / 9, w9 = q % 9;
+                    dst[12 * (3 * (sb % 4) + w9 / 3) + 3 * (sb / 4) + w9 % 3] = src[q];
+                }
             }
         }
-    }
-    const int levels = nlev;
-    for (int L = 0; L < levels; ++L) {
-        if (t == 0) restm = levm[L];
-        for (;;) {
-            __syncthreads();  // the previous chunk is written back, its tables are free
-            if (t < kWave) {
-                // wave 0, lane = body: the chunk is the lowest bodies of the level not yet
-                // eliminated whose augmented columns fit kLdltCols (prefix sums in body order;
-                // a single body always fits)
-                const unsigned long long rest = restm;
-                const bool member = t < m && ((rest >> t) & 1ull);
-                const int d = member ? __popcll(bits_above(adj[t], t)) : 0;
-                const int c0 = member ? 12 * (d + 2) : 0, s0 = member ? 144 * (d * (d + 1) / 2) : 0, w0 = d;
-                int ic = c0, is = s0, iw = w0;
-                for (int off = 1; off < kWave; off <<= 1) {
-                    const int vc = __shfl_up(ic, off, kWave), vs = __shfl_up(is, off, kWave),
-                              vw = __shfl_up(iw, off, kWave);
-                    if (t >= off) {
-                        ic += vc;
-                        is += vs;
-                        iw += vw;
+        const int levels = nlev;
+        for (int L = 0; L < levels; ++L) {
+            if (t == 0) restm = levm[L];
+            for (;;) {
+                __syncthreads();  // the previous chunk is written back, its tables are free
+                if (t < kWave) {
+                    // wave 0, lane = body: the chunk is the lowest bodies of the level not yet
+                    // eliminated whose augmented columns fit kLdltCols (prefix sums in body order;
+                    // a single body always fits)
+                    const unsigned long long rest = restm;
+                    const bool member = t < m && ((rest >> t) & 1ull);
+                    const int d = member ? __popcll(bits_above(adj[t], t)) : 0;
+                    const int c0 = member ? 12 * (d + 2) : 0, s0 = member ? 144 * (d * (d + 1) / 2) : 0, w0 = d;
+                    int ic = c0, is = s0, iw = w0;
+                    for (int off = 1; off < kWave; off <<= 1) {
+                        const int vc = __shfl_up(ic, off, kWave), vs = __shfl_up(is, off, kWave),
+                                  vw = __shfl_up(iw, off, kWave);
+                        if (t >= off) {
+                            ic += vc;
+                            is += vs;
+                            iw += vw;
+                        }
                     }
-                }
-                const bool take = member && ic <= kLdltCols;
-                const unsigned long long cmask = __ballot(take);
-                const int nc = __popcll(cmask);
-                if (take) {
-                    const int q = __popcll(cmask & bits_below(t));
-                    cnode[q] = t;
-                    cstart[q] = ic - c0;
-                    sstart[q] = is - s0;
-                    woff[t] = iw - w0;
-                    if (q == nc - 1) {
-                        cstart[nc] = ic;
-                        sstart[nc] = is;
+                    const bool take = member && ic <= kLdltCols;
+                    const unsigned long long cmask = __ballot(take);
+                    const int nc = __popcll(cmask);
+                    if (take) {
+                        const int q = __popcll(cmask & bits_below(t));
+                        cnode[q] = t;
+                        cstart[q] = ic - c0;
+                        sstart[q] = is - s0;
+                        woff[t] = iw - w0;
+                        if (q == nc - 1) {
+                            cstart[nc] = ic;
+                            sstart[nc] = is;
+                        }
                     }
-                }
-                if (t == 0) {
-                    if (nc == 0) cstart[0] = sstart[0] = 0;
-                    nchunk = nc;
-                    chunkmask = cmask;
-                    restm = rest & ~cmask;
-                }
-            }
-            __syncthreads();
-            const int nc = nchunk;
-            if (nc == 0) break;  // uniform: the level is done
-            const unsigned long long cmask = chunkmask;
-            const int C = cstart[nc];
-            double a[kLdltColsPerThread][12];
-            int cln[kLdltColsPerThread], cg[kLdltColsPerThread], cc_[kLdltColsPerThread], cd[kLdltColsPerThread];
-#pragma unroll
-            for (int s = 0; s < kLdltColsPerThread; ++s) {
-                const int col = kLdltColsPerThread * t + s;
-                cln[s] = -1;
-                cg[s] = cc_[s] = cd[s] = 0;
-                if (col < C) {
-                    const int ln = last_le(cstart, nc, col);
-                    const int k = cnode[ln], within = col - cstart[ln];
-                    const int g = within / 12, c = within % 12, d = (cstart[ln + 1] - cstart[ln]) / 12 - 2;
-                    cln[s] = ln;
-                    cg[s] = g;
-                    cc_[s] = c;
-                    cd[s] = d;
-                    if (g <= d) {
-                        const double* src = U + 144 * (int64_t)(pos[k] + g) + c;
-#pragma unroll
-                        for (int r = 0; r < 12; ++r) a[s][r] = src[12 * r];
-                    } else {
-#pragma unroll
-                        for (int r = 0; r < 12; ++r) a[s][r] = r == c ? 1.0 : 0.0;
+                    if (t == 0) {
+                        if (nc == 0) cstart[0] = sstart[0] = 0;
+                        nchunk = nc;
+                        chunkmask = cmask;
+                        restm = rest & ~cmask;
                     }
-                }
-            }
-            // two pivot columns per barrier.  A thread's columns are adjacent and a body's
-            // column count is a multiple of 12, so both belong to one body and the pivot pair
-            // (cp, cp + 1) of a body is one thread's: it publishes column cp as it stands and
-            // column cp + 1 as pivot cp leaves it (the update every column gets), with both
-            // reciprocal pivots; every column then takes pivot cp, then pivot cp + 1 -- the
-            // arithmetic of one pivot per step
-#pragma unroll
-            for (int cp = 0; cp < 12; cp += 2) {
-                const int bf = (cp >> 1) & 1;
-                if (cln[0] >= 0 && cg[0] == 0 && cc_[0] == cp) {
-                    const double inv0 = 1.0 / a[0][cp];
-                    if (!(a[0][cp] > 0.0) || !isfinite(a[0][cp])) fail = 1;
-                    const double sc1 = a[1][cp] * inv0;
-                    double piv1 = 0.0;
-#pragma unroll
-                    for (int r = 0; r < 12; ++r) {
-                        const double v = r == cp ? sc1 : a[1][r] - a[0][r] * sc1;
-                        colc[cln[0]][bf][0][r] = a[0][r];
-                        colc[cln[0]][bf][1][r] = v;
-                        if (r == cp + 1) piv1 = v;
-                    }
-                    if (!(piv1 > 0.0) || !isfinite(piv1)) fail = 1;
-                    colinv[cln[0]][bf][0] = inv0;
-                    colinv[cln[0]][bf][1] = 1.0 / piv1;
                 }
                 __syncthreads();
-                if (cln[0] >= 0) {
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {  // pivot cp + h
-                        const int pc = cp + h;
-                        double f[12];
-#pragma unroll
-                        for (int r = 0; r < 12; ++r) f[r] = colc[cln[0]][bf][h][r];
-                        const double inv = colinv[cln[0]][bf][h];
-#pragma unroll
-                        for (int s = 0; s < kLdltColsPerThread; ++s)
-                            if (cln[s] >= 0) {
-                                const double sc = a[s][pc] * inv;
-#pragma unroll
-                                for (int r = 0; r < 12; ++r)
-                                    if (r != pc) a[s][r] -= f[r] * sc;
-                                a[s][pc] = sc;
-                            }
+                const int nc = nchunk;
+                if (nc == 0) break;  // uniform: the level is done
+                const unsigned long long cmask = chunkmask;
+                const int C = cstart[nc];
+                double a[kLdltColsPerThread][12];
+                int cln[kLdltColsPerThread], cg[kLdltColsPerThread], cc_[kLdltColsPerThread], cd[kLdltColsPerThread];
+    #pragma unroll
+                for (int s = 0; s < kLdltColsPerThread; ++s) {
+                    const int col = kLdltColsPerThread * t + s;
+                    cln[s] = -1;
+                    cg[s] = cc_[s] = cd[s] = 0;
+                    if (col < C) {
+                        const int ln = last_le(cstart, nc, col);
+                        const int k = cnode[ln], within = col - cstart[ln];
+                        const int g = within / 12, c = within % 12, d = (cstart[ln + 1] - cstart[ln]) / 12 - 2;
+                        cln[s] = ln;
+                        cg[s] = g;
+                        cc_[s] = c;
+                        cd[s] = d;
+                        if (g <= d) {
+                            const auto src = U + 144 * (int64_t)(pos[k] + g) + c;
+    #pragma unroll
+                            for (int r = 0; r < 12; ++r) a[s][r] = src[12 * r];
+                        } else {
+    #pragma unroll
+                            for (int r = 0; r < 12; ++r) a[s][r] = r == c ? 1.0 : 0.0;
+                        }
                     }
                 }
-            }
-            if (fail) {  // uniform: written before the last pivot column's barrier
-                if (t == 0) status[sidx] = kLdltFailed;
-                return;
-            }
-            // W (g = 1..d) to the chunk's area, P^-1 (g = d + 1) over P
-#pragma unroll
-            for (int s = 0; s < kLdltColsPerThread; ++s)
-                if (cln[s] >= 0 && cg[s] >= 1) {
-                    const int k = cnode[cln[s]];
-                    double* dst = cg[s] <= cd[s] ? W + 144 * (int64_t)(woff[k] + cg[s] - 1) + cc_[s]
-                                                 : U + 144 * (int64_t)pos[k] + cc_[s];
-#pragma unroll
-                    for (int r = 0; r < 12; ++r) dst[12 * r] = a[s][r];
+                // two pivot columns per barrier.  A thread's columns are adjacent and a body's
+                // column count is a multiple of 12, so both belong to one body and the pivot pair
+                // (cp, cp + 1) of a body is one thread's: it publishes column cp as it stands and
+                // column cp + 1 as pivot cp leaves it (the update every column gets), with both
+                // reciprocal pivots; every column then takes pivot cp, then pivot cp + 1 -- the
+                // arithmetic of one pivot per step
+    #pragma unroll
+                for (int cp = 0; cp < 12; cp += 2) {
+                    const int bf = (cp >> 1) & 1;
+                    if (cln[0] >= 0 && cg[0] == 0 && cc_[0] == cp) {
+                        const double inv0 = 1.0 / a[0][cp];
+                        if (!(a[0][cp] > 0.0) || !isfinite(a[0][cp])) fail = 1;
+                        const double sc1 = a[1][cp] * inv0;
+                        double piv1 = 0.0;
+    #pragma unroll
+                        for (int r = 0; r < 12; ++r) {
+                            const double v = r == cp ? sc1 : a[1][r] - a[0][r] * sc1;
+                            colc[cln[0]][bf][0][r] = a[0][r];
+                            colc[cln[0]][bf][1][r] = v;
+                            if (r == cp + 1) piv1 = v;
+                        }
+                        if (!(piv1 > 0.0) || !isfinite(piv1)) fail = 1;
+                        colinv[cln[0]][bf][0] = inv0;
+                        colinv[cln[0]][bf][1] = 1.0 / piv1;
+                    }
+                    __syncthreads();
+                    if (cln[0] >= 0) {
+    #pragma unroll
+                        for (int h = 0; h < 2; ++h) {  // pivot cp + h
+                            const int pc = cp + h;
+                            double f[12];
+    #pragma unroll
+                            for (int r = 0; r < 12; ++r) f[r] = colc[cln[0]][bf][h][r];
+                            const double inv = colinv[cln[0]][bf][h];
+    #pragma unroll
+                            for (int s = 0; s < kLdltColsPerThread; ++s)
+                                if (cln[s] >= 0) {
+                                    const double sc = a[s][pc] * inv;
+    #pragma unroll
+                                    for (int r = 0; r < 12; ++r)
+                                        if (r != pc) a[s][r] -= f[r] * sc;
+                                    a[s][pc] = sc;
+                                }
+                        }
+                    }
                 }
-            __syncthreads();
-            // Schur complement: target (i, j), i <= j, updated by the thread of its lowest
-            // contributor in the chunk, the contributors in ascending order
-            for (int w = t; w < sstart[nc]; w += nt) {
-                const int ln = last_le(sstart, nc, w);
-                const int k = cnode[ln], within = w - sstart[ln];
-                const int pidx = within / 144, q = within % 144;
-                int v = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
-                while (v * (v + 1) / 2 > pidx) --v;
-                while ((v + 1) * (v + 2) / 2 <= pidx) ++v;
-                const int u = pidx - v * (v + 1) / 2;
-                const int i = mlist[k][u], j = mlist[k][v];
-                const unsigned long long contrib = cmask & colm[i] & colm[j];
-                if (__builtin_ctzll(contrib) != k) continue;
-                double* tgt = U + 144 * (int64_t)(pos[i] + __popcll(adj[i] & bits_below(j))) + q;
-                double val = *tgt;
-                for (unsigned long long bt = contrib; bt; bt &= bt - 1ull) {
-                    const int k2 = __builtin_ctzll(bt);
-                    const unsigned long long ab = bits_above(adj[k2], k2);
-                    const double* Aki = U + 144 * (int64_t)(pos[k2] + 1 + __popcll(ab & bits_below(i))) + q / 12;
-                    const double* Wj = W + 144 * (int64_t)(woff[k2] + __popcll(ab & bits_below(j))) + q % 12;
+                if (fail) {  // uniform: written before the last pivot column's barrier
+                    if (t == 0) status[sidx] = kLdltFailed;
+                    return false;
+                }
+                // W (g = 1..d) to the chunk's area, P^-1 (g = d + 1) over P
+    #pragma unroll
+                for (int s = 0; s < kLdltColsPerThread; ++s)
+                    if (cln[s] >= 0 && cg[s] >= 1) {
+                        const int k = cnode[cln[s]];
+                        auto dst = cg[s] <= cd[s] ? W + 144 * (int64_t)(woff[k] + cg[s] - 1) + cc_[s]
+                                                     : U + 144 * (int64_t)pos[k] + cc_[s];
+    #pragma unroll
+                        for (int r = 0; r < 12; ++r) dst[12 * r] = a[s][r];
+                    }
+                __syncthreads();
+                // Schur complement: target (i, j), i <= j, updated by the thread of its lowest
+                // contributor in the chunk, the contributors in ascending order
+                for (int w = t; w < sstart[nc]; w += nt) {
+                    const int ln = last_le(sstart, nc, w);
+                    const int k = cnode[ln], within = w - sstart[ln];
+                    const int pidx = within / 144, q = within % 144;
+                    int v = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
+                    while (v * (v + 1) / 2 > pidx) --v;
+                    while ((v + 1) * (v + 2) / 2 <= pidx) ++v;
+                    const int u = pidx - v * (v + 1) / 2;
+                    const int i = mlist[k][u], j = mlist[k][v];
+                    const unsigned long long contrib = cmask & colm[i] & colm[j];
+                    if (__builtin_ctzll(contrib) != k) continue;
+                    auto tgt = U + 144 * (int64_t)(pos[i] + __popcll(adj[i] & bits_below(j))) + q;
+                    double val = *tgt;
+                    for (unsigned long long bt = contrib; bt; bt &= bt - 1ull) {
+                        const int k2 = __builtin_ctzll(bt);
+                        const unsigned long long ab = bits_above(adj[k2], k2);
+                        const auto Aki = U + 144 * (int64_t)(pos[k2] + 1 + __popcll(ab & bits_below(i))) + q / 12;
+                        const auto Wj = W + 144 * (int64_t)(woff[k2] + __popcll(ab & bits_below(j))) + q % 12;
+                        double acc = 0.0;
+    #pragma unroll
+                        for (int r = 0; r < 12; ++r) acc += Aki[12 * r] * Wj[12 * r];
+                        val -= acc;
+                    }
+                    *tgt = val;
+                }
+                // forward solve: y_i -= W_i^T y_k over the chunk's bodies k coupled to i, ascending
+                for (int w = t; w < n; w += nt) {
+                    const int i = w / 12, a2 = w % 12;
+                    unsigned long long bt = cmask & colm[i];
+                    if (!bt) continue;
+                    double val = y[w];
+                    for (; bt; bt &= bt - 1ull) {
+                        const int k2 = __builtin_ctzll(bt);
+                        const unsigned long long ab = bits_above(adj[k2], k2);
+                        const auto Wi = W + 144 * (int64_t)(woff[k2] + __popcll(ab & bits_below(i))) + a2;
+                        double acc = 0.0;
+    #pragma unroll
+                        for (int r = 0; r < 12; ++r) acc += Wi[12 * r] * y[12 * k2 + r];
+                        val -= acc;
+                    }
+                    y[w] = val;
+                }
+                // z_k = P^-1 y_k for the chunk's bodies
+                for (int w = t; w < 12 * nc; w += nt) {
+                    const int k = cnode[w / 12], a2 = w % 12;
+                    const auto Di = U + 144 * (int64_t)pos[k] + 12 * a2;
                     double acc = 0.0;
-#pragma unroll
-                    for (int r = 0; r < 12; ++r) acc += Aki[12 * r] * Wj[12 * r];
-                    val -= acc;
+    #pragma unroll
+                    for (int r = 0; r < 12; ++r) acc += Di[r] * y[12 * k + r];
+                    z[12 * k + a2] = acc;
                 }
-                *tgt = val;
+                __syncthreads();
+                // rows of the factor: W
+    #pragma unroll
+                for (int s = 0; s < kLdltColsPerThread; ++s)
+                    if (cln[s] >= 0 && cg[s] >= 1 && cg[s] <= cd[s]) {
+                        auto dst = U + 144 * (int64_t)(pos[cnode[cln[s]]] + cg[s]) + cc_[s];
+    #pragma unroll
+                        for (int r = 0; r < 12; ++r) dst[12 * r] = a[s][r];
+                    }
             }
-            // forward solve: y_i -= W_i^T y_k over the chunk's bodies k coupled to i, ascending
+        }
+        // backward solve, levels from the top
+        for (int L = levels - 1; L >= 0; --L) {
+            __syncthreads();
             for (int w = t; w < n; w += nt) {
-                const int i = w / 12, a2 = w % 12;
-                unsigned long long bt = cmask & colm[i];
-                if (!bt) continue;
-                double val = y[w];
-                for (; bt; bt &= bt - 1ull) {
-                    const int k2 = __builtin_ctzll(bt);
-                    const unsigned long long ab = bits_above(adj[k2], k2);
-                    const double* Wi = W + 144 * (int64_t)(woff[k2] + __popcll(ab & bits_below(i))) + a2;
-                    double acc = 0.0;
-#pragma unroll
-                    for (int r = 0; r < 12; ++r) acc += Wi[12 * r] * y[12 * k2 + r];
-                    val -= acc;
-                }
-                y[w] = val;
-            }
-            // z_k = P^-1 y_k for the chunk's bodies
-            for (int w = t; w < 12 * nc; w += nt) {
-                const int k = cnode[w / 12], a2 = w % 12;
-                const double* Di = U + 144 * (int64_t)pos[k] + 12 * a2;
+                const int k = w / 12, a2 = w % 12;
+                if (lvl[k] != L) continue;
                 double acc = 0.0;
-#pragma unroll
-                for (int r = 0; r < 12; ++r) acc += Di[r] * y[12 * k + r];
-                z[12 * k + a2] = acc;
-            }
-            __syncthreads();
-            // rows of the factor: W
-#pragma unroll
-            for (int s = 0; s < kLdltColsPerThread; ++s)
-                if (cln[s] >= 0 && cg[s] >= 1 && cg[s] <= cd[s]) {
-                    double* dst = U + 144 * (int64_t)(pos[cnode[cln[s]]] + cg[s]) + cc_[s];
-#pragma unroll
-                    for (int r = 0; r < 12; ++r) dst[12 * r] = a[s][r];
+                int u = 0;
+                for (unsigned long long bt = bits_above(adj[k], k); bt; bt &= bt - 1ull, ++u) {
+                    const int i = __builtin_ctzll(bt);
+                    const auto Wu = U + 144 * (int64_t)(pos[k] + 1 + u) + 12 * a2;
+                    double p = 0.0;
+    #pragma unroll
+                    for (int c = 0; c < 12; ++c) p += Wu[c] * y[12 * i + c];
+                    acc += p;
                 }
-        }
-    }
-    // backward solve, levels from the top
-    for (int L = levels - 1; L >= 0; --L) {
-        __syncthreads();
-        for (int w = t; w < n; w += nt) {
-            const int k = w / 12, a2 = w % 12;
-            if (lvl[k] != L) continue;
-            double acc = 0.0;
-            int u = 0;
-            for (unsigned long long bt = bits_above(adj[k], k); bt; bt &= bt - 1ull, ++u) {
-                const int i = __builtin_ctzll(bt);
-                const double* Wu = U + 144 * (int64_t)(pos[k] + 1 + u) + 12 * a2;
-                double p = 0.0;
-#pragma unroll
-                for (int c = 0; c < 12; ++c) p += Wu[c] * y[12 * i + c];
-                acc += p;
+                y[w] = z[w] - acc;
             }
-            y[w] = z[w] - acc;
         }
+        return true;
+    };
+    bool ok;
+    if (in_lds) {
+        typedef __attribute__((address_space(3))) double lds_double;
+        lds_double* Ul = (lds_double*)ldlt_dyn;
+        ok = factor_solve(Ul, Ul + 144 * (int64_t)pos[m]);
+    } else {
+        ok = factor_solve(U, W);
     }
+    if (!ok) return;
     __syncthreads();
     for (int i = t; i < n; i += nt) xg[i] = y[i];
     double rr = 0.0;
