@@ -484,24 +484,50 @@ __global__ __launch_bounds__(kPairThreads) void pair_partial_dense_kernel(const 
     }
 }
 
-// stage D2 for rank-one contacts (the stored form of contact_terms): one work item per
-// workgroup, the chunk's contacts staged into LDS at once (without the pad), then one
-// thread per entry of the upper triangle of the 24x24 block summing c_k (V_k[r] V_k[c]) over
-// the chunk's contacts in order, and threads 0..23 the gradient; an entry of the lower
-// triangle is the same sum (V_k[r] V_k[c] == V_k[c] V_k[r]), written from its mirror.
+// stage D2 for rank-one contacts (the stored form of contact_terms): the items of a
+// workgroup (grid-stride) cut into batches of kRankBatch contacts, each staged into LDS
+// (without the pad) from registers loaded while the previous batch was summed; one thread
+// per entry of the upper triangle of the 24x24 block sums c_k (V_k[r] V_k[c]) over the
+// item's contacts in order, and threads 0..23 the gradient; an entry of the lower triangle
+// is the same sum (V_k[r] V_k[c] == V_k[c] V_k[r]), written from its mirror.
 constexpr int kRankThreads = 320;                   // 300 upper-triangle entries, 5 waves
+constexpr int kRankBatch = 32;                      // contacts staged at a time
 constexpr int kStaged = 126;                        // doubles per staged contact: c[5], b', V[5][24]
-constexpr int kRankPieces = (kPairChunk * 63 + kRankThreads - 1) / kRankThreads;  // 16-byte loads a thread
+constexpr int kRankPieces = (kRankBatch * 63 + kRankThreads - 1) / kRankThreads;  // 16-byte loads a thread
 static_assert(kContactStride == 128 && kCoefOff == 0 && kB1Off == 5 && kVOff == 8, "staged layout");
+static_assert(kPairChunk <= kWave, "one swap bit a lane");
+typedef double dbl2 __attribute__((ext_vector_type(2)));  // a 16-byte piece (kept in registers)
+
+// A thread's pieces of the batch of n contacts from src (63 pieces of 16 bytes a contact:
+// c[0..5] in pieces 0-2, V from piece 4 of the stored row), piece U onwards.  A thread past
+// the batch reloads its last piece (stored to a spare slot), so the loads are unconditional
+// and issue together; the recursion keeps every index a constant (the pieces in registers).
+template <int U>
+__device__ __forceinline__ void rank_batch_load(const dbl2* __restrict__ src, int np, int t,
+                                                dbl2 (&piece)[kRankPieces]) {
+    if constexpr (U < kRankPieces) {
+        const int q = min(t + U * kRankThreads, np - 1), e = q / 63, pc = q - 63 * e;
+        piece[U] = src[e * (kContactStride / 2) + pc + (pc >= 3 ? 1 : 0)];
+        rank_batch_load<U + 1>(src, np, t, piece);
+    }
+}
+// The pieces into the staged batch (the pieces past it into the spare slot).
+template <int U>
+__device__ __forceinline__ void rank_batch_store(dbl2* dst, int np, int t, const dbl2 (&piece)[kRankPieces]) {
+    if constexpr (U < kRankPieces) {
+        const int q = t + U * kRankThreads, e = q / 63, pc = q - 63 * e;
+        dst[q < np ? e * (kStaged / 2) + pc : kRankBatch * kStaged / 2] = piece[U];
+        rank_batch_store<U + 1>(dst, np, t, piece);
+    }
+}
 
 __global__ __launch_bounds__(kRankThreads) void pair_partial_rank_kernel(const double* __restrict__ cbuf,
                                                                          const uint8_t* __restrict__ swap,
                                                                          const int64_t* __restrict__ idesc,
                                                                          const int64_t* __restrict__ total,
                                                                          double* __restrict__ part) {
-    static_assert(kPairChunk <= kWave, "one swap bit a lane");
-    // the staged contacts, then a 16-byte slot the pieces past the chunk are stored to
-    __shared__ __align__(16) double st[kPairChunk * kStaged + 2];
+    // the staged batch, then a 16-byte slot the pieces past the batch are stored to
+    __shared__ __align__(16) double st[kRankBatch * kStaged + 2];
     const int t = threadIdx.x, lane = t % kWave;
     // this thread's entry (r, c), r <= c, of the upper triangle (rows of 24 - r entries)
     int r = 0, j = t;
@@ -516,52 +542,69 @@ __global__ __launch_bounds__(kRankThreads) void pair_partial_rank_kernel(const d
     const int tg = t % 24;
     const int rA = r, rB = (r + 12) % 24, cA = c, cB = (c + 12) % 24, gA = tg, gB = (tg + 12) % 24;
     const int64_t nitem = *total;
-    for (int64_t gc = blockIdx.x; gc < nitem; gc += gridDim.x) {
-        const int64_t d = idesc[gc];
-        const int64_t c0 = d & ((int64_t(1) << kItemShift) - 1);
-        const int nc = (int)(d >> kItemShift);
-        // 63 pieces of 16 bytes per contact: c[0..5] in pieces 0-2, V from piece 4 of the row;
-        // every load of the chunk is issued before the first store (one wait, not one a piece)
-        const double2* src = reinterpret_cast<const double2*>(cbuf + c0 * kContactStride);
-        double2* dst = reinterpret_cast<double2*>(st);
-        // (a thread past the chunk reloads its last piece into the spare slot, so neither the
-        // loads nor the stores are conditional and no load waits for an earlier store)
-        const int npiece = nc * 63;
-        double2 piece[kRankPieces];
-#pragma unroll
-        for (int u = 0; u < kRankPieces; ++u) {
-            const int q = min(t + u * kRankThreads, npiece - 1), e = q / 63, pc = q - 63 * e;
-            piece[u] = src[e * (kContactStride / 2) + pc + (pc >= 3 ? 1 : 0)];
-        }
-        // the chunk's swap flags, one bit a contact, in every wave
-        const uint64_t swm = __ballot(lane < nc && swap[c0 + lane] != 0);
-#pragma unroll
-        for (int u = 0; u < kRankPieces; ++u) {
-            const int q = t + u * kRankThreads, e = q / 63, pc = q - 63 * e;
-            dst[q < npiece ? e * (kStaged / 2) + pc : kPairChunk * kStaged / 2] = piece[u];
-        }
+    int64_t gc = blockIdx.x;
+    if (gc >= nitem) return;  // uniform
+    const int64_t kMask = (int64_t(1) << kItemShift) - 1;
+    int64_t c0 = idesc[gc] & kMask;
+    int nc = (int)(idesc[gc] >> kItemShift), b0 = 0;
+    dbl2 piece[kRankPieces];
+    rank_batch_load<0>(reinterpret_cast<const dbl2*>(cbuf + c0 * kContactStride), 63 * min(nc, kRankBatch), t,
+                       piece);
+    // the item's swap flags, one bit a contact, in every wave
+    uint64_t swm = __ballot(lane < nc && swap[c0 + lane] != 0);
+    double h = 0.0, gacc = 0.0;
+    for (;;) {
+        const int nbat = min(kRankBatch, nc - b0);
+        __syncthreads();  // the previous batch is summed: its LDS is free
+        rank_batch_store<0>(reinterpret_cast<dbl2*>(st), 63 * nbat, t, piece);
         __syncthreads();
-        double h = 0.0, gacc = 0.0;
+        // the next batch: the rest of this item, else the workgroup's next item; its loads
+        // are in flight while this batch is summed
+        int64_t gn = gc, cn = c0;
+        int nn = nc, bn = b0 + kRankBatch;
+        if (bn >= nc) {
+            gn = gc + gridDim.x;
+            bn = 0;
+            if (gn < nitem) {
+                const int64_t dn_ = idesc[gn];
+                cn = dn_ & kMask;
+                nn = (int)(dn_ >> kItemShift);
+            }
+        }
+        // (past the workgroup's last item this batch is loaded again, unused: the loads stay
+        // unconditional and the pieces in registers)
+        const bool more = gn < nitem;
+        rank_batch_load<0>(reinterpret_cast<const dbl2*>(cbuf + (more ? cn + bn : c0 + b0) * kContactStride),
+                           63 * (more ? min(nn - bn, kRankBatch) : nbat), t, piece);
+        const uint8_t swn = more && gn != gc && lane < nn ? swap[cn + lane] : 0;
         // every thread also runs the gradient chain of entry t % 24 (kept by threads 0..23),
         // so the loop has no branch and the loads of unrolled contacts issue together
 #pragma unroll 4
-        for (int e = 0; e < nc; ++e) {
+        for (int e = 0; e < nbat; ++e) {
             const double* S = st + e * kStaged;
             // body order [ij0 | ij1]: halves swapped when body I is ij1
-            const bool sw = (swm >> e) & 1;
+            const bool sw = (swm >> (b0 + e)) & 1;
             const double* Vr = S + 6 + (sw ? rB : rA);
             const double* Vc = S + 6 + (sw ? cB : cA);
 #pragma unroll
             for (int k = 0; k < 5; ++k) h += S[k] * (Vr[24 * k] * Vc[24 * k]);
             gacc += S[5] * S[6 + 96 + (sw ? gB : gA)];
         }
-        double* pp = part + gc * kPartStride;
-        if (ent) {
-            pp[24 * r + c] = h;
-            pp[24 * c + r] = h;
+        if (gn != gc) {  // uniform: the item is summed
+            double* pp = part + gc * kPartStride;
+            if (ent) {
+                pp[24 * r + c] = h;
+                pp[24 * c + r] = h;
+            }
+            if (t < 24) pp[576 + t] = gacc;
+            h = gacc = 0.0;
+            if (gn >= nitem) break;
+            swm = __ballot(swn != 0);
         }
-        if (t < 24) pp[576 + t] = gacc;
-        __syncthreads();  // the next item's staging overwrites the contacts
+        gc = gn;
+        c0 = cn;
+        nc = nn;
+        b0 = bn;
     }
 }
 

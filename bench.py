@@ -315,6 +315,8 @@ def main():
                     help="batched native driver: launch every kernel (no HIP graph replay of Newton iterations)")
     ap.add_argument("--roof-frames", type=int, default=None,
                     help="frames after the timed ones with HIP events on the dominant kernel (default: --steps)")
+    ap.add_argument("--roof-stages", default="",
+                    help="comma-separated profiled stages also timed in the roofline frames (e.g. gh_pair_reduce)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with one 256 MiB device copy (reference kernel for HBM counter scaling)")
     ap.add_argument("--python-driver", action="store_true", help="run the Python mirror of the driver")
@@ -377,6 +379,11 @@ def main():
         cands_dom = [k for _, k in ranked[:2]]
         if "solve" in aprof and "solve" not in cands_dom and aprof["solve"][2] > 0:
             cands_dom.append("solve")
+        for k in filter(None, a.roof_stages.split(",")):
+            if k not in _lib.PROF_NAMES:
+                raise SystemExit(f"unknown stage {k}: one of {_lib.PROF_NAMES}")
+            if k not in cands_dom and k in aprof and aprof[k][1] > 0:
+                cands_dom.append(k)
         dom = cands_dom[0] if cands_dom else None
     first = frame
     torch.cuda.synchronize()
