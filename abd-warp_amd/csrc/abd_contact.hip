@@ -420,12 +420,19 @@ __global__ __launch_bounds__(kBlock) void pair_chunks_kernel(int64_t nij, const 
 
 // stage D's items: pair ii owns the items [coff[ii], coff[ii] + cnt[ii]), its PT run's chunks
 // then its EE run's; item i's descriptor is its first contact | its contact count << 56.
+// The order the rank-one partial sums run them in (slot -> item, odesc / oitem) takes the full
+// chunks first (counted in counters[0], from the front) and the others after them (counters[1],
+// from the back): the longest sums start first.  Where an item lands in it changes no result.
 // One wave per pair, its lanes striding over the pair's items.
 constexpr int kItemShift = 56;
 __global__ __launch_bounds__(kBlock) void pair_items_kernel(int64_t nij, const int64_t* __restrict__ cnt,
                                                             const int64_t* __restrict__ coff,
                                                             const int64_t* __restrict__ pseg,
-                                                            int64_t* __restrict__ idesc) {
+                                                            const int64_t* __restrict__ total,
+                                                            int64_t* __restrict__ idesc, int* __restrict__ counters,
+                                                            int64_t* __restrict__ odesc,
+                                                            int64_t* __restrict__ oitem) {
+    const int64_t ntot = *total;
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nw = (int64_t)gridDim.x * (kBlock / kWave);
     for (int64_t ii = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; ii < nij; ii += nw) {
@@ -438,7 +445,13 @@ __global__ __launch_bounds__(kBlock) void pair_items_kernel(int64_t nij, const i
             const int64_t c0 = pt ? p0 + c * kPairChunk : e0 + (c - npc) * kPairChunk;
             const int64_t end = pt ? p1 : e1;
             const int64_t nc = end - c0 < kPairChunk ? end - c0 : kPairChunk;
-            idesc[o + c] = c0 | (nc << kItemShift);
+            const int64_t d = c0 | (nc << kItemShift);
+            idesc[o + c] = d;
+            const bool full = nc == kPairChunk;
+            const int64_t k = atomicAdd(counters + (full ? 0 : 1), 1);
+            const int64_t slot = full ? k : ntot - 1 - k;
+            odesc[slot] = d;
+            oitem[slot] = o + c;
         }
     }
 }
@@ -497,6 +510,8 @@ constexpr int kRankPieces = (kRankBatch * 63 + kRankThreads - 1) / kRankThreads;
 static_assert(kContactStride == 128 && kCoefOff == 0 && kB1Off == 5 && kVOff == 8, "staged layout");
 static_assert(kPairChunk <= kWave, "one swap bit a lane");
 typedef double dbl2 __attribute__((ext_vector_type(2)));  // a 16-byte piece (kept in registers)
+// read-only data reached through the constant address space: uniform loads become scalar ones
+typedef __attribute__((address_space(4))) const double kconst_double;
 
 // A thread's pieces of the batch of n contacts from src (63 pieces of 16 bytes a contact:
 // c[0..5] in pieces 0-2, V from piece 4 of the stored row), piece U onwards.  A thread past
@@ -521,9 +536,10 @@ __device__ __forceinline__ void rank_batch_store(dbl2* dst, int np, int t, const
     }
 }
 
-__global__ __launch_bounds__(kRankThreads) void pair_partial_rank_kernel(const double* __restrict__ cbuf,
+__global__ __launch_bounds__(kRankThreads, 5) void pair_partial_rank_kernel(const double* __restrict__ cbuf,
                                                                          const uint8_t* __restrict__ swap,
-                                                                         const int64_t* __restrict__ idesc,
+                                                                         const int64_t* __restrict__ odesc,
+                                                                         const int64_t* __restrict__ oitem,
                                                                          const int64_t* __restrict__ total,
                                                                          double* __restrict__ part) {
     // the staged batch, then a 16-byte slot the pieces past the batch are stored to
@@ -542,11 +558,11 @@ __global__ __launch_bounds__(kRankThreads) void pair_partial_rank_kernel(const d
     const int tg = t % 24;
     const int rA = r, rB = (r + 12) % 24, cA = c, cB = (c + 12) % 24, gA = tg, gB = (tg + 12) % 24;
     const int64_t nitem = *total;
-    int64_t gc = blockIdx.x;
+    int64_t gc = blockIdx.x;  // the execution slot (its item: oitem[gc])
     if (gc >= nitem) return;  // uniform
     const int64_t kMask = (int64_t(1) << kItemShift) - 1;
-    int64_t c0 = idesc[gc] & kMask;
-    int nc = (int)(idesc[gc] >> kItemShift), b0 = 0;
+    int64_t c0 = odesc[gc] & kMask;
+    int nc = (int)(odesc[gc] >> kItemShift), b0 = 0;
     dbl2 piece[kRankPieces];
     rank_batch_load<0>(reinterpret_cast<const dbl2*>(cbuf + c0 * kContactStride), 63 * min(nc, kRankBatch), t,
                        piece);
@@ -566,7 +582,7 @@ __global__ __launch_bounds__(kRankThreads) void pair_partial_rank_kernel(const d
             gn = gc + gridDim.x;
             bn = 0;
             if (gn < nitem) {
-                const int64_t dn_ = idesc[gn];
+                const int64_t dn_ = odesc[gn];
                 cn = dn_ & kMask;
                 nn = (int)(dn_ >> kItemShift);
             }
@@ -582,16 +598,18 @@ __global__ __launch_bounds__(kRankThreads) void pair_partial_rank_kernel(const d
 #pragma unroll 4
         for (int e = 0; e < nbat; ++e) {
             const double* S = st + e * kStaged;
+            // the coefficients are the same for every lane: scalar loads from the stored row
+            const kconst_double* K = (const kconst_double*)cbuf + (c0 + b0 + e) * kContactStride;
             // body order [ij0 | ij1]: halves swapped when body I is ij1
             const bool sw = (swm >> (b0 + e)) & 1;
             const double* Vr = S + 6 + (sw ? rB : rA);
             const double* Vc = S + 6 + (sw ? cB : cA);
 #pragma unroll
-            for (int k = 0; k < 5; ++k) h += S[k] * (Vr[24 * k] * Vc[24 * k]);
-            gacc += S[5] * S[6 + 96 + (sw ? gB : gA)];
+            for (int k = 0; k < 5; ++k) h += K[k] * (Vr[24 * k] * Vc[24 * k]);
+            gacc += K[5] * S[6 + 96 + (sw ? gB : gA)];
         }
         if (gn != gc) {  // uniform: the item is summed
-            double* pp = part + gc * kPartStride;
+            double* pp = part + oitem[gc] * kPartStride;
             if (ent) {
                 pp[24 * r + c] = h;
                 pp[24 * c + r] = h;
@@ -781,7 +799,10 @@ __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdP
 __global__ void gh_zero_kernel(int* __restrict__ err, uint8_t* __restrict__ a8, int64_t n8,
                                int64_t* __restrict__ a64, int64_t n64) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *err = 0;
+    if (i == 0) {
+        err[0] = 0;
+        err[1] = err[2] = 0;  // stage D's counters of full and partial chunks
+    }
     if (i < n8) a8[i] = 0;
     if (i < n64) a64[i] = 0;
 }
@@ -1011,12 +1032,14 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         const int64_t runs = 2 * nij < nact ? 2 * nij : nact;
         const int64_t cap_items = nact / kPairChunk + runs + 1;
         int64_t* pcnt;
-        if ((rc = arena_get(kSlotPairChunk, sizeof(int64_t) * (2 * (size_t)nij + 2 + cap_items),
+        if ((rc = arena_get(kSlotPairChunk, sizeof(int64_t) * (2 * (size_t)nij + 2 + 3 * (size_t)cap_items),
                             (void**)&pcnt)))
             return rc;
         int64_t* pcoff = pcnt + nij;
         int64_t* ptotal = pcoff + nij;
         int64_t* idesc = ptotal + 2;
+        int64_t* odesc = idesc + cap_items;
+        int64_t* oitem = odesc + cap_items;
         double* part;
         if ((rc = arena_get(kSlotPairPart, sizeof(double) * kPartStride * (size_t)cap_items, (void**)&part)))
             return rc;
@@ -1025,7 +1048,8 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         ABD_CHECK_LAUNCH();
         if ((rc = scan_exclusive_i64(s, pcnt, pcoff, nij, ptotal))) return rc;
         hipLaunchKernelGGL(pair_items_kernel, dim3(grid_for(nij, kBlock / kWave)), dim3(kBlock), 0, s, nij,
-                           (const int64_t*)pcnt, (const int64_t*)pcoff, (const int64_t*)pseg, idesc);
+                           (const int64_t*)pcnt, (const int64_t*)pcoff, (const int64_t*)pseg, (const int64_t*)ptotal,
+                           idesc, err + 1, odesc, oitem);
         ABD_CHECK_LAUNCH();
         // timed: the partial sums (each stored contact read once, its body-order flag once)
         ph = prof_start_ext(kProfPairReduce, dn ? 1.0 : (8.0 * stride + 1) * nact);
@@ -1035,8 +1059,8 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
                         part);
         else
             prof_launch(ph, pair_partial_rank_kernel, dim3(kPairGrid), dim3(kRankThreads), 0, s,
-                        (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)idesc, (const int64_t*)ptotal,
-                        part);
+                        (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)odesc, (const int64_t*)oitem,
+                        (const int64_t*)ptotal, part);
         ABD_CHECK_LAUNCH();
         if ((rc = prof_count_bytes(ph, s, dnact, 8.0 * stride + 1, 0.0, 0.0, 0.0, true))) return rc;
         hipLaunchKernelGGL(pair_reduce_kernel, dim3(grid_for(nij, 1, kPairGrid)), dim3(kPairThreads), 0, s, nb, nij,
