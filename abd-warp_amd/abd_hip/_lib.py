@@ -155,7 +155,7 @@ def stream_handle(device=None) -> int | None:
 
 PROF_NAMES = ["gh_classify", "gh_contact", "gh_pair_reduce", "gh_body_finalize", "energy", "ccd_toi", "cull_filter",
               "cull_count", "cull_fill", "cull_body_pairs", "cull_vg", "solve", "project", "boxes", "solve_precond",
-              "ccd_toi_solve"]
+              "ccd_toi_solve", "keep_filter"]
 
 
 def profile_begin():

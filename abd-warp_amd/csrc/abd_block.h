@@ -27,16 +27,20 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     __syncthreads();
     return r;
 }
-// Minimum over the workgroup (blockDim.x a power of two).
+// Minimum over the workgroup (blockDim.x a multiple of 64; sh: n_waves() entries): a
+// shuffle tree inside each wave, then the waves' minima (two barriers).  NaN never wins a
+// comparison, so a NaN is kept only where every other value is NaN.
 __device__ __forceinline__ double block_min(double v, double* sh) {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int s = (int)blockDim.x / 2; s > 0; s >>= 1) {
-        if (t < s) sh[t] = sh[t + s] < sh[t] ? sh[t + s] : sh[t];
-        __syncthreads();
+    double r = v;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const double y = __shfl_xor(r, off, kWave);
+        r = y < r ? y : r;
     }
-    double r = sh[0];
+    if (lane_id() == 0) sh[wave_id()] = r;
+    __syncthreads();
+    r = sh[0];
+    for (int k = 1; k < n_waves(); ++k) r = sh[k] < r ? sh[k] : r;
     __syncthreads();
     return r;
 }
@@ -120,20 +124,22 @@ __device__ __forceinline__ void block_sum2_fast(double& a, double& b, double* sh
     b = sh[33];
 }
 
-// Exclusive prefix sum of one int64 per thread (sh: blockDim.x entries).
+// Exclusive prefix sum of one int64 per thread (sh: at least n_waves() entries): a
+// shuffle scan inside each wave, then the totals of the waves before it (two barriers).
 __device__ __forceinline__ int64_t block_exclusive_scan_i64(int64_t v, int64_t* sh) {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-        int64_t add = t >= off ? sh[t - off] : 0;
-        __syncthreads();
-        sh[t] += add;
-        __syncthreads();
+    const int lane = lane_id(), w = wave_id();
+    int64_t incl = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int64_t y = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += y;
     }
-    int64_t incl = sh[t];
+    if (lane == kWave - 1) sh[w] = incl;
     __syncthreads();
-    return incl - v;
+    int64_t before = 0;
+    for (int k = 0; k < w; ++k) before += sh[k];
+    __syncthreads();  // sh is free again for the caller
+    return before + incl - v;
 }
 
 // Ordered compaction rank inside the workgroup: returns the number of threads

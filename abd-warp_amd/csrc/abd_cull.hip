@@ -966,6 +966,7 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
     // workgroups), their scans (totals to dcount), then every list written in one launch
     const int64_t nall = A.blk0[kKeepLists];
     if ((rc = arena_get(kSlotKeepFlags, (size_t)nflag + 1, (void**)&A.flag))) return rc;
+    const int ph = prof_start(kProfKeep, s, 0.0);
     if (nall > 0) {
         hipLaunchKernelGGL(keep_count_kernel, dim3((unsigned)nall), dim3(kBlock), 0, s, A);
         ABD_CHECK_LAUNCH();
@@ -979,6 +980,7 @@ int keep_from_superset(hipStream_t s, const AbdScene* sc, const KeepInput& in, c
         hipLaunchKernelGGL(keep_write_kernel, dim3((unsigned)nall), dim3(kBlock), 0, s, A, 0, kKeepLists);
         ABD_CHECK_LAUNCH();
     }
+    prof_stop(ph, s);
     // an empty list's total is 0 (its scan segment is skipped)
     for (int k = 0; k < kKeepLists; ++k)
         if (n[k] == 0 && (rc = zero_fill(s, dcount + k, sizeof(int64_t)))) return rc;

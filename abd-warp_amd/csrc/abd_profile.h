@@ -26,6 +26,7 @@ enum ProfKernel {
     kProfBoxes,         // primitive boxes
     kProfPrecond,       // block-Jacobi: diagonal-block inverses
     kProfToiSolve,      // CCD: root searches of the queued candidates
+    kProfKeep,          // superset filter: count, scans and write passes
     kProfCount
 };
 

@@ -365,7 +365,8 @@ int abd_sim_destroy(void* sim);
  * 0 gh classify, 1 gh contact terms, 2 gh pair reduce, 3 gh body finalize,
  * 4 energy, 5 CCD, 6 cull filter, 7 cull count, 8 cull fill, 9 body pairs,
  * 10 vertex-ground cull, 11 Krylov solve, 12 vertex projection, 13 boxes,
- * 14 block-Jacobi diagonal-block inverses, 15 CCD root searches. */
+ * 14 block-Jacobi diagonal-block inverses, 15 CCD root searches, 16 superset filter
+ * (count, scans and write). */
 int abd_profile_begin(void);
 int abd_profile_end(void);
 /* Kernel ids timed while profiling (bit k = id k; all by default): timing only the
