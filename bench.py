@@ -88,7 +88,7 @@ def make_workload(name: str):
 GH_KERNELS = ("gh_classify", "gh_contact", "gh_pair_reduce", "gh_body_finalize")
 # measured HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy of known
 # size) of the profiled kernels, kept under profiles/ as "<kernel>_<workload>_pmc.md"
-PMC_FILES = {"ccd_toi": "toi_scan_kernel", "solve": "ldlt_kernel", "gh_pair_reduce": "pair_partial_kernel",
+PMC_FILES = {"ccd_toi": "toi_scan_kernel", "solve": "ldlt_kernel", "gh_pair_reduce": "pair_partial_rank_kernel",
              "energy": "chunk_kernel_energy"}
 
 
