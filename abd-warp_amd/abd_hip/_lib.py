@@ -49,7 +49,8 @@ _lib = None
 
 class AbdTuning(C.Structure):
     """Mirror of AbdTuning (include/abd_hip.h)."""
-    _fields_ = [("cull_chunk", C.c_int32), ("cull_grid", C.c_int32), ("contact_bound_mib", C.c_int32)]
+    _fields_ = [("cull_chunk", C.c_int32), ("cull_grid", C.c_int32), ("contact_bound_mib", C.c_int32),
+                ("pair_grid", C.c_int32), ("toi_grid", C.c_int32)]
 
 
 def load():

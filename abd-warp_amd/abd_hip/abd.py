@@ -109,7 +109,7 @@ class AffineBodySimulator:
             raise RuntimeError("autotune() needs the native driver")
         t = _lib.AbdTuning()
         _lib.call("abd_sim_autotune", self._sim, _lib.stream_handle(), ctypes.byref(t))
-        return {"cull_chunk": t.cull_chunk, "cull_grid": t.cull_grid}
+        return {"cull_chunk": t.cull_chunk, "cull_grid": t.cull_grid, "pair_grid": t.pair_grid}
 
     def set_graphs(self, on: bool):
         """Graph replay of the batched native driver's Newton iterations (abd_sim_set_graphs;

@@ -204,7 +204,7 @@ int scan_exclusive_i64(hipStream_t s, const int64_t* in, int64_t* out, int64_t n
 }
 
 namespace {
-AbdTuning g_tuning{4096, 16384, 0};
+AbdTuning g_tuning{4096, 16384, 0, 0, 0};
 }
 AbdTuning tuning() { return g_tuning; }
 
@@ -250,7 +250,9 @@ int abd_get_tuning(AbdTuning* t) {
 
 int abd_set_tuning(const AbdTuning* t) {
     if (!t || t->cull_chunk < 256 || t->cull_chunk > 65536 || (t->cull_chunk & (t->cull_chunk - 1)) ||
-        t->cull_grid < 64 || t->cull_grid > (1 << 20) || t->contact_bound_mib < 0)
+        t->cull_grid < 64 || t->cull_grid > (1 << 20) || t->contact_bound_mib < 0 ||
+        (t->pair_grid != 0 && (t->pair_grid < 64 || t->pair_grid > 65536)) ||
+        (t->toi_grid != 0 && (t->toi_grid < 64 || t->toi_grid > 65536)))
         return ABD_ERR_ARG;
     g_tuning = *t;
     note_reallocation();  // graphs recorded under the old shapes are not replayed

@@ -1029,6 +1029,7 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         if ((rc = prof_count_bytes(ph, s, dnact, cbytes, 0.0, 0.0, 0.0, true))) return rc;
         // stage D: chunks of each pair's runs (count, scan), their partial sums, the per-pair sums
         // items: at most one per kPairChunk contacts plus one per run (a run per pair and type)
+        const int64_t pgrid = tuning().pair_grid > 0 ? tuning().pair_grid : kPairGrid;
         const int64_t runs = 2 * nij < nact ? 2 * nij : nact;
         const int64_t cap_items = nact / kPairChunk + runs + 1;
         int64_t* pcnt;
@@ -1054,16 +1055,16 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         // timed: the partial sums (each stored contact read once, its body-order flag once)
         ph = prof_start_ext(kProfPairReduce, dn ? 1.0 : (8.0 * stride + 1) * nact);
         if (dense)
-            prof_launch(ph, pair_partial_dense_kernel, dim3(kPairGrid), dim3(kPairThreads), 0, s,
+            prof_launch(ph, pair_partial_dense_kernel, dim3((unsigned)pgrid), dim3(kPairThreads), 0, s,
                         (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)idesc, (const int64_t*)ptotal,
                         part);
         else
-            prof_launch(ph, pair_partial_rank_kernel, dim3(kPairGrid), dim3(kRankThreads), 0, s,
+            prof_launch(ph, pair_partial_rank_kernel, dim3((unsigned)pgrid), dim3(kRankThreads), 0, s,
                         (const double*)cbuf, (const uint8_t*)swp, (const int64_t*)odesc, (const int64_t*)oitem,
                         (const int64_t*)ptotal, part);
         ABD_CHECK_LAUNCH();
         if ((rc = prof_count_bytes(ph, s, dnact, 8.0 * stride + 1, 0.0, 0.0, 0.0, true))) return rc;
-        hipLaunchKernelGGL(pair_reduce_kernel, dim3(grid_for(nij, 1, kPairGrid)), dim3(kPairThreads), 0, s, nb, nij,
+        hipLaunchKernelGGL(pair_reduce_kernel, dim3(grid_for(nij, 1, pgrid)), dim3(kPairThreads), 0, s, nb, nij,
                            dn, (const uint8_t*)pflag, (const int64_t*)pcoff, (const int64_t*)pcnt,
                            (const double*)part, pbuf, blocks);
         ABD_CHECK_LAUNCH();

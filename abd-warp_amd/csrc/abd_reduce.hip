@@ -604,7 +604,8 @@ int toi_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
                 seg, choff, active, w.part, w.queue, w.qcount);
     ABD_CHECK_LAUNCH();
     if ((rc = attach_seg_bytes(ph, s, S, 1, nullptr, active, seg, 228.0, 224.0, 56.0))) return rc;
-    const unsigned grid = (unsigned)std::min<int64_t>(kToiSolveGrid, std::max<int64_t>(1, ceil_div(nq, 4 * kToiSolveThreads)));
+    const int64_t gmax = tuning().toi_grid > 0 ? tuning().toi_grid : kToiSolveGrid;
+    const unsigned grid = (unsigned)std::min<int64_t>(gmax, std::max<int64_t>(1, ceil_div(nq, 4 * kToiSolveThreads)));
     const int ph2 = scr ? -1 : prof_start_ext(kProfToiSolve, 0.0);
     prof_launch(ph2, toi_solve_kernel, dim3(grid), dim3(kToiSolveThreads), 0, s, sc, L, S, x0, x1,
                 (const int32_t*)w.queue, (const int32_t*)w.qcount, w.slots);

@@ -1218,7 +1218,7 @@ int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen) {
     const int grids[] = {4096, 16384, 65536};
     for (int c : chunks)
         for (int g : grids) {
-            AbdTuning t{c, g, saved.contact_bound_mib};
+            AbdTuning t{c, g, saved.contact_bound_mib, saved.pair_grid, saved.toi_grid};
             if ((rc = abd_set_tuning(&t))) break;
             float ms_min = 1e30f;
             for (int rep = 0; rep < 3 && !rc; ++rep) {
@@ -1239,6 +1239,43 @@ int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen) {
                 best = t;
             }
         }
+    // stage D's grid, with the chosen tiles: the contact Hessian of the collision set at this
+    // state (its structure built once), best of three per grid; g and the Hessian slots are
+    // scratch that the next iteration rewrites
+    if (!rc && (rc = abd_set_tuning(&best)) == ABD_OK && (rc = collision_set(s, S, true, S.st.xk)) == ABD_OK) {
+        const size_t nslot = S.sc.n_bodies + 2 * (size_t)S.col.nij;
+        rc = S.blocks.reserve(nslot * 16 * 9);
+        const CandSet& C = S.col;
+        const GhLists L{C.ptl.p, C.npt, C.eel.p, C.nee, C.vgl.p, C.nvg, C.nij, C.counts()};
+        const int pgrids[] = {512, 1024, 2048, 4096, 8192};
+        float bestp = 1e30f;
+        int32_t pbest = best.pair_grid;
+        for (int pg : pgrids) {
+            if (rc) break;
+            AbdTuning t = best;
+            t.pair_grid = pg;
+            if ((rc = abd_set_tuning(&t))) break;
+            float ms_min = 1e30f;
+            for (int rep = 0; rep < 3 && !rc; ++rep) {
+                hipError_t he = hipEventRecord(e0, s);
+                if (he != hipSuccess) {
+                    rc = -static_cast<int>(he);
+                    break;
+                }
+                rc = contact_gh(s, &S.sc, &S.P, S.st.xk, L, S.row_ptr.p, S.slots.p, S.st.g, S.blocks.p, nullptr,
+                                nullptr, S.cnt.p + kCntNact);
+                if (!rc && (he = hipEventRecord(e1, s)) != hipSuccess) rc = -static_cast<int>(he);
+                float ms = 0.f;
+                if (!rc && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+                    ms_min = ms < ms_min ? ms : ms_min;
+            }
+            if (!rc && ms_min < bestp) {
+                bestp = ms_min;
+                pbest = pg;
+            }
+        }
+        best.pair_grid = pbest;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     abd_set_tuning(rc ? &saved : &best);

@@ -77,11 +77,15 @@ typedef void* abd_stream_t; /* hipStream_t */
  * 4096); cull_grid: most workgroups of its chunk passes (64..2^20; default 16384);
  * contact_bound_mib: the simulator's per-iteration contact buffer is sized by its candidate
  * lists' capacity while that needs at most this many MiB (0: 8192), else by the active
- * count read back (one host wait; such an iteration is launched, not replayed from a graph).
- * Process-wide; a change invalidates recorded graphs. */
+ * count read back (one host wait; such an iteration is launched, not replayed from a graph);
+ * pair_grid: workgroups of the contact Hessian's per-pair sums (64..65536; 0: 2048: each work
+ * item is summed by one workgroup in a fixed order whatever the grid); toi_grid: most
+ * workgroups of the CCD's root searches (64..65536; 0: 2048; the minima are exact).
+ * Process-wide; a change invalidates recorded graphs.  Zeroed trailing fields are the defaults. */
 typedef struct AbdTuning {
     int32_t cull_chunk, cull_grid;
     int32_t contact_bound_mib;
+    int32_t pair_grid, toi_grid;
 } AbdTuning;
 int abd_get_tuning(AbdTuning* tuning);
 int abd_set_tuning(const AbdTuning* tuning);
@@ -341,7 +345,8 @@ int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double
                           const double* rhs, double* y, int32_t* iters, double* resid);
 /* Autotuner (no reference counterpart): times the collision-set culling at the driver's
  * current state for a small set of AbdTuning shapes (chunk 1024..16384, grid 4096..65536,
- * best of three each), keeps the fastest with abd_set_tuning and returns it (nullable).
+ * best of three each), then with those tiles the contact gradient and Hessian of that set
+ * for pair_grid 512..8192; keeps the fastest with abd_set_tuning and returns it (nullable).
  * Results are unchanged by the choice; call it once the scene has representative contacts. */
 int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen);
 /* Newton iterations of each scene in the last abd_sim_step (iters: host [n_scenes];

@@ -96,10 +96,12 @@ def test_tuning_round_trip_and_bounds():
     L = _lib.load()
     t0 = _lib.AbdTuning()
     assert L.abd_get_tuning(ctypes.byref(t0)) == 0
-    assert (t0.cull_chunk, t0.cull_grid, t0.contact_bound_mib) == (4096, 16384, 0)
+    assert (t0.cull_chunk, t0.cull_grid, t0.contact_bound_mib, t0.pair_grid, t0.toi_grid) == (4096, 16384, 0, 0, 0)
     bad = _lib.AbdTuning(3000, 16384)   # not a power of two
     assert L.abd_set_tuning(ctypes.byref(bad)) != 0
     assert L.abd_set_tuning(ctypes.byref(_lib.AbdTuning(4096, 16384, -1))) != 0
+    assert L.abd_set_tuning(ctypes.byref(_lib.AbdTuning(4096, 16384, 0, 32, 0))) != 0   # pair_grid below 64
+    assert L.abd_set_tuning(ctypes.byref(_lib.AbdTuning(4096, 16384, 0, 0, 70000))) != 0  # toi_grid above 65536
     good = _lib.AbdTuning(8192, 4096)
     assert L.abd_set_tuning(ctypes.byref(good)) == 0
     t1 = _lib.AbdTuning()

@@ -139,7 +139,9 @@ def test_full_batched_config_sample_matches_plain():
 
 
 def test_autotuned_tiles_change_no_result():
-    """Different culling launch shapes (the autotuner's choices) give bit-identical frames."""
+    """Different launch shapes (the culling tiles, stage D's grid: 64 workgroups make each sum
+    several items, and the CCD root searches' grid; the autotuner's choices) give
+    bit-identical frames."""
     from abd_hip import _lib, scenes
     from abd_hip.abd import AffineBodySimulator
     from abd_hip.const_params import ContactParams
@@ -151,13 +153,14 @@ def test_autotuned_tiles_change_no_result():
     L.abd_get_tuning(ctypes.byref(t0))
     out = []
     try:
-        for chunk, grid in ((4096, 16384), (1024, 64), (65536, 4096)):
-            assert L.abd_set_tuning(ctypes.byref(_lib.AbdTuning(chunk, grid))) == 0
+        for chunk, grid, pgrid, tgrid in ((4096, 16384, 0, 0), (1024, 64, 64, 64), (65536, 4096, 8192, 4096)):
+            assert L.abd_set_tuning(ctypes.byref(_lib.AbdTuning(chunk, grid, 0, pgrid, tgrid))) == 0
             sim = AffineBodySimulator(spec, cp, "cuda", native=True)
             for f in range(5):
                 sim.step(f)
             out.append((sim.states.A.cpu().numpy(), sim.states.p.cpu().numpy(), dict(sim.stats)))
-        sim.autotune()
+        chosen = sim.autotune()
+        assert chosen["pair_grid"] in (512, 1024, 2048, 4096, 8192), chosen
     finally:
         L.abd_set_tuning(ctypes.byref(t0))
     for A, p, st in out[1:]:
