@@ -349,7 +349,7 @@ def main():
     if not a.python_driver:
         # launch shapes of the culling chunk passes, timed at the warmed-up state (no effect on results)
         tiles = sim.autotune()
-        log(f"[gpu] autotuned culling tiles: {tiles}")
+        log(f"[gpu] autotuned launch shapes: {tiles}")
     frame = a.warmup
     # attribution frames (untimed): every instrumented kernel timed with HIP events, for the
     # per-kernel table, the gh-stage rate and the candidates for the dominant kernel; the timed
