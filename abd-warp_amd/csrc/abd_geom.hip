@@ -84,6 +84,120 @@ __global__ __launch_bounds__(kBlock) void project_step_kernel(const int32_t* __r
     if (lo && __any(out) && lane_id() == 0) *flag = gen;  // every writer stores gen
 }
 
+// The body state of a step's end for project_step2_kernel: project_step_kernel's arithmetic.
+__device__ __forceinline__ void step_end_body(int b, const int32_t* __restrict__ body_scene,
+                                              const double* __restrict__ A, const double* __restrict__ p,
+                                              const double* __restrict__ dq, const double* __restrict__ alpha_dev,
+                                              double alpha_host, const uint8_t* __restrict__ active, double pb[3],
+                                              double Ab[9]) {
+    if (alpha_dev || active) {
+        const int sc = body_scene ? body_scene[b] : 0;
+        const double al = alpha_dev ? alpha_dev[sc] : (active[sc] ? alpha_host : 0.0);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) pb[r] = al == 0.0 ? p[3 * b + r] : p[3 * b + r] - dq[3 * (4 * b) + r] * al;
+#pragma unroll
+        for (int r = 0; r < 9; ++r)
+            Ab[r] = al == 0.0 ? A[9 * b + r] : A[9 * b + r] - dq[3 * (4 * b + 1 + r / 3) + r % 3] * al;
+    } else {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) pb[r] = p[3 * b + r] - dq[3 * (4 * b) + r] * alpha_host;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) Ab[r] = A[9 * b + r] - dq[3 * (4 * b + 1 + r / 3) + r % 3] * alpha_host;
+    }
+}
+
+// project_step_kernel over vertex pairs (2i, 2i + 1): the pair's six coordinates of x0, the ranges,
+// x_other and x as three 16-byte accesses each (the arrays 16-byte aligned), the body state computed
+// again for the second vertex only when it is of another body; an odd last vertex is the last
+// thread's.  Same
+// arithmetic, same results.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(kBlock) void project_step2_kernel(const int32_t* __restrict__ vert_body,
+                                                                const int32_t* __restrict__ body_scene,
+                                                                const double* __restrict__ x0,
+                                                                const double* __restrict__ A,
+                                                                const double* __restrict__ p,
+                                                                const double* __restrict__ dq,
+                                                                const double* __restrict__ alpha_dev,
+                                                                double alpha_host,
+                                                                const uint8_t* __restrict__ active,
+                                                                double* __restrict__ x,
+                                                                const double* __restrict__ x_other,
+                                                                const double* __restrict__ lo,
+                                                                const double* __restrict__ hi,
+                                                                int64_t* __restrict__ flag, int64_t gen, int64_t nv) {
+    bool out = false;
+    const int64_t npair = (nv + 1) / 2;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < npair; i += (int64_t)gridDim.x * kBlock) {
+        const int64_t v0 = 2 * i;
+        const bool two = v0 + 1 < nv;
+        const int b0 = vert_body[v0], b1 = two ? vert_body[v0 + 1] : b0;
+        double c[6], l[6], h[6], o[6], rr[6];
+        if (two) {
+            const dvec2* X0 = reinterpret_cast<const dvec2*>(x0 + 3 * v0);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const dvec2 a = X0[q];
+                c[2 * q] = a.x;
+                c[2 * q + 1] = a.y;
+            }
+            if (lo) {
+                const dvec2 *L = reinterpret_cast<const dvec2*>(lo + 3 * v0), *H = reinterpret_cast<const dvec2*>(hi + 3 * v0),
+                            *O = reinterpret_cast<const dvec2*>(x_other + 3 * v0);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const dvec2 a = L[q], b = H[q], e = O[q];
+                    l[2 * q] = a.x;
+                    l[2 * q + 1] = a.y;
+                    h[2 * q] = b.x;
+                    h[2 * q + 1] = b.y;
+                    o[2 * q] = e.x;
+                    o[2 * q + 1] = e.y;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                c[k] = x0[3 * v0 + k];
+                c[3 + k] = 0.0;
+                if (lo) {
+                    l[k] = lo[3 * v0 + k];
+                    h[k] = hi[3 * v0 + k];
+                    o[k] = x_other[3 * v0 + k];
+                }
+            }
+        }
+        // one body state in registers: vertex 1's is computed over vertex 0's when they differ
+        double pb[3], Ab[9];
+        step_end_body(b0, body_scene, A, p, dq, alpha_dev, alpha_host, active, pb, Ab);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (u == 1 && b1 != b0) step_end_body(b1, body_scene, A, p, dq, alpha_dev, alpha_host, active, pb, Ab);
+            const double r0 = c[3 * u], r1 = c[3 * u + 1], r2 = c[3 * u + 2];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double r = pb[k] + r0 * Ab[k];
+                r = r + r1 * Ab[3 + k];
+                r = r + r2 * Ab[6 + k];
+                rr[3 * u + k] = r;
+                if (lo && (u == 0 || two)) {
+                    const int j = 3 * u + k;
+                    out |= !(r >= l[j] && r <= h[j] && o[j] >= l[j] && o[j] <= h[j]);  // NaN counts as outside
+                }
+            }
+        }
+        if (two) {
+            dvec2* Xo = reinterpret_cast<dvec2*>(x + 3 * v0);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) Xo[q] = dvec2{rr[2 * q], rr[2 * q + 1]};
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) x[3 * v0 + k] = rr[k];
+        }
+    }
+    if (lo && __any(out) && lane_id() == 0) *flag = gen;  // every writer stores gen
+}
+
 // abd.py:337-353
 __global__ void update_q_kernel(int nb, const double* __restrict__ A, const double* __restrict__ p,
                                 const double* __restrict__ dq, double alpha, double* A_out, double* p_out,
@@ -331,6 +445,16 @@ int project_step(hipStream_t s, const AbdScene* sc, const double* A, const doubl
                  const double* alpha_dev, double alpha_host, const uint8_t* active, double* x_out,
                  const double* x_other, const double* lo, const double* hi, int64_t* flag, int64_t gen) {
     if (sc->n_verts == 0) return ABD_OK;
+    // vertex pairs with 16-byte accesses when every array allows them
+    const auto a16 = [](const void* q) { return q == nullptr || ((uintptr_t)q & 15) == 0; };
+    if (a16(sc->x0) && a16(x_out) && a16(x_other) && a16(lo) && a16(hi)) {
+        hipLaunchKernelGGL(project_step2_kernel, dim3(grid_for(((int64_t)sc->n_verts + 1) / 2, kBlock, 1 << 20)),
+                           dim3(kBlock), 0, s, sc->vert_body, sc->n_scenes > 1 ? sc->body_scene : nullptr, sc->x0,
+                           A, p, dq, alpha_dev, alpha_host, active, x_out, x_other, lo, hi, flag, gen,
+                           (int64_t)sc->n_verts);
+        ABD_CHECK_LAUNCH();
+        return ABD_OK;
+    }
     hipLaunchKernelGGL(project_step_kernel, dim3(grid_for(sc->n_verts, kBlock, 1 << 20)), dim3(kBlock), 0, s,
                        sc->vert_body, sc->n_scenes > 1 ? sc->body_scene : nullptr, sc->x0, A, p, dq, alpha_dev,
                        alpha_host, active, x_out, x_other, lo, hi, flag, gen, (int64_t)sc->n_verts);
