@@ -1247,7 +1247,10 @@ int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen) {
         rc = S.blocks.reserve(nslot * 16 * 9);
         const CandSet& C = S.col;
         const GhLists L{C.ptl.p, C.npt, C.eel.p, C.nee, C.vgl.p, C.nvg, C.nij, C.counts()};
-        const int pgrids[] = {512, 1024, 2048, 4096, 8192};
+        // one or two rounds of the workgroups the device holds at once (four a CU): larger
+        // grids, and smaller ones, measured faster on an early frame's few contacts and slower
+        // on the later frames' many
+        const int pgrids[] = {1024, 2048};
         float bestp = 1e30f;
         int32_t pbest = best.pair_grid;
         for (int pg : pgrids) {

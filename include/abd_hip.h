@@ -346,7 +346,7 @@ int abd_sim_hessian_solve(void* sim, abd_stream_t stream, int32_t method, double
 /* Autotuner (no reference counterpart): times the collision-set culling at the driver's
  * current state for a small set of AbdTuning shapes (chunk 1024..16384, grid 4096..65536,
  * best of three each), then with those tiles the contact gradient and Hessian of that set
- * for pair_grid 512..8192; keeps the fastest with abd_set_tuning and returns it (nullable).
+ * for pair_grid 1024 and 2048; keeps the fastest with abd_set_tuning and returns it (nullable).
  * Results are unchanged by the choice; call it once the scene has representative contacts. */
 int abd_sim_autotune(void* sim, abd_stream_t stream, AbdTuning* chosen);
 /* Newton iterations of each scene in the last abd_sim_step (iters: host [n_scenes];

@@ -160,7 +160,7 @@ def test_autotuned_tiles_change_no_result():
                 sim.step(f)
             out.append((sim.states.A.cpu().numpy(), sim.states.p.cpu().numpy(), dict(sim.stats)))
         chosen = sim.autotune()
-        assert chosen["pair_grid"] in (512, 1024, 2048, 4096, 8192), chosen
+        assert chosen["pair_grid"] in (1024, 2048), chosen
     finally:
         L.abd_set_tuning(ctypes.byref(t0))
     for A, p, st in out[1:]:
