@@ -202,6 +202,33 @@ def test_contact_bound_read_back_equals_bound_sizing():
                                                                                sized.graph_counts())
 
 
+def test_step_end_projection_unaligned_equals_pairs():
+    """The step's end projected by vertex pairs with 16-byte accesses (vertex buffers 16-byte
+    aligned) and by the per-vertex kernel (the buffers 8 bytes off) gives the same frames."""
+    from abd_hip import scenes
+    from abd_hip.abd import AffineBodySimulator
+    from abd_hip.const_params import ContactParams
+    spec = scenes.scene_of(scenes.config_batched(n_scenes=4, seed0=5, nb_range=(4, 8)), 2)
+    cp = ContactParams(ground=spec.ground, psd=1, solver=3)
+    aligned = AffineBodySimulator(spec, cp, "cuda", native=True)
+    shifted = AffineBodySimulator(spec, cp, "cuda", native=False)
+    m = shifted.meshes
+    for name in ("x", "xk"):
+        t = getattr(m, name)
+        buf = torch.empty(t.numel() + 1, dtype=t.dtype, device=t.device)
+        view = buf[1:].view_as(t)
+        view.copy_(t)
+        assert view.data_ptr() % 16 == 8
+        setattr(m, name, view)
+    shifted._make_native()
+    for f in range(6):
+        ia = aligned.step(f)
+        ib = shifted.step(f)
+        assert ia == ib, (f, ia, ib)
+        assert np.array_equal(aligned.states.A.cpu().numpy(), shifted.states.A.cpu().numpy()), f
+        assert np.array_equal(aligned.states.p.cpu().numpy(), shifted.states.p.cpu().numpy()), f
+
+
 @pytest.mark.parametrize("solver", [3, 1])
 def test_graph_replay_equals_launches(solver):
     """Newton iterations replayed from a recorded HIP graph give the states, iteration
