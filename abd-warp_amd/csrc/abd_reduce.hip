@@ -645,7 +645,14 @@ int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, co
 // (VertsTrial), and the inertia energy of every trial, with one plan of the lists for all.
 // E[2S k + s] is the contact energy and E[2S k + S + s] the inertia energy of scene s
 // at trial k, each bit-identical to the single-state reductions at that state.
-__global__ __launch_bounds__(kRedThreads) void inertia_trials_kernel(int S, const double* __restrict__ alpha,
+
+// scene_final_kernel<0> and the inertia energy of one (scene, trial) in one workgroup: the
+// contact energy from the trial's chunk partials (E[2 S k + s]) and the inertia energy at the
+// trial state (E[2 S k + S + s]), each with the arithmetic of its own kernel.
+__global__ __launch_bounds__(kRedThreads) void trial_energies_kernel(const int64_t* __restrict__ choff,
+                                                                     const double* __restrict__ part,
+                                                                     int64_t part_stride, int S,
+                                                                     const double* __restrict__ alpha,
                                                                      const int32_t* __restrict__ sboff, int nb,
                                                                      AbdParams P, const double* __restrict__ mass,
                                                                      const double* __restrict__ I0v,
@@ -657,22 +664,27 @@ __global__ __launch_bounds__(kRedThreads) void inertia_trials_kernel(int S, cons
                                                                      const double* __restrict__ Adot,
                                                                      const double* __restrict__ pdot,
                                                                      double* __restrict__ E) {
-    __shared__ double sh[17];
+    __shared__ double sh[kRedThreads];
     const int s = blockIdx.x, k = blockIdx.y;
+    part += part_stride * k;
+    double acc = 0.0;
+    for (int64_t i = choff[s] + threadIdx.x; i < choff[s + 1]; i += blockDim.x) acc += part[i];
+    const double rc = block_sum_fast(acc, sh);
+    if (threadIdx.x == 0) E[(int64_t)2 * S * k + s] = rc;
     const double al = alpha[(int64_t)S * k + s];
     if (al < 0.0) {  // skipped trial (uniform over the workgroup)
-        if (threadIdx.x == 0) E[(int64_t)2 * S * k + s] = 0.0;
+        if (threadIdx.x == 0) E[(int64_t)2 * S * k + S + s] = 0.0;
         return;
     }
     const int b0 = S > 1 ? sboff[s] : 0, b1 = S > 1 ? sboff[s + 1] : nb;
-    double acc = 0.0;
+    acc = 0.0;
     for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
         double Ab[9], pb[3];
         trial_state(b, al, A, p, dq, Ab, pb);
         acc += inertia_energy_q(P, mass, I0v, b, Ab, pb, A0, p0, Adot, pdot);
     }
-    const double r = block_sum_fast(acc, sh);
-    if (threadIdx.x == 0) E[(int64_t)2 * S * k + s] = r;
+    const double ri = block_sum_fast(acc, sh);
+    if (threadIdx.x == 0) E[(int64_t)2 * S * k + S + s] = ri;
 }
 
 int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, int K, const double* alpha,
@@ -697,11 +709,8 @@ int line_trials(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, in
                 part, bound);
     ABD_CHECK_LAUNCH();
     if ((rc = attach_seg_bytes(ph, s, S, K, alpha, nullptr, seg, 132.0, 128.0, 32.0))) return rc;
-    hipLaunchKernelGGL(scene_final_kernel<0>, dim3(S, K), dim3(kRedThreads), 0, s, choff, (const double*)part, E,
-                       bound, (int64_t)2 * S);
-    ABD_CHECK_LAUNCH();
-    hipLaunchKernelGGL(inertia_trials_kernel, dim3(S, K), dim3(kRedThreads), 0, s, S, alpha, sc->scene_boff,
-                       sc->n_bodies, *P, sc->mass, sc->I0, A, p, dq, A0, p0, Adot, pdot, E + S);
+    hipLaunchKernelGGL(trial_energies_kernel, dim3(S, K), dim3(kRedThreads), 0, s, choff, (const double*)part, bound,
+                       S, alpha, sc->scene_boff, sc->n_bodies, *P, sc->mass, sc->I0, A, p, dq, A0, p0, Adot, pdot, E);
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }
