@@ -905,6 +905,73 @@ __global__ void inertia_hessian_kernel(int nb, AbdParams P, const double* __rest
     for (int k = 0; k < 9; ++k) B[k] += dh[k];
 }
 
+// inertia_gradient_kernel and inertia_hessian_kernel in one launch: thread (body, sub-block e)
+// adds sub-block e of the Hessian, and threads e < 4 also write gradient row e (the rows of
+// inertia_gradient_kernel's loop, the same arithmetic).
+__global__ void inertia_gh_kernel(int nb, AbdParams P, const double* __restrict__ mass,
+                                  const double* __restrict__ I0v, const double* __restrict__ A,
+                                  const double* __restrict__ p, const double* __restrict__ A0,
+                                  const double* __restrict__ p0, const double* __restrict__ Adot,
+                                  const double* __restrict__ pdot, double* __restrict__ g,
+                                  double* __restrict__ blocks) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 16 * nb) return;
+    const int b = t / 16, e = t % 16;
+    if (e < 4) {
+        const double* Ab = A + 9 * b;
+        double At[9], pt[3];
+        tilde(P, A0 + 9 * b, Adot + 9 * b, p0 + 3 * b, pdot + 3 * b, At, pt);
+        const double m = mass ? mass[b] : P.mass, i0 = I0v ? I0v[b] : P.I0;
+        if (e == 0) {
+            for (int k = 0; k < 3; ++k) g[3 * (4 * b) + k] = 0.0 + (p[3 * b + k] - pt[k]) * m;
+        } else {
+            const int i = e - 1;
+            double gr[3] = {-Ab[3 * i], -Ab[3 * i + 1], -Ab[3 * i + 2]};
+            for (int j = 0; j < 3; ++j) {
+                double d = row_dot(Ab, i, j);
+                for (int k = 0; k < 3; ++k) gr[k] += d * Ab[3 * j + k];
+            }
+            for (int k = 0; k < 3; ++k)
+                g[3 * (4 * b + e) + k] = P.dt * P.dt * (gr[k] * 4.0 * P.stiffness) + (Ab[3 * i + k] - At[3 * i + k]) * i0;
+        }
+    }
+    const int ii = e % 4, jj = e / 4;
+    const double* Ab = A + 9 * b;
+    const double m_ = mass ? mass[b] : P.mass, i0 = I0v ? I0v[b] : P.I0;
+    const double m = ii == jj ? (ii == 0 ? m_ : i0) : 0.0;
+    double dh[9] = {m, 0, 0, 0, m, 0, 0, 0, m};
+    if (ii > 0 && jj > 0) {
+        const int i = ii - 1, j = jj - 1;
+        double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (i == j) {
+            const double qq = row_dot(Ab, i, i) - 1.0;
+            for (int k = 0; k < 3; ++k)
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) h[3 * r + c] += Ab[3 * k + r] * Ab[3 * k + c];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) h[3 * r + c] += Ab[3 * i + r] * Ab[3 * i + c] + (r == c ? qq : 0.0);
+        } else {
+            const double d = row_dot(Ab, j, i);
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) h[3 * r + c] = Ab[3 * j + r] * Ab[3 * i + c] + (r == c ? d : 0.0);
+        }
+        for (int k = 0; k < 9; ++k) dh[k] += h[k] * 4.0 * P.stiffness * P.dt * P.dt;
+    }
+    double* B = blocks + 9 * (16 * (int64_t)b + ii + jj * 4);
+    for (int k = 0; k < 9; ++k) B[k] += dh[k];
+}
+
+// The inertia gradient (overwriting g) and Hessian (added into the diagonal slots) in one launch.
+int inertia_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const double* A, const double* p,
+               const double* A0, const double* p0, const double* Adot, const double* pdot, double* g, double* blocks) {
+    const int nb = sc->n_bodies;
+    if (!nb) return ABD_OK;
+    hipLaunchKernelGGL(inertia_gh_kernel, dim3(grid_for(16 * (int64_t)nb)), dim3(kBlock), 0, s, nb, *P, sc->mass,
+                       sc->I0, A, p, A0, p0, Adot, pdot, g, blocks);
+    ABD_CHECK_LAUNCH();
+    return ABD_OK;
+}
+
 // The gh stage (stages A-E).  With L.dn the list sizes are on the device (L's counts bound
 // them and size the launches and buffers), nothing is read back and the lists are taken
 // as grouped (the drivers' own culling output); n_active_dev (nullable) then receives the

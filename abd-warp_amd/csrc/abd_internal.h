@@ -243,6 +243,9 @@ struct GhLists {
     int64_t nij;
     const int64_t* dn;
 };
+// The inertia gradient (overwriting g) and Hessian (added into the diagonal slots) in one launch.
+int inertia_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const double* A, const double* p,
+               const double* A0, const double* p0, const double* Adot, const double* pdot, double* g, double* blocks);
 int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const double* x, const GhLists& L,
                const int32_t* row_ptr, const int32_t* slots, double* g, double* blocks, uint8_t* highlight,
                int64_t* n_active, int64_t* n_active_dev);

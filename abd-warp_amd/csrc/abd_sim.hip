@@ -568,8 +568,8 @@ int step(hipStream_t s, Sim& S) {
         if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
         ABD_CHECK(hipMemsetAsync(S.blocks.p, 0, sizeof(double) * nslot * 16 * 9, s));
         // g: overwritten by the inertia gradient (abd.py:236-237 zeroes it first)
-        if ((rc = abd_inertia_gradient(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g))) return rc;
-        if ((rc = abd_inertia_hessian(s, &S.sc, &P, st.A, S.blocks.p))) return rc;
+        if ((rc = inertia_gh(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g, S.blocks.p)))
+            return rc;
         const CandSet& C = S.col;
         const GhLists L{C.ptl.p, C.npt, C.eel.p, C.nee, C.vgl.p, C.nvg, C.nij, C.counts()};
         if ((rc = contact_gh(s, &S.sc, &P, st.x, L, S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, nullptr,
@@ -902,8 +902,7 @@ int iteration_scenes(hipStream_t s, Sim& S) {
     if ((rc = S.blocks.reserve(nslot * 16 * 9))) return rc;
     if ((rc = zero_fill(s, S.blocks.p, sizeof(double) * nslot * 16 * 9))) return rc;
     // g: overwritten by the inertia gradient (abd.py:236-237 zeroes it first)
-    if ((rc = abd_inertia_gradient(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g))) return rc;
-    if ((rc = abd_inertia_hessian(s, &S.sc, &P, st.A, S.blocks.p))) return rc;
+    if ((rc = inertia_gh(s, &S.sc, &P, st.A, st.p, st.A0, st.p0, st.Adot, st.pdot, st.g, S.blocks.p))) return rc;
     const CandSet& C = S.col;
     const GhLists GL{C.ptl.p, C.npt, C.eel.p, C.nee, C.vgl.p, C.nvg, C.nij, C.counts()};
     if ((rc = contact_gh(s, &S.sc, &P, st.x, GL, S.row_ptr.p, S.slots.p, st.g, S.blocks.p, nullptr, nullptr,
