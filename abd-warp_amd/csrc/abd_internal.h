@@ -155,10 +155,18 @@ int reduce_plan(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, co
                 const int64_t* dn = nullptr);
 // Per-scene TOI (abd_scene_toi; S = 1: the whole system as one scene) with an optional plan;
 // active (nullable, [S]): scenes whose TOI is wanted, the others report 1 unevaluated.
+// Per-scene dot products a.b and c.d (seg_dot2_kernel's) that the TOI's final per-scene pass computes
+// after the scene's TOI when out1 is set (the driver's q.g and g.g, one launch less).
+struct SceneDots {
+    const int32_t* sboff = nullptr;
+    int64_t n = 0;
+    const double *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
+    double *out1 = nullptr, *out2 = nullptr;
+};
 int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
               const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
               const int64_t* plan, double* toi, const uint8_t* active, const int64_t* dn = nullptr,
-              double* first_alpha = nullptr, VertIds vids = {});
+              double* first_alpha = nullptr, VertIds vids = {}, const SceneDots* dots = nullptr);
 // The line search's first step length from a TOI (abd.py:300-302): clamped to [0, 1], x 0.9
 // below 1.  scene_toi's first_alpha (device, nullable) receives it per scene (-1 for the
 // scenes not active).

@@ -399,7 +399,7 @@ __global__ __launch_bounds__(kRedThreads) void toi_final_kernel(const int64_t* _
                                                                 const unsigned long long* __restrict__ slots,
                                                                 double* __restrict__ out,
                                                                 const uint8_t* __restrict__ active,
-                                                                double* __restrict__ alpha) {
+                                                                double* __restrict__ alpha, SceneDots D) {
     __shared__ double sh[kRedThreads];
     const int s = blockIdx.x;
     double acc = __longlong_as_double((long long)slots[s]);
@@ -411,6 +411,19 @@ __global__ __launch_bounds__(kRedThreads) void toi_final_kernel(const int64_t* _
     if (threadIdx.x == 0) {
         out[s] = r;
         if (alpha) alpha[s] = !active || active[s] ? first_step_length(r) : -1.0;
+    }
+    if (D.out1) {  // uniform: seg_dot2_kernel's sums of this scene
+        const int64_t i0 = D.sboff ? 12 * (int64_t)D.sboff[s] : 0, i1 = D.sboff ? 12 * (int64_t)D.sboff[s + 1] : D.n;
+        double acc1 = 0.0, acc2 = 0.0;
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            acc1 += D.a[i] * D.b[i];
+            acc2 += D.c[i] * D.d[i];
+        }
+        block_sum2_fast(acc1, acc2, sh);
+        if (threadIdx.x == 0) {
+            D.out1[s] = acc1;
+            D.out2[s] = acc2;
+        }
     }
 }
 
@@ -573,7 +586,7 @@ int attach_seg_bytes(int handle, hipStream_t s, int S, int K, const double* alph
 // scr (nullable: the arena; then the scan and the solve are timed when profiling).
 int toi_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, const double* x0, const double* x1,
                const Lists& L, const int64_t* plan_in, double* out, const uint8_t* active,
-               const ReduceScratch* scr, double* first_alpha = nullptr) {
+               const ReduceScratch* scr, double* first_alpha = nullptr, const SceneDots* dots = nullptr) {
     const int64_t bound = chunk_bound(L, S);
     if (bound * kToiSplit > INT32_MAX) return ABD_ERR_ARG;
     const int64_t nq = L.l[0].n + L.l[1].n;  // every PT and EE candidate may queue
@@ -611,7 +624,7 @@ int toi_reduce(hipStream_t s, const AbdScene& sc, const AbdParams& P, int S, con
                 (const int32_t*)w.queue, (const int32_t*)w.qcount, w.slots);
     ABD_CHECK_LAUNCH();
     hipLaunchKernelGGL(toi_final_kernel, dim3(S), dim3(kRedThreads), 0, s, choff, (const double*)w.part,
-                       (const unsigned long long*)w.slots, out, active, first_alpha);
+                       (const unsigned long long*)w.slots, out, active, first_alpha, dots ? *dots : SceneDots{});
     ABD_CHECK_LAUNCH();
     return ABD_OK;
 }
@@ -739,11 +752,11 @@ int contact_energy(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S,
 int scene_toi(hipStream_t s, const AbdScene* sc, const AbdParams* P, int S, const double* x_t0, const double* x_t1,
               const int32_t* pt, int64_t npt, const int32_t* ee, int64_t nee, const int32_t* vg, int64_t nvg,
               const int64_t* plan, double* toi, const uint8_t* active, const int64_t* dn, double* first_alpha,
-              VertIds vids) {
+              VertIds vids, const SceneDots* dots) {
     if (!P || !x_t0 || !x_t1 || !toi || npt < 0 || nee < 0 || nvg < 0) return ABD_ERR_ARG;
     if ((npt && !pt) || (nee && !ee) || (nvg && !vg)) return ABD_ERR_ARG;
     const Lists L = contact_lists(*P, pt, npt, ee, nee, vg, nvg, dn, vids);  // abd.py:288-299
-    return toi_reduce(s, *sc, *P, S, x_t0, x_t1, L, plan, toi, active, nullptr, first_alpha);
+    return toi_reduce(s, *sc, *P, S, x_t0, x_t1, L, plan, toi, active, nullptr, first_alpha, dots);
 }
 
 int dots(hipStream_t s, const AbdScene* sc, int S, const double* a, const double* b, const double* c,

@@ -861,12 +861,21 @@ int ccd_and_search_scenes(hipStream_t s, Sim& S, const LsDev& L) {
     int rc;
     if ((rc = reduce_plan(s, &S.sc, &P, ns, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg, S.plan.p, W.counts())))
         return rc;
-    // the TOI of the scenes still iterating and from it the first trial's step lengths
+    // the TOI of the scenes still iterating and from it the first trial's step lengths; in the
+    // same final pass q.g and g.g of the Armijo and stopping tests (abd.py:148,265: g and dq
+    // are final here)
+    SceneDots qg;
+    qg.sboff = ns > 1 ? S.sc.scene_boff : nullptr;
+    qg.n = 12 * (int64_t)S.sc.n_bodies;
+    qg.a = st.dq;
+    qg.b = st.g;
+    qg.c = st.g;
+    qg.d = st.g;
+    qg.out1 = S.sv.p + kSvQtg * ns;
+    qg.out2 = S.sv.p + kSvGtg * ns;
     if ((rc = scene_toi(s, &S.sc, &P, ns, st.x, st.xk, W.ptl.p, W.npt, W.eel.p, W.nee, W.vgl.p, W.nvg, S.plan.p,
-                        S.sv.p + kSvToi * ns, L.newton, W.counts(), S.cand_alpha.p, W.vertex_ids())))
+                        S.sv.p + kSvToi * ns, L.newton, W.counts(), S.cand_alpha.p, W.vertex_ids(), &qg)))
         return rc;
-    // q.g and g.g of the Armijo and stopping tests (abd.py:148,265): g and dq are final here
-    if ((rc = dots(s, &S.sc, ns, st.dq, st.g, st.g, st.g, S.sv.p + kSvQtg * ns, S.sv.p + kSvGtg * ns))) return rc;
     // the line search: the first trial, then every remaining halving of the scenes that
     // rejected it, each decided on the device
     if ((rc = line_pass(s, S, 1))) return rc;
