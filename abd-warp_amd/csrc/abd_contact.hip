@@ -277,7 +277,8 @@ __global__ __launch_bounds__(kBlock) void classify_kernel(AbdScene sc, AbdParams
                                                            const int32_t* __restrict__ ee, int64_t nee,
                                                            const int32_t* __restrict__ vg, int64_t nvg,
                                                            const int64_t* __restrict__ dn,
-                                                           uint8_t* __restrict__ flag, int* __restrict__ err) {
+                                                           uint8_t* __restrict__ flag, int* __restrict__ err,
+                                                           int64_t* __restrict__ bcnt) {
     const SceneView S = view_of(sc);
     if (dn) {
         npt = dn[2];
@@ -286,11 +287,11 @@ __global__ __launch_bounds__(kBlock) void classify_kernel(AbdScene sc, AbdParams
     }
     const int64_t n = npt + nee;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n + nvg; i += (int64_t)gridDim.x * kBlock) {
+        bool act = false;
         if (i >= n) {
             int64_t k = i - n;
             if (k > 0 && vg[2 * k] < vg[2 * (k - 1)]) *err = 1;
-            continue;
-        }
+        } else {
         const bool is_ee = i >= npt;
         const int64_t k = is_ee ? i - npt : i;
         const int32_t* t5 = (is_ee ? ee : pt) + 5 * k;
@@ -302,7 +303,6 @@ __global__ __launch_bounds__(kBlock) void classify_kernel(AbdScene sc, AbdParams
         else
             fetch_pt_idx(S, t5, vi);
         load4(x, vi, v);
-        bool act;
         if (P.psd == 3) {  // full IPC: the closest feature pair, no validity gate
             const Feat f = is_ee ? ee_feature(v) : pt_feature(v);
             act = (is_ee ? P.ee : P.pt) && feature_dist2_value(is_ee ? 1 : 0, f, v) < P.d2hat;
@@ -315,6 +315,15 @@ __global__ __launch_bounds__(kBlock) void classify_kernel(AbdScene sc, AbdParams
             act = P.pt && l * l < P.d2hat && valid_pt(c.beta, c.gamma);
         }
         flag[i] = act ? 1 : 0;
+        }
+        if (bcnt) {
+            // stage B's count: the wave's active candidates (its 64 consecutive ones lie in one
+            // compaction block) added to the block's count; lane 0 holds the wave's first one
+            const uint64_t m = __ballot(act);
+            if (lane_id() == 0 && m != 0)
+                atomicAdd(reinterpret_cast<unsigned long long*>(bcnt + i / ((int64_t)kBlock * kCItems)),
+                          (unsigned long long)__popcll(m));
+        }
     }
 }
 
@@ -797,8 +806,9 @@ __global__ __launch_bounds__(kBlock) void body_finalize_kernel(AbdScene sc, AbdP
 
 // the gh stage's flags and counters: the ordering flag, n8 bytes at a8, n64 int64 at a64
 __global__ void gh_zero_kernel(int* __restrict__ err, uint8_t* __restrict__ a8, int64_t n8,
-                               int64_t* __restrict__ a64, int64_t n64) {
+                               int64_t* __restrict__ a64, int64_t n64, int64_t* __restrict__ b64, int64_t nb64) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb64) b64[i] = 0;
     if (i == 0) {
         err[0] = 0;
         err[1] = err[2] = 0;  // stage D's counters of full and partial chunks
@@ -999,10 +1009,18 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
     // the pair runs of stage C (empty unless written), zeroed with the flags
     int64_t* pseg = nullptr;  // [2][nij][2]
     if (nij > 0 && (rc = arena_get(kSlotPairSeg, sizeof(int64_t) * 4 * (size_t)nij, (void**)&pseg))) return rc;
+    // stage B's per-block counts, added to by stage A (zeroed here with the flags)
+    const int64_t nblk = n > 0 ? ceil_div(n, (int64_t)kBlock * kCItems) : 0;
+    int64_t *bcnt = nullptr, *boff = nullptr;
+    if (n > 0) {
+        if ((rc = arena_get(kSlotScanA, sizeof(int64_t) * (nblk + 1), (void**)&bcnt))) return rc;
+        if ((rc = arena_get(kSlotScanB, sizeof(int64_t) * (nblk + 1), (void**)&boff))) return rc;
+    }
     {
         const int64_t n8 = 2 * (int64_t)nb + nij, n64 = 4 * nij;
-        hipLaunchKernelGGL(gh_zero_kernel, dim3(grid_for(n8 > n64 ? n8 : n64, kBlock, INT32_MAX)), dim3(kBlock), 0,
-                           s, err, hl_pt, n8, pseg, n64);
+        const int64_t nz = std::max(std::max(n8, n64), nblk);
+        hipLaunchKernelGGL(gh_zero_kernel, dim3(grid_for(nz, kBlock, INT32_MAX)), dim3(kBlock), 0, s, err, hl_pt, n8,
+                           pseg, n64, bcnt, nblk);
         ABD_CHECK_LAUNCH();
     }
     int64_t nact = 0;
@@ -1013,19 +1031,12 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         if ((rc = arena_get(kSlotFlags, (size_t)n + 1, (void**)&flag))) return rc;
         int ph = prof_start(kProfClassify, s, dn ? 1.0 : 129.0 * npt + 133.0 * nee + 8.0 * nvg);
         hipLaunchKernelGGL(classify_kernel, dim3(grid_for(n + nvg, kBlock, 8192)), dim3(kBlock), 0, s, *sc, *P, x,
-                           pt, npt, ee, nee, vg, nvg, dn, flag, err);
+                           pt, npt, ee, nee, vg, nvg, dn, flag, err, bcnt);
         ABD_CHECK_LAUNCH();
         prof_stop(ph, s);
         if ((rc = prof_count_bytes(ph, s, dn, 129.0, 133.0, 8.0, 0.0))) return rc;
         if (n > 0) {
-            const int64_t nblk = ceil_div(n, (int64_t)kBlock * kCItems);
-            int64_t *bcnt, *boff;
-            if ((rc = arena_get(kSlotScanA, sizeof(int64_t) * (nblk + 1), (void**)&bcnt))) return rc;
-            if ((rc = arena_get(kSlotScanB, sizeof(int64_t) * (nblk + 1), (void**)&boff))) return rc;
             if ((rc = arena_get(kSlotList, sizeof(int32_t) * (size_t)n, (void**)&act))) return rc;
-            hipLaunchKernelGGL(compact_flags_kernel<false>, dim3((unsigned)nblk), dim3(kBlock), 0, s,
-                               (const uint8_t*)flag, n, dn, bcnt, (const int64_t*)nullptr, (int32_t*)nullptr);
-            ABD_CHECK_LAUNCH();
             if ((rc = scan_exclusive_i64(s, bcnt, boff, nblk, boff + nblk, n_active_dev))) return rc;
             hipLaunchKernelGGL(compact_flags_kernel<true>, dim3((unsigned)nblk), dim3(kBlock), 0, s,
                                (const uint8_t*)flag, n, dn, bcnt, (const int64_t*)boff, act);
