@@ -165,4 +165,43 @@ __device__ __forceinline__ int block_rank(bool pred, int* sh_waves, int* total) 
     return off + r;
 }
 
+// Single workgroup (kScanThreads threads) exclusive scan of load(0), ..., load(n - 1) into out;
+// each pass covers 1024 threads x 8 items.  total / total2 (either may be null) receive the sum.
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 8;
+
+template <class Load>
+__device__ __forceinline__ void scan_block_exclusive_by(Load load, int64_t* __restrict__ out, int64_t n,
+                                                        int64_t* __restrict__ total, int64_t* __restrict__ total2) {
+    __shared__ int64_t sh[kScanThreads];
+    int64_t carry = 0;
+    const int t = threadIdx.x;
+    for (int64_t base = 0; base < n; base += (int64_t)kScanThreads * kScanItems) {
+        int64_t v[kScanItems];
+        int64_t s = 0;
+        int64_t i0 = base + (int64_t)t * kScanItems;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            int64_t i = i0 + k;
+            v[k] = i < n ? load(i) : 0;
+            s += v[k];
+        }
+        int64_t excl = block_exclusive_scan_i64(s, sh);
+        int64_t run = carry + excl;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            int64_t i = i0 + k;
+            if (i < n) out[i] = run;
+            run += v[k];
+        }
+        // last thread holds the tile total
+        if (t == kScanThreads - 1) sh[0] = excl + s;
+        __syncthreads();
+        carry += sh[0];
+        __syncthreads();
+    }
+    if (t == 0 && total) *total = carry;
+    if (t == 0 && total2) *total2 = carry;
+}
+
 }  // namespace abd

@@ -418,13 +418,16 @@ constexpr int kPairChunk = 64;  // contacts of one stage-D work item
 constexpr int kPartStride = 600;  // one chunk's partial sums: the 24x24 block, then g (24)
 
 // stage D1: the work items of each body pair: its PT run and its EE run cut into chunks of
-// kPairChunk contacts (runs from stage C; empty runs are [0, 0))
-__global__ __launch_bounds__(kBlock) void pair_chunks_kernel(int64_t nij, const int64_t* __restrict__ pseg,
-                                                             int64_t* __restrict__ cnt) {
-    for (int64_t ii = (int64_t)blockIdx.x * kBlock + threadIdx.x; ii < nij; ii += (int64_t)gridDim.x * kBlock) {
-        const int64_t npt = pseg[2 * ii + 1] - pseg[2 * ii], nee = pseg[2 * (nij + ii) + 1] - pseg[2 * (nij + ii)];
-        cnt[ii] = (npt + kPairChunk - 1) / kPairChunk + (nee + kPairChunk - 1) / kPairChunk;
-    }
+// kPairChunk contacts (runs from stage C; empty runs are [0, 0)).  One workgroup scans the pairs'
+// item counts into their first items coff and the total.
+__device__ __forceinline__ int64_t pair_item_count(const int64_t* __restrict__ pseg, int64_t nij, int64_t ii) {
+    const int64_t npt = pseg[2 * ii + 1] - pseg[2 * ii], nee = pseg[2 * (nij + ii) + 1] - pseg[2 * (nij + ii)];
+    return (npt + kPairChunk - 1) / kPairChunk + (nee + kPairChunk - 1) / kPairChunk;
+}
+__global__ __launch_bounds__(kScanThreads) void pair_chunks_scan_kernel(int64_t nij, const int64_t* __restrict__ pseg,
+                                                                        int64_t* __restrict__ coff,
+                                                                        int64_t* __restrict__ total) {
+    scan_block_exclusive_by([=](int64_t ii) { return pair_item_count(pseg, nij, ii); }, coff, nij, total, nullptr);
 }
 
 // stage D's items: pair ii owns the items [coff[ii], coff[ii] + cnt[ii]), its PT run's chunks
@@ -434,8 +437,7 @@ __global__ __launch_bounds__(kBlock) void pair_chunks_kernel(int64_t nij, const 
 // from the back): the longest sums start first.  Where an item lands in it changes no result.
 // One wave per pair, its lanes striding over the pair's items.
 constexpr int kItemShift = 56;
-__global__ __launch_bounds__(kBlock) void pair_items_kernel(int64_t nij, const int64_t* __restrict__ cnt,
-                                                            const int64_t* __restrict__ coff,
+__global__ __launch_bounds__(kBlock) void pair_items_kernel(int64_t nij, const int64_t* __restrict__ coff,
                                                             const int64_t* __restrict__ pseg,
                                                             const int64_t* __restrict__ total,
                                                             int64_t* __restrict__ idesc, int* __restrict__ counters,
@@ -445,10 +447,11 @@ __global__ __launch_bounds__(kBlock) void pair_items_kernel(int64_t nij, const i
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nw = (int64_t)gridDim.x * (kBlock / kWave);
     for (int64_t ii = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; ii < nij; ii += nw) {
-        const int64_t n = cnt[ii], o = coff[ii];
+        const int64_t o = coff[ii];
         const int64_t p0 = pseg[2 * ii], p1 = pseg[2 * ii + 1];
         const int64_t e0 = pseg[2 * (nij + ii)], e1 = pseg[2 * (nij + ii) + 1];
         const int64_t npc = (p1 - p0 + kPairChunk - 1) / kPairChunk;
+        const int64_t n = npc + (e1 - e0 + kPairChunk - 1) / kPairChunk;
         for (int64_t c = lane; c < n; c += kWave) {
             const bool pt = c < npc;
             const int64_t c0 = pt ? p0 + c * kPairChunk : e0 + (c - npc) * kPairChunk;
@@ -636,14 +639,14 @@ __global__ __launch_bounds__(kRankThreads, 5) void pair_partial_rank_kernel(cons
 }
 
 // stage D3: per body pair with active contacts (grid-stride): its chunks' partial sums
-// added in order (PT chunks, then EE), diagonal parts to the pair buffer (stage E adds
+// (items [coff[ii], coff[ii + 1]); coff has nij + 1 entries, the last the total) added in
+// order (PT chunks, then EE), diagonal parts to the pair buffer (stage E adds
 // them per body), the off-diagonal block to the matrix.  nij: pairs of the buffers (an upper
 // bound when dn gives the count of the matrix's pairs, dn[0], which places its lower slots).
 __global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64_t nij,
                                                                     const int64_t* __restrict__ dn,
                                                                     const uint8_t* __restrict__ pflag,
                                                                     const int64_t* __restrict__ coff,
-                                                                    const int64_t* __restrict__ cnt,
                                                                     const double* __restrict__ part,
                                                                     double* __restrict__ pbuf,
                                                                     double* __restrict__ blocks) {
@@ -652,7 +655,7 @@ __global__ __launch_bounds__(kPairThreads) void pair_reduce_kernel(int nb, int64
     const int64_t nij_m = dn ? dn[0] : nij;
     for (int64_t ii = blockIdx.x; ii < nij; ii += gridDim.x) {
         if (!pflag[ii]) continue;  // uniform over the workgroup
-        const int64_t c0 = coff[ii], nc = cnt[ii];
+        const int64_t c0 = coff[ii], nc = coff[ii + 1] - c0;  // coff[nij] holds the items' total
         double h = 0.0, gacc = 0.0;
         for (int64_t c = c0; c < c0 + nc; ++c) {
             h += part[c * kPartStride + t];
@@ -1110,11 +1113,10 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         const int64_t pgrid = tuning().pair_grid > 0 ? tuning().pair_grid : kPairGrid;
         const int64_t runs = 2 * nij < nact ? 2 * nij : nact;
         const int64_t cap_items = nact / kPairChunk + runs + 1;
-        int64_t* pcnt;
-        if ((rc = arena_get(kSlotPairChunk, sizeof(int64_t) * (2 * (size_t)nij + 2 + 3 * (size_t)cap_items),
-                            (void**)&pcnt)))
+        int64_t* pcoff;
+        if ((rc = arena_get(kSlotPairChunk, sizeof(int64_t) * ((size_t)nij + 2 + 3 * (size_t)cap_items),
+                            (void**)&pcoff)))
             return rc;
-        int64_t* pcoff = pcnt + nij;
         int64_t* ptotal = pcoff + nij;
         int64_t* idesc = ptotal + 2;
         int64_t* odesc = idesc + cap_items;
@@ -1122,12 +1124,11 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         double* part;
         if ((rc = arena_get(kSlotPairPart, sizeof(double) * kPartStride * (size_t)cap_items, (void**)&part)))
             return rc;
-        hipLaunchKernelGGL(pair_chunks_kernel, dim3(grid_for(nij)), dim3(kBlock), 0, s, nij, (const int64_t*)pseg,
-                           pcnt);
+        hipLaunchKernelGGL(pair_chunks_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, nij, (const int64_t*)pseg,
+                           pcoff, ptotal);
         ABD_CHECK_LAUNCH();
-        if ((rc = scan_exclusive_i64(s, pcnt, pcoff, nij, ptotal))) return rc;
         hipLaunchKernelGGL(pair_items_kernel, dim3(grid_for(nij, kBlock / kWave)), dim3(kBlock), 0, s, nij,
-                           (const int64_t*)pcnt, (const int64_t*)pcoff, (const int64_t*)pseg, (const int64_t*)ptotal,
+                           (const int64_t*)pcoff, (const int64_t*)pseg, (const int64_t*)ptotal,
                            idesc, err + 1, odesc, oitem);
         ABD_CHECK_LAUNCH();
         // timed: the partial sums (each stored contact read once, its body-order flag once)
@@ -1143,7 +1144,7 @@ int contact_gh(hipStream_t s, const AbdScene* sc, const AbdParams* P, const doub
         ABD_CHECK_LAUNCH();
         if ((rc = prof_count_bytes(ph, s, dnact, 8.0 * stride + 1, 0.0, 0.0, 0.0, true))) return rc;
         hipLaunchKernelGGL(pair_reduce_kernel, dim3(grid_for(nij, 1, pgrid)), dim3(kPairThreads), 0, s, nb, nij,
-                           dn, (const uint8_t*)pflag, (const int64_t*)pcoff, (const int64_t*)pcnt,
+                           dn, (const uint8_t*)pflag, (const int64_t*)pcoff,
                            (const double*)part, pbuf, blocks);
         ABD_CHECK_LAUNCH();
     }
